@@ -1,0 +1,836 @@
+/*
+ * CPU restatement of the SWALS time-stepping hot path (FP64, serial).
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker): loaded by tests/, smoke() and
+ * bench.py's cpu_baseline leg; never by the product.
+ *
+ * Each routine cites the reference file:line whose semantics it restates, with
+ * paths relative to propagation/SWALS/src/shallow_water/. Arithmetic is written in
+ * the same evaluation order as the reference expressions, and the build uses
+ * -ffp-contract=off, so on the same inputs the restatement reproduces the compiled
+ * reference to the last bit or a few ulps (libm differences only).
+ */
+#include "swals_oracle.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* global_mod.f90:37-44 */
+static const double GRAVITY = 9.8;
+static const double RADIUS_EARTH = 6371000.0;
+static const double MIN_DEPTH = 1.0e-05;
+static const double MAXIMUM_TIMESTEP = 1.0e+20;
+/* spherical_mod.f90:15 */
+static const double EARTH_ANGULAR_FREQ = 7.292115e-05;
+/* domain_mod.f90:111-117 */
+static const double LIMITER_COEF1 = 0.1, LIMITER_COEF2 = 0.3, LIMITER_COEF3 = 100.0;
+
+enum { STG = 0, UH = 1, VH = 2, ELV = 3 };
+enum { MX_STAGE = 0, MX_SPEED, MX_FLUX, MX_ARRIVAL, MX_TOMS, MX_MIN_STAGE, N_MAXQ };
+
+struct or_domain {
+    int nx, ny, method, spherical, coriolis_on, truely_linear, friction_type;
+    int xL, xU, yL, yU, n_ext;
+    int boundary_exterior[4];
+    double lw[2], ll[2], dx[2];
+    double msl_linear, linear_friction_coeff, cfl, theta, maximum_timestep;
+    double *x, *y, *dbe, *dle, *area, *cor, *cor_be, *tanlat;
+    double *U, *manning_sq, *friction_work; /* friction_work: 2 planes (UH, VH) */
+    int friction_work_is_setup;
+    /* finite-volume work arrays */
+    double *depth, *vel, *flux_NS, *flux_EW, *es, *es_vh_jm1, *backup_U;
+    /* scalars */
+    double time, max_dt, evolve_step_dt, dt_last_update;
+    double boundary_flux_store[4], boundary_flux_evolve_integral, boundary_flux_time_integral;
+    double negative_depth_fix_counter;
+    int nsteps_advanced;
+    /* max quantities */
+    int record_max_U;
+    double *max_U;
+    double arrival_stage_threshold;
+    /* stage boundary */
+    int bc_on;
+    double bc_amp, bc_period, bc_msl;
+};
+
+#define IX(d, i, j) ((size_t)((j) - 1) * (size_t)(d)->nx + (size_t)((i) - 1))
+#define P(d, k) ((d)->U + (size_t)(k) * (size_t)(d)->nx * (size_t)(d)->ny)
+#define A(d, k, i, j) (P(d, k)[IX(d, i, j)])
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+/* spherical_mod.f90:23-64 */
+double or_area_lonlat_rectangle(double lon1, double lat1, double dlon, double dlat, int flat)
+{
+    const double deg2rad = acos(-1.0) / 180.0;
+    if (flat) {
+        return RADIUS_EARTH * RADIUS_EARTH * cos((lat1 + 0.5 * dlat) * deg2rad) * fabs(dlat) * deg2rad *
+               fmod(fabs(dlon), 360.0) * deg2rad;
+    }
+    double s1 = sin(lat1 * deg2rad), s2 = sin((lat1 + dlat) * deg2rad);
+    return RADIUS_EARTH * RADIUS_EARTH * fabs(s1 - s2) * fmod(fabs(dlon), 360.0) * deg2rad;
+}
+
+/* Metric arrays: domain_mod.f90:918-1014 (x/y centres, cos/tan lat, Coriolis, edge
+ * lengths, flat spherical cell areas). */
+static void build_metrics(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    const double deg2rad = acos(-1.0) / 180.0;
+    d->dx[0] = d->lw[0] / (nx * 1.0);
+    d->dx[1] = d->lw[1] / (ny * 1.0);
+    for (int i = 1; i <= nx; i++) d->x[i - 1] = d->ll[0] + ((i - 0.5) / (nx * 1.0)) * d->lw[0];
+    for (int i = 1; i <= ny; i++) d->y[i - 1] = d->ll[1] + ((i - 0.5) / (ny * 1.0)) * d->lw[1];
+    if (d->spherical) {
+        double *coslat_be = calloc((size_t)ny + 1, sizeof(double));
+        for (int j = 0; j < ny; j++) coslat_be[j] = cos((d->y[j] - 0.5 * d->dx[1]) * deg2rad);
+        coslat_be[ny] = cos((d->y[ny - 1] + 0.5 * d->dx[1]) * deg2rad);
+        for (int j = 0; j < ny; j++) d->tanlat[j] = tan(d->y[j] * deg2rad) / RADIUS_EARTH;
+        if (d->coriolis_on) {
+            for (int j = 0; j < ny; j++) d->cor[j] = 2.0 * sin(d->y[j] * deg2rad) * EARTH_ANGULAR_FREQ;
+            for (int j = 0; j < ny; j++)
+                d->cor_be[j] = 2.0 * EARTH_ANGULAR_FREQ * sin((d->y[j] - 0.5 * d->dx[1]) * deg2rad);
+            d->cor_be[ny] = 2.0 * EARTH_ANGULAR_FREQ * sin((d->y[ny - 1] + 0.5 * d->dx[1]) * deg2rad);
+        }
+        for (int i = 0; i <= nx; i++) d->dle[i] = d->dx[1] * deg2rad * RADIUS_EARTH;
+        for (int j = 0; j <= ny; j++) d->dbe[j] = d->dx[0] * deg2rad * RADIUS_EARTH * coslat_be[j];
+        for (int j = 0; j < ny; j++)
+            d->area[j] = or_area_lonlat_rectangle(d->x[0] - d->dx[0] * 0.5, d->y[j] - d->dx[1] * 0.5, d->dx[0],
+                                                  d->dx[1], 1);
+        free(coslat_be);
+    } else {
+        for (int i = 0; i <= nx; i++) d->dle[i] = d->dx[1];
+        for (int j = 0; j <= ny; j++) d->dbe[j] = d->dx[0];
+        for (int j = 0; j < ny; j++) d->area[j] = d->dx[0] * d->dx[1];
+    }
+}
+
+or_domain *or_domain_create(int nx, int ny, const double lw[2], const double ll[2], int method, int spherical,
+                            int coriolis, int truely_linear, double msl_linear, double linear_friction_coeff,
+                            int friction_type, double cfl, double theta)
+{
+    or_domain *d = calloc(1, sizeof(or_domain));
+    size_t n = (size_t)nx * ny;
+    d->nx = nx; d->ny = ny; d->method = method;
+    d->spherical = spherical; d->coriolis_on = spherical && coriolis;
+    d->truely_linear = truely_linear; d->msl_linear = msl_linear;
+    d->linear_friction_coeff = linear_friction_coeff; d->friction_type = friction_type;
+    d->lw[0] = lw[0]; d->lw[1] = lw[1]; d->ll[0] = ll[0]; d->ll[1] = ll[1];
+    /* Default CFL / theta from timestepping_metadata_mod.f90:73-121 */
+    double def_cfl = (method == OR_RK2) ? 0.99 : (method == OR_EULER ? 0.9 : 0.7);
+    double def_theta = (method == OR_RK2) ? 1.6 : (method == OR_EULER ? 0.9 : -HUGE_VAL);
+    d->cfl = cfl > 0 ? cfl : def_cfl;
+    d->theta = theta > 0 ? theta : def_theta;
+    d->maximum_timestep = MAXIMUM_TIMESTEP;
+    d->xL = 1; d->xU = nx; d->yL = 1; d->yU = ny; d->n_ext = 2;
+    for (int k = 0; k < 4; k++) d->boundary_exterior[k] = 1;
+    d->x = calloc(nx, sizeof(double)); d->y = calloc(ny, sizeof(double));
+    d->dbe = calloc(ny + 1, sizeof(double)); d->dle = calloc(nx + 1, sizeof(double));
+    d->area = calloc(ny, sizeof(double)); d->cor = calloc(ny, sizeof(double));
+    d->cor_be = calloc(ny + 1, sizeof(double)); d->tanlat = calloc(ny, sizeof(double));
+    d->U = calloc(4 * n, sizeof(double));
+    d->manning_sq = calloc(n, sizeof(double));
+    d->friction_work = calloc(2 * n, sizeof(double));
+    d->depth = calloc(n, sizeof(double)); d->vel = calloc(2 * n, sizeof(double));
+    d->flux_NS = calloc(3 * (size_t)nx * (ny + 1), sizeof(double));
+    d->flux_EW = calloc(3 * (size_t)(nx + 1) * ny, sizeof(double));
+    d->es = calloc(2 * n, sizeof(double));
+    d->es_vh_jm1 = calloc((size_t)nx * (ny + 1), sizeof(double));
+    d->backup_U = calloc(3 * n, sizeof(double));
+    d->max_U = calloc(N_MAXQ * n, sizeof(double));
+    d->record_max_U = 1;
+    d->arrival_stage_threshold = 0.01; /* domain_mod.f90:434 default */
+    /* max_U initial values: domain_mod.f90:1119-1136 (computed in float32) */
+    float big = 0.99f * FLT_MAX;
+    for (size_t k = 0; k < N_MAXQ * n; k++) d->max_U[k] = -(double)big;
+    for (size_t k = 0; k < n; k++) d->max_U[MX_MIN_STAGE * n + k] = (double)big;
+    build_metrics(d);
+    return d;
+}
+
+void or_domain_destroy(or_domain *d)
+{
+    if (!d) return;
+    free(d->x); free(d->y); free(d->dbe); free(d->dle); free(d->area); free(d->cor); free(d->cor_be);
+    free(d->tanlat); free(d->U); free(d->manning_sq); free(d->friction_work); free(d->depth); free(d->vel);
+    free(d->flux_NS); free(d->flux_EW); free(d->es); free(d->es_vh_jm1); free(d->backup_U); free(d->max_U);
+    free(d);
+}
+
+void or_domain_set_U(or_domain *d, const double *U) { memcpy(d->U, U, 4 * sizeof(double) * d->nx * d->ny); }
+void or_domain_get_U(const or_domain *d, double *U) { memcpy(U, d->U, 4 * sizeof(double) * d->nx * d->ny); }
+void or_domain_set_manning_squared(or_domain *d, const double *m)
+{
+    memcpy(d->manning_sq, m, sizeof(double) * d->nx * d->ny);
+    d->friction_work_is_setup = 0;
+}
+void or_domain_set_time(or_domain *d, double t) { d->time = t; }
+void or_domain_set_record_max_U(or_domain *d, int on) { d->record_max_U = on; }
+
+void or_domain_get_metrics(const or_domain *d, double *x, double *y, double *dbe, double *dle, double *area,
+                           double *cor, double *cor_be, double *tanlat)
+{
+    memcpy(x, d->x, sizeof(double) * d->nx);
+    memcpy(y, d->y, sizeof(double) * d->ny);
+    memcpy(dbe, d->dbe, sizeof(double) * (d->ny + 1));
+    memcpy(dle, d->dle, sizeof(double) * (d->nx + 1));
+    memcpy(area, d->area, sizeof(double) * d->ny);
+    memcpy(cor, d->cor, sizeof(double) * d->ny);
+    memcpy(cor_be, d->cor_be, sizeof(double) * (d->ny + 1));
+    if (d->spherical) memcpy(tanlat, d->tanlat, sizeof(double) * d->ny);
+    else memset(tanlat, 0, sizeof(double) * d->ny);
+}
+
+/* boundary_mod.f90:78-164 (boundary_stage_transmissive_normal_momentum) with the
+ * stage function msl + amp*sin(2*pi*t/period) and elevation left unchanged. */
+static void update_boundary(or_domain *d)
+{
+    if (!d->bc_on) return;
+    const int nx = d->nx, ny = d->ny;
+    const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF);
+    const double two_pi = 2.0 * acos(-1.0);
+    double stage = d->bc_msl + d->bc_amp * sin(two_pi * d->time / d->bc_period);
+    for (int j = 1; j <= ny; j++) {
+        if (d->boundary_exterior[3]) {
+            int i = 1;
+            A(d, STG, i, j) = stage;
+            if (!stag) { A(d, UH, i, j) = A(d, UH, i + 1, j); A(d, VH, i, j) = 0.0; }
+        }
+        if (d->boundary_exterior[1]) {
+            int i = nx;
+            A(d, STG, i, j) = stage;
+            if (!stag) { A(d, UH, i, j) = A(d, UH, i - 1, j); A(d, VH, i, j) = 0.0; }
+        }
+    }
+    for (int i = 1; i <= nx; i++) {
+        if (d->boundary_exterior[2]) {
+            int j = 1;
+            A(d, STG, i, j) = stage;
+            if (!stag) { A(d, VH, i, j) = A(d, VH, i, j + 1); A(d, UH, i, j) = 0.0; }
+        }
+        if (d->boundary_exterior[0]) {
+            int j = ny;
+            A(d, STG, i, j) = stage;
+            if (!stag) { A(d, VH, i, j) = A(d, VH, i, j - 1); A(d, UH, i, j) = 0.0; }
+        }
+    }
+}
+
+void or_domain_set_stage_bc(or_domain *d, double amp, double period, double msl, const int sides[4])
+{
+    d->bc_on = 1; d->bc_amp = amp; d->bc_period = period; d->bc_msl = msl;
+    for (int k = 0; k < 4; k++) d->boundary_exterior[k] = sides[k] != 0;
+}
+
+/* domain_mod_leapfrog_solver_friction_include.f90:21-85 (precompute_friction_work) */
+static void precompute_friction_work(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    const double power = (d->friction_type == 1) ? -2.0 : -7.0 / 3.0;
+    double *fU = d->friction_work, *fV = d->friction_work + (size_t)nx * ny;
+    const double msl = d->msl_linear;
+    for (int j = 1; j <= ny; j++) {
+        for (int i = 1; i <= nx; i++) {
+            int ip1 = i + 1 < nx ? i + 1 : nx, jp1 = j + 1 < ny ? j + 1 : ny;
+            double depth_iph, depth_jph;
+            if (d->truely_linear) {
+                depth_iph = 0.5 * (msl - A(d, ELV, i, j) + msl - A(d, ELV, ip1, j));
+                depth_jph = 0.5 * (msl - A(d, ELV, i, j) + msl - A(d, ELV, i, jp1));
+            } else {
+                depth_iph = 0.5 * (A(d, STG, i, j) - A(d, ELV, i, j) + A(d, STG, ip1, j) - A(d, ELV, ip1, j));
+                depth_jph = 0.5 * (A(d, STG, i, j) - A(d, ELV, i, j) + A(d, STG, i, jp1) - A(d, ELV, i, jp1));
+            }
+            depth_iph = dmax(depth_iph, MIN_DEPTH);
+            depth_jph = dmax(depth_jph, MIN_DEPTH);
+            double nsq_iph = 0.5 * (d->manning_sq[IX(d, i, j)] + d->manning_sq[IX(d, ip1, j)]);
+            double nsq_jph = 0.5 * (d->manning_sq[IX(d, i, j)] + d->manning_sq[IX(d, i, jp1)]);
+            fU[IX(d, i, j)] = GRAVITY * nsq_iph * pow(depth_iph, power);
+            fV[IX(d, i, j)] = GRAVITY * nsq_jph * pow(depth_jph, power);
+        }
+    }
+    if (d->truely_linear) d->friction_work_is_setup = 1;
+}
+
+/* Staggered-grid leapfrog step: domain_mod_linear_solver_include.f90:59-571, with the
+ * four variants of domain_mod_timestepping_alternatives_include.f90:339-434. */
+static void leapfrog_step(or_domain *d, double dt)
+{
+    const int nx = d->nx, ny = d->ny;
+    const int xL = d->xL, xU = d->xU, yL = d->yL, yU = d->yU;
+    const int friction = (d->method == OR_LLPNF);
+    const int cor_or_fric = d->coriolis_on || friction;
+    const double msl = d->msl_linear;
+    d->dt_last_update = dt;
+
+    /* Stage half-step: include.f90:92-113 */
+    for (int j = yL + 1; j <= yU - 1; j++) {
+        double inv_cell_area_dt = dt / d->area[j - 1];
+        for (int i = xL + 1; i <= xU - 1; i++) {
+            A(d, STG, i, j) = A(d, STG, i, j) -
+                              inv_cell_area_dt * ((A(d, UH, i, j) - A(d, UH, i - 1, j)) * d->dle[i - 1] +
+                                                  (A(d, VH, i, j) * d->dbe[j] - A(d, VH, i, j - 1) * d->dbe[j - 1]));
+        }
+    }
+    d->time = d->time + dt * 0.5;
+    update_boundary(d);
+
+    /* Boundary flux integration: include.f90:126-205 (single domain => exterior == store) */
+    {
+        const int ne = d->n_ext;
+        double s;
+        s = 0.0; for (int i = ne + 1; i <= nx - ne; i++) s += A(d, VH, i, ny - ne);
+        d->boundary_flux_store[0] = s * d->dbe[ny - ne];
+        s = 0.0; for (int j = 1 + ne; j <= ny - ne; j++) s += A(d, UH, nx - ne, j);
+        d->boundary_flux_store[1] = s * d->dle[nx - ne];
+        s = 0.0; for (int i = ne + 1; i <= nx - ne; i++) s += A(d, VH, i, ne);
+        d->boundary_flux_store[2] = -s * d->dbe[ne];
+        s = 0.0; for (int j = 1 + ne; j <= ny - ne; j++) s += A(d, UH, ne, j);
+        d->boundary_flux_store[3] = -s * d->dle[ne];
+        d->boundary_flux_evolve_integral =
+            (((d->boundary_flux_store[0] + d->boundary_flux_store[1]) + d->boundary_flux_store[2]) +
+             d->boundary_flux_store[3]) * dt;
+    }
+
+    if (friction && !d->friction_work_is_setup) precompute_friction_work(d);
+    const double *fwU = d->friction_work, *fwV = d->friction_work + (size_t)nx * ny;
+
+    /* Momentum half-step: include.f90:211-571, serial loop order yL..yU-1. */
+    double *h_jph = calloc(nx + 1, sizeof(double)), *h_iph = calloc(nx + 1, sizeof(double));
+    double *wet_jph = calloc(nx + 1, sizeof(double)), *wet_iph = calloc(nx + 1, sizeof(double));
+    double *dw_j = calloc(nx + 1, sizeof(double));
+    double *vh_iph_jmh = calloc(nx + 1, sizeof(double)), *vh_iph_jph = calloc(nx + 1, sizeof(double));
+    double *uh_i_j = calloc(nx + 1, sizeof(double)), *uh_i_jp1 = calloc(nx + 1, sizeof(double));
+    double *fm_UH = calloc(nx + 1, sizeof(double)), *fm_VH = calloc(nx + 1, sizeof(double));
+    const int yl = yL, yu = yU - 1;
+    if (cor_or_fric) {
+        /* include.f90:295-331: 'old' values bounding the j-loop */
+        int jm = (yl == 1) ? 1 : yl - 1;
+        for (int i = xL; i <= xU - 1; i++) vh_iph_jmh[i] = 0.5 * (A(d, VH, i, jm) + A(d, VH, i + 1, jm));
+        for (int i = xL + 1; i <= xU - 1; i++) uh_i_j[i] = 0.5 * (A(d, UH, i, yl) + A(d, UH, i - 1, yl));
+        uh_i_j[xL] = 0.5 * (A(d, UH, xL, yl) + A(d, UH, xL - 1 > 1 ? xL - 1 : 1, yl));
+    }
+    for (int j = yl; j <= yu; j++) {
+        double inv_cell_area_dt_g = GRAVITY * dt / d->area[j - 1];
+        double inv_cell_area_dt_vh_g = GRAVITY * dt / (0.5 * (d->area[j - 1] + d->area[j]));
+        for (int i = xL; i <= xU - 1; i++) dw_j[i] = A(d, STG, i, j + 1) - A(d, STG, i, j);
+        if (d->truely_linear) {
+            for (int i = xL; i <= xU; i++) {
+                int c = (A(d, ELV, i, j + 1) < -MIN_DEPTH + msl) && (A(d, ELV, i, j) < -MIN_DEPTH + msl);
+                h_jph[i] = c ? msl - 0.5 * (A(d, ELV, i, j + 1) + A(d, ELV, i, j)) : 0.0;
+                wet_jph[i] = 1.0;
+            }
+            for (int i = xL; i <= xU - 1; i++) {
+                int c = (A(d, ELV, i, j) < -MIN_DEPTH + msl) && (A(d, ELV, i + 1, j) < -MIN_DEPTH + msl);
+                h_iph[i] = c ? msl - 0.5 * (A(d, ELV, i + 1, j) + A(d, ELV, i, j)) : 0.0;
+                wet_iph[i] = 1.0;
+            }
+        } else {
+            for (int i = xL; i <= xU; i++) {
+                int c = (A(d, STG, i, j + 1) - A(d, ELV, i, j + 1) > MIN_DEPTH) &&
+                        (A(d, STG, i, j) - A(d, ELV, i, j) > MIN_DEPTH);
+                h_jph[i] = c ? 0.5 * ((A(d, STG, i, j + 1) + A(d, STG, i, j)) -
+                                      (A(d, ELV, i, j + 1) + A(d, ELV, i, j)))
+                             : 0.0;
+                wet_jph[i] = h_jph[i] > 0.0 ? 1.0 : 0.0;
+            }
+            for (int i = xL; i <= xU - 1; i++) {
+                int c = (A(d, STG, i, j) - A(d, ELV, i, j) > MIN_DEPTH) &&
+                        (A(d, STG, i + 1, j) - A(d, ELV, i + 1, j) > MIN_DEPTH);
+                h_iph[i] = c ? 0.5 * ((A(d, STG, i + 1, j) + A(d, STG, i, j)) -
+                                      (A(d, ELV, i + 1, j) + A(d, ELV, i, j)))
+                             : 0.0;
+                wet_iph[i] = h_iph[i] > 0.0 ? 1.0 : 0.0;
+            }
+        }
+        if (cor_or_fric) {
+            /* include.f90:408-430 */
+            for (int i = xL; i <= xU - 1; i++) vh_iph_jph[i] = 0.5 * (A(d, VH, i, j) + A(d, VH, i + 1, j));
+            for (int i = xL + 1; i <= xU - 1; i++) uh_i_jp1[i] = 0.5 * (A(d, UH, i, j + 1) + A(d, UH, i - 1, j + 1));
+            uh_i_jp1[xL] = 0.5 * (A(d, UH, xL, j + 1) + A(d, UH, xL - 1 > 1 ? xL - 1 : 1, j + 1));
+        }
+        if (friction) {
+            /* include.f90:486-503 (no ambient flux) */
+            for (int i = xL; i <= xU - 1; i++) {
+                double uh = A(d, UH, i, j), vh = A(d, VH, i, j);
+                double vbar = 0.5 * (vh_iph_jmh[i] + vh_iph_jph[i]);
+                double ubar = 0.5 * (uh_i_j[i] + uh_i_jp1[i]);
+                fm_UH[i] = 1.0 / (1.0 + dt * d->linear_friction_coeff +
+                                  dt * fwU[IX(d, i, j)] * (sqrt(uh * uh + vbar * vbar)));
+                fm_VH[i] = 1.0 / (1.0 + dt * d->linear_friction_coeff +
+                                  dt * fwV[IX(d, i, j)] * (sqrt(vh * vh + ubar * ubar)));
+            }
+        }
+        for (int i = xL; i <= xU - 1; i++) {
+            double uh_new = A(d, UH, i, j) * wet_iph[i] - inv_cell_area_dt_g * h_iph[i] *
+                                                                (A(d, STG, i + 1, j) - A(d, STG, i, j)) * d->dle[i];
+            double vh_new = A(d, VH, i, j) * wet_jph[i] - inv_cell_area_dt_vh_g * h_jph[i] * dw_j[i] * d->dbe[j];
+            if (d->coriolis_on) {
+                /* include.f90:432-439, 523-540 */
+                double dhc = dt * d->cor[j - 1] * 0.5 * (h_iph[i] > 0.0 ? 1.0 : 0.0);
+                double dhc_jph = dt * d->cor_be[j] * 0.5 * (h_jph[i] > 0.0 ? 1.0 : 0.0);
+                uh_new = uh_new + dhc * (vh_iph_jmh[i] + vh_iph_jph[i]);
+                vh_new = vh_new - dhc_jph * (uh_i_j[i] + uh_i_jp1[i]);
+            }
+            if (friction) {
+                uh_new = fm_UH[i] * uh_new;
+                vh_new = fm_VH[i] * vh_new;
+            }
+            A(d, UH, i, j) = uh_new;
+            A(d, VH, i, j) = vh_new;
+        }
+        if (!friction && d->linear_friction_coeff != 0.0) {
+            /* include.f90:550-557 */
+            double f = 1.0 / (1.0 + dt * d->linear_friction_coeff);
+            for (int i = xL; i <= xU - 1; i++) {
+                A(d, UH, i, j) = f * A(d, UH, i, j);
+                A(d, VH, i, j) = f * A(d, VH, i, j);
+            }
+        }
+        if (cor_or_fric) {
+            for (int i = xL; i <= xU - 1; i++) { vh_iph_jmh[i] = vh_iph_jph[i]; uh_i_j[i] = uh_i_jp1[i]; }
+        }
+    }
+    d->time = d->time + 0.5 * dt;
+    free(h_jph); free(h_iph); free(wet_jph); free(wet_iph); free(dw_j); free(vh_iph_jmh); free(vh_iph_jph);
+    free(uh_i_j); free(uh_i_jp1); free(fm_UH); free(fm_VH);
+}
+
+/* domain_mod.f90:1360-1419 */
+static void compute_depth_and_velocity(or_domain *d)
+{
+    const size_t n = (size_t)d->nx * d->ny;
+    double *stg = P(d, STG), *uh = P(d, UH), *vh = P(d, VH), *elv = P(d, ELV);
+    for (size_t k = 0; k < n; k++) {
+        d->depth[k] = stg[k] - elv[k];
+        if (d->depth[k] > MIN_DEPTH) {
+            double depth_inv = 1.0 / d->depth[k];
+            d->vel[k] = uh[k] * depth_inv;
+            d->vel[n + k] = vh[k] * depth_inv;
+        } else {
+            if (d->depth[k] < 0.0) {
+                d->depth[k] = 0.0;
+                stg[k] = elv[k];
+                d->negative_depth_fix_counter += 1.0;
+            }
+            d->vel[k] = 0.0; d->vel[n + k] = 0.0;
+            uh[k] = 0.0; vh[k] = 0.0;
+        }
+    }
+}
+
+/* extrapolation_limiting_mod.f90:44-79 (limiter_type = 'MC') */
+void or_mc_limited_gradient(const double *u_local, const double *u_lower, const double *u_upper,
+                            const double *theta, double *grad, int n)
+{
+    for (int i = 0; i < n; i++) {
+        double a = u_upper[i] - u_local[i];
+        double b = u_local[i] - u_lower[i];
+        double th = theta[i];
+        double sa = copysign(1.0, a), sb = copysign(1.0, b);
+        double half_sasb = 0.5 * (sa + sb);
+        double dd = dmin(fabs(a), fabs(b)) * half_sasb * th;
+        double e = 0.5 * (a + b);
+        double c = (dd == 0.0) ? 0.0 : e;
+        grad[i] = (dd > 0.0) ? dmin(c, dd) : dmax(c, dd);
+    }
+}
+
+/* Wet-dry theta taper: domain_mod_compute_fluxes_DE1_inner_include.f90:751-757 / 816-822 */
+static double theta_wd(const or_domain *d, double d0, double d1, double d2)
+{
+    double mindep = dmin(dmin(d0, d1), d2) - MIN_DEPTH;
+    double maxdep = dmax(dmax(d0, d1), d2) + LIMITER_COEF3 * MIN_DEPTH;
+    double theta_local = (1.0 / (LIMITER_COEF2 - LIMITER_COEF1)) * (mindep / maxdep - LIMITER_COEF1);
+    return dmax(d->theta * dmin(1.0, theta_local), 0.0);
+}
+
+/* Row-j NS gradients: include.f90:735-795. g[0..3] = stage, depth, u, v (length nx, 0-based i-1). */
+static void ns_gradients(const or_domain *d, int j, double *th, double *g[4])
+{
+    const int nx = d->nx, ny = d->ny;
+    const size_t n = (size_t)nx * ny;
+    if (j > 1 && j < ny) {
+        for (int i = 1; i <= nx; i++)
+            th[i - 1] = theta_wd(d, d->depth[IX(d, i, j - 1)], d->depth[IX(d, i, j)], d->depth[IX(d, i, j + 1)]);
+        const double *src[4] = {P(d, STG), d->depth, d->vel, d->vel + n};
+        for (int q = 0; q < 4; q++)
+            or_mc_limited_gradient(src[q] + IX(d, 1, j), src[q] + IX(d, 1, j - 1), src[q] + IX(d, 1, j + 1), th, g[q],
+                                   nx);
+    } else {
+        for (int i = 0; i < nx; i++) { th[i] = 0.0; for (int q = 0; q < 4; q++) g[q][i] = 0.0; }
+    }
+}
+
+/* Row-j EW gradients: include.f90:800-862 (interior i = 2..nx-1, zero at i = 1 and nx) */
+static void ew_gradients(const or_domain *d, int j, double *th, double *g[4])
+{
+    const int nx = d->nx;
+    const size_t n = (size_t)nx * d->ny;
+    for (int i = 2; i <= nx - 1; i++)
+        th[i - 1] = theta_wd(d, d->depth[IX(d, i - 1, j)], d->depth[IX(d, i, j)], d->depth[IX(d, i + 1, j)]);
+    const double *src[4] = {P(d, STG), d->depth, d->vel, d->vel + n};
+    for (int q = 0; q < 4; q++) {
+        if (nx > 2)
+            or_mc_limited_gradient(src[q] + IX(d, 2, j), src[q] + IX(d, 1, j), src[q] + IX(d, 3, j), th + 1,
+                                   g[q] + 1, nx - 2);
+        g[q][0] = 0.0;
+        g[q][nx - 1] = 0.0;
+    }
+}
+
+/* compute_fluxes_DE1_low_fr_diffusion: domain_mod_compute_fluxes_DE1_inner_include.f90:78-727
+ * (reduced_momentum_diffusion = .true., no eddy viscosity, flux-conservative pressure)
+ * followed by the boundary flux sums of domain_mod.f90:1483-1499. Returns max_dt. */
+static double compute_fluxes(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    const size_t n = (size_t)nx * ny;
+    const double EPS = 1.0e-12;
+    const double EPS_gs = sqrt(GRAVITY * MIN_DEPTH);
+    const double half_gravity = 0.5 * GRAVITY;
+    const double half_cfl = 0.5 * d->cfl;
+    double *stg = P(d, STG), *uhp = P(d, UH), *vhp = P(d, VH), *elv = P(d, ELV);
+    double *u = d->vel, *v = d->vel + n;
+    double *fNS = d->flux_NS, *fEW = d->flux_EW;
+    const size_t nNS = (size_t)nx * (ny + 1), nEW = (size_t)(nx + 1) * ny;
+#define FNS(i, j, k) fNS[(size_t)(k) * nNS + (size_t)((j) - 1) * nx + ((i) - 1)]
+#define FEW(i, j, k) fEW[(size_t)(k) * nEW + (size_t)((j) - 1) * (nx + 1) + ((i) - 1)]
+#define ES(i, j, k) d->es[(size_t)((k) - 1) * n + IX(d, i, j)]
+#define ESJ(i, j) d->es_vh_jm1[(size_t)((j) - 1) * nx + ((i) - 1)]
+
+    compute_depth_and_velocity(d);
+
+    double max_dt = d->maximum_timestep;
+    double *work = malloc(sizeof(double) * nx);
+    for (int i = 0; i < nx; i++) work[i] = 1.0 / max_dt;
+    double *thL = calloc(nx, sizeof(double)), *th = calloc(nx, sizeof(double)), *thE = calloc(nx, sizeof(double));
+    double *gl[4], *gc[4], *ge[4];
+    for (int q = 0; q < 4; q++) {
+        gl[q] = calloc(nx, sizeof(double)); gc[q] = calloc(nx, sizeof(double)); ge[q] = calloc(nx, sizeof(double));
+    }
+    double *es_im1 = calloc(nx, sizeof(double));
+    ns_gradients(d, 1, th, gc);
+    for (int j = 2; j <= ny; j++) {
+        for (int i = 1; i <= nx; i++) { ES(i, j, 1) = 0.0; ES(i, j, 2) = 0.0; ESJ(i, j) = 0.0; es_im1[i - 1] = 0.0; }
+        double dx_cfl_half_inv1 = 1.0 / (d->dbe[j - 1] * half_cfl);
+        double dx_cfl_half_inv2 = 1.0 / (d->dle[0] * half_cfl);
+        for (int q = 0; q < 4; q++) { double *t = gl[q]; gl[q] = gc[q]; gc[q] = t; }
+        { double *t = thL; thL = th; th = t; }
+        ns_gradients(d, j, th, gc);
+        for (int i = 2; i <= nx; i++) {
+            /* NS edge between (i, j-1) and (i, j): include.f90:178-432 */
+            size_t km = IX(d, i, j - 1), kc = IX(d, i, j);
+            double stage_neg = stg[km] + 0.5 * gl[0][i - 1];
+            double stage_pos = stg[kc] - 0.5 * gc[0][i - 1];
+            double depth_neg = d->depth[km] + 0.5 * gl[1][i - 1];
+            double depth_pos = d->depth[kc] - 0.5 * gc[1][i - 1];
+            double u_neg = u[km] + 0.5 * gl[2][i - 1], u_pos = u[kc] - 0.5 * gc[2][i - 1];
+            double v_neg = v[km] + 0.5 * gl[3][i - 1], v_pos = v[kc] - 0.5 * gc[3][i - 1];
+            double depth_neg_c = d->depth[km], depth_pos_c = d->depth[kc];
+            double z_neg = stage_neg - depth_neg, z_pos = stage_pos - depth_pos;
+            double z_half = dmax(z_neg, z_pos);
+            double sns = dmax(stage_neg, z_half), dns = sns - z_half;
+            if (dns == 0.0) { v_neg = 0.0; u_neg = 0.0; }
+            double vd_neg = v_neg * dns, ud_neg = u_neg * dns;
+            double gs_neg = sqrt(GRAVITY * dns);
+            double sps = dmax(stage_pos, z_half), dps = sps - z_half;
+            if (dps == 0.0) { v_pos = 0.0; u_pos = 0.0; }
+            double vd_pos = v_pos * dps, ud_pos = u_pos * dps;
+            double gs_pos = sqrt(GRAVITY * dps);
+            double vbn = v_neg * 1.0, vbp = v_pos * 1.0;
+            double s_max = dmax(dmax(vbn + gs_neg, vbp + gs_pos), 0.0);
+            double s_min = dmin(dmin(vbn - gs_neg, vbp - gs_pos), 0.0);
+            double denom = s_max - s_min;
+            double inv_denom = (denom > EPS && (gs_pos > EPS_gs || gs_neg > EPS_gs)) ? d->dbe[j - 1] / denom : 0.0;
+            double sminsmax = s_min * s_max * 1.0;
+            double fr2 = 1.0 * sqrt(dmax(0.001, dmin(1.0, (v_neg * v_neg + v_pos * v_pos + u_neg * u_neg + u_pos * u_pos) /
+                                                              (gs_neg * gs_neg + gs_pos * gs_pos + 1.0e-10))));
+            FNS(i, j, 0) = (s_max * vd_neg - s_min * vd_pos + sminsmax * (sps - sns)) * inv_denom;
+            FNS(i, j, 1) = (s_max * ud_neg * vbn - s_min * ud_pos * vbp + fr2 * sminsmax * (ud_pos - ud_neg)) * inv_denom;
+            FNS(i, j, 2) = (s_max * vd_neg * vbn - s_min * vd_pos * vbp + fr2 * sminsmax * (vd_pos - vd_neg)) * inv_denom;
+            double hghh = inv_denom * half_gravity * (s_max * dns * dns - s_min * dps * dps);
+            double bsp_n = half_gravity * ((dns - depth_neg) * (dns + depth_neg) -
+                                           (depth_neg + depth_neg_c) * (z_neg - elv[km]));
+            ESJ(i, j) = bsp_n * d->dbe[j - 1] - hghh;
+            double bsp_s = half_gravity * ((dps - depth_pos) * (dps + depth_pos) -
+                                           (depth_pos + depth_pos_c) * (z_pos - elv[kc]));
+            ES(i, j, 2) = ES(i, j, 2) - bsp_s * d->dbe[j - 1] + hghh;
+            double max_speed = dmax(s_max, -s_min);
+            if (max_speed > EPS) work[i - 1] = dmax(work[i - 1], max_speed * dx_cfl_half_inv2);
+        }
+        ew_gradients(d, j, thE, ge);
+        for (int i = 2; i <= nx; i++) {
+            /* EW edge between (i-1, j) and (i, j): include.f90:454-718 */
+            size_t km = IX(d, i - 1, j), kc = IX(d, i, j);
+            double stage_neg = stg[km] + 0.5 * ge[0][i - 2];
+            double stage_pos = stg[kc] - 0.5 * ge[0][i - 1];
+            double depth_neg = d->depth[km] + 0.5 * ge[1][i - 2];
+            double depth_pos = d->depth[kc] - 0.5 * ge[1][i - 1];
+            double u_neg = u[km] + 0.5 * ge[2][i - 2], u_pos = u[kc] - 0.5 * ge[2][i - 1];
+            double v_neg = v[km] + 0.5 * ge[3][i - 2], v_pos = v[kc] - 0.5 * ge[3][i - 1];
+            double depth_neg_c = d->depth[km], depth_pos_c = d->depth[kc];
+            double z_neg = stage_neg - depth_neg, z_pos = stage_pos - depth_pos;
+            double z_half = dmax(z_neg, z_pos);
+            double sns = dmax(stage_neg, z_half), dns = sns - z_half;
+            if (dns == 0.0) { u_neg = 0.0; v_neg = 0.0; }
+            double ud_neg = u_neg * dns, vd_neg = v_neg * dns;
+            double gs_neg = sqrt(GRAVITY * dns);
+            double sps = dmax(stage_pos, z_half), dps = sps - z_half;
+            if (dps == 0.0) { u_pos = 0.0; v_pos = 0.0; }
+            double ud_pos = u_pos * dps, vd_pos = v_pos * dps;
+            double gs_pos = sqrt(GRAVITY * dps);
+            double vbn = u_neg * 1.0, vbp = u_pos * 1.0;
+            double s_max = dmax(dmax(vbn + gs_neg, vbp + gs_pos), 0.0);
+            double s_min = dmin(dmin(vbn - gs_neg, vbp - gs_pos), 0.0);
+            double denom = s_max - s_min;
+            double inv_denom = (denom > EPS && (gs_pos > EPS_gs || gs_neg > EPS_gs)) ? d->dle[i - 1] / denom : 0.0;
+            double sminsmax = s_min * s_max * 1.0;
+            double fr2 = 1.0 * sqrt(dmax(0.001, dmin(1.0, (v_neg * v_neg + v_pos * v_pos + u_neg * u_neg + u_pos * u_pos) /
+                                                              (gs_neg * gs_neg + gs_pos * gs_pos + 1.0e-10))));
+            FEW(i, j, 0) = (s_max * ud_neg - s_min * ud_pos + sminsmax * (sps - sns)) * inv_denom;
+            FEW(i, j, 1) = (s_max * ud_neg * vbn - s_min * ud_pos * vbp + fr2 * sminsmax * (ud_pos - ud_neg)) * inv_denom;
+            FEW(i, j, 2) = (s_max * vd_neg * vbn - s_min * vd_pos * vbp + fr2 * sminsmax * (vd_pos - vd_neg)) * inv_denom;
+            double hghh = inv_denom * half_gravity * (s_max * dns * dns - s_min * dps * dps);
+            double bsp_e = half_gravity * ((dns - depth_neg) * (dns + depth_neg) -
+                                           (depth_neg + depth_neg_c) * (z_neg - elv[km]));
+            es_im1[i - 2] = bsp_e * d->dle[i - 1] - hghh;
+            double bsp_w = half_gravity * ((dps - depth_pos) * (dps + depth_pos) -
+                                           (depth_pos + depth_pos_c) * (z_pos - elv[kc]));
+            ES(i, j, 1) = ES(i, j, 1) - bsp_w * d->dle[i - 1] + hghh;
+            if (d->spherical) {
+                /* include.f90:674-710 */
+                ES(i, j, 2) = ES(i, j, 2) + half_gravity * d->depth[kc] * d->depth[kc] * (d->dbe[j] - d->dbe[j - 1]);
+                double cm;
+                if (d->coriolis_on) {
+                    cm = d->area[j - 1] * d->cor[j - 1];
+                    ES(i, j, 1) = ES(i, j, 1) + cm * vhp[kc];
+                    ES(i, j, 2) = ES(i, j, 2) - cm * uhp[kc];
+                }
+                cm = d->area[j - 1] * d->tanlat[j - 1] * u[kc];
+                ES(i, j, 1) = ES(i, j, 1) + cm * vhp[kc];
+                ES(i, j, 2) = ES(i, j, 2) - cm * uhp[kc];
+            }
+            double max_speed = dmax(s_max, -s_min);
+            if (max_speed > EPS) work[i - 1] = dmax(work[i - 1], max_speed * dx_cfl_half_inv1);
+        }
+        for (int i = 1; i <= nx; i++) ES(i, j, 1) = ES(i, j, 1) + es_im1[i - 1];
+    }
+    double max_dt_inv = work[0];
+    for (int i = 1; i < nx; i++) max_dt_inv = dmax(max_dt_inv, work[i]);
+    max_dt = 1.0 / max_dt_inv;
+    free(work); free(thL); free(th); free(thE); free(es_im1);
+    for (int q = 0; q < 4; q++) { free(gl[q]); free(gc[q]); free(ge[q]); }
+
+    /* Boundary flux sums: domain_mod.f90:1483-1499 */
+    const int ne = d->n_ext;
+    double s;
+    s = 0.0; for (int i = 1 + ne; i <= nx - ne; i++) s += FNS(i, ny + 1 - ne, 0);
+    d->boundary_flux_store[0] = s;
+    s = 0.0; for (int j = 1 + ne; j <= ny - ne; j++) s += FEW(nx + 1 - ne, j, 0);
+    d->boundary_flux_store[1] = s;
+    s = 0.0; for (int i = 1 + ne; i <= nx - ne; i++) s += FNS(i, 1 + ne, 0);
+    d->boundary_flux_store[2] = -s;
+    s = 0.0; for (int j = 1 + ne; j <= ny - ne; j++) s += FEW(1 + ne, j, 0);
+    d->boundary_flux_store[3] = -s;
+    return max_dt;
+}
+
+/* update_U_manning / update_U_chezy: domain_mod_update_U_DE1_inner_include.f90:14-99,
+ * then update_boundary (domain_mod.f90:1564-1582). */
+static void update_U(or_domain *d, double dt)
+{
+    const int nx = d->nx, ny = d->ny;
+    const size_t n = (size_t)nx * ny;
+    const size_t nNS = (size_t)nx * (ny + 1), nEW = (size_t)(nx + 1) * ny;
+    double *fNS = d->flux_NS, *fEW = d->flux_EW;
+    const double power = (d->friction_type == 1) ? -2.0 : -7.0 / 3.0;
+    double *u = d->vel, *v = d->vel + n;
+    d->dt_last_update = dt;
+    double dt_gravity = dt * GRAVITY;
+    for (int j = 1; j <= ny; j++) {
+        double inv_cell_area_dt = dt / d->area[j - 1];
+        for (int i = 1; i <= nx; i++)
+            for (int kk = 0; kk < 3; kk++)
+                A(d, kk, i, j) = A(d, kk, i, j) - inv_cell_area_dt * ((FNS(i, j + 1, kk) - FNS(i, j, kk)) +
+                                                                      (FEW(i + 1, j, kk) - FEW(i, j, kk)));
+        for (int i = 1; i <= nx; i++) {
+            size_t k = IX(d, i, j);
+            double depth = d->depth[k];
+            if (A(d, STG, i, j) <= (A(d, ELV, i, j) + MIN_DEPTH)) {
+                A(d, UH, i, j) = 0.0;
+                A(d, VH, i, j) = 0.0;
+            } else {
+                double fs = d->manning_sq[k] * sqrt(u[k] * u[k] + v[k] * v[k]) * pow(dmax(depth, MIN_DEPTH), power);
+                double implicit_factor = 1.0 / (1.0 + dt_gravity * dmax(depth, 0.0) * fs + dt * d->linear_friction_coeff);
+                A(d, UH, i, j) = A(d, UH, i, j) + inv_cell_area_dt * ES(i, j, 1);
+                A(d, UH, i, j) = A(d, UH, i, j) * implicit_factor;
+                A(d, VH, i, j) = A(d, VH, i, j) + inv_cell_area_dt * (ES(i, j, 2) + ESJ(i, j + 1));
+                A(d, VH, i, j) = A(d, VH, i, j) * implicit_factor;
+            }
+        }
+    }
+    d->time = d->time + dt;
+    update_boundary(d);
+#undef FNS
+#undef FEW
+#undef ES
+#undef ESJ
+}
+
+static double boundary_flux_sum(const or_domain *d)
+{
+    return ((d->boundary_flux_store[0] + d->boundary_flux_store[1]) + d->boundary_flux_store[2]) +
+           d->boundary_flux_store[3];
+}
+
+/* one_euler_step: domain_mod_timestepping_alternatives_include.f90:8-62 */
+static void euler_step(or_domain *d, double timestep)
+{
+    double ts = compute_fluxes(d);
+    d->max_dt = ts;
+    if (timestep > 0.0) ts = timestep;
+    update_U(d, ts);
+    d->boundary_flux_evolve_integral = d->boundary_flux_evolve_integral + ts * boundary_flux_sum(d);
+}
+
+/* one_rk2_step: domain_mod_timestepping_alternatives_include.f90:64-146 */
+static void rk2_step(or_domain *d, double timestep)
+{
+    const size_t n = (size_t)d->nx * d->ny;
+    double backup_time = d->time;
+    memcpy(d->backup_U, d->U, 3 * n * sizeof(double));
+    double dt_first_step;
+    if (timestep > 0.0) { euler_step(d, timestep); dt_first_step = timestep; }
+    else { euler_step(d, -1.0); dt_first_step = d->max_dt; }
+    double max_dt_store = d->max_dt;
+    euler_step(d, dt_first_step);
+    for (size_t k = 0; k < 3 * n; k++) d->U[k] = 0.5 * (d->U[k] + d->backup_U[k]);
+    d->time = backup_time + dt_first_step;
+    d->boundary_flux_evolve_integral = 0.5 * d->boundary_flux_evolve_integral;
+    d->max_dt = max_dt_store;
+}
+
+/* update_max_quantities: domain_mod.f90:2243-2404, with all six max_U variables
+ * (so time_of_max_stage is tracked and the 'toms > 0' branch of max_stage applies). */
+static void update_max_quantities(or_domain *d)
+{
+    if (!d->record_max_U) return;
+    const int nx = d->nx, ny = d->ny;
+    const size_t n = (size_t)nx * ny;
+    double *m = d->max_U;
+    const int staggered = (d->method == OR_LINEAR || d->method == OR_LLPNF);
+    const int truely = d->truely_linear && staggered;
+    for (int j = d->yL; j <= d->yU; j++) {
+        for (int i = d->xL; i <= d->xU; i++) {
+            size_t k = IX(d, i, j);
+            double stg = A(d, STG, i, j), uh = A(d, UH, i, j), vh = A(d, VH, i, j), elv = A(d, ELV, i, j);
+            /* include_dry_cells = .true. (domain_mod.f90 default) */
+            if (stg > m[MX_STAGE * n + k]) { m[MX_STAGE * n + k] = stg; m[MX_TOMS * n + k] = d->time; }
+            if (staggered) {
+                int ip1 = i + 1 < nx ? i + 1 : nx, jp1 = j + 1 < ny ? j + 1 : ny;
+                double depth_E, depth_N;
+                if (truely) {
+                    depth_E = 0.5 * ((d->msl_linear - A(d, ELV, i, j)) + (d->msl_linear - A(d, ELV, ip1, j)));
+                    depth_N = 0.5 * ((d->msl_linear - A(d, ELV, i, j)) + (d->msl_linear - A(d, ELV, i, jp1)));
+                } else {
+                    depth_E = 0.5 * ((A(d, STG, i, j) - A(d, ELV, i, j)) + (A(d, STG, ip1, j) - A(d, ELV, ip1, j)));
+                    depth_N = 0.5 * ((A(d, STG, i, j) - A(d, ELV, i, j)) + (A(d, STG, i, jp1) - A(d, ELV, i, jp1)));
+                }
+                double iE = depth_E > MIN_DEPTH ? 1.0 / depth_E : 0.0;
+                double iN = depth_N > MIN_DEPTH ? 1.0 / depth_N : 0.0;
+                double sp = sqrt(uh * uh * (iE * iE) + vh * vh * (iN * iN));
+                m[MX_SPEED * n + k] = dmax(m[MX_SPEED * n + k], sp);
+            } else {
+                double ld = stg - elv;
+                double li = ld > MIN_DEPTH ? 1.0 / ld : 0.0;
+                m[MX_SPEED * n + k] = dmax(m[MX_SPEED * n + k], sqrt((uh * uh + vh * vh) * (li * li)));
+            }
+            m[MX_FLUX * n + k] = dmax(m[MX_FLUX * n + k], sqrt(uh * uh + vh * vh));
+            {
+                double ld = stg - elv;
+                double arrival_stage = d->msl_linear + d->arrival_stage_threshold;
+                if (ld > MIN_DEPTH && stg > arrival_stage && m[MX_ARRIVAL * n + k] < 0.0)
+                    m[MX_ARRIVAL * n + k] = d->time;
+            }
+            m[MX_MIN_STAGE * n + k] = dmin(m[MX_MIN_STAGE * n + k], stg);
+        }
+    }
+}
+
+/* evolve_one_step: domain_mod.f90:1679-1813 (max_U_update_frequency = 1) */
+static void evolve_one_step(or_domain *d, double dt)
+{
+    double time0 = d->time;
+    d->boundary_flux_evolve_integral = 0.0;
+    if (d->nsteps_advanced == 0) update_max_quantities(d);
+    switch (d->method) {
+    case OR_RK2: rk2_step(d, dt); break;
+    case OR_EULER: euler_step(d, dt); break;
+    case OR_LINEAR:
+    case OR_LLPNF: leapfrog_step(d, dt); break;
+    default: break;
+    }
+    update_max_quantities(d);
+    d->evolve_step_dt = d->time - time0;
+    d->boundary_flux_time_integral = d->boundary_flux_time_integral + d->boundary_flux_evolve_integral;
+    d->nsteps_advanced += 1;
+}
+
+double or_domain_evolve(or_domain *d, double dt, int nsteps)
+{
+    for (int k = 0; k < nsteps; k++) evolve_one_step(d, dt);
+    return d->time;
+}
+
+void or_domain_get_scalars(const or_domain *d, double *out)
+{
+    out[0] = d->time; out[1] = d->max_dt; out[2] = d->evolve_step_dt;
+    out[3] = d->boundary_flux_time_integral; out[4] = d->boundary_flux_evolve_integral;
+    for (int k = 0; k < 4; k++) out[5 + k] = d->boundary_flux_store[k];
+    out[9] = d->negative_depth_fix_counter; out[10] = d->nsteps_advanced; out[11] = d->dt_last_update;
+}
+
+/* volume_interior: domain_mod.f90:1815-1849 (serial row order) */
+double or_domain_volume_interior(const or_domain *d)
+{
+    const int ne = d->n_ext;
+    double vol = 0.0;
+    for (int j = 1 + ne; j <= d->ny - ne; j++) {
+        double local_sum = 0.0;
+        for (int i = 1 + ne; i <= d->nx - ne; i++) local_sum += A(d, STG, i, j) - A(d, ELV, i, j);
+        vol = vol + d->area[j - 1] * local_sum;
+    }
+    return vol;
+}
+
+/* stationary_timestep_max: domain_mod.f90:1906-2022 */
+double or_domain_stationary_timestep_max(const or_domain *d)
+{
+    double ts_max = HUGE_VAL;
+    const int linear = (d->method == OR_LINEAR || d->method == OR_LLPNF) && d->truely_linear;
+    for (int j = 1; j <= d->ny; j++) {
+        for (int i = 1; i <= d->nx; i++) {
+            double h = linear ? d->msl_linear - A(d, ELV, i, j) : A(d, STG, i, j) - A(d, ELV, i, j);
+            double dist = dmin(d->dbe[j - 1] + d->dbe[j], d->dle[i - 1] + d->dle[i]);
+            ts_max = dmin(ts_max, 0.5 * dist / sqrt(GRAVITY * dmax(h, MIN_DEPTH)));
+        }
+    }
+    double ts = ts_max * d->cfl;
+    if (d->method == OR_RK2 || d->method == OR_EULER) ts = ts * 0.5;
+    return ts;
+}
+
+void or_domain_get_max_U(const or_domain *d, int k, double *out)
+{
+    const size_t n = (size_t)d->nx * d->ny;
+    memcpy(out, d->max_U + (size_t)(k - 1) * n, n * sizeof(double));
+}
+
+/* update_gauge_var: point_gauge_mod.f90:208-245 ([STG, UH, VH] at 1-based (i, j)) */
+void or_domain_get_gauges(const or_domain *d, int n, const int *ij, double *out)
+{
+    for (int g = 0; g < n; g++)
+        for (int k = 0; k < 3; k++) out[(size_t)k * n + g] = A(d, k, ij[2 * g], ij[2 * g + 1]);
+}
