@@ -1,0 +1,155 @@
+/*
+ * swals_hip.h -- C ABI of the MI355X (gfx950) time-stepping backend for SWALS.
+ *
+ * The backend replaces the per-step loops behind SWALS's Fortran domain_type API.
+ * A Fortran host binds these symbols with ISO_C_BINDING interfaces
+ * (ptha_amd/fortran/swals_hip_mod.f90); Python binds them with ctypes
+ * (ptha_amd/backend.py). Signatures use plain pointers, sizes and C scalars only.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative SWH_E* code on failure;
+ *    swh_last_error() returns the message. A Fortran caller maps a non-zero status to
+ *    generic_stop (propagation/SWALS/src/util/stop_mod.f90:8-33), which is how the
+ *    reference reports fatal errors.
+ *  - Host arrays are in the reference's layout: U is Fortran U(nx, ny, 4),
+ *    column-major, i fastest, planes [STG, UH, VH, ELV]
+ *    (propagation/SWALS/src/shallow_water/domain_mod.f90:79,332). 2-D fields are
+ *    (nx, ny). Indices passed in (gauge cells) are 1-based (i, j) pairs.
+ *  - Host arrays are FP64 (the reference's default real(dp) = C_DOUBLE,
+ *    global_mod.f90:15-20) in both precision modes; the device converts.
+ *  - Calls on one handle must come from one host thread; the handle owns its device
+ *    memory and its HIP stream.
+ *
+ * Reference interfaces replaced are cited per entry point (paths relative to
+ * propagation/SWALS/src/shallow_water/ unless stated).
+ */
+#ifndef SWALS_HIP_H
+#define SWALS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes */
+#define SWH_OK 0
+#define SWH_EINVAL (-1)    /* invalid argument */
+#define SWH_EHIP (-2)      /* HIP runtime error */
+#define SWH_ESTATE (-3)    /* call not valid in the current state */
+#define SWH_ENOMEM (-4)    /* device allocation failed */
+
+/* timestepping_method ids: the index of the method in the reference's
+ * timestepping_metadata table (timestepping_metadata_mod.f90:69-140). */
+#define SWH_METHOD_RK2 1
+#define SWH_METHOD_LINEAR 5
+#define SWH_METHOD_LLPNF 6 /* 'leapfrog_linear_plus_nonlinear_friction' */
+
+/* Arithmetic of the bulk update */
+#define SWH_PREC_FP64 0    /* FP64 throughout (the reference's default dp = C_DOUBLE) */
+#define SWH_PREC_FP32K 1   /* FP32 state, Kahan-compensated stage, FP64 metrics/reductions */
+
+/* max_U tracking mask bits; names and order as in the reference's
+ * gridded_output_variables_max_U (util/netcdf_util_mod.f90:39-41) */
+#define SWH_MAXQ_MAX_STAGE 1
+#define SWH_MAXQ_MAX_SPEED 2
+#define SWH_MAXQ_MAX_FLUX 4
+#define SWH_MAXQ_ARRIVAL_TIME 8
+#define SWH_MAXQ_TIME_OF_MAX_STAGE 16
+#define SWH_MAXQ_MIN_STAGE 32
+
+typedef struct swh_domain swh_domain;
+
+/* Static description of one domain (domain_type fields, domain_mod.f90:137-592).
+ * Metric arrays are the FP64 arrays that allocate_quantities builds
+ * (domain_mod.f90:933-1014); the device copies them. */
+typedef struct swh_domain_desc {
+    int32_t nx, ny;                       /* domain%nx(1:2) */
+    int32_t timestepping_method;          /* SWH_METHOD_* */
+    int32_t precision;                    /* SWH_PREC_* */
+    int32_t linear_solver_is_truely_linear;
+    int32_t include_coriolis;             /* Coriolis terms (spherical builds without -DNOCORIOLIS) */
+    int32_t spherical;                    /* spherical metric source terms of the FV scheme */
+    int32_t friction_type;                /* 0 = 'manning', 1 = 'chezy' */
+    int32_t exterior_cells_width;         /* domain%exterior_cells_width (default 2) */
+    int32_t boundary_exterior[4];         /* N, E, S, W */
+    int32_t max_U_mask;                   /* SWH_MAXQ_* bits; 0 disables record_max_U */
+    int32_t max_U_update_frequency;       /* domain%max_U_update_frequency (>= 1) */
+    double msl_linear;
+    double cfl;                           /* <= 0: method default */
+    double theta;                         /* <= 0: method default */
+    double linear_friction_coeff;
+    double maximum_timestep;              /* <= 0: 1e20 (global_mod.f90:49) */
+    double arrival_stage_threshold_above_msl_linear;
+    const double *distance_bottom_edge;   /* ny+1 */
+    const double *distance_left_edge;     /* nx+1 */
+    const double *area_cell_y;            /* ny */
+    const double *coriolis;               /* ny   (may be NULL when include_coriolis == 0) */
+    const double *coriolis_bottom_edge;   /* ny+1 (may be NULL when include_coriolis == 0) */
+    const double *tanlat_on_radius_earth; /* ny   (may be NULL when spherical == 0) */
+} swh_domain_desc;
+
+/* Scalars of domain_type after the latest step (domain_mod.f90:160-240). */
+typedef struct swh_scalars {
+    double time;
+    double max_dt;
+    double evolve_step_dt;
+    double dt_last_update;
+    double boundary_flux_store[4];
+    double boundary_flux_evolve_integral;
+    double boundary_flux_time_integral;
+    double negative_depth_fix_counter;
+    int64_t nsteps_advanced;
+} swh_scalars;
+
+/* Message of the last error on this host thread (NUL-terminated, truncated to len). */
+int swh_last_error(char *buf, int len);
+
+/* Replaces allocate_quantities' device-side allocation (domain_mod.f90:852-1358). */
+int swh_domain_create(const swh_domain_desc *desc, swh_domain **out);
+int swh_domain_destroy(swh_domain *h);
+
+/* Host <-> device copies of domain%U (domain_mod.f90:332) and fields. */
+int swh_upload_U(swh_domain *h, const double *U);
+int swh_download_U(swh_domain *h, double *U);
+int swh_upload_manning_squared(swh_domain *h, const double *manning_squared); /* domain_mod.f90:347 */
+int swh_set_time(swh_domain *h, double time);
+
+/* Stage boundary: boundary_stage_transmissive_normal_momentum (boundary_mod.f90:78-164)
+ * evaluated on the device for the flagged sides (N, E, S, W) with the stage function
+ * msl + amp * sin(2 pi t / period). Replaces the host callback domain%boundary_subroutine
+ * for this case. */
+int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double msl, const int32_t sides[4]);
+
+/* domain%evolve_one_step(timestep) (domain_mod.f90:1679-1813). timestep <= 0 means
+ * "not present": only valid for rk2, which then takes the CFL step computed on the
+ * device. nsteps > 1 repeats the call without host synchronisation. */
+int swh_evolve_one_step(swh_domain *h, double timestep);
+int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps);
+
+int swh_get_scalars(swh_domain *h, swh_scalars *out);
+
+/* volume_interior (domain_mod.f90:1815-1849), FP64 compensated, deterministic. */
+int swh_volume_interior(swh_domain *h, double *out);
+
+/* stationary_timestep_max (domain_mod.f90:1964-2022) on the current state. */
+int swh_stationary_timestep_max(swh_domain *h, double *out);
+
+/* One max_U plane (update_max_quantities, domain_mod.f90:2243-2404); which = one
+ * SWH_MAXQ_* bit; out is (nx, ny) FP64. */
+int swh_download_max_U(swh_domain *h, int32_t which, double *out);
+
+/* Point gauges (point_gauge_mod.f90:208-294): n cells given as 1-based (i, j) pairs.
+ * swh_gauges_sample records [STG, UH, VH] of every gauge at the current time into a
+ * device time-series buffer; swh_gauges_download returns
+ * out[(sample * 3 + var) * n + gauge] and the sample times. */
+int swh_gauges_set(swh_domain *h, int32_t n, const int32_t *ij, int32_t max_samples);
+int swh_gauges_sample(swh_domain *h);
+int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *nsamples);
+
+int swh_synchronize(swh_domain *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,691 @@
+// C ABI of the gfx950 SWALS backend: handle lifetime, host <-> device copies and
+// the per-step orchestration that domain%evolve_one_step performs in the reference
+// (propagation/SWALS/src/shallow_water/domain_mod.f90:1679-1813).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "swh_internal.h"
+
+namespace swh {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    char buf[512];
+    snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    g_err = buf;
+    return e == hipErrorOutOfMemory ? SWH_ENOMEM : SWH_EHIP;
+}
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+}  // namespace swh
+
+using namespace swh;
+
+struct swh_domain {
+    int nx = 0, ny = 0, pitch = 0, fpitch = 0;
+    int method = 0, prec = 0, truely_linear = 1, coriolis = 0, spherical = 0, friction_type = 0, n_ext = 2;
+    int bex[4] = {1, 1, 1, 1};
+    int maxq_mask = 0, maxq_freq = 1;
+    double msl = 0.0, cfl = 0.7, theta = 1.6, lfc = 0.0, max_timestep = kMaxTimestep, arrival_thresh = 0.01;
+    size_t esz = 8;  // bytes per state element
+    hipStream_t stream = nullptr;
+    // host copies of the FP64 metrics
+    std::vector<double> dbe, dle, area, cor, corbe, tanlat;
+    std::vector<double> h_elev, h_nsq;
+    // device metrics (FP64)
+    double *d_dbe = nullptr, *d_dle = nullptr, *d_area = nullptr, *d_cor = nullptr, *d_tanlat = nullptr;
+    // state: ping-pong S, UH, VH (+ Kahan C), shared E
+    void *S[2] = {}, *UH[2] = {}, *VH[2] = {}, *C[2] = {}, *E = nullptr;
+    int cur = 0;
+    void *fwU = nullptr, *fwV = nullptr, *nsq = nullptr, *mx = nullptr;
+    // FV work
+    void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
+    double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
+    // leapfrog row coefficients for rc_dt
+    void *rc = nullptr;
+    double rc_dt = -1.0;
+    DevScalars *sc = nullptr;
+    double *d_work = nullptr;  // reductions scratch (2*ny + 8 doubles)
+    // host-side scalars (leapfrog keeps time on the host; rk2 on the device)
+    double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;
+    long long nsteps = 0;
+    int fw_ready = 0;
+    // boundary
+    int bc_mask = 0;
+    double bc_amp = 0.0, bc_period = 1.0, bc_msl = 0.0;
+    // gauges
+    int ng = 0, gmax = 0, gsamples = 0;
+    long long *d_gidx = nullptr;
+    double *d_gval = nullptr, *d_gtime = nullptr;
+    // launch shape
+    int block_x = 256, rows_per_block = 32;
+
+    bool leapfrog() const { return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF; }
+    size_t plane_elems() const { return (size_t)pitch * ny; }
+};
+
+namespace {
+
+int dmalloc(void **p, size_t bytes)
+{
+    SWH_HIP(hipMalloc(p, bytes));
+    SWH_HIP(hipMemset(*p, 0, bytes));
+    return SWH_OK;
+}
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+template <typename T>
+int upload_plane(swh_domain *h, void *dst, const double *src)
+{
+    std::vector<T> tmp(h->plane_elems(), T(0));
+    for (int j = 0; j < h->ny; j++)
+        for (int i = 0; i < h->nx; i++) tmp[(size_t)j * h->pitch + i] = (T)src[(size_t)j * h->nx + i];
+    SWH_HIP(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SWH_OK;
+}
+
+template <typename T>
+int download_plane(swh_domain *h, const void *src, const void *comp, double *dst)
+{
+    std::vector<T> tmp(h->plane_elems()), tc;
+    SWH_HIP(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+    if (comp) {
+        tc.resize(h->plane_elems());
+        SWH_HIP(hipMemcpy(tc.data(), comp, tc.size() * sizeof(T), hipMemcpyDeviceToHost));
+    }
+    for (int j = 0; j < h->ny; j++)
+        for (int i = 0; i < h->nx; i++) {
+            size_t o = (size_t)j * h->pitch + i;
+            dst[(size_t)j * h->nx + i] = comp ? (double)tmp[o] - (double)tc[o] : (double)tmp[o];
+        }
+    return SWH_OK;
+}
+
+int upload_plane_any(swh_domain *h, void *dst, const double *src)
+{
+    return h->prec == SWH_PREC_FP64 ? upload_plane<double>(h, dst, src) : upload_plane<float>(h, dst, src);
+}
+
+// friction_work for the truely-linear leapfrog + friction solver, built once on the
+// host in FP64 (domain_mod_leapfrog_solver_friction_include.f90:21-56).
+int build_friction_work(swh_domain *h)
+{
+    if (h->method != SWH_METHOD_LLPNF || !h->truely_linear) return SWH_OK;
+    const int nx = h->nx, ny = h->ny;
+    const double power = (h->friction_type == 1) ? -2.0 : -7.0 / 3.0;
+    std::vector<double> fu((size_t)nx * ny), fv((size_t)nx * ny);
+    const double msl = h->msl;
+    const double *e = h->h_elev.data(), *n2 = h->h_nsq.data();
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int ip1 = std::min(i + 1, nx - 1), jp1 = std::min(j + 1, ny - 1);
+            const size_t k = (size_t)j * nx + i;
+            double depth_iph = 0.5 * (msl - e[k] + msl - e[(size_t)j * nx + ip1]);
+            double depth_jph = 0.5 * (msl - e[k] + msl - e[(size_t)jp1 * nx + i]);
+            depth_iph = std::max(depth_iph, kMinDepth);
+            depth_jph = std::max(depth_jph, kMinDepth);
+            const double nsq_iph = 0.5 * (n2[k] + n2[(size_t)j * nx + ip1]);
+            const double nsq_jph = 0.5 * (n2[k] + n2[(size_t)jp1 * nx + i]);
+            fu[k] = kGravity * nsq_iph * std::pow(depth_iph, power);
+            fv[k] = kGravity * nsq_jph * std::pow(depth_jph, power);
+        }
+    int rc = upload_plane_any(h, h->fwU, fu.data());
+    if (rc) return rc;
+    rc = upload_plane_any(h, h->fwV, fv.data());
+    if (rc) return rc;
+    h->fw_ready = 1;
+    return SWH_OK;
+}
+
+// Leapfrog per-row coefficients for time step dt, in FP64 with the reference's
+// operation order, then converted to the storage type.
+template <typename T>
+int build_rc(swh_domain *h, double dt)
+{
+    const int nx = h->nx, ny = h->ny;
+    std::vector<T> buf((size_t)5 * ny + (nx + 1) + (ny + 1));
+    T *sa = buf.data(), *cu = sa + ny, *cv = cu + ny, *fu = cv + ny, *fv = fu + ny, *dle = fv + ny,
+      *dbe = dle + nx + 1;
+    for (int j = 0; j < ny; j++) {
+        sa[j] = (T)(dt / h->area[j]);
+        cu[j] = (T)(kGravity * dt / h->area[j]);
+        cv[j] = (j + 1 < ny) ? (T)(kGravity * dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
+        fu[j] = h->coriolis ? (T)(dt * h->cor[j] * 0.5) : T(0);
+        fv[j] = h->coriolis ? (T)(dt * h->corbe[j + 1] * 0.5) : T(0);
+    }
+    for (int i = 0; i <= nx; i++) dle[i] = (T)h->dle[i];
+    for (int j = 0; j <= ny; j++) dbe[j] = (T)h->dbe[j];
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(h->rc, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice));
+    h->rc_dt = dt;
+    return SWH_OK;
+}
+
+template <typename T>
+LFRowCoef<T> rc_view(swh_domain *h)
+{
+    const int nx = h->nx, ny = h->ny;
+    const T *b = (const T *)h->rc;
+    LFRowCoef<T> r;
+    r.sa = b;
+    r.cu = b + ny;
+    r.cv = b + 2 * ny;
+    r.fu = b + 3 * ny;
+    r.fv = b + 4 * ny;
+    r.dle = b + 5 * ny;
+    r.dbe = b + 5 * ny + nx + 1;
+    return r;
+}
+
+template <typename T>
+int maxq_now(swh_domain *h, const DevScalars *tsrc, double t)
+{
+    if (!h->maxq_mask) return SWH_OK;
+    const int c = h->cur;
+    SWH_HIP(launch_maxq<T>((const T *)h->S[c], (const T *)h->C[c], (const T *)h->UH[c], (const T *)h->VH[c],
+                           (const T *)h->E, h->nx, h->ny, h->pitch, h->leapfrog() ? 1 : 0, h->truely_linear,
+                           (T)h->msl, h->maxq_mask, (T *)h->mx, tsrc, (T)t, (T)(h->msl + h->arrival_thresh),
+                           h->stream));
+    return SWH_OK;
+}
+
+template <typename T, bool KAHAN>
+int leapfrog_step(swh_domain *h, double dt)
+{
+    if (h->rc_dt != dt) {
+        int rc = build_rc<T>(h, dt);
+        if (rc) return rc;
+    }
+    const double time0 = h->time;
+    if (h->nsteps == 0) {
+        int rc = maxq_now<T>(h, nullptr, h->time);
+        if (rc) return rc;
+    }
+    const int c = h->cur, o = 1 - c;
+    const double t_half = h->time + dt * 0.5;
+    const double t_after = t_half + 0.5 * dt;
+    // boundary fluxes read the pre-momentum UH/VH (include.f90:126-165)
+    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
+                                     h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+    LFArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
+    a.S = (const T *)h->S[c]; a.C = (const T *)h->C[c]; a.UH = (const T *)h->UH[c]; a.VH = (const T *)h->VH[c];
+    a.E = (const T *)h->E;
+    a.So = (T *)h->S[o]; a.Co = (T *)h->C[o]; a.UHo = (T *)h->UH[o]; a.VHo = (T *)h->VH[o];
+    a.rc = rc_view<T>(h);
+    a.fwU = (const T *)h->fwU; a.fwV = (const T *)h->fwV; a.nsq = (const T *)h->nsq;
+    a.msl = (T)h->msl; a.dt = (T)dt; a.dt_lfc = (T)(dt * h->lfc);
+    a.lin_fric_mult = (T)(1.0 / (1.0 + dt * h->lfc));
+    a.fric_power = (T)((h->friction_type == 1) ? -2.0 : -7.0 / 3.0);
+    a.bc_mask = h->bc_mask;
+    a.bc_stage = (T)(h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period));
+    a.rows_per_block = h->rows_per_block;
+    const bool upd = h->maxq_mask && ((h->nsteps % h->maxq_freq) == 0);
+    a.maxq_mask = upd ? h->maxq_mask : 0;
+    a.mx = (T *)h->mx;
+    a.t_after = (T)t_after;
+    a.arrival_stage = (T)(h->msl + h->arrival_thresh);
+    int fric_mode = 0;
+    if (h->method == SWH_METHOD_LLPNF) fric_mode = h->truely_linear ? 1 : 2;
+    if (fric_mode == 1 && !h->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload U first");
+    SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
+                                      h->stream));
+    h->cur = o;
+    h->time = t_after;
+    h->dt_last_update = dt;
+    h->evolve_step_dt = h->time - time0;
+    h->nsteps++;
+    return SWH_OK;
+}
+
+template <typename T>
+FVArgs<T> fv_args(swh_domain *h, int dst, int src)
+{
+    FVArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.fpitch = h->fpitch;
+    a.S = (T *)h->S[dst]; a.UH = (T *)h->UH[dst]; a.VH = (T *)h->VH[dst];
+    a.S0 = (const T *)h->S[src]; a.UH0 = (const T *)h->UH[src]; a.VH0 = (const T *)h->VH[src];
+    a.E = (const T *)h->E;
+    a.depth = (T *)h->depth; a.u = (T *)h->u; a.v = (T *)h->v;
+    a.fNS = (T *)h->fNS; a.fEW = (T *)h->fEW;
+    a.esU = h->esU; a.esV = h->esV; a.esJ = h->esJ; a.esE = h->esE;
+    a.nsq = (const T *)h->nsq;
+    a.dbe = h->d_dbe; a.dle = h->d_dle; a.area = h->d_area; a.cor = h->d_cor; a.tanlat = h->d_tanlat;
+    a.theta = h->theta; a.half_cfl = 0.5 * h->cfl; a.lfc = h->lfc;
+    a.fric_power = (h->friction_type == 1) ? -2.0 : -7.0 / 3.0;
+    a.spherical = h->spherical; a.coriolis = h->coriolis; a.n_ext = h->n_ext;
+    a.sc = h->sc;
+    return a;
+}
+
+template <typename T>
+int rk2_step(swh_domain *h, double dt)
+{
+    if (h->nsteps == 0) {
+        int rc = maxq_now<T>(h, h->sc, 0.0);
+        if (rc) return rc;
+    }
+    const int A = h->cur, B = 1 - A;
+    SWH_HIP(launch_fv_begin(h->sc, h->stream));
+    // first Euler stage: A -> B (A stays as backup_U)
+    FVArgs<T> a1 = fv_args<T>(h, B, A);
+    SWH_HIP(launch_fv_euler<T>(a1, dt, 1, h->stream));
+    if (h->bc_mask)
+        SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], h->nx, h->ny, h->pitch, h->bc_mask,
+                                      h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
+    // second Euler stage with the same step, in place on B
+    FVArgs<T> a2 = fv_args<T>(h, B, B);
+    SWH_HIP(launch_fv_euler<T>(a2, dt, 0, h->stream));
+    if (h->bc_mask)
+        SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], h->nx, h->ny, h->pitch, h->bc_mask,
+                                      h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
+    SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A], (const T *)h->UH[A],
+                                 (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->stream));
+    SWH_HIP(launch_fv_finish(h->sc, h->stream));
+    h->cur = B;
+    const bool upd = h->maxq_mask && ((h->nsteps % h->maxq_freq) == 0);
+    if (upd) {
+        int rc = maxq_now<T>(h, h->sc, 0.0);
+        if (rc) return rc;
+    }
+    h->nsteps++;
+    return SWH_OK;
+}
+
+int evolve(swh_domain *h, double dt)
+{
+    if (h->leapfrog()) {
+        if (!(dt > 0.0)) return fail(SWH_EINVAL, "timestep must be provided for leapfrog-type evolve_one_step");
+        return h->prec == SWH_PREC_FP64 ? leapfrog_step<double, false>(h, dt) : leapfrog_step<float, true>(h, dt);
+    }
+    return h->prec == SWH_PREC_FP64 ? rk2_step<double>(h, dt) : rk2_step<float>(h, dt);
+}
+
+int sync_scalars(swh_domain *h, DevScalars *out)
+{
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(out, h->sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+    return SWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swh_last_error(char *buf, int len)
+{
+    if (!buf || len <= 0) return SWH_EINVAL;
+    snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return SWH_OK;
+}
+
+int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
+{
+    if (!d || !out) return fail(SWH_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->nx < 3 || d->ny < 3) return fail(SWH_EINVAL, "nx and ny must be >= 3");
+    if (d->timestepping_method != SWH_METHOD_LINEAR && d->timestepping_method != SWH_METHOD_LLPNF &&
+        d->timestepping_method != SWH_METHOD_RK2)
+        return fail(SWH_EINVAL, "unsupported timestepping_method id");
+    if (d->precision != SWH_PREC_FP64 && d->precision != SWH_PREC_FP32K) return fail(SWH_EINVAL, "bad precision");
+    if (!d->distance_bottom_edge || !d->distance_left_edge || !d->area_cell_y)
+        return fail(SWH_EINVAL, "metric arrays are required");
+    if (d->include_coriolis && (!d->coriolis || !d->coriolis_bottom_edge)) return fail(SWH_EINVAL, "coriolis arrays missing");
+    if (d->spherical && !d->tanlat_on_radius_earth) return fail(SWH_EINVAL, "tanlat_on_radius_earth missing");
+    const int n_ext = d->exterior_cells_width > 0 ? d->exterior_cells_width : 2;
+    if (2 * n_ext >= d->nx || 2 * n_ext >= d->ny) return fail(SWH_EINVAL, "domain smaller than its exterior cells");
+
+    swh_domain *h = new swh_domain();
+    h->nx = d->nx; h->ny = d->ny;
+    h->pitch = (d->nx + 63) / 64 * 64;
+    h->fpitch = (d->nx + 1 + 63) / 64 * 64;
+    h->method = d->timestepping_method; h->prec = d->precision;
+    h->truely_linear = d->linear_solver_is_truely_linear != 0;
+    h->coriolis = d->include_coriolis != 0; h->spherical = d->spherical != 0;
+    h->friction_type = d->friction_type; h->n_ext = n_ext;
+    for (int k = 0; k < 4; k++) h->bex[k] = d->boundary_exterior[k] != 0;
+    h->maxq_mask = d->max_U_mask & 63;
+    h->maxq_freq = d->max_U_update_frequency > 0 ? d->max_U_update_frequency : 1;
+    h->msl = d->msl_linear;
+    const bool fv = h->method == SWH_METHOD_RK2;
+    h->cfl = d->cfl > 0 ? d->cfl : (fv ? 0.99 : 0.7);        // timestepping_metadata_mod.f90:73-121
+    h->theta = d->theta > 0 ? d->theta : (fv ? 1.6 : 0.0);
+    h->lfc = d->linear_friction_coeff;
+    h->max_timestep = d->maximum_timestep > 0 ? d->maximum_timestep : kMaxTimestep;
+    h->arrival_thresh = d->arrival_stage_threshold_above_msl_linear;
+    h->esz = h->prec == SWH_PREC_FP64 ? 8 : 4;
+    const int nx = h->nx, ny = h->ny;
+    h->dbe.assign(d->distance_bottom_edge, d->distance_bottom_edge + ny + 1);
+    h->dle.assign(d->distance_left_edge, d->distance_left_edge + nx + 1);
+    h->area.assign(d->area_cell_y, d->area_cell_y + ny);
+    h->cor.assign(ny, 0.0);
+    h->corbe.assign(ny + 1, 0.0);
+    h->tanlat.assign(ny, 0.0);
+    if (h->coriolis) {
+        h->cor.assign(d->coriolis, d->coriolis + ny);
+        h->corbe.assign(d->coriolis_bottom_edge, d->coriolis_bottom_edge + ny + 1);
+    }
+    if (h->spherical) h->tanlat.assign(d->tanlat_on_radius_earth, d->tanlat_on_radius_earth + ny);
+    h->h_elev.assign((size_t)nx * ny, 0.0);
+    h->h_nsq.assign((size_t)nx * ny, 0.0);
+
+    int rc = SWH_OK;
+    auto A = [&](void **p, size_t bytes) {
+        if (rc == SWH_OK) rc = dmalloc(p, bytes);
+    };
+    if (hipStreamCreate(&h->stream) != hipSuccess) {
+        delete h;
+        return fail(SWH_EHIP, "hipStreamCreate failed");
+    }
+    const size_t P = h->plane_elems() * h->esz;
+    A((void **)&h->d_dbe, sizeof(double) * (ny + 1));
+    A((void **)&h->d_dle, sizeof(double) * (nx + 1));
+    A((void **)&h->d_area, sizeof(double) * ny);
+    A((void **)&h->d_cor, sizeof(double) * ny);
+    A((void **)&h->d_tanlat, sizeof(double) * ny);
+    for (int b = 0; b < 2; b++) {
+        A(&h->S[b], P); A(&h->UH[b], P); A(&h->VH[b], P);
+        if (h->prec == SWH_PREC_FP32K && !fv) A(&h->C[b], P);
+    }
+    A(&h->E, P);
+    A(&h->nsq, P);
+    if (h->method == SWH_METHOD_LLPNF) { A(&h->fwU, P); A(&h->fwV, P); }
+    if (h->maxq_mask) A(&h->mx, P * N_MAXQ);
+    if (fv) {
+        const size_t FPn = (size_t)h->fpitch * (ny + 1);
+        A(&h->depth, P); A(&h->u, P); A(&h->v, P);
+        A(&h->fNS, FPn * 3 * h->esz);
+        A(&h->fEW, FPn * 3 * h->esz);
+        A((void **)&h->esU, h->plane_elems() * sizeof(double));
+        A((void **)&h->esV, h->plane_elems() * sizeof(double));
+        A((void **)&h->esJ, FPn * sizeof(double));
+        A((void **)&h->esE, FPn * sizeof(double));
+    }
+    A(&h->rc, h->esz * ((size_t)5 * ny + (nx + 1) + (ny + 1)));
+    A((void **)&h->sc, sizeof(DevScalars));
+    A((void **)&h->d_work, sizeof(double) * (2 * (size_t)ny + 16));
+    if (rc) {
+        swh_domain_destroy(h);
+        return rc;
+    }
+    if (hipMemcpy(h->d_dbe, h->dbe.data(), sizeof(double) * (ny + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_dle, h->dle.data(), sizeof(double) * (nx + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_area, h->area.data(), sizeof(double) * ny, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_cor, h->cor.data(), sizeof(double) * ny, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_tanlat, h->tanlat.data(), sizeof(double) * ny, hipMemcpyHostToDevice) != hipSuccess) {
+        swh_domain_destroy(h);
+        return fail(SWH_EHIP, "metric upload failed");
+    }
+    if (h->maxq_mask) {
+        // max_U initial values (domain_mod.f90:1119-1136): -0.99*huge(float32), and
+        // +0.99*huge(float32) for min_stage
+        const double big = (double)(0.99f * 3.40282347e+38f);
+        std::vector<double> init((size_t)nx * ny, -big);
+        for (int k = 0; k < N_MAXQ; k++) {
+            if (k == MX_MIN_STAGE) std::fill(init.begin(), init.end(), big);
+            void *plane = (char *)h->mx + (size_t)k * P;
+            rc = upload_plane_any(h, plane, init.data());
+            if (rc) {
+                swh_domain_destroy(h);
+                return rc;
+            }
+        }
+    }
+    *out = h;
+    return SWH_OK;
+}
+
+int swh_domain_destroy(swh_domain *h)
+{
+    if (!h) return SWH_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void *ptrs[] = {h->d_dbe, h->d_dle, h->d_area, h->d_cor, h->d_tanlat, h->S[0], h->S[1], h->UH[0], h->UH[1],
+                    h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
+                    h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
+                    h->d_gval, h->d_gtime};
+    for (void *p : ptrs) dfree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SWH_OK;
+}
+
+int swh_upload_U(swh_domain *h, const double *U)
+{
+    if (!h || !U) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->nx * h->ny;
+    const int c = h->cur;
+    int rc;
+    if ((rc = upload_plane_any(h, h->S[c], U))) return rc;
+    if ((rc = upload_plane_any(h, h->UH[c], U + n))) return rc;
+    if ((rc = upload_plane_any(h, h->VH[c], U + 2 * n))) return rc;
+    if ((rc = upload_plane_any(h, h->E, U + 3 * n))) return rc;
+    if (h->C[c]) SWH_HIP(hipMemset(h->C[c], 0, h->plane_elems() * h->esz));
+    h->h_elev.assign(U + 3 * n, U + 4 * n);
+    return build_friction_work(h);
+}
+
+int swh_download_U(swh_domain *h, double *U)
+{
+    if (!h || !U) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->nx * h->ny;
+    const int c = h->cur;
+    int rc;
+    if (h->prec == SWH_PREC_FP64) {
+        if ((rc = download_plane<double>(h, h->S[c], nullptr, U))) return rc;
+        if ((rc = download_plane<double>(h, h->UH[c], nullptr, U + n))) return rc;
+        if ((rc = download_plane<double>(h, h->VH[c], nullptr, U + 2 * n))) return rc;
+        if ((rc = download_plane<double>(h, h->E, nullptr, U + 3 * n))) return rc;
+    } else {
+        if ((rc = download_plane<float>(h, h->S[c], h->C[c], U))) return rc;
+        if ((rc = download_plane<float>(h, h->UH[c], nullptr, U + n))) return rc;
+        if ((rc = download_plane<float>(h, h->VH[c], nullptr, U + 2 * n))) return rc;
+        if ((rc = download_plane<float>(h, h->E, nullptr, U + 3 * n))) return rc;
+    }
+    return SWH_OK;
+}
+
+int swh_upload_manning_squared(swh_domain *h, const double *m)
+{
+    if (!h || !m) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    int rc = upload_plane_any(h, h->nsq, m);
+    if (rc) return rc;
+    h->h_nsq.assign(m, m + (size_t)h->nx * h->ny);
+    return build_friction_work(h);
+}
+
+int swh_set_time(swh_domain *h, double t)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    DevScalars s;
+    int rc = sync_scalars(h, &s);
+    if (rc) return rc;
+    s.time = t;
+    SWH_HIP(hipMemcpy(h->sc, &s, sizeof(DevScalars), hipMemcpyHostToDevice));
+    h->time = t;
+    return SWH_OK;
+}
+
+int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double msl, const int32_t sides[4])
+{
+    if (!h || !sides) return fail(SWH_EINVAL, "null argument");
+    if (!(period != 0.0)) return fail(SWH_EINVAL, "period must be non-zero");
+    h->bc_mask = (sides[0] ? 1 : 0) | (sides[1] ? 2 : 0) | (sides[2] ? 4 : 0) | (sides[3] ? 8 : 0);
+    for (int k = 0; k < 4; k++) h->bex[k] = sides[k] != 0;
+    h->bc_amp = amp; h->bc_period = period; h->bc_msl = msl;
+    return SWH_OK;
+}
+
+int swh_evolve_one_step(swh_domain *h, double timestep)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    return evolve(h, timestep);
+}
+
+int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (nsteps < 0) return fail(SWH_EINVAL, "nsteps must be >= 0");
+    for (int k = 0; k < nsteps; k++) {
+        int rc = evolve(h, timestep);
+        if (rc) return rc;
+    }
+    return SWH_OK;
+}
+
+int swh_get_scalars(swh_domain *h, swh_scalars *out)
+{
+    if (!h || !out) return fail(SWH_EINVAL, "null argument");
+    DevScalars s;
+    int rc = sync_scalars(h, &s);
+    if (rc) return rc;
+    if (h->leapfrog()) {
+        out->time = h->time;
+        out->max_dt = 0.0;
+        out->evolve_step_dt = h->evolve_step_dt;
+        out->dt_last_update = h->dt_last_update;
+    } else {
+        out->time = s.time;
+        out->max_dt = s.max_dt;
+        out->evolve_step_dt = h->nsteps ? s.dt : 0.0;
+        out->dt_last_update = h->nsteps ? s.dt : 0.0;
+        h->time = s.time;
+    }
+    for (int k = 0; k < 4; k++) out->boundary_flux_store[k] = s.bf_store[k];
+    out->boundary_flux_evolve_integral = s.bf_evolve;
+    out->boundary_flux_time_integral = s.bf_time;
+    out->negative_depth_fix_counter = (double)s.neg_depth_count;
+    out->nsteps_advanced = h->nsteps;
+    return SWH_OK;
+}
+
+int swh_volume_interior(swh_domain *h, double *out)
+{
+    if (!h || !out) return fail(SWH_EINVAL, "null argument");
+    const int c = h->cur;
+    double *res = h->d_work + 2 * (size_t)h->ny + 8;
+    if (h->prec == SWH_PREC_FP64)
+        SWH_HIP(launch_volume<double>((const double *)h->S[c], nullptr, (const double *)h->E, h->nx, h->ny, h->pitch,
+                                      h->n_ext, h->d_area, h->d_work, res, h->stream));
+    else
+        SWH_HIP(launch_volume<float>((const float *)h->S[c], (const float *)h->C[c], (const float *)h->E, h->nx,
+                                     h->ny, h->pitch, h->n_ext, h->d_area, h->d_work, res, h->stream));
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(out, res, sizeof(double), hipMemcpyDeviceToHost));
+    return SWH_OK;
+}
+
+int swh_stationary_timestep_max(swh_domain *h, double *out)
+{
+    if (!h || !out) return fail(SWH_EINVAL, "null argument");
+    const int c = h->cur;
+    double *res = h->d_work + 2 * (size_t)h->ny + 9;
+    const int linear = h->leapfrog() && h->truely_linear;
+    if (h->prec == SWH_PREC_FP64)
+        SWH_HIP(launch_timestep_max<double>((const double *)h->S[c], (const double *)h->E, h->nx, h->ny, h->pitch,
+                                            linear, h->msl, h->d_dbe, h->d_dle, res, h->stream));
+    else
+        SWH_HIP(launch_timestep_max<float>((const float *)h->S[c], (const float *)h->E, h->nx, h->ny, h->pitch,
+                                           linear, h->msl, h->d_dbe, h->d_dle, res, h->stream));
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    double v;
+    SWH_HIP(hipMemcpy(&v, res, sizeof(double), hipMemcpyDeviceToHost));
+    v = v * h->cfl;
+    if (!h->leapfrog()) v = v * 0.5;  // stationary_timestep_max, domain_mod.f90:2003-2008
+    *out = v;
+    return SWH_OK;
+}
+
+int swh_download_max_U(swh_domain *h, int32_t which, double *out)
+{
+    if (!h || !out) return fail(SWH_EINVAL, "null argument");
+    if (!h->maxq_mask) return fail(SWH_ESTATE, "max_U is not tracked (max_U_mask == 0)");
+    int k = -1;
+    for (int b = 0; b < N_MAXQ; b++)
+        if (which == (1 << b)) k = b;
+    if (k < 0 || !(h->maxq_mask & which)) return fail(SWH_EINVAL, "max_U plane not tracked");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    const void *plane = (const char *)h->mx + (size_t)k * h->plane_elems() * h->esz;
+    return h->prec == SWH_PREC_FP64 ? download_plane<double>(h, plane, nullptr, out)
+                                    : download_plane<float>(h, plane, nullptr, out);
+}
+
+int swh_gauges_set(swh_domain *h, int32_t n, const int32_t *ij, int32_t max_samples)
+{
+    if (!h || (n > 0 && !ij) || n < 0 || max_samples < 1) return fail(SWH_EINVAL, "bad gauge arguments");
+    std::vector<long long> idx(n);
+    for (int g = 0; g < n; g++) {
+        const int i = ij[2 * g] - 1, j = ij[2 * g + 1] - 1;
+        if (i < 0 || i >= h->nx || j < 0 || j >= h->ny) return fail(SWH_EINVAL, "gauge cell outside the domain");
+        idx[g] = (long long)j * h->pitch + i;
+    }
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    dfree(h->d_gidx); dfree(h->d_gval); dfree(h->d_gtime);
+    h->d_gidx = nullptr; h->d_gval = nullptr; h->d_gtime = nullptr;
+    int rc = dmalloc((void **)&h->d_gidx, sizeof(long long) * (n > 0 ? n : 1));
+    if (!rc) rc = dmalloc((void **)&h->d_gval, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1) * max_samples);
+    if (!rc) rc = dmalloc((void **)&h->d_gtime, sizeof(double) * max_samples);
+    if (rc) return rc;
+    if (n > 0) SWH_HIP(hipMemcpy(h->d_gidx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    h->ng = n; h->gmax = max_samples; h->gsamples = 0;
+    return SWH_OK;
+}
+
+int swh_gauges_sample(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    if (h->gsamples >= h->gmax) return fail(SWH_ESTATE, "gauge sample buffer full");
+    const int c = h->cur;
+    const DevScalars *tsrc = h->leapfrog() ? nullptr : h->sc;
+    if (h->prec == SWH_PREC_FP64)
+        SWH_HIP(launch_gauges<double>((const double *)h->S[c], nullptr, (const double *)h->UH[c],
+                                      (const double *)h->VH[c], h->ng, h->d_gidx, h->d_gval, h->d_gtime, h->gsamples,
+                                      tsrc, h->time, h->stream));
+    else
+        SWH_HIP(launch_gauges<float>((const float *)h->S[c], (const float *)h->C[c], (const float *)h->UH[c],
+                                     (const float *)h->VH[c], h->ng, h->d_gidx, h->d_gval, h->d_gtime, h->gsamples,
+                                     tsrc, h->time, h->stream));
+    h->gsamples++;
+    return SWH_OK;
+}
+
+int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *nsamples)
+{
+    if (!h || !values || !times || !nsamples) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    if (h->gsamples > 0) {
+        SWH_HIP(hipMemcpy(values, h->d_gval, sizeof(double) * 3 * (size_t)h->ng * h->gsamples,
+                          hipMemcpyDeviceToHost));
+        SWH_HIP(hipMemcpy(times, h->d_gtime, sizeof(double) * h->gsamples, hipMemcpyDeviceToHost));
+    }
+    *nsamples = h->gsamples;
+    return SWH_OK;
+}
+
+int swh_synchronize(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    return SWH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// Finite-volume 'rk2' step (DE1_low_fr_diffusion fluxes + update_U_manning) for gfx950.
+//
+// References (paths under propagation/SWALS/src/shallow_water/):
+//   compute_depth_and_velocity        domain_mod.f90:1360-1419
+//   DE1 fluxes, sources, CFL          domain_mod_compute_fluxes_DE1_inner_include.f90:78-727
+//   MC limiter                        extrapolation_limiting_mod.f90:44-79
+//   update_U (Manning / Chezy)        domain_mod_update_U_DE1_inner_include.f90:14-99
+//   one_euler_step / one_rk2_step     domain_mod_timestepping_alternatives_include.f90:8-146
+//
+// An Euler stage is four launches: clip + depth/velocity (cell), edge fluxes and
+// explicit sources (one thread per cell, owning its south and west edges, with the
+// limited gradients recomputed from the cell's column/row neighbours), a one-block
+// scalar pass (boundary-flux sums, CFL step choice, time), and the cell update.
+// The CFL step is a running max of 1/dt kept on the device, so an adaptive rk2 step
+// never returns to the host. Arithmetic follows the reference's evaluation order.
+#include "swh_internal.h"
+
+namespace swh {
+
+constexpr double kLimC1 = 0.1, kLimC2 = 0.3, kLimC3 = 100.0;  // domain_mod.f90:111-117
+
+__device__ __forceinline__ double fmax_(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double fmin_(double a, double b) { return a < b ? a : b; }
+
+// ---- pass 1: clip + depth/velocity (domain_mod.f90:1374-1412) ----
+template <typename T>
+__global__ void fv_depth_kernel(FVArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        // reset the running CFL maximum: 1/maximum_timestep (include.f90:92-95)
+        a.sc->max_dt_inv_bits = (unsigned long long)__double_as_longlong(1.0 / kMaxTimestep);
+    }
+    unsigned long long fixes = 0;
+    if (i < a.nx && j < a.ny) {
+        const long long o = (long long)j * a.pitch + i;
+        T s = a.S0[o], uh = a.UH0[o], vh = a.VH0[o];
+        const T e = a.E[o];
+        T d = s - e, u, v;
+        if (d > T(kMinDepth)) {
+            const T inv = T(1) / d;
+            u = uh * inv;
+            v = vh * inv;
+        } else {
+            if (d < T(0)) {
+                d = T(0);
+                s = e;
+                fixes = 1;
+            }
+            u = T(0); v = T(0); uh = T(0); vh = T(0);
+        }
+        a.depth[o] = d; a.u[o] = u; a.v[o] = v;
+        a.S[o] = s; a.UH[o] = uh; a.VH[o] = vh;
+    }
+    // wave-level count, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) fixes += __shfl_xor(fixes, off);
+    if ((threadIdx.x & 63) == 0 && fixes) atomicAdd(&a.sc->neg_depth_count, fixes);
+}
+
+// Wet-dry limiter coefficient (include.f90:751-757)
+__device__ __forceinline__ double theta_wd(double theta, double d0, double d1, double d2)
+{
+    const double mindep = fmin_(fmin_(d0, d1), d2) - kMinDepth;
+    const double maxdep = fmax_(fmax_(d0, d1), d2) + kLimC3 * kMinDepth;
+    const double tl = (1.0 / (kLimC2 - kLimC1)) * (mindep / maxdep - kLimC1);
+    return fmax_(theta * fmin_(1.0, tl), 0.0);
+}
+
+// MC limited gradient (extrapolation_limiting_mod.f90:57-74)
+__device__ __forceinline__ double mc_grad(double lo, double mid, double hi, double th)
+{
+    const double a = hi - mid, b = mid - lo;
+    const double sa = copysign(1.0, a), sb = copysign(1.0, b);
+    const double half_sasb = 0.5 * (sa + sb);
+    const double d = fmin_(fabs(a), fabs(b)) * half_sasb * th;
+    const double e = 0.5 * (a + b);
+    const double c = (d == 0.0) ? 0.0 : e;
+    return (d > 0.0) ? fmin_(c, d) : fmax_(c, d);
+}
+
+struct Grad4 {
+    double st, dp, u, v;
+};
+
+// Gradients of (stage, depth, u, v) at a cell from its two neighbours along one
+// axis; zero when the cell is on that axis' border.
+template <typename T>
+__device__ __forceinline__ Grad4 grad_at(const FVArgs<T> &a, long long o, long long step, bool border, double theta)
+{
+    Grad4 g{0.0, 0.0, 0.0, 0.0};
+    if (border) return g;
+    const double dl = a.depth[o - step], dc = a.depth[o], dh = a.depth[o + step];
+    const double th = theta_wd(theta, dl, dc, dh);
+    g.st = mc_grad(a.S[o - step], a.S[o], a.S[o + step], th);
+    g.dp = mc_grad(dl, dc, dh, th);
+    g.u = mc_grad(a.u[o - step], a.u[o], a.u[o + step], th);
+    g.v = mc_grad(a.v[o - step], a.v[o], a.v[o + step], th);
+    return g;
+}
+
+struct EdgeRes {
+    double f0, f1, f2;       // mass, momentum (reference UH, VH order)
+    double hghh;             // half_g_hh_edge
+    double bsp_neg, bsp_pos; // bed-slope pressure terms of the two sides
+    double max_speed;
+};
+
+// HLL/Audusse edge flux (include.f90:178-432 / 454-718). 'n' = normal velocity
+// component, 't' = transverse; ns_edge selects the reference's UH/VH ordering and
+// its fr2 summation order (v^2 terms first in both loops).
+__device__ __forceinline__ EdgeRes edge_flux(double stage_neg, double stage_pos, double depth_neg, double depth_pos,
+                                             double u_neg, double u_pos, double v_neg, double v_pos,
+                                             double depth_neg_c, double depth_pos_c, double elev_neg,
+                                             double elev_pos, double edge_len, bool ns_edge)
+{
+    const double EPS = 1.0e-12;
+    const double EPS_gs = sqrt(kGravity * kMinDepth);
+    const double half_gravity = 0.5 * kGravity;
+    EdgeRes r;
+    const double z_neg = stage_neg - depth_neg, z_pos = stage_pos - depth_pos;
+    const double z_half = fmax_(z_neg, z_pos);
+    const double sns = fmax_(stage_neg, z_half), dns = sns - z_half;
+    if (dns == 0.0) { u_neg = 0.0; v_neg = 0.0; }
+    const double ud_neg = u_neg * dns, vd_neg = v_neg * dns;
+    const double gs_neg = sqrt(kGravity * dns);
+    const double sps = fmax_(stage_pos, z_half), dps = sps - z_half;
+    if (dps == 0.0) { u_pos = 0.0; v_pos = 0.0; }
+    const double ud_pos = u_pos * dps, vd_pos = v_pos * dps;
+    const double gs_pos = sqrt(kGravity * dps);
+    const double vbn = (ns_edge ? v_neg : u_neg) * 1.0;
+    const double vbp = (ns_edge ? v_pos : u_pos) * 1.0;
+    const double s_max = fmax_(fmax_(vbn + gs_neg, vbp + gs_pos), 0.0);
+    const double s_min = fmin_(fmin_(vbn - gs_neg, vbp - gs_pos), 0.0);
+    const double denom = s_max - s_min;
+    const double inv_denom = (denom > EPS && (gs_pos > EPS_gs || gs_neg > EPS_gs)) ? edge_len / denom : 0.0;
+    const double sminsmax = s_min * s_max * 1.0;
+    const double fr2 = 1.0 * sqrt(fmax_(0.001, fmin_(1.0, (v_neg * v_neg + v_pos * v_pos + u_neg * u_neg + u_pos * u_pos) /
+                                                            (gs_neg * gs_neg + gs_pos * gs_pos + 1.0e-10))));
+    const double nd_neg = ns_edge ? vd_neg : ud_neg, nd_pos = ns_edge ? vd_pos : ud_pos;
+    r.f0 = (s_max * nd_neg - s_min * nd_pos + sminsmax * (sps - sns)) * inv_denom;
+    r.f1 = (s_max * ud_neg * vbn - s_min * ud_pos * vbp + fr2 * sminsmax * (ud_pos - ud_neg)) * inv_denom;
+    r.f2 = (s_max * vd_neg * vbn - s_min * vd_pos * vbp + fr2 * sminsmax * (vd_pos - vd_neg)) * inv_denom;
+    r.hghh = inv_denom * half_gravity * (s_max * dns * dns - s_min * dps * dps);
+    r.bsp_neg = half_gravity * ((dns - depth_neg) * (dns + depth_neg) - (depth_neg + depth_neg_c) * (z_neg - elev_neg));
+    r.bsp_pos = half_gravity * ((dps - depth_pos) * (dps + depth_pos) - (depth_pos + depth_pos_c) * (z_pos - elev_pos));
+    r.max_speed = fmax_(s_max, -s_min);
+    return r;
+}
+
+// ---- pass 2: fluxes + explicit sources + CFL ----
+template <typename T>
+__global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    double mx = 0.0;
+    if (i >= 1 && i < a.nx && j >= 1 && j < a.ny) {
+        const int nx = a.nx, ny = a.ny;
+        const long long P = a.pitch;
+        const long long o = (long long)j * P + i;
+        const long long FP = (long long)a.fpitch * (ny + 1);
+        const long long fo = (long long)j * a.fpitch + i;
+        const double half_gravity = 0.5 * kGravity;
+        const double EPS = 1.0e-12;
+
+        // NS edge between (i, j-1) and (i, j); gradients are zero on rows 0 and ny-1
+        {
+            const long long om = o - P;
+            const Grad4 gl = grad_at(a, om, P, (j - 1) == 0 || (j - 1) == ny - 1, a.theta);
+            const Grad4 gc = grad_at(a, o, P, j == 0 || j == ny - 1, a.theta);
+            const double stage_neg = (double)a.S[om] + 0.5 * gl.st;
+            const double stage_pos = (double)a.S[o] - 0.5 * gc.st;
+            const double depth_neg = (double)a.depth[om] + 0.5 * gl.dp;
+            const double depth_pos = (double)a.depth[o] - 0.5 * gc.dp;
+            const double u_neg = (double)a.u[om] + 0.5 * gl.u, u_pos = (double)a.u[o] - 0.5 * gc.u;
+            const double v_neg = (double)a.v[om] + 0.5 * gl.v, v_pos = (double)a.v[o] - 0.5 * gc.v;
+            const EdgeRes r = edge_flux(stage_neg, stage_pos, depth_neg, depth_pos, u_neg, u_pos, v_neg, v_pos,
+                                        (double)a.depth[om], (double)a.depth[o], (double)a.E[om], (double)a.E[o],
+                                        a.dbe[j], true);
+            a.fNS[fo] = (T)r.f0;
+            a.fNS[FP + fo] = (T)r.f1;
+            a.fNS[2 * FP + fo] = (T)r.f2;
+            a.esJ[fo] = r.bsp_neg * a.dbe[j] - r.hghh;
+            double esv = 0.0 - r.bsp_pos * a.dbe[j] + r.hghh;
+            if (r.max_speed > EPS) mx = fmax_(mx, r.max_speed * (1.0 / (a.dle[0] * a.half_cfl)));
+
+            // EW edge between (i-1, j) and (i, j); gradients are zero on columns 0 and nx-1
+            const long long ow = o - 1;
+            const Grad4 el = grad_at(a, ow, 1, (i - 1) == 0 || (i - 1) == nx - 1, a.theta);
+            const Grad4 ec = grad_at(a, o, 1, i == 0 || i == nx - 1, a.theta);
+            const double st_neg = (double)a.S[ow] + 0.5 * el.st;
+            const double st_pos = (double)a.S[o] - 0.5 * ec.st;
+            const double dp_neg = (double)a.depth[ow] + 0.5 * el.dp;
+            const double dp_pos = (double)a.depth[o] - 0.5 * ec.dp;
+            const double uu_neg = (double)a.u[ow] + 0.5 * el.u, uu_pos = (double)a.u[o] - 0.5 * ec.u;
+            const double vv_neg = (double)a.v[ow] + 0.5 * el.v, vv_pos = (double)a.v[o] - 0.5 * ec.v;
+            const EdgeRes q = edge_flux(st_neg, st_pos, dp_neg, dp_pos, uu_neg, uu_pos, vv_neg, vv_pos,
+                                        (double)a.depth[ow], (double)a.depth[o], (double)a.E[ow], (double)a.E[o],
+                                        a.dle[i], false);
+            const long long EPn = (long long)a.fpitch * ny;
+            a.fEW[fo] = (T)q.f0;
+            a.fEW[EPn + fo] = (T)q.f1;
+            a.fEW[2 * EPn + fo] = (T)q.f2;
+            a.esE[fo] = q.bsp_neg * a.dle[i] - q.hghh;
+            double esu = 0.0 - q.bsp_pos * a.dle[i] + q.hghh;
+            if (a.spherical) {
+                // include.f90:674-710
+                const double dc = (double)a.depth[o];
+                esv = esv + half_gravity * dc * dc * (a.dbe[j + 1] - a.dbe[j]);
+                const double uhc = (double)a.UH[o], vhc = (double)a.VH[o];
+                double cm;
+                if (a.coriolis) {
+                    cm = a.area[j] * a.cor[j];
+                    esu = esu + cm * vhc;
+                    esv = esv - cm * uhc;
+                }
+                cm = a.area[j] * a.tanlat[j] * (double)a.u[o];
+                esu = esu + cm * vhc;
+                esv = esv - cm * uhc;
+            }
+            a.esU[o] = esu;
+            a.esV[o] = esv;
+            if (q.max_speed > EPS) mx = fmax_(mx, q.max_speed * (1.0 / (a.dbe[j] * a.half_cfl)));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax_(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0 && mx > 0.0)
+        atomicMax(&a.sc->max_dt_inv_bits, (unsigned long long)__double_as_longlong(mx));
+}
+
+// ---- pass 3: one block: boundary-flux sums (domain_mod.f90:1483-1499), step choice,
+// time and flux-integral bookkeeping (timestepping_alternatives_include.f90:31-50) ----
+template <typename T>
+__global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny, int fpitch,
+                                                        int n_ext, double fixed_dt, int first_stage,
+                                                        DevScalars *sc)
+{
+    __shared__ double part[4][64];
+    const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
+    double acc = 0.0;
+    // plain per-lane partial sums combined in a fixed order (deterministic)
+    if (g == 0 || g == 2) {
+        const int jr = (g == 0) ? ny - n_ext : n_ext;
+        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += 64) acc += (double)fNS[(long long)jr * fpitch + i];
+    } else {
+        const int ic = (g == 1) ? nx - n_ext : n_ext;
+        for (int jj = n_ext + lane; jj <= ny - n_ext - 1; jj += 64) acc += (double)fEW[(long long)jj * fpitch + ic];
+    }
+    part[g][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double st[4];
+        for (int s = 0; s < 4; s++) {
+            double t = 0.0;
+            for (int l = 0; l < 64; l++) t += part[s][l];
+            st[s] = t;
+        }
+        sc->bf_store[0] = st[0];
+        sc->bf_store[1] = st[1];
+        sc->bf_store[2] = -st[2];
+        sc->bf_store[3] = -st[3];
+        const double max_dt = 1.0 / __longlong_as_double((long long)sc->max_dt_inv_bits);
+        sc->max_dt = max_dt;
+        double ts;
+        if (first_stage) {
+            ts = fixed_dt > 0.0 ? fixed_dt : max_dt;
+            sc->dt = ts;
+            sc->pad[0] = max_dt;  // max_dt of the first Euler stage (reported by rk2)
+        } else {
+            ts = sc->dt;
+        }
+        sc->bf_evolve = sc->bf_evolve + ts * (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]);
+        sc->time = sc->time + ts;
+    }
+}
+
+// ---- pass 4: update_U (update_U_DE1_inner_include.f90:38-97) ----
+template <typename T>
+__global__ void fv_update_kernel(FVArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const double dt = a.sc->dt;
+    const long long o = (long long)j * a.pitch + i;
+    const long long fo = (long long)j * a.fpitch + i;
+    const long long FP = (long long)a.fpitch * (a.ny + 1);
+    const long long EPn = (long long)a.fpitch * a.ny;
+    const double inv_cell_area_dt = dt / a.area[j];
+    T *q[3] = {a.S, a.UH, a.VH};
+    double un[3];
+    for (int kk = 0; kk < 3; kk++) {
+        const T *fn = a.fNS + kk * FP;
+        const T *fe = a.fEW + kk * EPn;
+        un[kk] = (double)q[kk][o] - inv_cell_area_dt * (((double)fn[fo + a.fpitch] - (double)fn[fo]) +
+                                                        ((double)fe[fo + 1] - (double)fe[fo]));
+    }
+    const double depth = (double)a.depth[o];
+    if (un[0] <= ((double)a.E[o] + kMinDepth)) {
+        un[1] = 0.0;
+        un[2] = 0.0;
+    } else {
+        const double uu = (double)a.u[o], vv = (double)a.v[o];
+        const double fs = (double)a.nsq[o] * sqrt(uu * uu + vv * vv) * pow(fmax_(depth, kMinDepth), a.fric_power);
+        const double implicit_factor = 1.0 / (1.0 + (dt * kGravity) * fmax_(depth, 0.0) * fs + dt * a.lfc);
+        un[1] = un[1] + inv_cell_area_dt * (a.esU[o] + a.esE[fo + 1]);
+        un[1] = un[1] * implicit_factor;
+        un[2] = un[2] + inv_cell_area_dt * (a.esV[o] + a.esJ[fo + a.fpitch]);
+        un[2] = un[2] * implicit_factor;
+    }
+    a.S[o] = (T)un[0];
+    a.UH[o] = (T)un[1];
+    a.VH[o] = (T)un[2];
+}
+
+// ---- rk2 average (timestepping_alternatives_include.f90:123-134) ----
+template <typename T>
+__global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny,
+                                  int pitch)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long long o = (long long)j * pitch + i;
+    S[o] = T(0.5) * (S[o] + S0[o]);
+    UH[o] = T(0.5) * (UH[o] + UH0[o]);
+    VH[o] = T(0.5) * (VH[o] + VH0[o]);
+}
+
+// ---- stage boundary on a co-located grid (boundary_mod.f90:78-164) ----
+// One block: west/east columns first, then south/north rows, as in the reference
+// (so corner cells end with the south/north treatment).
+template <typename T>
+__global__ __launch_bounds__(256) void fv_stage_bc_kernel(T *S, T *UH, T *VH, int nx, int ny, int pitch, int m,
+                                                          double amp, double period, double msl, const DevScalars *sc)
+{
+    const double two_pi = 2.0 * acos(-1.0);
+    const T stage = (T)(msl + amp * sin(two_pi * sc->time / period));
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        const long long o = (long long)j * pitch;
+        if (m & 8) { S[o] = stage; UH[o] = UH[o + 1]; VH[o] = T(0); }
+        if (m & 2) { const long long e = o + nx - 1; S[e] = stage; UH[e] = UH[e - 1]; VH[e] = T(0); }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        if (m & 4) { S[i] = stage; VH[i] = VH[pitch + i]; UH[i] = T(0); }
+        if (m & 1) {
+            const long long o = (long long)(ny - 1) * pitch + i;
+            S[o] = stage; VH[o] = VH[o - pitch]; UH[o] = T(0);
+        }
+    }
+}
+
+__global__ void fv_begin_kernel(DevScalars *sc)
+{
+    // evolve_one_step resets the per-step flux integral (domain_mod.f90:1713-1715);
+    // one_rk2_step keeps the start time (timestepping_alternatives_include.f90:80)
+    sc->bf_evolve = 0.0;
+    sc->pad[1] = sc->time;
+}
+
+__global__ void fv_finish_kernel(DevScalars *sc)
+{
+    // rk2 bookkeeping (timestepping_alternatives_include.f90:136-142)
+    sc->time = sc->pad[1] + sc->dt;
+    sc->bf_evolve = 0.5 * sc->bf_evolve;
+    sc->max_dt = sc->pad[0];
+    sc->bf_time = sc->bf_time + sc->bf_evolve;
+}
+
+template <typename T>
+hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, int first_stage, hipStream_t s)
+{
+    dim3 block(256), grid((a.nx + 255) / 256, a.ny);
+    hipLaunchKernelGGL(fv_depth_kernel<T>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(fv_flux_kernel<T>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(256), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
+                       fixed_dt, first_stage, a.sc);
+    hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny, int pitch,
+                             hipStream_t s)
+{
+    dim3 block(256), grid((nx + 255) / 256, ny);
+    hipLaunchKernelGGL(fv_average_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
+                              double msl, const DevScalars *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(fv_stage_bc_kernel<T>, dim3(1), dim3(256), 0, s, S, UH, VH, nx, ny, pitch, mask, amp, period,
+                       msl, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(fv_begin_kernel, dim3(1), dim3(1), 0, s, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_fv_finish(DevScalars *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(fv_finish_kernel, dim3(1), dim3(1), 0, s, sc);
+    return hipGetLastError();
+}
+
+template hipError_t launch_fv_euler<double>(const FVArgs<double> &, double, int, hipStream_t);
+template hipError_t launch_fv_euler<float>(const FVArgs<float> &, double, int, hipStream_t);
+template hipError_t launch_fv_average<double>(double *, double *, double *, const double *, const double *,
+                                              const double *, int, int, int, hipStream_t);
+template hipError_t launch_fv_average<float>(float *, float *, float *, const float *, const float *, const float *,
+                                             int, int, int, hipStream_t);
+template hipError_t launch_fv_stage_bc<double>(double *, double *, double *, int, int, int, int, double, double,
+                                               double, const DevScalars *, hipStream_t);
+template hipError_t launch_fv_stage_bc<float>(float *, float *, float *, int, int, int, int, double, double, double,
+                                              const DevScalars *, hipStream_t);
+
+}  // namespace swh

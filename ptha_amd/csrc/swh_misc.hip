@@ -1,0 +1,308 @@
+// Small kernels around the step: boundary-flux sums, max_U update, volume,
+// gauges and the stationary CFL step. All reductions are FP64, deterministic
+// (fixed assignment of work to lanes and a fixed combine order) and carry a
+// two-sum compensation term, so results do not depend on scheduling.
+#include "swh_internal.h"
+
+namespace swh {
+
+// ---- compensated (double-double) accumulation ----
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, double b)
+{
+    const double s = a.hi + b;
+    const double bp = s - a.hi;
+    const double err = (a.hi - (s - bp)) + (b - bp);
+    return DD{s, a.lo + err};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    DD r = dd_add(a, b.hi);
+    r.lo += b.lo;
+    return r;
+}
+
+// Reduce one DD per thread over a block of blockDim.x threads (power of two <= 256).
+__device__ DD block_reduce_dd(DD v)
+{
+    __shared__ double sh_hi[256], sh_lo[256];
+    const int t = threadIdx.x;
+    sh_hi[t] = v.hi;
+    sh_lo[t] = v.lo;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            DD a{sh_hi[t], sh_lo[t]}, b{sh_hi[t + w], sh_lo[t + w]};
+            DD c = dd_add(a, b);
+            sh_hi[t] = c.hi;
+            sh_lo[t] = c.lo;
+        }
+        __syncthreads();
+    }
+    DD r{sh_hi[0], sh_lo[0]};
+    __syncthreads();
+    return r;
+}
+
+// ---- leapfrog boundary flux (domain_mod_linear_solver_include.f90:126-165) ----
+// One block of 256 threads: 4 groups of 64, group g sums side g (N, E, S, W) of the
+// pre-momentum UH/VH. Thread 0 then forms boundary_flux_evolve_integral and adds it
+// to boundary_flux_time_integral (domain_mod.f90:1803-1807).
+template <typename T>
+__global__ __launch_bounds__(256) void bflux_leapfrog_kernel(const T *__restrict__ UH, const T *__restrict__ VH,
+                                                             int nx, int ny, int pitch, int n_ext,
+                                                             const double *__restrict__ dle,
+                                                             const double *__restrict__ dbe, double dt,
+                                                             double time_after, DevScalars *sc)
+{
+    __shared__ double part[4][64][2];
+    const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
+    DD acc{0.0, 0.0};
+    if (g == 0 || g == 2) {
+        const int j = (g == 0) ? ny - n_ext - 1 : n_ext - 1;
+        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += 64) acc = dd_add(acc, (double)VH[(long long)j * pitch + i]);
+    } else {
+        const int i = (g == 1) ? nx - n_ext - 1 : n_ext - 1;
+        for (int j = n_ext + lane; j <= ny - n_ext - 1; j += 64) acc = dd_add(acc, (double)UH[(long long)j * pitch + i]);
+    }
+    part[g][lane][0] = acc.hi;
+    part[g][lane][1] = acc.lo;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double store[4];
+        for (int s = 0; s < 4; s++) {
+            DD t{0.0, 0.0};
+            for (int l = 0; l < 64; l++) t = dd_add(t, DD{part[s][l][0], part[s][l][1]});
+            store[s] = t.hi + t.lo;
+        }
+        sc->bf_store[0] = store[0] * dbe[ny - n_ext];
+        sc->bf_store[1] = store[1] * dle[nx - n_ext];
+        sc->bf_store[2] = -store[2] * dbe[n_ext];
+        sc->bf_store[3] = -store[3] * dle[n_ext];
+        const double ev = (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]) * dt;
+        sc->bf_evolve = ev;
+        sc->bf_time = sc->bf_time + ev;
+        sc->time = time_after;
+    }
+}
+
+template <typename T>
+hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext, const double *dle,
+                                 const double *dbe, double dt, double time_after, DevScalars *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(bflux_leapfrog_kernel<T>, dim3(1), dim3(256), 0, s, UH, VH, nx, ny, pitch, n_ext, dle, dbe, dt,
+                       time_after, sc);
+    return hipGetLastError();
+}
+template hipError_t launch_bflux_leapfrog<double>(const double *, const double *, int, int, int, int,
+                                                  const double *, const double *, double, double, DevScalars *,
+                                                  hipStream_t);
+template hipError_t launch_bflux_leapfrog<float>(const float *, const float *, int, int, int, int, const double *,
+                                                 const double *, double, double, DevScalars *, hipStream_t);
+
+// ---- standalone max_U update (domain_mod.f90:2243-2404) ----
+template <typename T>
+__global__ void maxq_kernel(const T *__restrict__ S, const T *__restrict__ C, const T *__restrict__ UH,
+                            const T *__restrict__ VH, const T *__restrict__ E, int nx, int ny, int pitch,
+                            int staggered, int truely_linear, T msl, int m, T *__restrict__ mx,
+                            const DevScalars *tsrc, T t_host, T arrival_stage)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long long o = (long long)j * pitch + i;
+    const long long P = (long long)pitch * ny;
+    const T MIN = T(kMinDepth), HALF = T(0.5);
+    const T t = tsrc ? (T)tsrc->time : t_host;
+    const T s = C ? S[o] - C[o] : S[o];
+    const T uh = UH[o], vh = VH[o], e = E[o];
+    if (m & SWH_MAXQ_MAX_STAGE) {
+        if (m & SWH_MAXQ_TIME_OF_MAX_STAGE) {
+            if (s > mx[MX_STAGE * P + o]) {
+                mx[MX_STAGE * P + o] = s;
+                mx[MX_TOMS * P + o] = t;
+            }
+        } else if (s > mx[MX_STAGE * P + o]) {
+            mx[MX_STAGE * P + o] = s;
+        }
+    }
+    if (m & SWH_MAXQ_MAX_SPEED) {
+        T sp;
+        if (staggered) {
+            const long long oE = (long long)j * pitch + min(i + 1, nx - 1);
+            const long long oN = (long long)min(j + 1, ny - 1) * pitch + i;
+            T dE, dN;
+            if (truely_linear) {
+                dE = HALF * ((msl - e) + (msl - E[oE]));
+                dN = HALF * ((msl - e) + (msl - E[oN]));
+            } else {
+                const T sE = C ? S[oE] - C[oE] : S[oE], sN = C ? S[oN] - C[oN] : S[oN];
+                dE = HALF * ((s - e) + (sE - E[oE]));
+                dN = HALF * ((s - e) + (sN - E[oN]));
+            }
+            const T iE = dE > MIN ? T(1) / dE : T(0);
+            const T iN = dN > MIN ? T(1) / dN : T(0);
+            sp = sqrt(uh * uh * (iE * iE) + vh * vh * (iN * iN));
+        } else {
+            const T ld = s - e;
+            const T li = ld > MIN ? T(1) / ld : T(0);
+            sp = sqrt((uh * uh + vh * vh) * (li * li));
+        }
+        if (sp > mx[MX_SPEED * P + o]) mx[MX_SPEED * P + o] = sp;
+    }
+    if (m & SWH_MAXQ_MAX_FLUX) {
+        const T f = sqrt(uh * uh + vh * vh);
+        if (f > mx[MX_FLUX * P + o]) mx[MX_FLUX * P + o] = f;
+    }
+    if (m & SWH_MAXQ_ARRIVAL_TIME) {
+        if ((s - e) > MIN && s > arrival_stage && mx[MX_ARRIVAL * P + o] < T(0)) mx[MX_ARRIVAL * P + o] = t;
+    }
+    if (m & SWH_MAXQ_MIN_STAGE) {
+        if (s < mx[MX_MIN_STAGE * P + o]) mx[MX_MIN_STAGE * P + o] = s;
+    }
+}
+
+template <typename T>
+hipError_t launch_maxq(const T *S, const T *C, const T *UH, const T *VH, const T *E, int nx, int ny, int pitch,
+                       int staggered, int truely_linear, T msl, int mask, T *mx, const DevScalars *tsrc, T t,
+                       T arrival_stage, hipStream_t s)
+{
+    dim3 block(256), grid((nx + 255) / 256, ny);
+    hipLaunchKernelGGL(maxq_kernel<T>, grid, block, 0, s, S, C, UH, VH, E, nx, ny, pitch, staggered, truely_linear,
+                       msl, mask, mx, tsrc, t, arrival_stage);
+    return hipGetLastError();
+}
+template hipError_t launch_maxq<double>(const double *, const double *, const double *, const double *,
+                                        const double *, int, int, int, int, int, double, int, double *,
+                                        const DevScalars *, double, double, hipStream_t);
+template hipError_t launch_maxq<float>(const float *, const float *, const float *, const float *, const float *,
+                                       int, int, int, int, int, float, int, float *, const DevScalars *, float, float,
+                                       hipStream_t);
+
+// ---- volume_interior (domain_mod.f90:1815-1849) ----
+// Pass 1: one block per interior row: compensated row sum of (stage - elev), times
+// area_cell_y(j). Pass 2: one block sums the rows in order.
+template <typename T>
+__global__ __launch_bounds__(256) void volume_rows_kernel(const T *__restrict__ S, const T *__restrict__ C,
+                                                          const T *__restrict__ E, int nx, int pitch, int n_ext,
+                                                          const double *__restrict__ area, double *row_work)
+{
+    const int j = n_ext + blockIdx.x;
+    DD acc{0.0, 0.0};
+    for (int i = n_ext + threadIdx.x; i <= nx - n_ext - 1; i += blockDim.x) {
+        const long long o = (long long)j * pitch + i;
+        double sv = (double)S[o];
+        if (C) acc = dd_add(acc, -(double)C[o]);
+        acc = dd_add(acc, sv - (double)E[o]);
+    }
+    DD r = block_reduce_dd(acc);
+    if (threadIdx.x == 0) {
+        DD prod;
+        // area * (hi + lo) with the low-order part kept separately
+        prod.hi = area[j] * r.hi;
+        prod.lo = area[j] * r.lo;
+        row_work[2 * blockIdx.x] = prod.hi;
+        row_work[2 * blockIdx.x + 1] = prod.lo;
+    }
+}
+
+__global__ __launch_bounds__(256) void volume_sum_kernel(const double *row_work, int nrows, double *out)
+{
+    DD acc{0.0, 0.0};
+    for (int k = threadIdx.x; k < nrows; k += blockDim.x) acc = dd_add(acc, DD{row_work[2 * k], row_work[2 * k + 1]});
+    DD r = block_reduce_dd(acc);
+    if (threadIdx.x == 0) *out = r.hi + r.lo;
+}
+
+template <typename T>
+hipError_t launch_volume(const T *S, const T *C, const T *E, int nx, int ny, int pitch, int n_ext,
+                         const double *area, double *row_work, double *out, hipStream_t s)
+{
+    const int nrows = ny - 2 * n_ext;
+    if (nrows <= 0) {
+        return hipMemsetAsync(out, 0, sizeof(double), s);
+    }
+    hipLaunchKernelGGL(volume_rows_kernel<T>, dim3(nrows), dim3(256), 0, s, S, C, E, nx, pitch, n_ext, area,
+                       row_work);
+    hipLaunchKernelGGL(volume_sum_kernel, dim3(1), dim3(256), 0, s, row_work, nrows, out);
+    return hipGetLastError();
+}
+template hipError_t launch_volume<double>(const double *, const double *, const double *, int, int, int, int,
+                                          const double *, double *, double *, hipStream_t);
+template hipError_t launch_volume<float>(const float *, const float *, const float *, int, int, int, int,
+                                         const double *, double *, double *, hipStream_t);
+
+// ---- point gauges (point_gauge_mod.f90:208-245) ----
+template <typename T>
+__global__ void gauges_kernel(const T *__restrict__ S, const T *__restrict__ C, const T *__restrict__ UH,
+                              const T *__restrict__ VH, int n, const long long *__restrict__ idx, double *values,
+                              double *times, int sample, const DevScalars *tsrc, double t)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) times[sample] = tsrc ? tsrc->time : t;
+    if (g >= n) return;
+    const long long o = idx[g];
+    double *v = values + (long long)sample * 3 * n;
+    v[g] = C ? (double)S[o] - (double)C[o] : (double)S[o];
+    v[n + g] = (double)UH[o];
+    v[2 * n + g] = (double)VH[o];
+}
+
+template <typename T>
+hipError_t launch_gauges(const T *S, const T *C, const T *UH, const T *VH, int n, const long long *idx,
+                         double *values, double *times, int sample, const DevScalars *tsrc, double t, hipStream_t s)
+{
+    const int nb = n > 0 ? (n + 255) / 256 : 1;
+    hipLaunchKernelGGL(gauges_kernel<T>, dim3(nb), dim3(256), 0, s, S, C, UH, VH, n, idx, values, times, sample, tsrc,
+                       t);
+    return hipGetLastError();
+}
+template hipError_t launch_gauges<double>(const double *, const double *, const double *, const double *, int,
+                                          const long long *, double *, double *, int, const DevScalars *, double,
+                                          hipStream_t);
+template hipError_t launch_gauges<float>(const float *, const float *, const float *, const float *, int,
+                                         const long long *, double *, double *, int, const DevScalars *, double,
+                                         hipStream_t);
+
+// ---- stationary_timestep_max (domain_mod.f90:1906-1962) ----
+// The minimum of positive doubles is order-independent; it is taken on their bit
+// patterns, which order like the values.
+template <typename T>
+__global__ void timestep_max_kernel(const T *__restrict__ S, const T *__restrict__ E, int nx, int ny, int pitch,
+                                    int linear, double msl, const double *__restrict__ dbe,
+                                    const double *__restrict__ dle, unsigned long long *out_bits)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    double v = 1.0e300;
+    if (i < nx && j < ny) {
+        const long long o = (long long)j * pitch + i;
+        const double h = linear ? msl - (double)E[o] : (double)S[o] - (double)E[o];
+        const double dist = fmin(dbe[j] + dbe[j + 1], dle[i] + dle[i + 1]);
+        v = 0.5 * dist / sqrt(kGravity * fmax(h, kMinDepth));
+    }
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMin(out_bits, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void fill_double_kernel(double *p, double v) { *p = v; }
+
+template <typename T>
+hipError_t launch_timestep_max(const T *S, const T *E, int nx, int ny, int pitch, int linear, double msl,
+                               const double *dbe, const double *dle, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_double_kernel, dim3(1), dim3(1), 0, s, out, 1.0e300);
+    hipLaunchKernelGGL(timestep_max_kernel<T>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, S, E, nx, ny, pitch,
+                       linear, msl, dbe, dle, (unsigned long long *)out);
+    return hipGetLastError();
+}
+template hipError_t launch_timestep_max<double>(const double *, const double *, int, int, int, int, double,
+                                                const double *, const double *, double *, hipStream_t);
+template hipError_t launch_timestep_max<float>(const float *, const float *, int, int, int, int, double,
+                                               const double *, const double *, double *, hipStream_t);
+
+}  // namespace swh

@@ -1,0 +1,241 @@
+"""Host-side mirror of SWALS's domain_type for the HIP backend.
+
+``Domain`` keeps the reference's field names and call sequence
+(propagation/SWALS/src/shallow_water/domain_mod.f90): construct it with the grid
+geometry and ``timestepping_method``, which runs the metric part of
+``allocate_quantities`` (:852-1014) on the host in FP64, set ``U`` (and
+``manning_squared``), then call ``evolve_one_step(timestep)`` (:1679-1813) in a
+loop. The state lives on the GPU; ``U`` is only copied back when read.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import backend as B
+
+GRAVITY = 9.8
+RADIUS_EARTH = 6371000.0
+EARTH_ANGULAR_FREQ = 7.292115e-05  # spherical_mod.f90:15
+DEG2RAD = math.acos(-1.0) / 180.0  # spherical_mod.f90:8
+
+
+def area_lonlat_rectangle_flat(lon1, lat1, dlon, dlat):
+    """area_lonlat_rectangle(..., flat=.TRUE.) of spherical_mod.f90:46-50."""
+    return (RADIUS_EARTH * RADIUS_EARTH * math.cos((lat1 + 0.5 * dlat) * DEG2RAD) * abs(dlat) * DEG2RAD *
+            math.fmod(abs(dlon), 360.0) * DEG2RAD)
+
+
+def build_metrics(nx, ny, lw, ll, spherical, coriolis=True):
+    """Metric arrays of allocate_quantities (domain_mod.f90:918-1014), scalar libm
+    calls in the reference's operation order so the values match it bit for bit."""
+    dx = (lw[0] / (nx * 1.0), lw[1] / (ny * 1.0))
+    x = np.array([ll[0] + ((i - 0.5) / (nx * 1.0)) * lw[0] for i in range(1, nx + 1)])
+    y = np.array([ll[1] + ((j - 0.5) / (ny * 1.0)) * lw[1] for j in range(1, ny + 1)])
+    m = dict(x=x, y=y, dx=dx)
+    if spherical:
+        cbe = [math.cos((y[j] - 0.5 * dx[1]) * DEG2RAD) for j in range(ny)]
+        cbe.append(math.cos((y[ny - 1] + 0.5 * dx[1]) * DEG2RAD))
+        m["tanlat_on_radius_earth"] = np.array([math.tan(y[j] * DEG2RAD) / RADIUS_EARTH for j in range(ny)])
+        if coriolis:
+            m["coriolis"] = np.array([2.0 * math.sin(y[j] * DEG2RAD) * EARTH_ANGULAR_FREQ for j in range(ny)])
+            cb = [2.0 * EARTH_ANGULAR_FREQ * math.sin((y[j] - 0.5 * dx[1]) * DEG2RAD) for j in range(ny)]
+            cb.append(2.0 * EARTH_ANGULAR_FREQ * math.sin((y[ny - 1] + 0.5 * dx[1]) * DEG2RAD))
+            m["coriolis_bottom_edge"] = np.array(cb)
+        m["distance_left_edge"] = np.full(nx + 1, dx[1] * DEG2RAD * RADIUS_EARTH)
+        m["distance_bottom_edge"] = np.array([dx[0] * DEG2RAD * RADIUS_EARTH * c for c in cbe])
+        m["area_cell_y"] = np.array([area_lonlat_rectangle_flat(x[0] - dx[0] * 0.5, y[j] - dx[1] * 0.5, dx[0], dx[1])
+                                     for j in range(ny)])
+    else:
+        m["distance_left_edge"] = np.full(nx + 1, dx[1])
+        m["distance_bottom_edge"] = np.full(ny + 1, dx[0])
+        m["area_cell_y"] = np.full(ny, dx[0] * dx[1])
+    return m
+
+
+def _p(a):
+    return a.ctypes.data_as(B._PD)
+
+
+class Domain:
+    """One single-grid SWALS domain advanced on the GPU.
+
+    Arrays are numpy float64 of shape (4, ny, nx) for U (planes stage, uh, vh, elev)
+    and (ny, nx) for 2-D fields: the byte layout of the reference's Fortran arrays.
+    """
+
+    def __init__(self, nx, ny, lw, lower_left, timestepping_method="linear", precision="fp64", spherical=False,
+                 coriolis=None, linear_solver_is_truely_linear=True, msl_linear=0.0, linear_friction_coeff=0.0,
+                 friction_type="manning", cfl=None, theta=None, max_U_variables=("max_stage",),
+                 max_U_update_frequency=1, exterior_cells_width=2, arrival_stage_threshold_above_msl_linear=0.01):
+        lib = B.load()
+        if timestepping_method not in B.SWH_METHOD:
+            raise ValueError("timestepping_method %r is not provided by this backend" % (timestepping_method,))
+        if precision not in B.SWH_PREC:
+            raise ValueError("precision must be 'fp64' or 'fp32k'")
+        self.nx, self.ny = int(nx), int(ny)
+        self.lw, self.lower_left = tuple(map(float, lw)), tuple(map(float, lower_left))
+        self.timestepping_method = timestepping_method
+        self.precision = precision
+        self.spherical = bool(spherical)
+        self.coriolis = self.spherical if coriolis is None else bool(coriolis)
+        self.msl_linear = float(msl_linear)
+        self.linear_solver_is_truely_linear = bool(linear_solver_is_truely_linear)
+        self.max_U_variables = tuple(max_U_variables or ())
+        self.metrics = build_metrics(self.nx, self.ny, self.lw, self.lower_left, self.spherical, self.coriolis)
+        self.dx = self.metrics["dx"]
+        self.x, self.y = self.metrics["x"], self.metrics["y"]
+        d = B.DomainDesc()
+        d.nx, d.ny = self.nx, self.ny
+        d.timestepping_method = B.SWH_METHOD[timestepping_method]
+        d.precision = B.SWH_PREC[precision]
+        d.linear_solver_is_truely_linear = int(self.linear_solver_is_truely_linear)
+        d.include_coriolis = int(self.coriolis and self.spherical)
+        d.spherical = int(self.spherical)
+        d.friction_type = 1 if friction_type == "chezy" else 0
+        d.exterior_cells_width = int(exterior_cells_width)
+        for k in range(4):
+            d.boundary_exterior[k] = 1
+        mask = 0
+        for v in self.max_U_variables:
+            mask |= B.MAXQ_BITS[v]
+        d.max_U_mask = mask
+        d.max_U_update_frequency = int(max_U_update_frequency)
+        d.msl_linear = self.msl_linear
+        d.cfl = float(cfl) if cfl else -1.0
+        d.theta = float(theta) if theta else -1.0
+        d.linear_friction_coeff = float(linear_friction_coeff)
+        d.maximum_timestep = -1.0
+        d.arrival_stage_threshold_above_msl_linear = float(arrival_stage_threshold_above_msl_linear)
+        self._keep = {}
+        for k in ["distance_bottom_edge", "distance_left_edge", "area_cell_y", "coriolis", "coriolis_bottom_edge",
+                  "tanlat_on_radius_earth"]:
+            if k in self.metrics:
+                arr = np.ascontiguousarray(self.metrics[k], dtype=np.float64)
+                self._keep[k] = arr
+                setattr(d, k, _p(arr))
+        self._desc = d
+        h = ctypes.c_void_p()
+        B.check(lib.swh_domain_create(ctypes.byref(d), ctypes.byref(h)), "swh_domain_create")
+        self._h = h
+        self._lib = lib
+        self._gauge_n = 0
+
+    # -- lifetime --
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.swh_domain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state --
+    @property
+    def U(self):
+        out = np.empty((4, self.ny, self.nx))
+        B.check(self._lib.swh_download_U(self._h, _p(out)), "swh_download_U")
+        return out
+
+    @U.setter
+    def U(self, value):
+        a = np.ascontiguousarray(value, dtype=np.float64)
+        if a.shape != (4, self.ny, self.nx):
+            raise ValueError("U must have shape (4, ny, nx) = %s" % ((4, self.ny, self.nx),))
+        B.check(self._lib.swh_upload_U(self._h, _p(a)), "swh_upload_U")
+
+    def set_manning_squared(self, m):
+        a = np.ascontiguousarray(np.broadcast_to(m, (self.ny, self.nx)), dtype=np.float64)
+        B.check(self._lib.swh_upload_manning_squared(self._h, _p(a)), "swh_upload_manning_squared")
+
+    def set_time(self, t):
+        B.check(self._lib.swh_set_time(self._h, float(t)), "swh_set_time")
+
+    def set_stage_boundary_sine(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
+        s = (B._I32 * 4)(*[int(bool(v)) for v in sides])
+        B.check(self._lib.swh_set_stage_boundary_sine(self._h, float(amp), float(period), float(msl), s),
+                "swh_set_stage_boundary_sine")
+
+    # -- stepping --
+    def evolve_one_step(self, timestep=None):
+        B.check(self._lib.swh_evolve_one_step(self._h, -1.0 if timestep is None else float(timestep)),
+                "swh_evolve_one_step")
+
+    def evolve_steps(self, timestep, nsteps):
+        B.check(self._lib.swh_evolve_steps(self._h, -1.0 if timestep is None else float(timestep), int(nsteps)),
+                "swh_evolve_steps")
+
+    def synchronize(self):
+        B.check(self._lib.swh_synchronize(self._h), "swh_synchronize")
+
+    # -- diagnostics --
+    def scalars(self):
+        s = B.Scalars()
+        B.check(self._lib.swh_get_scalars(self._h, ctypes.byref(s)), "swh_get_scalars")
+        return dict(time=s.time, max_dt=s.max_dt, evolve_step_dt=s.evolve_step_dt, dt_last_update=s.dt_last_update,
+                    boundary_flux_store=list(s.boundary_flux_store),
+                    boundary_flux_evolve_integral=s.boundary_flux_evolve_integral,
+                    boundary_flux_time_integral=s.boundary_flux_time_integral,
+                    negative_depth_fix_counter=s.negative_depth_fix_counter, nsteps_advanced=s.nsteps_advanced)
+
+    @property
+    def time(self):
+        return self.scalars()["time"]
+
+    def volume_interior(self):
+        v = ctypes.c_double()
+        B.check(self._lib.swh_volume_interior(self._h, ctypes.byref(v)), "swh_volume_interior")
+        return v.value
+
+    def mass_balance_interior(self):
+        """volume_interior + boundary_flux_time_integral (domain_mod.f90:1891-1904)."""
+        return self.volume_interior() + self.scalars()["boundary_flux_time_integral"]
+
+    def stationary_timestep_max(self):
+        v = ctypes.c_double()
+        B.check(self._lib.swh_stationary_timestep_max(self._h, ctypes.byref(v)), "swh_stationary_timestep_max")
+        return v.value
+
+    def max_U(self, name):
+        out = np.empty((self.ny, self.nx))
+        B.check(self._lib.swh_download_max_U(self._h, B.MAXQ_BITS[name], _p(out)), "swh_download_max_U")
+        return out
+
+    # -- gauges (point_gauge_mod.f90) --
+    def setup_point_gauges(self, xy, max_samples=100000):
+        """Nearest-cell gauges: site index = 1 + min(floor((xy - ll)/dx), nx - 1)
+        (point_gauge_mod.f90:329-336)."""
+        xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2)
+        i = 1 + np.minimum(np.floor((xy[:, 0] - self.lower_left[0]) / self.dx[0]), self.nx - 1)
+        j = 1 + np.minimum(np.floor((xy[:, 1] - self.lower_left[1]) / self.dx[1]), self.ny - 1)
+        ij = np.ascontiguousarray(np.stack([i, j], axis=1).astype(np.int32))
+        if (ij < 1).any() or (ij[:, 0] > self.nx).any() or (ij[:, 1] > self.ny).any():
+            raise ValueError("gauge outside the domain")
+        self.gauge_ij = ij
+        self._gauge_n = len(ij)
+        self._gauge_max = int(max_samples)
+        B.check(self._lib.swh_gauges_set(self._h, len(ij), ij.ctypes.data_as(B._PI32), int(max_samples)),
+                "swh_gauges_set")
+        return ij
+
+    def sample_gauges(self):
+        B.check(self._lib.swh_gauges_sample(self._h), "swh_gauges_sample")
+
+    def gauge_series(self):
+        """Returns (times[nsamples], values[nsamples, 3, ngauges]) with vars [stage, uh, vh]."""
+        n = self._gauge_n
+        vals = np.empty(3 * max(n, 1) * self._gauge_max)
+        times = np.empty(self._gauge_max)
+        ns = B._I32()
+        B.check(self._lib.swh_gauges_download(self._h, _p(vals), _p(times), ctypes.byref(ns)), "swh_gauges_download")
+        k = ns.value
+        return times[:k].copy(), vals[:3 * n * k].reshape(k, 3, n).copy()

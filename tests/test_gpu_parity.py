@@ -1,0 +1,203 @@
+"""GPU parity: the HIP backend through its C ABI against the C restatement of the
+reference (oracle/), on identical seeded inputs.
+
+Bars:
+* FP64 leapfrog ('linear', truely-linear 'leapfrog_linear_plus_nonlinear_friction'):
+  bit-identical state, max_U grids and gauges (same operation order, no FMA, and
+  friction_work built on the host with the same libm as the reference).
+* FP64 rk2 and not-truely-linear friction: the device pow/sin differ from glibc by
+  ulps, so agreement is to a stated round-off tolerance.
+* FP32 with Kahan-compensated stage: stage error against the FP64 oracle within the
+  tolerance stated per test.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+MAXQ = ["max_stage", "max_speed", "max_flux", "arrival_time", "time_of_max_stage", "min_stage"]
+
+
+def _print_case(name):
+    print("CASE", name, flush=True)
+
+
+def _cmp_exact(g, o, nsteps_label=""):
+    Ug, Uo = g.U, o.get_U()
+    for k in range(4):
+        if not np.array_equal(Ug[k], Uo[k]):
+            d = np.abs(Ug[k] - Uo[k])
+            idx = np.unravel_index(np.argmax(d), d.shape)
+            raise AssertionError("plane %d differs %s: max |diff| %g at %s" % (k, nsteps_label, d.max(), idx))
+
+
+@pytest.mark.parametrize("case", ["linear_cart", "linear_cart_lfc", "linear_sph", "llpnf_sph", "linear_cart_ntl"])
+def test_leapfrog_fp64_bitexact(oracle_built, case):
+    _print_case(case)
+    kw = {}
+    if case.startswith("linear_cart") or case == "linear_cart_ntl":
+        cfg = H.small_cartesian_linear()
+    else:
+        cfg = H.small_spherical()
+    if case == "linear_cart_lfc":
+        kw["linear_friction_coeff"] = 1e-4
+    if case == "linear_cart_ntl":
+        kw["truely_linear"] = False
+    if case == "llpnf_sph":
+        cfg = dict(cfg, method="leapfrog_linear_plus_nonlinear_friction",
+                   manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.03 ** 2))
+    nsteps = 60
+    o, dt, _ = H.cpu_run("oracle", cfg, nsteps, **kw)
+    gkw = dict(kw)
+    if "truely_linear" in gkw:
+        gkw["linear_solver_is_truely_linear"] = gkw.pop("truely_linear")
+    g = H.gpu_domain(cfg, **gkw)
+    assert g.stationary_timestep_max() == dt
+    g.evolve_steps(dt, nsteps)
+    _cmp_exact(g, o)
+    for q in MAXQ:
+        assert np.array_equal(g.max_U(q), o.max_U(q)), q
+    sg, so = g.scalars(), o.scalars()
+    assert sg["time"] == so["time"]
+    assert sg["boundary_flux_time_integral"] == pytest.approx(so["boundary_flux_time_integral"], rel=1e-12, abs=1e-6)
+    assert g.volume_interior() == pytest.approx(o.volume_interior(), rel=1e-14)
+    g.close()
+
+
+def test_llpnf_not_truely_linear_fp64(oracle_built):
+    _print_case("llpnf_sph_ntl")
+    cfg = H.small_spherical()
+    cfg = dict(cfg, method="leapfrog_linear_plus_nonlinear_friction",
+               manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.03 ** 2))
+    o, dt, _ = H.cpu_run("oracle", cfg, 60, truely_linear=False)
+    g = H.gpu_domain(cfg, linear_solver_is_truely_linear=False)
+    g.evolve_steps(dt, 60)
+    Ug, Uo = g.U, o.get_U()
+    # device pow() vs glibc pow(): ulp-level differences only
+    for k, scale in [(0, 1.0), (1, 50.0), (2, 50.0)]:
+        assert np.abs(Ug[k] - Uo[k]).max() <= 1e-12 * scale, k
+    g.close()
+
+
+def test_config1_full_size_fp64(oracle_built):
+    """Config 1 at its real size: 400 x 400 'linear', 1000 steps; max-stage grid and
+    the 16 gauges sampled every 50 steps are bit-identical to the oracle, and mass is
+    conserved to round-off (walls)."""
+    from ptha_amd import configs as C
+    _print_case("config1_full")
+    cfg = C.gaussian_hump_cartesian()
+    o = pyoracle.CpuDomain("oracle", cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], method="linear")
+    o.set_U(cfg["U"])
+    dt = o.stationary_timestep_max()
+    assert dt == pytest.approx(3.5355339059327373, rel=1e-15)
+    g = H.gpu_domain(cfg, max_U_variables=("max_stage",))
+    ij = g.setup_point_gauges(cfg["gauges_xy"], max_samples=64)
+    v0 = g.volume_interior()
+    ref_g = []
+    for block in range(20):
+        o.evolve(dt, 50)
+        g.evolve_steps(dt, 50)
+        g.sample_gauges()
+        U = o.get_U()
+        ref_g.append(np.array([[U[k][j - 1, i - 1] for (i, j) in ij] for k in range(3)]))
+    _, series = g.gauge_series()
+    assert series.shape == (20, 3, 16)
+    assert np.array_equal(series, np.array(ref_g))
+    assert np.array_equal(g.max_U("max_stage"), o.max_U("max_stage"))
+    _cmp_exact(g, o, "after 1000 steps")
+    v1 = g.volume_interior()
+    bf = g.scalars()["boundary_flux_time_integral"]
+    assert abs((v1 + bf) - v0) <= 1e-12 * abs(v0)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["linear_sph", "linear_cart"])
+def test_leapfrog_fp32k_vs_fp64_oracle(oracle_built, case):
+    """FP32 state with Kahan-compensated stage: stage within 1e-5 m of the FP64
+    oracle over 400 steps (1 m hump)."""
+    _print_case("fp32k_" + case)
+    cfg = H.small_spherical() if case == "linear_sph" else H.small_cartesian_linear()
+    nsteps = 400
+    o, dt, _ = H.cpu_run("oracle", cfg, nsteps)
+    g = H.gpu_domain(cfg, precision="fp32k")
+    v0 = g.volume_interior()
+    g.evolve_steps(dt, nsteps)
+    Ug, Uo = g.U, o.get_U()
+    wet = cfg["U"][3] < 0
+    err = np.abs(Ug[0] - Uo[0])[wet].max()
+    print("fp32k max stage error", err)
+    assert err <= 1e-5
+    # mass conservation (walls): the change of the FP64 volume of the FP32 state,
+    # plus the boundary-flux integral, relative to the volume of the initial hump.
+    # (The absolute volume differs from the FP64 run by the rounding of the FP32
+    # elevation, a constant.)
+    hump = float((cfg["U"][0] * wet * H.cell_areas(cfg)).sum())
+    dv = g.volume_interior() + g.scalars()["boundary_flux_time_integral"] - v0
+    print("fp32k relative mass change", dv / hump)
+    assert abs(dv) <= 1e-6 * hump
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["beach", "sph"])
+def test_rk2_fp64(oracle_built, case):
+    """rk2 (DE1 low-Froude HLL, wet/dry, Manning) with adaptive CFL steps decided on
+    the device, against the oracle. Device pow/sin vs glibc differ by ulps, so the bar
+    is round-off: states within 1e-9 (absolute, metres and m^2/s), max_dt and time
+    within 1e-12 relative."""
+    _print_case("rk2_" + case)
+    if case == "beach":
+        cfg = H.small_beach()
+        nsteps = 60
+    else:
+        cfg = H.small_spherical()
+        U = cfg["U"].copy()
+        U[3] = np.maximum(U[3], -300.0)
+        U[0] = np.maximum(U[0], U[3])
+        cfg = dict(cfg, U=U, method="rk2", manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.025 ** 2))
+        nsteps = 30
+    o, _, hist = H.cpu_run("oracle", cfg, nsteps, record=lambda d: d.scalars())
+    g = H.gpu_domain(cfg)
+    for k in range(nsteps):
+        g.evolve_one_step()
+        sg = g.scalars()
+        assert sg["max_dt"] == pytest.approx(hist[k]["max_dt"], rel=1e-12)
+        assert sg["time"] == pytest.approx(hist[k]["time"], rel=1e-12)
+    Ug, Uo = g.U, o.get_U()
+    wet = cfg["U"][3] < 1e5
+    for k in range(3):
+        d = np.abs(Ug[k] - Uo[k])[wet].max()
+        print("rk2 plane", k, "max diff", d)
+        assert d <= 1e-9, k
+    sg, so = g.scalars(), o.scalars()
+    assert sg["negative_depth_fix_counter"] == so["negative_depth_fix_counter"]
+    assert sg["boundary_flux_time_integral"] == pytest.approx(so["boundary_flux_time_integral"], rel=1e-9, abs=1e-9)
+    for q in ["max_stage", "max_flux", "min_stage"]:
+        assert np.abs(g.max_U(q) - o.max_U(q))[wet].max() <= 1e-9, q
+    g.close()
+
+
+def test_rk2_lake_at_rest():
+    """Well-balancing (Audusse): a flat free surface over rough bed with dry land stays
+    at rest; velocities stay zero to round-off."""
+    _print_case("rk2_lake_at_rest")
+    from ptha_amd import configs as C
+    cfg = C.beach_rk2(nx=64, ny=48, dx=10.0)
+    cfg.pop("stage_bc")
+    # walls on every side: an edge with neither a wall nor a boundary condition is not
+    # balanced in the reference either (its outer cells feel one-sided pressure)
+    U = cfg["U"].copy()
+    U[3] = C.add_walls(U[3])
+    U[0] = np.maximum(0.0, U[3])
+    cfg["U"] = U
+    g = H.gpu_domain(cfg)
+    for _ in range(50):
+        g.evolve_one_step()
+    U = g.U
+    wet = U[3] < 1e5
+    assert np.abs(U[1][wet]).max() < 1e-10
+    assert np.abs(U[2][wet]).max() < 1e-10
+    assert np.abs(U[0] - cfg["U"][0])[wet].max() < 1e-10
+    g.close()
