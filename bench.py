@@ -1,0 +1,253 @@
+"""Benchmark of the SWALS leapfrog hot path on one MI355X.
+
+Workload (the batched unit-source configuration): 512 Gaussian unit sources (1 m,
+sigma 25 km, 32 x 16 lattice at 50 km) on one shared 1200 x 1200 spherical grid at
+1 arcmin, 'linear' leapfrog with Coriolis, FP32 state with Kahan-compensated stage,
+power-law bathymetry with a shelf ramp to land, reflective walls, 64 gauges every
+60 s, durations U(2 h, 12 h).
+
+A step is one leapfrog step of every source (all 512 active over the timed steps).
+`value` is cell-updates per second over the timed steps (Gcell-updates/s, inputs
+resident in HBM). The same steps are also timed in FP64 ('fp64' object); the whole
+configuration (every source to its own duration) gives unit-sources/hour; the
+compiled reference solver on the host cores gives `cpu_baseline`.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s/GPU FP32-comp vs FP64 leapfrog; % HBM peak; unit-sources/hr"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell-update of the fused batched leapfrog step (one read
+# and one write of stage, Kahan term, uh, vh per source; elevation shared):
+BYTES_PER_CELL = {"fp32k": 32.0, "fp64": 48.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def linear_timestep(elev, metrics, cfl=0.7):
+    """linear_timestep_max (domain_mod.f90:1906-1931), vectorised (sqrt and min are
+    exact, so this equals the reference's value)."""
+    dbe, dle = metrics["distance_bottom_edge"], metrics["distance_left_edge"]
+    dist = np.minimum((dbe[:-1] + dbe[1:])[:, None], (dle[:-1] + dle[1:])[None, :])
+    ts = (0.5 * dist / np.sqrt(9.8 * np.maximum(0.0 - elev, 1.0e-05))).min()
+    return float(ts * cfl)
+
+
+def build_batch(cfg, precision, nsources, nsteps_per_source, every, max_samples, block=None):
+    from ptha_amd import configs as C
+    from ptha_amd.batch import UnitSourceBatch
+    b = UnitSourceBatch(cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], nsources, precision=precision)
+    b.set_elevation(cfg["elev"])
+    b.set_nsteps(nsteps_per_source[:nsources])
+    x, y = cfg["x"], cfg["y"]
+    land = cfg["elev"] >= 0.0
+    sig = cfg["sigma_km"] * 1e3
+    for k in range(nsources):
+        cx, cy = cfg["centres"][k]
+        # separable Gaussian (local equirectangular distance, as configs.gaussian_on_sphere)
+        gx = np.exp(-(np.radians(x - cx) * C.RADIUS_EARTH * math.cos(math.radians(cy))) ** 2 / (2 * sig ** 2))
+        gy = np.exp(-(np.radians(y - cy) * C.RADIUS_EARTH) ** 2 / (2 * sig ** 2))
+        stage = np.outer(gy, gx)
+        stage[land] = 0.0
+        b.set_source(k, stage)
+    b.setup_point_gauges(cfg["gauges_xy"], every=every, max_samples=max_samples)
+    if block:
+        b.set_launch(*block)
+    return b
+
+
+def time_steps(b, dt, steps, warmup):
+    if warmup:
+        b.evolve(dt, warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    ms = b.evolve_timed(dt, steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ms
+
+
+def cpu_baseline_leg(cfg, dt, budget_s):
+    """Times the compiled reference solver (oracle/_ref/libswals_ref_sph.so, OpenMP) on
+    one unit source of the same grid, in a child process so OMP_NUM_THREADS applies."""
+    code = r"""
+import json, os, sys, time, math
+import numpy as np
+sys.path.insert(0, %(root)r); sys.path.insert(0, os.path.join(%(root)r, 'oracle'))
+import pyoracle
+from ptha_amd import configs as C
+cfg = C.unit_sources()
+nx, ny = cfg['nx'], cfg['ny']
+d = pyoracle.CpuDomain('ref', nx, ny, cfg['lw'], cfg['ll'], method='linear', spherical=True)
+d.set_record_max_U(False)
+U = np.zeros((4, ny, nx)); U[3] = cfg['elev']
+U[0] = C.gaussian_on_sphere(cfg['x'], cfg['y'], *cfg['centres'][0], cfg['sigma_km']*1e3)
+U[0][U[3] >= 0] = 0.0
+d.set_U(U)
+dt = %(dt)r
+t0 = time.perf_counter(); d.evolve(dt, 20); t1 = time.perf_counter()
+n = max(20, int(%(budget)r / max(t1 - t0, 1e-9) * 20))
+t0 = time.perf_counter(); d.evolve(dt, n); el = time.perf_counter() - t0
+print(json.dumps(dict(steps=n, seconds=el, cells=nx*ny)))
+""" % dict(root=ROOT, dt=dt, budget=budget_s)
+    env = dict(os.environ)
+    ncores = int(env.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    env["OMP_NUM_THREADS"] = str(ncores)
+    env.setdefault("OMP_PROC_BIND", "true")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
+    o = json.loads(r.stdout.strip().splitlines()[-1])
+    gcell = o["cells"] * o["steps"] / o["seconds"] / 1e9
+    return dict(value=gcell, unit="Gcell-updates/s", cores=ncores, kind="reference",
+                sample="compiled reference SWALS (amdflang -O2, OpenMP) 'linear' spherical+Coriolis leapfrog, "
+                       "one unit source of the 1200x1200 1-arcmin grid, %d steps (%.1f s), no max_U/gauges; "
+                       "CPU model: %s" % (o["steps"], o["seconds"], cpu_model()))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--sources", type=int, default=512)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-full", action="store_true", help="skip the whole-configuration unit-sources/hr leg")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--block", type=str, default="", help="block_x,rows_per_block of the step kernel")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path; rerun with --gpus 1",
+              file=sys.stderr)
+        sys.exit(3)
+
+    from ptha_amd import configs as C
+    from ptha_amd.domain import build_metrics
+    block = tuple(int(v) for v in args.block.split(",")) if args.block else None
+    log("building workload")
+    cfg = C.unit_sources()
+    nx, ny = cfg["nx"], cfg["ny"]
+    m = build_metrics(nx, ny, cfg["lw"], cfg["ll"], True)
+    dt = linear_timestep(cfg["elev"], m)
+    nsteps_src = np.ceil(cfg["durations"] / dt).astype(np.int32)
+    every = max(1, int(round(cfg["gauge_interval"] / dt)))
+    nsrc = args.sources
+    max_samples = int(nsteps_src.max()) // every + 2
+    total_steps = max(args.warmup + args.steps, 1)
+    if nsteps_src[:nsrc].min() < total_steps:
+        raise SystemExit("timed steps exceed the shortest source")
+
+    cells = nx * ny
+    res = {}
+    for prec in (["fp32k"] if args.no_fp64 else ["fp32k", "fp64"]):
+        log("timing", prec)
+        b = build_batch(cfg, prec, nsrc, nsteps_src, every, max_samples, block)
+        wall, ms = time_steps(b, dt, args.steps, args.warmup)
+        step, nact = b.status()
+        assert nact == nsrc
+        gcell = nsrc * cells * args.steps / wall / 1e9
+        launch_ms = ms / args.steps
+        alg = BYTES_PER_CELL[prec] * nsrc * cells
+        res[prec] = dict(wall=wall, event_ms=ms, gcell=gcell, launch_ms=launch_ms,
+                         achieved_gbs=alg / (launch_ms * 1e-3) / 1e9)
+        log(prec, "Gcell/s %.1f  ms/step %.3f  HBM %.0f GB/s" % (gcell, 1e3 * wall / args.steps,
+                                                                 res[prec]["achieved_gbs"]))
+        b.close()
+
+    full = None
+    if not args.no_full:
+        log("whole configuration (every source to its duration)")
+        b = build_batch(cfg, "fp32k", nsrc, nsteps_src, every, max_samples, block)
+        b.synchronize()
+        t0 = time.perf_counter()
+        chunk = 2000
+        while True:
+            step, nact = b.status()
+            if nact == 0:
+                break
+            b.evolve(dt, chunk)
+            b.synchronize()
+            log("  step", b.status()[0], "active", b.status()[1])
+        wall = time.perf_counter() - t0
+        total_cell_updates = float(cells) * float(nsteps_src[:nsrc].sum())
+        full = dict(unit_sources_per_hr=nsrc / (wall / 3600.0), seconds=wall,
+                    gcell_per_s=total_cell_updates / wall / 1e9, sources=nsrc,
+                    gauge_samples_per_source=int(b.gauge_series(0).shape[0]))
+        b.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            log("cpu baseline")
+            cpu = cpu_baseline_leg(cfg, dt, args.cpu_seconds)
+        except Exception as e:  # the reference library may be absent from a tree
+            log("cpu baseline unavailable:", e)
+
+    r32 = res["fp32k"]
+    out = {
+        "metric": METRIC,
+        "value": round(r32["gcell"], 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r32["wall"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched unit sources: %d Gaussian sources on one 1200x1200 spherical 1-arcmin "
+                               "'linear' leapfrog grid (Coriolis), FP32 state + Kahan-compensated stage" % nsrc,
+                   "nx": nx, "ny": ny, "sources": nsrc, "dt_s": dt, "gauges": 64, "gauge_every_steps": every},
+        "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("leapfrog_batch_fp32k")},
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in res:
+        r64 = res["fp64"]
+        out["fp64"] = {"value": round(r64["gcell"], 3), "ms_per_step": round(1e3 * r64["wall"] / args.steps, 4),
+                       "achieved_gbs": round(r64["achieved_gbs"], 1),
+                       "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                       "fp32k_over_fp64": round(r32["gcell"] / r64["gcell"], 3)}
+    if full:
+        out["unit_sources_per_hr"] = round(full["unit_sources_per_hr"], 1)
+        out["full_run"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in full.items()}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,44 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 
 int swh_synchronize(swh_domain *h);
 
+/* ---- Batched unit sources on one shared grid ----
+ * PTHA unit-source production runs one serial 'linear' model per source over a shared
+ * grid and bathymetry (R/examples/austptha_template/SOURCE_ZONES/TEMPLATE/
+ * TSUNAMI_UNIT_SOURCE/run_16.sh:6-36, template/model_namelist.in:7-12), each with its
+ * own duration, writing gauge series (point_gauge_mod.f90:247-294). A batch advances all
+ * of them together: one launch per step covers every active source; a source stops
+ * (its state and gauges freeze) once it has taken its own number of steps. The desc
+ * describes the shared grid; method must be SWH_METHOD_LINEAR or SWH_METHOD_LLPNF and
+ * max_U tracking is not available in a batch. */
+typedef struct swh_batch swh_batch;
+
+int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **out);
+int swh_batch_destroy(swh_batch *b);
+/* Shared elevation (nx, ny) and, for LLPNF, manning_squared (nx, ny). */
+int swh_batch_upload_elevation(swh_batch *b, const double *elev);
+int swh_batch_upload_manning_squared(swh_batch *b, const double *manning_squared);
+/* Initial [STG, UH, VH] of source k: 3 planes (nx, ny) in Fortran order. */
+int swh_batch_upload_source(swh_batch *b, int32_t k, const double *U3);
+int swh_batch_download_source(swh_batch *b, int32_t k, double *U3);
+/* Number of leapfrog steps each source takes (its duration / dt). */
+int swh_batch_set_nsteps(swh_batch *b, const int32_t *nsteps_per_source);
+/* Gauges sampled for every active source at the steps where step % every == 0
+ * (step 0 = initial state), up to max_samples per source. */
+int swh_batch_gauges_set(swh_batch *b, int32_t n, const int32_t *ij, int32_t every, int32_t max_samples);
+/* Advance the batch nsteps leapfrog steps of size dt (the shared fixed step). */
+int swh_batch_evolve(swh_batch *b, double dt, int32_t nsteps);
+/* Gauge values of source k: values[(sample * 3 + var) * n + gauge]; nsamples out. */
+int swh_batch_gauges_download(swh_batch *b, int32_t k, double *values, int32_t *nsamples);
+/* Steps taken so far by the batch, and number of sources still active. */
+int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active);
+/* Launch shape of the step kernel: threads per block (64..256, multiple of 64) and
+ * rows per thread strip. */
+int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block);
+int swh_batch_synchronize(swh_batch *b);
+/* swh_batch_evolve bracketed by HIP events recorded on the batch's stream; returns the
+ * device time of the region in milliseconds (for throughput measurement). */
+int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif

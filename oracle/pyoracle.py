@@ -71,6 +71,7 @@ def _load(kind, spherical):
     f("domain_stationary_timestep_max", _D, [hd])
     f("domain_get_max_U", None, [hd, _I, _PD])
     f("domain_set_stage_bc", None, [hd, _D, _D, _D, ctypes.POINTER(_I)])
+    f("domain_set_record_max_U", None, [hd, _I])
     _libs[key] = fns
     return fns
 
@@ -171,6 +172,9 @@ class CpuDomain:
         o = np.empty((self.ny, self.nx))
         self.f["domain_get_max_U"](self.h, MAXQ.index(name) + 1, _p(o))
         return o
+
+    def set_record_max_U(self, on):
+        self.f["domain_set_record_max_U"](self.h, int(bool(on)))
 
     def set_stage_bc(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
         s = (_I * 4)(*sides)

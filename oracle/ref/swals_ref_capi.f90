@@ -160,6 +160,12 @@ contains
         out(12) = pool(h)%dt_last_update
     end subroutine
 
+    subroutine ref_domain_set_record_max_U(h, flag) bind(C, name='ref_domain_set_record_max_U')
+        ! Switches update_max_quantities on or off (domain_mod.f90:2252 tests record_max_U)
+        integer(c_int), value :: h, flag
+        pool(h)%record_max_U = (flag /= 0)
+    end subroutine
+
     subroutine ref_domain_set_time(h, t) bind(C, name='ref_domain_set_time')
         integer(c_int), value :: h
         real(c_double), value :: t

@@ -67,6 +67,20 @@ SYMBOLS = [
     ("swh_gauges_sample", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_gauges_download", ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PI32]),
     ("swh_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_batch_create", ctypes.c_int, [ctypes.POINTER(DomainDesc), _I32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("swh_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_batch_upload_elevation", ctypes.c_int, [ctypes.c_void_p, _PD]),
+    ("swh_batch_upload_manning_squared", ctypes.c_int, [ctypes.c_void_p, _PD]),
+    ("swh_batch_upload_source", ctypes.c_int, [ctypes.c_void_p, _I32, _PD]),
+    ("swh_batch_download_source", ctypes.c_int, [ctypes.c_void_p, _I32, _PD]),
+    ("swh_batch_set_nsteps", ctypes.c_int, [ctypes.c_void_p, _PI32]),
+    ("swh_batch_gauges_set", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _I32, _I32]),
+    ("swh_batch_evolve", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
+    ("swh_batch_gauges_download", ctypes.c_int, [ctypes.c_void_p, _I32, _PD, _PI32]),
+    ("swh_batch_status", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _PI32]),
+    ("swh_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32]),
+    ("swh_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_batch_evolve_timed", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PD]),
 ]
 
 

@@ -1,0 +1,143 @@
+"""Batched unit sources on one shared grid (the PTHA unit-source workload).
+
+The reference runs one serial 'linear' SWALS model per unit source, each over the same
+grid and bathymetry, with its own duration, recording gauges
+(R/examples/austptha_template/SOURCE_ZONES/TEMPLATE/TSUNAMI_UNIT_SOURCE/run_16.sh,
+template/model_namelist.in). ``UnitSourceBatch`` advances all of them on one GPU: one
+kernel launch per leapfrog step covers every source still running.
+"""
+import ctypes
+
+import numpy as np
+
+from . import backend as B
+from .domain import build_metrics, _p
+
+
+class UnitSourceBatch:
+    def __init__(self, nx, ny, lw, lower_left, nsources, timestepping_method="linear", precision="fp32k",
+                 spherical=True, coriolis=None, msl_linear=0.0, linear_friction_coeff=0.0, friction_type="manning"):
+        lib = B.load()
+        self._lib = lib
+        self.nx, self.ny, self.nsources = int(nx), int(ny), int(nsources)
+        self.lw, self.lower_left = tuple(map(float, lw)), tuple(map(float, lower_left))
+        self.precision = precision
+        self.spherical = bool(spherical)
+        self.coriolis = self.spherical if coriolis is None else bool(coriolis)
+        self.metrics = build_metrics(self.nx, self.ny, self.lw, self.lower_left, self.spherical, self.coriolis)
+        self.dx = self.metrics["dx"]
+        d = B.DomainDesc()
+        d.nx, d.ny = self.nx, self.ny
+        d.timestepping_method = B.SWH_METHOD[timestepping_method]
+        d.precision = B.SWH_PREC[precision]
+        d.linear_solver_is_truely_linear = 1
+        d.include_coriolis = int(self.coriolis and self.spherical)
+        d.spherical = int(self.spherical)
+        d.friction_type = 1 if friction_type == "chezy" else 0
+        d.exterior_cells_width = 2
+        for k in range(4):
+            d.boundary_exterior[k] = 1
+        d.max_U_mask = 0
+        d.max_U_update_frequency = 1
+        d.msl_linear = float(msl_linear)
+        d.cfl = -1.0
+        d.theta = -1.0
+        d.linear_friction_coeff = float(linear_friction_coeff)
+        d.maximum_timestep = -1.0
+        d.arrival_stage_threshold_above_msl_linear = 0.01
+        self._keep = {}
+        for k in ["distance_bottom_edge", "distance_left_edge", "area_cell_y", "coriolis", "coriolis_bottom_edge",
+                  "tanlat_on_radius_earth"]:
+            if k in self.metrics:
+                arr = np.ascontiguousarray(self.metrics[k], dtype=np.float64)
+                self._keep[k] = arr
+                setattr(d, k, _p(arr))
+        h = ctypes.c_void_p()
+        B.check(lib.swh_batch_create(ctypes.byref(d), self.nsources, ctypes.byref(h)), "swh_batch_create")
+        self._h = h
+        self._ng = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.swh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_elevation(self, elev):
+        a = np.ascontiguousarray(elev, dtype=np.float64)
+        assert a.shape == (self.ny, self.nx)
+        B.check(self._lib.swh_batch_upload_elevation(self._h, _p(a)), "swh_batch_upload_elevation")
+
+    def set_manning_squared(self, m):
+        a = np.ascontiguousarray(np.broadcast_to(m, (self.ny, self.nx)), dtype=np.float64)
+        B.check(self._lib.swh_batch_upload_manning_squared(self._h, _p(a)), "swh_batch_upload_manning_squared")
+
+    def set_nsteps(self, nsteps):
+        a = np.ascontiguousarray(nsteps, dtype=np.int32)
+        assert a.shape == (self.nsources,)
+        B.check(self._lib.swh_batch_set_nsteps(self._h, a.ctypes.data_as(B._PI32)), "swh_batch_set_nsteps")
+
+    def set_source(self, k, stage, uh=None, vh=None):
+        U3 = np.zeros((3, self.ny, self.nx))
+        U3[0] = stage
+        if uh is not None:
+            U3[1] = uh
+        if vh is not None:
+            U3[2] = vh
+        B.check(self._lib.swh_batch_upload_source(self._h, int(k), _p(U3)), "swh_batch_upload_source")
+
+    def get_source(self, k):
+        U3 = np.empty((3, self.ny, self.nx))
+        B.check(self._lib.swh_batch_download_source(self._h, int(k), _p(U3)), "swh_batch_download_source")
+        return U3
+
+    def setup_point_gauges(self, xy, every, max_samples):
+        xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2)
+        i = 1 + np.minimum(np.floor((xy[:, 0] - self.lower_left[0]) / self.dx[0]), self.nx - 1)
+        j = 1 + np.minimum(np.floor((xy[:, 1] - self.lower_left[1]) / self.dx[1]), self.ny - 1)
+        ij = np.ascontiguousarray(np.stack([i, j], axis=1).astype(np.int32))
+        self.gauge_ij = ij
+        self._ng = len(ij)
+        self._gmax = int(max_samples)
+        B.check(self._lib.swh_batch_gauges_set(self._h, len(ij), ij.ctypes.data_as(B._PI32), int(every),
+                                               int(max_samples)), "swh_batch_gauges_set")
+        return ij
+
+    def gauge_series(self, k):
+        vals = np.empty(3 * max(self._ng, 1) * self._gmax)
+        ns = B._I32()
+        B.check(self._lib.swh_batch_gauges_download(self._h, int(k), _p(vals), ctypes.byref(ns)),
+                "swh_batch_gauges_download")
+        return vals[:3 * self._ng * ns.value].reshape(ns.value, 3, self._ng).copy()
+
+    def set_launch(self, block_x, rows_per_block):
+        B.check(self._lib.swh_batch_set_launch(self._h, int(block_x), int(rows_per_block)), "swh_batch_set_launch")
+
+    def evolve(self, dt, nsteps):
+        B.check(self._lib.swh_batch_evolve(self._h, float(dt), int(nsteps)), "swh_batch_evolve")
+
+    def evolve_timed(self, dt, nsteps):
+        ms = ctypes.c_double()
+        B.check(self._lib.swh_batch_evolve_timed(self._h, float(dt), int(nsteps), ctypes.byref(ms)),
+                "swh_batch_evolve_timed")
+        return ms.value
+
+    def synchronize(self):
+        B.check(self._lib.swh_batch_synchronize(self._h), "swh_batch_synchronize")
+
+    def status(self):
+        step = ctypes.c_int64()
+        na = B._I32()
+        B.check(self._lib.swh_batch_status(self._h, ctypes.byref(step), ctypes.byref(na)), "swh_batch_status")
+        return step.value, na.value

@@ -1,6 +1,7 @@
 // C ABI of the gfx950 SWALS backend: handle lifetime, host <-> device copies and
 // the per-step orchestration that domain%evolve_one_step performs in the reference
 // (propagation/SWALS/src/shallow_water/domain_mod.f90:1679-1813).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -685,6 +686,327 @@ int swh_synchronize(swh_domain *h)
 {
     if (!h) return fail(SWH_EINVAL, "null argument");
     SWH_HIP(hipStreamSynchronize(h->stream));
+    return SWH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched unit sources: one shared grid (metrics, elevation, friction work, row
+// coefficients held by an embedded single-grid handle) and nsrc state sets laid out
+// source-major, each source a pitch x ny plane per field. Sources are kept in slots
+// sorted by decreasing step count, so the sources still running at any step are a
+// prefix of the slots and one launch with gridDim.z = n_active covers them.
+// ---------------------------------------------------------------------------
+struct swh_batch {
+    swh_domain *g = nullptr;  // shared grid
+    int nsrc = 0;
+    void *S[2] = {}, *UH[2] = {}, *VH[2] = {}, *C[2] = {};
+    std::vector<int> nsteps_slot;  // steps of the source in each slot (descending)
+    std::vector<int> slot_of;      // user index -> slot
+    int nsteps_set = 0;
+    long long step = 0;
+    int cur0 = 0;                  // buffer holding step 0
+    // gauges
+    int ng = 0, every = 1, gmax = 0;
+    long long *d_gidx = nullptr;
+    double *d_gval = nullptr;      // [slot][sample][3][ng]
+    int block_x = 256, rows_per_block = 32;
+};
+
+namespace {
+
+template <typename T>
+__global__ void batch_gauges_kernel(const T *__restrict__ S, const T *__restrict__ C, const T *__restrict__ UH,
+                                    const T *__restrict__ VH, long long stride, int n,
+                                    const long long *__restrict__ idx, double *values, int gmax, int sample)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = blockIdx.y;
+    if (g >= n) return;
+    const long long o = (long long)slot * stride + idx[g];
+    double *v = values + ((long long)slot * gmax + sample) * 3 * n;
+    v[g] = C ? (double)S[o] - (double)C[o] : (double)S[o];
+    v[n + g] = (double)UH[o];
+    v[2 * n + g] = (double)VH[o];
+}
+
+int batch_active(const swh_batch *b, long long step)
+{
+    // number of slots whose step count exceeds 'step' (slots sorted descending)
+    int lo = 0, hi = b->nsrc;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (b->nsteps_slot[mid] > step) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T, bool KAHAN>
+int batch_sample(swh_batch *b, int n_active, int buf)
+{
+    if (!b->ng || n_active <= 0) return SWH_OK;
+    const long long sample = b->step / b->every;
+    if (sample >= b->gmax) return SWH_OK;
+    swh_domain *g = b->g;
+    dim3 grid((b->ng + 63) / 64, n_active);
+    hipLaunchKernelGGL(batch_gauges_kernel<T>, grid, dim3(64), 0, g->stream, (const T *)b->S[buf],
+                       KAHAN ? (const T *)b->C[buf] : (const T *)nullptr, (const T *)b->UH[buf], (const T *)b->VH[buf],
+                       (long long)g->plane_elems(), b->ng, b->d_gidx, b->d_gval, b->gmax, (int)sample);
+    SWH_HIP(hipGetLastError());
+    return SWH_OK;
+}
+
+template <typename T, bool KAHAN>
+int batch_evolve(swh_batch *b, double dt, int nsteps)
+{
+    swh_domain *g = b->g;
+    if (g->rc_dt != dt) {
+        int rc = build_rc<T>(g, dt);
+        if (rc) return rc;
+    }
+    int fric_mode = 0;
+    if (g->method == SWH_METHOD_LLPNF) fric_mode = 1;
+    if (fric_mode == 1 && !g->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload elevation first");
+    LFArgs<T> a;
+    a.nx = g->nx; a.ny = g->ny; a.pitch = g->pitch; a.src_stride = (long long)g->plane_elems();
+    a.E = (const T *)g->E;
+    a.rc = rc_view<T>(g);
+    a.fwU = (const T *)g->fwU; a.fwV = (const T *)g->fwV; a.nsq = (const T *)g->nsq;
+    a.msl = (T)g->msl; a.dt = (T)dt; a.dt_lfc = (T)(dt * g->lfc);
+    a.lin_fric_mult = (T)(1.0 / (1.0 + dt * g->lfc));
+    a.fric_power = (T)((g->friction_type == 1) ? -2.0 : -7.0 / 3.0);
+    a.bc_mask = 0; a.bc_stage = T(0);
+    a.rows_per_block = b->rows_per_block;
+    a.maxq_mask = 0; a.mx = nullptr; a.t_after = T(0); a.arrival_stage = T(0);
+    if (b->step == 0) {
+        int rc = batch_sample<T, KAHAN>(b, batch_active(b, -1), b->cur0);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < nsteps; k++) {
+        const int n_active = batch_active(b, b->step);
+        if (n_active == 0) break;
+        const int c = (int)((b->cur0 + b->step) & 1), o = 1 - c;
+        a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
+        a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
+        SWH_HIP(launch_leapfrog<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n_active,
+                                          b->block_x, g->stream));
+        b->step++;
+        if (b->ng && (b->step % b->every) == 0) {
+            int rc = batch_sample<T, KAHAN>(b, n_active, o);
+            if (rc) return rc;
+        }
+    }
+    return SWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **out)
+{
+    if (!desc || !out || nsources < 1) return fail(SWH_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    if (desc->timestepping_method != SWH_METHOD_LINEAR && desc->timestepping_method != SWH_METHOD_LLPNF)
+        return fail(SWH_EINVAL, "batches support the leapfrog methods only");
+    swh_domain_desc d = *desc;
+    d.max_U_mask = 0;
+    swh_batch *b = new swh_batch();
+    int rc = swh_domain_create(&d, &b->g);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    b->nsrc = nsources;
+    const size_t bytes = b->g->plane_elems() * b->g->esz * (size_t)nsources;
+    const bool kahan = b->g->prec == SWH_PREC_FP32K;
+    for (int k = 0; k < 2 && rc == SWH_OK; k++) {
+        rc = dmalloc(&b->S[k], bytes);
+        if (!rc) rc = dmalloc(&b->UH[k], bytes);
+        if (!rc) rc = dmalloc(&b->VH[k], bytes);
+        if (!rc && kahan) rc = dmalloc(&b->C[k], bytes);
+    }
+    if (rc) {
+        swh_batch_destroy(b);
+        return rc;
+    }
+    b->nsteps_slot.assign(nsources, 0);
+    b->slot_of.resize(nsources);
+    for (int k = 0; k < nsources; k++) b->slot_of[k] = k;
+    *out = b;
+    return SWH_OK;
+}
+
+int swh_batch_destroy(swh_batch *b)
+{
+    if (!b) return SWH_OK;
+    if (b->g && b->g->stream) (void)hipStreamSynchronize(b->g->stream);
+    for (int k = 0; k < 2; k++) { dfree(b->S[k]); dfree(b->UH[k]); dfree(b->VH[k]); dfree(b->C[k]); }
+    dfree(b->d_gidx); dfree(b->d_gval);
+    swh_domain_destroy(b->g);
+    delete b;
+    return SWH_OK;
+}
+
+int swh_batch_upload_elevation(swh_batch *b, const double *elev)
+{
+    if (!b || !elev) return fail(SWH_EINVAL, "null argument");
+    swh_domain *g = b->g;
+    SWH_HIP(hipStreamSynchronize(g->stream));
+    int rc = upload_plane_any(g, g->E, elev);
+    if (rc) return rc;
+    g->h_elev.assign(elev, elev + (size_t)g->nx * g->ny);
+    return build_friction_work(g);
+}
+
+int swh_batch_upload_manning_squared(swh_batch *b, const double *m)
+{
+    if (!b) return fail(SWH_EINVAL, "null argument");
+    return swh_upload_manning_squared(b->g, m);
+}
+
+int swh_batch_set_nsteps(swh_batch *b, const int32_t *ns)
+{
+    if (!b || !ns) return fail(SWH_EINVAL, "null argument");
+    if (b->step != 0) return fail(SWH_ESTATE, "step counts must be set before the batch starts");
+    std::vector<int> order(b->nsrc);
+    for (int k = 0; k < b->nsrc; k++) {
+        if (ns[k] < 0) return fail(SWH_EINVAL, "negative step count");
+        order[k] = k;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ns[x] > ns[y]; });
+    for (int s = 0; s < b->nsrc; s++) {
+        b->slot_of[order[s]] = s;
+        b->nsteps_slot[s] = ns[order[s]];
+    }
+    b->nsteps_set = 1;
+    return SWH_OK;
+}
+
+int swh_batch_upload_source(swh_batch *b, int32_t k, const double *U3)
+{
+    if (!b || !U3 || k < 0 || k >= b->nsrc) return fail(SWH_EINVAL, "bad source upload");
+    if (!b->nsteps_set) return fail(SWH_ESTATE, "call swh_batch_set_nsteps before uploading sources");
+    if (b->step != 0) return fail(SWH_ESTATE, "sources must be uploaded before the batch starts");
+    swh_domain *g = b->g;
+    SWH_HIP(hipStreamSynchronize(g->stream));
+    const size_t n = (size_t)g->nx * g->ny, off = (size_t)b->slot_of[k] * g->plane_elems() * g->esz;
+    const int c = b->cur0;
+    int rc;
+    if ((rc = upload_plane_any(g, (char *)b->S[c] + off, U3))) return rc;
+    if ((rc = upload_plane_any(g, (char *)b->UH[c] + off, U3 + n))) return rc;
+    if ((rc = upload_plane_any(g, (char *)b->VH[c] + off, U3 + 2 * n))) return rc;
+    if (b->C[c]) SWH_HIP(hipMemset((char *)b->C[c] + off, 0, g->plane_elems() * g->esz));
+    return SWH_OK;
+}
+
+int swh_batch_download_source(swh_batch *b, int32_t k, double *U3)
+{
+    if (!b || !U3 || k < 0 || k >= b->nsrc) return fail(SWH_EINVAL, "bad source download");
+    swh_domain *g = b->g;
+    SWH_HIP(hipStreamSynchronize(g->stream));
+    const int slot = b->slot_of[k];
+    const long long taken = std::min<long long>(b->step, b->nsteps_slot[slot]);
+    const int buf = (int)((b->cur0 + taken) & 1);
+    const size_t n = (size_t)g->nx * g->ny, off = (size_t)slot * g->plane_elems() * g->esz;
+    int rc;
+    if (g->prec == SWH_PREC_FP64) {
+        if ((rc = download_plane<double>(g, (char *)b->S[buf] + off, nullptr, U3))) return rc;
+        if ((rc = download_plane<double>(g, (char *)b->UH[buf] + off, nullptr, U3 + n))) return rc;
+        if ((rc = download_plane<double>(g, (char *)b->VH[buf] + off, nullptr, U3 + 2 * n))) return rc;
+    } else {
+        if ((rc = download_plane<float>(g, (char *)b->S[buf] + off, (char *)b->C[buf] + off, U3))) return rc;
+        if ((rc = download_plane<float>(g, (char *)b->UH[buf] + off, nullptr, U3 + n))) return rc;
+        if ((rc = download_plane<float>(g, (char *)b->VH[buf] + off, nullptr, U3 + 2 * n))) return rc;
+    }
+    return SWH_OK;
+}
+
+int swh_batch_gauges_set(swh_batch *b, int32_t n, const int32_t *ij, int32_t every, int32_t max_samples)
+{
+    if (!b || n < 0 || (n > 0 && !ij) || every < 1 || max_samples < 1) return fail(SWH_EINVAL, "bad gauge arguments");
+    swh_domain *g = b->g;
+    std::vector<long long> idx(n);
+    for (int q = 0; q < n; q++) {
+        const int i = ij[2 * q] - 1, j = ij[2 * q + 1] - 1;
+        if (i < 0 || i >= g->nx || j < 0 || j >= g->ny) return fail(SWH_EINVAL, "gauge cell outside the domain");
+        idx[q] = (long long)j * g->pitch + i;
+    }
+    SWH_HIP(hipStreamSynchronize(g->stream));
+    dfree(b->d_gidx); dfree(b->d_gval);
+    b->d_gidx = nullptr; b->d_gval = nullptr;
+    int rc = dmalloc((void **)&b->d_gidx, sizeof(long long) * (n > 0 ? n : 1));
+    if (!rc) rc = dmalloc((void **)&b->d_gval, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1) * max_samples * b->nsrc);
+    if (rc) return rc;
+    if (n > 0) SWH_HIP(hipMemcpy(b->d_gidx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    b->ng = n; b->every = every; b->gmax = max_samples;
+    return SWH_OK;
+}
+
+int swh_batch_evolve(swh_batch *b, double dt, int32_t nsteps)
+{
+    if (!b) return fail(SWH_EINVAL, "null argument");
+    if (!(dt > 0.0) || nsteps < 0) return fail(SWH_EINVAL, "dt must be > 0 and nsteps >= 0");
+    if (!b->nsteps_set) return fail(SWH_ESTATE, "call swh_batch_set_nsteps first");
+    return b->g->prec == SWH_PREC_FP64 ? batch_evolve<double, false>(b, dt, nsteps)
+                                       : batch_evolve<float, true>(b, dt, nsteps);
+}
+
+int swh_batch_gauges_download(swh_batch *b, int32_t k, double *values, int32_t *nsamples)
+{
+    if (!b || !values || !nsamples || k < 0 || k >= b->nsrc) return fail(SWH_EINVAL, "bad gauge download");
+    SWH_HIP(hipStreamSynchronize(b->g->stream));
+    const int slot = b->slot_of[k];
+    const long long taken = std::min<long long>(b->step, b->nsteps_slot[slot]);
+    const long long ns = std::min<long long>(taken / b->every + 1, b->gmax);
+    if (b->ng > 0 && ns > 0)
+        SWH_HIP(hipMemcpy(values, b->d_gval + (size_t)slot * b->gmax * 3 * b->ng, sizeof(double) * 3 * b->ng * ns,
+                          hipMemcpyDeviceToHost));
+    *nsamples = (int32_t)ns;
+    return SWH_OK;
+}
+
+int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active)
+{
+    if (!b || !step || !n_active) return fail(SWH_EINVAL, "null argument");
+    *step = b->step;
+    *n_active = batch_active(b, b->step);
+    return SWH_OK;
+}
+
+int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block)
+{
+    if (!b || block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1)
+        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1");
+    b->block_x = block_x;
+    b->rows_per_block = rows_per_block;
+    return SWH_OK;
+}
+
+int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elapsed_ms)
+{
+    if (!b || !elapsed_ms) return fail(SWH_EINVAL, "null argument");
+    hipEvent_t e0, e1;
+    SWH_HIP(hipEventCreate(&e0));
+    SWH_HIP(hipEventCreate(&e1));
+    SWH_HIP(hipEventRecord(e0, b->g->stream));
+    int rc = swh_batch_evolve(b, dt, nsteps);
+    SWH_HIP(hipEventRecord(e1, b->g->stream));
+    SWH_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    SWH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *elapsed_ms = ms;
+    return rc;
+}
+
+int swh_batch_synchronize(swh_batch *b)
+{
+    if (!b) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(b->g->stream));
     return SWH_OK;
 }
 
