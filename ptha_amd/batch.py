@@ -121,8 +121,9 @@ class UnitSourceBatch:
                 "swh_batch_gauges_download")
         return vals[:3 * self._ng * ns.value].reshape(ns.value, 3, self._ng).copy()
 
-    def set_launch(self, block_x, rows_per_block):
-        B.check(self._lib.swh_batch_set_launch(self._h, int(block_x), int(rows_per_block)), "swh_batch_set_launch")
+    def set_launch(self, block_x, rows_per_block, vec=1, prefetch=1):
+        B.check(self._lib.swh_batch_set_launch(self._h, int(block_x), int(rows_per_block), int(vec), int(prefetch)),
+                "swh_batch_set_launch")
 
     def evolve(self, dt, nsteps):
         B.check(self._lib.swh_batch_evolve(self._h, float(dt), int(nsteps)), "swh_batch_evolve")

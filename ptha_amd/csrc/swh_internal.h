@@ -79,7 +79,7 @@ struct LFArgs {
 // Kernel launchers (defined in the .hip files)
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                           int linear_friction, int nsrc, int block_x, hipStream_t s);
+                           int linear_friction, int nsrc, int block_x, int vec_mode, int prefetch, hipStream_t s);
 
 template <typename T>
 hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext,

@@ -21,40 +21,76 @@
 
 namespace swh {
 
+// Vector types for VEC consecutive columns per lane (16-byte loads and stores).
+template <typename T, int VEC>
+struct VecOf;
+template <>
+struct VecOf<float, 4> { typedef float4 type; };
+template <>
+struct VecOf<float, 2> { typedef float2 type; };
+template <>
+struct VecOf<double, 2> { typedef double2 type; };
 template <typename T>
-struct RowVals {
-    T s0, s1, c0, c1;   // stage and compensation at i, i+1
-    T uhm, uh0, uh1;    // UH at max(i-1,0), i, i+1
-    T vh0, vh1;         // VH at i, i+1
-    T e0, e1;           // ELV at i, i+1
+struct VecOf<T, 1> { typedef T type; };
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vload(const T *__restrict__ p, T *out)
+{
+    typedef typename VecOf<T, VEC>::type V;
+    const V v = *reinterpret_cast<const V *>(p);
+    const T *e = reinterpret_cast<const T *>(&v);
+#pragma unroll
+    for (int c = 0; c < VEC; c++) out[c] = e[c];
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore(T *__restrict__ p, const T *in, int nvalid)
+{
+    typedef typename VecOf<T, VEC>::type V;
+    if (nvalid >= VEC) {
+        V v;
+        T *e = reinterpret_cast<T *>(&v);
+#pragma unroll
+        for (int c = 0; c < VEC; c++) e[c] = in[c];
+        *reinterpret_cast<V *>(p) = v;
+    } else {
+        for (int c = 0; c < nvalid; c++) p[c] = in[c];
+    }
+}
+
+// One row of the fields a lane needs for columns i .. i+VEC-1 and their neighbours:
+// s, c, vh, e at columns i .. i+VEC; uh at columns i-1 .. i+VEC (index 0 = i-1).
+template <typename T, int VEC>
+struct VRow {
+    T s[VEC + 1], c[VEC + 1], vh[VEC + 1], e[VEC + 1];
+    T uh[VEC + 2];
 };
 
-template <typename T, bool KAHAN>
-__device__ __forceinline__ void load_row(const LFArgs<T> &a, const T *__restrict__ S, const T *__restrict__ C,
-                                         const T *__restrict__ UH, const T *__restrict__ VH, int i, int j,
-                                         RowVals<T> &r)
+template <typename T, int VEC, bool KAHAN>
+__device__ __forceinline__ void load_vrow(const LFArgs<T> &a, const T *__restrict__ S, const T *__restrict__ C,
+                                          const T *__restrict__ UH, const T *__restrict__ VH, int i, int j,
+                                          VRow<T, VEC> &r)
 {
+    // i is a multiple of VEC and the row pitch a multiple of 64, so the vector stays
+    // inside the (padded) row; the right-hand neighbour is read only when it is a cell.
     const long long o = (long long)j * a.pitch + i;
-    const bool has1 = (i + 1) < a.nx;
-    r.s0 = S[o];
-    r.uh0 = UH[o];
-    r.vh0 = VH[o];
-    r.e0 = a.E[o];
-    r.uhm = (i > 0) ? UH[o - 1] : r.uh0;
-    if (has1) {
-        r.s1 = S[o + 1];
-        r.uh1 = UH[o + 1];
-        r.vh1 = VH[o + 1];
-        r.e1 = a.E[o + 1];
-    } else {
-        r.s1 = r.s0; r.uh1 = r.uh0; r.vh1 = r.vh0; r.e1 = r.e0;
+    vload<T, VEC>(S + o, r.s);
+    vload<T, VEC>(UH + o, r.uh + 1);
+    vload<T, VEC>(VH + o, r.vh);
+    vload<T, VEC>(a.E + o, r.e);
+    if (KAHAN) vload<T, VEC>(C + o, r.c);
+    const bool right = (i + VEC) < a.nx;
+    r.s[VEC] = right ? S[o + VEC] : T(0);
+    r.uh[VEC + 1] = right ? UH[o + VEC] : T(0);
+    r.vh[VEC] = right ? VH[o + VEC] : T(0);
+    r.e[VEC] = right ? a.E[o + VEC] : T(0);
+    if (KAHAN) r.c[VEC] = right ? C[o + VEC] : T(0);
+    else {
+#pragma unroll
+        for (int c = 0; c <= VEC; c++) r.c[c] = T(0);
     }
-    if (KAHAN) {
-        r.c0 = C[o];
-        r.c1 = has1 ? C[o + 1] : r.c0;
-    } else {
-        r.c0 = T(0); r.c1 = T(0);
-    }
+    // UH at column i-1, clamped to column 0 at the western edge (include.f90:319-322)
+    r.uh[0] = (i > 0) ? UH[o - 1] : r.uh[1];
 }
 
 // New stage of cell (ii, jj): interior cells get the flux divergence
@@ -91,15 +127,27 @@ __device__ __forceinline__ void stage_new(const LFArgs<T> &a, int ii, int jj, T 
     }
 }
 
+// New stage of the VEC+1 cells i .. i+VEC of row jj (row values in q, VH of row jj-1 in vhm).
+template <typename T, int VEC, bool KAHAN>
+__device__ __forceinline__ void stage_row(const LFArgs<T> &a, int i, int jj, const VRow<T, VEC> &q,
+                                         const T *vhm, T *sn, T *cn)
+{
+#pragma unroll
+    for (int c = 0; c <= VEC; c++)
+        stage_new<T, KAHAN>(a, i + c, jj, q.s[c], q.c[c], q.uh[c + 1], q.uh[c], q.vh[c], vhm[c], sn[c], cn[c]);
+}
+
 template <typename T>
 __device__ __forceinline__ T dmaxT(T x, T y) { return x > y ? x : y; }
 template <typename T>
 __device__ __forceinline__ T dminT(T x, T y) { return x < y ? x : y; }
 
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+// The fused step. Each lane owns VEC adjacent columns and marches up a strip of
+// rows_per_block rows; PF keeps one more row of loads in flight (register prefetch).
+template <typename T, int VEC, bool PF, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) * VEC;
     if (i >= a.nx) return;
     const int j0 = blockIdx.y * a.rows_per_block;
     if (j0 >= a.ny) return;
@@ -116,195 +164,239 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
     const T MIN = T(kMinDepth);
     const T HALF = T(0.5);
     const T msl = a.msl;
+    const int nvalid = min(VEC, a.nx - i);
 
-    RowVals<T> r, q;
-    load_row<T, KAHAN>(a, S, C, UH, VH, i, j0, r);
+    VRow<T, VEC> r, q, p;
+    load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j0, r);
+    if (j0 + 1 < a.ny) load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j0 + 1, q);
     // VH one row below the strip (stage at j0, and the Coriolis 'j-1/2' average,
     // which the reference clamps to row 1 at the southern edge: include.f90:295-308)
-    T vhb0 = r.vh0, vhb1 = r.vh1;
+    T vhm[VEC + 1];
     if (j0 > 0) {
         const long long o = (long long)(j0 - 1) * a.pitch + i;
-        vhb0 = VH[o];
-        vhb1 = (i + 1 < a.nx) ? VH[o + 1] : vhb0;
+        vload<T, VEC>(VH + o, vhm);
+        vhm[VEC] = (i + VEC < a.nx) ? VH[o + VEC] : T(0);
+    } else {
+#pragma unroll
+        for (int c = 0; c <= VEC; c++) vhm[c] = r.vh[c];
     }
-    T sn0, cn0, sn1, cn1;
-    stage_new<T, KAHAN>(a, i, j0, r.s0, r.c0, r.uh0, r.uhm, r.vh0, vhb0, sn0, cn0);
-    stage_new<T, KAHAN>(a, i + 1, j0, r.s1, r.c1, r.uh1, r.uh0, r.vh1, vhb1, sn1, cn1);
-    T vhm0 = vhb0, vhm1 = vhb1;
+    T sn[VEC + 1], cn[VEC + 1];
+    stage_row<T, VEC, KAHAN>(a, i, j0, r, vhm, sn, cn);
 
     for (int j = j0; j < j1; j++) {
         const bool has_up = (j + 1) < a.ny;
-        T su0 = sn0, cu0 = cn0, su1 = sn1, cu1 = cn1;
+        if (PF && j + 2 < a.ny) load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j + 2, p);
+        if (!PF && has_up && j > j0) load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j + 1, q);
+        T su[VEC + 1], cu[VEC + 1];
         if (has_up) {
-            load_row<T, KAHAN>(a, S, C, UH, VH, i, j + 1, q);
-            stage_new<T, KAHAN>(a, i, j + 1, q.s0, q.c0, q.uh0, q.uhm, q.vh0, r.vh0, su0, cu0);
-            stage_new<T, KAHAN>(a, i + 1, j + 1, q.s1, q.c1, q.uh1, q.uh0, q.vh1, r.vh1, su1, cu1);
+            stage_row<T, VEC, KAHAN>(a, i, j + 1, q, r.vh, su, cu);
         } else {
             q = r;
+#pragma unroll
+            for (int c = 0; c <= VEC; c++) { su[c] = sn[c]; cu[c] = cn[c]; }
+        }
+        T uh_out[VEC], vh_out[VEC];
+#pragma unroll
+        for (int c = 0; c < VEC; c++) {
+            const int ii = i + c;
+            T uh_new = r.uh[c + 1], vh_new = r.vh[c];
+            if (ii < a.nx - 1 && j < a.ny - 1) {
+                // Depths at the UH (i+1/2) and VH (j+1/2) points: include.f90:257-345
+                const T e0 = r.e[c], e1 = r.e[c + 1], eu = q.e[c];
+                const T s0 = sn[c], s1 = sn[c + 1], sup = su[c];
+                T h_iph, h_jph, wet_iph, wet_jph;
+                if (TL) {
+                    h_iph = (e0 < -MIN + msl && e1 < -MIN + msl) ? msl - HALF * (e1 + e0) : T(0);
+                    h_jph = (eu < -MIN + msl && e0 < -MIN + msl) ? msl - HALF * (eu + e0) : T(0);
+                    wet_iph = T(1);
+                    wet_jph = T(1);
+                } else {
+                    h_iph = (s0 - e0 > MIN && s1 - e1 > MIN) ? HALF * ((s1 + s0) - (e1 + e0)) : T(0);
+                    h_jph = (sup - eu > MIN && s0 - e0 > MIN) ? HALF * ((sup + s0) - (eu + e0)) : T(0);
+                    wet_iph = h_iph > T(0) ? T(1) : T(0);
+                    wet_jph = h_jph > T(0) ? T(1) : T(0);
+                }
+                T ds_i = s1 - s0, ds_j = sup - s0;
+                if (KAHAN) {
+                    ds_i = ds_i - (cn[c + 1] - cn[c]);
+                    ds_j = ds_j - (cu[c] - cn[c]);
+                }
+                uh_new = r.uh[c + 1] * wet_iph - a.rc.cu[j] * h_iph * ds_i * a.rc.dle[ii + 1];
+                vh_new = r.vh[c] * wet_jph - a.rc.cv[j] * h_jph * ds_j * a.rc.dbe[j + 1];
+                if (COR || FRIC) {
+                    const T vh_jmh = HALF * (vhm[c] + vhm[c + 1]);
+                    const T vh_jph = HALF * (r.vh[c] + r.vh[c + 1]);
+                    const T uh_ij = HALF * (r.uh[c + 1] + r.uh[c]);
+                    const T uh_ijp1 = HALF * (q.uh[c + 1] + q.uh[c]);
+                    if (COR) {
+                        const T dhc = a.rc.fu[j] * (h_iph > T(0) ? T(1) : T(0));
+                        const T dhcj = a.rc.fv[j] * (h_jph > T(0) ? T(1) : T(0));
+                        uh_new = uh_new + dhc * (vh_jmh + vh_jph);
+                        vh_new = vh_new - dhcj * (uh_ij + uh_ijp1);
+                    }
+                    if (FRIC) {
+                        const long long o = (long long)j * a.pitch + ii;
+                        T fwu, fwv;
+                        if (FRIC == 1) {
+                            fwu = a.fwU[o];
+                            fwv = a.fwV[o];
+                        } else {
+                            // not-truely-linear: friction_work from the new stage each step
+                            // (leapfrog_solver_friction_include.f90:60-82)
+                            const T *__restrict__ ns = a.nsq;
+                            const T n0 = ns[o];
+                            const T dep_i = dmaxT(HALF * (s0 - e0 + s1 - e1), MIN);
+                            const T dep_j = dmaxT(HALF * (s0 - e0 + sup - eu), MIN);
+                            const T nsq_i = HALF * (n0 + ns[o + 1]);
+                            const T nsq_j = HALF * (n0 + ns[o + a.pitch]);
+                            fwu = T(kGravity) * nsq_i * pow(dep_i, a.fric_power);
+                            fwv = T(kGravity) * nsq_j * pow(dep_j, a.fric_power);
+                        }
+                        const T vbar = HALF * (vh_jmh + vh_jph);
+                        const T ubar = HALF * (uh_ij + uh_ijp1);
+                        const T ru = r.uh[c + 1], rv = r.vh[c];
+                        const T fmu = T(1) / (T(1) + a.dt_lfc + a.dt * fwu * (sqrt(ru * ru + vbar * vbar)));
+                        const T fmv = T(1) / (T(1) + a.dt_lfc + a.dt * fwv * (sqrt(rv * rv + ubar * ubar)));
+                        uh_new = fmu * uh_new;
+                        vh_new = fmv * vh_new;
+                    }
+                }
+                if (LINF) {
+                    uh_new = a.lin_fric_mult * uh_new;
+                    vh_new = a.lin_fric_mult * vh_new;
+                }
+            }
+            uh_out[c] = uh_new;
+            vh_out[c] = vh_new;
         }
         const long long o = (long long)j * a.pitch + i;
-        T uh_new = r.uh0, vh_new = r.vh0;
-        if (i < a.nx - 1 && j < a.ny - 1) {
-            // Depths at the UH (i+1/2) and VH (j+1/2) points: include.f90:257-345
-            T h_iph, h_jph, wet_iph, wet_jph;
-            if (TL) {
-                h_iph = (r.e0 < -MIN + msl && r.e1 < -MIN + msl) ? msl - HALF * (r.e1 + r.e0) : T(0);
-                h_jph = (q.e0 < -MIN + msl && r.e0 < -MIN + msl) ? msl - HALF * (q.e0 + r.e0) : T(0);
-                wet_iph = T(1);
-                wet_jph = T(1);
-            } else {
-                h_iph = (sn0 - r.e0 > MIN && sn1 - r.e1 > MIN) ? HALF * ((sn1 + sn0) - (r.e1 + r.e0)) : T(0);
-                h_jph = (su0 - q.e0 > MIN && sn0 - r.e0 > MIN) ? HALF * ((su0 + sn0) - (q.e0 + r.e0)) : T(0);
-                wet_iph = h_iph > T(0) ? T(1) : T(0);
-                wet_jph = h_jph > T(0) ? T(1) : T(0);
-            }
-            T ds_i = sn1 - sn0, ds_j = su0 - sn0;
-            if (KAHAN) {
-                ds_i = ds_i - (cn1 - cn0);
-                ds_j = ds_j - (cu0 - cn0);
-            }
-            uh_new = r.uh0 * wet_iph - a.rc.cu[j] * h_iph * ds_i * a.rc.dle[i + 1];
-            vh_new = r.vh0 * wet_jph - a.rc.cv[j] * h_jph * ds_j * a.rc.dbe[j + 1];
-            if (COR || FRIC) {
-                const T vh_jmh = HALF * (vhm0 + vhm1);
-                const T vh_jph = HALF * (r.vh0 + r.vh1);
-                const T uh_ij = HALF * (r.uh0 + r.uhm);
-                const T uh_ijp1 = HALF * (q.uh0 + q.uhm);
-                if (COR) {
-                    const T dhc = a.rc.fu[j] * (h_iph > T(0) ? T(1) : T(0));
-                    const T dhcj = a.rc.fv[j] * (h_jph > T(0) ? T(1) : T(0));
-                    uh_new = uh_new + dhc * (vh_jmh + vh_jph);
-                    vh_new = vh_new - dhcj * (uh_ij + uh_ijp1);
-                }
-                if (FRIC) {
-                    T fwu, fwv;
-                    if (FRIC == 1) {
-                        fwu = a.fwU[o];
-                        fwv = a.fwV[o];
-                    } else {
-                        // not-truely-linear: friction_work from the new stage each step
-                        // (leapfrog_solver_friction_include.f90:60-82)
-                        const int jp1 = min(j + 1, a.ny - 1);
-                        const T *__restrict__ ns = a.nsq;
-                        const T n0 = ns[o];
-                        const T dep_i = dmaxT(HALF * (sn0 - r.e0 + sn1 - r.e1), MIN);
-                        const T dep_j = dmaxT(HALF * (sn0 - r.e0 + su0 - q.e0), MIN);
-                        const T nsq_i = HALF * (n0 + ns[o + 1]);
-                        const T nsq_j = HALF * (n0 + ns[(long long)jp1 * a.pitch + i]);
-                        fwu = T(kGravity) * nsq_i * pow(dep_i, a.fric_power);
-                        fwv = T(kGravity) * nsq_j * pow(dep_j, a.fric_power);
-                    }
-                    const T vbar = HALF * (vh_jmh + vh_jph);
-                    const T ubar = HALF * (uh_ij + uh_ijp1);
-                    const T fmu = T(1) / (T(1) + a.dt_lfc + a.dt * fwu * (sqrt(r.uh0 * r.uh0 + vbar * vbar)));
-                    const T fmv = T(1) / (T(1) + a.dt_lfc + a.dt * fwv * (sqrt(r.vh0 * r.vh0 + ubar * ubar)));
-                    uh_new = fmu * uh_new;
-                    vh_new = fmv * vh_new;
-                }
-            }
-            if (LINF) {
-                uh_new = a.lin_fric_mult * uh_new;
-                vh_new = a.lin_fric_mult * vh_new;
-            }
-        }
-        So[o] = sn0;
-        if (KAHAN) Co[o] = cn0;
-        UHo[o] = uh_new;
-        VHo[o] = vh_new;
+        vstore<T, VEC>(So + o, sn, nvalid);
+        if (KAHAN) vstore<T, VEC>(Co + o, cn, nvalid);
+        vstore<T, VEC>(UHo + o, uh_out, nvalid);
+        vstore<T, VEC>(VHo + o, vh_out, nvalid);
 
         if (a.maxq_mask) {
             // update_max_quantities on the new state (domain_mod.f90:2243-2404),
-            // stage_extrema_include_dry_cells = .true.
+            // stage_extrema_include_dry_cells = .true.; single-source runs only
             const long long P = (long long)a.pitch * a.ny;
-            T *__restrict__ mx = a.mx;  // max_U is tracked for single-source runs only
+            T *__restrict__ mx = a.mx;
             const int m = a.maxq_mask;
-            if (m & SWH_MAXQ_MAX_STAGE) {
-                if (m & SWH_MAXQ_TIME_OF_MAX_STAGE) {
-                    if (sn0 > mx[MX_STAGE * P + o]) {
-                        mx[MX_STAGE * P + o] = sn0;
-                        mx[MX_TOMS * P + o] = a.t_after;
+            for (int c = 0; c < nvalid; c++) {
+                const long long oc = o + c;
+                const T s0 = sn[c], e0 = r.e[c], uhn = uh_out[c], vhn = vh_out[c];
+                if (m & SWH_MAXQ_MAX_STAGE) {
+                    if (m & SWH_MAXQ_TIME_OF_MAX_STAGE) {
+                        if (s0 > mx[MX_STAGE * P + oc]) {
+                            mx[MX_STAGE * P + oc] = s0;
+                            mx[MX_TOMS * P + oc] = a.t_after;
+                        }
+                    } else {
+                        mx[MX_STAGE * P + oc] = dmaxT(mx[MX_STAGE * P + oc], s0);
                     }
-                } else {
-                    mx[MX_STAGE * P + o] = dmaxT(mx[MX_STAGE * P + o], sn0);
                 }
-            }
-            if (m & SWH_MAXQ_MAX_SPEED) {
-                const T sE = (i + 1 < a.nx) ? sn1 : sn0, eE = (i + 1 < a.nx) ? r.e1 : r.e0;
-                const T sN = has_up ? su0 : sn0, eN = has_up ? q.e0 : r.e0;
-                T dE, dN;
-                if (TL) {
-                    dE = HALF * ((msl - r.e0) + (msl - eE));
-                    dN = HALF * ((msl - r.e0) + (msl - eN));
-                } else {
-                    dE = HALF * ((sn0 - r.e0) + (sE - eE));
-                    dN = HALF * ((sn0 - r.e0) + (sN - eN));
+                if (m & SWH_MAXQ_MAX_SPEED) {
+                    const bool rt = (i + c + 1) < a.nx;
+                    const T sE = rt ? sn[c + 1] : s0, eE = rt ? r.e[c + 1] : e0;
+                    const T sN = has_up ? su[c] : s0, eN = has_up ? q.e[c] : e0;
+                    T dE, dN;
+                    if (TL) {
+                        dE = HALF * ((msl - e0) + (msl - eE));
+                        dN = HALF * ((msl - e0) + (msl - eN));
+                    } else {
+                        dE = HALF * ((s0 - e0) + (sE - eE));
+                        dN = HALF * ((s0 - e0) + (sN - eN));
+                    }
+                    const T iE = dE > MIN ? T(1) / dE : T(0);
+                    const T iN = dN > MIN ? T(1) / dN : T(0);
+                    const T sp = sqrt(uhn * uhn * (iE * iE) + vhn * vhn * (iN * iN));
+                    mx[MX_SPEED * P + oc] = dmaxT(mx[MX_SPEED * P + oc], sp);
                 }
-                const T iE = dE > MIN ? T(1) / dE : T(0);
-                const T iN = dN > MIN ? T(1) / dN : T(0);
-                const T sp = sqrt(uh_new * uh_new * (iE * iE) + vh_new * vh_new * (iN * iN));
-                mx[MX_SPEED * P + o] = dmaxT(mx[MX_SPEED * P + o], sp);
+                if (m & SWH_MAXQ_MAX_FLUX)
+                    mx[MX_FLUX * P + oc] = dmaxT(mx[MX_FLUX * P + oc], sqrt(uhn * uhn + vhn * vhn));
+                if (m & SWH_MAXQ_ARRIVAL_TIME) {
+                    if ((s0 - e0) > MIN && s0 > a.arrival_stage && mx[MX_ARRIVAL * P + oc] < T(0))
+                        mx[MX_ARRIVAL * P + oc] = a.t_after;
+                }
+                if (m & SWH_MAXQ_MIN_STAGE) mx[MX_MIN_STAGE * P + oc] = dminT(mx[MX_MIN_STAGE * P + oc], s0);
             }
-            if (m & SWH_MAXQ_MAX_FLUX)
-                mx[MX_FLUX * P + o] = dmaxT(mx[MX_FLUX * P + o], sqrt(uh_new * uh_new + vh_new * vh_new));
-            if (m & SWH_MAXQ_ARRIVAL_TIME) {
-                if ((sn0 - r.e0) > MIN && sn0 > a.arrival_stage && mx[MX_ARRIVAL * P + o] < T(0))
-                    mx[MX_ARRIVAL * P + o] = a.t_after;
-            }
-            if (m & SWH_MAXQ_MIN_STAGE) mx[MX_MIN_STAGE * P + o] = dminT(mx[MX_MIN_STAGE * P + o], sn0);
         }
 
         // slide the window up one row
-        vhm0 = r.vh0;
-        vhm1 = r.vh1;
+#pragma unroll
+        for (int c = 0; c <= VEC; c++) {
+            vhm[c] = r.vh[c];
+            sn[c] = su[c];
+            cn[c] = cu[c];
+        }
         r = q;
-        sn0 = su0; cn0 = cu0; sn1 = su1; cn1 = cu1;
+        if (PF) q = p;
     }
 }
 
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+template <typename T, int VEC, bool PF, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 static hipError_t launch_lf(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)
 {
     dim3 block(block_x);
-    dim3 grid((a.nx + block_x - 1) / block_x, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
-    hipLaunchKernelGGL((leapfrog_kernel<T, KAHAN, COR, FRIC, TL, LINF>), grid, block, 0, s, a);
+    const int lanes = (a.nx + VEC - 1) / VEC;
+    dim3 grid((lanes + block_x - 1) / block_x, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
+    hipLaunchKernelGGL((leapfrog_kernel<T, VEC, PF, KAHAN, COR, FRIC, TL, LINF>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
-template <typename T, bool KAHAN, bool COR, int FRIC>
+template <typename T, int VEC, bool PF, bool KAHAN, bool COR, int FRIC>
 static hipError_t dispatch_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int bx, hipStream_t s)
 {
     if (tl) {
-        return linf ? launch_lf<T, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s)
-                    : launch_lf<T, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s);
+        return linf ? launch_lf<T, VEC, PF, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s)
+                    : launch_lf<T, VEC, PF, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s);
     }
-    return linf ? launch_lf<T, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s)
-                : launch_lf<T, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s);
+    return linf ? launch_lf<T, VEC, PF, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s)
+                : launch_lf<T, VEC, PF, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s);
 }
 
+template <typename T, int VEC, bool PF, bool KAHAN>
+static hipError_t dispatch_shape(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear, int linf,
+                                 int nsrc, int block_x, hipStream_t s)
+{
+    if (coriolis) {
+        switch (friction_mode) {
+        case 0: return dispatch_tl<T, VEC, PF, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s);
+        case 1: return dispatch_tl<T, VEC, PF, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s);
+        default: return dispatch_tl<T, VEC, PF, KAHAN, true, 2>(a, truely_linear, linf, nsrc, block_x, s);
+        }
+    }
+    switch (friction_mode) {
+    case 0: return dispatch_tl<T, VEC, PF, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s);
+    case 1: return dispatch_tl<T, VEC, PF, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s);
+    default: return dispatch_tl<T, VEC, PF, KAHAN, false, 2>(a, truely_linear, linf, nsrc, block_x, s);
+    }
+}
+
+// vec_mode: 0 = one column per lane, 1 = 16-byte vectors (4 floats / 2 doubles);
+// prefetch: keep the next row's loads in flight.
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                           int linear_friction, int nsrc, int block_x, hipStream_t s)
+                           int linear_friction, int nsrc, int block_x, int vec_mode, int prefetch, hipStream_t s)
 {
     // Linear drag is folded into the friction multiplier when friction is on
     // (include.f90:486-503 vs :550-557).
     const int linf = (friction_mode == 0) && linear_friction;
-    if (coriolis) {
-        switch (friction_mode) {
-        case 0: return dispatch_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s);
-        case 1: return dispatch_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s);
-        default: return dispatch_tl<T, KAHAN, true, 2>(a, truely_linear, linf, nsrc, block_x, s);
-        }
+    constexpr int V = sizeof(T) == 4 ? 4 : 2;
+    if (vec_mode) {
+        return prefetch ? dispatch_shape<T, V, true, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
+                                                            block_x, s)
+                        : dispatch_shape<T, V, false, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
+                                                             block_x, s);
     }
-    switch (friction_mode) {
-    case 0: return dispatch_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s);
-    case 1: return dispatch_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s);
-    default: return dispatch_tl<T, KAHAN, false, 2>(a, truely_linear, linf, nsrc, block_x, s);
-    }
+    return prefetch ? dispatch_shape<T, 1, true, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
+                                                        block_x, s)
+                    : dispatch_shape<T, 1, false, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
+                                                         block_x, s);
 }
 
-template hipError_t launch_leapfrog<double, false>(const LFArgs<double> &, int, int, int, int, int, int,
+template hipError_t launch_leapfrog<double, false>(const LFArgs<double> &, int, int, int, int, int, int, int, int,
                                                     hipStream_t);
-template hipError_t launch_leapfrog<float, true>(const LFArgs<float> &, int, int, int, int, int, int,
+template hipError_t launch_leapfrog<float, true>(const LFArgs<float> &, int, int, int, int, int, int, int, int,
                                                   hipStream_t);
 
 }  // namespace swh

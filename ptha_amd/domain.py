@@ -177,6 +177,11 @@ class Domain:
     def synchronize(self):
         B.check(self._lib.swh_synchronize(self._h), "swh_synchronize")
 
+    def set_launch(self, block_x=256, rows_per_block=32, vec=1, prefetch=1):
+        """Launch shape of the leapfrog step kernel (results do not depend on it)."""
+        B.check(self._lib.swh_domain_set_launch(self._h, int(block_x), int(rows_per_block), int(vec), int(prefetch)),
+                "swh_domain_set_launch")
+
     # -- diagnostics --
     def scalars(self):
         s = B.Scalars()

@@ -201,3 +201,29 @@ def test_rk2_lake_at_rest():
     assert np.abs(U[2][wet]).max() < 1e-10
     assert np.abs(U[0] - cfg["U"][0])[wet].max() < 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("shape", [(256, 32, 0, 0), (64, 7, 1, 1), (128, 64, 1, 0), (192, 5, 0, 1)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_launch_shapes_do_not_change_results(oracle_built, shape, precision):
+    """Every launch shape of the step kernel (lanes per block, rows per strip, vector
+    lanes, row prefetch) gives the same bits; odd sizes exercise partial vectors and
+    partial strips. FP64 is also bit-identical to the oracle."""
+    _print_case("shape %s %s" % (shape, precision))
+    cfg = H.small_spherical(nx=91, ny=77)
+    cfg = dict(cfg, method="leapfrog_linear_plus_nonlinear_friction",
+               manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.03 ** 2))
+    o, dt, _ = H.cpu_run("oracle", cfg, 40)
+    ref = H.gpu_domain(cfg, precision=precision)
+    ref.set_launch(256, 32, 0, 0)
+    ref.evolve_steps(dt, 40)
+    g = H.gpu_domain(cfg, precision=precision)
+    g.set_launch(*shape)
+    g.evolve_steps(dt, 40)
+    assert np.array_equal(g.U, ref.U)
+    for q in MAXQ:
+        assert np.array_equal(g.max_U(q), ref.max_U(q)), q
+    if precision == "fp64":
+        _cmp_exact(g, o)
+    g.close()
+    ref.close()
