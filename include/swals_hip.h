@@ -150,7 +150,7 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 int swh_synchronize(swh_domain *h);
 
 /* Launch shape of the leapfrog step kernel (see swh_batch_set_launch). */
-int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t vec, int32_t prefetch);
+int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t variant);
 
 /* ---- Batched unit sources on one shared grid ----
  * PTHA unit-source production runs one serial 'linear' model per source over a shared
@@ -183,9 +183,11 @@ int swh_batch_gauges_download(swh_batch *b, int32_t k, double *values, int32_t *
 /* Steps taken so far by the batch, and number of sources still active. */
 int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active);
 /* Launch shape of the leapfrog step kernel: threads per block (64..256, multiple of
- * 64), rows per thread strip, vec = 1 for 16-byte vector lanes (4 floats / 2 doubles
- * per lane), prefetch = 1 to keep the next row's loads in flight. */
-int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t vec, int32_t prefetch);
+ * 64), rows per thread strip, and variant bits: 1 = 16-byte vector lanes (4 floats /
+ * 2 doubles per lane), 2 = keep the next row's loads in flight, 4 = non-temporal
+ * state traffic, 8 = (batches) consecutive workgroups walk the sources of one tile.
+ * Results do not depend on the launch shape. */
+int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t variant);
 int swh_batch_synchronize(swh_batch *b);
 /* swh_batch_evolve bracketed by HIP events recorded on the batch's stream; returns the
  * device time of the region in milliseconds (for throughput measurement). */

@@ -78,8 +78,8 @@ SYMBOLS = [
     ("swh_batch_evolve", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_batch_gauges_download", ctypes.c_int, [ctypes.c_void_p, _I32, _PD, _PI32]),
     ("swh_batch_status", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _PI32]),
-    ("swh_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32, _I32]),
-    ("swh_domain_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32, _I32]),
+    ("swh_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
+    ("swh_domain_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
     ("swh_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_batch_evolve_timed", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PD]),
 ]

@@ -121,8 +121,10 @@ class UnitSourceBatch:
                 "swh_batch_gauges_download")
         return vals[:3 * self._ng * ns.value].reshape(ns.value, 3, self._ng).copy()
 
-    def set_launch(self, block_x, rows_per_block, vec=1, prefetch=1):
-        B.check(self._lib.swh_batch_set_launch(self._h, int(block_x), int(rows_per_block), int(vec), int(prefetch)),
+    def set_launch(self, block_x, rows_per_block, variant=2):
+        """Step-kernel launch shape; variant bits 1 vector lanes, 2 row prefetch,
+        4 non-temporal state traffic, 8 sources-fastest block order."""
+        B.check(self._lib.swh_batch_set_launch(self._h, int(block_x), int(rows_per_block), int(variant)),
                 "swh_batch_set_launch")
 
     def evolve(self, dt, nsteps):

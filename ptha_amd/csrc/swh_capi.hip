@@ -71,7 +71,7 @@ struct swh_domain {
     long long *d_gidx = nullptr;
     double *d_gval = nullptr, *d_gtime = nullptr;
     // launch shape of the step kernel
-    int block_x = 256, rows_per_block = 32, vec = 1, prefetch = 1;
+    int block_x = 256, rows_per_block = 32, variant = 2;
 
     bool leapfrog() const { return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF; }
     size_t plane_elems() const { return (size_t)pitch * ny; }
@@ -237,6 +237,7 @@ int leapfrog_step(swh_domain *h, double dt)
     a.bc_mask = h->bc_mask;
     a.bc_stage = (T)(h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period));
     a.rows_per_block = h->rows_per_block;
+    a.src_fast = 0;
     const bool upd = h->maxq_mask && ((h->nsteps % h->maxq_freq) == 0);
     a.maxq_mask = upd ? h->maxq_mask : 0;
     a.mx = (T *)h->mx;
@@ -246,7 +247,7 @@ int leapfrog_step(swh_domain *h, double dt)
     if (h->method == SWH_METHOD_LLPNF) fric_mode = h->truely_linear ? 1 : 2;
     if (fric_mode == 1 && !h->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload U first");
     SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
-                                      h->vec, h->prefetch, h->stream));
+                                      h->variant, h->stream));
     h->cur = o;
     h->time = t_after;
     h->dt_last_update = dt;
@@ -711,7 +712,7 @@ struct swh_batch {
     int ng = 0, every = 1, gmax = 0;
     long long *d_gidx = nullptr;
     double *d_gval = nullptr;      // [slot][sample][3][ng]
-    int block_x = 256, rows_per_block = 32, vec = 1, prefetch = 1;
+    int block_x = 256, rows_per_block = 32, variant = 2, src_fast = 0;
 };
 
 namespace {
@@ -779,6 +780,7 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
     a.fric_power = (T)((g->friction_type == 1) ? -2.0 : -7.0 / 3.0);
     a.bc_mask = 0; a.bc_stage = T(0);
     a.rows_per_block = b->rows_per_block;
+    a.src_fast = b->src_fast;
     a.maxq_mask = 0; a.mx = nullptr; a.t_after = T(0); a.arrival_stage = T(0);
     if (b->step == 0) {
         int rc = batch_sample<T, KAHAN>(b, batch_active(b, -1), b->cur0);
@@ -791,7 +793,7 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
         a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
         SWH_HIP(launch_leapfrog<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n_active,
-                                          b->block_x, b->vec, b->prefetch, g->stream));
+                                          b->block_x, b->variant, g->stream));
         b->step++;
         if (b->ng && (b->step % b->every) == 0) {
             int rc = batch_sample<T, KAHAN>(b, n_active, o);
@@ -976,29 +978,29 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active)
     return SWH_OK;
 }
 
-static int check_launch(int32_t block_x, int32_t rows_per_block, int32_t vec, int32_t prefetch)
+static int check_launch(int32_t block_x, int32_t rows_per_block, int32_t variant)
 {
-    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || vec < 0 || vec > 1 ||
-        prefetch < 0 || prefetch > 1)
-        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, vec and prefetch 0/1");
+    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 15)
+        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..15");
     return SWH_OK;
 }
 
-int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t vec, int32_t prefetch)
+int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t variant)
 {
     if (!b) return fail(SWH_EINVAL, "null argument");
-    int rc = check_launch(block_x, rows_per_block, vec, prefetch);
+    int rc = check_launch(block_x, rows_per_block, variant);
     if (rc) return rc;
-    b->block_x = block_x; b->rows_per_block = rows_per_block; b->vec = vec; b->prefetch = prefetch;
+    b->block_x = block_x; b->rows_per_block = rows_per_block;
+    b->variant = variant & 7; b->src_fast = (variant >> 3) & 1;
     return SWH_OK;
 }
 
-int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t vec, int32_t prefetch)
+int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t variant)
 {
     if (!h) return fail(SWH_EINVAL, "null argument");
-    int rc = check_launch(block_x, rows_per_block, vec, prefetch);
+    int rc = check_launch(block_x, rows_per_block, variant);
     if (rc) return rc;
-    h->block_x = block_x; h->rows_per_block = rows_per_block; h->vec = vec; h->prefetch = prefetch;
+    h->block_x = block_x; h->rows_per_block = rows_per_block; h->variant = variant & 7;
     return SWH_OK;
 }
 

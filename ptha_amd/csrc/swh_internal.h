@@ -70,6 +70,7 @@ struct LFArgs {
     int bc_mask;               // bit 0 N, 1 E, 2 S, 3 W
     T bc_stage;
     int rows_per_block;
+    int src_fast;              // batched runs: block x index walks the sources
     // fused max_U update (mask of SWH_MAXQ bits; 0 = none this step)
     int maxq_mask;
     T *mx;                     // N_MAXQ planes of pitch*ny
@@ -79,7 +80,7 @@ struct LFArgs {
 // Kernel launchers (defined in the .hip files)
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                           int linear_friction, int nsrc, int block_x, int vec_mode, int prefetch, hipStream_t s);
+                           int linear_friction, int nsrc, int block_x, int variant, hipStream_t s);
 
 template <typename T>
 hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext,

@@ -23,38 +23,56 @@ namespace swh {
 
 // Vector types for VEC consecutive columns per lane (16-byte loads and stores).
 template <typename T, int VEC>
-struct VecOf;
-template <>
-struct VecOf<float, 4> { typedef float4 type; };
-template <>
-struct VecOf<float, 2> { typedef float2 type; };
-template <>
-struct VecOf<double, 2> { typedef double2 type; };
+struct VecOf {
+    typedef T type __attribute__((ext_vector_type(VEC)));
+};
 template <typename T>
-struct VecOf<T, 1> { typedef T type; };
+struct VecOf<T, 1> {
+    typedef T type;
+};
 
-template <typename T, int VEC>
+// NT: non-temporal (streaming) access for the per-source state, which is far larger
+// than the caches; elevation and coefficients use ordinary loads.
+template <typename T, int VEC, bool NT>
 __device__ __forceinline__ void vload(const T *__restrict__ p, T *out)
 {
     typedef typename VecOf<T, VEC>::type V;
-    const V v = *reinterpret_cast<const V *>(p);
-    const T *e = reinterpret_cast<const T *>(&v);
+    V v;
+    if (NT) v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+    else v = *reinterpret_cast<const V *>(p);
+    if constexpr (VEC == 1) {
+        out[0] = v;
+    } else {
 #pragma unroll
-    for (int c = 0; c < VEC; c++) out[c] = e[c];
+        for (int c = 0; c < VEC; c++) out[c] = v[c];
+    }
 }
 
-template <typename T, int VEC>
+template <typename T, bool NT>
+__device__ __forceinline__ T sload(const T *__restrict__ p)
+{
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+template <typename T, int VEC, bool NT>
 __device__ __forceinline__ void vstore(T *__restrict__ p, const T *in, int nvalid)
 {
     typedef typename VecOf<T, VEC>::type V;
     if (nvalid >= VEC) {
         V v;
-        T *e = reinterpret_cast<T *>(&v);
+        if constexpr (VEC == 1) {
+            v = in[0];
+        } else {
 #pragma unroll
-        for (int c = 0; c < VEC; c++) e[c] = in[c];
-        *reinterpret_cast<V *>(p) = v;
+            for (int c = 0; c < VEC; c++) v[c] = in[c];
+        }
+        if (NT) __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+        else *reinterpret_cast<V *>(p) = v;
     } else {
-        for (int c = 0; c < nvalid; c++) p[c] = in[c];
+#pragma unroll
+        for (int c = 0; c < VEC; c++)
+            if (c < nvalid) p[c] = in[c];
     }
 }
 
@@ -66,7 +84,7 @@ struct VRow {
     T uh[VEC + 2];
 };
 
-template <typename T, int VEC, bool KAHAN>
+template <typename T, int VEC, bool KAHAN, bool NT>
 __device__ __forceinline__ void load_vrow(const LFArgs<T> &a, const T *__restrict__ S, const T *__restrict__ C,
                                           const T *__restrict__ UH, const T *__restrict__ VH, int i, int j,
                                           VRow<T, VEC> &r)
@@ -74,23 +92,23 @@ __device__ __forceinline__ void load_vrow(const LFArgs<T> &a, const T *__restric
     // i is a multiple of VEC and the row pitch a multiple of 64, so the vector stays
     // inside the (padded) row; the right-hand neighbour is read only when it is a cell.
     const long long o = (long long)j * a.pitch + i;
-    vload<T, VEC>(S + o, r.s);
-    vload<T, VEC>(UH + o, r.uh + 1);
-    vload<T, VEC>(VH + o, r.vh);
-    vload<T, VEC>(a.E + o, r.e);
-    if (KAHAN) vload<T, VEC>(C + o, r.c);
+    vload<T, VEC, NT>(S + o, r.s);
+    vload<T, VEC, NT>(UH + o, r.uh + 1);
+    vload<T, VEC, NT>(VH + o, r.vh);
+    vload<T, VEC, false>(a.E + o, r.e);
+    if (KAHAN) vload<T, VEC, NT>(C + o, r.c);
     const bool right = (i + VEC) < a.nx;
-    r.s[VEC] = right ? S[o + VEC] : T(0);
-    r.uh[VEC + 1] = right ? UH[o + VEC] : T(0);
-    r.vh[VEC] = right ? VH[o + VEC] : T(0);
+    r.s[VEC] = right ? sload<T, NT>(S + o + VEC) : T(0);
+    r.uh[VEC + 1] = right ? sload<T, NT>(UH + o + VEC) : T(0);
+    r.vh[VEC] = right ? sload<T, NT>(VH + o + VEC) : T(0);
     r.e[VEC] = right ? a.E[o + VEC] : T(0);
-    if (KAHAN) r.c[VEC] = right ? C[o + VEC] : T(0);
+    if (KAHAN) r.c[VEC] = right ? sload<T, NT>(C + o + VEC) : T(0);
     else {
 #pragma unroll
         for (int c = 0; c <= VEC; c++) r.c[c] = T(0);
     }
     // UH at column i-1, clamped to column 0 at the western edge (include.f90:319-322)
-    r.uh[0] = (i > 0) ? UH[o - 1] : r.uh[1];
+    r.uh[0] = (i > 0) ? sload<T, NT>(UH + o - 1) : r.uh[1];
 }
 
 // New stage of cell (ii, jj): interior cells get the flux divergence
@@ -144,15 +162,19 @@ __device__ __forceinline__ T dminT(T x, T y) { return x < y ? x : y; }
 
 // The fused step. Each lane owns VEC adjacent columns and marches up a strip of
 // rows_per_block rows; PF keeps one more row of loads in flight (register prefetch).
-template <typename T, int VEC, bool PF, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+template <typename T, int VEC, bool PF, bool NT, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
 {
-    const int i = (blockIdx.x * blockDim.x + threadIdx.x) * VEC;
+    // Block order: with src_fast, consecutive workgroups are the same tile of
+    // consecutive sources (they share the elevation tile in L2).
+    const int bsrc = a.src_fast ? blockIdx.x : blockIdx.z;
+    const int bxt = a.src_fast ? blockIdx.z : blockIdx.x;
+    const int i = (bxt * blockDim.x + threadIdx.x) * VEC;
     if (i >= a.nx) return;
     const int j0 = blockIdx.y * a.rows_per_block;
     if (j0 >= a.ny) return;
     const int j1 = min(j0 + a.rows_per_block, a.ny);
-    const long long so = (long long)blockIdx.z * a.src_stride;
+    const long long so = (long long)bsrc * a.src_stride;
     const T *__restrict__ S = a.S + so;
     const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
     const T *__restrict__ UH = a.UH + so;
@@ -167,14 +189,14 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
     const int nvalid = min(VEC, a.nx - i);
 
     VRow<T, VEC> r, q, p;
-    load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j0, r);
-    if (j0 + 1 < a.ny) load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j0 + 1, q);
+    load_vrow<T, VEC, KAHAN, NT>(a, S, C, UH, VH, i, j0, r);
+    if (j0 + 1 < a.ny) load_vrow<T, VEC, KAHAN, NT>(a, S, C, UH, VH, i, j0 + 1, q);
     // VH one row below the strip (stage at j0, and the Coriolis 'j-1/2' average,
     // which the reference clamps to row 1 at the southern edge: include.f90:295-308)
     T vhm[VEC + 1];
     if (j0 > 0) {
         const long long o = (long long)(j0 - 1) * a.pitch + i;
-        vload<T, VEC>(VH + o, vhm);
+        vload<T, VEC, NT>(VH + o, vhm);
         vhm[VEC] = (i + VEC < a.nx) ? VH[o + VEC] : T(0);
     } else {
 #pragma unroll
@@ -185,8 +207,8 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
 
     for (int j = j0; j < j1; j++) {
         const bool has_up = (j + 1) < a.ny;
-        if (PF && j + 2 < a.ny) load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j + 2, p);
-        if (!PF && has_up && j > j0) load_vrow<T, VEC, KAHAN>(a, S, C, UH, VH, i, j + 1, q);
+        if (PF && j + 2 < a.ny) load_vrow<T, VEC, KAHAN, NT>(a, S, C, UH, VH, i, j + 2, p);
+        if (!PF && has_up && j > j0) load_vrow<T, VEC, KAHAN, NT>(a, S, C, UH, VH, i, j + 1, q);
         T su[VEC + 1], cu[VEC + 1];
         if (has_up) {
             stage_row<T, VEC, KAHAN>(a, i, j + 1, q, r.vh, su, cu);
@@ -270,10 +292,10 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
             vh_out[c] = vh_new;
         }
         const long long o = (long long)j * a.pitch + i;
-        vstore<T, VEC>(So + o, sn, nvalid);
-        if (KAHAN) vstore<T, VEC>(Co + o, cn, nvalid);
-        vstore<T, VEC>(UHo + o, uh_out, nvalid);
-        vstore<T, VEC>(VHo + o, vh_out, nvalid);
+        vstore<T, VEC, NT>(So + o, sn, nvalid);
+        if (KAHAN) vstore<T, VEC, NT>(Co + o, cn, nvalid);
+        vstore<T, VEC, NT>(UHo + o, uh_out, nvalid);
+        vstore<T, VEC, NT>(VHo + o, vh_out, nvalid);
 
         if (a.maxq_mask) {
             // update_max_quantities on the new state (domain_mod.f90:2243-2404),
@@ -281,7 +303,9 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
             const long long P = (long long)a.pitch * a.ny;
             T *__restrict__ mx = a.mx;
             const int m = a.maxq_mask;
-            for (int c = 0; c < nvalid; c++) {
+#pragma unroll
+            for (int c = 0; c < VEC; c++) {
+                if (c >= nvalid) continue;
                 const long long oc = o + c;
                 const T s0 = sn[c], e0 = r.e[c], uhn = uh_out[c], vhn = vh_out[c];
                 if (m & SWH_MAXQ_MAX_STAGE) {
@@ -333,70 +357,74 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
     }
 }
 
-template <typename T, int VEC, bool PF, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+template <typename T, int VEC, bool PF, bool NT, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 static hipError_t launch_lf(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)
 {
     dim3 block(block_x);
     const int lanes = (a.nx + VEC - 1) / VEC;
-    dim3 grid((lanes + block_x - 1) / block_x, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
-    hipLaunchKernelGGL((leapfrog_kernel<T, VEC, PF, KAHAN, COR, FRIC, TL, LINF>), grid, block, 0, s, a);
+    const int gx = (lanes + block_x - 1) / block_x, gy = (a.ny + a.rows_per_block - 1) / a.rows_per_block;
+    dim3 grid = a.src_fast ? dim3(nsrc, gy, gx) : dim3(gx, gy, nsrc);
+    hipLaunchKernelGGL((leapfrog_kernel<T, VEC, PF, NT, KAHAN, COR, FRIC, TL, LINF>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
-template <typename T, int VEC, bool PF, bool KAHAN, bool COR, int FRIC>
+template <typename T, int VEC, bool PF, bool NT, bool KAHAN, bool COR, int FRIC>
 static hipError_t dispatch_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int bx, hipStream_t s)
 {
     if (tl) {
-        return linf ? launch_lf<T, VEC, PF, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s)
-                    : launch_lf<T, VEC, PF, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s);
+        return linf ? launch_lf<T, VEC, PF, NT, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s)
+                    : launch_lf<T, VEC, PF, NT, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s);
     }
-    return linf ? launch_lf<T, VEC, PF, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s)
-                : launch_lf<T, VEC, PF, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s);
+    return linf ? launch_lf<T, VEC, PF, NT, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s)
+                : launch_lf<T, VEC, PF, NT, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s);
 }
 
-template <typename T, int VEC, bool PF, bool KAHAN>
+template <typename T, int VEC, bool PF, bool NT, bool KAHAN>
 static hipError_t dispatch_shape(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear, int linf,
                                  int nsrc, int block_x, hipStream_t s)
 {
     if (coriolis) {
         switch (friction_mode) {
-        case 0: return dispatch_tl<T, VEC, PF, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s);
-        case 1: return dispatch_tl<T, VEC, PF, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s);
-        default: return dispatch_tl<T, VEC, PF, KAHAN, true, 2>(a, truely_linear, linf, nsrc, block_x, s);
+        case 0: return dispatch_tl<T, VEC, PF, NT, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s);
+        case 1: return dispatch_tl<T, VEC, PF, NT, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s);
+        default: return dispatch_tl<T, VEC, PF, NT, KAHAN, true, 2>(a, truely_linear, linf, nsrc, block_x, s);
         }
     }
     switch (friction_mode) {
-    case 0: return dispatch_tl<T, VEC, PF, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s);
-    case 1: return dispatch_tl<T, VEC, PF, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s);
-    default: return dispatch_tl<T, VEC, PF, KAHAN, false, 2>(a, truely_linear, linf, nsrc, block_x, s);
+    case 0: return dispatch_tl<T, VEC, PF, NT, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s);
+    case 1: return dispatch_tl<T, VEC, PF, NT, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s);
+    default: return dispatch_tl<T, VEC, PF, NT, KAHAN, false, 2>(a, truely_linear, linf, nsrc, block_x, s);
     }
 }
 
-// vec_mode: 0 = one column per lane, 1 = 16-byte vectors (4 floats / 2 doubles);
-// prefetch: keep the next row's loads in flight.
+// variant bits: 1 = 16-byte vector lanes (4 floats / 2 doubles), 2 = next-row
+// prefetch, 4 = non-temporal state traffic.
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                           int linear_friction, int nsrc, int block_x, int vec_mode, int prefetch, hipStream_t s)
+                           int linear_friction, int nsrc, int block_x, int variant, hipStream_t s)
 {
     // Linear drag is folded into the friction multiplier when friction is on
     // (include.f90:486-503 vs :550-557).
     const int linf = (friction_mode == 0) && linear_friction;
     constexpr int V = sizeof(T) == 4 ? 4 : 2;
-    if (vec_mode) {
-        return prefetch ? dispatch_shape<T, V, true, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
-                                                            block_x, s)
-                        : dispatch_shape<T, V, false, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
-                                                             block_x, s);
+#define SWH_LF_CASE(VV, PF, NT)                                                                              \
+    return dispatch_shape<T, VV, PF, NT, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc, block_x, s)
+    switch (variant & 7) {
+    case 0: SWH_LF_CASE(1, false, false);
+    case 1: SWH_LF_CASE(V, false, false);
+    case 2: SWH_LF_CASE(1, true, false);
+    case 3: SWH_LF_CASE(V, true, false);
+    case 4: SWH_LF_CASE(1, false, true);
+    case 5: SWH_LF_CASE(V, false, true);
+    case 6: SWH_LF_CASE(1, true, true);
+    default: SWH_LF_CASE(V, true, true);
     }
-    return prefetch ? dispatch_shape<T, 1, true, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
-                                                        block_x, s)
-                    : dispatch_shape<T, 1, false, KAHAN>(a, coriolis, friction_mode, truely_linear, linf, nsrc,
-                                                         block_x, s);
+#undef SWH_LF_CASE
 }
 
-template hipError_t launch_leapfrog<double, false>(const LFArgs<double> &, int, int, int, int, int, int, int, int,
+template hipError_t launch_leapfrog<double, false>(const LFArgs<double> &, int, int, int, int, int, int, int,
                                                     hipStream_t);
-template hipError_t launch_leapfrog<float, true>(const LFArgs<float> &, int, int, int, int, int, int, int, int,
+template hipError_t launch_leapfrog<float, true>(const LFArgs<float> &, int, int, int, int, int, int, int,
                                                   hipStream_t);
 
 }  // namespace swh

@@ -177,9 +177,10 @@ class Domain:
     def synchronize(self):
         B.check(self._lib.swh_synchronize(self._h), "swh_synchronize")
 
-    def set_launch(self, block_x=256, rows_per_block=32, vec=1, prefetch=1):
-        """Launch shape of the leapfrog step kernel (results do not depend on it)."""
-        B.check(self._lib.swh_domain_set_launch(self._h, int(block_x), int(rows_per_block), int(vec), int(prefetch)),
+    def set_launch(self, block_x=256, rows_per_block=32, variant=2):
+        """Launch shape of the leapfrog step kernel (results do not depend on it);
+        variant bits: 1 vector lanes, 2 row prefetch, 4 non-temporal state traffic."""
+        B.check(self._lib.swh_domain_set_launch(self._h, int(block_x), int(rows_per_block), int(variant)),
                 "swh_domain_set_launch")
 
     # -- diagnostics --

@@ -1,0 +1,187 @@
+! ISO_C_BINDING interfaces to the MI355X time-stepping backend (include/swals_hip.h).
+!
+! This is the Fortran side of the drop-in boundary: SWALS's domain_type
+! (propagation/SWALS/src/shallow_water/domain_mod.f90) keeps its setup, I/O and API,
+! and its evolve_one_step hands the per-step work to these functions, following the
+! pattern of the reference's existing bind(C) layer
+! (propagation/SWALS/src/raster/read_raster_mod.f90:87-144). Arrays are passed with
+! c_loc of the reference's own contiguous allocatables: domain%U(nx, ny, 4),
+! domain%distance_bottom_edge, ... A non-zero status maps to generic_stop
+! (propagation/SWALS/src/util/stop_mod.f90:8-33); swh_check does that here.
+module swals_hip_mod
+    use iso_c_binding
+    implicit none
+    private
+
+    integer(c_int32_t), parameter, public :: SWH_METHOD_RK2 = 1, SWH_METHOD_LINEAR = 5, SWH_METHOD_LLPNF = 6
+    integer(c_int32_t), parameter, public :: SWH_PREC_FP64 = 0, SWH_PREC_FP32K = 1
+    integer(c_int32_t), parameter, public :: SWH_MAXQ_MAX_STAGE = 1, SWH_MAXQ_MAX_SPEED = 2, &
+        SWH_MAXQ_MAX_FLUX = 4, SWH_MAXQ_ARRIVAL_TIME = 8, SWH_MAXQ_TIME_OF_MAX_STAGE = 16, SWH_MAXQ_MIN_STAGE = 32
+
+    ! Mirrors struct swh_domain_desc field by field
+    type, bind(C), public :: swh_domain_desc
+        integer(c_int32_t) :: nx = 0, ny = 0
+        integer(c_int32_t) :: timestepping_method = SWH_METHOD_LINEAR
+        integer(c_int32_t) :: precision = SWH_PREC_FP64
+        integer(c_int32_t) :: linear_solver_is_truely_linear = 1
+        integer(c_int32_t) :: include_coriolis = 0
+        integer(c_int32_t) :: spherical = 0
+        integer(c_int32_t) :: friction_type = 0
+        integer(c_int32_t) :: exterior_cells_width = 2
+        integer(c_int32_t) :: boundary_exterior(4) = 1
+        integer(c_int32_t) :: max_U_mask = 0
+        integer(c_int32_t) :: max_U_update_frequency = 1
+        real(c_double) :: msl_linear = 0.0_c_double
+        real(c_double) :: cfl = -1.0_c_double
+        real(c_double) :: theta = -1.0_c_double
+        real(c_double) :: linear_friction_coeff = 0.0_c_double
+        real(c_double) :: maximum_timestep = -1.0_c_double
+        real(c_double) :: arrival_stage_threshold_above_msl_linear = 0.01_c_double
+        type(c_ptr) :: distance_bottom_edge = c_null_ptr
+        type(c_ptr) :: distance_left_edge = c_null_ptr
+        type(c_ptr) :: area_cell_y = c_null_ptr
+        type(c_ptr) :: coriolis = c_null_ptr
+        type(c_ptr) :: coriolis_bottom_edge = c_null_ptr
+        type(c_ptr) :: tanlat_on_radius_earth = c_null_ptr
+    end type
+
+    ! Mirrors struct swh_scalars
+    type, bind(C), public :: swh_scalars
+        real(c_double) :: time, max_dt, evolve_step_dt, dt_last_update
+        real(c_double) :: boundary_flux_store(4)
+        real(c_double) :: boundary_flux_evolve_integral, boundary_flux_time_integral
+        real(c_double) :: negative_depth_fix_counter
+        integer(c_int64_t) :: nsteps_advanced
+    end type
+
+    public :: swh_last_error, swh_domain_create, swh_domain_destroy, swh_upload_U, swh_download_U, &
+        swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_evolve_one_step, &
+        swh_evolve_steps, swh_get_scalars, swh_volume_interior, swh_stationary_timestep_max, &
+        swh_download_max_U, swh_gauges_set, swh_gauges_sample, swh_gauges_download, swh_synchronize, &
+        swh_domain_set_launch, swh_check
+
+    interface
+        integer(c_int) function swh_last_error(buf, len) bind(C, name='swh_last_error')
+            import c_int, c_char
+            character(kind=c_char), intent(out) :: buf(*)
+            integer(c_int), value :: len
+        end function
+        integer(c_int) function swh_domain_create(desc, h) bind(C, name='swh_domain_create')
+            import c_int, c_ptr, swh_domain_desc
+            type(swh_domain_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: h
+        end function
+        integer(c_int) function swh_domain_destroy(h) bind(C, name='swh_domain_destroy')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function swh_upload_U(h, U) bind(C, name='swh_upload_U')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: U(*)
+        end function
+        integer(c_int) function swh_download_U(h, U) bind(C, name='swh_download_U')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: U(*)
+        end function
+        integer(c_int) function swh_upload_manning_squared(h, m) bind(C, name='swh_upload_manning_squared')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: m(*)
+        end function
+        integer(c_int) function swh_set_time(h, t) bind(C, name='swh_set_time')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: t
+        end function
+        integer(c_int) function swh_set_stage_boundary_sine(h, amp, period, msl, sides) &
+                bind(C, name='swh_set_stage_boundary_sine')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            real(c_double), value :: amp, period, msl
+            integer(c_int32_t), intent(in) :: sides(4)
+        end function
+        integer(c_int) function swh_evolve_one_step(h, timestep) bind(C, name='swh_evolve_one_step')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: timestep
+        end function
+        integer(c_int) function swh_evolve_steps(h, timestep, nsteps) bind(C, name='swh_evolve_steps')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            real(c_double), value :: timestep
+            integer(c_int32_t), value :: nsteps
+        end function
+        integer(c_int) function swh_get_scalars(h, s) bind(C, name='swh_get_scalars')
+            import c_int, c_ptr, swh_scalars
+            type(c_ptr), value :: h
+            type(swh_scalars), intent(out) :: s
+        end function
+        integer(c_int) function swh_volume_interior(h, v) bind(C, name='swh_volume_interior')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: v
+        end function
+        integer(c_int) function swh_stationary_timestep_max(h, v) bind(C, name='swh_stationary_timestep_max')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: v
+        end function
+        integer(c_int) function swh_download_max_U(h, which, out) bind(C, name='swh_download_max_U')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: which
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function swh_gauges_set(h, n, ij, max_samples) bind(C, name='swh_gauges_set')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: n, max_samples
+            integer(c_int32_t), intent(in) :: ij(*)
+        end function
+        integer(c_int) function swh_gauges_sample(h) bind(C, name='swh_gauges_sample')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function swh_gauges_download(h, values, times, nsamples) bind(C, name='swh_gauges_download')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: values(*), times(*)
+            integer(c_int32_t), intent(out) :: nsamples
+        end function
+        integer(c_int) function swh_synchronize(h) bind(C, name='swh_synchronize')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function swh_domain_set_launch(h, block_x, rows_per_block, variant) &
+                bind(C, name='swh_domain_set_launch')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: block_x, rows_per_block, variant
+        end function
+    end interface
+
+contains
+
+    subroutine swh_check(status, what)
+        !! Stops with the backend's message on a non-zero status (the reference's
+        !! generic_stop behaviour: print, then stop).
+        integer(c_int), intent(in) :: status
+        character(len=*), intent(in) :: what
+        character(kind=c_char) :: buf(512)
+        character(len=512) :: msg
+        integer :: k, rc
+        if(status == 0) return
+        buf = c_null_char
+        rc = swh_last_error(buf, 512_c_int)
+        msg = ''
+        do k = 1, 512
+            if(buf(k) == c_null_char) exit
+            msg(k:k) = buf(k)
+        end do
+        write(*, *) 'swals_hip: ', trim(what), ' failed (status ', status, '): ', trim(msg)
+        error stop 1
+    end subroutine
+
+end module

@@ -203,7 +203,8 @@ def test_rk2_lake_at_rest():
     g.close()
 
 
-@pytest.mark.parametrize("shape", [(256, 32, 0, 0), (64, 7, 1, 1), (128, 64, 1, 0), (192, 5, 0, 1)])
+@pytest.mark.parametrize("shape", [(256, 32, 0), (64, 7, 3), (128, 64, 1), (192, 5, 2), (64, 9, 7), (256, 3, 6),
+                                   (128, 11, 5), (64, 16, 4)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32k"])
 def test_launch_shapes_do_not_change_results(oracle_built, shape, precision):
     """Every launch shape of the step kernel (lanes per block, rows per strip, vector
@@ -215,7 +216,7 @@ def test_launch_shapes_do_not_change_results(oracle_built, shape, precision):
                manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.03 ** 2))
     o, dt, _ = H.cpu_run("oracle", cfg, 40)
     ref = H.gpu_domain(cfg, precision=precision)
-    ref.set_launch(256, 32, 0, 0)
+    ref.set_launch(256, 32, 0)
     ref.evolve_steps(dt, 40)
     g = H.gpu_domain(cfg, precision=precision)
     g.set_launch(*shape)
