@@ -149,6 +149,25 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 
 int swh_synchronize(swh_domain *h);
 
+/* Forward run with gauges: samples the gauges set by swh_gauges_set now and after
+ * every 'every' steps of nsteps (nsteps/every + 1 samples in all). */
+int swh_evolve_record_gauges(swh_domain *h, double timestep, int32_t nsteps, int32_t every);
+
+/* ---- Adjoint of the 'linear' (truely-linear) leapfrog step ----
+ * No counterpart in the reference (which has no adjoint); it serves source inversion
+ * over unit sources. The adjoint state (lS, lU, lV) lives on the device; a step
+ * applies the exact transpose of one forward step of size dt (walls, no stage
+ * boundary). swh_adjoint_run performs the backward sweep of a forward
+ * swh_evolve_record_gauges(h, dt, nsteps, every): grad[(sample * ngauges) + g] is
+ * d(loss)/d(stage at gauge g in that sample); afterwards lS is d(loss)/d(initial
+ * stage) and lU, lV the gradients with respect to the initial UH, VH. */
+int swh_adjoint_reset(swh_domain *h);
+int swh_adjoint_upload(swh_domain *h, const double *lambda3);   /* 3 planes (nx, ny): lS, lU, lV */
+int swh_adjoint_download(swh_domain *h, double *lambda3);
+int swh_adjoint_step(swh_domain *h, double dt, int32_t nsteps);
+int swh_adjoint_inject(swh_domain *h, const double *values);   /* lS[gauge g] += values[g] */
+int swh_adjoint_run(swh_domain *h, double dt, int32_t nsteps, int32_t every, const double *grad);
+
 /* Launch shape of the leapfrog step kernel (see swh_batch_set_launch). */
 int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t variant);
 

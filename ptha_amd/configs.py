@@ -166,3 +166,28 @@ def gauge_site_index(xy, ll, dx, nx, ny):
     i = 1 + np.minimum(np.floor((xy[:, 0] - ll[0]) / dx[0]), nx - 1)
     j = 1 + np.minimum(np.floor((xy[:, 1] - ll[1]) / dx[1]), ny - 1)
     return np.stack([i, j], axis=1).astype(np.int32)
+
+
+def adjoint_inversion(nx=2048, ny=2048, arcmin=1.0, ll=(140.0, -45.0), n_src_x=40, n_src_y=20, spacing_km=20.0,
+                      sigma_km=15.0, n_gauges=32, hours=4.0, gauge_interval=60.0, seed=20261202, noise=0.01):
+    """Source-inversion workload: 'linear' leapfrog on a spherical grid, initial stage a
+    weighted sum of Gaussian unit sources on a lattice (weights ~ N(0, 1), the hidden
+    truth), gauges on a ring around the source region, synthetic observations with 1%
+    multiplicative Gaussian noise added by the caller (see ``noise``)."""
+    lw = (nx * arcmin / 60.0, ny * arcmin / 60.0)
+    x, y = cell_centres(nx, ny, lw, ll)
+    elev = add_walls(power_law_bathymetry(nx, ny, seed + 7))
+    cx0, cy0 = ll[0] + 0.45 * lw[0], ll[1] + 0.5 * lw[1]
+    coslat = math.cos(math.radians(cy0))
+    dlon = spacing_km * 1e3 / (RADIUS_EARTH * coslat) * 180 / math.pi
+    dlat = spacing_km * 1e3 / RADIUS_EARTH * 180 / math.pi
+    lon = cx0 + (np.arange(n_src_x) - (n_src_x - 1) / 2) * dlon
+    lat = cy0 + (np.arange(n_src_y) - (n_src_y - 1) / 2) * dlat
+    rng = np.random.default_rng(seed)
+    w_true = rng.standard_normal((n_src_y, n_src_x))
+    ang = 2 * math.pi * (np.arange(n_gauges) + 0.5) / n_gauges
+    rad = 0.3 * min(lw)
+    gauges_xy = np.stack([cx0 + rad * np.cos(ang), cy0 + rad * np.sin(ang)], axis=1)
+    return dict(nx=nx, ny=ny, lw=lw, ll=ll, x=x, y=y, elev=elev, src_lon=lon, src_lat=lat, sigma_km=sigma_km,
+                w_true=w_true, gauges_xy=gauges_xy, hours=hours, gauge_interval=gauge_interval, noise=noise,
+                seed=seed)

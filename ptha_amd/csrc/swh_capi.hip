@@ -70,6 +70,10 @@ struct swh_domain {
     int ng = 0, gmax = 0, gsamples = 0;
     long long *d_gidx = nullptr;
     double *d_gval = nullptr, *d_gtime = nullptr;
+    // adjoint state (ping-pong lS, lU, lV), allocated on first use
+    void *adj[2][3] = {};
+    int adj_cur = 0;
+    double *d_gvec = nullptr;  // one sample of gauge values for injection
     // launch shape of the step kernel
     int block_x = 256, rows_per_block = 32, variant = 2;
 
@@ -460,7 +464,8 @@ int swh_domain_destroy(swh_domain *h)
     void *ptrs[] = {h->d_dbe, h->d_dle, h->d_area, h->d_cor, h->d_tanlat, h->S[0], h->S[1], h->UH[0], h->UH[1],
                     h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
-                    h->d_gval, h->d_gtime};
+                    h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
+                    h->adj[1][1], h->adj[1][2], h->d_gvec};
     for (void *p : ptrs) dfree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -687,6 +692,162 @@ int swh_synchronize(swh_domain *h)
 {
     if (!h) return fail(SWH_EINVAL, "null argument");
     SWH_HIP(hipStreamSynchronize(h->stream));
+    return SWH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint of the truely-linear leapfrog step (see swh_adjoint.hip)
+// ---------------------------------------------------------------------------
+namespace {
+
+int adj_check(swh_domain *h)
+{
+    if (h->method != SWH_METHOD_LINEAR || !h->truely_linear)
+        return fail(SWH_ESTATE, "the adjoint is provided for the truely-linear 'linear' method");
+    if (h->bc_mask) return fail(SWH_ESTATE, "the adjoint does not support a stage boundary");
+    if (!h->adj[0][0]) {
+        const size_t P = h->plane_elems() * h->esz;
+        for (int b = 0; b < 2; b++)
+            for (int k = 0; k < 3; k++) {
+                int rc = dmalloc(&h->adj[b][k], P);
+                if (rc) return rc;
+            }
+    }
+    return SWH_OK;
+}
+
+template <typename T>
+int adj_steps(swh_domain *h, double dt, int nsteps)
+{
+    if (h->rc_dt != dt) {
+        int rc = build_rc<T>(h, dt);
+        if (rc) return rc;
+    }
+    AdjArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch;
+    a.E = (const T *)h->E;
+    a.rc = rc_view<T>(h);
+    a.msl = (T)h->msl;
+    a.lin_fric_mult = (T)(1.0 / (1.0 + dt * h->lfc));
+    a.coriolis = h->coriolis;
+    for (int k = 0; k < nsteps; k++) {
+        const int c = h->adj_cur, o = 1 - c;
+        a.lS = (const T *)h->adj[c][0]; a.lU = (const T *)h->adj[c][1]; a.lV = (const T *)h->adj[c][2];
+        a.oS = (T *)h->adj[o][0]; a.oU = (T *)h->adj[o][1]; a.oV = (T *)h->adj[o][2];
+        SWH_HIP(launch_adjoint_step<T>(a, h->stream));
+        h->adj_cur = o;
+    }
+    return SWH_OK;
+}
+
+int adj_inject_host(swh_domain *h, const double *values)
+{
+    if (!h->d_gvec) {
+        int rc = dmalloc((void **)&h->d_gvec, sizeof(double) * (h->ng > 0 ? h->ng : 1));
+        if (rc) return rc;
+    }
+    SWH_HIP(hipMemcpyAsync(h->d_gvec, values, sizeof(double) * h->ng, hipMemcpyHostToDevice, h->stream));
+    // the host buffer must stay valid until the copy has been consumed
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    void *lS = h->adj[h->adj_cur][0];
+    if (h->prec == SWH_PREC_FP64) SWH_HIP(launch_adjoint_inject<double>((double *)lS, h->ng, h->d_gidx, h->d_gvec, h->stream));
+    else SWH_HIP(launch_adjoint_inject<float>((float *)lS, h->ng, h->d_gidx, h->d_gvec, h->stream));
+    return SWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swh_adjoint_reset(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    for (int b = 0; b < 2; b++)
+        for (int k = 0; k < 3; k++) SWH_HIP(hipMemset(h->adj[b][k], 0, h->plane_elems() * h->esz));
+    h->adj_cur = 0;
+    return SWH_OK;
+}
+
+int swh_adjoint_upload(swh_domain *h, const double *l3)
+{
+    if (!h || !l3) return fail(SWH_EINVAL, "null argument");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->nx * h->ny;
+    for (int k = 0; k < 3; k++)
+        if ((rc = upload_plane_any(h, h->adj[h->adj_cur][k], l3 + k * n))) return rc;
+    return SWH_OK;
+}
+
+int swh_adjoint_download(swh_domain *h, double *l3)
+{
+    if (!h || !l3) return fail(SWH_EINVAL, "null argument");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->nx * h->ny;
+    for (int k = 0; k < 3; k++) {
+        rc = h->prec == SWH_PREC_FP64 ? download_plane<double>(h, h->adj[h->adj_cur][k], nullptr, l3 + k * n)
+                                      : download_plane<float>(h, h->adj[h->adj_cur][k], nullptr, l3 + k * n);
+        if (rc) return rc;
+    }
+    return SWH_OK;
+}
+
+int swh_adjoint_step(swh_domain *h, double dt, int32_t nsteps)
+{
+    if (!h || !(dt > 0.0) || nsteps < 0) return fail(SWH_EINVAL, "bad adjoint step arguments");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    return h->prec == SWH_PREC_FP64 ? adj_steps<double>(h, dt, nsteps) : adj_steps<float>(h, dt, nsteps);
+}
+
+int swh_adjoint_inject(swh_domain *h, const double *values)
+{
+    if (!h || !values) return fail(SWH_EINVAL, "null argument");
+    if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    return adj_inject_host(h, values);
+}
+
+int swh_adjoint_run(swh_domain *h, double dt, int32_t nsteps, int32_t every, const double *grad)
+{
+    if (!h || !grad || !(dt > 0.0) || nsteps < 0 || every < 1) return fail(SWH_EINVAL, "bad adjoint run arguments");
+    if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    const int ng = h->ng;
+    // backward sweep: the sample taken after forward step n (n % every == 0) is
+    // injected before the transposed step n is applied
+    for (int n = nsteps; n >= 0; n--) {
+        if (n % every == 0) {
+            if ((rc = adj_inject_host(h, grad + (size_t)(n / every) * ng))) return rc;
+        }
+        if (n > 0) {
+            rc = h->prec == SWH_PREC_FP64 ? adj_steps<double>(h, dt, 1) : adj_steps<float>(h, dt, 1);
+            if (rc) return rc;
+        }
+    }
+    return SWH_OK;
+}
+
+int swh_evolve_record_gauges(swh_domain *h, double timestep, int32_t nsteps, int32_t every)
+{
+    if (!h || nsteps < 0 || every < 1) return fail(SWH_EINVAL, "bad arguments");
+    if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    int rc = swh_gauges_sample(h);
+    if (rc) return rc;
+    for (int n = 1; n <= nsteps; n++) {
+        if ((rc = evolve(h, timestep))) return rc;
+        if (n % every == 0 && (rc = swh_gauges_sample(h))) return rc;
+    }
     return SWH_OK;
 }
 

@@ -105,6 +105,23 @@ template <typename T>
 hipError_t launch_timestep_max(const T *S, const T *E, int nx, int ny, int pitch, int linear, double msl,
                                const double *dbe, const double *dle, double *out, hipStream_t s);
 
+// ---- adjoint of the truely-linear leapfrog step ----
+template <typename T>
+struct AdjArgs {
+    int nx, ny, pitch;
+    const T *lS, *lU, *lV;   // adjoints of the step's outputs
+    T *oS, *oU, *oV;         // adjoints of the step's inputs
+    const T *E;
+    LFRowCoef<T> rc;
+    T msl, lin_fric_mult;
+    int coriolis;
+};
+
+template <typename T>
+hipError_t launch_adjoint_step(const AdjArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_adjoint_inject(T *lS, int n, const long long *idx, const double *values, hipStream_t s);
+
 // ---- finite-volume (rk2) ----
 template <typename T>
 struct FVArgs {
