@@ -31,7 +31,6 @@ def _problem(nx=96, ny=80, nsx=4, nsy=2):
 
 
 def _eta(w, GX, GY, elev):
-    eta = np.einsum("ba,bn,ban->", w[:, :, None] * 0 + w[:, :, None], GY[:, :1] * 0 + 1, GX) if False else None
     eta = GY.T @ np.stack([w[b] @ GX[b] for b in range(w.shape[0])])
     return np.where(elev < 0, eta, 0.0)
 
