@@ -136,6 +136,29 @@ def read_traffic(name):
     return None
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: the reference point that scales
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes, and the achievable
+    streaming bandwidth of this GPU."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 10
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    log("copy probe: %d MiB read + %d MiB written per copy, %.3f ms, %.0f GB/s" % (mib, mib, ms,
+                                                                                2 * mib * (1 << 20) / ms / 1e6))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +169,11 @@ def main():
     ap.add_argument("--no-full", action="store_true", help="skip the whole-configuration unit-sources/hr leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--block", type=str, default="", help="block_x,rows_per_block of the step kernel")
+    ap.add_argument("--block", type=str, default="",
+                    help="block_x,rows_per_block,variant of the step kernel (default: the batch's autotuned shape)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timed legs, run a plain device copy of MIB MiB (for calibrating "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE) and log its bandwidth")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path; rerun with --gpus 1",
@@ -206,6 +233,9 @@ def main():
                     gcell_per_s=total_cell_updates / wall / 1e9, sources=nsrc,
                     gauge_samples_per_source=int(b.gauge_series(0).shape[0]))
         b.close()
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
 
     cpu = None
     if not args.no_cpu:

@@ -212,6 +212,72 @@ int swh_batch_synchronize(swh_batch *b);
  * device time of the region in milliseconds (for throughput measurement). */
 int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elapsed_ms);
 
+/* ---------------------------------------------------------------------------
+ * Multidomain nesting on one GPU (multidomain_mod.f90, nested_grid_comms_mod.f90).
+ *
+ * The host runs the reference's own md%setup (it decides halos, priority domains
+ * and the send/receive boxes); these calls hand the result to the device, which
+ * then does the whole multidomain step: every domain's sub-steps with flux-integral
+ * tracking, packing each send box directly into the receiving box's buffer (one
+ * GPU: the point-to-point copy is a device write), unpacking, the staggered-grid
+ * wet/dry fix-ups and flux correction. Domains step concurrently on their own
+ * streams; the exchange runs on the multidomain's stream.
+ * ------------------------------------------------------------------------- */
+typedef struct swh_md swh_md;
+
+/* One two_way_nesting_comms_type box (nested_grid_comms_mod.f90:69-219), as seen
+ * from the domain that owns it. Indices are 1-based as in the reference. */
+typedef struct swh_nest_box {
+    int32_t inds[6];                   /* send_inds or recv_inds: iL, iU, jL, jU, kL, kU */
+    int32_t neighbour_domain_index;    /* 1-based index of the other domain */
+    int32_t neighbour_comms_index;     /* 1-based index of the matching box in the other domain's
+                                          receive list (for a send box) or send list (for a receive box) */
+    int32_t my_domain_is_finer;
+    int32_t equal_cell_ratios;
+    int32_t my_domain_staggered_grid;
+    int32_t neighbour_domain_staggered_grid;
+    int32_t use_wetdry_limiting;
+    int32_t buffer_size;               /* size(send_buffer) / size(recv_buffer) */
+    int32_t flux_start;                /* send_flux_start / recv_flux_start (1-based) */
+    int32_t time_index;                /* send_timeind / recv_timeind (1-based) */
+    int32_t flux_rows[4];              /* size(box_flux_integral(d)%x, 1), d = N, S, E, W */
+    int32_t flux_nvar;                 /* size(box_flux_integral(d)%x, 2) */
+    double cell_ratios[2];             /* my dx / neighbour dx */
+} swh_nest_box;
+
+/* Starts a multidomain over existing domain handles (all of one precision).
+ * all_dx = md%all_dx_md(1:2, k, 1) for k = 1..n (2*n values), all_methods =
+ * md%all_timestepping_methods_md(k, 1), trf = timestepping_refinement_factor
+ * (multidomain_mod.f90:171-276, 1093-1212). */
+int swh_md_create(int32_t n_domains, swh_domain *const *domains, const double *all_dx, const int32_t *all_methods,
+                  const int32_t *timestepping_refinement_factor, swh_md **out);
+int swh_md_destroy(swh_md *md);
+/* domain%nesting%priority_domain_index and is_priority_domain_not_periodic (nx, ny),
+ * domain k (1-based) (nested_grid_comms_mod.f90 domain_nesting_type). */
+int swh_md_set_priority(swh_md *md, int32_t k, const int32_t *priority_domain_index,
+                        const int32_t *is_priority_domain_not_periodic);
+/* domain%flux_correct_dry_cells_outside_priority_domain (domain_mod.f90:283), default 1. */
+int swh_md_set_flux_correct_dry_cells(swh_md *md, int32_t k, int32_t on);
+/* Appends domain k's next send box (is_recv = 0) or receive box (is_recv = 1), in
+ * the order of domain%nesting%send_comms / recv_comms. */
+int swh_md_add_box(swh_md *md, int32_t k, int32_t is_recv, const swh_nest_box *box);
+/* Pairs the boxes, builds the device tables and the flux-correction work
+ * (domain_mod.f90:2766-3612), and allocates the buffers. */
+int swh_md_finalize(swh_md *md);
+/* md%make_initial_conditions_consistent (multidomain_mod.f90:1917-1931): one halo
+ * exchange without flux correction; friction work of truely-linear
+ * leapfrog+friction domains is then built from the exchanged elevation. */
+int swh_md_make_initial_conditions_consistent(swh_md *md);
+/* md%evolve_one_step(dt) (multidomain_mod.f90:1093-1212). */
+int swh_md_evolve_one_step(swh_md *md, double dt);
+int swh_md_evolve_steps(swh_md *md, double dt, int32_t nsteps);
+/* md%get_flow_volume per domain: volume_in_priority_domain (domain_mod.f90:1851-1888). */
+int swh_md_volumes(swh_md *md, double *out);
+/* The receive buffer of domain k's receive box c (both 1-based), as filled by the
+ * last exchange (diagnostics). */
+int swh_md_download_recv_buffer(swh_md *md, int32_t k, int32_t c, double *out);
+int swh_md_synchronize(swh_md *md);
+
 #ifdef __cplusplus
 }
 #endif

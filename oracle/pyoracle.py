@@ -179,3 +179,183 @@ class CpuDomain:
     def set_stage_bc(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
         s = (_I * 4)(*sides)
         self.f["domain_set_stage_bc"](self.h, float(amp), float(period), float(msl), s)
+
+
+def _load_md(spherical):
+    """The multidomain entry points of the compiled reference (ref kind only)."""
+    key = ("ref_md", spherical)
+    if key in _libs:
+        return _libs[key]
+    path = lib_path("ref", spherical)
+    if not os.path.exists(path):
+        raise FileNotFoundError(path + " is not built (run `make -C oracle ref`)")
+    lib = ctypes.CDLL(path)
+    _PI = ctypes.POINTER(_I)
+    fns = {}
+
+    def f(name, res, args):
+        fn = getattr(lib, "ref_md_" + name)
+        fn.restype = res
+        fn.argtypes = args
+        fns[name] = fn
+
+    f("begin", None, [_I])
+    f("set_domain", None, [_I, _I, _I, _PD, _PD, _I, _I, _D])
+    f("set_nested", None, [_I, _I, _PD, _PD, _D, _I, _I])
+    f("set_params", None, [_I, _I, _D, _D])
+    f("setup", None, [])
+    f("dims", None, [_I, _PI])
+    f("geometry", None, [_I, _PD, _PD, _PD])
+    f("metrics", None, [_I] + [_PD] * 6)
+    f("set_U", None, [_I, _PD])
+    f("get_U", None, [_I, _PD])
+    f("set_manning_squared", None, [_I, _PD])
+    f("make_consistent", None, [])
+    f("evolve", None, [_D, _I])
+    f("time", None, [_I, _PD])
+    f("priority", None, [_I, _PI, _PI])
+    f("all", None, [_PD, _PI])
+    f("comm", None, [_I, _I, _I, _PI, _PD])
+    f("comm_buffer", None, [_I, _I, _I, _PD])
+    f("volume", None, [_PD])
+    _libs[key] = fns
+    return fns
+
+
+def _pi(a):
+    return a.ctypes.data_as(ctypes.POINTER(_I))
+
+
+class RefMultidomain:
+    """The reference's own multidomain_type (multidomain_mod.f90), driven through the
+    bind(C) shim: the same setup (md%setup builds the nesting halos, priority
+    domains and send/recv boxes), initial exchange and evolve_one_step.
+
+    ``domains`` is a list of dicts: the first (and any other top-level domain) has
+    ``lw, ll, nx, ny``; nested ones have ``parent`` (0-based), ``ll``, ``ur`` and
+    ``dx_refinement_factor``. Every dict has ``method`` and
+    ``timestepping_refinement_factor`` (default 1), and optionally
+    ``linear_solver_is_truely_linear``, ``msl_linear``, ``linear_friction_coeff``.
+
+    The reference writes its log and output folders under the working directory,
+    so ``setup`` runs inside ``workdir`` (a scratch folder)."""
+
+    def __init__(self, domains, spherical=True, workdir=None):
+        import tempfile
+        self.spherical = spherical
+        self.f = _load_md(spherical)
+        self.nd = len(domains)
+        self.f["begin"](self.nd)
+        for k, d in enumerate(domains):
+            m = METHODS[d["method"]]
+            trf = int(d.get("timestepping_refinement_factor", 1))
+            if "parent" in d:
+                ll = np.asarray(d["ll"], dtype=np.float64)
+                ur = np.asarray(d["ur"], dtype=np.float64)
+                self.f["set_nested"](k + 1, d["parent"] + 1, _p(ll), _p(ur), float(d["dx_refinement_factor"]), trf, m)
+            else:
+                lw = np.asarray(d["lw"], dtype=np.float64)
+                ll = np.asarray(d["ll"], dtype=np.float64)
+                self.f["set_domain"](k + 1, int(d["nx"]), int(d["ny"]), _p(lw), _p(ll), m, trf, 1.0)
+            self.f["set_params"](k + 1, int(d.get("linear_solver_is_truely_linear", 1)), float(d.get("msl_linear", 0.0)),
+                                 float(d.get("linear_friction_coeff", 0.0)))
+        self.workdir = workdir or tempfile.mkdtemp(prefix="swals_md_")
+        # the first domain's setup copies '*.f* make*' from the working directory
+        # (domain_mod.f90:2061) and stops if nothing matches
+        for name in ("make_placeholder", "placeholder.f90"):
+            open(os.path.join(self.workdir, name), "a").close()
+        cwd = os.getcwd()
+        os.chdir(self.workdir)
+        try:
+            self.f["setup"]()
+        finally:
+            os.chdir(cwd)
+        self.methods = [d["method"] for d in domains]
+
+    def dims(self, k):
+        out = np.zeros(8, dtype=np.int32)
+        self.f["dims"](k + 1, _pi(out))
+        return dict(nx=int(out[0]), ny=int(out[1]), n_send=int(out[2]), n_recv=int(out[3]), staggered=int(out[4]),
+                    method_index=int(out[5]), timestepping_refinement_factor=int(out[6]),
+                    exterior_cells_width=int(out[7]))
+
+    def geometry(self, k):
+        lw, ll, dx = np.zeros(2), np.zeros(2), np.zeros(2)
+        self.f["geometry"](k + 1, _p(lw), _p(ll), _p(dx))
+        return lw, ll, dx
+
+    def metrics(self, k):
+        d = self.dims(k)
+        nx, ny = d["nx"], d["ny"]
+        dbe, dle, area = np.zeros(ny + 1), np.zeros(nx + 1), np.zeros(ny)
+        cor, corbe, tl = np.zeros(ny), np.zeros(ny + 1), np.zeros(ny)
+        self.f["metrics"](k + 1, _p(dbe), _p(dle), _p(area), _p(cor), _p(corbe), _p(tl))
+        return dict(distance_bottom_edge=dbe, distance_left_edge=dle, area_cell_y=area, coriolis=cor,
+                    coriolis_bottom_edge=corbe, tanlat_on_radius_earth=tl)
+
+    def set_U(self, k, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        self.f["set_U"](k + 1, _p(U))
+
+    def get_U(self, k):
+        d = self.dims(k)
+        U = np.zeros((4, d["ny"], d["nx"]))
+        self.f["get_U"](k + 1, _p(U))
+        return U
+
+    def set_manning_squared(self, k, m):
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        self.f["set_manning_squared"](k + 1, _p(m))
+
+    def make_initial_conditions_consistent(self):
+        self.f["make_consistent"]()
+
+    def evolve(self, dt, nsteps=1):
+        self.f["evolve"](float(dt), int(nsteps))
+
+    def time(self, k):
+        t = np.zeros(1)
+        self.f["time"](k + 1, _p(t))
+        return float(t[0])
+
+    def priority(self, k):
+        d = self.dims(k)
+        pdi = np.zeros((d["ny"], d["nx"]), dtype=np.int32)
+        pnp = np.zeros((d["ny"], d["nx"]), dtype=np.int32)
+        self.f["priority"](k + 1, _pi(pdi), _pi(pnp))
+        return pdi, pnp
+
+    def all_dx_and_methods(self):
+        dx = np.zeros((self.nd, 2))
+        m = np.zeros(self.nd, dtype=np.int32)
+        self.f["all"](_p(dx), _pi(m))
+        return dx, m
+
+    def comms(self, k, recv):
+        """The send (recv=False) or receive (recv=True) boxes of domain k, as dicts."""
+        d = self.dims(k)
+        n = d["n_recv"] if recv else d["n_send"]
+        out = []
+        for c in range(n):
+            io = np.zeros(32, dtype=np.int32)
+            do = np.zeros(4)
+            self.f["comm"](k + 1, int(recv), c + 1, _pi(io), _p(do))
+            out.append(dict(inds=io[0:6].reshape(3, 2).copy(), my_index=int(io[6]), neighbour_index=int(io[7]),
+                            my_comms_index=int(io[8]), neighbour_comms_index=int(io[9]),
+                            my_domain_is_finer=int(io[10]), equal_cell_ratios=int(io[11]),
+                            my_staggered=int(io[12]), neighbour_staggered=int(io[13]), wetdry_limiting=int(io[14]),
+                            buffer_size=int(io[15]), flux_start=int(io[16]), time_index=int(io[17]),
+                            flux_shape=io[18:26].reshape(4, 2).copy(), active=int(io[26]),
+                            cell_ratios=do[0:2].copy()))
+        return out
+
+    def comm_buffer(self, k, recv, c):
+        info = self.comms(k, recv)[c]
+        b = np.zeros(info["buffer_size"])
+        self.f["comm_buffer"](k + 1, int(recv), c + 1, _p(b))
+        return b
+
+    def volumes(self):
+        v = np.zeros(self.nd)
+        self.f["volume"](_p(v))
+        return v

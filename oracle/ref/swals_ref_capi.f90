@@ -12,6 +12,8 @@ module swals_ref_capi
     use global_mod, only: dp, ip, charlen, force_double
     use domain_mod, only: domain_type, STG, UH, VH, ELV
     use boundary_mod, only: boundary_stage_transmissive_normal_momentum
+    use multidomain_mod, only: multidomain_type
+    use timestepping_metadata_mod, only: timestepping_method_index
     implicit none
 
     integer, parameter :: max_domains = 32
@@ -21,6 +23,9 @@ module swals_ref_capi
     ! Parameters of the sinusoidal stage boundary (one set per handle)
     real(dp), save :: bc_amp(max_domains) = 0.0_dp, bc_period(max_domains) = 1.0_dp, &
         bc_msl(max_domains) = 0.0_dp
+
+    ! One reference multidomain (multidomain_mod.f90:171-276) for nesting checks
+    type(multidomain_type), target, save :: md
 
 contains
 
@@ -220,6 +225,235 @@ contains
         pool(h)%boundary_exterior = (sides /= 0)
         pool(h)%boundary_function => ref_stage_fun
         pool(h)%boundary_subroutine => boundary_stage_transmissive_normal_momentum
+    end subroutine
+
+    ! ---------------------------------------------------------------------
+    ! Multidomain: setup through the reference's own md%setup, stepping with
+    ! md%evolve_one_step (multidomain_mod.f90:1081-1212), and access to the
+    ! nesting tables that setup builds (nested_grid_comms_mod.f90:69-219).
+    ! ---------------------------------------------------------------------
+
+    subroutine ref_md_begin(nd) bind(C, name='ref_md_begin')
+        integer(c_int), value :: nd
+        type(multidomain_type) :: fresh
+        md = fresh
+        allocate(md%domains(nd))
+    end subroutine
+
+    subroutine ref_md_set_domain(k, nx, ny, lw, ll, method, trf, dxrf) bind(C, name='ref_md_set_domain')
+        integer(c_int), value :: k, nx, ny, method, trf
+        real(c_double), intent(in) :: lw(2), ll(2)
+        real(c_double), value :: dxrf
+        md%domains(k)%lower_left = real(ll, dp)
+        md%domains(k)%lw = real(lw, dp)
+        md%domains(k)%nx = [nx, ny]
+        md%domains(k)%dx = md%domains(k)%lw / md%domains(k)%nx
+        md%domains(k)%timestepping_refinement_factor = trf
+        md%domains(k)%dx_refinement_factor = real(dxrf, dp)
+        md%domains(k)%timestepping_method = method_name(method)
+    end subroutine
+
+    subroutine ref_md_set_nested(k, parent, ll, ur, dxrf, trf, method) bind(C, name='ref_md_set_nested')
+        integer(c_int), value :: k, parent, trf, method
+        real(c_double), intent(in) :: ll(2), ur(2)
+        real(c_double), value :: dxrf
+        call md%domains(k)%match_geometry_to_parent(parent_domain=md%domains(parent), &
+            lower_left=real(ll, dp), upper_right=real(ur, dp), dx_refinement_factor=int(dxrf, ip), &
+            timestepping_refinement_factor=trf)
+        md%domains(k)%timestepping_method = method_name(method)
+    end subroutine
+
+    subroutine ref_md_set_params(k, truely_linear, msl, lfc) bind(C, name='ref_md_set_params')
+        integer(c_int), value :: k, truely_linear
+        real(c_double), value :: msl, lfc
+        md%domains(k)%linear_solver_is_truely_linear = (truely_linear /= 0)
+        md%domains(k)%msl_linear = msl
+        md%domains(k)%linear_friction_coeff = lfc
+    end subroutine
+
+    subroutine ref_md_setup() bind(C, name='ref_md_setup')
+        integer :: k
+        do k = 1, size(md%domains)
+            md%domains(k)%record_max_U = .false.
+            md%domains(k)%nontemporal_grids_to_store = ['']
+        end do
+        call md%setup(verbose=.false.)
+    end subroutine
+
+    subroutine ref_md_dims(k, out) bind(C, name='ref_md_dims')
+        ! out = nx, ny, n_send_comms, n_recv_comms, staggered, method index,
+        !       timestepping_refinement_factor, exterior_cells_width
+        integer(c_int), value :: k
+        integer(c_int), intent(out) :: out(8)
+        associate(d => md%domains(k))
+            out(1:2) = d%nx
+            out(3) = 0; out(4) = 0
+            if(allocated(d%nesting%send_comms)) out(3) = size(d%nesting%send_comms)
+            if(allocated(d%nesting%recv_comms)) out(4) = size(d%nesting%recv_comms)
+            out(5) = merge(1, 0, d%is_staggered_grid)
+            out(6) = timestepping_method_index(d%timestepping_method)
+            out(7) = d%timestepping_refinement_factor
+            out(8) = d%exterior_cells_width
+        end associate
+    end subroutine
+
+    subroutine ref_md_geometry(k, lw, ll, dx) bind(C, name='ref_md_geometry')
+        integer(c_int), value :: k
+        real(c_double), intent(out) :: lw(2), ll(2), dx(2)
+        lw = md%domains(k)%lw; ll = md%domains(k)%lower_left; dx = md%domains(k)%dx
+    end subroutine
+
+    subroutine ref_md_metrics(k, dbe, dle, area, cor, cor_be, tanlat) bind(C, name='ref_md_metrics')
+        integer(c_int), value :: k
+        real(c_double), intent(out) :: dbe(*), dle(*), area(*), cor(*), cor_be(*), tanlat(*)
+        integer :: nx, ny
+        associate(d => md%domains(k))
+            nx = d%nx(1); ny = d%nx(2)
+            dbe(1:(ny+1)) = d%distance_bottom_edge
+            dle(1:(nx+1)) = d%distance_left_edge
+            area(1:ny) = d%area_cell_y
+            cor(1:ny) = 0.0_c_double; cor_be(1:(ny+1)) = 0.0_c_double; tanlat(1:ny) = 0.0_c_double
+            if(allocated(d%coriolis)) cor(1:ny) = d%coriolis
+            if(allocated(d%coriolis_bottom_edge)) cor_be(1:(ny+1)) = d%coriolis_bottom_edge
+            if(allocated(d%tanlat_on_radius_earth)) tanlat(1:ny) = d%tanlat_on_radius_earth
+        end associate
+    end subroutine
+
+    subroutine ref_md_set_U(k, U) bind(C, name='ref_md_set_U')
+        integer(c_int), value :: k
+        real(c_double), intent(in) :: U(*)
+        integer :: n
+        n = md%domains(k)%nx(1) * md%domains(k)%nx(2)
+        md%domains(k)%U = reshape(real(U(1:4*n), dp), [md%domains(k)%nx(1), md%domains(k)%nx(2), 4])
+    end subroutine
+
+    subroutine ref_md_get_U(k, U) bind(C, name='ref_md_get_U')
+        integer(c_int), value :: k
+        real(c_double), intent(out) :: U(*)
+        integer :: n
+        n = md%domains(k)%nx(1) * md%domains(k)%nx(2)
+        U(1:4*n) = real(reshape(md%domains(k)%U(:,:,1:4), [4*n]), c_double)
+    end subroutine
+
+    subroutine ref_md_set_manning_squared(k, m) bind(C, name='ref_md_set_manning_squared')
+        integer(c_int), value :: k
+        real(c_double), intent(in) :: m(*)
+        integer :: n
+        n = md%domains(k)%nx(1) * md%domains(k)%nx(2)
+        md%domains(k)%manning_squared = reshape(real(m(1:n), dp), [md%domains(k)%nx(1), md%domains(k)%nx(2)])
+    end subroutine
+
+    subroutine ref_md_make_consistent() bind(C, name='ref_md_make_consistent')
+        call md%make_initial_conditions_consistent()
+    end subroutine
+
+    subroutine ref_md_evolve(dt, nsteps) bind(C, name='ref_md_evolve')
+        real(c_double), value :: dt
+        integer(c_int), value :: nsteps
+        integer :: n
+        do n = 1, nsteps
+            call md%evolve_one_step(real(dt, dp))
+        end do
+    end subroutine
+
+    subroutine ref_md_time(k, t) bind(C, name='ref_md_time')
+        integer(c_int), value :: k
+        real(c_double), intent(out) :: t
+        t = md%domains(k)%time
+    end subroutine
+
+    subroutine ref_md_priority(k, pdi, pnp) bind(C, name='ref_md_priority')
+        ! priority_domain_index(nx, ny) and is_priority_domain_not_periodic(nx, ny)
+        integer(c_int), value :: k
+        integer(c_int), intent(out) :: pdi(*), pnp(*)
+        integer :: n
+        n = md%domains(k)%nx(1) * md%domains(k)%nx(2)
+        pdi(1:n) = reshape(md%domains(k)%nesting%priority_domain_index, [n])
+        pnp(1:n) = reshape(md%domains(k)%nesting%is_priority_domain_not_periodic, [n])
+    end subroutine
+
+    subroutine ref_md_all(dx, methods) bind(C, name='ref_md_all')
+        ! all_dx_md(1:2, k, 1) and all_timestepping_methods_md(k, 1), k = 1..nd
+        real(c_double), intent(out) :: dx(*)
+        integer(c_int), intent(out) :: methods(*)
+        integer :: k, nd
+        nd = size(md%domains)
+        do k = 1, nd
+            dx(2*k-1:2*k) = md%all_dx_md(1:2, k, 1)
+            methods(k) = md%all_timestepping_methods_md(k, 1)
+        end do
+    end subroutine
+
+    subroutine ref_md_comm(k, is_recv, c, iout, dout) bind(C, name='ref_md_comm')
+        ! iout(1:6)   send_inds or recv_inds (iL, iU, jL, jU, kL, kU)
+        ! iout(7:10)  my_domain_index, neighbour_domain_index, my_domain_comms_index,
+        !             neighbour_domain_comms_index
+        ! iout(11:15) my_domain_is_finer, equal_cell_ratios, my_domain_staggered_grid,
+        !             neighbour_domain_staggered_grid, use_wetdry_limiting
+        ! iout(16:18) buffer size, flux start, time index
+        ! iout(19:26) (rows, nvar) of the box flux integral for N, S, E, W
+        ! iout(27)    active
+        ! dout(1:2)   cell_ratios
+        integer(c_int), value :: k, is_recv, c
+        integer(c_int), intent(out) :: iout(32)
+        real(c_double), intent(out) :: dout(4)
+        integer :: d
+        iout = 0; dout = 0.0_c_double
+        if(is_recv /= 0) then
+            associate(cm => md%domains(k)%nesting%recv_comms(c))
+                iout(1:6) = reshape(cm%recv_inds, [6])
+                iout(16) = size(cm%recv_buffer); iout(17) = cm%recv_flux_start; iout(18) = cm%recv_timeind
+                do d = 1, 4
+                    iout(17 + 2*d) = size(cm%recv_box_flux_integral(d)%x, 1)
+                    iout(18 + 2*d) = size(cm%recv_box_flux_integral(d)%x, 2)
+                end do
+                iout(27) = merge(1, 0, cm%recv_active)
+                call common(cm)
+            end associate
+        else
+            associate(cm => md%domains(k)%nesting%send_comms(c))
+                iout(1:6) = reshape(cm%send_inds, [6])
+                iout(16) = size(cm%send_buffer); iout(17) = cm%send_flux_start; iout(18) = cm%send_timeind
+                do d = 1, 4
+                    iout(17 + 2*d) = size(cm%send_box_flux_integral(d)%x, 1)
+                    iout(18 + 2*d) = size(cm%send_box_flux_integral(d)%x, 2)
+                end do
+                iout(27) = merge(1, 0, cm%send_active)
+                call common(cm)
+            end associate
+        end if
+    contains
+        subroutine common(cm)
+            use nested_grid_comms_mod, only: two_way_nesting_comms_type
+            type(two_way_nesting_comms_type), intent(in) :: cm
+            iout(7) = cm%my_domain_index; iout(8) = cm%neighbour_domain_index
+            iout(9) = cm%my_domain_comms_index; iout(10) = cm%neighbour_domain_comms_index
+            iout(11) = merge(1, 0, cm%my_domain_is_finer); iout(12) = merge(1, 0, cm%equal_cell_ratios)
+            iout(13) = cm%my_domain_staggered_grid; iout(14) = cm%neighbour_domain_staggered_grid
+            iout(15) = merge(1, 0, cm%use_wetdry_limiting)
+            dout(1:2) = cm%cell_ratios
+        end subroutine
+    end subroutine
+
+    subroutine ref_md_comm_buffer(k, is_recv, c, out) bind(C, name='ref_md_comm_buffer')
+        integer(c_int), value :: k, is_recv, c
+        real(c_double), intent(out) :: out(*)
+        if(is_recv /= 0) then
+            out(1:size(md%domains(k)%nesting%recv_comms(c)%recv_buffer)) = md%domains(k)%nesting%recv_comms(c)%recv_buffer
+        else
+            out(1:size(md%domains(k)%nesting%send_comms(c)%send_buffer)) = md%domains(k)%nesting%send_comms(c)%send_buffer
+        end if
+    end subroutine
+
+    subroutine ref_md_volume(out) bind(C, name='ref_md_volume')
+        ! per-domain volume in the priority-domain cells (volume_in_priority_domain)
+        real(c_double), intent(out) :: out(*)
+        integer :: k
+        real(force_double) :: v
+        do k = 1, size(md%domains)
+            call md%domains(k)%volume_in_priority_domain(v)
+            out(k) = v
+        end do
     end subroutine
 
 end module

@@ -41,6 +41,17 @@ class Scalars(ctypes.Structure):
     ]
 
 
+class NestBox(ctypes.Structure):
+    """swh_nest_box: one send or receive box of a multidomain (1-based indices)."""
+    _fields_ = [
+        ("inds", _I32 * 6), ("neighbour_domain_index", _I32), ("neighbour_comms_index", _I32),
+        ("my_domain_is_finer", _I32), ("equal_cell_ratios", _I32), ("my_domain_staggered_grid", _I32),
+        ("neighbour_domain_staggered_grid", _I32), ("use_wetdry_limiting", _I32), ("buffer_size", _I32),
+        ("flux_start", _I32), ("time_index", _I32), ("flux_rows", _I32 * 4), ("flux_nvar", _I32),
+        ("cell_ratios", _D * 2),
+    ]
+
+
 class BackendError(RuntimeError):
     pass
 
@@ -89,6 +100,19 @@ SYMBOLS = [
     ("swh_adjoint_step", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_adjoint_inject", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_adjoint_run", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _I32, _PD]),
+    ("swh_md_create", ctypes.c_int, [_I32, ctypes.POINTER(ctypes.c_void_p), _PD, _PI32, _PI32,
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+    ("swh_md_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_md_set_priority", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _PI32]),
+    ("swh_md_set_flux_correct_dry_cells", ctypes.c_int, [ctypes.c_void_p, _I32, _I32]),
+    ("swh_md_add_box", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, ctypes.POINTER(NestBox)]),
+    ("swh_md_finalize", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_md_make_initial_conditions_consistent", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_md_evolve_one_step", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("swh_md_evolve_steps", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
+    ("swh_md_volumes", ctypes.c_int, [ctypes.c_void_p, _PD]),
+    ("swh_md_download_recv_buffer", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _PD]),
+    ("swh_md_synchronize", ctypes.c_int, [ctypes.c_void_p]),
 ]
 
 

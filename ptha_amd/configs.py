@@ -191,3 +191,34 @@ def adjoint_inversion(nx=2048, ny=2048, arcmin=1.0, ll=(140.0, -45.0), n_src_x=4
     return dict(nx=nx, ny=ny, lw=lw, ll=ll, x=x, y=y, elev=elev, src_lon=lon, src_lat=lat, sigma_km=sigma_km,
                 w_true=w_true, gauges_xy=gauges_xy, hours=hours, gauge_interval=gauge_interval, noise=noise,
                 seed=seed)
+
+
+def nested_small():
+    """A three-level nest in the layout of the nested multidomain workload, scaled
+    down for parity tests: a 'linear' parent (lon 100-120, lat 60S-48S at 20
+    arcmin), a 'leapfrog_linear_plus_nonlinear_friction' child at 1/3 of its cell
+    size (Manning n = 0.03) and an 'rk2' grandchild at 1/9, stepped 3 times per
+    global step. The shore runs north-south at lon 109.5 inside the grandchild
+    (4000 m deep to the west, land to +50 m to the east), and a 1 m Gaussian hump
+    (sigma 50 km) sits in the child west of the grandchild. Global step 10 s."""
+    domains = [
+        dict(lw=(20.0, 12.0), ll=(100.0, -60.0), nx=60, ny=36, method="linear"),
+        dict(parent=0, ll=(105.0, -57.0), ur=(113.0, -50.0), dx_refinement_factor=3,
+             timestepping_refinement_factor=1, method="leapfrog_linear_plus_nonlinear_friction"),
+        dict(parent=1, ll=(108.0, -55.0), ur=(110.0, -53.0), dx_refinement_factor=3,
+             timestepping_refinement_factor=3, method="rk2"),
+    ]
+    return dict(domains=domains, dt=10.0, manning_n=0.03, spherical=True)
+
+
+def nested_small_state(x, y):
+    """Initial U (4, ny, nx) of ``nested_small`` at cell centres x (nx,), y (ny,)."""
+    X, Y = np.meshgrid(x, y)
+    elev = -4000.0 * np.clip((109.5 - X) / 1.0, 0.0, 1.0) + 50.0 * np.clip((X - 109.5) / 0.5, 0.0, 1.0)
+    elev = np.where(X < 109.5, np.minimum(elev, -5.0), elev)
+    hump = np.exp(-(((X - 107.0) * math.cos(math.radians(-54.0)) * 111e3) ** 2 + ((Y + 54.0) * 111e3) ** 2)
+                  / (2 * 50e3 ** 2))
+    U = np.zeros((4,) + X.shape)
+    U[3] = elev
+    U[0] = np.maximum(0.0, elev) + hump * (elev < 0)
+    return U

@@ -8,7 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "swh_internal.h"
+#include "swh_domain.h"
 
 namespace swh {
 
@@ -34,52 +34,6 @@ static int fail(int code, const std::string &msg)
 
 using namespace swh;
 
-struct swh_domain {
-    int nx = 0, ny = 0, pitch = 0, fpitch = 0;
-    int method = 0, prec = 0, truely_linear = 1, coriolis = 0, spherical = 0, friction_type = 0, n_ext = 2;
-    int bex[4] = {1, 1, 1, 1};
-    int maxq_mask = 0, maxq_freq = 1;
-    double msl = 0.0, cfl = 0.7, theta = 1.6, lfc = 0.0, max_timestep = kMaxTimestep, arrival_thresh = 0.01;
-    size_t esz = 8;  // bytes per state element
-    hipStream_t stream = nullptr;
-    // host copies of the FP64 metrics
-    std::vector<double> dbe, dle, area, cor, corbe, tanlat;
-    std::vector<double> h_elev, h_nsq;
-    // device metrics (FP64)
-    double *d_dbe = nullptr, *d_dle = nullptr, *d_area = nullptr, *d_cor = nullptr, *d_tanlat = nullptr;
-    // state: ping-pong S, UH, VH (+ Kahan C), shared E
-    void *S[2] = {}, *UH[2] = {}, *VH[2] = {}, *C[2] = {}, *E = nullptr;
-    int cur = 0;
-    void *fwU = nullptr, *fwV = nullptr, *nsq = nullptr, *mx = nullptr;
-    // FV work
-    void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
-    double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
-    // leapfrog row coefficients for rc_dt
-    void *rc = nullptr;
-    double rc_dt = -1.0;
-    DevScalars *sc = nullptr;
-    double *d_work = nullptr;  // reductions scratch (2*ny + 8 doubles)
-    // host-side scalars (leapfrog keeps time on the host; rk2 on the device)
-    double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;
-    long long nsteps = 0;
-    int fw_ready = 0;
-    // boundary
-    int bc_mask = 0;
-    double bc_amp = 0.0, bc_period = 1.0, bc_msl = 0.0;
-    // gauges
-    int ng = 0, gmax = 0, gsamples = 0;
-    long long *d_gidx = nullptr;
-    double *d_gval = nullptr, *d_gtime = nullptr;
-    // adjoint state (ping-pong lS, lU, lV), allocated on first use
-    void *adj[2][3] = {};
-    int adj_cur = 0;
-    double *d_gvec = nullptr;  // one sample of gauge values for injection
-    // launch shape of the step kernel
-    int block_x = 256, rows_per_block = 32, variant = 2;
-
-    bool leapfrog() const { return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF; }
-    size_t plane_elems() const { return (size_t)pitch * ny; }
-};
 
 namespace {
 
@@ -225,6 +179,11 @@ int leapfrog_step(swh_domain *h, double dt)
     const int c = h->cur, o = 1 - c;
     const double t_half = h->time + dt * 0.5;
     const double t_after = t_half + 0.5 * dt;
+    // nesting flux integrals: dt * (UH, VH) before the update, mass only
+    // (domain_mod_linear_solver_include.f90:74-77)
+    if (h->n_nest_strips)
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->VH[c], (const T *)h->UH[c], 0,
+                                     0, h->pitch, 1, dt, h->d_nest_int, h->stream));
     // boundary fluxes read the pre-momentum UH/VH (include.f90:126-165)
     SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
                                      h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
@@ -290,14 +249,25 @@ int rk2_step(swh_domain *h, double dt)
     const int A = h->cur, B = 1 - A;
     SWH_HIP(launch_fv_begin(h->sc, h->stream));
     // first Euler stage: A -> B (A stays as backup_U)
+    if (h->n_nest_strips && !(dt > 0.0))
+        return fail(SWH_EINVAL, "nested rk2 domains need the timestep (the multidomain step provides it)");
+    const long long FPn = (long long)h->fpitch * (h->ny + 1), EPn = (long long)h->fpitch * h->ny;
     FVArgs<T> a1 = fv_args<T>(h, B, A);
     SWH_HIP(launch_fv_euler<T>(a1, dt, 1, h->stream));
+    // nesting flux integrals, half a step per Euler stage
+    // (domain_mod_timestepping_alternatives_include.f90:98-118)
+    if (h->n_nest_strips)
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
+                                     EPn, h->fpitch, 3, dt * 0.5, h->d_nest_int, h->stream));
     if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], h->nx, h->ny, h->pitch, h->bc_mask,
                                       h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
     // second Euler stage with the same step, in place on B
     FVArgs<T> a2 = fv_args<T>(h, B, B);
     SWH_HIP(launch_fv_euler<T>(a2, dt, 0, h->stream));
+    if (h->n_nest_strips)
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
+                                     EPn, h->fpitch, 3, dt * 0.5, h->d_nest_int, h->stream));
     if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], h->nx, h->ny, h->pitch, h->bc_mask,
                                       h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
@@ -322,6 +292,32 @@ int evolve(swh_domain *h, double dt)
     }
     return h->prec == SWH_PREC_FP64 ? rk2_step<double>(h, dt) : rk2_step<float>(h, dt);
 }
+
+}  // namespace
+
+namespace swh {
+
+int domain_evolve(swh_domain *h, double dt) { return evolve(h, dt); }
+
+int domain_refresh_elevation(swh_domain *h)
+{
+    std::vector<double> e((size_t)h->nx * h->ny);
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    int rc = h->prec == SWH_PREC_FP64 ? download_plane<double>(h, h->E, nullptr, e.data())
+                                      : download_plane<float>(h, h->E, nullptr, e.data());
+    if (rc) return rc;
+    // keep the FP64 host value wherever the device still holds its rounding
+    for (size_t k = 0; k < e.size(); k++) {
+        const bool same = h->prec == SWH_PREC_FP64 ? e[k] == h->h_elev[k] : (float)h->h_elev[k] == (float)e[k];
+        if (!same) h->h_elev[k] = e[k];
+    }
+    if (h->method == SWH_METHOD_LLPNF && h->truely_linear && !h->h_nsq.empty()) return build_friction_work(h);
+    return SWH_OK;
+}
+
+}  // namespace swh
+
+namespace {
 
 int sync_scalars(swh_domain *h, DevScalars *out)
 {

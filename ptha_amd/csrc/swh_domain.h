@@ -1,0 +1,67 @@
+// The device-side state of one SWALS domain (shared by the C ABI translation units).
+#pragma once
+#include <vector>
+
+#include "swh_internal.h"
+
+using namespace swh;
+
+struct swh_domain {
+    int nx = 0, ny = 0, pitch = 0, fpitch = 0;
+    int method = 0, prec = 0, truely_linear = 1, coriolis = 0, spherical = 0, friction_type = 0, n_ext = 2;
+    int bex[4] = {1, 1, 1, 1};
+    int maxq_mask = 0, maxq_freq = 1;
+    double msl = 0.0, cfl = 0.7, theta = 1.6, lfc = 0.0, max_timestep = kMaxTimestep, arrival_thresh = 0.01;
+    size_t esz = 8;  // bytes per state element
+    hipStream_t stream = nullptr;
+    // host copies of the FP64 metrics
+    std::vector<double> dbe, dle, area, cor, corbe, tanlat;
+    std::vector<double> h_elev, h_nsq;
+    // device metrics (FP64)
+    double *d_dbe = nullptr, *d_dle = nullptr, *d_area = nullptr, *d_cor = nullptr, *d_tanlat = nullptr;
+    // state: ping-pong S, UH, VH (+ Kahan C), shared E
+    void *S[2] = {}, *UH[2] = {}, *VH[2] = {}, *C[2] = {}, *E = nullptr;
+    int cur = 0;
+    void *fwU = nullptr, *fwV = nullptr, *nsq = nullptr, *mx = nullptr;
+    // FV work
+    void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
+    double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
+    // leapfrog row coefficients for rc_dt
+    void *rc = nullptr;
+    double rc_dt = -1.0;
+    DevScalars *sc = nullptr;
+    double *d_work = nullptr;  // reductions scratch (2*ny + 8 doubles)
+    // host-side scalars (leapfrog keeps time on the host; rk2 on the device)
+    double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;
+    long long nsteps = 0;
+    int fw_ready = 0;
+    // boundary
+    int bc_mask = 0;
+    double bc_amp = 0.0, bc_period = 1.0, bc_msl = 0.0;
+    // gauges
+    int ng = 0, gmax = 0, gsamples = 0;
+    long long *d_gidx = nullptr;
+    double *d_gval = nullptr, *d_gtime = nullptr;
+    // adjoint state (ping-pong lS, lU, lV), allocated on first use
+    void *adj[2][3] = {};
+    int adj_cur = 0;
+    double *d_gvec = nullptr;  // one sample of gauge values for injection
+    // multidomain nesting: box-edge strips whose flux integrals each step accumulates
+    const swh::NestStrip *d_nest_strips = nullptr;
+    int n_nest_strips = 0;
+    double *d_nest_int = nullptr;
+    // launch shape of the step kernel
+    int block_x = 256, rows_per_block = 32, variant = 2;
+
+    bool leapfrog() const { return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF; }
+    size_t plane_elems() const { return (size_t)pitch * ny; }
+};
+
+namespace swh {
+// One domain%evolve_one_step on the device (leapfrog or rk2), as swh_evolve_one_step.
+int domain_evolve(swh_domain *h, double dt);
+// Re-reads the elevation from the device and rebuilds the leapfrog friction work
+// from it (the reference builds friction_work on the first step, after any halo
+// exchange has set the elevation in the halos).
+int domain_refresh_elevation(swh_domain *h);
+}  // namespace swh

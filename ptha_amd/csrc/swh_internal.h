@@ -94,7 +94,8 @@ hipError_t launch_maxq(const T *S, const T *C, const T *UH, const T *VH, const T
 
 template <typename T>
 hipError_t launch_volume(const T *S, const T *C, const T *E, int nx, int ny, int pitch, int n_ext,
-                         const double *area, double *row_work, double *out, hipStream_t s);
+                         const double *area, double *row_work, double *out, hipStream_t s,
+                         const int *mask = nullptr);
 
 template <typename T>
 hipError_t launch_gauges(const T *S, const T *C, const T *UH, const T *VH, int n, const long long *idx,
@@ -153,5 +154,129 @@ hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int
                               double msl, const DevScalars *sc, hipStream_t s);
 hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s);
 hipError_t launch_fv_finish(DevScalars *sc, hipStream_t s);
+
+
+// ---------------------------------------------------------------------------
+// Multidomain nesting (one GPU). All box indices are 1-based and inclusive, as in
+// the reference's send_inds / recv_inds (nested_grid_comms_mod.f90:69-219); the
+// kernels convert when they address the 0-based pitched planes.
+// ---------------------------------------------------------------------------
+constexpr int kMaxNestDomains = 16;
+constexpr int kNestGradients = 4;   // number_of_gradients_coarse_to_fine
+constexpr double kMissingTime = -1.7976931348623157e308;   // -HUGE(1.0_dp)
+
+// Pointers to one domain's current state (the leapfrog ping-pong buffer in use).
+template <typename T>
+struct NestDomPtr {
+    T *S, *C, *UH, *VH;    // C: Kahan compensation of S (represented stage S - C), or null
+    const T *E;
+    int nx, ny, pitch;
+};
+
+template <typename T>
+struct NestDoms {
+    NestDomPtr<T> d[kMaxNestDomains];
+};
+
+// One edge strip of a send or receive box, over which dt * flux * edge is
+// accumulated (nested_grid_comms_mod.f90:2308-2461).
+struct NestStrip {
+    int kind;      // 0: a row of north-south fluxes, 1: a column of east-west fluxes
+    int line;      // 0-based row (kind 0) or column (kind 1) in the flux planes; -1: out of range
+    int lo, n;     // 0-based start and length along the strip
+    int out;       // offset of x(1, 1) of this strip in the domain's integral array; x(r, k) at out + k*n + r
+    double edge;   // distance_bottom_edge / distance_left_edge factor (1 when fluxes are pre-multiplied)
+};
+
+// Packing of one send box into the receiver's buffer (process_data_to_send_2022Feb,
+// nested_grid_comms_mod.f90:1387-2046).
+struct NestPack {
+    int dom;                  // sending domain (0-based)
+    int mode;                 // 0: equal cells, same grid type; 1: fine-to-coarse; 2: coarse-to-fine
+    int iL, iU, jL, jU, kL, kU;
+    int iR, jR, kR;           // receive-grid extent of the box
+    int inv_r[2], r_ip[2];    // nint(1/cell_ratios), nint(cell_ratios)
+    double cr[2], inv_cr[2];  // cell_ratios, 1/cell_ratios
+    int my_stag, nbr_stag, wetdry;
+    int flux_start;           // 0-based offset of the flux section in the buffer
+    int time_index;           // 0-based offset of the time
+    int strip_out[4], strip_n[4], nvar;   // the box's N, S, E, W integral strips
+    int n_data, n_flux;       // counts of data / flux entries written
+    long long buf;            // offset of the buffer in the buffer arena
+    int block0;               // first block of this box in the batched launch
+};
+
+// Unpacking of one receive box (process_received_data, nested_grid_comms_mod.f90:2048-2249).
+struct NestUnpack {
+    int dom;                 // receiving domain
+    int fine;                // my_domain_is_finer: interpolate coarse values + gradients
+    int iL, iU, jL, jU, kL, kU;
+    int inv_r[2];
+    double cr[2], mid[2];
+    int niC, njC, nC, n_data;
+    int strip_out[4], strip_n[4], nvar;   // the box's own integrals (recv side)
+    int err_out;             // offset of the flux error in the error arena (same layout as the strips)
+    long long buf;
+    int block0;
+    int stag_fix;            // 0: none, 1: elevation rule (truely linear), 2: depth rule
+    double elev_lim;         // msl_linear - minimum_allowed_depth
+};
+
+// One flux-correction update (domain_mod.f90:2803-3226): cells p0..p1 along
+// 'line', all variables var1..varN, U += sgn * err(k)/area.
+struct NestFix {
+    int is_ew;        // 0: column ni (cells (ni, p)), 1: row mi (cells (p, mi)); 0-based line
+    int line, p0, p1;
+    int var1, varN;   // 0-based inclusive; varN < var1: nothing
+    double sgn;       // +1 or -1 with the N/S/E/W sign rule folded in
+    double area;
+    int err;          // offset of err(row, var 0) in the error arena; var k at err + k*stride
+    int stride;
+};
+
+// The flux-correction work of one domain: 'fix' grouped in n_seg segments
+// (seg[s] .. seg[s+1]-1), applied in order.
+struct NestFixDom {
+    const NestFix *fix;
+    const int *seg;
+    int n_seg, dom;
+    const double *err;      // the domain's flux-error arena
+    const int *pnp;         // is_priority_domain_not_periodic, nx*ny, 0-based row-major
+    int dry_all;            // flux_correct_dry_cells_outside_priority_domain
+};
+
+// Re-interpolation of a receive box of a finer domain after flux correction
+// (domain_mod.f90:3228-3468).
+struct NestReint {
+    int dom;
+    int n0, n1, m0, m1;     // 1-based box
+    int inv_r[2];
+    int ind[4];             // 1-based copy row/col for N, S, E, W
+    int copy;               // offset of the copied strips in the copy arena: [side][var][pos]
+    int len_ns, len_ew;     // n1-n0+1, m1-m0+1
+    int ncx, ncy;           // coarse cells in x and y
+    int block0;
+};
+
+template <typename T>
+hipError_t launch_nest_zero(double *p, long long n, hipStream_t s);
+template <typename T>
+hipError_t launch_nest_accum(const NestStrip *strips, int n_strips, const T *fNS, const T *fEW, long long plane_ns,
+                             long long plane_ew, int pitch, int nvar, double dt, double *integ, hipStream_t s);
+template <typename T>
+hipError_t launch_nest_pack(const NestDoms<T> &doms, const NestPack *packs, int n_packs, int n_blocks,
+                            double *const *integ, double *arena, hipStream_t s);
+template <typename T>
+hipError_t launch_nest_unpack(const NestDoms<T> &doms, const NestUnpack *ups, int n_ups, int n_blocks,
+                              double *const *integ, double *const *err, const double *arena, hipStream_t s);
+template <typename T>
+hipError_t launch_nest_stag_fix(const NestDoms<T> &doms, const NestUnpack *ups, int n_ups, int n_blocks,
+                                hipStream_t s);
+template <typename T>
+hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, int n, hipStream_t s);
+template <typename T>
+hipError_t launch_nest_reint(const NestDoms<T> &doms, const NestReint *rs, int n_rs, int n_blocks, double *copy,
+                             hipStream_t s);
+
 
 }  // namespace swh

@@ -189,12 +189,15 @@ template hipError_t launch_maxq<float>(const float *, const float *, const float
 template <typename T>
 __global__ __launch_bounds__(256) void volume_rows_kernel(const T *__restrict__ S, const T *__restrict__ C,
                                                           const T *__restrict__ E, int nx, int pitch, int n_ext,
-                                                          const double *__restrict__ area, double *row_work)
+                                                          const double *__restrict__ area, double *row_work,
+                                                          const int *__restrict__ mask)
 {
     const int j = n_ext + blockIdx.x;
     DD acc{0.0, 0.0};
     for (int i = n_ext + threadIdx.x; i <= nx - n_ext - 1; i += blockDim.x) {
         const long long o = (long long)j * pitch + i;
+        // priority-domain cells only (domain_mod.f90:1872-1876), mask is nx*ny row-major
+        if (mask && mask[(long long)j * nx + i] != 1) continue;
         double sv = (double)S[o];
         if (C) acc = dd_add(acc, -(double)C[o]);
         acc = dd_add(acc, sv - (double)E[o]);
@@ -220,21 +223,21 @@ __global__ __launch_bounds__(256) void volume_sum_kernel(const double *row_work,
 
 template <typename T>
 hipError_t launch_volume(const T *S, const T *C, const T *E, int nx, int ny, int pitch, int n_ext,
-                         const double *area, double *row_work, double *out, hipStream_t s)
+                         const double *area, double *row_work, double *out, hipStream_t s, const int *mask)
 {
     const int nrows = ny - 2 * n_ext;
     if (nrows <= 0) {
         return hipMemsetAsync(out, 0, sizeof(double), s);
     }
     hipLaunchKernelGGL(volume_rows_kernel<T>, dim3(nrows), dim3(256), 0, s, S, C, E, nx, pitch, n_ext, area,
-                       row_work);
+                       row_work, mask);
     hipLaunchKernelGGL(volume_sum_kernel, dim3(1), dim3(256), 0, s, row_work, nrows, out);
     return hipGetLastError();
 }
 template hipError_t launch_volume<double>(const double *, const double *, const double *, int, int, int, int,
-                                          const double *, double *, double *, hipStream_t);
+                                          const double *, double *, double *, hipStream_t, const int *);
 template hipError_t launch_volume<float>(const float *, const float *, const float *, int, int, int, int,
-                                         const double *, double *, double *, hipStream_t);
+                                         const double *, double *, double *, hipStream_t, const int *);
 
 // ---- point gauges (point_gauge_mod.f90:208-245) ----
 template <typename T>
