@@ -159,6 +159,53 @@ def copy_probe(mib):
                                                                                 2 * mib * (1 << 20) / ms / 1e6))
 
 
+def secondary_legs(steps):
+    """Throughput of the other single-grid paths (not the headline workload):
+    rk2 on the 2000 x 1000 wet/dry beach (FP64 and FP32 state) and one 2048^2
+    spherical 'linear' grid (the adjoint-inversion grid) in FP32 + Kahan and FP64.
+    Timed with host wall clock around synchronised step loops (inputs resident)."""
+    from ptha_amd import configs as C
+    from ptha_amd.domain import Domain
+    out = {}
+    cb = C.beach_rk2()
+    for prec in ("fp64", "fp32k"):
+        d = Domain(cb["nx"], cb["ny"], cb["lw"], cb["ll"], "rk2", precision=prec, max_U_variables=())
+        d.U = cb["U"]
+        d.set_manning_squared(cb["manning_squared"])
+        bc = cb["stage_bc"]
+        d.set_stage_boundary_sine(bc["amp"], bc["period"], sides=bc["sides"])
+        d.evolve_steps(None, 5)
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.evolve_steps(None, steps)
+        d.synchronize()
+        wall = time.perf_counter() - t0
+        out["rk2_beach_" + prec] = dict(gcell_per_s=round(cb["nx"] * cb["ny"] * steps / wall / 1e9, 3),
+                                         ms_per_step=round(1e3 * wall / steps, 4))
+        d.close()
+    ca = C.adjoint_inversion()
+    from ptha_amd.domain import build_metrics
+    m = build_metrics(ca["nx"], ca["ny"], ca["lw"], ca["ll"], True)
+    dt = linear_timestep(ca["elev"], m)
+    for prec, bpc in (("fp32k", 36.0), ("fp64", 56.0)):
+        d = Domain(ca["nx"], ca["ny"], ca["lw"], ca["ll"], "linear", precision=prec, spherical=True,
+                   max_U_variables=())
+        U = np.zeros((4, ca["ny"], ca["nx"]))
+        U[3] = ca["elev"]
+        d.U = U
+        d.evolve_steps(dt, 5)
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.evolve_steps(dt, steps)
+        d.synchronize()
+        wall = time.perf_counter() - t0
+        g = ca["nx"] * ca["ny"] * steps / wall / 1e9
+        out["linear_2048sq_" + prec] = dict(gcell_per_s=round(g, 3), ms_per_step=round(1e3 * wall / steps, 4),
+                                            hbm_frac=round(g * bpc / HBM_PEAK_GBS, 4))
+        d.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +218,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=str, default="",
                     help="block_x,rows_per_block,variant of the step kernel (default: the batch's autotuned shape)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the rk2 beach and single 2048^2 grid throughput legs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed legs, run a plain device copy of MIB MiB (for calibrating "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE) and log its bandwidth")
@@ -198,9 +247,15 @@ def main():
 
     cells = nx * ny
     res = {}
+    tuned = {}
     for prec in (["fp32k"] if args.no_fp64 else ["fp32k", "fp64"]):
         log("timing", prec)
         b = build_batch(cfg, prec, nsrc, nsteps_src, every, max_samples, block)
+        shape = block
+        if not block:
+            shape, tune_ms = b.autotune(dt, reps=3)
+            log(prec, "autotuned launch shape (block_x, rows, variant) =", shape, "%.3f ms/step" % tune_ms)
+        tuned[prec] = shape
         wall, ms = time_steps(b, dt, args.steps, args.warmup)
         step, nact = b.status()
         assert nact == nsrc
@@ -216,7 +271,7 @@ def main():
     full = None
     if not args.no_full:
         log("whole configuration (every source to its duration)")
-        b = build_batch(cfg, "fp32k", nsrc, nsteps_src, every, max_samples, block)
+        b = build_batch(cfg, "fp32k", nsrc, nsteps_src, every, max_samples, tuned.get("fp32k"))
         b.synchronize()
         t0 = time.perf_counter()
         chunk = 2000
@@ -236,6 +291,11 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+
+    secondary = None
+    if not args.no_secondary:
+        log("secondary legs: rk2 beach, single 2048^2 grid")
+        secondary = secondary_legs(50)
 
     cpu = None
     if not args.no_cpu:
@@ -261,7 +321,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched unit sources: %d Gaussian sources on one 1200x1200 spherical 1-arcmin "
                                "'linear' leapfrog grid (Coriolis), FP32 state + Kahan-compensated stage" % nsrc,
-                   "nx": nx, "ny": ny, "sources": nsrc, "dt_s": dt, "gauges": 64, "gauge_every_steps": every},
+                   "nx": nx, "ny": ny, "sources": nsrc, "dt_s": dt, "gauges": 64, "gauge_every_steps": every,
+                   "launch_fp32k": list(tuned["fp32k"])},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic("leapfrog_batch_fp32k")},
@@ -273,6 +334,8 @@ def main():
                        "achieved_gbs": round(r64["achieved_gbs"], 1),
                        "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
                        "fp32k_over_fp64": round(r32["gcell"] / r64["gcell"], 3)}
+    if secondary:
+        out["secondary"] = secondary
     if full:
         out["unit_sources_per_hr"] = round(full["unit_sources_per_hr"], 1)
         out["full_run"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in full.items()}

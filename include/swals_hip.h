@@ -204,9 +204,18 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active);
 /* Launch shape of the leapfrog step kernel: threads per block (64..256, multiple of
  * 64), rows per thread strip, and variant bits: 1 = 16-byte vector lanes (4 floats /
  * 2 doubles per lane), 2 = keep the next row's loads in flight, 4 = non-temporal
- * state traffic, 8 = (batches) consecutive workgroups walk the sources of one tile.
+ * state traffic, 8 = (batches) consecutive workgroups walk the sources of one tile,
+ * 16 = (batches) two fused steps per launch wherever no gauge sample falls between
+ * them (friction modes without per-step friction work only).
  * Results do not depend on the launch shape. */
 int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t variant);
+/* Launch-shape autotuner (no counterpart in the reference): times every shape
+ * (variant 0..15 x block_x {64,128,256} x rows {16,32,64}) over reps repeats of the
+ * next step on this batch's own grid and running sources, keeps the fastest, and
+ * returns it in best[3] = {block_x, rows_per_block, variant} with its time per step.
+ * The state and step count are left as they were. Needs the step counts and the
+ * elevation set. */
+int swh_batch_autotune(swh_batch *b, double dt, int32_t reps, int32_t *best, double *best_ms);
 int swh_batch_synchronize(swh_batch *b);
 /* swh_batch_evolve bracketed by HIP events recorded on the batch's stream; returns the
  * device time of the region in milliseconds (for throughput measurement). */

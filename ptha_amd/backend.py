@@ -90,6 +90,7 @@ SYMBOLS = [
     ("swh_batch_gauges_download", ctypes.c_int, [ctypes.c_void_p, _I32, _PD, _PI32]),
     ("swh_batch_status", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _PI32]),
     ("swh_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
+    ("swh_batch_autotune", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PI32, _PD]),
     ("swh_domain_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
     ("swh_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_batch_evolve_timed", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PD]),

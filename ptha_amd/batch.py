@@ -127,6 +127,16 @@ class UnitSourceBatch:
         B.check(self._lib.swh_batch_set_launch(self._h, int(block_x), int(rows_per_block), int(variant)),
                 "swh_batch_set_launch")
 
+    def autotune(self, dt, reps=3):
+        """Pick the fastest step-kernel launch shape on this batch (state unchanged).
+        Returns ((block_x, rows_per_block, variant), ms_per_step)."""
+        best = np.zeros(3, dtype=np.int32)
+        ms = ctypes.c_double()
+        B.check(self._lib.swh_batch_autotune(self._h, float(dt), int(reps),
+                                             best.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(ms)),
+                "swh_batch_autotune")
+        return tuple(int(v) for v in best), ms.value
+
     def evolve(self, dt, nsteps):
         B.check(self._lib.swh_batch_evolve(self._h, float(dt), int(nsteps)), "swh_batch_evolve")
 

@@ -372,6 +372,10 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     h->max_timestep = d->maximum_timestep > 0 ? d->maximum_timestep : kMaxTimestep;
     h->arrival_thresh = d->arrival_stage_threshold_above_msl_linear;
     h->esz = h->prec == SWH_PREC_FP64 ? 8 : 4;
+    // default launch shape of the step kernel, from the measured sweep over shapes
+    // (16-byte vector lanes; FP32 also keeps the next row's loads in flight)
+    if (h->prec == SWH_PREC_FP64) { h->block_x = 128; h->rows_per_block = 16; h->variant = 1; }
+    else { h->block_x = 64; h->rows_per_block = 32; h->variant = 3; }
     const int nx = h->nx, ny = h->ny;
     h->dbe.assign(d->distance_bottom_edge, d->distance_bottom_edge + ny + 1);
     h->dle.assign(d->distance_left_edge, d->distance_left_edge + nx + 1);
@@ -864,12 +868,14 @@ struct swh_batch {
     std::vector<int> slot_of;      // user index -> slot
     int nsteps_set = 0;
     long long step = 0;
-    int cur0 = 0;                  // buffer holding step 0
+    int cur = 0;                   // buffer holding the running sources' state
+    std::vector<int> slot_buf;     // buffer holding each finished source's final state
     // gauges
     int ng = 0, every = 1, gmax = 0;
     long long *d_gidx = nullptr;
     double *d_gval = nullptr;      // [slot][sample][3][ng]
     int block_x = 256, rows_per_block = 32, variant = 2, src_fast = 0;
+    int two_step = 0;              // two fused steps per launch where possible
 };
 
 namespace {
@@ -917,17 +923,16 @@ int batch_sample(swh_batch *b, int n_active, int buf)
 }
 
 template <typename T, bool KAHAN>
-int batch_evolve(swh_batch *b, double dt, int nsteps)
+int batch_args(swh_batch *b, double dt, LFArgs<T> &a, int &fric_mode)
 {
     swh_domain *g = b->g;
     if (g->rc_dt != dt) {
         int rc = build_rc<T>(g, dt);
         if (rc) return rc;
     }
-    int fric_mode = 0;
+    fric_mode = 0;
     if (g->method == SWH_METHOD_LLPNF) fric_mode = 1;
     if (fric_mode == 1 && !g->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload elevation first");
-    LFArgs<T> a;
     a.nx = g->nx; a.ny = g->ny; a.pitch = g->pitch; a.src_stride = (long long)g->plane_elems();
     a.E = (const T *)g->E;
     a.rc = rc_view<T>(g);
@@ -936,27 +941,162 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
     a.lin_fric_mult = (T)(1.0 / (1.0 + dt * g->lfc));
     a.fric_power = (T)((g->friction_type == 1) ? -2.0 : -7.0 / 3.0);
     a.bc_mask = 0; a.bc_stage = T(0);
+    a.maxq_mask = 0; a.mx = nullptr; a.t_after = T(0); a.arrival_stage = T(0);
+    return SWH_OK;
+}
+
+// One batched step reading buffer c and writing the other.
+template <typename T, bool KAHAN>
+int batch_launch(swh_batch *b, LFArgs<T> &a, int fric_mode, int n_active, int c)
+{
+    swh_domain *g = b->g;
+    const int o = 1 - c;
     a.rows_per_block = b->rows_per_block;
     a.src_fast = b->src_fast;
-    a.maxq_mask = 0; a.mx = nullptr; a.t_after = T(0); a.arrival_stage = T(0);
+    a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
+    a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
+    SWH_HIP(launch_leapfrog<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n_active,
+                                      b->block_x, b->variant, g->stream));
+    return SWH_OK;
+}
+
+// One launch advancing slots [first, first + n) by two steps (reads buffer c).
+template <typename T, bool KAHAN>
+int batch_launch2(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, int c)
+{
+    swh_domain *g = b->g;
+    const int o = 1 - c;
+    a.rows_per_block = b->rows_per_block;
+    a.src_fast = 0;
+    a.src_base = first;
+    a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
+    a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
+    SWH_HIP(launch_leapfrog2<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n, b->block_x,
+                                       g->stream));
+    a.src_base = 0;
+    return SWH_OK;
+}
+
+template <typename T, bool KAHAN>
+int batch_evolve(swh_batch *b, double dt, int nsteps)
+{
+    LFArgs<T> a;
+    int fric_mode = 0;
+    int rc = batch_args<T, KAHAN>(b, dt, a, fric_mode);
+    if (rc) return rc;
     if (b->step == 0) {
-        int rc = batch_sample<T, KAHAN>(b, batch_active(b, -1), b->cur0);
+        rc = batch_sample<T, KAHAN>(b, batch_active(b, -1), b->cur);
         if (rc) return rc;
     }
-    for (int k = 0; k < nsteps; k++) {
+    const bool can2 = b->two_step && fric_mode <= 1;
+    for (int k = 0; k < nsteps;) {
         const int n_active = batch_active(b, b->step);
         if (n_active == 0) break;
-        const int c = (int)((b->cur0 + b->step) & 1), o = 1 - c;
-        a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
-        a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
-        SWH_HIP(launch_leapfrog<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n_active,
-                                          b->block_x, b->variant, g->stream));
-        b->step++;
-        if (b->ng && (b->step % b->every) == 0) {
-            int rc = batch_sample<T, KAHAN>(b, n_active, o);
+        const int c = b->cur, o = 1 - c;
+        const bool sample_next = b->ng && ((b->step + 1) % b->every) == 0;
+        int n_last;   // sources that took the last step of this round
+        if (can2 && k + 1 < nsteps && !sample_next) {
+            // sources with at least two steps left take both in one launch; the
+            // ones with one step left take it in a one-step launch
+            const int n2 = batch_active(b, b->step + 1);
+            if (n2 > 0) {
+                rc = batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c);
+                if (rc) return rc;
+            }
+            if (n_active > n2) {
+                a.src_base = n2;
+                rc = batch_launch<T, KAHAN>(b, a, fric_mode, n_active - n2, c);
+                a.src_base = 0;
+                if (rc) return rc;
+            }
+            b->step += 2;
+            k += 2;
+            n_last = n2;
+        } else {
+            rc = batch_launch<T, KAHAN>(b, a, fric_mode, n_active, c);
+            if (rc) return rc;
+            b->step++;
+            k++;
+            n_last = n_active;
+        }
+        b->cur = o;
+        for (int sl = batch_active(b, b->step); sl < n_active; sl++) b->slot_buf[sl] = o;
+        if (b->ng && (b->step % b->every) == 0 && n_last > 0) {
+            rc = batch_sample<T, KAHAN>(b, n_last, o);
             if (rc) return rc;
         }
     }
+    return SWH_OK;
+}
+
+// Launch-shape autotuner: times every candidate shape on the batch's own grid and
+// sources by repeating the next step (reads the current buffer, writes the other,
+// which the next real step overwrites), so the state and step count are unchanged.
+// One-step shapes: variant 0..15 x block_x {64,128,256} x rows {16,32,64}; then the
+// two-steps-per-launch kernel (variant bit 16) over block_x x rows {16,...,128},
+// compared per cell-update.
+template <typename T, bool KAHAN>
+int batch_autotune(swh_batch *b, double dt, int reps, int32_t *best, double *best_ms)
+{
+    swh_domain *g = b->g;
+    LFArgs<T> a;
+    int fric_mode = 0;
+    int rc = batch_args<T, KAHAN>(b, dt, a, fric_mode);
+    if (rc) return rc;
+    const int n_active = batch_active(b, b->step);
+    if (n_active == 0) return fail(SWH_ESTATE, "autotune: no source is still running");
+    const int n2 = batch_active(b, b->step + 1);
+    const int c = b->cur;
+    hipEvent_t e0, e1;
+    SWH_HIP(hipEventCreate(&e0));
+    SWH_HIP(hipEventCreate(&e1));
+    const int bxs[3] = {64, 128, 256}, rws[4] = {16, 32, 64, 128};
+    double bms = 1e300;   // ms per step of n_active sources
+    int bb = b->block_x, br = b->rows_per_block, bv = b->variant | (b->src_fast << 3) | (b->two_step << 4);
+    auto timed = [&](bool two, double &ms_out) -> int {
+        int r = two ? batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c)
+                    : batch_launch<T, KAHAN>(b, a, fric_mode, n_active, c);   // warm-up
+        if (r) return r;
+        if (hipEventRecord(e0, g->stream) != hipSuccess) return fail(SWH_EHIP, "event record");
+        for (int k = 0; k < reps; k++) {
+            r = two ? batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c)
+                    : batch_launch<T, KAHAN>(b, a, fric_mode, n_active, c);
+            if (r) return r;
+        }
+        float ms = 0.f;
+        if (hipEventRecord(e1, g->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return fail(SWH_EHIP, "event timing");
+        // per step of n_active sources
+        ms_out = two ? (double)ms / reps / 2.0 * ((double)n_active / n2) : (double)ms / reps;
+        return SWH_OK;
+    };
+    for (int v = 0; v < 16 && rc == SWH_OK; v++)
+        for (int ib = 0; ib < 3 && rc == SWH_OK; ib++)
+            for (int ir = 0; ir < 3 && rc == SWH_OK; ir++) {
+                b->block_x = bxs[ib]; b->rows_per_block = rws[ir]; b->variant = v & 7; b->src_fast = (v >> 3) & 1;
+                double ms;
+                rc = timed(false, ms);
+                if (!rc && ms < bms) { bms = ms; bb = bxs[ib]; br = rws[ir]; bv = v; }
+            }
+    if (!rc && fric_mode <= 1 && n2 > 0) {
+        const int v1 = bv;
+        b->variant = v1 & 7; b->src_fast = (v1 >> 3) & 1;
+        for (int ib = 0; ib < 3 && rc == SWH_OK; ib++)
+            for (int ir = 0; ir < 4 && rc == SWH_OK; ir++) {
+                b->block_x = bxs[ib]; b->rows_per_block = rws[ir];
+                double ms;
+                rc = timed(true, ms);
+                if (!rc && ms < bms) { bms = ms; bb = bxs[ib]; br = rws[ir]; bv = v1 | 16; }
+            }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->block_x = bb; b->rows_per_block = br; b->variant = bv & 7; b->src_fast = (bv >> 3) & 1;
+    b->two_step = (bv >> 4) & 1;
+    if (rc) return rc;
+    if (best) { best[0] = bb; best[1] = br; best[2] = bv; }
+    if (best_ms) *best_ms = bms;
     return SWH_OK;
 }
 
@@ -979,6 +1119,7 @@ int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **
         return rc;
     }
     b->nsrc = nsources;
+    b->block_x = b->g->block_x; b->rows_per_block = b->g->rows_per_block; b->variant = b->g->variant;
     const size_t bytes = b->g->plane_elems() * b->g->esz * (size_t)nsources;
     const bool kahan = b->g->prec == SWH_PREC_FP32K;
     for (int k = 0; k < 2 && rc == SWH_OK; k++) {
@@ -992,6 +1133,7 @@ int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **
         return rc;
     }
     b->nsteps_slot.assign(nsources, 0);
+    b->slot_buf.assign(nsources, 0);
     b->slot_of.resize(nsources);
     for (int k = 0; k < nsources; k++) b->slot_of[k] = k;
     *out = b;
@@ -1052,7 +1194,7 @@ int swh_batch_upload_source(swh_batch *b, int32_t k, const double *U3)
     swh_domain *g = b->g;
     SWH_HIP(hipStreamSynchronize(g->stream));
     const size_t n = (size_t)g->nx * g->ny, off = (size_t)b->slot_of[k] * g->plane_elems() * g->esz;
-    const int c = b->cur0;
+    const int c = b->cur;
     int rc;
     if ((rc = upload_plane_any(g, (char *)b->S[c] + off, U3))) return rc;
     if ((rc = upload_plane_any(g, (char *)b->UH[c] + off, U3 + n))) return rc;
@@ -1067,8 +1209,7 @@ int swh_batch_download_source(swh_batch *b, int32_t k, double *U3)
     swh_domain *g = b->g;
     SWH_HIP(hipStreamSynchronize(g->stream));
     const int slot = b->slot_of[k];
-    const long long taken = std::min<long long>(b->step, b->nsteps_slot[slot]);
-    const int buf = (int)((b->cur0 + taken) & 1);
+    const int buf = b->nsteps_slot[slot] > b->step ? b->cur : b->slot_buf[slot];
     const size_t n = (size_t)g->nx * g->ny, off = (size_t)slot * g->plane_elems() * g->esz;
     int rc;
     if (g->prec == SWH_PREC_FP64) {
@@ -1137,9 +1278,17 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active)
 
 static int check_launch(int32_t block_x, int32_t rows_per_block, int32_t variant)
 {
-    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 15)
-        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..15");
+    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 31)
+        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..31");
     return SWH_OK;
+}
+
+int swh_batch_autotune(swh_batch *b, double dt, int32_t reps, int32_t *best, double *best_ms)
+{
+    if (!b || !(dt > 0.0) || reps < 1) return fail(SWH_EINVAL, "swh_batch_autotune: bad argument");
+    if (!b->nsteps_set) return fail(SWH_ESTATE, "swh_batch_autotune: set the step counts first");
+    return b->g->prec == SWH_PREC_FP64 ? batch_autotune<double, false>(b, dt, reps, best, best_ms)
+                                       : batch_autotune<float, true>(b, dt, reps, best, best_ms);
 }
 
 int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t variant)
@@ -1148,7 +1297,7 @@ int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, 
     int rc = check_launch(block_x, rows_per_block, variant);
     if (rc) return rc;
     b->block_x = block_x; b->rows_per_block = rows_per_block;
-    b->variant = variant & 7; b->src_fast = (variant >> 3) & 1;
+    b->variant = variant & 7; b->src_fast = (variant >> 3) & 1; b->two_step = (variant >> 4) & 1;
     return SWH_OK;
 }
 

@@ -71,6 +71,7 @@ struct LFArgs {
     T bc_stage;
     int rows_per_block;
     int src_fast;              // batched runs: block x index walks the sources
+    int src_base = 0;          // batched runs: first source of this launch
     // fused max_U update (mask of SWH_MAXQ bits; 0 = none this step)
     int maxq_mask;
     T *mx;                     // N_MAXQ planes of pitch*ny
@@ -81,6 +82,12 @@ struct LFArgs {
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                            int linear_friction, int nsrc, int block_x, int variant, hipStream_t s);
+
+// Two fused leapfrog steps per launch (temporal blocking) for batched runs without
+// boundary stage, max_U or flux tracking; friction modes 0 and 1 only.
+template <typename T, bool KAHAN>
+hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                            int linear_friction, int nsrc, int block_x, hipStream_t s);
 
 template <typename T>
 hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext,

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
     const int j0 = blockIdx.y * a.rows_per_block;
     if (j0 >= a.ny) return;
     const int j1 = min(j0 + a.rows_per_block, a.ny);
-    const long long so = (long long)bsrc * a.src_stride;
+    const long long so = (long long)(bsrc + a.src_base) * a.src_stride;
     const T *__restrict__ S = a.S + so;
     const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
     const T *__restrict__ UH = a.UH + so;

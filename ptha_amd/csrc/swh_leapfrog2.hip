@@ -1,0 +1,262 @@
+// Two leapfrog steps per launch (temporal blocking) for batched unit sources.
+//
+// Each step is exactly the step of domain_mod_linear_solver_include.f90 (stage
+// half-step :92-113, UH/VH half-step :211-571 with Coriolis :408-439, friction
+// :441-505 and linear drag :550-557), computed with the same operations in the
+// same order as the one-step kernel, so two steps here equal two one-step launches
+// bit for bit. What changes is the schedule: the state is read from HBM once and
+// written once per two steps, halving the traffic per cell-update.
+//
+// Work split: a wavefront owns 61 consecutive columns and holds 64 (one halo column
+// to the west, two to the east); each lane owns one column and marches up a strip
+// of rows. At row m it forms step 1 at row m (stage, UH) and m-1 (VH), then step 2
+// at row m-1 (stage, UH) and m-2 (VH), and writes row m-2. East/west neighbours come
+// from the adjacent lanes (cross-lane permutes, no LDS tile, no barriers); the
+// stencil reaches one column each way per step, which the 3 halo lanes cover.
+// Rows below/above the strip are recomputed by the strip itself (3 extra rows).
+#include "swh_internal.h"
+
+namespace swh {
+
+namespace {
+
+constexpr int kOwn = 61;   // columns a wavefront writes (lanes 1..61)
+
+template <typename T>
+__device__ __forceinline__ T shl1(T v) { return __shfl_down(v, 1); }   // value of lane + 1
+template <typename T>
+__device__ __forceinline__ T shr1(T v) { return __shfl_up(v, 1); }     // value of lane - 1
+
+// New stage (include.f90:97-113); no stage boundary in this kernel.
+template <typename T, bool KAHAN>
+__device__ __forceinline__ void stage_cell(const LFArgs<T> &a, int ii, int jj, T s, T c, T uh, T uhm, T vh, T vhm,
+                                           T &sn, T &cn)
+{
+    const bool interior = ii > 0 && ii < a.nx - 1 && jj > 0 && jj < a.ny - 1;
+    if (interior) {
+        const T delta = a.rc.sa[jj] * ((uh - uhm) * a.rc.dle[ii] + (vh * a.rc.dbe[jj + 1] - vhm * a.rc.dbe[jj]));
+        if (KAHAN) {
+            const T y = -delta - c;
+            const T t = s + y;
+            cn = (t - s) - y;
+            sn = t;
+        } else {
+            sn = s - delta;
+            cn = T(0);
+        }
+    } else {
+        sn = s;
+        cn = c;
+    }
+}
+
+// UH at (ii + 1/2, j) from the new stage of cells ii, ii+1 and the pre-step VH of
+// rows j-1 (vhm0, vhm1 at ii, ii+1) and j (vh0, vh1).
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__device__ __forceinline__ T mom_u(const LFArgs<T> &a, int ii, int j, T uh_old, T s0, T c0, T s1, T c1, T e0, T e1,
+                                   T vhm0, T vhm1, T vh0, T vh1)
+{
+    if (!(ii >= 0 && ii < a.nx - 1 && j >= 0 && j < a.ny - 1)) return uh_old;
+    const T MIN = T(kMinDepth), HALF = T(0.5), msl = a.msl;
+    T h_iph, wet_iph;
+    if (TL) {
+        h_iph = (e0 < -MIN + msl && e1 < -MIN + msl) ? msl - HALF * (e1 + e0) : T(0);
+        wet_iph = T(1);
+    } else {
+        h_iph = (s0 - e0 > MIN && s1 - e1 > MIN) ? HALF * ((s1 + s0) - (e1 + e0)) : T(0);
+        wet_iph = h_iph > T(0) ? T(1) : T(0);
+    }
+    T ds_i = s1 - s0;
+    if (KAHAN) ds_i = ds_i - (c1 - c0);
+    T uh_new = uh_old * wet_iph - a.rc.cu[j] * h_iph * ds_i * a.rc.dle[ii + 1];
+    if (COR || FRIC) {
+        const T vh_jmh = HALF * (vhm0 + vhm1);
+        const T vh_jph = HALF * (vh0 + vh1);
+        if (COR) {
+            const T dhc = a.rc.fu[j] * (h_iph > T(0) ? T(1) : T(0));
+            uh_new = uh_new + dhc * (vh_jmh + vh_jph);
+        }
+        if (FRIC) {
+            const T fwu = a.fwU[(long long)j * a.pitch + ii];
+            const T vbar = HALF * (vh_jmh + vh_jph);
+            const T fmu = T(1) / (T(1) + a.dt_lfc + a.dt * fwu * (sqrt(uh_old * uh_old + vbar * vbar)));
+            uh_new = fmu * uh_new;
+        }
+    }
+    if (LINF) uh_new = a.lin_fric_mult * uh_new;
+    return uh_new;
+}
+
+// VH at (ii, j + 1/2) from the new stage of rows j, j+1 and the pre-step UH of
+// row j (uh_i at ii, uh_im1 at ii-1) and row j+1 (uhu_i, uhu_im1).
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__device__ __forceinline__ T mom_v(const LFArgs<T> &a, int ii, int j, T vh_old, T s0, T c0, T sup, T cup, T e0,
+                                   T eu, T uh_i, T uh_im1, T uhu_i, T uhu_im1)
+{
+    if (!(ii >= 0 && ii < a.nx - 1 && j >= 0 && j < a.ny - 1)) return vh_old;
+    const T MIN = T(kMinDepth), HALF = T(0.5), msl = a.msl;
+    T h_jph, wet_jph;
+    if (TL) {
+        h_jph = (eu < -MIN + msl && e0 < -MIN + msl) ? msl - HALF * (eu + e0) : T(0);
+        wet_jph = T(1);
+    } else {
+        h_jph = (sup - eu > MIN && s0 - e0 > MIN) ? HALF * ((sup + s0) - (eu + e0)) : T(0);
+        wet_jph = h_jph > T(0) ? T(1) : T(0);
+    }
+    T ds_j = sup - s0;
+    if (KAHAN) ds_j = ds_j - (cup - c0);
+    T vh_new = vh_old * wet_jph - a.rc.cv[j] * h_jph * ds_j * a.rc.dbe[j + 1];
+    if (COR || FRIC) {
+        const T uh_ij = HALF * (uh_i + uh_im1);
+        const T uh_ijp1 = HALF * (uhu_i + uhu_im1);
+        if (COR) {
+            const T dhcj = a.rc.fv[j] * (h_jph > T(0) ? T(1) : T(0));
+            vh_new = vh_new - dhcj * (uh_ij + uh_ijp1);
+        }
+        if (FRIC) {
+            const T fwv = a.fwV[(long long)j * a.pitch + ii];
+            const T ubar = HALF * (uh_ij + uh_ijp1);
+            const T fmv = T(1) / (T(1) + a.dt_lfc + a.dt * fwv * (sqrt(vh_old * vh_old + ubar * ubar)));
+            vh_new = fmv * vh_new;
+        }
+    }
+    if (LINF) vh_new = a.lin_fric_mult * vh_new;
+    return vh_new;
+}
+
+}  // namespace
+
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wg = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wg * kOwn >= a.nx) return;                       // whole wavefront past the east edge
+    const int col = wg * kOwn - 1 + lane;                // column of this lane (may be -1 or >= nx)
+    const int ci = min(max(col, 0), a.nx - 1);           // column it loads (clamped into the row)
+    const bool own = lane >= 1 && lane <= kOwn && col < a.nx;
+    const int j0 = blockIdx.y * a.rows_per_block;
+    if (j0 >= a.ny) return;
+    const int j1 = min(j0 + a.rows_per_block, a.ny);
+    const long long so = (long long)(blockIdx.z + a.src_base) * a.src_stride;
+    const T *__restrict__ S = a.S + so;
+    const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
+    const T *__restrict__ UH = a.UH + so;
+    const T *__restrict__ VH = a.VH + so;
+    T *__restrict__ So = a.So + so;
+    T *__restrict__ Co = KAHAN ? a.Co + so : nullptr;
+    T *__restrict__ UHo = a.UHo + so;
+    T *__restrict__ VHo = a.VHo + so;
+    const bool west = col == 0;                          // UH(i-1) clamps to UH(0) (include.f90:319-322)
+
+    // registers carried between rows (named by the row they belong to, relative to m)
+    T vh0_p = T(0), vh0_p_r = T(0), uh0_p = T(0), uh0_p_l = T(0), e_p = T(0), e_p_r = T(0), e_pp = T(0);
+    T s1_p = T(0), c1_p = T(0), s1_p_r = T(0), c1_p_r = T(0);
+    T uh1_p = T(0), uh1_p_l = T(0), uh1_pp = T(0), uh1_pp_l = T(0);
+    T vh1_pp = T(0), vh1_pp_r = T(0);
+    T s2_p = T(0), c2_p = T(0), uh2_p = T(0);
+    {
+        // pre-step VH/UH/E of row j0-2 (feeds only rows the strip does not write)
+        const int r = min(max(j0 - 2, 0), a.ny - 1);
+        const long long o = (long long)r * a.pitch + ci;
+        vh0_p = VH[o];
+        uh0_p = UH[o];
+        e_p = a.E[o];
+        vh0_p_r = shl1(vh0_p);
+        uh0_p_l = west ? uh0_p : shr1(uh0_p);
+        e_p_r = shl1(e_p);
+    }
+
+    for (int m = j0 - 1; m <= j1 + 1; m++) {
+        const int mr = min(max(m, 0), a.ny - 1);
+        const long long o = (long long)mr * a.pitch + ci;
+        const T s0 = S[o];
+        const T c0 = KAHAN ? C[o] : T(0);
+        const T uh0 = UH[o];
+        const T vh0 = VH[o];
+        const T e = a.E[o];
+        const T uh0_l = west ? uh0 : shr1(uh0);
+        const T vh0_r = shl1(vh0), e_r = shl1(e);
+        // VH of the row below, clamped to row 0 at the southern edge (include.f90:295-308)
+        const T vhb = m > 0 ? vh0_p : vh0, vhb_r = m > 0 ? vh0_p_r : vh0_r;
+
+        // step 1: stage and UH at row m
+        T s1, c1;
+        stage_cell<T, KAHAN>(a, col, m, s0, c0, uh0, uh0_l, vh0, vhb, s1, c1);
+        const T s1_r = shl1(s1);
+        const T c1_r = KAHAN ? shl1(c1) : T(0);
+        const T uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m, uh0, s1, c1, s1_r, c1_r, e, e_r, vhb, vhb_r,
+                                                            vh0, vh0_r);
+        const T uh1_l = west ? uh1 : shr1(uh1);
+        // step 1: VH at row m-1
+        const T vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m - 1, vh0_p, s1_p, c1_p, s1, c1, e_p, e, uh0_p,
+                                                            uh0_p_l, uh0, uh0_l);
+        const T vh1_r = shl1(vh1);
+        // step 2: stage and UH at row m-1 (VH of row m-2 clamped to row 0 at the southern edge)
+        const T vhb1 = m - 1 > 0 ? vh1_pp : vh1, vhb1_r = m - 1 > 0 ? vh1_pp_r : vh1_r;
+        T s2, c2;
+        stage_cell<T, KAHAN>(a, col, m - 1, s1_p, c1_p, uh1_p, uh1_p_l, vh1, vhb1, s2, c2);
+        const T s2_r = shl1(s2);
+        const T c2_r = KAHAN ? shl1(c2) : T(0);
+        const T uh2 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m - 1, uh1_p, s2, c2, s2_r, c2_r, e_p, e_p_r, vhb1,
+                                                            vhb1_r, vh1, vh1_r);
+        // step 2: VH at row m-2
+        const T vh2 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m - 2, vh1_pp, s2_p, c2_p, s2, c2, e_pp, e_p,
+                                                            uh1_pp, uh1_pp_l, uh1_p, uh1_p_l);
+        const int k = m - 2;
+        if (own && k >= j0 && k < j1) {
+            const long long ow = (long long)k * a.pitch + col;
+            So[ow] = s2_p;
+            if (KAHAN) Co[ow] = c2_p;
+            UHo[ow] = uh2_p;
+            VHo[ow] = vh2;
+        }
+        // slide up one row
+        s2_p = s2; c2_p = c2; uh2_p = uh2;
+        uh1_pp = uh1_p; uh1_pp_l = uh1_p_l; uh1_p = uh1; uh1_p_l = uh1_l;
+        vh1_pp = vh1; vh1_pp_r = vh1_r;
+        s1_p = s1; c1_p = c1; s1_p_r = s1_r; c1_p_r = c1_r;
+        vh0_p = vh0; vh0_p_r = vh0_r; uh0_p = uh0; uh0_p_l = uh0_l;
+        e_pp = e_p; e_p = e; e_p_r = e_r;
+    }
+    (void)s1_p_r;
+    (void)c1_p_r;
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+static hipError_t launch_lf2(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)
+{
+    const int waves = (a.nx + kOwn - 1) / kOwn, wpb = block_x / 64;
+    dim3 grid((waves + wpb - 1) / wpb, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
+    hipLaunchKernelGGL((leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF>), grid, dim3(block_x), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC>
+static hipError_t lf2_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int bx, hipStream_t s)
+{
+    if (tl) return linf ? launch_lf2<T, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s)
+                        : launch_lf2<T, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s);
+    return linf ? launch_lf2<T, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s)
+                : launch_lf2<T, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s);
+}
+
+template <typename T, bool KAHAN>
+hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                            int linear_friction, int nsrc, int block_x, hipStream_t s)
+{
+    if (friction_mode > 1 || a.bc_mask || a.maxq_mask || (block_x % 64) || block_x < 64 || block_x > 256)
+        return hipErrorInvalidValue;
+    const int linf = (friction_mode == 0) && linear_friction;
+    if (coriolis) {
+        return friction_mode ? lf2_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s)
+                             : lf2_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s);
+    }
+    return friction_mode ? lf2_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s)
+                         : lf2_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s);
+}
+
+template hipError_t launch_leapfrog2<double, false>(const LFArgs<double> &, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_leapfrog2<float, true>(const LFArgs<float> &, int, int, int, int, int, int, hipStream_t);
+
+}  // namespace swh

@@ -54,6 +54,22 @@ module swals_hip_mod
         integer(c_int64_t) :: nsteps_advanced
     end type
 
+    ! Mirrors struct swh_nest_box: one two_way_nesting_comms_type box (1-based indices)
+    type, bind(C), public :: swh_nest_box
+        integer(c_int32_t) :: inds(6) = 0
+        integer(c_int32_t) :: neighbour_domain_index = 0, neighbour_comms_index = 0
+        integer(c_int32_t) :: my_domain_is_finer = 0, equal_cell_ratios = 0
+        integer(c_int32_t) :: my_domain_staggered_grid = 0, neighbour_domain_staggered_grid = 0
+        integer(c_int32_t) :: use_wetdry_limiting = 1
+        integer(c_int32_t) :: buffer_size = 0, flux_start = 0, time_index = 0
+        integer(c_int32_t) :: flux_rows(4) = 0, flux_nvar = 0
+        real(c_double) :: cell_ratios(2) = 1.0_c_double
+    end type
+
+    public :: swh_md_create, swh_md_destroy, swh_md_set_priority, swh_md_set_flux_correct_dry_cells, &
+        swh_md_add_box, swh_md_finalize, swh_md_make_initial_conditions_consistent, swh_md_evolve_one_step, &
+        swh_md_evolve_steps, swh_md_volumes, swh_md_synchronize
+
     public :: swh_last_error, swh_domain_create, swh_domain_destroy, swh_upload_U, swh_download_U, &
         swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_evolve_one_step, &
         swh_evolve_steps, swh_get_scalars, swh_volume_interior, swh_stationary_timestep_max, &
@@ -159,6 +175,66 @@ module swals_hip_mod
             import c_int, c_ptr, c_int32_t
             type(c_ptr), value :: h
             integer(c_int32_t), value :: block_x, rows_per_block, variant
+        end function
+        ! ---- multidomain (one GPU) ----
+        integer(c_int) function swh_md_create(n, domains, all_dx, all_methods, trf, md) bind(C, name='swh_md_create')
+            import c_int, c_ptr, c_int32_t, c_double
+            integer(c_int32_t), value :: n
+            type(c_ptr), intent(in) :: domains(*)
+            real(c_double), intent(in) :: all_dx(*)
+            integer(c_int32_t), intent(in) :: all_methods(*), trf(*)
+            type(c_ptr), intent(out) :: md
+        end function
+        integer(c_int) function swh_md_destroy(md) bind(C, name='swh_md_destroy')
+            import c_int, c_ptr
+            type(c_ptr), value :: md
+        end function
+        integer(c_int) function swh_md_set_priority(md, k, pdi, pnp) bind(C, name='swh_md_set_priority')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: md
+            integer(c_int32_t), value :: k
+            integer(c_int32_t), intent(in) :: pdi(*), pnp(*)
+        end function
+        integer(c_int) function swh_md_set_flux_correct_dry_cells(md, k, on) &
+                bind(C, name='swh_md_set_flux_correct_dry_cells')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: md
+            integer(c_int32_t), value :: k, on
+        end function
+        integer(c_int) function swh_md_add_box(md, k, is_recv, box) bind(C, name='swh_md_add_box')
+            import c_int, c_ptr, c_int32_t, swh_nest_box
+            type(c_ptr), value :: md
+            integer(c_int32_t), value :: k, is_recv
+            type(swh_nest_box), intent(in) :: box
+        end function
+        integer(c_int) function swh_md_finalize(md) bind(C, name='swh_md_finalize')
+            import c_int, c_ptr
+            type(c_ptr), value :: md
+        end function
+        integer(c_int) function swh_md_make_initial_conditions_consistent(md) &
+                bind(C, name='swh_md_make_initial_conditions_consistent')
+            import c_int, c_ptr
+            type(c_ptr), value :: md
+        end function
+        integer(c_int) function swh_md_evolve_one_step(md, dt) bind(C, name='swh_md_evolve_one_step')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: md
+            real(c_double), value :: dt
+        end function
+        integer(c_int) function swh_md_evolve_steps(md, dt, nsteps) bind(C, name='swh_md_evolve_steps')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: md
+            real(c_double), value :: dt
+            integer(c_int32_t), value :: nsteps
+        end function
+        integer(c_int) function swh_md_volumes(md, out) bind(C, name='swh_md_volumes')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: md
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function swh_md_synchronize(md) bind(C, name='swh_md_synchronize')
+            import c_int, c_ptr
+            type(c_ptr), value :: md
         end function
     end interface
 

@@ -1,8 +1,10 @@
-"""GPU: batched unit sources (one launch per step over all running sources).
+"""GPU: batched unit sources (one launch per step, or per two steps, over all
+running sources).
 
 Each source of an FP64 batch must equal, bit for bit, the oracle run of that source
 alone for its own number of steps (sources of different durations freeze at their
 own end), and its gauge series must equal the oracle state sampled at the same steps.
+The two-steps-per-launch schedule must give the same bits as one step per launch.
 """
 import numpy as np
 import pytest
@@ -19,15 +21,19 @@ def _cfg():
     return u
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
-def test_batch_matches_single_source_runs(oracle_built, precision):
-    print("CASE batch", precision, flush=True)
+@pytest.mark.parametrize("precision,two_step", [("fp64", False), ("fp32k", False), ("fp64", True),
+                                                ("fp32k", True)])
+def test_batch_matches_single_source_runs(oracle_built, precision, two_step):
+    print("CASE batch", precision, "two-step" if two_step else "one-step", flush=True)
     from ptha_amd.batch import UnitSourceBatch
     u = _cfg()
     nx, ny = u["nx"], u["ny"]
-    nsteps = np.array([50, 80, 30], dtype=np.int32)
+    # odd counts and a sampling period that splits two-step rounds
+    nsteps = np.array([51, 80, 33], dtype=np.int32) if two_step else np.array([50, 80, 30], dtype=np.int32)
     every = 10
     b = UnitSourceBatch(nx, ny, u["lw"], u["ll"], 3, precision=precision)
+    if two_step:
+        b.set_launch(64, 16, 16 | (3 if precision == "fp32k" else 1))
     b.set_elevation(u["elev"])
     b.set_nsteps(nsteps)
     stages = []
@@ -67,3 +73,44 @@ def test_batch_matches_single_source_runs(oracle_built, precision):
             assert np.abs(got[0] - ref[0])[wet].max() <= 1e-5, k
             assert np.abs(series[:, 0] - np.array(ref_series)[:, 0]).max() <= 1e-5, k
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_two_step_friction_and_autotune_bitwise(precision):
+    """'leapfrog_linear_plus_nonlinear_friction' batch (Manning n = 0.03): two steps
+    per launch equal one step per launch bit for bit; running the autotuner first
+    changes nothing in the results."""
+    print("CASE batch llpnf two-step/autotune", precision, flush=True)
+    from ptha_amd.batch import UnitSourceBatch
+    u = _cfg()
+    nx, ny = u["nx"], u["ny"]
+    nsteps = np.array([41, 64, 27], dtype=np.int32)
+    runs = []
+    for mode in ["one", "two", "tuned"]:
+        b = UnitSourceBatch(nx, ny, u["lw"], u["ll"], 3, precision=precision,
+                            timestepping_method="leapfrog_linear_plus_nonlinear_friction")
+        b.set_manning_squared(np.full((ny, nx), 0.03 ** 2))
+        b.set_elevation(u["elev"])
+        b.set_nsteps(nsteps)
+        for k in range(3):
+            st = C.gaussian_on_sphere(u["x"], u["y"], *u["centres"][k], 60e3)
+            st[u["elev"] >= 0] = 0.0
+            b.set_source(k, st)
+        b.setup_point_gauges(u["gauges_xy"], every=7, max_samples=20)
+        dt = 20.0
+        if mode == "one":
+            b.set_launch(128, 16, 1)
+        elif mode == "two":
+            b.set_launch(128, 32, 17)
+        else:
+            shape, ms = b.autotune(dt, reps=1)
+            assert ms > 0 and len(shape) == 3
+        b.evolve(dt, 30)
+        b.evolve(dt, 50)
+        assert b.status() == (64, 0)
+        runs.append(([b.get_source(k) for k in range(3)], [b.gauge_series(k) for k in range(3)]))
+        b.close()
+    for other in runs[1:]:
+        for k in range(3):
+            assert np.array_equal(runs[0][0][k], other[0][k]), k
+            assert np.array_equal(runs[0][1][k], other[1][k]), k
