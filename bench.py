@@ -137,26 +137,37 @@ def read_traffic(name):
 
 
 def copy_probe(mib):
-    """A plain device-to-device copy of known size: the reference point that scales
-    rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes, and the achievable
+    """A plain device-to-device copy of known size (the HIP runtime's copy kernel,
+    called through the same runtime the backend uses): the reference point that
+    scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes, and the achievable
     streaming bandwidth of this GPU."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    import ctypes
+    from ptha_amd import backend as B
+    B.load()
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib * (1 << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(n)) == 0
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+        hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3)
+    hip.hipDeviceSynchronize()
     reps = 10
+    hip.hipEventRecord(e0, None)
     for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    log("copy probe: %d MiB read + %d MiB written per copy, %.3f ms, %.0f GB/s" % (mib, mib, ms,
-                                                                                2 * mib * (1 << 20) / ms / 1e6))
+        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(n), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    per = ms.value / reps
+    log("copy probe: %d MiB read + %d MiB written per copy, %.3f ms, %.0f GB/s" % (mib, mib, per, 2 * n / per / 1e6))
+    hip.hipFree(src)
+    hip.hipFree(dst)
 
 
 def secondary_legs(steps):
@@ -206,6 +217,88 @@ def secondary_legs(steps):
     return out
 
 
+def load_nested_tables(name="nested_c3_tables.npz"):
+    """Nesting geometry of the nested workload (what the reference's md%setup decides
+    for ptha_amd.configs.nested_c3; written by tests/golden/make_nested_tables.py)."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", name))
+    tables = json.loads(bytes(z["tables_json"]).decode())
+    arrays = {k: z[k] for k in z.files if k != "tables_json"}
+    return tables, arrays
+
+
+def nested_leg(prec, steps, warmup):
+    """Three-level nested multidomain on the GPU: Gcell-updates/s over all domains'
+    cells and sub-steps, wall time per global step, and the relative change of the
+    total priority-domain volume over the timed steps."""
+    from ptha_amd import configs as C
+    from ptha_amd.domain import Domain
+    from ptha_amd.multidomain import MultiDomain
+    tables, arrays = load_nested_tables()
+    cfg = C.nested_c3(scale=tables["scale"])
+    doms = []
+    cells = 0
+    for k, td in enumerate(tables["domains"]):
+        d = td["dims"]
+        dom = Domain(d["nx"], d["ny"], td["lw"], td["ll"], timestepping_method=tables["methods"][k], precision=prec,
+                     spherical=True, max_U_variables=(), exterior_cells_width=d["exterior_cells_width"])
+        dom.U = cfg["state"](dom.x, dom.y)
+        if tables["methods"][k] != "linear":
+            dom.set_manning_squared(np.full((d["ny"], d["nx"]), cfg["manning_n"] ** 2))
+        doms.append(dom)
+        cells += d["nx"] * d["ny"] * tables["trf"][k]
+    prio = [(arrays["pdi_%d" % k].astype(np.int32), arrays["pnp_%d" % k].astype(np.int32)) for k in range(len(doms))]
+    md = MultiDomain(doms, arrays["all_dx"], arrays["all_methods"], tables["trf"],
+                     [td["send"] for td in tables["domains"]], [td["recv"] for td in tables["domains"]], prio)
+    md.make_initial_conditions_consistent()
+    dt = tables["dt"]
+    md.evolve_steps(dt, warmup)
+    md.synchronize()
+    v0 = md.volumes()
+    t0 = time.perf_counter()
+    md.evolve_steps(dt, steps)
+    md.synchronize()
+    wall = time.perf_counter() - t0
+    v1 = md.volumes()
+    out = dict(gcell_per_s=round(cells * steps / wall / 1e9, 3), ms_per_global_step=round(1e3 * wall / steps, 3),
+               simulated_s_per_wall_s=round(dt * steps / wall, 1), cell_updates_per_global_step=cells,
+               relative_volume_change=float(abs(v1.sum() - v0.sum()) / v0.sum()))
+    md.close()
+    for d in doms:
+        d.close()
+    return out
+
+
+def nested_main(args):
+    res = {}
+    for prec in (["fp32k"] if args.no_fp64 else ["fp32k", "fp64"]):
+        log("nested multidomain", prec)
+        res[prec] = nested_leg(prec, args.steps, args.warmup)
+        log(prec, res[prec])
+    r = res["fp32k"]
+    tables, _ = load_nested_tables()
+    out = {
+        "metric": METRIC,
+        "value": r["gcell_per_s"],
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_global_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "three-level nested multidomain: 'linear' parent 3000x1800 at 2 arcmin, "
+                               "'leapfrog_linear_plus_nonlinear_friction' child at 1/3, 'rk2' grandchild at 1/15 "
+                               "(shore inside), FP32 + Kahan-compensated stage",
+                   "dt_s": tables["dt"], "timestepping_refinement_factor": tables["trf"],
+                   "grids": [[td["dims"]["nx"], td["dims"]["ny"]] for td in tables["domains"]]},
+        "nested": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +311,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=str, default="",
                     help="block_x,rows_per_block,variant of the step kernel (default: the batch's autotuned shape)")
+    ap.add_argument("--workload", choices=["unit_sources", "nested"], default="unit_sources",
+                    help="unit_sources (the default measurement) or the three-level nested multidomain")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the rk2 beach and single 2048^2 grid throughput legs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -228,6 +323,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path; rerun with --gpus 1",
               file=sys.stderr)
         sys.exit(3)
+
+    if args.workload == "nested":
+        return nested_main(args)
 
     from ptha_amd import configs as C
     from ptha_amd.domain import build_metrics

@@ -222,3 +222,56 @@ def nested_small_state(x, y):
     U[3] = elev
     U[0] = np.maximum(0.0, elev) + hump * (elev < 0)
     return U
+
+
+def nested_c3(scale=1, seed=20261202):
+    """The three-level nested multidomain workload: a 'linear' parent over lon
+    100-200E, lat 60S-0 at 2 arcmin (3000 x 1800), a
+    'leapfrog_linear_plus_nonlinear_friction' child at 1/3 of its cell size (600 x 600,
+    Manning n = 0.03) and an 'rk2' grandchild at 1/5 of the child's (1000 x 1000, n =
+    0.03), both on the eastern shelf; power-law bathymetry (slope -2, mean -4000 m,
+    std 1000 m) with a shelf ramp over lon 191-197 to +50 m land (the shore lies
+    inside the grandchild); an elliptical Gaussian uplift of 3 m peak, sigma 200 km
+    north-south by 50 km east-west, at lon 190, lat 26.5S; 6 h simulated.
+
+    ``scale`` > 1 coarsens every grid by that factor (same extents) for quick runs.
+    Returns the domain list for a multidomain setup and ``state(x, y)``, which gives
+    the initial U of any domain from its cell-centre coordinates; the parent's
+    random field is interpolated bilinearly for the nested grids.
+    """
+    nxp, nyp = 3000 // scale, 1800 // scale
+    llp, lwp = (100.0, -60.0), (100.0, 60.0)
+    field = power_law_bathymetry(nxp, nyp, seed, shelf=False)
+    xp, yp = cell_centres(nxp, nyp, lwp, llp)
+
+    def elevation(x, y):
+        # bilinear interpolation of the parent's random field, then the shelf ramp
+        fx = np.clip((np.asarray(x) - xp[0]) / (xp[1] - xp[0]), 0, nxp - 1 - 1e-9)
+        fy = np.clip((np.asarray(y) - yp[0]) / (yp[1] - yp[0]), 0, nyp - 1 - 1e-9)
+        i0, j0 = np.floor(fx).astype(int), np.floor(fy).astype(int)
+        tx, ty = fx - i0, fy - j0
+        F = (field[np.ix_(j0, i0)] * np.outer(1 - ty, 1 - tx) + field[np.ix_(j0, i0 + 1)] * np.outer(1 - ty, tx) +
+             field[np.ix_(j0 + 1, i0)] * np.outer(ty, 1 - tx) + field[np.ix_(j0 + 1, i0 + 1)] * np.outer(ty, tx))
+        w = np.clip((np.asarray(x) - 191.0) / 6.0, 0.0, 1.0)[None, :]
+        return F * (1.0 - w) + 50.0 * w
+
+    def state(x, y):
+        X, Y = np.meshgrid(x, y)
+        U = np.zeros((4,) + X.shape)
+        U[3] = elevation(x, y)
+        dxm = np.radians(X - 190.0) * RADIUS_EARTH * np.cos(np.radians(Y))
+        dym = np.radians(Y + 26.5) * RADIUS_EARTH
+        uplift = 3.0 * np.exp(-0.5 * ((dxm / 50e3) ** 2 + (dym / 200e3) ** 2))
+        U[0] = np.where(U[3] < 0.0, uplift, U[3])
+        return U
+
+    a2 = (2.0 / 60.0) * scale
+    domains = [
+        dict(lw=lwp, ll=llp, nx=nxp, ny=nyp, method="linear"),
+        dict(parent=0, ll=(192.0, -30.0), ur=(192.0 + 600 * a2 / 3, -30.0 + 600 * a2 / 3), dx_refinement_factor=3,
+             method="leapfrog_linear_plus_nonlinear_friction"),
+        dict(parent=1, ll=(195.5, -27.5), ur=(195.5 + 1000 * a2 / 15, -27.5 + 1000 * a2 / 15),
+             dx_refinement_factor=5, method="rk2"),
+    ]
+    return dict(domains=domains, state=state, elevation=elevation, manning_n=0.03, duration=6 * 3600.0,
+                spherical=True)

@@ -960,6 +960,14 @@ int batch_launch(swh_batch *b, LFArgs<T> &a, int fric_mode, int n_active, int c)
     return SWH_OK;
 }
 
+// The step arguments with buffer c as the input state.
+template <typename T>
+const LFArgs<T> &batch_cur_args(swh_batch *b, LFArgs<T> &a, int c)
+{
+    a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
+    return a;
+}
+
 // One launch advancing slots [first, first + n) by two steps (reads buffer c).
 template <typename T, bool KAHAN>
 int batch_launch2(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, int c)
@@ -988,21 +996,27 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         rc = batch_sample<T, KAHAN>(b, batch_active(b, -1), b->cur);
         if (rc) return rc;
     }
-    const bool can2 = b->two_step && fric_mode <= 1;
+    const bool can2 = b->two_step && fric_mode <= 1 && (long long)b->g->plane_elems() < (1LL << 31);
     for (int k = 0; k < nsteps;) {
         const int n_active = batch_active(b, b->step);
         if (n_active == 0) break;
         const int c = b->cur, o = 1 - c;
         const bool sample_next = b->ng && ((b->step + 1) % b->every) == 0;
         int n_last;   // sources that took the last step of this round
-        if (can2 && k + 1 < nsteps && !sample_next) {
-            // sources with at least two steps left take both in one launch; the
-            // ones with one step left take it in a one-step launch
-            const int n2 = batch_active(b, b->step + 1);
-            if (n2 > 0) {
-                rc = batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c);
-                if (rc) return rc;
+        const int n2 = can2 ? batch_active(b, b->step + 1) : 0;   // sources with two or more steps left
+        if (n2 > 0 && k + 1 < nsteps) {
+            // those take both steps in one launch; the ones with one step left take
+            // it in a one-step launch. A gauge sample due after the first step is
+            // taken one step ahead from the current state (same operations).
+            if (sample_next && (b->step + 1) / b->every < b->gmax) {
+                a.src_base = 0;
+                SWH_HIP(launch_gauge_step1<T, KAHAN>(batch_cur_args(b, a, c), b->g->coriolis, fric_mode,
+                                                     b->g->truely_linear, b->g->lfc != 0.0, n_active, b->ng,
+                                                     b->d_gidx, b->d_gval, b->gmax,
+                                                     (int)((b->step + 1) / b->every), b->g->stream));
             }
+            rc = batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c);
+            if (rc) return rc;
             if (n_active > n2) {
                 a.src_base = n2;
                 rc = batch_launch<T, KAHAN>(b, a, fric_mode, n_active - n2, c);
@@ -1079,7 +1093,7 @@ int batch_autotune(swh_batch *b, double dt, int reps, int32_t *best, double *bes
                 rc = timed(false, ms);
                 if (!rc && ms < bms) { bms = ms; bb = bxs[ib]; br = rws[ir]; bv = v; }
             }
-    if (!rc && fric_mode <= 1 && n2 > 0) {
+    if (!rc && fric_mode <= 1 && n2 > 0 && (long long)g->plane_elems() < (1LL << 31)) {
         const int v1 = bv;
         b->variant = v1 & 7; b->src_fast = (v1 >> 3) & 1;
         for (int ib = 0; ib < 3 && rc == SWH_OK; ib++)

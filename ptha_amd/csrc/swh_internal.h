@@ -89,6 +89,12 @@ template <typename T, bool KAHAN>
 hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                             int linear_friction, int nsrc, int block_x, hipStream_t s);
 
+// Gauge values after one step, from the current state (batched runs).
+template <typename T, bool KAHAN>
+hipError_t launch_gauge_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                              int linear_friction, int nsrc, int n, const long long *idx, double *values, int gmax,
+                              int sample, hipStream_t s);
+
 template <typename T>
 hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext,
                                  const double *dle, const double *dbe, double dt, double time_after,

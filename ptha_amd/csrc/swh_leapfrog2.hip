@@ -8,9 +8,10 @@
 // written once per two steps, halving the traffic per cell-update.
 //
 // Work split: a wavefront owns 61 consecutive columns and holds 64 (one halo column
-// to the west, two to the east); each lane owns one column and marches up a strip
-// of rows. At row m it forms step 1 at row m (stage, UH) and m-1 (VH), then step 2
-// at row m-1 (stage, UH) and m-2 (VH), and writes row m-2. East/west neighbours come
+// to the west, two to the east; the westmost lane reads its own west neighbour's
+// UH from memory); each lane owns one column and marches up a strip of rows. At
+// row m it forms step 1 at row m (stage, UH) and m-1 (VH), then step 2 at row m-1
+// (stage, UH) and m-2 (VH), and writes row m-2. East/west neighbours come
 // from the adjacent lanes (cross-lane permutes, no LDS tile, no barriers); the
 // stencil reaches one column each way per step, which the 3 halo lanes cover.
 // Rows below/above the strip are recomputed by the strip itself (3 extra rows).
@@ -22,41 +23,77 @@ namespace {
 
 constexpr int kOwn = 61;   // columns a wavefront writes (lanes 1..61)
 
-template <typename T>
-__device__ __forceinline__ T shl1(T v) { return __shfl_down(v, 1); }   // value of lane + 1
-template <typename T>
-__device__ __forceinline__ T shr1(T v) { return __shfl_up(v, 1); }     // value of lane - 1
+// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1, VALU moves without the
+// LDS crossbar); the lane at the open end keeps its own value.
+__device__ __forceinline__ int dpp_shr1_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shl1_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
 
-// New stage (include.f90:97-113); no stage boundary in this kernel.
-template <typename T, bool KAHAN>
-__device__ __forceinline__ void stage_cell(const LFArgs<T> &a, int ii, int jj, T s, T c, T uh, T uhm, T vh, T vhm,
-                                           T &sn, T &cn)
+__device__ __forceinline__ float shr1(float v) { return __int_as_float(dpp_shr1_i(__float_as_int(v))); }
+__device__ __forceinline__ float shl1(float v) { return __int_as_float(dpp_shl1_i(__float_as_int(v))); }
+__device__ __forceinline__ double shr1(double v)
 {
-    const bool interior = ii > 0 && ii < a.nx - 1 && jj > 0 && jj < a.ny - 1;
-    if (interior) {
-        const T delta = a.rc.sa[jj] * ((uh - uhm) * a.rc.dle[ii] + (vh * a.rc.dbe[jj + 1] - vhm * a.rc.dbe[jj]));
-        if (KAHAN) {
-            const T y = -delta - c;
-            const T t = s + y;
-            cn = (t - s) - y;
-            sn = t;
-        } else {
-            sn = s - delta;
-            cn = T(0);
-        }
+    const long long u = __double_as_longlong(v);
+    const int lo = dpp_shr1_i((int)(u & 0xffffffffLL)), hi = dpp_shr1_i((int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double shl1(double v)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = dpp_shl1_i((int)(u & 0xffffffffLL)), hi = dpp_shl1_i((int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Per-row coefficients (uniform across the wavefront), loaded at a clamped row so
+// the loads are always in bounds; 'interior' / 'valid' say whether the row updates.
+template <typename T>
+struct RowCo {
+    T sa, cu, cv, fu, fv, dbe_lo, dbe_hi;   // dbe_lo = dbe(j), dbe_hi = dbe(j+1)
+    bool interior, valid;                   // stage row (1..ny-2), momentum row (0..ny-2)
+};
+
+template <typename T>
+__device__ __forceinline__ RowCo<T> row_co(const LFArgs<T> &a, int r)
+{
+    const int rc = min(max(r, 0), a.ny - 1);
+    RowCo<T> q;
+    q.sa = a.rc.sa[rc];
+    q.cu = a.rc.cu[rc];
+    q.cv = a.rc.cv[rc];
+    q.fu = a.rc.fu[rc];
+    q.fv = a.rc.fv[rc];
+    q.dbe_lo = a.rc.dbe[rc];
+    q.dbe_hi = a.rc.dbe[rc + 1];
+    q.interior = r > 0 && r < a.ny - 1;
+    q.valid = r >= 0 && r < a.ny - 1;
+    return q;
+}
+
+// New stage (include.f90:97-113), computed on every lane and kept where the cell
+// is interior; no stage boundary in this kernel.
+template <typename T, bool KAHAN>
+__device__ __forceinline__ void stage_cell(bool interior, T sa, T dle_i, T dbe_lo, T dbe_hi, T s, T c, T uh, T uhm,
+                                           T vh, T vhm, T &sn, T &cn)
+{
+    const T delta = sa * ((uh - uhm) * dle_i + (vh * dbe_hi - vhm * dbe_lo));
+    T t, cc;
+    if (KAHAN) {
+        const T y = -delta - c;
+        t = s + y;
+        cc = (t - s) - y;
     } else {
-        sn = s;
-        cn = c;
+        t = s - delta;
+        cc = T(0);
     }
+    sn = interior ? t : s;
+    cn = interior ? cc : c;
 }
 
 // UH at (ii + 1/2, j) from the new stage of cells ii, ii+1 and the pre-step VH of
 // rows j-1 (vhm0, vhm1 at ii, ii+1) and j (vh0, vh1).
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-__device__ __forceinline__ T mom_u(const LFArgs<T> &a, int ii, int j, T uh_old, T s0, T c0, T s1, T c1, T e0, T e1,
-                                   T vhm0, T vhm1, T vh0, T vh1)
+__device__ __forceinline__ T mom_u(const LFArgs<T> &a, bool valid, T cu, T fu, T dle_ip1, T fwu, T uh_old, T s0,
+                                   T c0, T s1, T c1, T e0, T e1, T vhm0, T vhm1, T vh0, T vh1)
 {
-    if (!(ii >= 0 && ii < a.nx - 1 && j >= 0 && j < a.ny - 1)) return uh_old;
     const T MIN = T(kMinDepth), HALF = T(0.5), msl = a.msl;
     T h_iph, wet_iph;
     if (TL) {
@@ -68,32 +105,30 @@ __device__ __forceinline__ T mom_u(const LFArgs<T> &a, int ii, int j, T uh_old, 
     }
     T ds_i = s1 - s0;
     if (KAHAN) ds_i = ds_i - (c1 - c0);
-    T uh_new = uh_old * wet_iph - a.rc.cu[j] * h_iph * ds_i * a.rc.dle[ii + 1];
+    T uh_new = uh_old * wet_iph - cu * h_iph * ds_i * dle_ip1;
     if (COR || FRIC) {
         const T vh_jmh = HALF * (vhm0 + vhm1);
         const T vh_jph = HALF * (vh0 + vh1);
         if (COR) {
-            const T dhc = a.rc.fu[j] * (h_iph > T(0) ? T(1) : T(0));
+            const T dhc = fu * (h_iph > T(0) ? T(1) : T(0));
             uh_new = uh_new + dhc * (vh_jmh + vh_jph);
         }
         if (FRIC) {
-            const T fwu = a.fwU[(long long)j * a.pitch + ii];
             const T vbar = HALF * (vh_jmh + vh_jph);
             const T fmu = T(1) / (T(1) + a.dt_lfc + a.dt * fwu * (sqrt(uh_old * uh_old + vbar * vbar)));
             uh_new = fmu * uh_new;
         }
     }
     if (LINF) uh_new = a.lin_fric_mult * uh_new;
-    return uh_new;
+    return valid ? uh_new : uh_old;
 }
 
 // VH at (ii, j + 1/2) from the new stage of rows j, j+1 and the pre-step UH of
 // row j (uh_i at ii, uh_im1 at ii-1) and row j+1 (uhu_i, uhu_im1).
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-__device__ __forceinline__ T mom_v(const LFArgs<T> &a, int ii, int j, T vh_old, T s0, T c0, T sup, T cup, T e0,
-                                   T eu, T uh_i, T uh_im1, T uhu_i, T uhu_im1)
+__device__ __forceinline__ T mom_v(const LFArgs<T> &a, bool valid, T cv, T fv, T dbe_hi, T fwv, T vh_old, T s0, T c0,
+                                   T sup, T cup, T e0, T eu, T uh_i, T uh_im1, T uhu_i, T uhu_im1)
 {
-    if (!(ii >= 0 && ii < a.nx - 1 && j >= 0 && j < a.ny - 1)) return vh_old;
     const T MIN = T(kMinDepth), HALF = T(0.5), msl = a.msl;
     T h_jph, wet_jph;
     if (TL) {
@@ -105,23 +140,22 @@ __device__ __forceinline__ T mom_v(const LFArgs<T> &a, int ii, int j, T vh_old, 
     }
     T ds_j = sup - s0;
     if (KAHAN) ds_j = ds_j - (cup - c0);
-    T vh_new = vh_old * wet_jph - a.rc.cv[j] * h_jph * ds_j * a.rc.dbe[j + 1];
+    T vh_new = vh_old * wet_jph - cv * h_jph * ds_j * dbe_hi;
     if (COR || FRIC) {
         const T uh_ij = HALF * (uh_i + uh_im1);
         const T uh_ijp1 = HALF * (uhu_i + uhu_im1);
         if (COR) {
-            const T dhcj = a.rc.fv[j] * (h_jph > T(0) ? T(1) : T(0));
+            const T dhcj = fv * (h_jph > T(0) ? T(1) : T(0));
             vh_new = vh_new - dhcj * (uh_ij + uh_ijp1);
         }
         if (FRIC) {
-            const T fwv = a.fwV[(long long)j * a.pitch + ii];
             const T ubar = HALF * (uh_ij + uh_ijp1);
             const T fmv = T(1) / (T(1) + a.dt_lfc + a.dt * fwv * (sqrt(vh_old * vh_old + ubar * ubar)));
             vh_new = fmv * vh_new;
         }
     }
     if (LINF) vh_new = a.lin_fric_mult * vh_new;
-    return vh_new;
+    return valid ? vh_new : vh_old;
 }
 
 }  // namespace
@@ -148,17 +182,25 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
     T *__restrict__ UHo = a.UHo + so;
     T *__restrict__ VHo = a.VHo + so;
     const bool west = col == 0;                          // UH(i-1) clamps to UH(0) (include.f90:319-322)
+    // per-lane constants: edge lengths and where the column updates
+    const T dle_i = a.rc.dle[min(max(col, 0), a.nx)];
+    const T dle_ip1 = a.rc.dle[min(max(col + 1, 0), a.nx)];
+    const bool col_int = col > 0 && col < a.nx - 1;      // stage: interior column
+    const bool col_mom = col >= 0 && col < a.nx - 1;     // momentum: UH/VH of this column update
+    const int cf = min(max(col, 0), a.nx - 1);           // friction-work column
 
     // registers carried between rows (named by the row they belong to, relative to m)
     T vh0_p = T(0), vh0_p_r = T(0), uh0_p = T(0), uh0_p_l = T(0), e_p = T(0), e_p_r = T(0), e_pp = T(0);
-    T s1_p = T(0), c1_p = T(0), s1_p_r = T(0), c1_p_r = T(0);
+    T s1_p = T(0), c1_p = T(0);
     T uh1_p = T(0), uh1_p_l = T(0), uh1_pp = T(0), uh1_pp_l = T(0);
     T vh1_pp = T(0), vh1_pp_r = T(0);
     T s2_p = T(0), c2_p = T(0), uh2_p = T(0);
+    RowCo<T> q_p = row_co(a, j0 - 2), q_pp = row_co(a, j0 - 3);
+    T fw_u_p = T(0), fw_v_p = T(0);
     {
         // pre-step VH/UH/E of row j0-2 (feeds only rows the strip does not write)
         const int r = min(max(j0 - 2, 0), a.ny - 1);
-        const long long o = (long long)r * a.pitch + ci;
+        const int o = r * a.pitch + ci;
         vh0_p = VH[o];
         uh0_p = UH[o];
         e_p = a.E[o];
@@ -167,45 +209,62 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
         e_p_r = shl1(e_p);
     }
 
+    // offsets within one source's plane fit in 32 bits (uniform base + per-lane offset)
     for (int m = j0 - 1; m <= j1 + 1; m++) {
         const int mr = min(max(m, 0), a.ny - 1);
-        const long long o = (long long)mr * a.pitch + ci;
+        const int o = mr * a.pitch + ci;
         const T s0 = S[o];
         const T c0 = KAHAN ? C[o] : T(0);
         const T uh0 = UH[o];
         const T vh0 = VH[o];
         const T e = a.E[o];
-        const T uh0_l = west ? uh0 : shr1(uh0);
+        const RowCo<T> q = row_co(a, m);
+        // UH(i-1): from the lane to the west; lane 0 has none and reads it itself
+        const T uh0_mem = (lane == 0 && col > 0) ? UH[o - 1] : uh0;
+        const T uh0_sh = shr1(uh0);
+        const T uh0_l = west ? uh0 : (lane == 0 ? uh0_mem : uh0_sh);
         const T vh0_r = shl1(vh0), e_r = shl1(e);
         // VH of the row below, clamped to row 0 at the southern edge (include.f90:295-308)
         const T vhb = m > 0 ? vh0_p : vh0, vhb_r = m > 0 ? vh0_p_r : vh0_r;
+        // friction work: UH at row m (and m-1 from the last row), VH at row m-1 (and m-2)
+        T fw_u1 = T(0), fw_v1 = T(0);
+        if (FRIC) {
+            fw_u1 = a.fwU[mr * a.pitch + cf];
+            fw_v1 = a.fwV[min(max(m - 1, 0), a.ny - 1) * a.pitch + cf];
+        }
+        const T fw_u2 = fw_u_p, fw_v2 = fw_v_p;
 
         // step 1: stage and UH at row m
         T s1, c1;
-        stage_cell<T, KAHAN>(a, col, m, s0, c0, uh0, uh0_l, vh0, vhb, s1, c1);
+        stage_cell<T, KAHAN>(col_int && q.interior, q.sa, dle_i, q.dbe_lo, q.dbe_hi, s0, c0, uh0, uh0_l, vh0, vhb, s1,
+                             c1);
         const T s1_r = shl1(s1);
         const T c1_r = KAHAN ? shl1(c1) : T(0);
-        const T uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m, uh0, s1, c1, s1_r, c1_r, e, e_r, vhb, vhb_r,
-                                                            vh0, vh0_r);
+        const T uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q.valid, q.cu, q.fu, dle_ip1, fw_u1, uh0, s1,
+                                                            c1, s1_r, c1_r, e, e_r, vhb, vhb_r, vh0, vh0_r);
         const T uh1_l = west ? uh1 : shr1(uh1);
         // step 1: VH at row m-1
-        const T vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m - 1, vh0_p, s1_p, c1_p, s1, c1, e_p, e, uh0_p,
-                                                            uh0_p_l, uh0, uh0_l);
+        const T vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q_p.valid, q_p.cv, q_p.fv, q_p.dbe_hi, fw_v1,
+                                                            vh0_p, s1_p, c1_p, s1, c1, e_p, e, uh0_p, uh0_p_l, uh0,
+                                                            uh0_l);
         const T vh1_r = shl1(vh1);
         // step 2: stage and UH at row m-1 (VH of row m-2 clamped to row 0 at the southern edge)
         const T vhb1 = m - 1 > 0 ? vh1_pp : vh1, vhb1_r = m - 1 > 0 ? vh1_pp_r : vh1_r;
         T s2, c2;
-        stage_cell<T, KAHAN>(a, col, m - 1, s1_p, c1_p, uh1_p, uh1_p_l, vh1, vhb1, s2, c2);
+        stage_cell<T, KAHAN>(col_int && q_p.interior, q_p.sa, dle_i, q_p.dbe_lo, q_p.dbe_hi, s1_p, c1_p, uh1_p,
+                             uh1_p_l, vh1, vhb1, s2, c2);
         const T s2_r = shl1(s2);
         const T c2_r = KAHAN ? shl1(c2) : T(0);
-        const T uh2 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m - 1, uh1_p, s2, c2, s2_r, c2_r, e_p, e_p_r, vhb1,
-                                                            vhb1_r, vh1, vh1_r);
+        const T uh2 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q_p.valid, q_p.cu, q_p.fu, dle_ip1, fw_u2,
+                                                            uh1_p, s2, c2, s2_r, c2_r, e_p, e_p_r, vhb1, vhb1_r, vh1,
+                                                            vh1_r);
         // step 2: VH at row m-2
-        const T vh2 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col, m - 2, vh1_pp, s2_p, c2_p, s2, c2, e_pp, e_p,
-                                                            uh1_pp, uh1_pp_l, uh1_p, uh1_p_l);
+        const T vh2 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q_pp.valid, q_pp.cv, q_pp.fv, q_pp.dbe_hi,
+                                                            fw_v2, vh1_pp, s2_p, c2_p, s2, c2, e_pp, e_p, uh1_pp,
+                                                            uh1_pp_l, uh1_p, uh1_p_l);
         const int k = m - 2;
         if (own && k >= j0 && k < j1) {
-            const long long ow = (long long)k * a.pitch + col;
+            const int ow = k * a.pitch + col;
             So[ow] = s2_p;
             if (KAHAN) Co[ow] = c2_p;
             UHo[ow] = uh2_p;
@@ -215,13 +274,110 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
         s2_p = s2; c2_p = c2; uh2_p = uh2;
         uh1_pp = uh1_p; uh1_pp_l = uh1_p_l; uh1_p = uh1; uh1_p_l = uh1_l;
         vh1_pp = vh1; vh1_pp_r = vh1_r;
-        s1_p = s1; c1_p = c1; s1_p_r = s1_r; c1_p_r = c1_r;
+        s1_p = s1; c1_p = c1;
         vh0_p = vh0; vh0_p_r = vh0_r; uh0_p = uh0; uh0_p_l = uh0_l;
         e_pp = e_p; e_p = e; e_p_r = e_r;
+        q_pp = q_p; q_p = q;
+        fw_u_p = fw_u1; fw_v_p = fw_v1;
     }
-    (void)s1_p_r;
-    (void)c1_p_r;
 }
+
+// Gauge values one step ahead: [stage, UH, VH] after one step at each gauge cell,
+// computed from the current state with the same per-cell operations as the step
+// kernels (so a two-step launch can pass over a sampling step). One lane per
+// (gauge, source); values[((slot * gmax + sample) * 3 + var) * n + g].
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, const long long *__restrict__ idx,
+                                                         double *values, int gmax, int sample)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int slot = blockIdx.y + a.src_base;
+    const long long so = (long long)slot * a.src_stride;
+    const T *__restrict__ S = a.S + so;
+    const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
+    const T *__restrict__ UH = a.UH + so;
+    const T *__restrict__ VH = a.VH + so;
+    const T *__restrict__ E = a.E;
+    const int i = (int)(idx[g] % a.pitch), j = (int)(idx[g] / a.pitch);
+    const int ip = min(i + 1, a.nx - 1), im = max(i - 1, 0), jp = min(j + 1, a.ny - 1), jm = max(j - 1, 0);
+    auto at = [&](int ii, int jj) { return (long long)jj * a.pitch + ii; };
+    auto cval = [&](long long o) { return KAHAN ? C[o] : T(0); };
+    // VH of the row below, clamped to row 0 at the southern edge
+    const T vhb_i = j > 0 ? VH[at(i, jm)] : VH[at(i, j)];
+    const T vhb_ip = j > 0 ? VH[at(ip, jm)] : VH[at(ip, j)];
+    const RowCo<T> q = row_co(a, j), qu = row_co(a, jp);
+    const T dle_i = a.rc.dle[i], dle_ip = a.rc.dle[ip], dle_ip1 = a.rc.dle[i + 1];
+    const bool ci_int = i > 0 && i < a.nx - 1, cip_int = ip > 0 && ip < a.nx - 1;
+    // new stage at (i, j), (i+1, j), (i, j+1); UH(i-1) clamps to UH(0) at the west edge
+    T s_c, c_c, s_e, c_e, s_n, c_n;
+    {
+        const long long o = at(i, j);
+        stage_cell<T, KAHAN>(ci_int && q.interior, q.sa, dle_i, q.dbe_lo, q.dbe_hi, S[o], cval(o), UH[o],
+                             UH[at(im, j)], VH[o], vhb_i, s_c, c_c);
+    }
+    {
+        const long long o = at(ip, j);
+        stage_cell<T, KAHAN>(cip_int && q.interior, q.sa, dle_ip, q.dbe_lo, q.dbe_hi, S[o], cval(o), UH[o],
+                             UH[at(i, j)], VH[o], vhb_ip, s_e, c_e);
+    }
+    {
+        const long long o = at(i, jp);
+        stage_cell<T, KAHAN>(ci_int && qu.interior, qu.sa, dle_i, qu.dbe_lo, qu.dbe_hi, S[o], cval(o), UH[o],
+                             UH[at(im, jp)], VH[o], VH[at(i, j)], s_n, c_n);
+    }
+    const bool mom = i < a.nx - 1 && q.valid;
+    const T fwu = FRIC ? a.fwU[at(i, j)] : T(0), fwv = FRIC ? a.fwV[at(i, j)] : T(0);
+    const T uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, mom, q.cu, q.fu, dle_ip1, fwu, UH[at(i, j)], s_c, c_c, s_e,
+                                                        c_e, E[at(i, j)], E[at(ip, j)], vhb_i, vhb_ip, VH[at(i, j)],
+                                                        VH[at(ip, j)]);
+    const T vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, mom, q.cv, q.fv, q.dbe_hi, fwv, VH[at(i, j)], s_c, c_c, s_n,
+                                                        c_n, E[at(i, j)], E[at(i, jp)], UH[at(i, j)], UH[at(im, j)],
+                                                        UH[at(i, jp)], UH[at(im, jp)]);
+    double *v = values + ((long long)slot * gmax + sample) * 3 * n;
+    v[g] = KAHAN ? (double)s_c - (double)c_c : (double)s_c;
+    v[n + g] = (double)uh1;
+    v[2 * n + g] = (double)vh1;
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+static hipError_t launch_g1(const LFArgs<T> &a, int nsrc, int n, const long long *idx, double *values, int gmax,
+                            int sample, hipStream_t s)
+{
+    hipLaunchKernelGGL((gauge_step1_kernel<T, KAHAN, COR, FRIC, TL, LINF>), dim3((n + 63) / 64, nsrc), dim3(64), 0, s,
+                       a, n, idx, values, gmax, sample);
+    return hipGetLastError();
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC>
+static hipError_t g1_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int n, const long long *idx, double *values,
+                        int gmax, int sample, hipStream_t s)
+{
+    if (tl) return linf ? launch_g1<T, KAHAN, COR, FRIC, true, true>(a, nsrc, n, idx, values, gmax, sample, s)
+                        : launch_g1<T, KAHAN, COR, FRIC, true, false>(a, nsrc, n, idx, values, gmax, sample, s);
+    return linf ? launch_g1<T, KAHAN, COR, FRIC, false, true>(a, nsrc, n, idx, values, gmax, sample, s)
+                : launch_g1<T, KAHAN, COR, FRIC, false, false>(a, nsrc, n, idx, values, gmax, sample, s);
+}
+
+template <typename T, bool KAHAN>
+hipError_t launch_gauge_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                              int linear_friction, int nsrc, int n, const long long *idx, double *values, int gmax,
+                              int sample, hipStream_t s)
+{
+    if (friction_mode > 1 || a.bc_mask || n <= 0 || nsrc <= 0) return hipErrorInvalidValue;
+    const int linf = (friction_mode == 0) && linear_friction;
+    if (coriolis) {
+        return friction_mode ? g1_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s)
+                             : g1_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s);
+    }
+    return friction_mode ? g1_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s)
+                         : g1_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s);
+}
+
+template hipError_t launch_gauge_step1<double, false>(const LFArgs<double> &, int, int, int, int, int, int,
+                                                      const long long *, double *, int, int, hipStream_t);
+template hipError_t launch_gauge_step1<float, true>(const LFArgs<float> &, int, int, int, int, int, int,
+                                                    const long long *, double *, int, int, hipStream_t);
 
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 static hipError_t launch_lf2(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)
@@ -245,7 +401,8 @@ template <typename T, bool KAHAN>
 hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                             int linear_friction, int nsrc, int block_x, hipStream_t s)
 {
-    if (friction_mode > 1 || a.bc_mask || a.maxq_mask || (block_x % 64) || block_x < 64 || block_x > 256)
+    if (friction_mode > 1 || a.bc_mask || a.maxq_mask || (block_x % 64) || block_x < 64 || block_x > 256 ||
+        (long long)a.pitch * a.ny >= (1LL << 31))
         return hipErrorInvalidValue;
     const int linf = (friction_mode == 0) && linear_friction;
     if (coriolis) {
