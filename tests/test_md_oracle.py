@@ -94,3 +94,23 @@ def test_mass_conservation_and_layout(tmp_path):
             rb = doms[b["neighbour_index"] - 1]["recv"][b["neighbour_comms_index"] - 1]
             assert rb["buffer_size"] == b["buffer_size"]
             assert (rb["neighbour_index"], rb["neighbour_comms_index"]) == (k + 1, c + 1)
+
+
+def test_nested_tables_fixture_matches_setup():
+    """Structure of the committed nesting tables of the nested workload
+    (tests/golden/nested_c3_tables.npz, written from the reference's md%setup):
+    full-size parent, priority arrays over the three domains, every domain has
+    priority cells, and no receive box holds its own domain's priority cells."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "nested_c3_tables.npz"))
+    tables = json.loads(bytes(z["tables_json"]).decode())
+    assert tables["scale"] == 1 and len(tables["domains"]) == 3
+    assert [td["dims"]["nx"] for td in tables["domains"]][0] == 3000
+    for k, td in enumerate(tables["domains"]):
+        pdi = z["pdi_%d" % k]
+        assert pdi.shape == (td["dims"]["ny"], td["dims"]["nx"])
+        assert set(np.unique(pdi)) <= {1, 2, 3}
+        assert (pdi == k + 1).any()
+        for b in td["recv"]:
+            (iL, iU), (jL, jU), _ = np.asarray(b["inds"])
+            # a receive box never holds this domain's own priority cells
+            assert not (pdi[jL - 1:jU, iL - 1:iU] == k + 1).any()
