@@ -137,37 +137,18 @@ def read_traffic(name):
 
 
 def copy_probe(mib):
-    """A plain device-to-device copy of known size (the HIP runtime's copy kernel,
-    called through the same runtime the backend uses): the reference point that
-    scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes, and the achievable
-    streaming bandwidth of this GPU."""
+    """A plain device copy kernel of known size (swh_probe_copy): the reference
+    point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes, and
+    the streaming bandwidth this GPU reaches with a copy."""
     import ctypes
     from ptha_amd import backend as B
-    B.load()
-    hip = ctypes.CDLL("libamdhip64.so")
+    lib = B.load()
+    ms = ctypes.c_double()
     n = mib * (1 << 20)
-    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
-    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
-    assert hip.hipMemset(src, 1, ctypes.c_size_t(n)) == 0
-    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-    hip.hipEventCreate(ctypes.byref(e0))
-    hip.hipEventCreate(ctypes.byref(e1))
-    for _ in range(3):
-        hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3)
-    hip.hipDeviceSynchronize()
-    reps = 10
-    hip.hipEventRecord(e0, None)
-    for _ in range(reps):
-        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(n), 3, None)
-    hip.hipEventRecord(e1, None)
-    hip.hipEventSynchronize(e1)
-    ms = ctypes.c_float()
-    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-    per = ms.value / reps
-    log("copy probe: %d MiB read + %d MiB written per copy, %.3f ms, %.0f GB/s" % (mib, mib, per, 2 * n / per / 1e6))
-    hip.hipFree(src)
-    hip.hipFree(dst)
+    B.check(lib.swh_probe_copy(n, 10, ctypes.byref(ms)), "swh_probe_copy")
+    log("copy probe: %d MiB read + %d MiB written per copy, %.3f ms, %.0f GB/s" % (mib, mib, ms.value,
+                                                                                2 * n / ms.value / 1e6))
+    return 2 * n / ms.value / 1e6
 
 
 def secondary_legs(steps):

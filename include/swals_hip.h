@@ -221,6 +221,13 @@ int swh_batch_synchronize(swh_batch *b);
  * device time of the region in milliseconds (for throughput measurement). */
 int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elapsed_ms);
 
+/* Measurement utility (no counterpart in the reference): a plain device copy
+ * kernel of 'bytes' (multiple of 16) between two fresh buffers, repeated reps times
+ * after one warm-up; returns the average time per copy. It is the known-size
+ * reference for rocprofv3 FETCH_SIZE / WRITE_SIZE and the streaming-bandwidth
+ * ceiling quoted next to the roofline. */
+int swh_probe_copy(int64_t bytes, int32_t reps, double *ms_per_copy);
+
 /* ---------------------------------------------------------------------------
  * Multidomain nesting on one GPU (multidomain_mod.f90, nested_grid_comms_mod.f90).
  *

@@ -101,6 +101,7 @@ SYMBOLS = [
     ("swh_adjoint_step", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_adjoint_inject", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_adjoint_run", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _I32, _PD]),
+    ("swh_probe_copy", ctypes.c_int, [ctypes.c_int64, _I32, _PD]),
     ("swh_md_create", ctypes.c_int, [_I32, ctypes.POINTER(ctypes.c_void_p), _PD, _PI32, _PI32,
                                      ctypes.POINTER(ctypes.c_void_p)]),
     ("swh_md_destroy", ctypes.c_int, [ctypes.c_void_p]),

@@ -376,6 +376,15 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     // (16-byte vector lanes; FP32 also keeps the next row's loads in flight)
     if (h->prec == SWH_PREC_FP64) { h->block_x = 128; h->rows_per_block = 16; h->variant = 1; }
     else { h->block_x = 64; h->rows_per_block = 32; h->variant = 3; }
+    {
+        // a single grid must still fill the GPU: shorten the row strips until the
+        // launch has at least 4096 wavefronts (16 per CU)
+        const int vec = (h->variant & 1) ? (h->prec == SWH_PREC_FP64 ? 2 : 4) : 1;
+        const long long lanes = (d->nx + vec - 1) / vec;
+        const long long waves_x = (lanes + 63) / 64;
+        while (h->rows_per_block > 4 && waves_x * ((d->ny + h->rows_per_block - 1) / h->rows_per_block) < 4096)
+            h->rows_per_block /= 2;
+    }
     const int nx = h->nx, ny = h->ny;
     h->dbe.assign(d->distance_bottom_edge, d->distance_bottom_edge + ny + 1);
     h->dle.assign(d->distance_left_edge, d->distance_left_edge + nx + 1);
@@ -1133,7 +1142,11 @@ int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **
         return rc;
     }
     b->nsrc = nsources;
-    b->block_x = b->g->block_x; b->rows_per_block = b->g->rows_per_block; b->variant = b->g->variant;
+    // default launch: two steps per launch, 4-wave blocks (the autotuner's choice on
+    // the 512-source 1200^2 workload; swh_batch_autotune refines it per batch)
+    b->block_x = 256; b->two_step = 1; b->src_fast = 0;
+    if (b->g->prec == SWH_PREC_FP64) { b->rows_per_block = 64; b->variant = 1; }
+    else { b->rows_per_block = 16; b->variant = 3; }
     const size_t bytes = b->g->plane_elems() * b->g->esz * (size_t)nsources;
     const bool kahan = b->g->prec == SWH_PREC_FP32K;
     for (int k = 0; k < 2 && rc == SWH_OK; k++) {
@@ -1295,6 +1308,31 @@ static int check_launch(int32_t block_x, int32_t rows_per_block, int32_t variant
     if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 31)
         return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..31");
     return SWH_OK;
+}
+
+int swh_probe_copy(int64_t bytes, int32_t reps, double *ms_per_copy)
+{
+    if (bytes < 16 || (bytes % 16) || reps < 1 || !ms_per_copy) return fail(SWH_EINVAL, "swh_probe_copy: bad argument");
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = dmalloc(&a, (size_t)bytes);
+    if (!rc) rc = dmalloc(&b, (size_t)bytes);
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(SWH_EHIP, "swh_probe_copy: event creation");
+    float ms = 0.f;
+    if (!rc && launch_probe_copy(a, b, bytes, nullptr) != hipSuccess) rc = fail(SWH_EHIP, "probe copy launch");
+    if (!rc && hipEventRecord(e0, nullptr) != hipSuccess) rc = fail(SWH_EHIP, "event record");
+    for (int k = 0; k < reps && !rc; k++)
+        if (launch_probe_copy(a, b, bytes, nullptr) != hipSuccess) rc = fail(SWH_EHIP, "probe copy launch");
+    if (!rc && (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(SWH_EHIP, "event timing");
+    if (!rc) *ms_per_copy = (double)ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    return rc;
 }
 
 int swh_batch_autotune(swh_batch *b, double dt, int32_t reps, int32_t *best, double *best_ms)

@@ -7,8 +7,8 @@
 //   update_U (Manning / Chezy)        domain_mod_update_U_DE1_inner_include.f90:14-99
 //   one_euler_step / one_rk2_step     domain_mod_timestepping_alternatives_include.f90:8-146
 //
-// An Euler stage is four launches: clip + depth/velocity (cell), edge fluxes and
-// explicit sources (one thread per cell, owning its south and west edges, with the
+// An Euler stage is three launches: edge fluxes and explicit sources (one thread per
+// cell, owning its south and west edges, with the clipped depth/velocity and the
 // limited gradients recomputed from the cell's column/row neighbours), a one-block
 // scalar pass (boundary-flux sums, CFL step choice, time), and the cell update.
 // The CFL step is a running max of 1/dt kept on the device, so an adaptive rk2 step
@@ -22,40 +22,39 @@ constexpr double kLimC1 = 0.1, kLimC2 = 0.3, kLimC3 = 100.0;  // domain_mod.f90:
 __device__ __forceinline__ double fmax_(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double fmin_(double a, double b) { return a < b ? a : b; }
 
-// ---- pass 1: clip + depth/velocity (domain_mod.f90:1374-1412) ----
+// ---- clip + depth/velocity of one cell (domain_mod.f90:1374-1412) ----
+// Evaluated where it is needed (by the flux pass for a cell and its neighbours, by
+// the update for its own cell) instead of being stored: the values are the same
+// bits as a stored pass would give, and the state is read once instead of being
+// copied into three more planes.
 template <typename T>
-__global__ void fv_depth_kernel(FVArgs<T> a)
+struct Cell {
+    T s, uh, vh, d, u, v;
+    bool fixed;   // negative depth clipped (negative_depth_fix_counter)
+};
+
+template <typename T>
+__device__ __forceinline__ Cell<T> cell_at(const FVArgs<T> &a, long long o)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        // reset the running CFL maximum: 1/maximum_timestep (include.f90:92-95)
-        a.sc->max_dt_inv_bits = (unsigned long long)__double_as_longlong(1.0 / kMaxTimestep);
-    }
-    unsigned long long fixes = 0;
-    if (i < a.nx && j < a.ny) {
-        const long long o = (long long)j * a.pitch + i;
-        T s = a.S0[o], uh = a.UH0[o], vh = a.VH0[o];
-        const T e = a.E[o];
-        T d = s - e, u, v;
-        if (d > T(kMinDepth)) {
-            const T inv = T(1) / d;
-            u = uh * inv;
-            v = vh * inv;
-        } else {
-            if (d < T(0)) {
-                d = T(0);
-                s = e;
-                fixes = 1;
-            }
-            u = T(0); v = T(0); uh = T(0); vh = T(0);
+    Cell<T> c;
+    T s = a.S0[o], uh = a.UH0[o], vh = a.VH0[o];
+    const T e = a.E[o];
+    T d = s - e, u, v;
+    c.fixed = false;
+    if (d > T(kMinDepth)) {
+        const T inv = T(1) / d;
+        u = uh * inv;
+        v = vh * inv;
+    } else {
+        if (d < T(0)) {
+            d = T(0);
+            s = e;
+            c.fixed = true;
         }
-        a.depth[o] = d; a.u[o] = u; a.v[o] = v;
-        a.S[o] = s; a.UH[o] = uh; a.VH[o] = vh;
+        u = T(0); v = T(0); uh = T(0); vh = T(0);
     }
-    // wave-level count, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) fixes += __shfl_xor(fixes, off);
-    if ((threadIdx.x & 63) == 0 && fixes) atomicAdd(&a.sc->neg_depth_count, fixes);
+    c.s = s; c.uh = uh; c.vh = vh; c.d = d; c.u = u; c.v = v;
+    return c;
 }
 
 // Wet-dry limiter coefficient (include.f90:751-757)
@@ -84,18 +83,17 @@ struct Grad4 {
 };
 
 // Gradients of (stage, depth, u, v) at a cell from its two neighbours along one
-// axis; zero when the cell is on that axis' border.
+// axis (the caller passes zero gradients for cells on that axis' border).
 template <typename T>
-__device__ __forceinline__ Grad4 grad_at(const FVArgs<T> &a, long long o, long long step, bool border, double theta)
+__device__ __forceinline__ Grad4 grad3(const Cell<T> &lo, const Cell<T> &mid, const Cell<T> &hi, double theta)
 {
-    Grad4 g{0.0, 0.0, 0.0, 0.0};
-    if (border) return g;
-    const double dl = a.depth[o - step], dc = a.depth[o], dh = a.depth[o + step];
+    Grad4 g;
+    const double dl = lo.d, dc = mid.d, dh = hi.d;
     const double th = theta_wd(theta, dl, dc, dh);
-    g.st = mc_grad(a.S[o - step], a.S[o], a.S[o + step], th);
+    g.st = mc_grad(lo.s, mid.s, hi.s, th);
     g.dp = mc_grad(dl, dc, dh, th);
-    g.u = mc_grad(a.u[o - step], a.u[o], a.u[o + step], th);
-    g.v = mc_grad(a.v[o - step], a.v[o], a.v[o + step], th);
+    g.u = mc_grad(lo.u, mid.u, hi.u, th);
+    g.v = mc_grad(lo.v, mid.v, hi.v, th);
     return g;
 }
 
@@ -164,19 +162,24 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
         const double half_gravity = 0.5 * kGravity;
         const double EPS = 1.0e-12;
 
+        // the cell, its south and west neighbours, and (where gradients need them) the
+        // next cells along each axis
+        const long long om = o - P, ow = o - 1;
+        const Cell<T> C = cell_at(a, o), Sc = cell_at(a, om), Wc = cell_at(a, ow);
+        const Grad4 zero{0.0, 0.0, 0.0, 0.0};
+
         // NS edge between (i, j-1) and (i, j); gradients are zero on rows 0 and ny-1
         {
-            const long long om = o - P;
-            const Grad4 gl = grad_at(a, om, P, (j - 1) == 0 || (j - 1) == ny - 1, a.theta);
-            const Grad4 gc = grad_at(a, o, P, j == 0 || j == ny - 1, a.theta);
-            const double stage_neg = (double)a.S[om] + 0.5 * gl.st;
-            const double stage_pos = (double)a.S[o] - 0.5 * gc.st;
-            const double depth_neg = (double)a.depth[om] + 0.5 * gl.dp;
-            const double depth_pos = (double)a.depth[o] - 0.5 * gc.dp;
-            const double u_neg = (double)a.u[om] + 0.5 * gl.u, u_pos = (double)a.u[o] - 0.5 * gc.u;
-            const double v_neg = (double)a.v[om] + 0.5 * gl.v, v_pos = (double)a.v[o] - 0.5 * gc.v;
+            const Grad4 gl = ((j - 1) == 0 || (j - 1) == ny - 1) ? zero : grad3(cell_at(a, om - P), Sc, C, a.theta);
+            const Grad4 gc = (j == 0 || j == ny - 1) ? zero : grad3(Sc, C, cell_at(a, o + P), a.theta);
+            const double stage_neg = (double)Sc.s + 0.5 * gl.st;
+            const double stage_pos = (double)C.s - 0.5 * gc.st;
+            const double depth_neg = (double)Sc.d + 0.5 * gl.dp;
+            const double depth_pos = (double)C.d - 0.5 * gc.dp;
+            const double u_neg = (double)Sc.u + 0.5 * gl.u, u_pos = (double)C.u - 0.5 * gc.u;
+            const double v_neg = (double)Sc.v + 0.5 * gl.v, v_pos = (double)C.v - 0.5 * gc.v;
             const EdgeRes r = edge_flux(stage_neg, stage_pos, depth_neg, depth_pos, u_neg, u_pos, v_neg, v_pos,
-                                        (double)a.depth[om], (double)a.depth[o], (double)a.E[om], (double)a.E[o],
+                                        (double)Sc.d, (double)C.d, (double)a.E[om], (double)a.E[o],
                                         a.dbe[j], true);
             a.fNS[fo] = (T)r.f0;
             a.fNS[FP + fo] = (T)r.f1;
@@ -186,17 +189,16 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             if (r.max_speed > EPS) mx = fmax_(mx, r.max_speed * (1.0 / (a.dle[0] * a.half_cfl)));
 
             // EW edge between (i-1, j) and (i, j); gradients are zero on columns 0 and nx-1
-            const long long ow = o - 1;
-            const Grad4 el = grad_at(a, ow, 1, (i - 1) == 0 || (i - 1) == nx - 1, a.theta);
-            const Grad4 ec = grad_at(a, o, 1, i == 0 || i == nx - 1, a.theta);
-            const double st_neg = (double)a.S[ow] + 0.5 * el.st;
-            const double st_pos = (double)a.S[o] - 0.5 * ec.st;
-            const double dp_neg = (double)a.depth[ow] + 0.5 * el.dp;
-            const double dp_pos = (double)a.depth[o] - 0.5 * ec.dp;
-            const double uu_neg = (double)a.u[ow] + 0.5 * el.u, uu_pos = (double)a.u[o] - 0.5 * ec.u;
-            const double vv_neg = (double)a.v[ow] + 0.5 * el.v, vv_pos = (double)a.v[o] - 0.5 * ec.v;
+            const Grad4 el = ((i - 1) == 0 || (i - 1) == nx - 1) ? zero : grad3(cell_at(a, ow - 1), Wc, C, a.theta);
+            const Grad4 ec = (i == 0 || i == nx - 1) ? zero : grad3(Wc, C, cell_at(a, o + 1), a.theta);
+            const double st_neg = (double)Wc.s + 0.5 * el.st;
+            const double st_pos = (double)C.s - 0.5 * ec.st;
+            const double dp_neg = (double)Wc.d + 0.5 * el.dp;
+            const double dp_pos = (double)C.d - 0.5 * ec.dp;
+            const double uu_neg = (double)Wc.u + 0.5 * el.u, uu_pos = (double)C.u - 0.5 * ec.u;
+            const double vv_neg = (double)Wc.v + 0.5 * el.v, vv_pos = (double)C.v - 0.5 * ec.v;
             const EdgeRes q = edge_flux(st_neg, st_pos, dp_neg, dp_pos, uu_neg, uu_pos, vv_neg, vv_pos,
-                                        (double)a.depth[ow], (double)a.depth[o], (double)a.E[ow], (double)a.E[o],
+                                        (double)Wc.d, (double)C.d, (double)a.E[ow], (double)a.E[o],
                                         a.dle[i], false);
             const long long EPn = (long long)a.fpitch * ny;
             a.fEW[fo] = (T)q.f0;
@@ -206,16 +208,16 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             double esu = 0.0 - q.bsp_pos * a.dle[i] + q.hghh;
             if (a.spherical) {
                 // include.f90:674-710
-                const double dc = (double)a.depth[o];
+                const double dc = (double)C.d;
                 esv = esv + half_gravity * dc * dc * (a.dbe[j + 1] - a.dbe[j]);
-                const double uhc = (double)a.UH[o], vhc = (double)a.VH[o];
+                const double uhc = (double)C.uh, vhc = (double)C.vh;
                 double cm;
                 if (a.coriolis) {
                     cm = a.area[j] * a.cor[j];
                     esu = esu + cm * vhc;
                     esv = esv - cm * uhc;
                 }
-                cm = a.area[j] * a.tanlat[j] * (double)a.u[o];
+                cm = a.area[j] * a.tanlat[j] * (double)C.u;
                 esu = esu + cm * vhc;
                 esv = esv - cm * uhc;
             }
@@ -262,6 +264,9 @@ __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *f
         sc->bf_store[3] = -st[3];
         const double max_dt = 1.0 / __longlong_as_double((long long)sc->max_dt_inv_bits);
         sc->max_dt = max_dt;
+        // restart the running CFL maximum for the next flux pass: 1/maximum_timestep
+        // (compute_fluxes_DE1_inner_include.f90:92-95)
+        sc->max_dt_inv_bits = (unsigned long long)__double_as_longlong(1.0 / kMaxTimestep);
         double ts;
         if (first_stage) {
             ts = fixed_dt > 0.0 ? fixed_dt : max_dt;
@@ -281,27 +286,29 @@ __global__ void fv_update_kernel(FVArgs<T> a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
-    if (i >= a.nx || j >= a.ny) return;
+    unsigned long long fixed = 0;
+    if (i < a.nx && j < a.ny) {
     const double dt = a.sc->dt;
     const long long o = (long long)j * a.pitch + i;
     const long long fo = (long long)j * a.fpitch + i;
     const long long FP = (long long)a.fpitch * (a.ny + 1);
     const long long EPn = (long long)a.fpitch * a.ny;
     const double inv_cell_area_dt = dt / a.area[j];
-    T *q[3] = {a.S, a.UH, a.VH};
+    const Cell<T> C = cell_at(a, o);
+    const T q[3] = {C.s, C.uh, C.vh};
     double un[3];
     for (int kk = 0; kk < 3; kk++) {
         const T *fn = a.fNS + kk * FP;
         const T *fe = a.fEW + kk * EPn;
-        un[kk] = (double)q[kk][o] - inv_cell_area_dt * (((double)fn[fo + a.fpitch] - (double)fn[fo]) +
-                                                        ((double)fe[fo + 1] - (double)fe[fo]));
+        un[kk] = (double)q[kk] - inv_cell_area_dt * (((double)fn[fo + a.fpitch] - (double)fn[fo]) +
+                                                     ((double)fe[fo + 1] - (double)fe[fo]));
     }
-    const double depth = (double)a.depth[o];
+    const double depth = (double)C.d;
     if (un[0] <= ((double)a.E[o] + kMinDepth)) {
         un[1] = 0.0;
         un[2] = 0.0;
     } else {
-        const double uu = (double)a.u[o], vv = (double)a.v[o];
+        const double uu = (double)C.u, vv = (double)C.v;
         const double fs = (double)a.nsq[o] * sqrt(uu * uu + vv * vv) * pow(fmax_(depth, kMinDepth), a.fric_power);
         const double implicit_factor = 1.0 / (1.0 + (dt * kGravity) * fmax_(depth, 0.0) * fs + dt * a.lfc);
         un[1] = un[1] + inv_cell_area_dt * (a.esU[o] + a.esE[fo + 1]);
@@ -312,6 +319,11 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     a.S[o] = (T)un[0];
     a.UH[o] = (T)un[1];
     a.VH[o] = (T)un[2];
+    fixed = C.fixed ? 1ull : 0ull;
+    }
+    // negative_depth_fix_counter: wave-level count, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) fixed += __shfl_xor(fixed, off);
+    if ((threadIdx.x & 63) == 0 && fixed) atomicAdd(&a.sc->neg_depth_count, fixed);
 }
 
 // ---- rk2 average (timestepping_alternatives_include.f90:123-134) ----
@@ -358,6 +370,7 @@ __global__ void fv_begin_kernel(DevScalars *sc)
     // one_rk2_step keeps the start time (timestepping_alternatives_include.f90:80)
     sc->bf_evolve = 0.0;
     sc->pad[1] = sc->time;
+    sc->max_dt_inv_bits = (unsigned long long)__double_as_longlong(1.0 / kMaxTimestep);
 }
 
 __global__ void fv_finish_kernel(DevScalars *sc)
@@ -373,7 +386,6 @@ template <typename T>
 hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, int first_stage, hipStream_t s)
 {
     dim3 block(256), grid((a.nx + 255) / 256, a.ny);
-    hipLaunchKernelGGL(fv_depth_kernel<T>, grid, block, 0, s, a);
     hipLaunchKernelGGL(fv_flux_kernel<T>, grid, block, 0, s, a);
     hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(256), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
                        fixed_dt, first_stage, a.sc);

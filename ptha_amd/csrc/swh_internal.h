@@ -89,6 +89,9 @@ template <typename T, bool KAHAN>
 hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                             int linear_friction, int nsrc, int block_x, hipStream_t s);
 
+// Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
+hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
+
 // Gauge values after one step, from the current state (batched runs).
 template <typename T, bool KAHAN>
 hipError_t launch_gauge_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,

@@ -23,25 +23,11 @@ namespace {
 
 constexpr int kOwn = 61;   // columns a wavefront writes (lanes 1..61)
 
-// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1, VALU moves without the
-// LDS crossbar); the lane at the open end keeps its own value.
-__device__ __forceinline__ int dpp_shr1_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_shl1_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
-
-__device__ __forceinline__ float shr1(float v) { return __int_as_float(dpp_shr1_i(__float_as_int(v))); }
-__device__ __forceinline__ float shl1(float v) { return __int_as_float(dpp_shl1_i(__float_as_int(v))); }
-__device__ __forceinline__ double shr1(double v)
-{
-    const long long u = __double_as_longlong(v);
-    const int lo = dpp_shr1_i((int)(u & 0xffffffffLL)), hi = dpp_shr1_i((int)(u >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double shl1(double v)
-{
-    const long long u = __double_as_longlong(v);
-    const int lo = dpp_shl1_i((int)(u & 0xffffffffLL)), hi = dpp_shl1_i((int)(u >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
+// Lane shifts across the wavefront (the lane at the open end keeps its own value).
+template <typename T>
+__device__ __forceinline__ T shl1(T v) { return __shfl_down(v, 1); }   // value of lane + 1
+template <typename T>
+__device__ __forceinline__ T shr1(T v) { return __shfl_up(v, 1); }     // value of lane - 1
 
 // Per-row coefficients (uniform across the wavefront), loaded at a clamped row so
 // the loads are always in bounds; 'interior' / 'valid' say whether the row updates.

@@ -2,6 +2,8 @@
 // gauges and the stationary CFL step. All reductions are FP64, deterministic
 // (fixed assignment of work to lanes and a fixed combine order) and carry a
 // two-sum compensation term, so results do not depend on scheduling.
+#include <algorithm>
+
 #include "swh_internal.h"
 
 namespace swh {
@@ -307,5 +309,28 @@ template hipError_t launch_timestep_max<double>(const double *, const double *, 
                                                 const double *, const double *, double *, hipStream_t);
 template hipError_t launch_timestep_max<float>(const float *, const float *, int, int, int, int, double,
                                                const double *, const double *, double *, hipStream_t);
+
+}  // namespace swh
+
+namespace swh {
+
+// ---- plain streaming copy (measurement reference) ----
+// 16-byte loads and stores, grid-stride; used to scale rocprofv3's FETCH_SIZE /
+// WRITE_SIZE counters to bytes and to measure this GPU's streaming bandwidth.
+__global__ __launch_bounds__(256) void probe_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                         long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s)
+{
+    const long long n = bytes / 16;
+    const long long blocks = std::min<long long>((n + 255) / 256, 256LL * 64);
+    hipLaunchKernelGGL(probe_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)src,
+                       (float4 *)dst, n);
+    return hipGetLastError();
+}
 
 }  // namespace swh

@@ -34,6 +34,8 @@ def test_batch_matches_single_source_runs(oracle_built, precision, two_step):
     b = UnitSourceBatch(nx, ny, u["lw"], u["ll"], 3, precision=precision)
     if two_step:
         b.set_launch(64, 16, 16 | (3 if precision == "fp32k" else 1))
+    else:
+        b.set_launch(128, 16, 3 if precision == "fp32k" else 1)
     b.set_elevation(u["elev"])
     b.set_nsteps(nsteps)
     stages = []
