@@ -147,11 +147,39 @@ __device__ __forceinline__ EdgeRes edge_flux(double stage_neg, double stage_pos,
 }
 
 // ---- pass 2: fluxes + explicit sources + CFL ----
+// A block owns a 64 x 4 tile of cells (each cell owns its south and west edges). It
+// first evaluates the clipped state of the tile and a halo of two cells west/south
+// and one cell east/north into LDS (each cell once, instead of once per edge that
+// needs it), then each thread forms its cell's two edges from LDS.
+constexpr int kFTX = 64, kFTY = 4, kFTW = kFTX + 3, kFTH = kFTY + 3;
+
 template <typename T>
 __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    __shared__ T ls_s[kFTH][kFTW], ls_d[kFTH][kFTW], ls_u[kFTH][kFTW], ls_v[kFTH][kFTW];
+    __shared__ T ls_uh[kFTH][kFTW], ls_vh[kFTH][kFTW], ls_e[kFTH][kFTW];
+    const int i0 = blockIdx.x * kFTX, j0 = blockIdx.y * kFTY;
+    const int gx0 = i0 - 2, gy0 = j0 - 2;
+    for (int t = threadIdx.x; t < kFTW * kFTH; t += blockDim.x) {
+        const int lx = t % kFTW, ly = t / kFTW;
+        const int gi = min(max(gx0 + lx, 0), a.nx - 1), gj = min(max(gy0 + ly, 0), a.ny - 1);
+        const long long go = (long long)gj * a.pitch + gi;
+        const Cell<T> c = cell_at(a, go);
+        ls_s[ly][lx] = c.s; ls_d[ly][lx] = c.d; ls_u[ly][lx] = c.u; ls_v[ly][lx] = c.v;
+        ls_uh[ly][lx] = c.uh; ls_vh[ly][lx] = c.vh; ls_e[ly][lx] = a.E[go];
+    }
+    __syncthreads();
+    const int i = i0 + (int)(threadIdx.x % kFTX);
+    const int j = j0 + (int)(threadIdx.x / kFTX);
+    // cell (i + di, j + dj) from the tile
+    auto L = [&](int di, int dj) {
+        const int lx = i + di - gx0, ly = j + dj - gy0;
+        Cell<T> c;
+        c.s = ls_s[ly][lx]; c.d = ls_d[ly][lx]; c.u = ls_u[ly][lx]; c.v = ls_v[ly][lx];
+        c.uh = ls_uh[ly][lx]; c.vh = ls_vh[ly][lx]; c.fixed = false;
+        return c;
+    };
+    auto LE = [&](int di, int dj) { return ls_e[j + dj - gy0][i + di - gx0]; };
     double mx = 0.0;
     if (i >= 1 && i < a.nx && j >= 1 && j < a.ny) {
         const int nx = a.nx, ny = a.ny;
@@ -164,14 +192,13 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
 
         // the cell, its south and west neighbours, and (where gradients need them) the
         // next cells along each axis
-        const long long om = o - P, ow = o - 1;
-        const Cell<T> C = cell_at(a, o), Sc = cell_at(a, om), Wc = cell_at(a, ow);
+        const Cell<T> C = L(0, 0), Sc = L(0, -1), Wc = L(-1, 0);
         const Grad4 zero{0.0, 0.0, 0.0, 0.0};
 
         // NS edge between (i, j-1) and (i, j); gradients are zero on rows 0 and ny-1
         {
-            const Grad4 gl = ((j - 1) == 0 || (j - 1) == ny - 1) ? zero : grad3(cell_at(a, om - P), Sc, C, a.theta);
-            const Grad4 gc = (j == 0 || j == ny - 1) ? zero : grad3(Sc, C, cell_at(a, o + P), a.theta);
+            const Grad4 gl = ((j - 1) == 0 || (j - 1) == ny - 1) ? zero : grad3(L(0, -2), Sc, C, a.theta);
+            const Grad4 gc = (j == 0 || j == ny - 1) ? zero : grad3(Sc, C, L(0, 1), a.theta);
             const double stage_neg = (double)Sc.s + 0.5 * gl.st;
             const double stage_pos = (double)C.s - 0.5 * gc.st;
             const double depth_neg = (double)Sc.d + 0.5 * gl.dp;
@@ -179,7 +206,7 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             const double u_neg = (double)Sc.u + 0.5 * gl.u, u_pos = (double)C.u - 0.5 * gc.u;
             const double v_neg = (double)Sc.v + 0.5 * gl.v, v_pos = (double)C.v - 0.5 * gc.v;
             const EdgeRes r = edge_flux(stage_neg, stage_pos, depth_neg, depth_pos, u_neg, u_pos, v_neg, v_pos,
-                                        (double)Sc.d, (double)C.d, (double)a.E[om], (double)a.E[o],
+                                        (double)Sc.d, (double)C.d, (double)LE(0, -1), (double)LE(0, 0),
                                         a.dbe[j], true);
             a.fNS[fo] = (T)r.f0;
             a.fNS[FP + fo] = (T)r.f1;
@@ -189,8 +216,8 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             if (r.max_speed > EPS) mx = fmax_(mx, r.max_speed * (1.0 / (a.dle[0] * a.half_cfl)));
 
             // EW edge between (i-1, j) and (i, j); gradients are zero on columns 0 and nx-1
-            const Grad4 el = ((i - 1) == 0 || (i - 1) == nx - 1) ? zero : grad3(cell_at(a, ow - 1), Wc, C, a.theta);
-            const Grad4 ec = (i == 0 || i == nx - 1) ? zero : grad3(Wc, C, cell_at(a, o + 1), a.theta);
+            const Grad4 el = ((i - 1) == 0 || (i - 1) == nx - 1) ? zero : grad3(L(-2, 0), Wc, C, a.theta);
+            const Grad4 ec = (i == 0 || i == nx - 1) ? zero : grad3(Wc, C, L(1, 0), a.theta);
             const double st_neg = (double)Wc.s + 0.5 * el.st;
             const double st_pos = (double)C.s - 0.5 * ec.st;
             const double dp_neg = (double)Wc.d + 0.5 * el.dp;
@@ -198,7 +225,7 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             const double uu_neg = (double)Wc.u + 0.5 * el.u, uu_pos = (double)C.u - 0.5 * ec.u;
             const double vv_neg = (double)Wc.v + 0.5 * el.v, vv_pos = (double)C.v - 0.5 * ec.v;
             const EdgeRes q = edge_flux(st_neg, st_pos, dp_neg, dp_pos, uu_neg, uu_pos, vv_neg, vv_pos,
-                                        (double)Wc.d, (double)C.d, (double)a.E[ow], (double)a.E[o],
+                                        (double)Wc.d, (double)C.d, (double)LE(-1, 0), (double)LE(0, 0),
                                         a.dle[i], false);
             const long long EPn = (long long)a.fpitch * ny;
             a.fEW[fo] = (T)q.f0;
@@ -386,7 +413,7 @@ template <typename T>
 hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, int first_stage, hipStream_t s)
 {
     dim3 block(256), grid((a.nx + 255) / 256, a.ny);
-    hipLaunchKernelGGL(fv_flux_kernel<T>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(fv_flux_kernel<T>, dim3((a.nx + kFTX - 1) / kFTX, (a.ny + kFTY - 1) / kFTY), block, 0, s, a);
     hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(256), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
                        fixed_dt, first_stage, a.sc);
     hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
