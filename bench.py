@@ -195,6 +195,24 @@ def secondary_legs(steps):
         out["linear_2048sq_" + prec] = dict(gcell_per_s=round(g, 3), ms_per_step=round(1e3 * wall / steps, 4),
                                             hbm_frac=round(g * bpc / HBM_PEAK_GBS, 4))
         d.close()
+    # adjoint source inversion grid: forward (gauges every 17 steps, checkpoints every
+    # 256) and the adjoint sweep, FP32 + Kahan
+    from ptha_amd.adjoint import GaugeMap
+    nsteps = 4 * steps
+    gm = GaugeMap(ca["nx"], ca["ny"], ca["lw"], ca["ll"], ca["elev"], dt, nsteps, 17, ca["gauges_xy"],
+                  precision="fp32k", spherical=True)
+    X, Y = np.meshgrid(ca["x"], ca["y"])
+    eta0 = np.exp(-((X - ca["src_lon"].mean()) ** 2 + (Y - ca["src_lat"].mean()) ** 2) / 0.5) * (ca["elev"] < 0)
+    t0 = time.perf_counter()
+    g = gm.forward(eta0)
+    t1 = time.perf_counter()
+    lam = gm.adjoint(np.ones_like(g))
+    t2 = time.perf_counter()
+    cells = ca["nx"] * ca["ny"] * nsteps
+    out["adjoint_2048sq_fp32k"] = dict(forward_gcell_per_s=round(cells / (t1 - t0) / 1e9, 3),
+                                        adjoint_gcell_per_s=round(cells / (t2 - t1) / 1e9, 3), steps=nsteps)
+    assert np.isfinite(lam).all()
+    gm.close()
     return out
 
 

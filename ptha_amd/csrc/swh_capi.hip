@@ -227,7 +227,8 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
     a.S = (T *)h->S[dst]; a.UH = (T *)h->UH[dst]; a.VH = (T *)h->VH[dst];
     a.S0 = (const T *)h->S[src]; a.UH0 = (const T *)h->UH[src]; a.VH0 = (const T *)h->VH[src];
     a.E = (const T *)h->E;
-    a.depth = (T *)h->depth; a.u = (T *)h->u; a.v = (T *)h->v;
+    a.depth = nullptr; a.u = nullptr; a.v = nullptr;
+    a.bmax = h->d_bmax;
     a.fNS = (T *)h->fNS; a.fEW = (T *)h->fEW;
     a.esU = h->esU; a.esV = h->esV; a.esJ = h->esJ; a.esE = h->esE;
     a.nsq = (const T *)h->nsq;
@@ -424,7 +425,7 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     if (h->maxq_mask) A(&h->mx, P * N_MAXQ);
     if (fv) {
         const size_t FPn = (size_t)h->fpitch * (ny + 1);
-        A(&h->depth, P); A(&h->u, P); A(&h->v, P);
+        A((void **)&h->d_bmax, fv_flux_blocks(nx, ny) * sizeof(double));
         A(&h->fNS, FPn * 3 * h->esz);
         A(&h->fEW, FPn * 3 * h->esz);
         A((void **)&h->esU, h->plane_elems() * sizeof(double));
@@ -474,7 +475,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
-                    h->adj[1][1], h->adj[1][2], h->d_gvec};
+                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax};
     for (void *p : ptrs) dfree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

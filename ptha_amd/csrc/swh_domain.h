@@ -26,6 +26,7 @@ struct swh_domain {
     // FV work
     void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
     double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
+    double *d_bmax = nullptr;   // per-block CFL maxima of the flux pass
     // leapfrog row coefficients for rc_dt
     void *rc = nullptr;
     double rc_dt = -1.0;

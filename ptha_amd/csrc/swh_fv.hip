@@ -253,17 +253,28 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             if (q.max_speed > EPS) mx = fmax_(mx, q.max_speed * (1.0 / (a.dbe[j] * a.half_cfl)));
         }
     }
+    // block maximum (max is exact, so the order does not matter); the scalar pass
+    // takes the maximum over blocks, instead of every wavefront contending for one
+    // atomic
+    __shared__ double wmax[4];
     for (int off = 32; off > 0; off >>= 1) mx = fmax_(mx, __shfl_xor(mx, off));
-    if ((threadIdx.x & 63) == 0 && mx > 0.0)
-        atomicMax(&a.sc->max_dt_inv_bits, (unsigned long long)__double_as_longlong(mx));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax_(m, wmax[w]);
+        a.bmax[(long long)blockIdx.y * gridDim.x + blockIdx.x] = m;
+    }
 }
+
+int fv_flux_blocks(int nx, int ny) { return ((nx + kFTX - 1) / kFTX) * ((ny + kFTY - 1) / kFTY); }
 
 // ---- pass 3: one block: boundary-flux sums (domain_mod.f90:1483-1499), step choice,
 // time and flux-integral bookkeeping (timestepping_alternatives_include.f90:31-50) ----
 template <typename T>
 __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny, int fpitch,
                                                         int n_ext, double fixed_dt, int first_stage,
-                                                        DevScalars *sc)
+                                                        DevScalars *sc, const double *bmax, int nbmax)
 {
     __shared__ double part[4][64];
     const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -277,7 +288,17 @@ __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *f
         for (int jj = n_ext + lane; jj <= ny - n_ext - 1; jj += 64) acc += (double)fEW[(long long)jj * fpitch + ic];
     }
     part[g][lane] = acc;
+    // CFL: maximum over the flux pass's blocks, from 1/maximum_timestep
+    // (compute_fluxes_DE1_inner_include.f90:92-95)
+    __shared__ double pm[256];
+    double m = 1.0 / kMaxTimestep;
+    for (int k = threadIdx.x; k < nbmax; k += blockDim.x) m = fmax_(m, bmax[k]);
+    pm[threadIdx.x] = m;
     __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) pm[threadIdx.x] = fmax_(pm[threadIdx.x], pm[threadIdx.x + w]);
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         double st[4];
         for (int s = 0; s < 4; s++) {
@@ -289,11 +310,8 @@ __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *f
         sc->bf_store[1] = st[1];
         sc->bf_store[2] = -st[2];
         sc->bf_store[3] = -st[3];
-        const double max_dt = 1.0 / __longlong_as_double((long long)sc->max_dt_inv_bits);
+        const double max_dt = 1.0 / pm[0];
         sc->max_dt = max_dt;
-        // restart the running CFL maximum for the next flux pass: 1/maximum_timestep
-        // (compute_fluxes_DE1_inner_include.f90:92-95)
-        sc->max_dt_inv_bits = (unsigned long long)__double_as_longlong(1.0 / kMaxTimestep);
         double ts;
         if (first_stage) {
             ts = fixed_dt > 0.0 ? fixed_dt : max_dt;
@@ -397,7 +415,6 @@ __global__ void fv_begin_kernel(DevScalars *sc)
     // one_rk2_step keeps the start time (timestepping_alternatives_include.f90:80)
     sc->bf_evolve = 0.0;
     sc->pad[1] = sc->time;
-    sc->max_dt_inv_bits = (unsigned long long)__double_as_longlong(1.0 / kMaxTimestep);
 }
 
 __global__ void fv_finish_kernel(DevScalars *sc)
@@ -415,7 +432,7 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, int first_stage,
     dim3 block(256), grid((a.nx + 255) / 256, a.ny);
     hipLaunchKernelGGL(fv_flux_kernel<T>, dim3((a.nx + kFTX - 1) / kFTX, (a.ny + kFTY - 1) / kFTY), block, 0, s, a);
     hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(256), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
-                       fixed_dt, first_stage, a.sc);
+                       fixed_dt, first_stage, a.sc, (const double *)a.bmax, fv_flux_blocks(a.nx, a.ny));
     hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
     return hipGetLastError();
 }

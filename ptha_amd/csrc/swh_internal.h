@@ -147,7 +147,8 @@ struct FVArgs {
     T *S, *UH, *VH;              // state being stepped (in place)
     const T *S0, *UH0, *VH0;     // state at the start of the Euler stage (read by the clip pass)
     const T *E;
-    T *depth, *u, *v;            // per-cell work
+    T *depth, *u, *v;            // per-cell work (unused: evaluated where needed)
+    double *bmax;                // per-block maximum of max_speed / (dx * cfl / 2) of the flux pass
     T *fNS, *fEW;                // 3 planes each: mass, UH, VH fluxes (pre-multiplied by edge length)
     double *esU, *esV;           // cell-local explicit sources
     double *esJ;                 // NS edge contribution to the cell below (explicit_source_VH_j_minus_1)
@@ -169,6 +170,8 @@ template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
                               double msl, const DevScalars *sc, hipStream_t s);
 hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s);
+// number of blocks of the flux pass (size of FVArgs::bmax)
+int fv_flux_blocks(int nx, int ny);
 hipError_t launch_fv_finish(DevScalars *sc, hipStream_t s);
 
 

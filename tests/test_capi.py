@@ -43,3 +43,18 @@ def test_error_reporting_without_device():
     buf = ctypes.create_string_buffer(256)
     lib.swh_last_error(buf, 256)
     assert b"nx and ny" in buf.value
+
+
+def test_argument_checks_of_md_batch_and_probe_without_device():
+    """The multidomain, batch-tuning and copy-probe entry points reject bad arguments
+    on the host, before any device call."""
+    from ptha_amd import backend as B
+    lib = B.load()
+    ms = ctypes.c_double()
+    assert lib.swh_probe_copy(15, 1, ctypes.byref(ms)) == -1          # not a multiple of 16
+    assert lib.swh_probe_copy(1024, 0, ctypes.byref(ms)) == -1        # no repetitions
+    h = ctypes.c_void_p()
+    assert lib.swh_md_create(0, None, None, None, None, ctypes.byref(h)) == -1
+    assert lib.swh_md_finalize(None) == -1
+    assert lib.swh_md_evolve_one_step(None, 1.0) != 0
+    assert lib.swh_batch_autotune(None, 1.0, 1, None, None) == -1
