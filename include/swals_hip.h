@@ -42,6 +42,9 @@ extern "C" {
 /* timestepping_method ids: the index of the method in the reference's
  * timestepping_metadata table (timestepping_metadata_mod.f90:69-140). */
 #define SWH_METHOD_RK2 1
+#define SWH_METHOD_RK2N 2
+#define SWH_METHOD_MIDPOINT 3
+#define SWH_METHOD_EULER 4
 #define SWH_METHOD_LINEAR 5
 #define SWH_METHOD_LLPNF 6 /* 'leapfrog_linear_plus_nonlinear_friction' */
 
@@ -121,9 +124,12 @@ int swh_set_time(swh_domain *h, double time);
  * for this case. */
 int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double msl, const int32_t sides[4]);
 
-/* domain%evolve_one_step(timestep) (domain_mod.f90:1679-1813). timestep <= 0 means
- * "not present": only valid for rk2, which then takes the CFL step computed on the
- * device. nsteps > 1 repeats the call without host synchronisation. */
+/* domain%evolve_one_step(timestep) (domain_mod.f90:1679-1813) for 'linear',
+ * 'leapfrog_linear_plus_nonlinear_friction', 'rk2', 'euler', 'midpoint' and 'rk2n'
+ * (domain_mod_timestepping_alternatives_include.f90:8-336). timestep <= 0 means
+ * "not present": only valid for the finite-volume methods, which then take the CFL
+ * step computed on the device. nsteps > 1 repeats the call without host
+ * synchronisation. */
 int swh_evolve_one_step(swh_domain *h, double timestep);
 int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps);
 

@@ -36,6 +36,8 @@ contains
         character(len=charlen) :: name
         select case(method)
         case(1); name = 'rk2'
+        case(2); name = 'rk2n'
+        case(3); name = 'midpoint'
         case(4); name = 'euler'
         case(5); name = 'linear'
         case(6); name = 'leapfrog_linear_plus_nonlinear_friction'

@@ -119,8 +119,9 @@ or_domain *or_domain_create(int nx, int ny, const double lw[2], const double ll[
     d->linear_friction_coeff = linear_friction_coeff; d->friction_type = friction_type;
     d->lw[0] = lw[0]; d->lw[1] = lw[1]; d->ll[0] = ll[0]; d->ll[1] = ll[1];
     /* Default CFL / theta from timestepping_metadata_mod.f90:73-121 */
-    double def_cfl = (method == OR_RK2) ? 0.99 : (method == OR_EULER ? 0.9 : 0.7);
-    double def_theta = (method == OR_RK2) ? 1.6 : (method == OR_EULER ? 0.9 : -HUGE_VAL);
+    const int fv2 = method == OR_RK2 || method == OR_RK2N || method == OR_MIDPOINT;
+    double def_cfl = fv2 ? 0.99 : (method == OR_EULER ? 0.9 : 0.7);
+    double def_theta = fv2 ? 1.6 : (method == OR_EULER ? 0.9 : -HUGE_VAL);
     d->cfl = cfl > 0 ? cfl : def_cfl;
     d->theta = theta > 0 ? theta : def_theta;
     d->maximum_timestep = MAXIMUM_TIMESTEP;
@@ -712,6 +713,54 @@ static void rk2_step(or_domain *d, double timestep)
     d->max_dt = max_dt_store;
 }
 
+/* one_rk2n_step: domain_mod_timestepping_alternatives_include.f90:149-254, with
+ * rk2n_n_value = 5 (domain_mod.f90:88-90): n-1 Euler steps of dt/(n-1), subtract 1/n of
+ * the change, one more Euler step of (n-1)/n of a sub-step, add the 1/n back. */
+#define RK2N_N 5
+static void rk2n_step(or_domain *d, double timestep)
+{
+    const size_t n3 = 3 * (size_t)d->nx * d->ny;
+    const double n_inverse = 1.0 / (1.0 * RK2N_N);
+    double backup_time = d->time;
+    memcpy(d->backup_U, d->U, n3 * sizeof(double));
+    double dt_first_step;
+    if (timestep > 0.0) {
+        dt_first_step = timestep / (RK2N_N - 1.0);
+        euler_step(d, dt_first_step);
+    } else {
+        euler_step(d, -1.0);
+        dt_first_step = d->max_dt;
+    }
+    double max_dt_store = d->max_dt;
+    for (int j = 2; j <= RK2N_N - 1; j++) euler_step(d, dt_first_step);
+    for (size_t k = 0; k < n3; k++) d->backup_U[k] = (d->backup_U[k] - d->U[k]) * n_inverse;
+    d->boundary_flux_evolve_integral = d->boundary_flux_evolve_integral * (RK2N_N - 1.0) * n_inverse;
+    double reduced_dt = (1.0 * RK2N_N - 1.0) * n_inverse * dt_first_step;
+    euler_step(d, reduced_dt);
+    for (size_t k = 0; k < n3; k++) d->U[k] = d->U[k] + d->backup_U[k];
+    d->time = backup_time + dt_first_step * (RK2N_N - 1);
+    d->max_dt = max_dt_store;
+}
+
+/* one_midpoint_step: domain_mod_timestepping_alternatives_include.f90:257-336 (no
+ * dispersion): half an Euler step for the fluxes, then a full step from the start
+ * state with those fluxes (and the depth/velocity of the half-step state). */
+static void midpoint_step(or_domain *d, double timestep)
+{
+    const size_t n3 = 3 * (size_t)d->nx * d->ny;
+    double backup_time = d->time;
+    memcpy(d->backup_U, d->U, n3 * sizeof(double));
+    double dt_first_step = compute_fluxes(d);
+    d->max_dt = dt_first_step;
+    if (timestep > 0.0) dt_first_step = timestep;
+    update_U(d, dt_first_step * 0.5);
+    compute_fluxes(d);
+    memcpy(d->U, d->backup_U, n3 * sizeof(double));
+    d->time = backup_time;
+    update_U(d, dt_first_step);
+    d->boundary_flux_evolve_integral = boundary_flux_sum(d) * dt_first_step;
+}
+
 /* update_max_quantities: domain_mod.f90:2243-2404, with all six max_U variables
  * (so time_of_max_stage is tracked and the 'toms > 0' branch of max_stage applies). */
 static void update_max_quantities(or_domain *d)
@@ -768,6 +817,8 @@ static void evolve_one_step(or_domain *d, double dt)
     switch (d->method) {
     case OR_RK2: rk2_step(d, dt); break;
     case OR_EULER: euler_step(d, dt); break;
+    case OR_RK2N: rk2n_step(d, dt); break;
+    case OR_MIDPOINT: midpoint_step(d, dt); break;
     case OR_LINEAR:
     case OR_LLPNF: leapfrog_step(d, dt); break;
     default: break;
@@ -818,7 +869,8 @@ double or_domain_stationary_timestep_max(const or_domain *d)
         }
     }
     double ts = ts_max * d->cfl;
-    if (d->method == OR_RK2 || d->method == OR_EULER) ts = ts * 0.5;
+    if (d->method == OR_RK2 || d->method == OR_EULER || d->method == OR_MIDPOINT) ts = ts * 0.5;
+    if (d->method == OR_RK2N) ts = ts * 0.5 * (RK2N_N - 1.0);
     return ts;
 }
 

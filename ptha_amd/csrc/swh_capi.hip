@@ -227,7 +227,7 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
     a.S = (T *)h->S[dst]; a.UH = (T *)h->UH[dst]; a.VH = (T *)h->VH[dst];
     a.S0 = (const T *)h->S[src]; a.UH0 = (const T *)h->UH[src]; a.VH0 = (const T *)h->VH[src];
     a.E = (const T *)h->E;
-    a.depth = nullptr; a.u = nullptr; a.v = nullptr;
+    a.Sq = nullptr; a.UHq = nullptr; a.VHq = nullptr;
     a.bmax = h->d_bmax;
     a.fNS = (T *)h->fNS; a.fEW = (T *)h->fEW;
     a.esU = h->esU; a.esV = h->esV; a.esJ = h->esJ; a.esE = h->esE;
@@ -240,45 +240,94 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
     return a;
 }
 
+// One Euler stage of the finite-volume schemes: fluxes from state `src`, update into
+// `dst` (from `qsrc`'s values when given, else from the clipped `src`), then the
+// nesting flux integrals (`nest_dt` times the pre-multiplied fluxes; 0: not tracked
+// in this stage) and the boundary condition (update_U calls update_boundary,
+// domain_mod.f90:1564-1582).
 template <typename T>
-int rk2_step(swh_domain *h, double dt)
+int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double dt_mult, int mode, double nest_dt)
+{
+    FVArgs<T> a = fv_args<T>(h, dst, src);
+    if (qsrc >= 0) {
+        a.Sq = (const T *)h->S[qsrc]; a.UHq = (const T *)h->UH[qsrc]; a.VHq = (const T *)h->VH[qsrc];
+    }
+    SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
+    if (h->n_nest_strips && nest_dt != 0.0) {
+        const long long FPn = (long long)h->fpitch * (h->ny + 1), EPn = (long long)h->fpitch * h->ny;
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
+                                     EPn, h->fpitch, 3, nest_dt, h->d_nest_int, h->stream));
+    }
+    if (h->bc_mask)
+        SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], h->nx, h->ny, h->pitch,
+                                      h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
+    return SWH_OK;
+}
+
+// The finite-volume schemes (domain_mod_timestepping_alternatives_include.f90): the
+// state at the start of the step stays in buffer A (backup_U), the result ends in B.
+// A timestep dt <= 0 means "no timestep argument": the CFL step of the first flux
+// computation, decided on the device.
+template <typename T>
+int fv_step(swh_domain *h, double dt)
 {
     if (h->nsteps == 0) {
         int rc = maxq_now<T>(h, h->sc, 0.0);
         if (rc) return rc;
     }
-    const int A = h->cur, B = 1 - A;
-    SWH_HIP(launch_fv_begin(h->sc, h->stream));
-    // first Euler stage: A -> B (A stays as backup_U)
     if (h->n_nest_strips && !(dt > 0.0))
-        return fail(SWH_EINVAL, "nested rk2 domains need the timestep (the multidomain step provides it)");
-    const long long FPn = (long long)h->fpitch * (h->ny + 1), EPn = (long long)h->fpitch * h->ny;
-    FVArgs<T> a1 = fv_args<T>(h, B, A);
-    SWH_HIP(launch_fv_euler<T>(a1, dt, 1, h->stream));
-    // nesting flux integrals, half a step per Euler stage
-    // (domain_mod_timestepping_alternatives_include.f90:98-118)
-    if (h->n_nest_strips)
-        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
-                                     EPn, h->fpitch, 3, dt * 0.5, h->d_nest_int, h->stream));
-    if (h->bc_mask)
-        SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], h->nx, h->ny, h->pitch, h->bc_mask,
-                                      h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
-    // second Euler stage with the same step, in place on B
-    FVArgs<T> a2 = fv_args<T>(h, B, B);
-    SWH_HIP(launch_fv_euler<T>(a2, dt, 0, h->stream));
-    if (h->n_nest_strips)
-        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
-                                     EPn, h->fpitch, 3, dt * 0.5, h->d_nest_int, h->stream));
-    if (h->bc_mask)
-        SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], h->nx, h->ny, h->pitch, h->bc_mask,
-                                      h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
-    SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A], (const T *)h->UH[A],
-                                 (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->stream));
-    SWH_HIP(launch_fv_finish(h->sc, h->stream));
+        return fail(SWH_EINVAL, "nested finite-volume domains need the timestep (the multidomain step provides it)");
+    const int A = h->cur, B = 1 - A;
+    const int acc = kStageAccBf;
+    int rc = SWH_OK;
+#define STAGE(...) do { rc = fv_stage<T>(h, __VA_ARGS__); if (rc) return rc; } while (0)
+    SWH_HIP(launch_fv_begin(h->sc, h->stream));
+    switch (h->method) {
+    case SWH_METHOD_EULER:
+        // one_euler_step (include.f90:8-62)
+        STAGE(B, A, -1, dt, 1.0, kStageFirst | acc, dt);
+        SWH_HIP(launch_fv_finish(h->sc, kFinishEuler, h->stream));
+        break;
+    case SWH_METHOD_RK2:
+        // one_rk2_step (include.f90:64-146): two Euler stages with the same step,
+        // nesting fluxes tracked half a step each, then the average with backup_U
+        STAGE(B, A, -1, dt, 1.0, kStageFirst | acc, dt * 0.5);
+        STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5);
+        SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A],
+                                     (const T *)h->UH[A], (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->stream));
+        SWH_HIP(launch_fv_finish(h->sc, kFinishRk2, h->stream));
+        break;
+    case SWH_METHOD_MIDPOINT:
+        // one_midpoint_step (include.f90:257-336): half a step for the fluxes, then the
+        // full step from backup_U with them
+        STAGE(B, A, -1, dt, 0.5, kStageFirst, 0.0);
+        STAGE(B, B, A, dt, 1.0, acc | kStageFromTime0, dt);
+        SWH_HIP(launch_fv_finish(h->sc, kFinishEuler, h->stream));
+        break;
+    case SWH_METHOD_RK2N: {
+        // one_rk2n_step (include.f90:149-254), n = kRk2nN
+        const double n_inverse = 1.0 / (1.0 * kRk2nN);
+        const double dt1 = dt > 0.0 ? dt / (kRk2nN - 1.0) : -1.0;
+        const double nest1 = dt > 0.0 ? dt1 * (kRk2nN - 1.0) * n_inverse : 0.0;
+        const double reduce = (1.0 * kRk2nN - 1.0) * n_inverse;
+        STAGE(B, A, -1, dt1, 1.0, kStageFirst | acc, nest1);
+        for (int j = 2; j <= kRk2nN - 1; j++) STAGE(B, B, -1, dt1, 1.0, acc, nest1);
+        SWH_HIP(launch_fv_rk2n_combine<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (T *)h->S[A], (T *)h->UH[A],
+                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 0, h->stream));
+        STAGE(B, B, -1, dt1, reduce, acc | kStageScaleBfN, dt > 0.0 ? reduce * dt1 : 0.0);
+        SWH_HIP(launch_fv_rk2n_combine<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (T *)h->S[A], (T *)h->UH[A],
+                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 1, h->stream));
+        SWH_HIP(launch_fv_finish(h->sc, kFinishRk2n, h->stream));
+        break;
+    }
+    default:
+        return fail(SWH_EINVAL, "not a finite-volume timestepping_method");
+    }
+#undef STAGE
     h->cur = B;
     const bool upd = h->maxq_mask && ((h->nsteps % h->maxq_freq) == 0);
     if (upd) {
-        int rc = maxq_now<T>(h, h->sc, 0.0);
+        rc = maxq_now<T>(h, h->sc, 0.0);
         if (rc) return rc;
     }
     h->nsteps++;
@@ -291,7 +340,7 @@ int evolve(swh_domain *h, double dt)
         if (!(dt > 0.0)) return fail(SWH_EINVAL, "timestep must be provided for leapfrog-type evolve_one_step");
         return h->prec == SWH_PREC_FP64 ? leapfrog_step<double, false>(h, dt) : leapfrog_step<float, true>(h, dt);
     }
-    return h->prec == SWH_PREC_FP64 ? rk2_step<double>(h, dt) : rk2_step<float>(h, dt);
+    return h->prec == SWH_PREC_FP64 ? fv_step<double>(h, dt) : fv_step<float>(h, dt);
 }
 
 }  // namespace
@@ -343,8 +392,7 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     if (!d || !out) return fail(SWH_EINVAL, "null argument");
     *out = nullptr;
     if (d->nx < 3 || d->ny < 3) return fail(SWH_EINVAL, "nx and ny must be >= 3");
-    if (d->timestepping_method != SWH_METHOD_LINEAR && d->timestepping_method != SWH_METHOD_LLPNF &&
-        d->timestepping_method != SWH_METHOD_RK2)
+    if (d->timestepping_method < SWH_METHOD_RK2 || d->timestepping_method > SWH_METHOD_LLPNF)
         return fail(SWH_EINVAL, "unsupported timestepping_method id");
     if (d->precision != SWH_PREC_FP64 && d->precision != SWH_PREC_FP32K) return fail(SWH_EINVAL, "bad precision");
     if (!d->distance_bottom_edge || !d->distance_left_edge || !d->area_cell_y)
@@ -366,9 +414,11 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     h->maxq_mask = d->max_U_mask & 63;
     h->maxq_freq = d->max_U_update_frequency > 0 ? d->max_U_update_frequency : 1;
     h->msl = d->msl_linear;
-    const bool fv = h->method == SWH_METHOD_RK2;
-    h->cfl = d->cfl > 0 ? d->cfl : (fv ? 0.99 : 0.7);        // timestepping_metadata_mod.f90:73-121
-    h->theta = d->theta > 0 ? d->theta : (fv ? 1.6 : 0.0);
+    const bool fv = !h->leapfrog();
+    const bool euler = h->method == SWH_METHOD_EULER;
+    // timestepping_metadata_mod.f90:73-121
+    h->cfl = d->cfl > 0 ? d->cfl : (fv ? (euler ? 0.9 : 0.99) : 0.7);
+    h->theta = d->theta > 0 ? d->theta : (fv ? (euler ? 0.9 : 1.6) : 0.0);
     h->lfc = d->linear_friction_coeff;
     h->max_timestep = d->maximum_timestep > 0 ? d->maximum_timestep : kMaxTimestep;
     h->arrival_thresh = d->arrival_stage_threshold_above_msl_linear;
@@ -582,8 +632,8 @@ int swh_get_scalars(swh_domain *h, swh_scalars *out)
     } else {
         out->time = s.time;
         out->max_dt = s.max_dt;
-        out->evolve_step_dt = h->nsteps ? s.dt : 0.0;
-        out->dt_last_update = h->nsteps ? s.dt : 0.0;
+        out->evolve_step_dt = h->nsteps ? s.evolve_dt : 0.0;
+        out->dt_last_update = h->nsteps ? s.stage_dt : 0.0;
         h->time = s.time;
     }
     for (int k = 0; k < 4; k++) out->boundary_flux_store[k] = s.bf_store[k];
@@ -626,7 +676,9 @@ int swh_stationary_timestep_max(swh_domain *h, double *out)
     double v;
     SWH_HIP(hipMemcpy(&v, res, sizeof(double), hipMemcpyDeviceToHost));
     v = v * h->cfl;
-    if (!h->leapfrog()) v = v * 0.5;  // stationary_timestep_max, domain_mod.f90:2003-2008
+    // stationary_timestep_max, domain_mod.f90:2000-2013
+    if (!h->leapfrog()) v = v * 0.5;
+    if (h->method == SWH_METHOD_RK2N) v = v * (kRk2nN - 1.0);
     *out = v;
     return SWH_OK;
 }

@@ -273,7 +273,7 @@ int fv_flux_blocks(int nx, int ny) { return ((nx + kFTX - 1) / kFTX) * ((ny + kF
 // time and flux-integral bookkeeping (timestepping_alternatives_include.f90:31-50) ----
 template <typename T>
 __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny, int fpitch,
-                                                        int n_ext, double fixed_dt, int first_stage,
+                                                        int n_ext, double fixed_dt, double dt_mult, int mode,
                                                         DevScalars *sc, const double *bmax, int nbmax)
 {
     __shared__ double part[4][64];
@@ -312,16 +312,20 @@ __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *f
         sc->bf_store[3] = -st[3];
         const double max_dt = 1.0 / pm[0];
         sc->max_dt = max_dt;
-        double ts;
-        if (first_stage) {
-            ts = fixed_dt > 0.0 ? fixed_dt : max_dt;
-            sc->dt = ts;
-            sc->pad[0] = max_dt;  // max_dt of the first Euler stage (reported by rk2)
-        } else {
-            ts = sc->dt;
+        if (mode & kStageFirst) {
+            sc->dt = fixed_dt > 0.0 ? fixed_dt : max_dt;
+            sc->first_max_dt = max_dt;
         }
-        sc->bf_evolve = sc->bf_evolve + ts * (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]);
-        sc->time = sc->time + ts;
+        // the stage's step: the base step, or midpoint's half step / rk2n's reduced step
+        const double ts = (dt_mult == 1.0) ? sc->dt : dt_mult * sc->dt;
+        sc->stage_dt = ts;
+        if (mode & kStageScaleBfN)
+            sc->bf_evolve = sc->bf_evolve * (kRk2nN - 1.0) * (1.0 / (1.0 * kRk2nN));
+        if (mode & kStageAccBf)
+            sc->bf_evolve =
+                sc->bf_evolve + ts * (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]);
+        // update_U advances the time (update_U_DE1_inner_include.f90:98)
+        sc->time = (mode & kStageFromTime0) ? sc->time0 + ts : sc->time + ts;
     }
 }
 
@@ -333,14 +337,19 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     const int j = blockIdx.y;
     unsigned long long fixed = 0;
     if (i < a.nx && j < a.ny) {
-    const double dt = a.sc->dt;
+    const double dt = a.sc->stage_dt;
     const long long o = (long long)j * a.pitch + i;
     const long long fo = (long long)j * a.fpitch + i;
     const long long FP = (long long)a.fpitch * (a.ny + 1);
     const long long EPn = (long long)a.fpitch * a.ny;
     const double inv_cell_area_dt = dt / a.area[j];
     const Cell<T> C = cell_at(a, o);
-    const T q[3] = {C.s, C.uh, C.vh};
+    T q[3] = {C.s, C.uh, C.vh};
+    if (a.Sq) {
+        q[0] = a.Sq[o];
+        q[1] = a.UHq[o];
+        q[2] = a.VHq[o];
+    }
     double un[3];
     for (int kk = 0; kk < 3; kk++) {
         const T *fn = a.fNS + kk * FP;
@@ -412,27 +421,57 @@ __global__ __launch_bounds__(256) void fv_stage_bc_kernel(T *S, T *UH, T *VH, in
 __global__ void fv_begin_kernel(DevScalars *sc)
 {
     // evolve_one_step resets the per-step flux integral (domain_mod.f90:1713-1715);
-    // one_rk2_step keeps the start time (timestepping_alternatives_include.f90:80)
+    // the schemes keep the start time (timestepping_alternatives_include.f90:78, 167, 271)
     sc->bf_evolve = 0.0;
-    sc->pad[1] = sc->time;
+    sc->time0 = sc->time;
 }
 
-__global__ void fv_finish_kernel(DevScalars *sc)
+__global__ void fv_finish_kernel(DevScalars *sc, int mode)
 {
-    // rk2 bookkeeping (timestepping_alternatives_include.f90:136-142)
-    sc->time = sc->pad[1] + sc->dt;
-    sc->bf_evolve = 0.5 * sc->bf_evolve;
-    sc->max_dt = sc->pad[0];
+    if (mode == kFinishRk2) {
+        // timestepping_alternatives_include.f90:136-142
+        sc->time = sc->time0 + sc->dt;
+        sc->bf_evolve = 0.5 * sc->bf_evolve;
+    } else if (mode == kFinishRk2n) {
+        // timestepping_alternatives_include.f90:244-249
+        sc->time = sc->time0 + sc->dt * (kRk2nN - 1);
+    }
+    // the reported CFL step is the first flux computation's (euler and midpoint
+    // have only that one as max_dt; rk2 and rk2n restore it)
+    sc->max_dt = sc->first_max_dt;
+    // evolve_one_step (domain_mod.f90:1799-1803)
+    sc->evolve_dt = sc->time - sc->time0;
     sc->bf_time = sc->bf_time + sc->bf_evolve;
 }
 
+// ---- rk2n combination (timestepping_alternatives_include.f90:212-241) ----
 template <typename T>
-hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, int first_stage, hipStream_t s)
+__global__ void fv_rk2n_combine_kernel(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch,
+                                       int part)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long long o = (long long)j * pitch + i;
+    if (part == 0) {
+        const T n_inverse = T(1.0 / (1.0 * kRk2nN));
+        S0[o] = (S0[o] - S[o]) * n_inverse;
+        UH0[o] = (UH0[o] - UH[o]) * n_inverse;
+        VH0[o] = (VH0[o] - VH[o]) * n_inverse;
+    } else {
+        S[o] = S[o] + S0[o];
+        UH[o] = UH[o] + UH0[o];
+        VH[o] = VH[o] + VH0[o];
+    }
+}
+
+template <typename T>
+hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, int mode, hipStream_t s)
 {
     dim3 block(256), grid((a.nx + 255) / 256, a.ny);
     hipLaunchKernelGGL(fv_flux_kernel<T>, dim3((a.nx + kFTX - 1) / kFTX, (a.ny + kFTY - 1) / kFTY), block, 0, s, a);
     hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(256), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
-                       fixed_dt, first_stage, a.sc, (const double *)a.bmax, fv_flux_blocks(a.nx, a.ny));
+                       fixed_dt, dt_mult, mode, a.sc, (const double *)a.bmax, fv_flux_blocks(a.nx, a.ny));
     hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
     return hipGetLastError();
 }
@@ -461,14 +500,27 @@ hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_fv_finish(DevScalars *sc, hipStream_t s)
+hipError_t launch_fv_finish(DevScalars *sc, int mode, hipStream_t s)
 {
-    hipLaunchKernelGGL(fv_finish_kernel, dim3(1), dim3(1), 0, s, sc);
+    hipLaunchKernelGGL(fv_finish_kernel, dim3(1), dim3(1), 0, s, sc, mode);
     return hipGetLastError();
 }
 
-template hipError_t launch_fv_euler<double>(const FVArgs<double> &, double, int, hipStream_t);
-template hipError_t launch_fv_euler<float>(const FVArgs<float> &, double, int, hipStream_t);
+template <typename T>
+hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
+                                  hipStream_t s)
+{
+    dim3 block(256), grid((nx + 255) / 256, ny);
+    hipLaunchKernelGGL(fv_rk2n_combine_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, part);
+    return hipGetLastError();
+}
+
+template hipError_t launch_fv_euler<double>(const FVArgs<double> &, double, double, int, hipStream_t);
+template hipError_t launch_fv_euler<float>(const FVArgs<float> &, double, double, int, hipStream_t);
+template hipError_t launch_fv_rk2n_combine<double>(double *, double *, double *, double *, double *, double *, int,
+                                                   int, int, int, hipStream_t);
+template hipError_t launch_fv_rk2n_combine<float>(float *, float *, float *, float *, float *, float *, int, int, int,
+                                                  int, hipStream_t);
 template hipError_t launch_fv_average<double>(double *, double *, double *, const double *, const double *,
                                               const double *, int, int, int, hipStream_t);
 template hipError_t launch_fv_average<float>(float *, float *, float *, const float *, const float *, const float *,

@@ -33,15 +33,31 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 // Device-resident scalars of one domain (FP64 throughout).
 struct DevScalars {
     double time;
-    double dt;                 // rk2: step taken (device-decided when adaptive)
+    double dt;                 // finite volume: the step's base sub-step (device-decided when adaptive)
     double max_dt;             // CFL step from the latest flux computation
     double bf_store[4];        // boundary_flux_store N, E, S, W
     double bf_evolve;          // boundary_flux_evolve_integral
     double bf_time;            // boundary_flux_time_integral
-    unsigned long long max_dt_inv_bits;   // running max of 1/dt (non-negative doubles order as u64)
     unsigned long long neg_depth_count;   // negative_depth_fix_counter
-    double pad[4];
+    double first_max_dt;       // CFL step of the step's first flux computation (the reported max_dt)
+    double time0;              // time at the start of the step
+    double stage_dt;           // step applied by the current update_U (dt_last_update)
+    double evolve_dt;          // evolve_step_dt = time - time0
 };
+
+// Bookkeeping of one finite-volume Euler stage (fv_scalar_kernel), after
+// one_euler_step / one_midpoint_step / one_rk2n_step
+// (domain_mod_timestepping_alternatives_include.f90:8-336):
+//   kStageFirst      the step's first flux computation: dt = fixed step or the CFL step,
+//                    and its CFL step is the one reported
+//   kStageAccBf      boundary_flux_evolve_integral += stage_dt * sum(boundary_flux_store)
+//   kStageFromTime0  time = time0 + stage_dt (midpoint's full step from the backup)
+//   kStageScaleBfN   first scale boundary_flux_evolve_integral by (n-1)/n (rk2n)
+enum : int { kStageFirst = 1, kStageAccBf = 2, kStageFromTime0 = 4, kStageScaleBfN = 8 };
+// rk2n_n_value (domain_mod.f90:88-90)
+constexpr int kRk2nN = 5;
+// fv_finish_kernel modes
+enum : int { kFinishEuler = 0, kFinishRk2 = 1, kFinishRk2n = 2 };
 
 // Per-row step coefficients of the leapfrog scheme, built in FP64 on the host from
 // the reference's metric arrays and converted to the storage type.
@@ -145,9 +161,10 @@ struct FVArgs {
     int nx, ny, pitch;           // state planes: pitch x ny
     int fpitch;                  // flux planes: fpitch x (ny+1), fpitch >= nx+1
     T *S, *UH, *VH;              // state being stepped (in place)
-    const T *S0, *UH0, *VH0;     // state at the start of the Euler stage (read by the clip pass)
+    const T *S0, *UH0, *VH0;     // state the fluxes are computed from (clipped where it is read)
     const T *E;
-    T *depth, *u, *v;            // per-cell work (unused: evaluated where needed)
+    const T *Sq, *UHq, *VHq;     // state update_U adds to; null: the clipped S0/UH0/VH0
+                                 // (midpoint's second update starts from the backup)
     double *bmax;                // per-block maximum of max_speed / (dx * cfl / 2) of the flux pass
     T *fNS, *fEW;                // 3 planes each: mass, UH, VH fluxes (pre-multiplied by edge length)
     double *esU, *esV;           // cell-local explicit sources
@@ -162,7 +179,11 @@ struct FVArgs {
 };
 
 template <typename T>
-hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, int first_stage, hipStream_t s);
+hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, int mode, hipStream_t s);
+// rk2n: part 0: B0 = (B0 - U) / n;  part 1: U = U + B0
+template <typename T>
+hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
+                                  hipStream_t s);
 template <typename T>
 hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny,
                              int pitch, hipStream_t s);
@@ -172,7 +193,7 @@ hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int
 hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s);
 // number of blocks of the flux pass (size of FVArgs::bmax)
 int fv_flux_blocks(int nx, int ny);
-hipError_t launch_fv_finish(DevScalars *sc, hipStream_t s);
+hipError_t launch_fv_finish(DevScalars *sc, int mode, hipStream_t s);
 
 
 // ---------------------------------------------------------------------------

@@ -3,7 +3,7 @@ solver (compiled from the reference sources into oracle/_ref/ by oracle/Makefile
 small seeded problems. Each .npz holds the inputs, the parameters and the reference's
 outputs: U after the run, the max_U grids, and per-step scalars.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]     (default: every problem)
 """
 import json
 import os
@@ -32,11 +32,16 @@ def problems():
     U[0] = np.maximum(U[0], U[3])
     rk2s = dict(sph, U=U, method="rk2", manning_squared=np.full((40, 48), 0.025 ** 2))
     return {"linear_cart": (lin, {}, 30), "llpnf_sph": (llp, {}, 30), "rk2_beach": (beach, {}, 25),
-            "rk2_sph": (rk2s, {}, 15), "linear_cart_ntl": (lin, {"truely_linear": False}, 30)}
+            "rk2_sph": (rk2s, {}, 15), "linear_cart_ntl": (lin, {"truely_linear": False}, 30),
+            "euler_beach": (dict(beach, method="euler"), {}, 25),
+            "midpoint_beach": (dict(beach, method="midpoint"), {}, 25),
+            "rk2n_sph": (dict(rk2s, method="rk2n"), {}, 12)}
 
 
-def main():
+def main(names=None):
     for name, (cfg, kw, nsteps) in problems().items():
+        if names and name not in names:
+            continue
         d, dt, hist = H.cpu_run("ref", cfg, nsteps, record=lambda x: x.scalars(), **kw)
         meta = dict(nx=cfg["nx"], ny=cfg["ny"], lw=list(cfg["lw"]), ll=list(cfg["ll"]), method=cfg["method"],
                     spherical=bool(cfg["spherical"]), nsteps=nsteps, dt=dt, kw=kw,
@@ -54,4 +59,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -23,6 +23,10 @@ def small_beach(nx=80, ny=40, dx=10.0):
     return C.beach_rk2(nx=nx, ny=ny, dx=dx)
 
 
+# finite-volume methods: without a timestep argument they take the CFL step
+FV_METHODS = ("rk2", "rk2n", "midpoint", "euler")
+
+
 def cpu_run(kind, cfg, nsteps, dt=None, record=None, **kw):
     """Run nsteps of evolve_one_step; returns (domain, per-step scalars list)."""
     d = pyoracle.CpuDomain(kind, cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], method=cfg["method"],
@@ -33,7 +37,7 @@ def cpu_run(kind, cfg, nsteps, dt=None, record=None, **kw):
     if "stage_bc" in cfg:
         b = cfg["stage_bc"]
         d.set_stage_bc(b["amp"], b["period"], 0.0, b["sides"])
-    if dt is None and cfg["method"] != "rk2":
+    if dt is None and cfg["method"] not in FV_METHODS:
         dt = d.stationary_timestep_max()
     hist = []
     for _ in range(nsteps):

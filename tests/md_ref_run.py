@@ -3,8 +3,11 @@ writes what the GPU parity tests compare against. Started as a child process by
 tests/test_gpu_multidomain.py so that the reference (whose md%setup runs shell
 commands to make its output folders) never shares a process with the GPU.
 
-usage: python tests/md_ref_run.py OUT_PREFIX NSTEPS [planar]
+usage: python tests/md_ref_run.py OUT_PREFIX NSTEPS [planar | flow:METHOD]
 writes OUT_PREFIX.json (setup tables) and OUT_PREFIX.npz (arrays).
+
+With 'flow:METHOD' the grandchild uses that finite-volume method ('euler',
+'midpoint', 'rk2n') instead of 'rk2'.
 
 With 'planar', every domain is 'rk2' and holds the planar field of the
 reference's own nesting test (multidomain_mod.f90:4088-4101): U(:, :, k) =
@@ -39,6 +42,8 @@ def planar(x, y, k, x0, y0):
 
 def main(prefix, nsteps, mode="flow"):
     cfg = configs.nested_small()
+    if mode.startswith("flow:"):
+        cfg["domains"][2]["method"] = mode.split(":", 1)[1]
     if mode == "planar":
         for d in cfg["domains"]:
             d["method"] = "rk2"

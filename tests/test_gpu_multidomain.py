@@ -152,6 +152,45 @@ def test_steps_fp32k_error_and_mass(ref):
         _close(doms, md)
 
 
+@pytest.fixture(scope="module", params=["euler", "midpoint", "rk2n"])
+def ref_fv(request, tmp_path_factory):
+    """The same nest with another finite-volume method in the grandchild."""
+    if not os.path.exists(REF_LIB):
+        pytest.skip("compiled reference not built (make -C oracle ref)")
+    prefix = str(tmp_path_factory.mktemp("mdref_" + request.param) / "run")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    print("reference multidomain run, grandchild", request.param, prefix, flush=True)
+    subprocess.run([sys.executable, os.path.join(ROOT, "tests", "md_ref_run.py"), prefix, str(NSTEPS),
+                    "flow:" + request.param], check=True, env=env, timeout=300)
+    with open(prefix + ".json") as f:
+        tables = json.load(f)
+    return tables, dict(np.load(prefix + ".npz"))
+
+
+def test_steps_other_fv_grandchild_fp64(ref_fv):
+    """20 global steps with an 'euler', 'midpoint' or 'rk2n' grandchild (their own
+    nesting flux-integral bookkeeping, timestepping_alternatives_include.f90:8-336)
+    against the reference's multidomain, at the bar of the rk2 nest."""
+    tables, arrays = ref_fv
+    dt = tables["dt"]
+    print("multidomain fp64, grandchild", tables["methods"][2], flush=True)
+    doms, md = build_gpu(tables, arrays, "fp64")
+    try:
+        md.make_initial_conditions_consistent()
+        v0 = md.volumes()
+        md.evolve_steps(dt, NSTEPS)
+        for k, d in enumerate(doms):
+            U = d.U
+            for v in range(3):
+                err = _rel(U[v], arrays["UN_%d" % k][v])
+                assert err < 1e-9, (k, v, err)
+        v = md.volumes()
+        np.testing.assert_allclose(v, arrays["volN"], rtol=1e-9)
+        assert abs(v.sum() - v0.sum()) <= 1e-12 * v0.sum()
+    finally:
+        _close(doms, md)
+
+
 @pytest.fixture(scope="module")
 def ref_planar(tmp_path_factory):
     if not os.path.exists(REF_LIB):

@@ -141,23 +141,25 @@ def test_leapfrog_fp32k_vs_fp64_oracle(oracle_built, case):
     g.close()
 
 
-@pytest.mark.parametrize("case", ["beach", "sph"])
-def test_rk2_fp64(oracle_built, case):
-    """rk2 (DE1 low-Froude HLL, wet/dry, Manning) with adaptive CFL steps decided on
-    the device, against the oracle. Device pow/sin vs glibc differ by ulps, so the bar
-    is round-off: states within 1e-9 (absolute, metres and m^2/s), max_dt and time
-    within 1e-12 relative."""
-    _print_case("rk2_" + case)
+def _fv_case(case, method):
     if case == "beach":
-        cfg = H.small_beach()
-        nsteps = 60
-    else:
-        cfg = H.small_spherical()
-        U = cfg["U"].copy()
-        U[3] = np.maximum(U[3], -300.0)
-        U[0] = np.maximum(U[0], U[3])
-        cfg = dict(cfg, U=U, method="rk2", manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.025 ** 2))
-        nsteps = 30
+        return dict(H.small_beach(), method=method), 60
+    cfg = H.small_spherical()
+    U = cfg["U"].copy()
+    U[3] = np.maximum(U[3], -300.0)
+    U[0] = np.maximum(U[0], U[3])
+    return dict(cfg, U=U, method=method, manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.025 ** 2)), 30
+
+
+@pytest.mark.parametrize("method", ["rk2", "euler", "midpoint", "rk2n"])
+@pytest.mark.parametrize("case", ["beach", "sph"])
+def test_fv_fp64(oracle_built, case, method):
+    """The finite-volume schemes (DE1 low-Froude HLL, wet/dry, Manning): rk2, euler,
+    midpoint and rk2n, with adaptive CFL steps decided on the device, against the
+    oracle. Device pow/sin vs glibc differ by ulps, so the bar is round-off: states
+    within 1e-9 (absolute, metres and m^2/s), max_dt and time within 1e-12 relative."""
+    _print_case(method + "_" + case)
+    cfg, nsteps = _fv_case(case, method)
     o, _, hist = H.cpu_run("oracle", cfg, nsteps, record=lambda d: d.scalars())
     g = H.gpu_domain(cfg)
     for k in range(nsteps):
@@ -165,17 +167,71 @@ def test_rk2_fp64(oracle_built, case):
         sg = g.scalars()
         assert sg["max_dt"] == pytest.approx(hist[k]["max_dt"], rel=1e-12)
         assert sg["time"] == pytest.approx(hist[k]["time"], rel=1e-12)
+        assert sg["evolve_step_dt"] == pytest.approx(hist[k]["evolve_step_dt"], rel=1e-12)
+        assert sg["dt_last_update"] == pytest.approx(hist[k]["dt_last_update"], rel=1e-12)
     Ug, Uo = g.U, o.get_U()
     wet = cfg["U"][3] < 1e5
     for k in range(3):
         d = np.abs(Ug[k] - Uo[k])[wet].max()
-        print("rk2 plane", k, "max diff", d)
+        print(method, "plane", k, "max diff", d)
         assert d <= 1e-9, k
     sg, so = g.scalars(), o.scalars()
     assert sg["negative_depth_fix_counter"] == so["negative_depth_fix_counter"]
     assert sg["boundary_flux_time_integral"] == pytest.approx(so["boundary_flux_time_integral"], rel=1e-9, abs=1e-9)
     for q in ["max_stage", "max_flux", "min_stage"]:
         assert np.abs(g.max_U(q) - o.max_U(q))[wet].max() <= 1e-9, q
+    g.close()
+
+
+@pytest.mark.parametrize("method", ["rk2", "euler", "midpoint", "rk2n"])
+def test_fv_fixed_timestep_fp64(oracle_built, method):
+    """The same schemes with a timestep argument (60 % of the stationary CFL step),
+    several steps per call (swh_evolve_steps), against the oracle to round-off."""
+    _print_case(method + "_fixed_dt")
+    cfg, _ = _fv_case("beach", method)
+    o = H.cpu_run("oracle", cfg, 0)[0]
+    dt = 0.6 * o.stationary_timestep_max()
+    o, _, _ = H.cpu_run("oracle", cfg, 40, dt=dt)
+    g = H.gpu_domain(cfg)
+    g.evolve_steps(dt, 40)
+    sg, so = g.scalars(), o.scalars()
+    for k in ["time", "max_dt", "evolve_step_dt", "dt_last_update"]:
+        assert sg[k] == pytest.approx(so[k], rel=1e-12), k
+    assert sg["boundary_flux_time_integral"] == pytest.approx(so["boundary_flux_time_integral"], rel=1e-9, abs=1e-9)
+    Ug, Uo = g.U, o.get_U()
+    for k in range(3):
+        assert np.abs(Ug[k] - Uo[k]).max() <= 1e-9, k
+    g.close()
+
+
+@pytest.mark.parametrize("method", ["rk2", "euler", "midpoint", "rk2n"])
+def test_fv_fp32_vs_fp64_oracle(oracle_built, method):
+    """FP32 state (FP64 fluxes of mass and the explicit sources, FP64 CFL and flux
+    reductions) against the FP64 oracle on the 80 x 40 beach (1 m sine at the west
+    edge, 1:50 shore with wet/dry): the CFL steps decided on the device stay within
+    1e-3 relative of the oracle's (the fastest cell sits at the wet/dry front, where
+    u = uh / h magnifies FP32 rounding of a small depth), and after 60 steps the
+    stage is within 1e-4 m wherever the oracle is wet.
+
+    rk2n runs at CFL 0.5 here: at its default 0.99 the scheme itself is unstable on
+    this beach (the FP64 oracle and the reference grow a 1 m wave to 68 m within 200
+    steps; a float32 build of the reference diverges after 25 steps), so FP32
+    rounding only seeds the growth sooner."""
+    _print_case(method + "_fp32")
+    cfg, nsteps = _fv_case("beach", method)
+    kw = dict(cfl=0.5) if method == "rk2n" else {}
+    o, _, hist = H.cpu_run("oracle", cfg, nsteps, record=lambda d: d.scalars(), **kw)
+    g = H.gpu_domain(cfg, precision="fp32k", **kw)
+    dt_dev = 0.0
+    for k in range(nsteps):
+        g.evolve_one_step()
+        dt_dev = max(dt_dev, abs(g.scalars()["max_dt"] / hist[k]["max_dt"] - 1.0))
+    Ug, Uo = g.U, o.get_U()
+    wet = (Uo[0] - Uo[3]) > 1e-3
+    err = np.abs(Ug[0] - Uo[0])[wet].max()
+    print(method, "fp32 stage error (m):", err, "largest relative CFL-step deviation:", dt_dev)
+    assert dt_dev <= 1e-3
+    assert err <= 1e-4
     g.close()
 
 

@@ -45,6 +45,26 @@ def _rk2_sph():
 
 
 CASES["rk2_sph"] = (_rk2_sph, {}, 30)
+# the other finite-volume schemes (timestepping_alternatives_include.f90:8-62, 149-336)
+for _m in ("euler", "midpoint", "rk2n"):
+    CASES[_m + "_beach"] = ((lambda m: lambda: dict(H.small_beach(), method=m))(_m), {}, 40)
+    CASES[_m + "_sph"] = ((lambda m: lambda: dict(_rk2_sph(), method=m))(_m), {}, 20)
+
+
+@pytest.mark.parametrize("method", ["euler", "midpoint", "rk2n", "rk2"])
+def test_fv_fixed_timestep_equals_compiled_reference(oracle_built, method):
+    """The finite-volume schemes with a timestep argument (60 % of the stationary CFL
+    step) instead of the CFL step."""
+    cfg = dict(H.small_beach(), method=method)
+    _need_ref(False)
+    o = H.cpu_run("oracle", cfg, 0)[0]
+    dt = 0.6 * o.stationary_timestep_max()
+    o, _, ho = H.cpu_run("oracle", cfg, 30, dt=dt, record=lambda d: d.scalars())
+    r, _, hr = H.cpu_run("ref", cfg, 30, dt=dt, record=lambda d: d.scalars())
+    assert np.array_equal(o.get_U(), r.get_U())
+    for a, b in zip(ho, hr):
+        for k in a:
+            assert a[k] == b[k], k
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
