@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s/GPU FP32-comp vs FP64 leapfrog; % HBM peak; unit-sources/hr"
 HBM_PEAK_GBS = 8000.0
+MFMA_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
 # Algorithmic HBM bytes per cell-update of the fused batched leapfrog step (one read
 # and one write of stage, Kahan term, uh, vh per source; elevation shared):
 BYTES_PER_CELL = {"fp32k": 32.0, "fp64": 48.0}
@@ -216,6 +217,56 @@ def secondary_legs(steps):
     return out
 
 
+def superpose_leg(b, nsrc, max_samples, n_events=4096, seed=20261203):
+    """Earthquake events from the finished batch's gauge series on the FP64 matrix
+    cores: rupture patches of 1-8 x 1-16 unit sources of the 32 x 16 lattice
+    (uniform slip, then stochastic slip), summarised as the maximum stage per gauge;
+    and 64 events as whole series. FLOP/s counts the dense product the kernel does
+    (2 x events x sources (padded to 4) x columns), against the FP64 MFMA peak."""
+    rng = np.random.default_rng(seed)
+    nxs, nys = 32, max(1, nsrc // 32)
+    events, weights = [], []
+    for _ in range(n_events):
+        w, l = int(rng.integers(1, min(8, nys) + 1)), int(rng.integers(1, 17))
+        i0, j0 = int(rng.integers(0, nxs - l + 1)), int(rng.integers(0, nys - w + 1))
+        src = [j * nxs + i for j in range(j0, j0 + w) for i in range(i0, i0 + l) if j * nxs + i < nsrc]
+        events.append(src)
+        weights.append(rng.lognormal(0.0, 0.5, size=len(src)))
+    slip = rng.uniform(0.5, 10.0, size=n_events)
+    ng = b._ng
+    kpad = (nsrc + 3) // 4 * 4
+    out = {}
+    for name, kw in (("uniform", dict(slip=slip)), ("stochastic", dict(weights=weights))):
+        b.superpose(events[:64], summary="max_stage", **{k: v[:64] for k, v in kw.items()})
+        _, nv, ms = b.superpose(events, summary="max_stage", timed=True, **kw)
+        flops = 2.0 * n_events * kpad * float(nv.max()) * ng
+        out["max_stage_" + name] = dict(events=n_events, kernel_ms=round(ms, 3),
+                                        events_per_s=round(n_events / (ms * 1e-3), 1),
+                                        dense_tflops=round(flops / (ms * 1e-3) / 1e12, 2),
+                                        mfma_fp64_frac=round(flops / (ms * 1e-3) / 1e12 / MFMA_FP64_TFLOPS, 4))
+    _, nv, ms = b.superpose(events[:64], slip=slip[:64], summary="series", timed=True)
+    flops = 2.0 * 64 * kpad * float(max_samples) * 3 * ng
+    out["series_64_events"] = dict(kernel_ms=round(ms, 3), dense_tflops=round(flops / (ms * 1e-3) / 1e12, 2))
+    out["_events"] = (events[:64], slip[:64])
+    log("superpose", out)
+    return out
+
+
+def cpu_baseline_superpose(b, events, slip):
+    """CPU baseline of the event summation: the oracle's restatement of the
+    reference's R summation (numpy, one core) on 64 events (maximum stage)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import superpose as SP
+    need = sorted({k for ev in events for k in ev})
+    series = {k: b.gauge_series(k) for k in need}
+    ns = {k: len(v) for k, v in series.items()}
+    t0 = time.perf_counter()
+    for e in range(len(events)):
+        SP.event_max_stage(series, ns, events[e], slip=slip[e])
+    return dict(value=round(len(events) / (time.perf_counter() - t0), 1), unit="events/s", cores=1, kind="port",
+                sample="%d events (maximum stage at every gauge) of the same series" % len(events))
+
+
 def load_nested_tables(name="nested_c3_tables.npz"):
     """Nesting geometry of the nested workload (what the reference's md%setup decides
     for ptha_amd.configs.nested_c3; written by tests/golden/make_nested_tables.py)."""
@@ -384,6 +435,14 @@ def main():
         full = dict(unit_sources_per_hr=nsrc / (wall / 3600.0), seconds=wall,
                     gcell_per_s=total_cell_updates / wall / 1e9, sources=nsrc,
                     gauge_samples_per_source=int(b.gauge_series(0).shape[0]))
+        log("earthquake events from the unit-source gauge series")
+        superpose = superpose_leg(b, nsrc, max_samples)
+        ev64, slip64 = superpose.pop("_events")
+        if not args.no_cpu:
+            try:
+                superpose["cpu_baseline"] = cpu_baseline_superpose(b, ev64, slip64)
+            except Exception as e:  # the oracle may be absent from a tree
+                log("superposition cpu baseline unavailable:", e)
         b.close()
 
     if args.copy_probe:
@@ -434,6 +493,7 @@ def main():
     if secondary:
         out["secondary"] = secondary
     if full:
+        out["superpose"] = superpose
         out["unit_sources_per_hr"] = round(full["unit_sources_per_hr"], 1)
         out["full_run"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in full.items()}
     print(json.dumps(out))

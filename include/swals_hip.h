@@ -223,6 +223,24 @@ int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, 
  * elevation set. */
 int swh_batch_autotune(swh_batch *b, double dt, int32_t reps, int32_t *best, double *best_ms);
 int swh_batch_synchronize(swh_batch *b);
+
+/* Earthquake events from the batch's unit-source gauge series, on the device
+ * (make_tsunami_event_from_unit_sources, R/examples/austptha_template/SOURCE_ZONES/
+ * TEMPLATE/TSUNAMI_EVENTS/sum_tsunami_unit_sources.R:555-711). Event e sums the
+ * sources event_src[event_ptr[e] .. event_ptr[e+1]-1] (0-based source indices):
+ * with event_w == NULL each with weight 1, the sum then scaled by event_slip[e]
+ * (uniform slip); otherwise with weights event_w (stochastic slip; event_slip NULL
+ * means 1). The stage has msl subtracted before the sum and added after it.
+ * summary SWH_SUPERPOSE_SERIES: out[e][sample][3][gauge] (max_samples samples per
+ * event; samples past nvalid[e] are not meaningful); SWH_SUPERPOSE_MAX_STAGE:
+ * out[e][gauge] = maximum over the event's valid samples of its stage.
+ * nvalid[e] = samples recorded by every source of the event. kernel_ms (may be NULL)
+ * returns the device time of the products. */
+#define SWH_SUPERPOSE_SERIES 0
+#define SWH_SUPERPOSE_MAX_STAGE 1
+int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr, const int32_t *event_src,
+                        const double *event_w, const double *event_slip, double msl, int32_t summary, double *out,
+                        int32_t *nvalid, double *kernel_ms);
 /* swh_batch_evolve bracketed by HIP events recorded on the batch's stream; returns the
  * device time of the region in milliseconds (for throughput measurement). */
 int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elapsed_ms);

@@ -149,6 +149,42 @@ class UnitSourceBatch:
     def synchronize(self):
         B.check(self._lib.swh_batch_synchronize(self._h), "swh_batch_synchronize")
 
+    def superpose(self, events, slip=None, weights=None, msl=0.0, summary="series", timed=False):
+        """Earthquake events from the recorded unit-source gauge series, on the GPU
+        (make_tsunami_event_from_unit_sources, sum_tsunami_unit_sources.R:555-711).
+
+        events   list of source-index lists (0-based), one per event
+        slip     per-event uniform slip (weights 1, sum scaled by the slip), or None
+        weights  per-event lists of per-source slips (stochastic slip), or None
+        summary  "series": (n_events, samples, 3, gauges) array;
+                 "max_stage": (n_events, gauges) maximum stage over time
+        Returns (values, nvalid) where nvalid[e] is the number of samples recorded by
+        every source of event e (series past it are not meaningful); with timed=True
+        also the device time of the products in ms."""
+        ne = len(events)
+        ptr = np.zeros(ne + 1, dtype=np.int32)
+        for e, ev in enumerate(events):
+            ptr[e + 1] = ptr[e] + len(ev)
+        src = np.ascontiguousarray(np.concatenate([np.asarray(ev, dtype=np.int32) for ev in events])
+                                   if ptr[-1] else np.zeros(1, dtype=np.int32))
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in weights]))
+            if len(w) != ptr[-1]:
+                raise ValueError("weights must give one slip per event source")
+        sl = None if slip is None else np.ascontiguousarray(np.asarray(slip, dtype=np.float64).reshape(ne))
+        mode = {"series": 0, "max_stage": 1}[summary]
+        ncol = self._gmax * 3 * self._ng if mode == 0 else self._ng
+        out = np.empty((ne, ncol))
+        nvalid = np.zeros(ne, dtype=np.int32)
+        ms = ctypes.c_double()
+        B.check(self._lib.swh_batch_superpose(self._h, ne, ptr.ctypes.data_as(B._PI32), src.ctypes.data_as(B._PI32),
+                                              None if w is None else _p(w), None if sl is None else _p(sl),
+                                              float(msl), mode, _p(out), nvalid.ctypes.data_as(B._PI32),
+                                              ctypes.byref(ms)), "swh_batch_superpose")
+        vals = out.reshape(ne, self._gmax, 3, self._ng) if mode == 0 else out
+        return (vals, nvalid, ms.value) if timed else (vals, nvalid)
+
     def status(self):
         step = ctypes.c_int64()
         na = B._I32()

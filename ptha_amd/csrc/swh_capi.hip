@@ -1433,6 +1433,102 @@ int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elap
     return rc;
 }
 
+int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr, const int32_t *event_src,
+                        const double *event_w, const double *event_slip, double msl, int32_t summary, double *out,
+                        int32_t *nvalid, double *kernel_ms)
+{
+    if (!b || n_events < 1 || !event_ptr || !event_src || !out || !nvalid)
+        return fail(SWH_EINVAL, "bad superposition arguments");
+    if (summary != SWH_SUPERPOSE_SERIES && summary != SWH_SUPERPOSE_MAX_STAGE)
+        return fail(SWH_EINVAL, "summary must be SWH_SUPERPOSE_SERIES or SWH_SUPERPOSE_MAX_STAGE");
+    if (b->ng < 1 || !b->d_gval) return fail(SWH_ESTATE, "no gauges recorded (swh_batch_gauges_set)");
+    if (event_ptr[0] != 0) return fail(SWH_EINVAL, "event_ptr[0] must be 0");
+    for (int e = 0; e < n_events; e++)
+        if (event_ptr[e + 1] < event_ptr[e]) return fail(SWH_EINVAL, "event_ptr must be non-decreasing");
+    for (int q = 0; q < event_ptr[n_events]; q++)
+        if (event_src[q] < 0 || event_src[q] >= b->nsrc) return fail(SWH_EINVAL, "event names a source outside the batch");
+    swh_domain *g = b->g;
+    SWH_HIP(hipStreamSynchronize(g->stream));
+    // samples each slot has recorded (as swh_batch_gauges_download reports them)
+    std::vector<int> ns_slot(b->nsrc);
+    int ns_all = 0;
+    for (int k = 0; k < b->nsrc; k++) {
+        const long long taken = std::min<long long>(b->step, b->nsteps_slot[k]);
+        ns_slot[k] = (int)std::min<long long>(taken / b->every + 1, b->gmax);
+        ns_all = std::max(ns_all, ns_slot[k]);
+    }
+    const int TE = superpose_event_tile();
+    const int Kpad = (b->nsrc + 3) / 4 * 4;
+    const long long ncol = summary == SWH_SUPERPOSE_SERIES ? (long long)b->gmax * 3 * b->ng : b->ng;
+    // events per launch: bounded device output (<= 2^28 doubles) and weight matrix
+    long long chunk = std::max<long long>(TE, ((1LL << 28) / std::max<long long>(ncol, Kpad)) / TE * TE);
+    chunk = std::min<long long>(chunk, (n_events + TE - 1) / TE * TE);
+    double *d_w = nullptr, *d_scale = nullptr, *d_out = nullptr;
+    int *d_nv = nullptr;
+    int rc = dmalloc((void **)&d_w, sizeof(double) * chunk * Kpad);
+    if (!rc) rc = dmalloc((void **)&d_scale, sizeof(double) * chunk);
+    if (!rc) rc = dmalloc((void **)&d_nv, sizeof(int) * chunk);
+    if (!rc) rc = dmalloc((void **)&d_out, sizeof(double) * chunk * ncol);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double total_ms = 0.0;
+    if (!rc && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess))
+        rc = fail(SWH_EHIP, "hipEventCreate failed");
+    std::vector<double> W, scale;
+    std::vector<int> nv;
+    for (long long e0 = 0; !rc && e0 < n_events; e0 += chunk) {
+        const int E = (int)std::min<long long>(chunk, n_events - e0);
+        W.assign((size_t)chunk * Kpad, 0.0);
+        scale.assign(chunk, 0.0);
+        nv.assign(chunk, 0);
+        int smax = 1;
+        for (int e = 0; e < E; e++) {
+            const int ev = (int)e0 + e;
+            int n_e = event_ptr[ev + 1] > event_ptr[ev] ? b->gmax : ns_all;
+            for (int q = event_ptr[ev]; q < event_ptr[ev + 1]; q++) {
+                const int slot = b->slot_of[event_src[q]];
+                // uniform slip: unit weights, the slip scales the sum
+                // (sum_tsunami_unit_sources.R:658-669); stochastic: per-source slips (:672-687)
+                W[(size_t)e * Kpad + slot] += event_w ? event_w[q] : 1.0;
+                n_e = std::min(n_e, ns_slot[slot]);
+            }
+            scale[e] = event_slip ? event_slip[ev] : 1.0;
+            nv[e] = n_e;
+            nvalid[ev] = n_e;
+            smax = std::max(smax, n_e);
+        }
+        if (hipMemcpy(d_w, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_scale, scale.data(), sizeof(double) * chunk, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_nv, nv.data(), sizeof(int) * chunk, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(SWH_EHIP, "superposition upload failed");
+            break;
+        }
+        SupArgs a;
+        a.X = b->d_gval; a.slot_stride = (long long)b->gmax * 3 * b->ng;
+        a.nsrc = b->nsrc; a.ng = b->ng; a.gmax = b->gmax;
+        a.W = d_w; a.Kpad = Kpad; a.E = E; a.scale = d_scale; a.nvalid = d_nv; a.msl = msl;
+        a.s_chunk = 16; a.s_max = std::min(smax, b->gmax);
+        a.out = d_out;
+        hipError_t he = hipEventRecord(t0, g->stream);
+        if (he == hipSuccess) he = launch_superpose(a, summary == SWH_SUPERPOSE_SERIES ? 0 : 1, g->stream);
+        if (he == hipSuccess) he = hipEventRecord(t1, g->stream);
+        if (he == hipSuccess) he = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, t0, t1);
+        if (he == hipSuccess)
+            he = hipMemcpy(out + e0 * ncol, d_out, sizeof(double) * E * ncol, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) {
+            rc = hip_fail(he, "superposition", __FILE__, __LINE__);
+            break;
+        }
+        total_ms += ms;
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    dfree(d_w); dfree(d_scale); dfree(d_nv); dfree(d_out);
+    if (!rc && kernel_ms) *kernel_ms = total_ms;
+    return rc;
+}
+
 int swh_batch_synchronize(swh_batch *b)
 {
     if (!b) return fail(SWH_EINVAL, "null argument");

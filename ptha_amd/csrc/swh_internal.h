@@ -108,6 +108,25 @@ hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode,
 // Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
 hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
 
+// Events from unit-source gauge series (swh_superpose.hip): out = W * (X - msl on the
+// stage) scaled and shifted per event.
+struct SupArgs {
+    const double *X;         // gauge series [slot][sample][3][ng]
+    long long slot_stride;   // gmax * 3 * ng
+    int nsrc, ng, gmax;
+    const double *W;         // [Epad][Kpad] event weights by slot, zero padded
+    int Kpad;                // nsrc rounded up to a multiple of 4
+    int E;                   // events (W, scale and nvalid hold Epad rows)
+    const double *scale;     // per event: uniform slip (or 1)
+    const int *nvalid;       // per event: samples common to all its sources
+    double msl;
+    int s_chunk, s_max;      // maximum-stage summary: samples per block, samples to scan
+    double *out;             // mode 0: [E][gmax * 3 * ng]; mode 1: [E][ng]
+};
+// rows of the event tile (W, scale and nvalid must hold a multiple of it)
+int superpose_event_tile();
+hipError_t launch_superpose(const SupArgs &a, int mode, hipStream_t s);
+
 // Gauge values after one step, from the current state (batched runs).
 template <typename T, bool KAHAN>
 hipError_t launch_gauge_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,

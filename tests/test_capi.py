@@ -58,3 +58,4 @@ def test_argument_checks_of_md_batch_and_probe_without_device():
     assert lib.swh_md_finalize(None) == -1
     assert lib.swh_md_evolve_one_step(None, 1.0) != 0
     assert lib.swh_batch_autotune(None, 1.0, 1, None, None) == -1
+    assert lib.swh_batch_superpose(None, 1, None, None, None, None, 0.0, 0, None, None, None) == -1

@@ -1,0 +1,96 @@
+"""GPU: earthquake events from the batch's unit-source gauge series
+(swh_batch_superpose, FP64 matrix-core products) against the oracle's restatement of
+make_tsunami_event_from_unit_sources (oracle/superpose.py), and against the solver
+run of an event's own initial condition (linearity).
+
+The device sums the sources in a different order (4 sources per matrix instruction)
+from the reference's sequential axpy, so the bar is FP64 round-off: 1e-12 of the
+largest series value.
+"""
+import numpy as np
+import pytest
+
+import superpose as SP
+from ptha_amd import configs as C
+
+pytestmark = pytest.mark.gpu
+
+NSRC = 5
+
+
+def _batch(nsteps, every=4, max_samples=40, precision="fp64"):
+    from ptha_amd.batch import UnitSourceBatch
+    u = C.unit_sources(nx=96, ny=72, arcmin=10.0, n_src_x=5, n_src_y=1, spacing_km=150, sigma_km=50,
+                       n_gauges_side=3)
+    b = UnitSourceBatch(u["nx"], u["ny"], u["lw"], u["ll"], NSRC, precision=precision)
+    b.set_elevation(u["elev"])
+    b.set_nsteps(np.asarray(nsteps, dtype=np.int32))
+    bumps = []
+    for k in range(NSRC):
+        s = C.gaussian_on_sphere(u["x"], u["y"], *u["centres"][k], 50e3)
+        s[u["elev"] >= 0] = 0.0
+        bumps.append(s)
+        b.set_source(k, s)
+    b.setup_point_gauges(u["gauges_xy"], every=every, max_samples=max_samples)
+    return u, b, bumps
+
+
+def _events(rng, n):
+    ev, w, slip = [], [], []
+    for _ in range(n):
+        m = int(rng.integers(1, NSRC + 1))
+        src = sorted(rng.choice(NSRC, size=m, replace=False).tolist())
+        ev.append(src)
+        w.append(rng.normal(size=m).tolist())
+        slip.append(float(rng.uniform(0.5, 8.0)))
+    return ev, w, slip
+
+
+def test_superpose_matches_oracle():
+    print("CASE superpose fp64: 150 events, 5 sources, 9 gauges", flush=True)
+    nsteps = [150, 120, 150, 97, 133]
+    u, b, _ = _batch(nsteps)
+    dt = 20.0
+    b.evolve(dt, 160)
+    series = [b.gauge_series(k) for k in range(NSRC)]
+    ns = [len(s) for s in series]
+    scale = max(np.abs(s).max() for s in series)
+    rng = np.random.default_rng(11)
+    ev, w, slip = _events(rng, 150)
+    for msl in [0.0, 0.4]:
+        # uniform slip, whole series
+        got, nv = b.superpose(ev, slip=slip, msl=msl, summary="series")
+        for e in range(len(ev)):
+            ref = SP.event_series(series, ns, ev[e], slip=slip[e], msl=msl)
+            assert nv[e] == len(ref), e
+            assert np.abs(got[e, :nv[e]] - ref).max() <= 1e-12 * scale * max(slip), e
+        # stochastic slip, maximum stage
+        got, nv = b.superpose(ev, weights=w, msl=msl, summary="max_stage")
+        for e in range(len(ev)):
+            ref = SP.event_max_stage(series, ns, ev[e], weights=w[e], msl=msl)
+            assert np.abs(got[e] - ref).max() <= 1e-12 * scale * NSRC * 3, e
+    b.close()
+
+
+def test_superposed_event_equals_run_of_event():
+    """Linearity: the event built on the device equals the solver's run of the event's
+    initial condition (a one-source batch), to round-off."""
+    print("CASE superpose linearity", flush=True)
+    nsteps = [80] * NSRC
+    u, b, bumps = _batch(nsteps, every=5, max_samples=17)
+    dt = 20.0
+    b.evolve(dt, 80)
+    w = [0.5, -1.0, 0.0, 2.0, 1.5]
+    got, nv = b.superpose([list(range(NSRC))], weights=[w], summary="series")
+    from ptha_amd.batch import UnitSourceBatch
+    one = UnitSourceBatch(u["nx"], u["ny"], u["lw"], u["ll"], 1, precision="fp64")
+    one.set_elevation(u["elev"])
+    one.set_nsteps(np.array([80], dtype=np.int32))
+    one.set_source(0, sum(wk * bk for wk, bk in zip(w, bumps)))
+    one.setup_point_gauges(u["gauges_xy"], every=5, max_samples=17)
+    one.evolve(dt, 80)
+    direct = one.gauge_series(0)
+    assert nv[0] == len(direct) == 17
+    assert np.abs(got[0] - direct).max() <= 1e-11 * np.abs(direct).max()
+    one.close()
+    b.close()
