@@ -72,6 +72,8 @@ def _load(kind, spherical):
     f("domain_get_max_U", None, [hd, _I, _PD])
     f("domain_set_stage_bc", None, [hd, _D, _D, _D, ctypes.POINTER(_I)])
     f("domain_set_record_max_U", None, [hd, _I])
+    if kind == "ref":
+        f("domain_sample", None, [hd, _I, ctypes.POINTER(_I), _PD])
     _libs[key] = fns
     return fns
 
@@ -180,6 +182,13 @@ class CpuDomain:
         s = (_I * 4)(*sides)
         self.f["domain_set_stage_bc"](self.h, float(amp), float(period), float(msl), s)
 
+    def sample(self, ij):
+        """[stage, uh, vh] (3, n) at 1-based (i, j) cells (compiled reference only)."""
+        ij = np.ascontiguousarray(np.asarray(ij, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros(3 * len(ij))
+        self.f["domain_sample"](self.h, len(ij), ij.ctypes.data_as(ctypes.POINTER(_I)), _p(out))
+        return out.reshape(3, len(ij))
+
 
 def _load_md(spherical):
     """The multidomain entry points of the compiled reference (ref kind only)."""
@@ -203,7 +212,9 @@ def _load_md(spherical):
     f("set_domain", None, [_I, _I, _I, _PD, _PD, _I, _I, _D])
     f("set_nested", None, [_I, _I, _PD, _PD, _D, _I, _I])
     f("set_params", None, [_I, _I, _D, _D])
-    f("setup", None, [])
+    f("setup", None, [_I])
+    f("sample", None, [_I, _I, _PI, _PD])
+    f("get_max_stage", None, [_I, _PD])
     f("dims", None, [_I, _PI])
     f("geometry", None, [_I, _PD, _PD, _PD])
     f("metrics", None, [_I] + [_PD] * 6)
@@ -240,7 +251,7 @@ class RefMultidomain:
     The reference writes its log and output folders under the working directory,
     so ``setup`` runs inside ``workdir`` (a scratch folder)."""
 
-    def __init__(self, domains, spherical=True, workdir=None):
+    def __init__(self, domains, spherical=True, workdir=None, max_stage=False):
         import tempfile
         self.spherical = spherical
         self.f = _load_md(spherical)
@@ -267,10 +278,23 @@ class RefMultidomain:
         cwd = os.getcwd()
         os.chdir(self.workdir)
         try:
-            self.f["setup"]()
+            self.f["setup"](int(bool(max_stage)))
         finally:
             os.chdir(cwd)
         self.methods = [d["method"] for d in domains]
+
+    def sample(self, k, ij):
+        """[stage, uh, vh] (3, n) of domain k at 1-based (i, j) cells."""
+        ij = np.ascontiguousarray(np.asarray(ij, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros(3 * len(ij))
+        self.f["sample"](k + 1, len(ij), _pi(ij), _p(out))
+        return out.reshape(3, len(ij))
+
+    def max_stage(self, k):
+        d = self.dims(k)
+        out = np.zeros((d["ny"], d["nx"]))
+        self.f["get_max_stage"](k + 1, _p(out))
+        return out
 
     def dims(self, k):
         out = np.zeros(8, dtype=np.int32)

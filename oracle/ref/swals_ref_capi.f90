@@ -107,6 +107,19 @@ contains
         U(1:4*n) = real(reshape(pool(h)%U(:,:,1:4), [4*n]), c_double)
     end subroutine
 
+    subroutine ref_domain_sample(h, n, ij, out) bind(C, name='ref_domain_sample')
+        ! [STG, UH, VH] at n cells (1-based i, j pairs): out((var - 1) * n + g)
+        integer(c_int), value :: h, n
+        integer(c_int), intent(in) :: ij(2, *)
+        real(c_double), intent(out) :: out(*)
+        integer :: g, v
+        do v = 1, 3
+            do g = 1, n
+                out((v - 1) * n + g) = real(pool(h)%U(ij(1, g), ij(2, g), v), c_double)
+            end do
+        end do
+    end subroutine
+
     subroutine ref_domain_set_manning_squared(h, m) bind(C, name='ref_domain_set_manning_squared')
         integer(c_int), value :: h
         real(c_double), intent(in) :: m(*)
@@ -273,13 +286,40 @@ contains
         md%domains(k)%linear_friction_coeff = lfc
     end subroutine
 
-    subroutine ref_md_setup() bind(C, name='ref_md_setup')
+    subroutine ref_md_setup(max_stage) bind(C, name='ref_md_setup')
+        ! max_stage /= 0: every domain tracks max_stage (update_max_quantities)
+        integer(c_int), value :: max_stage
         integer :: k
         do k = 1, size(md%domains)
-            md%domains(k)%record_max_U = .false.
-            md%domains(k)%nontemporal_grids_to_store = ['']
+            md%domains(k)%record_max_U = (max_stage /= 0)
+            if(max_stage /= 0) then
+                md%domains(k)%nontemporal_grids_to_store = [character(len=charlen):: 'max_stage']
+            else
+                md%domains(k)%nontemporal_grids_to_store = ['']
+            end if
         end do
         call md%setup(verbose=.false.)
+    end subroutine
+
+    subroutine ref_md_sample(k, n, ij, out) bind(C, name='ref_md_sample')
+        ! [STG, UH, VH] of domain k at n cells (1-based i, j pairs): out((var - 1) * n + g)
+        integer(c_int), value :: k, n
+        integer(c_int), intent(in) :: ij(2, *)
+        real(c_double), intent(out) :: out(*)
+        integer :: g, v
+        do v = 1, 3
+            do g = 1, n
+                out((v - 1) * n + g) = real(md%domains(k)%U(ij(1, g), ij(2, g), v), c_double)
+            end do
+        end do
+    end subroutine
+
+    subroutine ref_md_get_max_stage(k, out) bind(C, name='ref_md_get_max_stage')
+        integer(c_int), value :: k
+        real(c_double), intent(out) :: out(*)
+        integer :: n
+        n = md%domains(k)%nx(1) * md%domains(k)%nx(2)
+        out(1:n) = real(reshape(md%domains(k)%max_U(:, :, 1), [n]), c_double)
     end subroutine
 
     subroutine ref_md_dims(k, out) bind(C, name='ref_md_dims')

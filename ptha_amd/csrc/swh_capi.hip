@@ -281,28 +281,26 @@ int fv_step(swh_domain *h, double dt)
     const int acc = kStageAccBf;
     int rc = SWH_OK;
 #define STAGE(...) do { rc = fv_stage<T>(h, __VA_ARGS__); if (rc) return rc; } while (0)
-    SWH_HIP(launch_fv_begin(h->sc, h->stream));
+    const int first = kStageFirst | kStageBegin;
     switch (h->method) {
     case SWH_METHOD_EULER:
         // one_euler_step (include.f90:8-62)
-        STAGE(B, A, -1, dt, 1.0, kStageFirst | acc, dt);
-        SWH_HIP(launch_fv_finish(h->sc, kFinishEuler, h->stream));
+        STAGE(B, A, -1, dt, 1.0, first | acc | kStageFinish, dt);
         break;
     case SWH_METHOD_RK2:
         // one_rk2_step (include.f90:64-146): two Euler stages with the same step,
         // nesting fluxes tracked half a step each, then the average with backup_U
-        STAGE(B, A, -1, dt, 1.0, kStageFirst | acc, dt * 0.5);
+        STAGE(B, A, -1, dt, 1.0, first | acc, dt * 0.5);
         STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5);
         SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A],
-                                     (const T *)h->UH[A], (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->stream));
-        SWH_HIP(launch_fv_finish(h->sc, kFinishRk2, h->stream));
+                                     (const T *)h->UH[A], (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->sc,
+                                     h->stream));
         break;
     case SWH_METHOD_MIDPOINT:
         // one_midpoint_step (include.f90:257-336): half a step for the fluxes, then the
         // full step from backup_U with them
-        STAGE(B, A, -1, dt, 0.5, kStageFirst, 0.0);
-        STAGE(B, B, A, dt, 1.0, acc | kStageFromTime0, dt);
-        SWH_HIP(launch_fv_finish(h->sc, kFinishEuler, h->stream));
+        STAGE(B, A, -1, dt, 0.5, first, 0.0);
+        STAGE(B, B, A, dt, 1.0, acc | kStageFromTime0 | kStageFinish, dt);
         break;
     case SWH_METHOD_RK2N: {
         // one_rk2n_step (include.f90:149-254), n = kRk2nN
@@ -310,14 +308,13 @@ int fv_step(swh_domain *h, double dt)
         const double dt1 = dt > 0.0 ? dt / (kRk2nN - 1.0) : -1.0;
         const double nest1 = dt > 0.0 ? dt1 * (kRk2nN - 1.0) * n_inverse : 0.0;
         const double reduce = (1.0 * kRk2nN - 1.0) * n_inverse;
-        STAGE(B, A, -1, dt1, 1.0, kStageFirst | acc, nest1);
+        STAGE(B, A, -1, dt1, 1.0, first | acc, nest1);
         for (int j = 2; j <= kRk2nN - 1; j++) STAGE(B, B, -1, dt1, 1.0, acc, nest1);
         SWH_HIP(launch_fv_rk2n_combine<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (T *)h->S[A], (T *)h->UH[A],
-                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 0, h->stream));
+                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 0, h->sc, h->stream));
         STAGE(B, B, -1, dt1, reduce, acc | kStageScaleBfN, dt > 0.0 ? reduce * dt1 : 0.0);
         SWH_HIP(launch_fv_rk2n_combine<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (T *)h->S[A], (T *)h->UH[A],
-                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 1, h->stream));
-        SWH_HIP(launch_fv_finish(h->sc, kFinishRk2n, h->stream));
+                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 1, h->sc, h->stream));
         break;
     }
     default:

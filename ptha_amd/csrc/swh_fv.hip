@@ -271,45 +271,63 @@ int fv_flux_blocks(int nx, int ny) { return ((nx + kFTX - 1) / kFTX) * ((ny + kF
 
 // ---- pass 3: one block: boundary-flux sums (domain_mod.f90:1483-1499), step choice,
 // time and flux-integral bookkeeping (timestepping_alternatives_include.f90:31-50) ----
+// 1024 threads: a quarter per boundary line, each thread summing a fixed stride of it
+// (loads unrolled so they are in flight together), then a fixed-order tree in LDS:
+// deterministic, and a few memory latencies instead of one per 64 elements.
+constexpr int kScalarThreads = 1024;
+
+// the step's closing bookkeeping (defined with the kernels that run it)
+__device__ void fv_finish(DevScalars *sc, int mode);
+
 template <typename T>
-__global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny, int fpitch,
-                                                        int n_ext, double fixed_dt, double dt_mult, int mode,
-                                                        DevScalars *sc, const double *bmax, int nbmax)
+__global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny,
+                                                                   int fpitch, int n_ext, double fixed_dt,
+                                                                   double dt_mult, int mode, DevScalars *sc,
+                                                                   const double *bmax, int nbmax)
 {
-    __shared__ double part[4][64];
-    const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
+    constexpr int Q = kScalarThreads / 4;
+    __shared__ double part[4][Q];
+    __shared__ double pm[kScalarThreads];
+    const int g = threadIdx.x / Q, lane = threadIdx.x % Q;
     double acc = 0.0;
-    // plain per-lane partial sums combined in a fixed order (deterministic)
     if (g == 0 || g == 2) {
-        const int jr = (g == 0) ? ny - n_ext : n_ext;
-        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += 64) acc += (double)fNS[(long long)jr * fpitch + i];
+        const long long row = (long long)((g == 0) ? ny - n_ext : n_ext) * fpitch;
+#pragma unroll 8
+        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += Q) acc += (double)fNS[row + i];
     } else {
         const int ic = (g == 1) ? nx - n_ext : n_ext;
-        for (int jj = n_ext + lane; jj <= ny - n_ext - 1; jj += 64) acc += (double)fEW[(long long)jj * fpitch + ic];
+#pragma unroll 8
+        for (int jj = n_ext + lane; jj <= ny - n_ext - 1; jj += Q) acc += (double)fEW[(long long)jj * fpitch + ic];
     }
     part[g][lane] = acc;
     // CFL: maximum over the flux pass's blocks, from 1/maximum_timestep
     // (compute_fluxes_DE1_inner_include.f90:92-95)
-    __shared__ double pm[256];
     double m = 1.0 / kMaxTimestep;
-    for (int k = threadIdx.x; k < nbmax; k += blockDim.x) m = fmax_(m, bmax[k]);
+#pragma unroll 8
+    for (int k = threadIdx.x; k < nbmax; k += kScalarThreads) m = fmax_(m, bmax[k]);
     pm[threadIdx.x] = m;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    for (int w = Q / 2; w > 0; w >>= 1) {
+        if (lane < w) part[g][lane] = part[g][lane] + part[g][lane + w];
+        __syncthreads();
+    }
+    for (int w = kScalarThreads / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) pm[threadIdx.x] = fmax_(pm[threadIdx.x], pm[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         double st[4];
-        for (int s = 0; s < 4; s++) {
-            double t = 0.0;
-            for (int l = 0; l < 64; l++) t += part[s][l];
-            st[s] = t;
-        }
+        for (int q = 0; q < 4; q++) st[q] = part[q][0];
         sc->bf_store[0] = st[0];
         sc->bf_store[1] = st[1];
         sc->bf_store[2] = -st[2];
         sc->bf_store[3] = -st[3];
+        if (mode & kStageBegin) {
+            // evolve_one_step resets the step's flux integral (domain_mod.f90:1713-1715);
+            // the schemes keep the start time (timestepping_alternatives_include.f90:78, 167, 271)
+            sc->bf_evolve = 0.0;
+            sc->time0 = sc->time;
+        }
         const double max_dt = 1.0 / pm[0];
         sc->max_dt = max_dt;
         if (mode & kStageFirst) {
@@ -326,6 +344,7 @@ __global__ __launch_bounds__(256) void fv_scalar_kernel(const T *fNS, const T *f
                 sc->bf_evolve + ts * (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]);
         // update_U advances the time (update_U_DE1_inner_include.f90:98)
         sc->time = (mode & kStageFromTime0) ? sc->time0 + ts : sc->time + ts;
+        if (mode & kStageFinish) fv_finish(sc, kFinishEuler);
     }
 }
 
@@ -383,10 +402,11 @@ __global__ void fv_update_kernel(FVArgs<T> a)
 // ---- rk2 average (timestepping_alternatives_include.f90:123-134) ----
 template <typename T>
 __global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny,
-                                  int pitch)
+                                  int pitch, DevScalars *sc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
+    if (i == 0 && j == 0) fv_finish(sc, kFinishRk2);
     if (i >= nx || j >= ny) return;
     const long long o = (long long)j * pitch + i;
     S[o] = T(0.5) * (S[o] + S0[o]);
@@ -418,15 +438,7 @@ __global__ __launch_bounds__(256) void fv_stage_bc_kernel(T *S, T *UH, T *VH, in
     }
 }
 
-__global__ void fv_begin_kernel(DevScalars *sc)
-{
-    // evolve_one_step resets the per-step flux integral (domain_mod.f90:1713-1715);
-    // the schemes keep the start time (timestepping_alternatives_include.f90:78, 167, 271)
-    sc->bf_evolve = 0.0;
-    sc->time0 = sc->time;
-}
-
-__global__ void fv_finish_kernel(DevScalars *sc, int mode)
+__device__ void fv_finish(DevScalars *sc, int mode)
 {
     if (mode == kFinishRk2) {
         // timestepping_alternatives_include.f90:136-142
@@ -447,10 +459,11 @@ __global__ void fv_finish_kernel(DevScalars *sc, int mode)
 // ---- rk2n combination (timestepping_alternatives_include.f90:212-241) ----
 template <typename T>
 __global__ void fv_rk2n_combine_kernel(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch,
-                                       int part)
+                                       int part, DevScalars *sc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
+    if (part == 1 && i == 0 && j == 0) fv_finish(sc, kFinishRk2n);
     if (i >= nx || j >= ny) return;
     const long long o = (long long)j * pitch + i;
     if (part == 0) {
@@ -470,7 +483,7 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
 {
     dim3 block(256), grid((a.nx + 255) / 256, a.ny);
     hipLaunchKernelGGL(fv_flux_kernel<T>, dim3((a.nx + kFTX - 1) / kFTX, (a.ny + kFTY - 1) / kFTY), block, 0, s, a);
-    hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(256), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
+    hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(kScalarThreads), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
                        fixed_dt, dt_mult, mode, a.sc, (const double *)a.bmax, fv_flux_blocks(a.nx, a.ny));
     hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
     return hipGetLastError();
@@ -478,10 +491,10 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
 
 template <typename T>
 hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny, int pitch,
-                             hipStream_t s)
+                             DevScalars *sc, hipStream_t s)
 {
     dim3 block(256), grid((nx + 255) / 256, ny);
-    hipLaunchKernelGGL(fv_average_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch);
+    hipLaunchKernelGGL(fv_average_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, sc);
     return hipGetLastError();
 }
 
@@ -494,37 +507,26 @@ hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int
     return hipGetLastError();
 }
 
-hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s)
-{
-    hipLaunchKernelGGL(fv_begin_kernel, dim3(1), dim3(1), 0, s, sc);
-    return hipGetLastError();
-}
-
-hipError_t launch_fv_finish(DevScalars *sc, int mode, hipStream_t s)
-{
-    hipLaunchKernelGGL(fv_finish_kernel, dim3(1), dim3(1), 0, s, sc, mode);
-    return hipGetLastError();
-}
-
 template <typename T>
 hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
-                                  hipStream_t s)
+                                  DevScalars *sc, hipStream_t s)
 {
     dim3 block(256), grid((nx + 255) / 256, ny);
-    hipLaunchKernelGGL(fv_rk2n_combine_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, part);
+    hipLaunchKernelGGL(fv_rk2n_combine_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, part,
+                       sc);
     return hipGetLastError();
 }
 
 template hipError_t launch_fv_euler<double>(const FVArgs<double> &, double, double, int, hipStream_t);
 template hipError_t launch_fv_euler<float>(const FVArgs<float> &, double, double, int, hipStream_t);
 template hipError_t launch_fv_rk2n_combine<double>(double *, double *, double *, double *, double *, double *, int,
-                                                   int, int, int, hipStream_t);
+                                                   int, int, int, DevScalars *, hipStream_t);
 template hipError_t launch_fv_rk2n_combine<float>(float *, float *, float *, float *, float *, float *, int, int, int,
-                                                  int, hipStream_t);
+                                                  int, DevScalars *, hipStream_t);
 template hipError_t launch_fv_average<double>(double *, double *, double *, const double *, const double *,
-                                              const double *, int, int, int, hipStream_t);
+                                              const double *, int, int, int, DevScalars *, hipStream_t);
 template hipError_t launch_fv_average<float>(float *, float *, float *, const float *, const float *, const float *,
-                                             int, int, int, hipStream_t);
+                                             int, int, int, DevScalars *, hipStream_t);
 template hipError_t launch_fv_stage_bc<double>(double *, double *, double *, int, int, int, int, double, double,
                                                double, const DevScalars *, hipStream_t);
 template hipError_t launch_fv_stage_bc<float>(float *, float *, float *, int, int, int, int, double, double, double,

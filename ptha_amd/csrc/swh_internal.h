@@ -53,7 +53,13 @@ struct DevScalars {
 //   kStageAccBf      boundary_flux_evolve_integral += stage_dt * sum(boundary_flux_store)
 //   kStageFromTime0  time = time0 + stage_dt (midpoint's full step from the backup)
 //   kStageScaleBfN   first scale boundary_flux_evolve_integral by (n-1)/n (rk2n)
-enum : int { kStageFirst = 1, kStageAccBf = 2, kStageFromTime0 = 4, kStageScaleBfN = 8 };
+//   kStageBegin      the step's first stage: evolve_one_step's reset of the step's flux
+//                    integral and the start time (domain_mod.f90:1713-1715)
+//   kStageFinish     the step's last stage of euler/midpoint: the step's bookkeeping
+//                    (reported CFL step, evolve_step_dt, time integral) after it
+enum : int {
+    kStageFirst = 1, kStageAccBf = 2, kStageFromTime0 = 4, kStageScaleBfN = 8, kStageBegin = 16, kStageFinish = 32
+};
 // rk2n_n_value (domain_mod.f90:88-90)
 constexpr int kRk2nN = 5;
 // fv_finish_kernel modes
@@ -199,20 +205,20 @@ struct FVArgs {
 
 template <typename T>
 hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, int mode, hipStream_t s);
-// rk2n: part 0: B0 = (B0 - U) / n;  part 1: U = U + B0
+// rk2n: part 0: B0 = (B0 - U) / n;  part 1: U = U + B0 and the step's bookkeeping
+// (kFinishRk2n)
 template <typename T>
 hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
-                                  hipStream_t s);
+                                  DevScalars *sc, hipStream_t s);
+// rk2 average with backup_U; the step's bookkeeping (kFinishRk2) rides along
 template <typename T>
 hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny,
-                             int pitch, hipStream_t s);
+                             int pitch, DevScalars *sc, hipStream_t s);
 template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
                               double msl, const DevScalars *sc, hipStream_t s);
-hipError_t launch_fv_begin(DevScalars *sc, hipStream_t s);
 // number of blocks of the flux pass (size of FVArgs::bmax)
 int fv_flux_blocks(int nx, int ny);
-hipError_t launch_fv_finish(DevScalars *sc, int mode, hipStream_t s);
 
 
 // ---------------------------------------------------------------------------

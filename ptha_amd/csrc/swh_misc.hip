@@ -51,36 +51,48 @@ __device__ DD block_reduce_dd(DD v)
 }
 
 // ---- leapfrog boundary flux (domain_mod_linear_solver_include.f90:126-165) ----
-// One block of 256 threads: 4 groups of 64, group g sums side g (N, E, S, W) of the
-// pre-momentum UH/VH. Thread 0 then forms boundary_flux_evolve_integral and adds it
-// to boundary_flux_time_integral (domain_mod.f90:1803-1807).
+// One block of 1024 threads: 4 groups of 256, group g sums side g (N, E, S, W) of the
+// pre-momentum UH/VH, each thread a fixed stride of the line with its loads unrolled
+// (so a few memory latencies cover the line), then a fixed-order compensated tree per
+// group. Thread 0 forms boundary_flux_evolve_integral and adds it to
+// boundary_flux_time_integral (domain_mod.f90:1803-1807).
+constexpr int kBfThreads = 1024;
+
 template <typename T>
-__global__ __launch_bounds__(256) void bflux_leapfrog_kernel(const T *__restrict__ UH, const T *__restrict__ VH,
-                                                             int nx, int ny, int pitch, int n_ext,
-                                                             const double *__restrict__ dle,
-                                                             const double *__restrict__ dbe, double dt,
-                                                             double time_after, DevScalars *sc)
+__global__ __launch_bounds__(kBfThreads) void bflux_leapfrog_kernel(const T *__restrict__ UH,
+                                                                   const T *__restrict__ VH, int nx, int ny,
+                                                                   int pitch, int n_ext,
+                                                                   const double *__restrict__ dle,
+                                                                   const double *__restrict__ dbe, double dt,
+                                                                   double time_after, DevScalars *sc)
 {
-    __shared__ double part[4][64][2];
-    const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
+    constexpr int Q = kBfThreads / 4;
+    __shared__ double part_hi[4][Q], part_lo[4][Q];
+    const int g = threadIdx.x / Q, lane = threadIdx.x % Q;
     DD acc{0.0, 0.0};
     if (g == 0 || g == 2) {
-        const int j = (g == 0) ? ny - n_ext - 1 : n_ext - 1;
-        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += 64) acc = dd_add(acc, (double)VH[(long long)j * pitch + i]);
+        const long long row = (long long)((g == 0) ? ny - n_ext - 1 : n_ext - 1) * pitch;
+#pragma unroll 8
+        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += Q) acc = dd_add(acc, (double)VH[row + i]);
     } else {
         const int i = (g == 1) ? nx - n_ext - 1 : n_ext - 1;
-        for (int j = n_ext + lane; j <= ny - n_ext - 1; j += 64) acc = dd_add(acc, (double)UH[(long long)j * pitch + i]);
+#pragma unroll 8
+        for (int j = n_ext + lane; j <= ny - n_ext - 1; j += Q) acc = dd_add(acc, (double)UH[(long long)j * pitch + i]);
     }
-    part[g][lane][0] = acc.hi;
-    part[g][lane][1] = acc.lo;
+    part_hi[g][lane] = acc.hi;
+    part_lo[g][lane] = acc.lo;
     __syncthreads();
+    for (int w = Q / 2; w > 0; w >>= 1) {
+        if (lane < w) {
+            const DD c = dd_add(DD{part_hi[g][lane], part_lo[g][lane]}, DD{part_hi[g][lane + w], part_lo[g][lane + w]});
+            part_hi[g][lane] = c.hi;
+            part_lo[g][lane] = c.lo;
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         double store[4];
-        for (int s = 0; s < 4; s++) {
-            DD t{0.0, 0.0};
-            for (int l = 0; l < 64; l++) t = dd_add(t, DD{part[s][l][0], part[s][l][1]});
-            store[s] = t.hi + t.lo;
-        }
+        for (int q = 0; q < 4; q++) store[q] = part_hi[q][0] + part_lo[q][0];
         sc->bf_store[0] = store[0] * dbe[ny - n_ext];
         sc->bf_store[1] = store[1] * dle[nx - n_ext];
         sc->bf_store[2] = -store[2] * dbe[n_ext];
@@ -96,7 +108,7 @@ template <typename T>
 hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext, const double *dle,
                                  const double *dbe, double dt, double time_after, DevScalars *sc, hipStream_t s)
 {
-    hipLaunchKernelGGL(bflux_leapfrog_kernel<T>, dim3(1), dim3(256), 0, s, UH, VH, nx, ny, pitch, n_ext, dle, dbe, dt,
+    hipLaunchKernelGGL(bflux_leapfrog_kernel<T>, dim3(1), dim3(kBfThreads), 0, s, UH, VH, nx, ny, pitch, n_ext, dle, dbe, dt,
                        time_after, sc);
     return hipGetLastError();
 }

@@ -217,13 +217,15 @@ class Domain:
         return out
 
     # -- gauges (point_gauge_mod.f90) --
-    def setup_point_gauges(self, xy, max_samples=100000):
+    def setup_point_gauges(self, xy=None, max_samples=100000, ij=None):
         """Nearest-cell gauges: site index = 1 + min(floor((xy - ll)/dx), nx - 1)
-        (point_gauge_mod.f90:329-336)."""
-        xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2)
-        i = 1 + np.minimum(np.floor((xy[:, 0] - self.lower_left[0]) / self.dx[0]), self.nx - 1)
-        j = 1 + np.minimum(np.floor((xy[:, 1] - self.lower_left[1]) / self.dx[1]), self.ny - 1)
-        ij = np.ascontiguousarray(np.stack([i, j], axis=1).astype(np.int32))
+        (point_gauge_mod.f90:329-336), or the 1-based cells ij (n, 2) directly."""
+        if ij is None:
+            xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2)
+            i = 1 + np.minimum(np.floor((xy[:, 0] - self.lower_left[0]) / self.dx[0]), self.nx - 1)
+            j = 1 + np.minimum(np.floor((xy[:, 1] - self.lower_left[1]) / self.dx[1]), self.ny - 1)
+            ij = np.stack([i, j], axis=1)
+        ij = np.ascontiguousarray(np.asarray(ij).reshape(-1, 2).astype(np.int32))
         if (ij < 1).any() or (ij[:, 0] > self.nx).any() or (ij[:, 1] > self.ny).any():
             raise ValueError("gauge outside the domain")
         self.gauge_ij = ij

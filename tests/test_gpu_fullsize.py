@@ -1,0 +1,141 @@
+"""GPU: full-size configurations against FP64 runs of the compiled reference
+(fixtures written by tests/golden/make_fullsize_config{2,3}.py: gauge series, the
+max-stage grid and the final stage on every 10th cell, volumes).
+
+* rk2 wet/dry beach, 2000 x 1000, 3600 s (about 32 000 adaptive steps);
+* three-level nested multidomain, 6 h (about 4 600 global steps).
+
+These runs are long enough for round-off to grow (adaptive steps decided from the
+state, wet/dry switching at the shore), so the bars are stated per quantity: the
+FP64 path against the reference shows the round-off floor, the FP32 path (FP32
+state, Kahan-compensated stage for the leapfrog domains) the cost of the precision.
+Every figure is printed.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ptha_amd import configs as C
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _fixture(name):
+    p = os.path.join(GOLDEN, name)
+    if not os.path.exists(p):
+        pytest.skip(name + " not generated")
+    return dict(np.load(p))
+
+
+def _report(tag, d):
+    print(tag, json.dumps({k: (float(v) if isinstance(v, (np.floating, float)) else v) for k, v in d.items()}),
+          flush=True)
+
+
+# bars: (FP64, FP32) per quantity, in metres unless relative
+C2_BARS = {"gauge_stage": (1e-6, 1e-3), "max_stage": (1e-5, 1e-2), "volume_rel": (1e-12, 1e-6)}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_config2_fullsize(precision):
+    from ptha_amd.domain import Domain
+    z = _fixture("fullsize_config2.npz")
+    print("CASE config 2 full size", precision, flush=True)
+    cfg = C.beach_rk2()
+    d = Domain(cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], "rk2", precision=precision, max_U_variables=("max_stage",))
+    d.U = cfg["U"]
+    d.set_manning_squared(cfg["manning_squared"])
+    b = cfg["stage_bc"]
+    d.set_stage_boundary_sine(b["amp"], b["period"], sides=b["sides"])
+    every = int(z["every"])
+    nsamp = len(z["steps"])
+    d.setup_point_gauges(ij=z["gauge_ij"], max_samples=nsamp)
+    d.sample_gauges()
+    for _ in range(nsamp - 1):
+        d.evolve_steps(None, every)
+        d.sample_gauges()
+    times, vals = d.gauge_series()
+    sub = int(z["sub"])
+    U = d.U
+    ref_series = z["series"]
+    wet = (z["max_stage"] - z["elev"]) > 1e-2
+    res = dict(steps=int(z["nsteps"]), time=float(d.time), ref_time=float(z["final_time"]),
+               gauge_stage=np.abs(vals[:, 0] - ref_series[:, 0]).max(),
+               gauge_uh=np.abs(vals[:, 1] - ref_series[:, 1]).max(),
+               max_stage=np.abs(d.max_U("max_stage")[::sub, ::sub] - z["max_stage"])[wet].max(),
+               final_stage=np.abs(U[0][::sub, ::sub] - z["stage"])[wet].max(),
+               volume_rel=abs(d.volume_interior() - float(z["volume"])) / float(z["volume"]),
+               neg_depth=d.scalars()["negative_depth_fix_counter"], ref_neg_depth=float(z["negative_depth_fix_counter"]))
+    _report("config2 " + precision, res)
+    d.close()
+    k = 0 if precision == "fp64" else 1
+    for q, bars in C2_BARS.items():
+        assert res[q] <= bars[k], (q, res[q])
+
+
+C3_BARS = {"gauge_stage": (1e-6, 1e-3), "max_stage": (1e-5, 1e-2), "volume_rel": (1e-12, 1e-6)}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_config3_fullsize(precision):
+    from ptha_amd.domain import Domain
+    from ptha_amd.multidomain import MultiDomain
+    z = _fixture("fullsize_config3.npz")
+    print("CASE config 3 full size", precision, flush=True)
+    g = np.load(os.path.join(GOLDEN, "nested_c3_tables.npz"))
+    tables = json.loads(bytes(g["tables_json"]).decode())
+    cfg = C.nested_c3(scale=tables["scale"])
+    doms = []
+    for k, td in enumerate(tables["domains"]):
+        dd = td["dims"]
+        dom = Domain(dd["nx"], dd["ny"], td["lw"], td["ll"], timestepping_method=tables["methods"][k],
+                     precision=precision, spherical=True, max_U_variables=("max_stage",),
+                     exterior_cells_width=dd["exterior_cells_width"])
+        x, y = C.cell_centres(dd["nx"], dd["ny"], td["lw"], td["ll"])
+        dom.U = cfg["state"](x, y)
+        if tables["methods"][k] != "linear":
+            dom.set_manning_squared(np.full((dd["ny"], dd["nx"]), cfg["manning_n"] ** 2))
+        doms.append(dom)
+    prio = [(g["pdi_%d" % k].astype(np.int32), g["pnp_%d" % k].astype(np.int32)) for k in range(len(doms))]
+    md = MultiDomain(doms, g["all_dx"], g["all_methods"], tables["trf"], [td["send"] for td in tables["domains"]],
+                     [td["recv"] for td in tables["domains"]], prio)
+    md.make_initial_conditions_consistent()
+    v0 = md.volumes()
+    every, nglobal, dt = int(z["every"]), int(z["nglobal"]), float(z["dt"])
+    nsamp = len(z["series_0"])
+    for k, dom in enumerate(doms):
+        dom.setup_point_gauges(ij=z["gauge_ij_%d" % k], max_samples=nsamp)
+        dom.sample_gauges()
+    done = 0
+    while done < nglobal:
+        n = min(every, nglobal - done)
+        md.evolve_steps(dt, n)
+        done += n
+        if done % every == 0:
+            md.synchronize()
+            for dom in doms:
+                dom.sample_gauges()
+    md.synchronize()
+    v1 = md.volumes()
+    sub = int(z["sub"])
+    res = dict(global_steps=nglobal, volume_rel=float(abs(v1.sum() - z["vol1"].sum()) / z["vol1"].sum()),
+               mass_change_rel=float(abs(v1.sum() - v0.sum()) / v0.sum()))
+    gs, ms = 0.0, 0.0
+    for k, dom in enumerate(doms):
+        _, vals = dom.gauge_series()
+        e = np.abs(vals[:, 0] - z["series_%d" % k][:, 0]).max()
+        wet = (z["max_stage_%d" % k] - z["elev_%d" % k]) > 1e-2
+        m = np.abs(dom.max_U("max_stage")[::sub, ::sub] - z["max_stage_%d" % k])[wet].max()
+        res["gauge_stage_%d" % k], res["max_stage_%d" % k] = float(e), float(m)
+        gs, ms = max(gs, e), max(ms, m)
+    res["gauge_stage"], res["max_stage"] = gs, ms
+    _report("config3 " + precision, res)
+    md.close()
+    for dom in doms:
+        dom.close()
+    k = 0 if precision == "fp64" else 1
+    for q, bars in C3_BARS.items():
+        assert res[q] <= bars[k], (q, res[q])
