@@ -146,18 +146,93 @@ __device__ __forceinline__ EdgeRes edge_flux(double stage_neg, double stage_pos,
     return r;
 }
 
+// One edge of a cell that owns it (its south or west edge), from the clipped states
+// and limited gradients of the two cells: the three pre-multiplied fluxes, the
+// explicit-source contributions to the cell below/left (es_neg: explicit_source_VH_j_
+// minus_1 / explicit_source_im1_work) and to the owner (es_pos), and the CFL term
+// (include.f90:178-432 north-south, 454-718 east-west).
+struct EdgeOut {
+    double f0, f1, f2, es_neg, es_pos, mx;
+};
+
+template <typename T>
+__device__ __forceinline__ EdgeOut ns_edge(const FVArgs<T> &a, int j, const Cell<T> &Sc, const Cell<T> &C,
+                                           const Grad4 &gl, const Grad4 &gc, double e_s, double e_c)
+{
+    const double stage_neg = (double)Sc.s + 0.5 * gl.st;
+    const double stage_pos = (double)C.s - 0.5 * gc.st;
+    const double depth_neg = (double)Sc.d + 0.5 * gl.dp;
+    const double depth_pos = (double)C.d - 0.5 * gc.dp;
+    const double u_neg = (double)Sc.u + 0.5 * gl.u, u_pos = (double)C.u - 0.5 * gc.u;
+    const double v_neg = (double)Sc.v + 0.5 * gl.v, v_pos = (double)C.v - 0.5 * gc.v;
+    const EdgeRes r = edge_flux(stage_neg, stage_pos, depth_neg, depth_pos, u_neg, u_pos, v_neg, v_pos,
+                                (double)Sc.d, (double)C.d, e_s, e_c, a.dbe[j], true);
+    EdgeOut o;
+    o.f0 = r.f0; o.f1 = r.f1; o.f2 = r.f2;
+    o.es_neg = r.bsp_neg * a.dbe[j] - r.hghh;
+    o.es_pos = 0.0 - r.bsp_pos * a.dbe[j] + r.hghh;
+    o.mx = r.max_speed > 1.0e-12 ? r.max_speed * (1.0 / (a.dle[0] * a.half_cfl)) : 0.0;
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ EdgeOut ew_edge(const FVArgs<T> &a, int i, int j, const Cell<T> &Wc, const Cell<T> &C,
+                                           const Grad4 &el, const Grad4 &ec, double e_w, double e_c)
+{
+    const double st_neg = (double)Wc.s + 0.5 * el.st;
+    const double st_pos = (double)C.s - 0.5 * ec.st;
+    const double dp_neg = (double)Wc.d + 0.5 * el.dp;
+    const double dp_pos = (double)C.d - 0.5 * ec.dp;
+    const double uu_neg = (double)Wc.u + 0.5 * el.u, uu_pos = (double)C.u - 0.5 * ec.u;
+    const double vv_neg = (double)Wc.v + 0.5 * el.v, vv_pos = (double)C.v - 0.5 * ec.v;
+    const EdgeRes q = edge_flux(st_neg, st_pos, dp_neg, dp_pos, uu_neg, uu_pos, vv_neg, vv_pos, (double)Wc.d,
+                                (double)C.d, e_w, e_c, a.dle[i], false);
+    EdgeOut o;
+    o.f0 = q.f0; o.f1 = q.f1; o.f2 = q.f2;
+    o.es_neg = q.bsp_neg * a.dle[i] - q.hghh;
+    o.es_pos = 0.0 - q.bsp_pos * a.dle[i] + q.hghh;
+    o.mx = q.max_speed > 1.0e-12 ? q.max_speed * (1.0 / (a.dbe[j] * a.half_cfl)) : 0.0;
+    return o;
+}
+
+// The owner cell's spherical terms: the 1/2 g h^2 metric source, Coriolis and the
+// Williamson tan(lat)/R term (include.f90:674-710), added to its explicit sources.
+template <typename T>
+__device__ __forceinline__ void cell_sources(const FVArgs<T> &a, int j, const Cell<T> &C, double &esu, double &esv)
+{
+    if (!a.spherical) return;
+    const double half_gravity = 0.5 * kGravity;
+    const double dc = (double)C.d;
+    esv = esv + half_gravity * dc * dc * (a.dbe[j + 1] - a.dbe[j]);
+    const double uhc = (double)C.uh, vhc = (double)C.vh;
+    double cm;
+    if (a.coriolis) {
+        cm = a.area[j] * a.cor[j];
+        esu = esu + cm * vhc;
+        esv = esv - cm * uhc;
+    }
+    cm = a.area[j] * a.tanlat[j] * (double)C.u;
+    esu = esu + cm * vhc;
+    esv = esv - cm * uhc;
+}
+
 // ---- pass 2: fluxes + explicit sources + CFL ----
 // A block owns a 64 x 4 tile of cells (each cell owns its south and west edges). It
 // first evaluates the clipped state of the tile and a halo of two cells west/south
 // and one cell east/north into LDS (each cell once, instead of once per edge that
-// needs it), then each thread forms its cell's two edges from LDS.
+// needs it), then the limited gradients of every cell that an edge of the tile needs
+// (north-south ones for rows j0-1..j0+3, east-west ones for columns i0-1..i0+63: each
+// once, where neighbouring edges would otherwise both form it), then each thread
+// forms its cell's two edges from LDS.
 constexpr int kFTX = 64, kFTY = 4, kFTW = kFTX + 3, kFTH = kFTY + 3;
+constexpr int kGNS = (kFTY + 1) * kFTX;   // north-south gradients per tile
+constexpr int kGEW = kFTY * (kFTX + 1);   // east-west gradients per tile
 
 template <typename T>
 __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
 {
-    __shared__ T ls_s[kFTH][kFTW], ls_d[kFTH][kFTW], ls_u[kFTH][kFTW], ls_v[kFTH][kFTW];
-    __shared__ T ls_uh[kFTH][kFTW], ls_vh[kFTH][kFTW], ls_e[kFTH][kFTW];
+    __shared__ T ls_s[kFTH][kFTW], ls_d[kFTH][kFTW], ls_u[kFTH][kFTW], ls_v[kFTH][kFTW], ls_e[kFTH][kFTW];
+    __shared__ Grad4 g_ns[kFTY + 1][kFTX], g_ew[kFTY][kFTX + 1];
     const int i0 = blockIdx.x * kFTX, j0 = blockIdx.y * kFTY;
     const int gx0 = i0 - 2, gy0 = j0 - 2;
     for (int t = threadIdx.x; t < kFTW * kFTH; t += blockDim.x) {
@@ -166,92 +241,66 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
         const long long go = (long long)gj * a.pitch + gi;
         const Cell<T> c = cell_at(a, go);
         ls_s[ly][lx] = c.s; ls_d[ly][lx] = c.d; ls_u[ly][lx] = c.u; ls_v[ly][lx] = c.v;
-        ls_uh[ly][lx] = c.uh; ls_vh[ly][lx] = c.vh; ls_e[ly][lx] = a.E[go];
+        ls_e[ly][lx] = a.E[go];
     }
     __syncthreads();
-    const int i = i0 + (int)(threadIdx.x % kFTX);
-    const int j = j0 + (int)(threadIdx.x / kFTX);
-    // cell (i + di, j + dj) from the tile
-    auto L = [&](int di, int dj) {
-        const int lx = i + di - gx0, ly = j + dj - gy0;
+    // cell (ii, jj) of the grid from the tile
+    auto G = [&](int ii, int jj) {
+        const int lx = ii - gx0, ly = jj - gy0;
         Cell<T> c;
         c.s = ls_s[ly][lx]; c.d = ls_d[ly][lx]; c.u = ls_u[ly][lx]; c.v = ls_v[ly][lx];
-        c.uh = ls_uh[ly][lx]; c.vh = ls_vh[ly][lx]; c.fixed = false;
+        c.uh = T(0); c.vh = T(0); c.fixed = false;
         return c;
     };
+    const Grad4 zero{0.0, 0.0, 0.0, 0.0};
+    // gradients: zero on the border rows (north-south) and columns (east-west)
+    // (include.f90:735-862)
+    for (int t = threadIdx.x; t < kGNS + kGEW; t += blockDim.x) {
+        if (t < kGNS) {
+            const int r = t / kFTX, c = t % kFTX;
+            const int ii = i0 + c, jj = j0 - 1 + r;
+            Grad4 g = zero;
+            if (ii < a.nx && jj > 0 && jj < a.ny - 1) g = grad3(G(ii, jj - 1), G(ii, jj), G(ii, jj + 1), a.theta);
+            g_ns[r][c] = g;
+        } else {
+            const int q = t - kGNS, r = q / (kFTX + 1), c = q % (kFTX + 1);
+            const int ii = i0 - 1 + c, jj = j0 + r;
+            Grad4 g = zero;
+            if (jj < a.ny && ii > 0 && ii < a.nx - 1) g = grad3(G(ii - 1, jj), G(ii, jj), G(ii + 1, jj), a.theta);
+            g_ew[r][c] = g;
+        }
+    }
+    __syncthreads();
+    const int ci = (int)(threadIdx.x % kFTX), rj = (int)(threadIdx.x / kFTX);
+    const int i = i0 + ci;
+    const int j = j0 + rj;
+    auto L = [&](int di, int dj) { return G(i + di, j + dj); };
     auto LE = [&](int di, int dj) { return ls_e[j + dj - gy0][i + di - gx0]; };
     double mx = 0.0;
     if (i >= 1 && i < a.nx && j >= 1 && j < a.ny) {
-        const int nx = a.nx, ny = a.ny;
-        const long long P = a.pitch;
-        const long long o = (long long)j * P + i;
-        const long long FP = (long long)a.fpitch * (ny + 1);
+        const long long o = (long long)j * a.pitch + i;
+        const long long FP = (long long)a.fpitch * (a.ny + 1), EPn = (long long)a.fpitch * a.ny;
         const long long fo = (long long)j * a.fpitch + i;
-        const double half_gravity = 0.5 * kGravity;
-        const double EPS = 1.0e-12;
-
-        // the cell, its south and west neighbours, and (where gradients need them) the
-        // next cells along each axis
-        const Cell<T> C = L(0, 0), Sc = L(0, -1), Wc = L(-1, 0);
-        const Grad4 zero{0.0, 0.0, 0.0, 0.0};
-
-        // NS edge between (i, j-1) and (i, j); gradients are zero on rows 0 and ny-1
-        {
-            const Grad4 gl = ((j - 1) == 0 || (j - 1) == ny - 1) ? zero : grad3(L(0, -2), Sc, C, a.theta);
-            const Grad4 gc = (j == 0 || j == ny - 1) ? zero : grad3(Sc, C, L(0, 1), a.theta);
-            const double stage_neg = (double)Sc.s + 0.5 * gl.st;
-            const double stage_pos = (double)C.s - 0.5 * gc.st;
-            const double depth_neg = (double)Sc.d + 0.5 * gl.dp;
-            const double depth_pos = (double)C.d - 0.5 * gc.dp;
-            const double u_neg = (double)Sc.u + 0.5 * gl.u, u_pos = (double)C.u - 0.5 * gc.u;
-            const double v_neg = (double)Sc.v + 0.5 * gl.v, v_pos = (double)C.v - 0.5 * gc.v;
-            const EdgeRes r = edge_flux(stage_neg, stage_pos, depth_neg, depth_pos, u_neg, u_pos, v_neg, v_pos,
-                                        (double)Sc.d, (double)C.d, (double)LE(0, -1), (double)LE(0, 0),
-                                        a.dbe[j], true);
-            a.fNS[fo] = (T)r.f0;
-            a.fNS[FP + fo] = (T)r.f1;
-            a.fNS[2 * FP + fo] = (T)r.f2;
-            a.esJ[fo] = r.bsp_neg * a.dbe[j] - r.hghh;
-            double esv = 0.0 - r.bsp_pos * a.dbe[j] + r.hghh;
-            if (r.max_speed > EPS) mx = fmax_(mx, r.max_speed * (1.0 / (a.dle[0] * a.half_cfl)));
-
-            // EW edge between (i-1, j) and (i, j); gradients are zero on columns 0 and nx-1
-            const Grad4 el = ((i - 1) == 0 || (i - 1) == nx - 1) ? zero : grad3(L(-2, 0), Wc, C, a.theta);
-            const Grad4 ec = (i == 0 || i == nx - 1) ? zero : grad3(Wc, C, L(1, 0), a.theta);
-            const double st_neg = (double)Wc.s + 0.5 * el.st;
-            const double st_pos = (double)C.s - 0.5 * ec.st;
-            const double dp_neg = (double)Wc.d + 0.5 * el.dp;
-            const double dp_pos = (double)C.d - 0.5 * ec.dp;
-            const double uu_neg = (double)Wc.u + 0.5 * el.u, uu_pos = (double)C.u - 0.5 * ec.u;
-            const double vv_neg = (double)Wc.v + 0.5 * el.v, vv_pos = (double)C.v - 0.5 * ec.v;
-            const EdgeRes q = edge_flux(st_neg, st_pos, dp_neg, dp_pos, uu_neg, uu_pos, vv_neg, vv_pos,
-                                        (double)Wc.d, (double)C.d, (double)LE(-1, 0), (double)LE(0, 0),
-                                        a.dle[i], false);
-            const long long EPn = (long long)a.fpitch * ny;
-            a.fEW[fo] = (T)q.f0;
-            a.fEW[EPn + fo] = (T)q.f1;
-            a.fEW[2 * EPn + fo] = (T)q.f2;
-            a.esE[fo] = q.bsp_neg * a.dle[i] - q.hghh;
-            double esu = 0.0 - q.bsp_pos * a.dle[i] + q.hghh;
-            if (a.spherical) {
-                // include.f90:674-710
-                const double dc = (double)C.d;
-                esv = esv + half_gravity * dc * dc * (a.dbe[j + 1] - a.dbe[j]);
-                const double uhc = (double)C.uh, vhc = (double)C.vh;
-                double cm;
-                if (a.coriolis) {
-                    cm = a.area[j] * a.cor[j];
-                    esu = esu + cm * vhc;
-                    esv = esv - cm * uhc;
-                }
-                cm = a.area[j] * a.tanlat[j] * (double)C.u;
-                esu = esu + cm * vhc;
-                esv = esv - cm * uhc;
-            }
-            a.esU[o] = esu;
-            a.esV[o] = esv;
-            if (q.max_speed > EPS) mx = fmax_(mx, q.max_speed * (1.0 / (a.dbe[j] * a.half_cfl)));
-        }
+        // the cell (with its momenta, for the spherical terms) and its south and west
+        // neighbours; gradients are zero on the border rows / columns
+        const Cell<T> C = cell_at(a, o);
+        const EdgeOut r = ns_edge(a, j, L(0, -1), C, g_ns[rj][ci], g_ns[rj + 1][ci], (double)LE(0, -1),
+                                  (double)LE(0, 0));
+        a.fNS[fo] = (T)r.f0;
+        a.fNS[FP + fo] = (T)r.f1;
+        a.fNS[2 * FP + fo] = (T)r.f2;
+        a.esJ[fo] = r.es_neg;
+        const EdgeOut q = ew_edge(a, i, j, L(-1, 0), C, g_ew[rj][ci], g_ew[rj][ci + 1], (double)LE(-1, 0),
+                                  (double)LE(0, 0));
+        a.fEW[fo] = (T)q.f0;
+        a.fEW[EPn + fo] = (T)q.f1;
+        a.fEW[2 * EPn + fo] = (T)q.f2;
+        a.esE[fo] = q.es_neg;
+        double esu = q.es_pos, esv = r.es_pos;
+        cell_sources(a, j, C, esu, esv);
+        a.esU[o] = esu;
+        a.esV[o] = esv;
+        mx = fmax_(r.mx, q.mx);
     }
     // block maximum (max is exact, so the order does not matter); the scalar pass
     // takes the maximum over blocks, instead of every wavefront contending for one
