@@ -10,8 +10,10 @@ import pytest
 import pyoracle
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-# single-grid run fixtures written by make_golden.py (nested_*: setup tables, tested elsewhere)
-NAMES = sorted(f[:-4] for f in os.listdir(HERE) if f.endswith(".npz") and not f.startswith("nested_"))
+# single-grid run fixtures written by make_golden.py (nested_*: setup tables, fullsize_*:
+# full-size runs; both tested elsewhere)
+NAMES = sorted(f[:-4] for f in os.listdir(HERE)
+               if f.endswith(".npz") and not f.startswith(("nested_", "fullsize_")))
 
 
 def load(name):
