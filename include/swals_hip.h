@@ -212,7 +212,9 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active);
  * 2 doubles per lane), 2 = keep the next row's loads in flight, 4 = non-temporal
  * state traffic, 8 = (batches) consecutive workgroups walk the sources of one tile,
  * 16 = (batches) two fused steps per launch wherever no gauge sample falls between
- * them (friction modes without per-step friction work only).
+ * them (friction modes without per-step friction work only), 32 = (single grids)
+ * one step per launch in swh_evolve_steps (by default pairs of steps run as one
+ * two-step launch where no stage boundary, nesting or max_U update falls between).
  * Results do not depend on the launch shape. */
 int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t variant);
 /* Launch-shape autotuner (no counterpart in the reference): times every shape

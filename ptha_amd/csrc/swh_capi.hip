@@ -219,6 +219,98 @@ int leapfrog_step(swh_domain *h, double dt)
     return SWH_OK;
 }
 
+// Two leapfrog steps in one launch (swh_leapfrog2.hip) for a single grid: possible
+// where no stage boundary, nesting strips or per-step friction work are involved and
+// max_U is not due after the first of the two steps. The boundary fluxes of the second
+// step need UH/VH after the first on the four boundary-flux lines: a small kernel
+// forms them there (same per-cell operations) into the output planes, the flux pass
+// reads them, and the two-step launch then overwrites the planes.
+bool two_step_ok(const swh_domain *h)
+{
+    if (!h->two_step || !h->leapfrog() || h->bc_mask || h->n_nest_strips) return false;
+    if (h->method == SWH_METHOD_LLPNF && !h->truely_linear) return false;
+    if (h->maxq_mask && (h->nsteps % h->maxq_freq) == 0) return false;
+    if ((long long)h->pitch * h->ny >= (1LL << 31)) return false;
+    return true;
+}
+
+int bfl_cells(swh_domain *h)
+{
+    if (h->d_bfl_idx) return SWH_OK;
+    // the rows and columns bflux_leapfrog_kernel reads (include.f90:126-165)
+    std::vector<long long> idx;
+    const int ne = h->n_ext;
+    for (int i = ne; i <= h->nx - ne - 1; i++) {
+        idx.push_back((long long)(h->ny - ne - 1) * h->pitch + i);
+        idx.push_back((long long)(ne - 1) * h->pitch + i);
+    }
+    for (int j = ne; j <= h->ny - ne - 1; j++) {
+        idx.push_back((long long)j * h->pitch + h->nx - ne - 1);
+        idx.push_back((long long)j * h->pitch + ne - 1);
+    }
+    int rc = dmalloc((void **)&h->d_bfl_idx, sizeof(long long) * idx.size());
+    if (rc) return rc;
+    SWH_HIP(hipMemcpy(h->d_bfl_idx, idx.data(), sizeof(long long) * idx.size(), hipMemcpyHostToDevice));
+    h->n_bfl = (int)idx.size();
+    return SWH_OK;
+}
+
+template <typename T, bool KAHAN>
+int leapfrog_step2(swh_domain *h, double dt)
+{
+    if (h->rc_dt != dt) {
+        int rc = build_rc<T>(h, dt);
+        if (rc) return rc;
+    }
+    int rc = bfl_cells(h);
+    if (rc) return rc;
+    int fric_mode = 0;
+    if (h->method == SWH_METHOD_LLPNF) fric_mode = 1;
+    if (fric_mode == 1 && !h->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload U first");
+    const int c = h->cur, o = 1 - c;
+    const double t_half1 = h->time + dt * 0.5, t_after1 = t_half1 + 0.5 * dt;
+    const double t_half2 = t_after1 + dt * 0.5, t_after2 = t_half2 + 0.5 * dt;
+    LFArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
+    a.S = (const T *)h->S[c]; a.C = (const T *)h->C[c]; a.UH = (const T *)h->UH[c]; a.VH = (const T *)h->VH[c];
+    a.E = (const T *)h->E;
+    a.So = (T *)h->S[o]; a.Co = (T *)h->C[o]; a.UHo = (T *)h->UH[o]; a.VHo = (T *)h->VH[o];
+    a.rc = rc_view<T>(h);
+    a.fwU = (const T *)h->fwU; a.fwV = (const T *)h->fwV; a.nsq = (const T *)h->nsq;
+    a.msl = (T)h->msl; a.dt = (T)dt; a.dt_lfc = (T)(dt * h->lfc);
+    a.lin_fric_mult = (T)(1.0 / (1.0 + dt * h->lfc));
+    a.fric_power = (T)((h->friction_type == 1) ? -2.0 : -7.0 / 3.0);
+    a.bc_mask = 0;
+    a.bc_stage = T(0);
+    a.rows_per_block = h->rows2;
+    a.src_fast = 0;
+    a.maxq_mask = 0;
+    a.mx = (T *)h->mx;
+    a.t_after = (T)t_after2;
+    a.arrival_stage = (T)(h->msl + h->arrival_thresh);
+    const int lin = h->lfc != 0.0;
+    // first step's boundary fluxes from the current UH/VH, the second's from UH/VH one
+    // step ahead on the lines (in the output planes, overwritten by the launch after)
+    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
+                                     h->d_dle, h->d_dbe, dt, t_after1, h->sc, h->stream));
+    SWH_HIP(launch_line_step1<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, lin, h->n_bfl, h->d_bfl_idx,
+                                        h->stream));
+    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[o], (const T *)h->VH[o], h->nx, h->ny, h->pitch, h->n_ext,
+                                     h->d_dle, h->d_dbe, dt, t_after2, h->sc, h->stream));
+    SWH_HIP(launch_leapfrog2<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, lin, 1, 256, h->stream));
+    h->cur = o;
+    h->time = t_after2;
+    h->dt_last_update = dt;
+    h->evolve_step_dt = t_after2 - t_after1;
+    h->nsteps += 2;
+    // max_U due after the second step (not after the first: two_step_ok)
+    if (h->maxq_mask && ((h->nsteps - 1) % h->maxq_freq) == 0) {
+        rc = maxq_now<T>(h, nullptr, h->time);
+        if (rc) return rc;
+    }
+    return SWH_OK;
+}
+
 template <typename T>
 FVArgs<T> fv_args(swh_domain *h, int dst, int src)
 {
@@ -432,6 +524,9 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
         const long long waves_x = (lanes + 63) / 64;
         while (h->rows_per_block > 4 && waves_x * ((d->ny + h->rows_per_block - 1) / h->rows_per_block) < 4096)
             h->rows_per_block /= 2;
+        // the two-step kernel: a wavefront owns 61 columns of a strip (3 more rows each)
+        const long long waves2_x = (d->nx + 60) / 61;
+        while (h->rows2 > 8 && waves2_x * ((d->ny + h->rows2 - 1) / h->rows2) < 2048) h->rows2 /= 2;
     }
     const int nx = h->nx, ny = h->ny;
     h->dbe.assign(d->distance_bottom_edge, d->distance_bottom_edge + ny + 1);
@@ -522,7 +617,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
-                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax};
+                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx};
     for (void *p : ptrs) dfree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -608,8 +703,16 @@ int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps)
 {
     if (!h) return fail(SWH_EINVAL, "null argument");
     if (nsteps < 0) return fail(SWH_EINVAL, "nsteps must be >= 0");
-    for (int k = 0; k < nsteps; k++) {
-        int rc = evolve(h, timestep);
+    for (int k = 0; k < nsteps;) {
+        int rc;
+        if (k + 1 < nsteps && timestep > 0.0 && two_step_ok(h)) {
+            rc = h->prec == SWH_PREC_FP64 ? leapfrog_step2<double, false>(h, timestep)
+                                          : leapfrog_step2<float, true>(h, timestep);
+            k += 2;
+        } else {
+            rc = evolve(h, timestep);
+            k += 1;
+        }
         if (rc) return rc;
     }
     return SWH_OK;
@@ -1355,8 +1458,8 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active)
 
 static int check_launch(int32_t block_x, int32_t rows_per_block, int32_t variant)
 {
-    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 31)
-        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..31");
+    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 63)
+        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..63");
     return SWH_OK;
 }
 
@@ -1409,6 +1512,7 @@ int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block
     int rc = check_launch(block_x, rows_per_block, variant);
     if (rc) return rc;
     h->block_x = block_x; h->rows_per_block = rows_per_block; h->variant = variant & 7;
+    h->two_step = (variant & 32) ? 0 : 1;
     return SWH_OK;
 }
 

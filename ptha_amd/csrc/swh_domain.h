@@ -53,6 +53,11 @@ struct swh_domain {
     double *d_nest_int = nullptr;
     // launch shape of the step kernel
     int block_x = 256, rows_per_block = 32, variant = 2;
+    // two leapfrog steps per launch in swh_evolve_steps (where nothing is needed between
+    // them): rows per strip of that kernel, and the cells of the boundary-flux lines
+    int two_step = 1, rows2 = 32;
+    long long *d_bfl_idx = nullptr;
+    int n_bfl = 0;
 
     bool leapfrog() const { return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF; }
     size_t plane_elems() const { return (size_t)pitch * ny; }

@@ -111,6 +111,12 @@ template <typename T, bool KAHAN>
 hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                             int linear_friction, int nsrc, int block_x, hipStream_t s);
 
+// UH/VH one step ahead at cells idx of a single grid (src_stride 0), written into
+// a.UHo / a.VHo there; friction modes 0 and 1, no boundary stage.
+template <typename T, bool KAHAN>
+hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                             int linear_friction, int n, const long long *idx, hipStream_t s);
+
 // Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
 hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
 

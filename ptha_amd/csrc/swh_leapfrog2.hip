@@ -24,6 +24,8 @@ namespace {
 constexpr int kOwn = 61;   // columns a wavefront writes (lanes 1..61)
 
 // Lane shifts across the wavefront (the lane at the open end keeps its own value).
+// Call them on every lane, outside lane-dependent conditions: a lane that does not
+// execute the shuffle leaves the lane reading from it with an undefined value.
 template <typename T>
 __device__ __forceinline__ T shl1(T v) { return __shfl_down(v, 1); }   // value of lane + 1
 template <typename T>
@@ -191,7 +193,10 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
         uh0_p = UH[o];
         e_p = a.E[o];
         vh0_p_r = shl1(vh0_p);
-        uh0_p_l = west ? uh0_p : shr1(uh0_p);
+        // every lane takes part in a shuffle (a lane that skips it leaves its neighbour
+        // reading an inactive lane), then the western edge keeps its own value
+        const T uh0_p_sh = shr1(uh0_p);
+        uh0_p_l = west ? uh0_p : uh0_p_sh;
         e_p_r = shl1(e_p);
     }
 
@@ -228,7 +233,8 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
         const T c1_r = KAHAN ? shl1(c1) : T(0);
         const T uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q.valid, q.cu, q.fu, dle_ip1, fw_u1, uh0, s1,
                                                             c1, s1_r, c1_r, e, e_r, vhb, vhb_r, vh0, vh0_r);
-        const T uh1_l = west ? uh1 : shr1(uh1);
+        const T uh1_sh = shr1(uh1);
+        const T uh1_l = west ? uh1 : uh1_sh;
         // step 1: VH at row m-1
         const T vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q_p.valid, q_p.cv, q_p.fv, q_p.dbe_hi, fw_v1,
                                                             vh0_p, s1_p, c1_p, s1, c1, e_p, e, uh0_p, uh0_p_l, uh0,
@@ -268,24 +274,15 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
     }
 }
 
-// Gauge values one step ahead: [stage, UH, VH] after one step at each gauge cell,
-// computed from the current state with the same per-cell operations as the step
-// kernels (so a two-step launch can pass over a sampling step). One lane per
-// (gauge, source); values[((slot * gmax + sample) * 3 + var) * n + g].
+// One step ahead at one cell: [stage (S, C), UH, VH] after one step at (i, j),
+// computed from the state (S, C, UH, VH) with the same per-cell operations as the step
+// kernels.
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-__global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, const long long *__restrict__ idx,
-                                                         double *values, int gmax, int sample)
+__device__ __forceinline__ void step1_cell(const LFArgs<T> &a, const T *__restrict__ S, const T *__restrict__ C,
+                                           const T *__restrict__ UH, const T *__restrict__ VH, int i, int j, T &s_c,
+                                           T &c_c, T &uh1, T &vh1)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int slot = blockIdx.y + a.src_base;
-    const long long so = (long long)slot * a.src_stride;
-    const T *__restrict__ S = a.S + so;
-    const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
-    const T *__restrict__ UH = a.UH + so;
-    const T *__restrict__ VH = a.VH + so;
     const T *__restrict__ E = a.E;
-    const int i = (int)(idx[g] % a.pitch), j = (int)(idx[g] / a.pitch);
     const int ip = min(i + 1, a.nx - 1), im = max(i - 1, 0), jp = min(j + 1, a.ny - 1), jm = max(j - 1, 0);
     auto at = [&](int ii, int jj) { return (long long)jj * a.pitch + ii; };
     auto cval = [&](long long o) { return KAHAN ? C[o] : T(0); };
@@ -296,7 +293,7 @@ __global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, con
     const T dle_i = a.rc.dle[i], dle_ip = a.rc.dle[ip], dle_ip1 = a.rc.dle[i + 1];
     const bool ci_int = i > 0 && i < a.nx - 1, cip_int = ip > 0 && ip < a.nx - 1;
     // new stage at (i, j), (i+1, j), (i, j+1); UH(i-1) clamps to UH(0) at the west edge
-    T s_c, c_c, s_e, c_e, s_n, c_n;
+    T s_e, c_e, s_n, c_n;
     {
         const long long o = at(i, j);
         stage_cell<T, KAHAN>(ci_int && q.interior, q.sa, dle_i, q.dbe_lo, q.dbe_hi, S[o], cval(o), UH[o],
@@ -314,17 +311,84 @@ __global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, con
     }
     const bool mom = i < a.nx - 1 && q.valid;
     const T fwu = FRIC ? a.fwU[at(i, j)] : T(0), fwv = FRIC ? a.fwV[at(i, j)] : T(0);
-    const T uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, mom, q.cu, q.fu, dle_ip1, fwu, UH[at(i, j)], s_c, c_c, s_e,
-                                                        c_e, E[at(i, j)], E[at(ip, j)], vhb_i, vhb_ip, VH[at(i, j)],
-                                                        VH[at(ip, j)]);
-    const T vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, mom, q.cv, q.fv, q.dbe_hi, fwv, VH[at(i, j)], s_c, c_c, s_n,
-                                                        c_n, E[at(i, j)], E[at(i, jp)], UH[at(i, j)], UH[at(im, j)],
-                                                        UH[at(i, jp)], UH[at(im, jp)]);
+    uh1 = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, mom, q.cu, q.fu, dle_ip1, fwu, UH[at(i, j)], s_c, c_c, s_e, c_e,
+                                               E[at(i, j)], E[at(ip, j)], vhb_i, vhb_ip, VH[at(i, j)],
+                                               VH[at(ip, j)]);
+    vh1 = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, mom, q.cv, q.fv, q.dbe_hi, fwv, VH[at(i, j)], s_c, c_c, s_n, c_n,
+                                               E[at(i, j)], E[at(i, jp)], UH[at(i, j)], UH[at(im, j)],
+                                               UH[at(i, jp)], UH[at(im, jp)]);
+}
+
+// Gauge values one step ahead (so a two-step launch can pass over a sampling step).
+// One lane per (gauge, source); values[((slot * gmax + sample) * 3 + var) * n + g].
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, const long long *__restrict__ idx,
+                                                         double *values, int gmax, int sample)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int slot = blockIdx.y + a.src_base;
+    const long long so = (long long)slot * a.src_stride;
+    const int i = (int)(idx[g] % a.pitch), j = (int)(idx[g] / a.pitch);
+    T s_c, c_c, uh1, vh1;
+    step1_cell<T, KAHAN, COR, FRIC, TL, LINF>(a, a.S + so, KAHAN ? a.C + so : nullptr, a.UH + so, a.VH + so, i, j,
+                                              s_c, c_c, uh1, vh1);
     double *v = values + ((long long)slot * gmax + sample) * 3 * n;
     v[g] = KAHAN ? (double)s_c - (double)c_c : (double)s_c;
     v[n + g] = (double)uh1;
     v[2 * n + g] = (double)vh1;
 }
+
+// UH and VH one step ahead at the given cells of a single grid, written into the
+// output planes there (the next step's boundary-flux lines, read before a two-step
+// launch overwrites the planes).
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__global__ __launch_bounds__(64) void line_step1_kernel(LFArgs<T> a, int n, const long long *__restrict__ idx)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long long o = idx[g];
+    const int i = (int)(o % a.pitch), j = (int)(o / a.pitch);
+    T s_c, c_c, uh1, vh1;
+    step1_cell<T, KAHAN, COR, FRIC, TL, LINF>(a, a.S, a.C, a.UH, a.VH, i, j, s_c, c_c, uh1, vh1);
+    a.UHo[o] = uh1;
+    a.VHo[o] = vh1;
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+static hipError_t launch_l1(const LFArgs<T> &a, int n, const long long *idx, hipStream_t s)
+{
+    hipLaunchKernelGGL((line_step1_kernel<T, KAHAN, COR, FRIC, TL, LINF>), dim3((n + 63) / 64), dim3(64), 0, s, a, n,
+                       idx);
+    return hipGetLastError();
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC>
+static hipError_t l1_tl(const LFArgs<T> &a, int tl, int linf, int n, const long long *idx, hipStream_t s)
+{
+    if (tl) return linf ? launch_l1<T, KAHAN, COR, FRIC, true, true>(a, n, idx, s)
+                        : launch_l1<T, KAHAN, COR, FRIC, true, false>(a, n, idx, s);
+    return linf ? launch_l1<T, KAHAN, COR, FRIC, false, true>(a, n, idx, s)
+                : launch_l1<T, KAHAN, COR, FRIC, false, false>(a, n, idx, s);
+}
+
+template <typename T, bool KAHAN>
+hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                             int linear_friction, int n, const long long *idx, hipStream_t s)
+{
+    if (friction_mode > 1 || a.bc_mask || n <= 0 || a.src_stride != 0) return hipErrorInvalidValue;
+    const int linf = (friction_mode == 0) && linear_friction;
+    if (coriolis)
+        return friction_mode ? l1_tl<T, KAHAN, true, 1>(a, truely_linear, linf, n, idx, s)
+                             : l1_tl<T, KAHAN, true, 0>(a, truely_linear, linf, n, idx, s);
+    return friction_mode ? l1_tl<T, KAHAN, false, 1>(a, truely_linear, linf, n, idx, s)
+                         : l1_tl<T, KAHAN, false, 0>(a, truely_linear, linf, n, idx, s);
+}
+
+template hipError_t launch_line_step1<double, false>(const LFArgs<double> &, int, int, int, int, int,
+                                                     const long long *, hipStream_t);
+template hipError_t launch_line_step1<float, true>(const LFArgs<float> &, int, int, int, int, int, const long long *,
+                                                   hipStream_t);
 
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 static hipError_t launch_g1(const LFArgs<T> &a, int nsrc, int n, const long long *idx, double *values, int gmax,

@@ -179,7 +179,8 @@ class Domain:
 
     def set_launch(self, block_x=256, rows_per_block=32, variant=2):
         """Launch shape of the leapfrog step kernel (results do not depend on it);
-        variant bits: 1 vector lanes, 2 row prefetch, 4 non-temporal state traffic."""
+        variant bits: 1 vector lanes, 2 row prefetch, 4 non-temporal state traffic,
+        32 one step per launch in evolve_steps (no two-step launches)."""
         B.check(self._lib.swh_domain_set_launch(self._h, int(block_x), int(rows_per_block), int(variant)),
                 "swh_domain_set_launch")
 

@@ -259,6 +259,53 @@ def test_rk2_lake_at_rest():
     g.close()
 
 
+@pytest.mark.parametrize("case", ["linear_cart", "linear_cart_lfc", "linear_sph", "llpnf_sph", "random_sph"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_single_grid_two_step_bitwise(oracle_built, case, precision):
+    """swh_evolve_steps runs pairs of leapfrog steps as one two-step launch (the
+    second step's boundary fluxes from UH/VH formed one step ahead on the flux lines):
+    states, every scalar and the max_U grids (updated every third step, so after some
+    pairs) equal the one-step-per-launch run bit for bit; an odd step count ends with a
+    single step. FP64 is also bit-identical to the oracle. 'random_sph' starts from a
+    random state on every cell, border and wall cells included (a regression case: the
+    two-step kernel once let the western edge lane skip a lane shuffle, so the column
+    next to it read an undefined neighbour value whenever UH on column 0 was not zero)."""
+    _print_case("two-step %s %s" % (case, precision))
+    kw = {}
+    if case == "random_sph":
+        cfg = H.small_spherical(nx=91, ny=77)
+        U = cfg["U"].copy()
+        U[:3] = np.random.default_rng(5).standard_normal((3, 77, 91))
+        cfg = dict(cfg, U=U)
+    elif case.startswith("linear_cart"):
+        cfg = H.small_cartesian_linear(nx=83, ny=67)
+        if case.endswith("lfc"):
+            kw["linear_friction_coeff"] = 1e-4
+    else:
+        cfg = H.small_spherical(nx=91, ny=77)
+        if case.startswith("llpnf"):
+            cfg = dict(cfg, method="leapfrog_linear_plus_nonlinear_friction",
+                       manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.03 ** 2))
+    o, dt, _ = H.cpu_run("oracle", cfg, 0, **kw)
+    runs = []
+    for variant in (2, 2 | 32):
+        g = H.gpu_domain(cfg, precision=precision, max_U_update_frequency=3, **kw)
+        g.set_launch(variant=variant)
+        g.evolve_steps(dt, 20)
+        g.evolve_steps(dt, 21)
+        runs.append((g.U, g.scalars(), [g.max_U(q) for q in MAXQ]))
+        g.close()
+    (u0, s0, m0), (u1, s1, m1) = runs
+    assert np.array_equal(u0, u1)
+    for k in s0:
+        assert s0[k] == s1[k], k
+    for q, a, b in zip(MAXQ, m0, m1):
+        assert np.array_equal(a, b), q
+    if precision == "fp64":
+        o, _, _ = H.cpu_run("oracle", cfg, 41, dt=dt, **kw)
+        assert np.array_equal(u0, o.get_U())
+
+
 @pytest.mark.parametrize("shape", [(256, 32, 0), (64, 7, 3), (128, 64, 1), (192, 5, 2), (64, 9, 7), (256, 3, 6),
                                    (128, 11, 5), (64, 16, 4)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32k"])
