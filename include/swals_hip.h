@@ -133,6 +133,18 @@ int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double
 int swh_evolve_one_step(swh_domain *h, double timestep);
 int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps);
 
+/* Forcing patch (forcing_patch_type with apply_forcing_patch, forcing_mod.f90:24-56,
+ * 158-239): work(i0:i1, j0:j1, k0:k1) in Fortran order (1-based, variables 1 = STG ..
+ * 4 = ELV) is added to U over [start_time, end_time] at a constant rate, each update
+ * adding the fraction of the interval its step covers (add_forcing_work_to_U_over_time,
+ * :80-156): after the stage half-step for the leapfrog methods (stage and elevation
+ * forcing; UH/VH work must be zero), after each update_U for the finite-volume methods
+ * (stage, UH, VH; elevation work must be zero). One patch per domain; a new call
+ * replaces it. */
+int swh_set_forcing_patch(swh_domain *h, double start_time, double end_time, int32_t i0, int32_t i1, int32_t j0,
+                          int32_t j1, int32_t k0, int32_t k1, const double *work);
+int swh_clear_forcing_patch(swh_domain *h);
+
 int swh_get_scalars(swh_domain *h, swh_scalars *out);
 
 /* volume_interior (domain_mod.f90:1815-1849), FP64 compensated, deterministic. */

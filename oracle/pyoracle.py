@@ -74,6 +74,7 @@ def _load(kind, spherical):
     f("domain_set_record_max_U", None, [hd, _I])
     if kind == "ref":
         f("domain_sample", None, [hd, _I, ctypes.POINTER(_I), _PD])
+        f("domain_set_forcing", None, [hd, _D, _D, _I, _I, _I, _I, _I, _I, _PD])
     _libs[key] = fns
     return fns
 
@@ -181,6 +182,13 @@ class CpuDomain:
     def set_stage_bc(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
         s = (_I * 4)(*sides)
         self.f["domain_set_stage_bc"](self.h, float(amp), float(period), float(msl), s)
+
+    def set_forcing_patch(self, start_time, end_time, i0, i1, j0, j1, k0, k1, work):
+        """forcing_patch_type applied by apply_forcing_patch (compiled reference only):
+        work has shape (k1-k0+1, j1-j0+1, i1-i0+1) (1-based inclusive indices)."""
+        w = np.ascontiguousarray(work, dtype=np.float64)
+        assert w.shape == (k1 - k0 + 1, j1 - j0 + 1, i1 - i0 + 1)
+        self.f["domain_set_forcing"](self.h, float(start_time), float(end_time), i0, i1, j0, j1, k0, k1, _p(w))
 
     def sample(self, ij):
         """[stage, uh, vh] (3, n) at 1-based (i, j) cells (compiled reference only)."""

@@ -12,6 +12,7 @@ module swals_ref_capi
     use global_mod, only: dp, ip, charlen, force_double
     use domain_mod, only: domain_type, STG, UH, VH, ELV
     use boundary_mod, only: boundary_stage_transmissive_normal_momentum
+    use forcing_mod, only: forcing_patch_type, apply_forcing_patch
     use multidomain_mod, only: multidomain_type
     use timestepping_metadata_mod, only: timestepping_method_index
     implicit none
@@ -23,6 +24,9 @@ module swals_ref_capi
     ! Parameters of the sinusoidal stage boundary (one set per handle)
     real(dp), save :: bc_amp(max_domains) = 0.0_dp, bc_period(max_domains) = 1.0_dp, &
         bc_msl(max_domains) = 0.0_dp
+
+    ! One forcing patch per handle (forcing_mod.f90:24-56)
+    type(forcing_patch_type), target, save :: fpatch(max_domains)
 
     ! One reference multidomain (multidomain_mod.f90:171-276) for nesting checks
     type(multidomain_type), target, save :: md
@@ -105,6 +109,21 @@ contains
         integer :: n
         n = pool(h)%nx(1) * pool(h)%nx(2)
         U(1:4*n) = real(reshape(pool(h)%U(:,:,1:4), [4*n]), c_double)
+    end subroutine
+
+    subroutine ref_domain_set_forcing(h, start_time, end_time, i0, i1, j0, j1, k0, k1, work) &
+            bind(C, name='ref_domain_set_forcing')
+        ! A forcing patch applied by the reference's own apply_forcing_patch
+        ! (forcing_mod.f90:158-239): work(i0:i1, j0:j1, k0:k1) in Fortran order.
+        integer(c_int), value :: h, i0, i1, j0, j1, k0, k1
+        real(c_double), value :: start_time, end_time
+        real(c_double), intent(in) :: work(*)
+        integer :: n
+        call fpatch(h)%setup(real(start_time, dp), real(end_time, dp), i0, i1, j0, j1, k0, k1)
+        n = (i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1)
+        fpatch(h)%forcing_work = reshape(real(work(1:n), dp), [i1 - i0 + 1, j1 - j0 + 1, k1 - k0 + 1])
+        pool(h)%forcing_context_cptr = c_loc(fpatch(h))
+        pool(h)%forcing_subroutine => apply_forcing_patch
     end subroutine
 
     subroutine ref_domain_sample(h, n, ij, out) bind(C, name='ref_domain_sample')

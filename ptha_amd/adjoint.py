@@ -64,13 +64,14 @@ class GaugeMap:
         n = 0
         B.check(lib.swh_gauges_sample(self.dom._h), "swh_gauges_sample")
         while n < self.nsteps:
-            k = min(self.checkpoint_every - (n % self.checkpoint_every), self.nsteps - n)
-            # advance k steps, sampling on the global 'every' grid
-            for _ in range(k):
-                self.dom.evolve_one_step(self.dt)
-                n += 1
-                if n % self.every == 0:
-                    B.check(lib.swh_gauges_sample(self.dom._h), "swh_gauges_sample")
+            # advance to the next gauge sample or checkpoint, whichever comes first
+            # (runs of steps go to the device as one call, two steps per launch)
+            k = min(self.every - (n % self.every), self.checkpoint_every - (n % self.checkpoint_every),
+                    self.nsteps - n)
+            self.dom.evolve_steps(self.dt, k)
+            n += k
+            if n % self.every == 0:
+                B.check(lib.swh_gauges_sample(self.dom._h), "swh_gauges_sample")
             if n % self.checkpoint_every == 0 and n < self.nsteps:
                 self.checkpoints[n] = self.dom.U[:3].copy()
         _, vals = self.dom.gauge_series()

@@ -164,6 +164,16 @@ int maxq_now(swh_domain *h, const DevScalars *tsrc, double t)
     return SWH_OK;
 }
 
+// Fraction of a forcing patch's work applied in a step that ends at `time` and lasts
+// `dt` (add_forcing_work_to_U_over_time, forcing_mod.f90:101-120; constant rate).
+double forcing_fraction(const swh_domain *h, double time, double dt)
+{
+    if (!h->f_on) return 0.0;
+    if (!((time - dt) <= h->f_end && time >= h->f_start)) return 0.0;
+    if (h->f_start == h->f_end) return 1.0;
+    return (std::min(time, h->f_end) - std::max(time - dt, h->f_start)) / (h->f_end - h->f_start);
+}
+
 template <typename T, bool KAHAN>
 int leapfrog_step(swh_domain *h, double dt)
 {
@@ -209,8 +219,16 @@ int leapfrog_step(swh_domain *h, double dt)
     int fric_mode = 0;
     if (h->method == SWH_METHOD_LLPNF) fric_mode = h->truely_linear ? 1 : 2;
     if (fric_mode == 1 && !h->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload U first");
+    // forcing applied after the stage half-step, with time at its end (include.f90:117-119)
+    const double frac = forcing_fraction(h, t_half, dt);
+    if (frac != 0.0) {
+        a.fS = (const T *)h->fS; a.fE = (const T *)h->fE; a.f_frac = (T)frac;
+    }
     SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
                                       h->variant, h->stream));
+    if (frac != 0.0 && h->fE)
+        SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
+                                     (T)frac, h->stream));
     h->cur = o;
     h->time = t_after;
     h->dt_last_update = dt;
@@ -225,9 +243,15 @@ int leapfrog_step(swh_domain *h, double dt)
 // step need UH/VH after the first on the four boundary-flux lines: a small kernel
 // forms them there (same per-cell operations) into the output planes, the flux pass
 // reads them, and the two-step launch then overwrites the planes.
-bool two_step_ok(const swh_domain *h)
+bool two_step_ok(const swh_domain *h, double dt)
 {
     if (!h->two_step || !h->leapfrog() || h->bc_mask || h->n_nest_strips) return false;
+    if (h->f_on) {
+        // no forcing in either step of the pair
+        const double t_half1 = h->time + dt * 0.5, t_after1 = t_half1 + 0.5 * dt;
+        if (forcing_fraction(h, t_half1, dt) != 0.0 || forcing_fraction(h, t_after1 + dt * 0.5, dt) != 0.0)
+            return false;
+    }
     if (h->method == SWH_METHOD_LLPNF && !h->truely_linear) return false;
     if (h->maxq_mask && (h->nsteps % h->maxq_freq) == 0) return false;
     if ((long long)h->pitch * h->ny >= (1LL << 31)) return false;
@@ -353,6 +377,11 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
     if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], h->nx, h->ny, h->pitch,
                                       h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
+    if (h->f_on)
+        // update_U then apply_forcing (domain_mod.f90:1579-1580), with the stage's step
+        SWH_HIP(launch_fv_forcing<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->fS,
+                                     (const T *)h->fUH, (const T *)h->fVH, h->pitch, h->f_i0, h->f_i1, h->f_j0,
+                                     h->f_j1, h->f_start, h->f_end, h->sc, h->stream));
     return SWH_OK;
 }
 
@@ -617,7 +646,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
-                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx};
+                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE};
     for (void *p : ptrs) dfree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -705,7 +734,7 @@ int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps)
     if (nsteps < 0) return fail(SWH_EINVAL, "nsteps must be >= 0");
     for (int k = 0; k < nsteps;) {
         int rc;
-        if (k + 1 < nsteps && timestep > 0.0 && two_step_ok(h)) {
+        if (k + 1 < nsteps && timestep > 0.0 && two_step_ok(h, timestep)) {
             rc = h->prec == SWH_PREC_FP64 ? leapfrog_step2<double, false>(h, timestep)
                                           : leapfrog_step2<float, true>(h, timestep);
             k += 2;
@@ -715,6 +744,61 @@ int swh_evolve_steps(swh_domain *h, double timestep, int32_t nsteps)
         }
         if (rc) return rc;
     }
+    return SWH_OK;
+}
+
+int swh_set_forcing_patch(swh_domain *h, double start_time, double end_time, int32_t i0, int32_t i1, int32_t j0,
+                          int32_t j1, int32_t k0, int32_t k1, const double *work)
+{
+    if (!h || !work) return fail(SWH_EINVAL, "null argument");
+    // apply_forcing_patch_base's checks (forcing_mod.f90:177-205)
+    if (i0 > i1 || j0 > j1 || i0 < 1 || j0 < 1 || i1 > h->nx || j1 > h->ny)
+        return fail(SWH_EINVAL, "forcing patch indices outside the domain or not ordered");
+    if (start_time > end_time) return fail(SWH_EINVAL, "forcing patch start_time > end_time");
+    if (!(k0 >= 1 && k1 <= 4 && k0 <= k1)) return fail(SWH_EINVAL, "forcing patch variables must be k0..k1 in 1..4");
+    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+    const size_t np = (size_t)ni * nj;
+    auto nonzero = [&](int k) {
+        if (k < k0 || k > k1) return false;
+        const double *w = work + (size_t)(k - k0) * np;
+        for (size_t q = 0; q < np; q++)
+            if (w[q] != 0.0) return true;
+        return false;
+    };
+    if (h->leapfrog() && (nonzero(2) || nonzero(3)))
+        return fail(SWH_EINVAL, "forcing of UH/VH is not provided for the leapfrog methods");
+    if (!h->leapfrog() && nonzero(4))
+        return fail(SWH_EINVAL,
+                    "elevation forcing needs support_elevation_forcing, which the finite-volume methods here do not have");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    void **planes[4] = {&h->fS, &h->fUH, &h->fVH, &h->fE};
+    std::vector<double> full((size_t)h->nx * h->ny);
+    for (int k = 1; k <= 4; k++) {
+        dfree(*planes[k - 1]);
+        *planes[k - 1] = nullptr;
+        if (!nonzero(k)) continue;
+        std::fill(full.begin(), full.end(), 0.0);
+        const double *w = work + (size_t)(k - k0) * np;
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) full[(size_t)(j0 - 1 + j) * h->nx + (i0 - 1 + i)] = w[(size_t)j * ni + i];
+        int rc = dmalloc(planes[k - 1], h->plane_elems() * h->esz);
+        if (!rc) rc = upload_plane_any(h, *planes[k - 1], full.data());
+        if (rc) return rc;
+    }
+    (void)nk;
+    h->f_on = 1;
+    h->f_start = start_time; h->f_end = end_time;
+    h->f_i0 = i0 - 1; h->f_i1 = i1 - 1; h->f_j0 = j0 - 1; h->f_j1 = j1 - 1;
+    return SWH_OK;
+}
+
+int swh_clear_forcing_patch(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    dfree(h->fS); dfree(h->fUH); dfree(h->fVH); dfree(h->fE);
+    h->fS = h->fUH = h->fVH = h->fE = nullptr;
+    h->f_on = 0;
     return SWH_OK;
 }
 

@@ -36,6 +36,11 @@ struct swh_domain {
     double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;
     long long nsteps = 0;
     int fw_ready = 0;
+    // forcing patch (forcing_patch_type, forcing_mod.f90:24-56): work planes over the
+    // whole grid, zero outside the patch; bounds 0-based inclusive
+    void *fS = nullptr, *fUH = nullptr, *fVH = nullptr, *fE = nullptr;
+    int f_on = 0, f_i0 = 0, f_i1 = -1, f_j0 = 0, f_j1 = -1;
+    double f_start = 0.0, f_end = 0.0;
     // boundary
     int bc_mask = 0;
     double bc_amp = 0.0, bc_period = 1.0, bc_msl = 0.0;

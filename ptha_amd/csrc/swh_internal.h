@@ -98,6 +98,11 @@ struct LFArgs {
     int maxq_mask;
     T *mx;                     // N_MAXQ planes of pitch*ny
     T t_after, arrival_stage;
+    // forcing patch of this step (apply_forcing after the stage half-step,
+    // forcing_mod.f90:80-156): stage += f_frac * fS, elevation += f_frac * fE
+    // (planes zero outside the patch; null: no such forcing this step)
+    const T *fS = nullptr, *fE = nullptr;
+    T f_frac = T(0);
 };
 
 // Kernel launchers (defined in the .hip files)
@@ -135,6 +140,17 @@ struct SupArgs {
     int s_chunk, s_max;      // maximum-stage summary: samples per block, samples to scan
     double *out;             // mode 0: [E][gmax * 3 * ng]; mode 1: [E][ng]
 };
+// Forcing patch on a finite-volume state after update_U (forcing_mod.f90:80-156):
+// the fraction of [start, end] that the stage (time - stage_dt, time] covers, read
+// from the device scalars, times the work planes (S, UH, VH; null: none) added to
+// the state over the patch cells (0-based, inclusive).
+template <typename T>
+hipError_t launch_fv_forcing(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0, int i1,
+                             int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s);
+// elevation += frac * fE over the patch (the leapfrog step's elevation forcing)
+template <typename T>
+hipError_t launch_add_scaled(T *E, const T *fE, int pitch, int i0, int i1, int j0, int j1, T frac, hipStream_t s);
+
 // rows of the event tile (W, scale and nvalid must hold a multiple of it)
 int superpose_event_tile();
 hipError_t launch_superpose(const SupArgs &a, int mode, hipStream_t s);

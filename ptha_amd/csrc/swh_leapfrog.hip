@@ -109,6 +109,13 @@ __device__ __forceinline__ void load_vrow(const LFArgs<T> &a, const T *__restric
     }
     // UH at column i-1, clamped to column 0 at the western edge (include.f90:319-322)
     r.uh[0] = (i > 0) ? sload<T, NT>(UH + o - 1) : r.uh[1];
+    if (a.fE) {
+        // the elevation after this step's forcing (applied after the stage half-step,
+        // so the momentum half-step sees it); the plane is updated after the step
+#pragma unroll
+        for (int c = 0; c < VEC; c++) r.e[c] = r.e[c] + a.f_frac * a.fE[o + c];
+        if (right) r.e[VEC] = r.e[VEC] + a.f_frac * a.fE[o + VEC];
+    }
 }
 
 // New stage of cell (ii, jj): interior cells get the flux divergence
@@ -153,6 +160,23 @@ __device__ __forceinline__ void stage_row(const LFArgs<T> &a, int i, int jj, con
 #pragma unroll
     for (int c = 0; c <= VEC; c++)
         stage_new<T, KAHAN>(a, i + c, jj, q.s[c], q.c[c], q.uh[c + 1], q.uh[c], q.vh[c], vhm[c], sn[c], cn[c]);
+    if (a.fS) {
+        // apply_forcing after update_boundary (include.f90:117-119; forcing_mod.f90:80-156)
+        const long long o = (long long)jj * a.pitch + i;
+#pragma unroll
+        for (int c = 0; c <= VEC; c++) {
+            if (i + c >= a.nx) continue;
+            const T inc = a.f_frac * a.fS[o + c];
+            if (KAHAN) {
+                const T y = inc - cn[c];
+                const T t = sn[c] + y;
+                cn[c] = (t - sn[c]) - y;
+                sn[c] = t;
+            } else {
+                sn[c] = sn[c] + inc;
+            }
+        }
+    }
 }
 
 template <typename T>
@@ -307,7 +331,8 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
             for (int c = 0; c < VEC; c++) {
                 if (c >= nvalid) continue;
                 const long long oc = o + c;
-                const T s0 = sn[c], e0 = r.e[c], uhn = uh_out[c], vhn = vh_out[c];
+                // the compensated stage is sn - cn (as the stand-alone max_U pass reads it)
+                const T s0 = KAHAN ? sn[c] - cn[c] : sn[c], e0 = r.e[c], uhn = uh_out[c], vhn = vh_out[c];
                 if (m & SWH_MAXQ_MAX_STAGE) {
                     if (m & SWH_MAXQ_TIME_OF_MAX_STAGE) {
                         if (s0 > mx[MX_STAGE * P + oc]) {
@@ -320,8 +345,8 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
                 }
                 if (m & SWH_MAXQ_MAX_SPEED) {
                     const bool rt = (i + c + 1) < a.nx;
-                    const T sE = rt ? sn[c + 1] : s0, eE = rt ? r.e[c + 1] : e0;
-                    const T sN = has_up ? su[c] : s0, eN = has_up ? q.e[c] : e0;
+                    const T sE = rt ? (KAHAN ? sn[c + 1] - cn[c + 1] : sn[c + 1]) : s0, eE = rt ? r.e[c + 1] : e0;
+                    const T sN = has_up ? (KAHAN ? su[c] - cu[c] : su[c]) : s0, eN = has_up ? q.e[c] : e0;
                     T dE, dN;
                     if (TL) {
                         dE = HALF * ((msl - e0) + (msl - eE));

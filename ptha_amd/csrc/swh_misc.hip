@@ -118,6 +118,63 @@ template hipError_t launch_bflux_leapfrog<double>(const double *, const double *
 template hipError_t launch_bflux_leapfrog<float>(const float *, const float *, int, int, int, int, const double *,
                                                  const double *, double, double, DevScalars *, hipStream_t);
 
+// ---- forcing patches (forcing_mod.f90:80-156) ----
+template <typename T>
+__global__ void fv_forcing_kernel(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0,
+                                  int i1, int j0, int j1, double start, double end, const DevScalars *sc)
+{
+    // update_U advanced the time by the stage's step: the step covers (time - dt, time]
+    const double time = sc->time, dt = sc->stage_dt;
+    if (!((time - dt) <= end && time >= start)) return;
+    double frac;
+    if (start == end) frac = 1.0;
+    else frac = (fmin(time, end) - fmax(time - dt, start)) / (end - start);
+    const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + blockIdx.y;
+    if (i > i1 || j > j1) return;
+    const long long o = (long long)j * pitch + i;
+    if (fS) S[o] = (T)((double)S[o] + frac * (double)fS[o]);
+    if (fUH) UH[o] = (T)((double)UH[o] + frac * (double)fUH[o]);
+    if (fVH) VH[o] = (T)((double)VH[o] + frac * (double)fVH[o]);
+}
+
+template <typename T>
+__global__ void add_scaled_kernel(T *E, const T *fE, int pitch, int i0, int i1, int j0, int j1, T frac)
+{
+    const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + blockIdx.y;
+    if (i > i1 || j > j1) return;
+    const long long o = (long long)j * pitch + i;
+    E[o] = E[o] + frac * fE[o];
+}
+
+template <typename T>
+hipError_t launch_fv_forcing(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0, int i1,
+                             int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s)
+{
+    if (i1 < i0 || j1 < j0) return hipSuccess;
+    hipLaunchKernelGGL(fv_forcing_kernel<T>, dim3((i1 - i0 + 256) / 256, j1 - j0 + 1), dim3(256), 0, s, S, UH, VH, fS,
+                       fUH, fVH, pitch, i0, i1, j0, j1, start, end, sc);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_add_scaled(T *E, const T *fE, int pitch, int i0, int i1, int j0, int j1, T frac, hipStream_t s)
+{
+    if (i1 < i0 || j1 < j0) return hipSuccess;
+    hipLaunchKernelGGL(add_scaled_kernel<T>, dim3((i1 - i0 + 256) / 256, j1 - j0 + 1), dim3(256), 0, s, E, fE, pitch,
+                       i0, i1, j0, j1, frac);
+    return hipGetLastError();
+}
+
+template hipError_t launch_fv_forcing<double>(double *, double *, double *, const double *, const double *,
+                                              const double *, int, int, int, int, int, double, double,
+                                              const DevScalars *, hipStream_t);
+template hipError_t launch_fv_forcing<float>(float *, float *, float *, const float *, const float *, const float *,
+                                             int, int, int, int, int, double, double, const DevScalars *, hipStream_t);
+template hipError_t launch_add_scaled<double>(double *, const double *, int, int, int, int, int, double, hipStream_t);
+template hipError_t launch_add_scaled<float>(float *, const float *, int, int, int, int, int, float, hipStream_t);
+
 // ---- standalone max_U update (domain_mod.f90:2243-2404) ----
 template <typename T>
 __global__ void maxq_kernel(const T *__restrict__ S, const T *__restrict__ C, const T *__restrict__ UH,

@@ -160,6 +160,19 @@ class Domain:
     def set_time(self, t):
         B.check(self._lib.swh_set_time(self._h, float(t)), "swh_set_time")
 
+    def set_forcing_patch(self, start_time, end_time, i0, i1, j0, j1, k0, k1, work):
+        """forcing_patch_type + apply_forcing_patch (forcing_mod.f90:24-56, 158-239):
+        work (k1-k0+1, j1-j0+1, i1-i0+1) is added to U(i0:i1, j0:j1, k0:k1) (1-based,
+        variables 1 stage .. 4 elevation) at a constant rate over [start_time, end_time]."""
+        w = np.ascontiguousarray(work, dtype=np.float64)
+        if w.shape != (k1 - k0 + 1, j1 - j0 + 1, i1 - i0 + 1):
+            raise ValueError("work must have shape (k1-k0+1, j1-j0+1, i1-i0+1)")
+        B.check(self._lib.swh_set_forcing_patch(self._h, float(start_time), float(end_time), int(i0), int(i1),
+                                                int(j0), int(j1), int(k0), int(k1), _p(w)), "swh_set_forcing_patch")
+
+    def clear_forcing_patch(self):
+        B.check(self._lib.swh_clear_forcing_patch(self._h), "swh_clear_forcing_patch")
+
     def set_stage_boundary_sine(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
         s = (B._I32 * 4)(*[int(bool(v)) for v in sides])
         B.check(self._lib.swh_set_stage_boundary_sine(self._h, float(amp), float(period), float(msl), s),
