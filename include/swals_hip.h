@@ -124,6 +124,15 @@ int swh_set_time(swh_domain *h, double time);
  * for this case. */
 int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double msl, const int32_t sides[4]);
 
+/* Flather boundary: flather_boundary (boundary_mod.f90:502-720) evaluated on the device
+ * for the flagged sides (N, E, S, W), at the reference's update_boundary calls (after
+ * the leapfrog stage half-step; after each finite-volume update). The outside state is
+ * the sea at rest at max(msl_linear, elevation) (no boundary_function), or with
+ * use_stage_function the stage msl + amp * sin(2 pi t / period) with zero momenta.
+ * Replaces domain%boundary_subroutine => flather_boundary. Needs nx, ny >= 3. */
+int swh_set_flather_boundary(swh_domain *h, const int32_t sides[4], int32_t use_stage_function, double amp,
+                             double period, double msl);
+
 /* domain%evolve_one_step(timestep) (domain_mod.f90:1679-1813) for 'linear',
  * 'leapfrog_linear_plus_nonlinear_friction', 'rk2', 'euler', 'midpoint' and 'rk2n'
  * (domain_mod_timestepping_alternatives_include.f90:8-336). timestep <= 0 means

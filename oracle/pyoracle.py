@@ -71,6 +71,7 @@ def _load(kind, spherical):
     f("domain_stationary_timestep_max", _D, [hd])
     f("domain_get_max_U", None, [hd, _I, _PD])
     f("domain_set_stage_bc", None, [hd, _D, _D, _D, ctypes.POINTER(_I)])
+    f("domain_set_flather_bc", None, [hd, _I, _D, _D, _D, ctypes.POINTER(_I)])
     f("domain_set_record_max_U", None, [hd, _I])
     if kind == "ref":
         f("domain_sample", None, [hd, _I, ctypes.POINTER(_I), _PD])
@@ -182,6 +183,13 @@ class CpuDomain:
     def set_stage_bc(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
         s = (_I * 4)(*sides)
         self.f["domain_set_stage_bc"](self.h, float(amp), float(period), float(msl), s)
+
+    def set_flather_bc(self, sides=(1, 1, 1, 1), amp=None, period=1.0, msl=0.0):
+        """Flather boundary on sides [N, E, S, W]; amp None: outside sea at rest at
+        msl_linear, else the sine stage msl + amp*sin(2*pi*t/period)."""
+        s = (_I * 4)(*sides)
+        use_fun = amp is not None
+        self.f["domain_set_flather_bc"](self.h, int(use_fun), float(amp or 0.0), float(period), float(msl), s)
 
     def set_forcing_patch(self, start_time, end_time, i0, i1, j0, j1, k0, k1, work):
         """forcing_patch_type applied by apply_forcing_patch (compiled reference only):

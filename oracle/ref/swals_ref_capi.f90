@@ -11,7 +11,7 @@ module swals_ref_capi
     use iso_c_binding
     use global_mod, only: dp, ip, charlen, force_double
     use domain_mod, only: domain_type, STG, UH, VH, ELV
-    use boundary_mod, only: boundary_stage_transmissive_normal_momentum
+    use boundary_mod, only: boundary_stage_transmissive_normal_momentum, flather_boundary
     use forcing_mod, only: forcing_patch_type, apply_forcing_patch
     use multidomain_mod, only: multidomain_type
     use timestepping_metadata_mod, only: timestepping_method_index
@@ -259,6 +259,24 @@ contains
         pool(h)%boundary_exterior = (sides /= 0)
         pool(h)%boundary_function => ref_stage_fun
         pool(h)%boundary_subroutine => boundary_stage_transmissive_normal_momentum
+    end subroutine
+
+    subroutine ref_domain_set_flather_bc(h, use_fun, amp, period, msl, sides) bind(C, name='ref_domain_set_flather_bc')
+        ! Installs flather_boundary (propagation/SWALS/src/shallow_water/boundary_mod.f90:502-720)
+        ! on the sides flagged in sides(1:4) = [N, E, S, W]; with use_fun /= 0 the outside
+        ! state comes from the sine stage function, else from the sea at rest.
+        integer(c_int), value :: h, use_fun
+        real(c_double), value :: amp, period, msl
+        integer(c_int), intent(in) :: sides(4)
+        bc_amp(h) = amp; bc_period(h) = period; bc_msl(h) = msl
+        pool(h)%local_index = h
+        pool(h)%boundary_exterior = (sides /= 0)
+        if(use_fun /= 0) then
+            pool(h)%boundary_function => ref_stage_fun
+        else
+            pool(h)%boundary_function => null()
+        end if
+        pool(h)%boundary_subroutine => flather_boundary
     end subroutine
 
     ! ---------------------------------------------------------------------

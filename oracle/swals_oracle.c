@@ -49,8 +49,9 @@ struct or_domain {
     int record_max_U;
     double *max_U;
     double arrival_stage_threshold;
-    /* stage boundary */
-    int bc_on;
+    /* boundary: bc_on 1 = transmissive stage boundary, 2 = Flather; bc_fun: the Flather
+     * outside state comes from the sine stage function (else msl_linear at rest) */
+    int bc_on, bc_fun;
     double bc_amp, bc_period, bc_msl;
 };
 
@@ -186,9 +187,94 @@ void or_domain_get_metrics(const or_domain *d, double *x, double *y, double *dbe
 
 /* boundary_mod.f90:78-164 (boundary_stage_transmissive_normal_momentum) with the
  * stage function msl + amp*sin(2*pi*t/period) and elevation left unchanged. */
+/* Outside state of the Flather boundary at cell (i, j) (boundary_mod.f90:528-545):
+ * without a boundary function, the sea at rest at max(msl_linear, elevation); with
+ * one, its stage and momenta (here the sine stage, zero momenta, the cell's
+ * elevation), velocities zero where the outside is dry. */
+static void flather_outside(const or_domain *d, int i, int j, double *stage_o, double *u_o, double *v_o,
+                            double *depth_o)
+{
+    if (!d->bc_fun) {
+        *stage_o = dmax(d->msl_linear, A(d, ELV, i, j));
+        *u_o = 0.0;
+        *v_o = 0.0;
+        *depth_o = *stage_o - A(d, ELV, i, j);
+    } else {
+        const double two_pi = 2.0 * acos(-1.0);
+        const double st = d->bc_msl + d->bc_amp * sin(two_pi * d->time / d->bc_period);
+        const double uh_o = 0.0, vh_o = 0.0;
+        *stage_o = st;
+        *depth_o = st - A(d, ELV, i, j);
+        if (*depth_o <= MIN_DEPTH) {
+            *u_o = 0.0;
+            *v_o = 0.0;
+        } else {
+            *u_o = uh_o / *depth_o;
+            *v_o = vh_o / *depth_o;
+        }
+    }
+}
+
+/* One Flather edge cell (boundary_mod.f90:547-569 and its three mirror images).
+ * (b, ib, jb): boundary cell; (ii, ji): the cell inside it; sgn = -1 on the west and
+ * south edges (the characteristic entering from outside moves in +x / +y), +1 on
+ * the east and north; normal = UH on the west/east edges, VH on the south/north. */
+static void flather_cell(or_domain *d, int ib, int jb, int ii, int ji, double sgn, int normal)
+{
+    const int tang = normal == UH ? VH : UH;
+    double stage_o, u_o, v_o, depth_o;
+    flather_outside(d, ib, jb, &stage_o, &u_o, &v_o, &depth_o);
+    const double vel_o = normal == UH ? u_o : v_o;
+    const double depth_inside = A(d, STG, ii, ji) - A(d, ELV, ii, ji);
+    if ((depth_inside > MIN_DEPTH) && (depth_o > MIN_DEPTH)) {
+        const double depth_inside_inv = 1.0 / depth_inside;
+        const double sqrt_g_on_di = sqrt(GRAVITY * depth_inside_inv);
+        const double qn = A(d, normal, ii, ji);
+        double w1, w2, w3, stage;
+        if (sgn < 0) {
+            w1 = vel_o + sqrt_g_on_di * stage_o;
+            w2 = (qn < 0.0) ? A(d, tang, ii, ji) * depth_inside_inv : 0.0;
+            w3 = qn * depth_inside_inv - sqrt_g_on_di * A(d, STG, ii, ji);
+            stage = (-w3 + w1) / (2.0 * sqrt_g_on_di);
+        } else {
+            w1 = vel_o - sqrt_g_on_di * stage_o;
+            w2 = (qn > 0.0) ? A(d, tang, ii, ji) * depth_inside_inv : 0.0;
+            w3 = qn * depth_inside_inv + sqrt_g_on_di * A(d, STG, ii, ji);
+            stage = (w3 - w1) / (2.0 * sqrt_g_on_di);
+        }
+        A(d, STG, ib, jb) = stage;
+        A(d, normal, ib, jb) = (w3 + w1) * 0.5 * depth_inside;
+        A(d, tang, ib, jb) = w2 * depth_inside;
+    } else {
+        A(d, STG, ib, jb) = stage_o;
+        A(d, UH, ib, jb) = 0.0;
+        A(d, VH, ib, jb) = 0.0;
+    }
+}
+
+/* boundary_mod.f90:502-720 (flather_boundary): west and east columns for every row,
+ * then south and north rows for every column (so the corners end with the
+ * south/north treatment, reading the west/east results next to them). */
+static void flather_boundary(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    for (int j = 1; j <= ny; j++) {
+        if (d->boundary_exterior[3]) flather_cell(d, 1, j, 2, j, -1.0, UH);
+        if (d->boundary_exterior[1]) flather_cell(d, nx, j, nx - 1, j, 1.0, UH);
+    }
+    for (int i = 1; i <= nx; i++) {
+        if (d->boundary_exterior[2]) flather_cell(d, i, 1, i, 2, -1.0, VH);
+        if (d->boundary_exterior[0]) flather_cell(d, i, ny, i, ny - 1, 1.0, VH);
+    }
+}
+
 static void update_boundary(or_domain *d)
 {
     if (!d->bc_on) return;
+    if (d->bc_on == 2) {
+        flather_boundary(d);
+        return;
+    }
     const int nx = d->nx, ny = d->ny;
     const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF);
     const double two_pi = 2.0 * acos(-1.0);
@@ -221,7 +307,13 @@ static void update_boundary(or_domain *d)
 
 void or_domain_set_stage_bc(or_domain *d, double amp, double period, double msl, const int sides[4])
 {
-    d->bc_on = 1; d->bc_amp = amp; d->bc_period = period; d->bc_msl = msl;
+    d->bc_on = 1; d->bc_fun = 1; d->bc_amp = amp; d->bc_period = period; d->bc_msl = msl;
+    for (int k = 0; k < 4; k++) d->boundary_exterior[k] = sides[k] != 0;
+}
+
+void or_domain_set_flather_bc(or_domain *d, int use_fun, double amp, double period, double msl, const int sides[4])
+{
+    d->bc_on = 2; d->bc_fun = use_fun != 0; d->bc_amp = amp; d->bc_period = period; d->bc_msl = msl;
     for (int k = 0; k < 4; k++) d->boundary_exterior[k] = sides[k] != 0;
 }
 
@@ -778,15 +870,17 @@ static void update_max_quantities(or_domain *d)
             /* include_dry_cells = .true. (domain_mod.f90 default) */
             if (stg > m[MX_STAGE * n + k]) { m[MX_STAGE * n + k] = stg; m[MX_TOMS * n + k] = d->time; }
             if (staggered) {
-                int ip1 = i + 1 < nx ? i + 1 : nx, jp1 = j + 1 < ny ? j + 1 : ny;
-                double depth_E, depth_N;
-                if (truely) {
-                    depth_E = 0.5 * ((d->msl_linear - A(d, ELV, i, j)) + (d->msl_linear - A(d, ELV, ip1, j)));
-                    depth_N = 0.5 * ((d->msl_linear - A(d, ELV, i, j)) + (d->msl_linear - A(d, ELV, i, jp1)));
-                } else {
-                    depth_E = 0.5 * ((A(d, STG, i, j) - A(d, ELV, i, j)) + (A(d, STG, ip1, j) - A(d, ELV, ip1, j)));
-                    depth_N = 0.5 * ((A(d, STG, i, j) - A(d, ELV, i, j)) + (A(d, STG, i, jp1) - A(d, ELV, i, jp1)));
-                }
+                /* depth_E = 0.5 * sum(depth(i:ip1, j)) with ip1 = min(i+1, nx): on the east
+                 * column the sum has one term (half the cell's depth), likewise depth_N on
+                 * the north row (domain_mod.f90:2311-2320) */
+                const double h0 = truely ? (d->msl_linear - A(d, ELV, i, j)) : (A(d, STG, i, j) - A(d, ELV, i, j));
+                double depth_E = h0, depth_N = h0;
+                if (i < nx)
+                    depth_E = h0 + (truely ? (d->msl_linear - A(d, ELV, i + 1, j)) : (A(d, STG, i + 1, j) - A(d, ELV, i + 1, j)));
+                if (j < ny)
+                    depth_N = h0 + (truely ? (d->msl_linear - A(d, ELV, i, j + 1)) : (A(d, STG, i, j + 1) - A(d, ELV, i, j + 1)));
+                depth_E = 0.5 * depth_E;
+                depth_N = 0.5 * depth_N;
                 double iE = depth_E > MIN_DEPTH ? 1.0 / depth_E : 0.0;
                 double iN = depth_N > MIN_DEPTH ? 1.0 / depth_N : 0.0;
                 double sp = sqrt(uh * uh * (iE * iE) + vh * vh * (iN * iN));

@@ -40,6 +40,9 @@ double or_domain_volume_interior(const or_domain *d);
 double or_domain_stationary_timestep_max(const or_domain *d);
 void or_domain_get_max_U(const or_domain *d, int k, double *out);
 void or_domain_set_stage_bc(or_domain *d, double amp, double period, double msl, const int sides[4]);
+/* Flather boundary (boundary_mod.f90:502-720) on sides [N, E, S, W]; use_fun: outside
+ * state from the sine stage msl + amp*sin(2*pi*t/period) instead of the sea at rest. */
+void or_domain_set_flather_bc(or_domain *d, int use_fun, double amp, double period, double msl, const int sides[4]);
 void or_domain_set_record_max_U(or_domain *d, int on);
 void or_domain_get_gauges(const or_domain *d, int n, const int *ij, double *out);
 

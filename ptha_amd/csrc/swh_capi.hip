@@ -194,9 +194,13 @@ int leapfrog_step(swh_domain *h, double dt)
     if (h->n_nest_strips)
         SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->VH[c], (const T *)h->UH[c], 0,
                                      0, h->pitch, 1, dt, h->d_nest_int, h->stream));
+    // Flather boundary: it needs the new stage inside the edges before the momentum
+    // half-step, and changes UH/VH on the edges that the momentum half-step reads
+    const bool flather = h->bc_mask && h->bc_kind == 1;
     // boundary fluxes read the pre-momentum UH/VH (include.f90:126-165)
-    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                     h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+    if (!flather)
+        SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
+                                         h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
     LFArgs<T> a;
     a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
     a.S = (const T *)h->S[c]; a.C = (const T *)h->C[c]; a.UH = (const T *)h->UH[c]; a.VH = (const T *)h->VH[c];
@@ -208,7 +212,8 @@ int leapfrog_step(swh_domain *h, double dt)
     a.lin_fric_mult = (T)(1.0 / (1.0 + dt * h->lfc));
     a.fric_power = (T)((h->friction_type == 1) ? -2.0 : -7.0 / 3.0);
     a.bc_mask = h->bc_mask;
-    a.bc_stage = (T)(h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period));
+    const double bc_stage = h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period);
+    a.bc_stage = (T)bc_stage;
     a.rows_per_block = h->rows_per_block;
     a.src_fast = 0;
     const bool upd = h->maxq_mask && ((h->nsteps % h->maxq_freq) == 0);
@@ -221,14 +226,39 @@ int leapfrog_step(swh_domain *h, double dt)
     if (fric_mode == 1 && !h->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload U first");
     // forcing applied after the stage half-step, with time at its end (include.f90:117-119)
     const double frac = forcing_fraction(h, t_half, dt);
-    if (frac != 0.0) {
-        a.fS = (const T *)h->fS; a.fE = (const T *)h->fE; a.f_frac = (T)frac;
+    if (flather) {
+        // stage half-step alone into the output planes, then the boundary
+        // (include.f90:92-119: stage, update_boundary, apply_forcing), then the
+        // boundary fluxes and the momentum half-step from the edited planes
+        a.bc_mask = 0;
+        SWH_HIP(launch_lf_stage<T, KAHAN>(a, h->stream));
+        T *Cb = KAHAN ? (T *)h->C[o] : nullptr;
+        SWH_HIP(launch_flather<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (const T *)h->E, h->nx, h->ny,
+                                  h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl,
+                                  bc_stage, nullptr, h->stream));
+        if (frac != 0.0 && h->fS)
+            SWH_HIP(launch_add_scaled_kahan<T>((T *)h->S[o], Cb, (const T *)h->fS, h->pitch, h->f_i0, h->f_i1,
+                                               h->f_j0, h->f_j1, (T)frac, h->stream));
+        if (frac != 0.0 && h->fE)
+            SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
+                                         (T)frac, h->stream));
+        SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
+                                         h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+        a.S = (const T *)h->S[o];
+        a.C = (const T *)h->C[o];
+        a.stage_given = 1;
+        SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
+                                          h->variant, h->stream));
+    } else {
+        if (frac != 0.0) {
+            a.fS = (const T *)h->fS; a.fE = (const T *)h->fE; a.f_frac = (T)frac;
+        }
+        SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
+                                          h->variant, h->stream));
+        if (frac != 0.0 && h->fE)
+            SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
+                                         (T)frac, h->stream));
     }
-    SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
-                                      h->variant, h->stream));
-    if (frac != 0.0 && h->fE)
-        SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
-                                     (T)frac, h->stream));
     h->cur = o;
     h->time = t_after;
     h->dt_last_update = dt;
@@ -374,7 +404,11 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
         SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
                                      EPn, h->fpitch, 3, nest_dt, h->d_nest_int, h->stream));
     }
-    if (h->bc_mask)
+    if (h->bc_mask && h->bc_kind == 1)
+        SWH_HIP(launch_flather<T>((T *)h->S[dst], nullptr, (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->E, h->nx,
+                                  h->ny, h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period,
+                                  h->bc_msl, 0.0, h->sc, h->stream));
+    else if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], h->nx, h->ny, h->pitch,
                                       h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
     if (h->f_on)
@@ -719,6 +753,20 @@ int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double
     h->bc_mask = (sides[0] ? 1 : 0) | (sides[1] ? 2 : 0) | (sides[2] ? 4 : 0) | (sides[3] ? 8 : 0);
     for (int k = 0; k < 4; k++) h->bex[k] = sides[k] != 0;
     h->bc_amp = amp; h->bc_period = period; h->bc_msl = msl;
+    h->bc_kind = 0; h->bc_use_fun = 1;
+    return SWH_OK;
+}
+
+int swh_set_flather_boundary(swh_domain *h, const int32_t sides[4], int32_t use_stage_function, double amp,
+                             double period, double msl)
+{
+    if (!h || !sides) return fail(SWH_EINVAL, "null argument");
+    if (use_stage_function && !(period != 0.0)) return fail(SWH_EINVAL, "period must be non-zero");
+    if (h->nx < 3 || h->ny < 3) return fail(SWH_EINVAL, "the Flather boundary needs at least 3 x 3 cells");
+    h->bc_mask = (sides[0] ? 1 : 0) | (sides[1] ? 2 : 0) | (sides[2] ? 4 : 0) | (sides[3] ? 8 : 0);
+    for (int k = 0; k < 4; k++) h->bex[k] = sides[k] != 0;
+    h->bc_amp = amp; h->bc_period = period; h->bc_msl = msl;
+    h->bc_kind = 1; h->bc_use_fun = use_stage_function != 0;
     return SWH_OK;
 }
 

@@ -44,6 +44,9 @@ struct swh_domain {
     // boundary
     int bc_mask = 0;
     double bc_amp = 0.0, bc_period = 1.0, bc_msl = 0.0;
+    // 0: transmissive stage boundary (sine stage); 1: Flather, whose outside state is the
+    // sine stage when bc_use_fun, else the sea at rest at msl_linear
+    int bc_kind = 0, bc_use_fun = 1;
     // gauges
     int ng = 0, gmax = 0, gsamples = 0;
     long long *d_gidx = nullptr;

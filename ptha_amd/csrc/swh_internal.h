@@ -103,6 +103,9 @@ struct LFArgs {
     // (planes zero outside the patch; null: no such forcing this step)
     const T *fS = nullptr, *fE = nullptr;
     T f_frac = T(0);
+    // 1: the inputs S/C already hold the new stage (formed by lf_stage_kernel and the
+    // boundary); the step passes it through and writes only UH/VH
+    int stage_given = 0;
 };
 
 // Kernel launchers (defined in the .hip files)
@@ -148,6 +151,25 @@ template <typename T>
 hipError_t launch_fv_forcing(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0, int i1,
                              int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s);
 // elevation += frac * fE over the patch (the leapfrog step's elevation forcing)
+// New stage only (the stage half-step of the fused step, same per-cell operations),
+// S/C -> So/Co over the whole grid, border cells passed through
+template <typename T, bool KAHAN>
+hipError_t launch_lf_stage(const LFArgs<T> &a, hipStream_t s);
+// Flather boundary (boundary_mod.f90:502-720) on the edges flagged in mask (1 N, 2 E,
+// 4 S, 8 W): one block, west/east columns then south/north rows. Outside state: the
+// sea at rest at max(msl_linear, elevation), or (use_fun) the stage
+// fun_msl + amp*sin(2*pi*t/period) with zero momenta, t from the device scalars when
+// sc is given, else fun_stage is the host-computed stage. C (Kahan compensation, may
+// be null) is zeroed on the cells written.
+template <typename T>
+hipError_t launch_flather(T *S, T *C, T *UH, T *VH, const T *E, int nx, int ny, int pitch, int mask, double msl_linear,
+                          int use_fun, double amp, double period, double fun_msl, double fun_stage,
+                          const DevScalars *sc, hipStream_t s);
+// Kahan-compensated stage += frac * fS over the patch (a leapfrog step's stage forcing
+// applied outside the fused step)
+template <typename T>
+hipError_t launch_add_scaled_kahan(T *S, T *C, const T *fS, int pitch, int i0, int i1, int j0, int j1, T frac,
+                                   hipStream_t s);
 template <typename T>
 hipError_t launch_add_scaled(T *E, const T *fE, int pitch, int i0, int i1, int j0, int j1, T frac, hipStream_t s);
 

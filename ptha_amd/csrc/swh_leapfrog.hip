@@ -125,7 +125,7 @@ template <typename T, bool KAHAN>
 __device__ __forceinline__ void stage_new(const LFArgs<T> &a, int ii, int jj, T s, T c, T uh, T uhm, T vh, T vhm,
                                          T &sn, T &cn)
 {
-    const bool interior = ii > 0 && ii < a.nx - 1 && jj > 0 && jj < a.ny - 1;
+    const bool interior = ii > 0 && ii < a.nx - 1 && jj > 0 && jj < a.ny - 1 && !a.stage_given;
     if (interior) {
         const T delta = a.rc.sa[jj] * ((uh - uhm) * a.rc.dle[ii] + (vh * a.rc.dbe[jj + 1] - vhm * a.rc.dbe[jj]));
         if (KAHAN) {
@@ -316,8 +316,10 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
             vh_out[c] = vh_new;
         }
         const long long o = (long long)j * a.pitch + i;
-        vstore<T, VEC, NT>(So + o, sn, nvalid);
-        if (KAHAN) vstore<T, VEC, NT>(Co + o, cn, nvalid);
+        if (!a.stage_given) {
+            vstore<T, VEC, NT>(So + o, sn, nvalid);
+            if (KAHAN) vstore<T, VEC, NT>(Co + o, cn, nvalid);
+        }
         vstore<T, VEC, NT>(UHo + o, uh_out, nvalid);
         vstore<T, VEC, NT>(VHo + o, vh_out, nvalid);
 
@@ -344,17 +346,18 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
                     }
                 }
                 if (m & SWH_MAXQ_MAX_SPEED) {
+                    // depth at the UH / VH points: half the sum over the cell and its
+                    // east / north neighbour, which on the east column / north row is
+                    // the cell alone (sum over i:min(i+1,nx), domain_mod.f90:2311-2320)
                     const bool rt = (i + c + 1) < a.nx;
                     const T sE = rt ? (KAHAN ? sn[c + 1] - cn[c + 1] : sn[c + 1]) : s0, eE = rt ? r.e[c + 1] : e0;
                     const T sN = has_up ? (KAHAN ? su[c] - cu[c] : su[c]) : s0, eN = has_up ? q.e[c] : e0;
-                    T dE, dN;
-                    if (TL) {
-                        dE = HALF * ((msl - e0) + (msl - eE));
-                        dN = HALF * ((msl - e0) + (msl - eN));
-                    } else {
-                        dE = HALF * ((s0 - e0) + (sE - eE));
-                        dN = HALF * ((s0 - e0) + (sN - eN));
-                    }
+                    const T h0 = TL ? (msl - e0) : (s0 - e0);
+                    T dE = h0, dN = h0;
+                    if (rt) dE = h0 + (TL ? (msl - eE) : (sE - eE));
+                    if (has_up) dN = h0 + (TL ? (msl - eN) : (sN - eN));
+                    dE = HALF * dE;
+                    dN = HALF * dN;
                     const T iE = dE > MIN ? T(1) / dE : T(0);
                     const T iN = dN > MIN ? T(1) / dN : T(0);
                     const T sp = sqrt(uhn * uhn * (iE * iE) + vhn * vhn * (iN * iN));
@@ -381,6 +384,34 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
         if (PF) q = p;
     }
 }
+
+// The stage half-step alone (for steps whose boundary must see the new stage before
+// the momentum half-step, e.g. the Flather boundary): stage_new on every cell.
+template <typename T, bool KAHAN>
+__global__ __launch_bounds__(256) void lf_stage_kernel(LFArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const long long o = (long long)j * a.pitch + i;
+    const T s = a.S[o], c = KAHAN ? a.C[o] : T(0);
+    const T uh = a.UH[o], uhm = i > 0 ? a.UH[o - 1] : uh;
+    const T vh = a.VH[o], vhm = j > 0 ? a.VH[o - a.pitch] : vh;
+    T sn, cn;
+    stage_new<T, KAHAN>(a, i, j, s, c, uh, uhm, vh, vhm, sn, cn);
+    a.So[o] = sn;
+    if (KAHAN) a.Co[o] = cn;
+}
+
+template <typename T, bool KAHAN>
+hipError_t launch_lf_stage(const LFArgs<T> &a, hipStream_t s)
+{
+    if (a.stage_given || a.src_stride) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((lf_stage_kernel<T, KAHAN>), dim3((a.nx + 255) / 256, a.ny), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template hipError_t launch_lf_stage<double, false>(const LFArgs<double> &, hipStream_t);
+template hipError_t launch_lf_stage<float, true>(const LFArgs<float> &, hipStream_t);
 
 template <typename T, int VEC, bool PF, bool NT, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 static hipError_t launch_lf(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)

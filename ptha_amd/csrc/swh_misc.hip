@@ -167,6 +167,136 @@ hipError_t launch_add_scaled(T *E, const T *fE, int pitch, int i0, int i1, int j
     return hipGetLastError();
 }
 
+template <typename T>
+__global__ void add_scaled_kahan_kernel(T *S, T *C, const T *fS, int pitch, int i0, int i1, int j0, int j1, T frac)
+{
+    const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + blockIdx.y;
+    if (i > i1 || j > j1) return;
+    const long long o = (long long)j * pitch + i;
+    const T inc = frac * fS[o];
+    if (C) {
+        // the same compensated add as the fused step's stage forcing
+        const T y = inc - C[o];
+        const T t = S[o] + y;
+        C[o] = (t - S[o]) - y;
+        S[o] = t;
+    } else {
+        S[o] = S[o] + inc;
+    }
+}
+
+template <typename T>
+hipError_t launch_add_scaled_kahan(T *S, T *C, const T *fS, int pitch, int i0, int i1, int j0, int j1, T frac,
+                                   hipStream_t s)
+{
+    if (i1 < i0 || j1 < j0) return hipSuccess;
+    hipLaunchKernelGGL(add_scaled_kahan_kernel<T>, dim3((i1 - i0 + 256) / 256, j1 - j0 + 1), dim3(256), 0, s, S, C,
+                       fS, pitch, i0, i1, j0, j1, frac);
+    return hipGetLastError();
+}
+template hipError_t launch_add_scaled_kahan<double>(double *, double *, const double *, int, int, int, int, int,
+                                                    double, hipStream_t);
+template hipError_t launch_add_scaled_kahan<float>(float *, float *, const float *, int, int, int, int, int, float,
+                                                   hipStream_t);
+
+// ---- Flather boundary (boundary_mod.f90:502-720) ----
+// One edge cell b with the cell n inside it. West and south edges (sgn < 0) take the
+// incoming characteristic from outside as w1 = vel + sqrt(g/h) stage, east and north
+// (sgn > 0) as vel - sqrt(g/h) stage; normal / tang: the momentum planes across and
+// along the edge. Arithmetic in FP64 in both precisions, in the reference's order.
+template <typename T>
+__device__ void flather_cell(T *S, T *C, T *Qn, T *Qt, const T *E, long long b, long long n, bool neg,
+                             double msl_linear, int use_fun, double stage_fun)
+{
+    const double kMin = kMinDepth, kG = kGravity;
+    const double eb = (double)E[b];
+    double stage_o, vel_o, depth_o;
+    if (!use_fun) {
+        stage_o = msl_linear > eb ? msl_linear : eb;  // max(msl_linear, elevation)
+        vel_o = 0.0;
+        depth_o = stage_o - eb;
+    } else {
+        stage_o = stage_fun;
+        depth_o = stage_fun - eb;
+        const double q_o = 0.0;  // the boundary function's momenta
+        vel_o = depth_o <= kMin ? 0.0 : q_o / depth_o;
+    }
+    const double sn = C ? (double)S[n] - (double)C[n] : (double)S[n];
+    const double depth_inside = sn - (double)E[n];
+    double st, qn, qt;
+    if ((depth_inside > kMin) && (depth_o > kMin)) {
+        const double dinv = 1.0 / depth_inside;
+        const double sg = sqrt(kG * dinv);
+        const double qin = (double)Qn[n];
+        double w1, w2, w3;
+        if (neg) {
+            w1 = vel_o + sg * stage_o;
+            w2 = (qin < 0.0) ? (double)Qt[n] * dinv : 0.0;
+            w3 = qin * dinv - sg * sn;
+            st = (-w3 + w1) / (2.0 * sg);
+        } else {
+            w1 = vel_o - sg * stage_o;
+            w2 = (qin > 0.0) ? (double)Qt[n] * dinv : 0.0;
+            w3 = qin * dinv + sg * sn;
+            st = (w3 - w1) / (2.0 * sg);
+        }
+        qn = (w3 + w1) * 0.5 * depth_inside;
+        qt = w2 * depth_inside;
+    } else {
+        st = stage_o;
+        qn = 0.0;
+        qt = 0.0;
+    }
+    S[b] = (T)st;
+    if (C) C[b] = T(0);
+    Qn[b] = (T)qn;
+    Qt[b] = (T)qt;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void flather_kernel(T *S, T *C, T *UH, T *VH, const T *E, int nx, int ny,
+                                                       int pitch, int m, double msl_linear, int use_fun, double amp,
+                                                       double period, double fun_msl, double fun_stage,
+                                                       const DevScalars *sc)
+{
+    double stage_fun = fun_stage;
+    if (use_fun && sc) {
+        const double two_pi = 2.0 * acos(-1.0);
+        stage_fun = fun_msl + amp * sin(two_pi * sc->time / period);
+    }
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        const long long o = (long long)j * pitch;
+        if (m & 8) flather_cell<T>(S, C, UH, VH, E, o, o + 1, true, msl_linear, use_fun, stage_fun);
+        if (m & 2) flather_cell<T>(S, C, UH, VH, E, o + nx - 1, o + nx - 2, false, msl_linear, use_fun, stage_fun);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        if (m & 4) flather_cell<T>(S, C, VH, UH, E, i, (long long)pitch + i, true, msl_linear, use_fun, stage_fun);
+        if (m & 1) {
+            const long long o = (long long)(ny - 1) * pitch + i;
+            flather_cell<T>(S, C, VH, UH, E, o, o - pitch, false, msl_linear, use_fun, stage_fun);
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_flather(T *S, T *C, T *UH, T *VH, const T *E, int nx, int ny, int pitch, int mask, double msl_linear,
+                          int use_fun, double amp, double period, double fun_msl, double fun_stage,
+                          const DevScalars *sc, hipStream_t s)
+{
+    if (nx < 3 || ny < 3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(flather_kernel<T>, dim3(1), dim3(1024), 0, s, S, C, UH, VH, E, nx, ny, pitch, mask, msl_linear,
+                       use_fun, amp, period, fun_msl, fun_stage, sc);
+    return hipGetLastError();
+}
+template hipError_t launch_flather<double>(double *, double *, double *, double *, const double *, int, int, int, int,
+                                           double, int, double, double, double, double, const DevScalars *,
+                                           hipStream_t);
+template hipError_t launch_flather<float>(float *, float *, float *, float *, const float *, int, int, int, int,
+                                          double, int, double, double, double, double, const DevScalars *,
+                                          hipStream_t);
+
 template hipError_t launch_fv_forcing<double>(double *, double *, double *, const double *, const double *,
                                               const double *, int, int, int, int, int, double, double,
                                               const DevScalars *, hipStream_t);
@@ -204,17 +334,16 @@ __global__ void maxq_kernel(const T *__restrict__ S, const T *__restrict__ C, co
     if (m & SWH_MAXQ_MAX_SPEED) {
         T sp;
         if (staggered) {
-            const long long oE = (long long)j * pitch + min(i + 1, nx - 1);
-            const long long oN = (long long)min(j + 1, ny - 1) * pitch + i;
-            T dE, dN;
-            if (truely_linear) {
-                dE = HALF * ((msl - e) + (msl - E[oE]));
-                dN = HALF * ((msl - e) + (msl - E[oN]));
-            } else {
-                const T sE = C ? S[oE] - C[oE] : S[oE], sN = C ? S[oN] - C[oN] : S[oN];
-                dE = HALF * ((s - e) + (sE - E[oE]));
-                dN = HALF * ((s - e) + (sN - E[oN]));
-            }
+            // half the depth summed over the cell and its east / north neighbour; on the
+            // east column / north row the sum is the cell alone (domain_mod.f90:2311-2320)
+            const long long oE = (long long)j * pitch + i + 1;
+            const long long oN = (long long)(j + 1) * pitch + i;
+            const T h0 = truely_linear ? (msl - e) : (s - e);
+            T dE = h0, dN = h0;
+            if (i + 1 < nx) dE = h0 + (truely_linear ? (msl - E[oE]) : ((C ? S[oE] - C[oE] : S[oE]) - E[oE]));
+            if (j + 1 < ny) dN = h0 + (truely_linear ? (msl - E[oN]) : ((C ? S[oN] - C[oN] : S[oN]) - E[oN]));
+            dE = HALF * dE;
+            dN = HALF * dN;
             const T iE = dE > MIN ? T(1) / dE : T(0);
             const T iN = dN > MIN ? T(1) / dN : T(0);
             sp = sqrt(uh * uh * (iE * iE) + vh * vh * (iN * iN));

@@ -178,6 +178,15 @@ class Domain:
         B.check(self._lib.swh_set_stage_boundary_sine(self._h, float(amp), float(period), float(msl), s),
                 "swh_set_stage_boundary_sine")
 
+    def set_flather_boundary(self, sides=(1, 1, 1, 1), amp=None, period=1.0, msl=0.0):
+        """domain%boundary_subroutine => flather_boundary (boundary_mod.f90:502-720) on
+        sides [N, E, S, W]. amp None: no boundary_function (outside sea at rest at
+        msl_linear); else the outside stage msl + amp*sin(2*pi*t/period), zero momenta."""
+        s = (B._I32 * 4)(*[int(bool(v)) for v in sides])
+        use_fun = amp is not None
+        B.check(self._lib.swh_set_flather_boundary(self._h, s, int(use_fun), float(amp or 0.0), float(period),
+                                                   float(msl)), "swh_set_flather_boundary")
+
     # -- stepping --
     def evolve_one_step(self, timestep=None):
         B.check(self._lib.swh_evolve_one_step(self._h, -1.0 if timestep is None else float(timestep)),

@@ -72,7 +72,8 @@ module swals_hip_mod
         swh_md_evolve_steps, swh_md_volumes, swh_md_synchronize
 
     public :: swh_last_error, swh_domain_create, swh_domain_destroy, swh_upload_U, swh_download_U, &
-        swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_evolve_one_step, &
+        swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_set_flather_boundary, &
+        swh_set_forcing_patch, swh_clear_forcing_patch, swh_evolve_one_step, &
         swh_evolve_steps, swh_get_scalars, swh_volume_interior, swh_stationary_timestep_max, &
         swh_download_max_U, swh_gauges_set, swh_gauges_sample, swh_gauges_download, swh_synchronize, &
         swh_domain_set_launch, swh_check
@@ -118,6 +119,29 @@ module swals_hip_mod
             type(c_ptr), value :: h
             real(c_double), value :: amp, period, msl
             integer(c_int32_t), intent(in) :: sides(4)
+        end function
+        ! forcing_patch_type with apply_forcing_patch (forcing_mod.f90:24-56, 80-156); the
+        ! patch's work array as the reference holds it, work(i0:i1, j0:j1, k0:k1)
+        integer(c_int) function swh_set_forcing_patch(h, start_time, end_time, i0, i1, j0, j1, k0, k1, work) &
+                bind(C, name='swh_set_forcing_patch')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            real(c_double), value :: start_time, end_time
+            integer(c_int32_t), value :: i0, i1, j0, j1, k0, k1
+            real(c_double), intent(in) :: work(*)
+        end function
+        integer(c_int) function swh_clear_forcing_patch(h) bind(C, name='swh_clear_forcing_patch')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        ! flather_boundary (boundary_mod.f90:502-720) on sides [N, E, S, W]
+        integer(c_int) function swh_set_flather_boundary(h, sides, use_stage_function, amp, period, msl) &
+                bind(C, name='swh_set_flather_boundary')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(in) :: sides(4)
+            integer(c_int32_t), value :: use_stage_function
+            real(c_double), value :: amp, period, msl
         end function
         integer(c_int) function swh_evolve_one_step(h, timestep) bind(C, name='swh_evolve_one_step')
             import c_int, c_ptr, c_double

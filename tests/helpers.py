@@ -23,6 +23,30 @@ def small_beach(nx=80, ny=40, dx=10.0):
     return C.beach_rk2(nx=nx, ny=ny, dx=dx)
 
 
+def open_basin(nx=90, ny=80, arcmin=10.0, method="linear", depth=3000.0):
+    """Spherical basin with no land and no walls, a 2 m hump 300 km from the west edge
+    and a northward current near the north edge, so waves and flow cross every side
+    within tens of steps (a test bed for open boundaries)."""
+    cfg = small_spherical(nx=nx, ny=ny, arcmin=arcmin)
+    x = np.linspace(0.0, 1.0, nx)[None, :]
+    y = np.linspace(0.0, 1.0, ny)[:, None]
+    U = np.zeros((4, ny, nx))
+    U[3] = -depth + 0.2 * depth * np.sin(3.0 * x + 1.0) * np.cos(2.0 * y)
+    U[0] = 2.0 * np.exp(-((x - 0.2) ** 2 + (y - 0.6) ** 2) / 0.02)
+    U[2] = 50.0 * np.exp(-((x - 0.7) ** 2) / 0.05) * np.exp(-((y - 0.95) ** 2) / 0.01)
+    U[1] = -30.0 * np.exp(-((y - 0.3) ** 2) / 0.05) * np.exp(-((x - 0.97) ** 2) / 0.01)
+    return dict(cfg, U=U, method=method)
+
+
+def with_flather(cfg, sides=(1, 1, 1, 1), amp=None, period=600.0):
+    """cfg with a Flather boundary (boundary_mod.f90:502-720) instead of any stage
+    boundary: outside sea at rest, or the sine stage of amplitude amp."""
+    cfg = dict(cfg)
+    cfg.pop("stage_bc", None)
+    cfg["flather"] = dict(sides=sides, amp=amp, period=period)
+    return cfg
+
+
 # finite-volume methods: without a timestep argument they take the CFL step
 FV_METHODS = ("rk2", "rk2n", "midpoint", "euler")
 
@@ -37,6 +61,8 @@ def cpu_run(kind, cfg, nsteps, dt=None, record=None, **kw):
     if "stage_bc" in cfg:
         b = cfg["stage_bc"]
         d.set_stage_bc(b["amp"], b["period"], 0.0, b["sides"])
+    if "flather" in cfg:
+        d.set_flather_bc(**cfg["flather"])
     if dt is None and cfg["method"] not in FV_METHODS:
         dt = d.stationary_timestep_max()
     hist = []
@@ -58,6 +84,8 @@ def gpu_domain(cfg, precision="fp64", max_U_variables=("max_stage", "max_speed",
     if "stage_bc" in cfg:
         b = cfg["stage_bc"]
         g.set_stage_boundary_sine(b["amp"], b["period"], 0.0, b["sides"])
+    if "flather" in cfg:
+        g.set_flather_boundary(**cfg["flather"])
     return g
 
 

@@ -51,6 +51,21 @@ for _m in ("euler", "midpoint", "rk2n"):
     CASES[_m + "_sph"] = ((lambda m: lambda: dict(_rk2_sph(), method=m))(_m), {}, 20)
 
 
+# Flather boundaries (boundary_mod.f90:502-720): sea at rest outside, or the sine stage
+# outside (a boundary function), on all or some sides
+CASES["linear_sph_flather"] = (lambda: H.with_flather(H.open_basin()), {}, 60)
+CASES["linear_sph_flather_fun"] = (lambda: H.with_flather(H.open_basin(), sides=(1, 0, 0, 1), amp=0.3), {}, 60)
+CASES["linear_cart_flather"] = (lambda: H.with_flather(dict(H.open_basin(), spherical=False)), {}, 60)
+CASES["llpnf_sph_flather"] = (lambda: H.with_flather(dict(H.open_basin(method="leapfrog_linear_plus_nonlinear_friction"),
+                                                          manning_squared=np.full((80, 90), 0.03 ** 2))), {}, 60)
+CASES["rk2_beach_flather_fun"] = (lambda: H.with_flather(H.small_beach(), amp=1.0), {}, 60)
+CASES["rk2_sph_flather"] = (lambda: H.with_flather(H.open_basin(method="rk2")), {}, 30)
+CASES["midpoint_sph_flather_fun"] = (lambda: H.with_flather(H.open_basin(method="midpoint"), sides=(1, 1, 1, 1),
+                                                            amp=0.5), {}, 30)
+CASES["euler_beach_flather"] = (lambda: H.with_flather(dict(H.small_beach(), method="euler"), sides=(0, 0, 0, 1),
+                                                      amp=1.0), {}, 40)
+
+
 @pytest.mark.parametrize("method", ["euler", "midpoint", "rk2n", "rk2"])
 def test_fv_fixed_timestep_equals_compiled_reference(oracle_built, method):
     """The finite-volume schemes with a timestep argument (60 % of the stationary CFL
