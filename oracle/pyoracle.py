@@ -18,7 +18,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 
-METHODS = {"rk2": 1, "rk2n": 2, "midpoint": 3, "euler": 4, "linear": 5, "leapfrog_linear_plus_nonlinear_friction": 6}
+METHODS = {"rk2": 1, "rk2n": 2, "midpoint": 3, "euler": 4, "linear": 5, "leapfrog_linear_plus_nonlinear_friction": 6,
+           "leapfrog_nonlinear": 8}
 MAXQ = ["max_stage", "max_speed", "max_flux", "arrival_time", "time_of_max_stage", "min_stage"]
 
 _D = ctypes.c_double

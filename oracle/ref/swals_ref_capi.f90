@@ -45,6 +45,7 @@ contains
         case(4); name = 'euler'
         case(5); name = 'linear'
         case(6); name = 'leapfrog_linear_plus_nonlinear_friction'
+        case(8); name = 'leapfrog_nonlinear'
         case default; name = 'unknown'
         end select
     end function

@@ -276,7 +276,7 @@ static void update_boundary(or_domain *d)
         return;
     }
     const int nx = d->nx, ny = d->ny;
-    const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF);
+    const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF || d->method == OR_LEAPFROG_NONLINEAR);
     const double two_pi = 2.0 * acos(-1.0);
     double stage = d->bc_msl + d->bc_amp * sin(two_pi * d->time / d->bc_period);
     for (int j = 1; j <= ny; j++) {
@@ -318,7 +318,7 @@ void or_domain_set_flather_bc(or_domain *d, int use_fun, double amp, double peri
 }
 
 /* domain_mod_leapfrog_solver_friction_include.f90:21-85 (precompute_friction_work) */
-static void precompute_friction_work(or_domain *d)
+static void precompute_friction_work(or_domain *d, int truely_linear)
 {
     const int nx = d->nx, ny = d->ny;
     const double power = (d->friction_type == 1) ? -2.0 : -7.0 / 3.0;
@@ -328,7 +328,7 @@ static void precompute_friction_work(or_domain *d)
         for (int i = 1; i <= nx; i++) {
             int ip1 = i + 1 < nx ? i + 1 : nx, jp1 = j + 1 < ny ? j + 1 : ny;
             double depth_iph, depth_jph;
-            if (d->truely_linear) {
+            if (truely_linear) {
                 depth_iph = 0.5 * (msl - A(d, ELV, i, j) + msl - A(d, ELV, ip1, j));
                 depth_jph = 0.5 * (msl - A(d, ELV, i, j) + msl - A(d, ELV, i, jp1));
             } else {
@@ -343,20 +343,15 @@ static void precompute_friction_work(or_domain *d)
             fV[IX(d, i, j)] = GRAVITY * nsq_jph * pow(depth_jph, power);
         }
     }
-    if (d->truely_linear) d->friction_work_is_setup = 1;
+    if (truely_linear) d->friction_work_is_setup = 1;
 }
 
 /* Staggered-grid leapfrog step: domain_mod_linear_solver_include.f90:59-571, with the
  * four variants of domain_mod_timestepping_alternatives_include.f90:339-434. */
-static void leapfrog_step(or_domain *d, double dt)
+static void lf_stage_boundary(or_domain *d, double dt)
 {
     const int nx = d->nx, ny = d->ny;
     const int xL = d->xL, xU = d->xU, yL = d->yL, yU = d->yU;
-    const int friction = (d->method == OR_LLPNF);
-    const int cor_or_fric = d->coriolis_on || friction;
-    const double msl = d->msl_linear;
-    d->dt_last_update = dt;
-
     /* Stage half-step: include.f90:92-113 */
     for (int j = yL + 1; j <= yU - 1; j++) {
         double inv_cell_area_dt = dt / d->area[j - 1];
@@ -385,8 +380,19 @@ static void leapfrog_step(or_domain *d, double dt)
             (((d->boundary_flux_store[0] + d->boundary_flux_store[1]) + d->boundary_flux_store[2]) +
              d->boundary_flux_store[3]) * dt;
     }
+}
 
-    if (friction && !d->friction_work_is_setup) precompute_friction_work(d);
+static void leapfrog_step(or_domain *d, double dt)
+{
+    const int nx = d->nx, ny = d->ny;
+    const int xL = d->xL, xU = d->xU, yL = d->yL, yU = d->yU;
+    const int friction = (d->method == OR_LLPNF);
+    const int cor_or_fric = d->coriolis_on || friction;
+    const double msl = d->msl_linear;
+    d->dt_last_update = dt;
+    lf_stage_boundary(d, dt);
+
+    if (friction && !d->friction_work_is_setup) precompute_friction_work(d, d->truely_linear);
     const double *fwU = d->friction_work, *fwV = d->friction_work + (size_t)nx * ny;
 
     /* Momentum half-step: include.f90:211-571, serial loop order yL..yU-1. */
@@ -861,8 +867,9 @@ static void update_max_quantities(or_domain *d)
     const int nx = d->nx, ny = d->ny;
     const size_t n = (size_t)nx * ny;
     double *m = d->max_U;
-    const int staggered = (d->method == OR_LINEAR || d->method == OR_LLPNF);
-    const int truely = d->truely_linear && staggered;
+    const int staggered = (d->method == OR_LINEAR || d->method == OR_LLPNF || d->method == OR_LEAPFROG_NONLINEAR);
+    /* depth relative to msl only for the truely-linear 'linear' / llpnf solvers (:2300-2302) */
+    const int truely = d->truely_linear && (d->method == OR_LINEAR || d->method == OR_LLPNF);
     for (int j = d->yL; j <= d->yU; j++) {
         for (int i = d->xL; i <= d->xU; i++) {
             size_t k = IX(d, i, j);
@@ -902,6 +909,232 @@ static void update_max_quantities(or_domain *d)
     }
 }
 
+/* Nonlinear staggered leapfrog ('leapfrog_nonlinear'):
+ * domain_mod_leapfrog_nonlinear_include.f90:61-826 with FROUDE_LIMITING
+ * (domain_mod_timestepping_alternatives_include.f90:439-447), serial loop order.
+ * Upwind momentum advection from work arrays formed from the pre-update UH/VH and a
+ * time-lagged depth, wet/dry switches, Coriolis and the spherical metric terms
+ * (spherical builds), semi-implicit friction from friction_work recomputed every step
+ * from the new stage, a Froude limiter, and zero outflow from dry cells. */
+static void leapfrog_nonlinear_step(or_domain *d, double dt)
+{
+    const int nx = d->nx, ny = d->ny;
+    const size_t n = (size_t)nx * ny;
+    const double beta = 1.0;      /* advection_beta, global_mod.f90:38 */
+    const double fr_lim = 10.0;   /* leapfrog_froude_limit, domain_mod.f90:210 */
+    d->dt_last_update = dt;
+    double *wk = calloc(9 * n, sizeof(double));
+    double *DL = wk, *UUH = wk + n, *UVH = wk + 2 * n, *VUH = wk + 3 * n, *VVH = wk + 4 * n;
+    double *fEWu = wk + 5 * n, *fEWv = wk + 6 * n, *fNSu = wk + 7 * n, *fNSv = wk + 8 * n;
+#define W(a, i, j) ((a)[IX(d, i, j)])
+#define WET(i, j) (A(d, STG, i, j) > A(d, ELV, i, j) + MIN_DEPTH)
+    /* :63-71 depth before the stage update; work arrays and fluxes cleared */
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) W(DL, i, j) = dmax(A(d, STG, i, j) - A(d, ELV, i, j), 0.0);
+
+    lf_stage_boundary(d, dt);   /* :97-204 */
+    precompute_friction_work(d, 0);   /* :218, not truely linear: every step */
+
+    /* :265-268 time-lagged depth on rows 1..ny-1 (row ny keeps the old depth) */
+    for (int j = 1; j <= ny - 1; j++)
+        for (int i = 1; i <= nx; i++)
+            W(DL, i, j) = 0.5 * (W(DL, i, j) + dmax(A(d, STG, i, j) - A(d, ELV, i, j), 0.0));
+
+    /* :274-299 U*UH and U*VH at (i+1/2, j), extrapolated to the east column and row ny */
+    for (int j = 1; j <= ny - 1; j++) {
+        const int jm1 = j - 1 > 1 ? j - 1 : 1;
+        for (int i = 1; i <= nx - 1; i++) {
+            const double depth_local = 0.5 * (W(DL, i, j) + W(DL, i + 1, j));
+            const double inv = depth_local > MIN_DEPTH ? 1.0 / depth_local : 0.0;
+            W(UUH, i, j) = (A(d, UH, i, j) * A(d, UH, i, j)) * inv;
+            if (i < nx - 1)
+                W(UVH, i, j) = A(d, UH, i, j) * inv * 0.25 *
+                               (A(d, VH, i, jm1) + A(d, VH, i + 1, jm1) + A(d, VH, i, j) + A(d, VH, i + 1, j));
+        }
+        W(UUH, nx, j) = W(UUH, nx - 1, j);
+        W(UVH, nx, j) = W(UVH, nx - 2, j);
+        W(UVH, nx - 1, j) = W(UVH, nx - 2, j);
+    }
+    for (int i = 1; i <= nx; i++) {
+        W(UUH, i, ny) = W(UUH, i, ny - 1);
+        W(UVH, i, ny) = W(UVH, i, ny - 1);
+    }
+    /* :305-328 V*VH and V*UH at (i, j+1/2) */
+    for (int j = 1; j <= ny - 1; j++) {
+        for (int i = 1; i <= nx; i++) {
+            const double depth_local = 0.5 * (W(DL, i, j) + W(DL, i, j + 1));
+            const double inv = depth_local > MIN_DEPTH ? 1.0 / depth_local : 0.0;
+            W(VVH, i, j) = A(d, VH, i, j) * A(d, VH, i, j) * inv;
+            if (i > 1 && j < ny - 1)
+                W(VUH, i, j) = A(d, VH, i, j) * inv * 0.25 *
+                               (A(d, UH, i - 1, j) + A(d, UH, i, j) + A(d, UH, i - 1, j + 1) + A(d, UH, i, j + 1));
+        }
+        W(VUH, 1, j) = W(VUH, 2, j);
+    }
+    for (int i = 1; i <= nx; i++) {
+        W(VVH, i, ny) = W(VVH, i, ny - 1);
+        W(VUH, i, ny) = W(VUH, i, ny - 2);
+        W(VUH, i, ny - 1) = W(VUH, i, ny - 2);
+    }
+
+    /* :338-363 EW flux of UH (upwind; the mean where the neighbours cancel; dry: 0) */
+    for (int j = 1; j <= ny - 1; j++) {
+        for (int i = 2; i <= nx - 1; i++)
+            W(fEWu, i, j) = d->dle[i - 1] * ((A(d, UH, i, j) + A(d, UH, i - 1, j) > 0.0) ? W(UUH, i - 1, j) : W(UUH, i, j));
+        W(fEWu, 1, j) = W(UUH, 1, j) * d->dle[0];
+        W(fEWu, nx, j) = W(UUH, nx, j) * d->dle[nx - 1];
+        for (int i = 2; i <= nx - 1; i++)
+            if ((A(d, UH, i, j) + A(d, UH, i - 1, j)) == 0.0)
+                W(fEWu, i, j) = d->dle[i - 1] * 0.5 * (W(UUH, i - 1, j) + W(UUH, i, j));
+        for (int i = 1; i <= nx; i++)
+            if (!WET(i, j)) W(fEWu, i, j) = 0.0;
+    }
+    /* :366-409 NS flux of UH on the top edge of row j, then the bottom edge of row 1 */
+    for (int j = 0; j <= ny - 1; j++) {
+        const int jt = j == 0 ? 1 : j + 1;        /* the edge written */
+        const int ja = j == 0 ? 1 : j;            /* the row below it (row 1 for the bottom edge) */
+        const int jb = j == 0 ? 1 : j + 1;        /* the row above it */
+        const double dbe = d->dbe[jt - 1];
+        for (int i = 1; i <= nx - 1; i++) {
+            const double vs = A(d, VH, i, ja) + A(d, VH, i + 1, ja);
+            double f = dbe * (vs > 0.0 ? W(UVH, i, ja) : W(UVH, i, jb));
+            if (vs == 0.0) f = dbe * 0.5 * (W(UVH, i, ja) + W(UVH, i, jb));
+            if (!(WET(i, ja) && WET(i + 1, ja) && WET(i, jb) && WET(i + 1, jb))) f = 0.0;
+            W(fNSu, i, jt) = f;
+        }
+    }
+    /* :412-455 NS flux of VH on the bottom edge of rows 1..ny */
+    for (int j = 1; j <= ny; j++) {
+        const int jm1 = j - 1 > 1 ? j - 1 : 1;
+        const double dbe = 0.5 * (d->dbe[j - 1] + d->dbe[j]);
+        for (int i = 1; i <= nx - 1; i++) {
+            const double vs = A(d, VH, i, j) + A(d, VH, i, jm1);
+            double f = dbe * (vs > 0.0 ? W(VVH, i, jm1) : W(VVH, i, j));
+            if (vs == 0.0) f = dbe * 0.5 * (W(VVH, i, jm1) + W(VVH, i, j));
+            if (!WET(i, j)) f = 0.0;
+            W(fNSv, i, j) = f;
+        }
+    }
+    /* :457-481 EW flux of VH; the west column copies column 2 before its mean/dry fixes */
+    for (int j = 1; j <= ny - 1; j++) {
+        for (int i = 2; i <= nx; i++)
+            W(fEWv, i, j) = d->dle[i - 2] *
+                            ((A(d, UH, i - 1, j) + A(d, UH, i - 1, j + 1) > 0.0) ? W(VUH, i - 1, j) : W(VUH, i, j));
+        W(fEWv, 1, j) = W(fEWv, 2, j);
+        for (int i = 2; i <= nx; i++) {
+            if ((A(d, UH, i - 1, j) + A(d, UH, i - 1, j + 1)) == 0.0)
+                W(fEWv, i, j) = d->dle[i - 2] * 0.5 * (W(VUH, i - 1, j) + W(VUH, i, j));
+            if (!(WET(i - 1, j) && WET(i, j) && WET(i - 1, j + 1) && WET(i, j + 1))) W(fEWv, i, j) = 0.0;
+        }
+    }
+
+    /* :575-787 momentum update, rows 1..ny-1, columns 1..nx-1, from the pre-update
+     * UH/VH (the row-carried Coriolis averages are read from a copy) */
+    double *old = malloc(2 * n * sizeof(double));
+    memcpy(old, P(d, UH), 2 * n * sizeof(double));
+#define OUH(i, j) (old[IX(d, i, j)])
+#define OVH(i, j) (old[n + IX(d, i, j)])
+    const double *fwU = d->friction_work, *fwV = d->friction_work + n;
+    for (int j = 1; j <= ny - 1; j++) {
+        const double ica_dt = dt / d->area[j - 1];
+        const double ica_dt_g = GRAVITY * ica_dt;
+        const double ica_dt_vh = dt / (0.5 * (d->area[j - 1] + d->area[j]));
+        const double ica_dt_vh_g = GRAVITY * ica_dt_vh;
+        const int jr = j > 1 ? j - 1 : 1;
+        for (int i = 1; i <= nx - 1; i++) {
+            const double dw_j = A(d, STG, i, j + 1) - A(d, STG, i, j);
+            double h_jph = 0.5 * ((A(d, STG, i, j + 1) + A(d, STG, i, j)) - (A(d, ELV, i, j + 1) + A(d, ELV, i, j)));
+            if (h_jph < MIN_DEPTH) h_jph = 0.0;
+            double h_iph = 0.5 * ((A(d, STG, i + 1, j) + A(d, STG, i, j)) - (A(d, ELV, i + 1, j) + A(d, ELV, i, j)));
+            if (h_iph < MIN_DEPTH) h_iph = 0.0;
+            const int jph_strict = WET(i, j + 1) && WET(i, j);
+            const double jph_wet = (jph_strict ||
+                                    (WET(i, j + 1) && A(d, STG, i, j + 1) > A(d, ELV, i, j) + MIN_DEPTH) ||
+                                    (WET(i, j) && A(d, STG, i, j) > A(d, ELV, i, j + 1) + MIN_DEPTH)) ? 1.0 : 0.0;
+            const int iph_strict = WET(i + 1, j) && WET(i, j);
+            const double iph_wet = (iph_strict ||
+                                    (WET(i + 1, j) && A(d, STG, i + 1, j) > A(d, ELV, i, j) + MIN_DEPTH) ||
+                                    (WET(i, j) && A(d, STG, i, j) > A(d, ELV, i + 1, j) + MIN_DEPTH)) ? 1.0 : 0.0;
+            const double vh_iph_jmh = 0.5 * (OVH(i, jr) + OVH(i + 1, jr));
+            const double vh_iph_jph = 0.5 * (OVH(i, j) + OVH(i + 1, j));
+            const int im1 = i > 1 ? i - 1 : 1;
+            const double uh_i_j = 0.5 * (OUH(i, j) + OUH(im1, j));
+            const double uh_i_jp1 = 0.5 * (OUH(i, j + 1) + OUH(im1, j + 1));
+            double dt_half_cor = 0.0, dt_half_cor_jph = 0.0, t_uvh = 0.0, t_uuh = 0.0;
+            if (d->coriolis_on) {
+                dt_half_cor = dt * d->cor[j - 1] * 0.5 * (iph_strict ? 1.0 : 0.0);
+                dt_half_cor_jph = dt * d->cor_be[j] * 0.5 * (jph_strict ? 1.0 : 0.0);
+            }
+            if (d->spherical) {
+                double inv = 0.0;
+                if (iph_strict) inv = 1.0 / h_iph;
+                t_uvh = dt * d->tanlat[j - 1] * OUH(i, j) * 0.5 * (vh_iph_jmh + vh_iph_jph) * inv;
+                inv = 0.0;
+                if (jph_strict) inv = 1.0 / h_jph;
+                t_uuh = dt * 0.5 * (d->tanlat[j - 1] + d->tanlat[j]) * 0.5 * (uh_i_jp1 + uh_i_j) * 0.5 *
+                        (uh_i_jp1 + uh_i_j) * inv;
+            }
+            const double vbar = 0.5 * (vh_iph_jmh + vh_iph_jph);
+            const double ubar = 0.5 * (uh_i_j + uh_i_jp1);
+            const double fm_UH = 1.0 / (1.0 + dt * d->linear_friction_coeff +
+                                        dt * fwU[IX(d, i, j)] * sqrt(OUH(i, j) * OUH(i, j) + vbar * vbar));
+            const double fm_VH = 1.0 / (1.0 + dt * d->linear_friction_coeff +
+                                        dt * fwV[IX(d, i, j)] * sqrt(OVH(i, j) * OVH(i, j) + ubar * ubar));
+            double uh = OUH(i, j) * iph_wet -
+                        ica_dt_g * h_iph * iph_wet * (A(d, STG, i + 1, j) - A(d, STG, i, j)) * d->dle[i];
+            double vh = OVH(i, j) * jph_wet - ica_dt_vh_g * h_jph * jph_wet * dw_j * d->dbe[j];
+            uh = uh - ica_dt * beta * iph_wet *
+                          ((W(fEWu, i + 1, j) - W(fEWu, i, j)) + (W(fNSu, i, j + 1) - W(fNSu, i, j)));
+            vh = vh - ica_dt_vh * beta * jph_wet *
+                          ((W(fEWv, i + 1, j) - W(fEWv, i, j)) + (W(fNSv, i, j + 1) - W(fNSv, i, j)));
+            if (d->coriolis_on) {
+                uh = uh + dt_half_cor * (vh_iph_jmh + vh_iph_jph);
+                vh = vh - dt_half_cor_jph * (uh_i_j + uh_i_jp1);
+            }
+            if (d->spherical) {
+                uh = uh + t_uvh;
+                vh = vh - t_uuh;
+            }
+            uh = fm_UH * uh;
+            vh = fm_VH * vh;
+            A(d, UH, i, j) = uh;
+            A(d, VH, i, j) = vh;
+        }
+        /* :763-783 Froude limiter on this row */
+        for (int i = 1; i <= nx - 1; i++) {
+            double dl = 0.5 * (A(d, STG, i, j) - A(d, ELV, i, j) + A(d, STG, i + 1, j) - A(d, ELV, i + 1, j));
+            if (dl < MIN_DEPTH) dl = 0.0;
+            const double lim = fr_lim * sqrt(dl * GRAVITY) * dl * copysign(1.0, A(d, UH, i, j));
+            if (A(d, UH, i, j) * A(d, UH, i, j) > lim * lim) A(d, UH, i, j) = lim;
+        }
+        for (int i = 1; i <= nx - 1; i++) {
+            double dl = 0.5 * (A(d, STG, i, j) - A(d, ELV, i, j) + A(d, STG, i, j + 1) - A(d, ELV, i, j + 1));
+            if (dl < MIN_DEPTH) dl = 0.0;
+            const double lim = fr_lim * sqrt(dl * GRAVITY) * dl * copysign(1.0, A(d, VH, i, j));
+            if (A(d, VH, i, j) * A(d, VH, i, j) > lim * lim) A(d, VH, i, j) = lim;
+        }
+    }
+#undef OUH
+#undef OVH
+    free(old);
+    /* :792-815 no outflow from dry cells */
+    for (int j = 1; j <= ny - 1; j++) {
+        for (int i = 1; i <= nx; i++) {
+            if (A(d, STG, i, j) < A(d, ELV, i, j) + MIN_DEPTH) {
+                if (A(d, UH, i, j) > 0.0) A(d, UH, i, j) = 0.0;
+                if (i > 1 && A(d, UH, i - 1, j) < 0.0) A(d, UH, i - 1, j) = 0.0;
+                if (A(d, VH, i, j) > 0.0) A(d, VH, i, j) = 0.0;
+            }
+            if (A(d, STG, i, j + 1) < A(d, ELV, i, j + 1) + MIN_DEPTH)
+                if (A(d, VH, i, j) < 0.0) A(d, VH, i, j) = 0.0;
+        }
+    }
+#undef W
+#undef WET
+    d->time = d->time + 0.5 * dt;
+    free(wk);
+}
+
 /* evolve_one_step: domain_mod.f90:1679-1813 (max_U_update_frequency = 1) */
 static void evolve_one_step(or_domain *d, double dt)
 {
@@ -915,6 +1148,7 @@ static void evolve_one_step(or_domain *d, double dt)
     case OR_MIDPOINT: midpoint_step(d, dt); break;
     case OR_LINEAR:
     case OR_LLPNF: leapfrog_step(d, dt); break;
+    case OR_LEAPFROG_NONLINEAR: leapfrog_nonlinear_step(d, dt); break;
     default: break;
     }
     update_max_quantities(d);

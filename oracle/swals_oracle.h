@@ -19,7 +19,7 @@ extern "C" {
 
 /* timestepping_method ids: index into the reference's timestepping_metadata table
  * (timestepping_metadata_mod.f90:69-140). */
-enum { OR_RK2 = 1, OR_RK2N = 2, OR_MIDPOINT = 3, OR_EULER = 4, OR_LINEAR = 5, OR_LLPNF = 6 };
+enum { OR_RK2 = 1, OR_RK2N = 2, OR_MIDPOINT = 3, OR_EULER = 4, OR_LINEAR = 5, OR_LLPNF = 6, OR_LEAPFROG_NONLINEAR = 8 };
 
 typedef struct or_domain or_domain;
 

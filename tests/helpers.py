@@ -38,6 +38,34 @@ def open_basin(nx=90, ny=80, arcmin=10.0, method="linear", depth=3000.0):
     return dict(cfg, U=U, method=method)
 
 
+def nonlinear_leapfrog_cases():
+    """Configurations for 'leapfrog_nonlinear' (the nonlinear staggered leapfrog):
+    an all-wet spherical basin with currents, a spherical grid with land (wet/dry), the
+    Cartesian beach (shore, stage boundary), a dam break onto a dry slope (its wetting
+    front engages the Froude limiter), the basin with Flather boundaries.
+    Each entry: (make config, steps, domain keywords)."""
+    def manning(cfg, n=0.025):
+        return dict(cfg, method="leapfrog_nonlinear", manning_squared=np.full((cfg["ny"], cfg["nx"]), n ** 2))
+
+    def dambreak():
+        # a 2 m column released onto a dry bed sloping gently up to the east: the thin
+        # wetting front engages the Froude limiter (results change without it)
+        cfg = C.gaussian_hump_cartesian(nx=96, ny=24, dx=1.0, sigma=10.0)
+        x = np.arange(96)[None, :] * np.ones((24, 1))
+        U = np.zeros((4, 24, 96))
+        U[3] = 0.01 * (x - 30.0)
+        U[0] = np.where(x < 30, 2.0, U[3])
+        return dict(cfg, U=U)
+
+    return {
+        "basin_sph": (lambda: manning(open_basin()), 60, {}),
+        "land_sph": (lambda: manning(small_spherical()), 60, {}),
+        "beach_cart": (lambda: manning(small_beach()), 80, {}),
+        "dambreak_cart": (lambda: manning(dambreak(), 0.02), 120, {"cfl": 0.2}),
+        "basin_sph_flather": (lambda: with_flather(manning(open_basin())), 60, {}),
+    }
+
+
 def with_flather(cfg, sides=(1, 1, 1, 1), amp=None, period=600.0):
     """cfg with a Flather boundary (boundary_mod.f90:502-720) instead of any stage
     boundary: outside sea at rest, or the sine stage of amplitude amp."""

@@ -66,6 +66,11 @@ CASES["euler_beach_flather"] = (lambda: H.with_flather(dict(H.small_beach(), met
                                                       amp=1.0), {}, 40)
 
 
+# 'leapfrog_nonlinear' (domain_mod_leapfrog_nonlinear_include.f90)
+for _k, (_make, _n, _kw) in H.nonlinear_leapfrog_cases().items():
+    CASES["lfnl_" + _k] = (_make, _kw, _n)
+
+
 @pytest.mark.parametrize("method", ["euler", "midpoint", "rk2n", "rk2"])
 def test_fv_fixed_timestep_equals_compiled_reference(oracle_built, method):
     """The finite-volume schemes with a timestep argument (60 % of the stationary CFL
