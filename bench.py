@@ -155,7 +155,8 @@ def copy_probe(mib):
 def secondary_legs(steps):
     """Throughput of the other single-grid paths (not the headline workload):
     rk2 on the 2000 x 1000 wet/dry beach (FP64 and FP32 state) and one 2048^2
-    spherical 'linear' grid (the adjoint-inversion grid) in FP32 + Kahan and FP64.
+    spherical 'linear' grid (the adjoint-inversion grid) in FP32 + Kahan and FP64, and
+    'leapfrog_nonlinear' on that grid.
     Timed with host wall clock around synchronised step loops (inputs resident)."""
     from ptha_amd import configs as C
     from ptha_amd.domain import Domain
@@ -195,6 +196,26 @@ def secondary_legs(steps):
         g = ca["nx"] * ca["ny"] * steps / wall / 1e9
         out["linear_2048sq_" + prec] = dict(gcell_per_s=round(g, 3), ms_per_step=round(1e3 * wall / steps, 4),
                                             hbm_frac=round(g * bpc / HBM_PEAK_GBS, 4))
+        d.close()
+    # the nonlinear leapfrog on the same grid (Manning 0.025, a 1 m hump), stationary CFL step
+    X, Y = np.meshgrid(ca["x"], ca["y"])
+    for prec in ("fp64", "fp32k"):
+        d = Domain(ca["nx"], ca["ny"], ca["lw"], ca["ll"], "leapfrog_nonlinear", precision=prec, spherical=True,
+                   max_U_variables=())
+        U = np.zeros((4, ca["ny"], ca["nx"]))
+        U[3] = ca["elev"]
+        U[0] = np.maximum(np.exp(-((X - ca["src_lon"].mean()) ** 2 + (Y - ca["src_lat"].mean()) ** 2) / 0.5), U[3])
+        d.U = U
+        d.set_manning_squared(np.full((ca["ny"], ca["nx"]), 0.025 ** 2))
+        dtn = d.stationary_timestep_max()
+        d.evolve_steps(dtn, 5)
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.evolve_steps(dtn, steps)
+        d.synchronize()
+        wall = time.perf_counter() - t0
+        out["nonlinear_2048sq_" + prec] = dict(gcell_per_s=round(ca["nx"] * ca["ny"] * steps / wall / 1e9, 3),
+                                               ms_per_step=round(1e3 * wall / steps, 4))
         d.close()
     # adjoint source inversion grid: forward (gauges every 17 steps, checkpoints every
     # 256) and the adjoint sweep, FP32 + Kahan

@@ -9,7 +9,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libswals_hip.so")
 
-SWH_METHOD = {"rk2": 1, "rk2n": 2, "midpoint": 3, "euler": 4, "linear": 5, "leapfrog_linear_plus_nonlinear_friction": 6}
+SWH_METHOD = {"rk2": 1, "rk2n": 2, "midpoint": 3, "euler": 4, "linear": 5, "leapfrog_linear_plus_nonlinear_friction": 6,
+              "leapfrog_nonlinear": 8}
 SWH_PREC = {"fp64": 0, "fp32k": 1}
 MAXQ_BITS = {"max_stage": 1, "max_speed": 2, "max_flux": 4, "arrival_time": 8, "time_of_max_stage": 16,
              "min_stage": 32}

@@ -158,7 +158,7 @@ int maxq_now(swh_domain *h, const DevScalars *tsrc, double t)
     if (!h->maxq_mask) return SWH_OK;
     const int c = h->cur;
     SWH_HIP(launch_maxq<T>((const T *)h->S[c], (const T *)h->C[c], (const T *)h->UH[c], (const T *)h->VH[c],
-                           (const T *)h->E, h->nx, h->ny, h->pitch, h->leapfrog() ? 1 : 0, h->truely_linear,
+                           (const T *)h->E, h->nx, h->ny, h->pitch, h->leapfrog() ? 1 : 0, h->tl_solver() ? 1 : 0,
                            (T)h->msl, h->maxq_mask, (T *)h->mx, tsrc, (T)t, (T)(h->msl + h->arrival_thresh),
                            h->stream));
     return SWH_OK;
@@ -267,6 +267,83 @@ int leapfrog_step(swh_domain *h, double dt)
     return SWH_OK;
 }
 
+// 'leapfrog_nonlinear' step (domain_mod_leapfrog_nonlinear_include.f90:61-826): the
+// depth before the step, the stage half-step alone, the boundary, forcing and boundary
+// fluxes as in the linear solver, then the nonlinear momentum passes into the spare
+// planes. max_U after the step on the update cadence.
+template <typename T, bool KAHAN>
+int nonlinear_step(swh_domain *h, double dt)
+{
+    if (h->n_nest_strips) return fail(SWH_ESTATE, "'leapfrog_nonlinear' is not provided inside a multidomain");
+    if (h->rc_dt != dt) {
+        int rc = build_rc<T>(h, dt);
+        if (rc) return rc;
+    }
+    const double time0 = h->time;
+    if (h->nsteps == 0) {
+        int rc = maxq_now<T>(h, nullptr, h->time);
+        if (rc) return rc;
+    }
+    const int c = h->cur, o = 1 - c;
+    const double t_half = h->time + dt * 0.5;
+    const double t_after = t_half + 0.5 * dt;
+    const size_t P = h->plane_elems();
+    T *w = (T *)h->nlw;
+    NLArgs<T> n;
+    n.nx = h->nx; n.ny = h->ny; n.pitch = h->pitch;
+    n.S0 = (const T *)h->S[c]; n.S = (const T *)h->S[o]; n.E = (const T *)h->E;
+    n.UH = (const T *)h->UH[c]; n.VH = (const T *)h->VH[c]; n.UHo = (T *)h->UH[o]; n.VHo = (T *)h->VH[o];
+    n.nsq = (const T *)h->nsq;
+    n.DL = w; n.UUH = w + P; n.UVH = w + 2 * P; n.VUH = w + 3 * P; n.VVH = w + 4 * P;
+    n.fEWu = w + 5 * P; n.fEWv = w + 6 * P; n.fNSu = w + 7 * P; n.fNSv = w + 8 * P;
+    n.dbe = h->d_dbe; n.dle = h->d_dle; n.area = h->d_area; n.cor = h->d_cor; n.corbe = h->d_corbe;
+    n.tanlat = h->d_tanlat;
+    n.dt = (T)dt; n.lfc = (T)h->lfc;
+    n.fric_power = (T)((h->friction_type == 1) ? -2.0 : -7.0 / 3.0);
+    n.beta = T(1);            // advection_beta (global_mod.f90:38)
+    n.froude_limit = T(10);   // leapfrog_froude_limit (domain_mod.f90:210)
+    n.coriolis = h->coriolis; n.spherical = h->spherical;
+    SWH_HIP(launch_nonlinear_depth0<T>(n, h->stream));
+
+    LFArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
+    a.S = (const T *)h->S[c]; a.C = (const T *)h->C[c]; a.UH = (const T *)h->UH[c]; a.VH = (const T *)h->VH[c];
+    a.E = (const T *)h->E;
+    a.So = (T *)h->S[o]; a.Co = (T *)h->C[o]; a.UHo = (T *)h->UH[o]; a.VHo = (T *)h->VH[o];
+    a.rc = rc_view<T>(h);
+    const bool flather = h->bc_mask && h->bc_kind == 1;
+    const double bc_stage = h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period);
+    a.bc_mask = flather ? 0 : h->bc_mask;
+    a.bc_stage = (T)bc_stage;
+    a.maxq_mask = 0;
+    SWH_HIP(launch_lf_stage<T, KAHAN>(a, h->stream));
+    T *Cb = KAHAN ? (T *)h->C[o] : nullptr;
+    if (flather)
+        SWH_HIP(launch_flather<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (const T *)h->E, h->nx, h->ny,
+                                  h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl,
+                                  bc_stage, nullptr, h->stream));
+    const double frac = forcing_fraction(h, t_half, dt);
+    if (frac != 0.0 && h->fS)
+        SWH_HIP(launch_add_scaled_kahan<T>((T *)h->S[o], Cb, (const T *)h->fS, h->pitch, h->f_i0, h->f_i1, h->f_j0,
+                                           h->f_j1, (T)frac, h->stream));
+    if (frac != 0.0 && h->fE)
+        SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
+                                     (T)frac, h->stream));
+    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
+                                     h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+    SWH_HIP(launch_nonlinear_momentum<T>(n, h->stream));
+    h->cur = o;
+    h->time = t_after;
+    h->dt_last_update = dt;
+    h->evolve_step_dt = h->time - time0;
+    h->nsteps++;
+    if (h->maxq_mask && ((h->nsteps - 1) % h->maxq_freq) == 0) {
+        int rc = maxq_now<T>(h, nullptr, h->time);
+        if (rc) return rc;
+    }
+    return SWH_OK;
+}
+
 // Two leapfrog steps in one launch (swh_leapfrog2.hip) for a single grid: possible
 // where no stage boundary, nesting strips or per-step friction work are involved and
 // max_U is not due after the first of the two steps. The boundary fluxes of the second
@@ -276,6 +353,7 @@ int leapfrog_step(swh_domain *h, double dt)
 bool two_step_ok(const swh_domain *h, double dt)
 {
     if (!h->two_step || !h->leapfrog() || h->bc_mask || h->n_nest_strips) return false;
+    if (h->method == SWH_METHOD_LEAPFROG_NONLINEAR) return false;
     if (h->f_on) {
         // no forcing in either step of the pair
         const double t_half1 = h->time + dt * 0.5, t_after1 = t_half1 + 0.5 * dt;
@@ -488,6 +566,10 @@ int fv_step(swh_domain *h, double dt)
 
 int evolve(swh_domain *h, double dt)
 {
+    if (h->method == SWH_METHOD_LEAPFROG_NONLINEAR) {
+        if (!(dt > 0.0)) return fail(SWH_EINVAL, "timestep must be provided for leapfrog-type evolve_one_step");
+        return h->prec == SWH_PREC_FP64 ? nonlinear_step<double, false>(h, dt) : nonlinear_step<float, true>(h, dt);
+    }
     if (h->leapfrog()) {
         if (!(dt > 0.0)) return fail(SWH_EINVAL, "timestep must be provided for leapfrog-type evolve_one_step");
         return h->prec == SWH_PREC_FP64 ? leapfrog_step<double, false>(h, dt) : leapfrog_step<float, true>(h, dt);
@@ -544,7 +626,8 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     if (!d || !out) return fail(SWH_EINVAL, "null argument");
     *out = nullptr;
     if (d->nx < 3 || d->ny < 3) return fail(SWH_EINVAL, "nx and ny must be >= 3");
-    if (d->timestepping_method < SWH_METHOD_RK2 || d->timestepping_method > SWH_METHOD_LLPNF)
+    if (!((d->timestepping_method >= SWH_METHOD_RK2 && d->timestepping_method <= SWH_METHOD_LLPNF) ||
+          d->timestepping_method == SWH_METHOD_LEAPFROG_NONLINEAR))
         return fail(SWH_EINVAL, "unsupported timestepping_method id");
     if (d->precision != SWH_PREC_FP64 && d->precision != SWH_PREC_FP32K) return fail(SWH_EINVAL, "bad precision");
     if (!d->distance_bottom_edge || !d->distance_left_edge || !d->area_cell_y)
@@ -627,6 +710,10 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     A(&h->E, P);
     A(&h->nsq, P);
     if (h->method == SWH_METHOD_LLPNF) { A(&h->fwU, P); A(&h->fwV, P); }
+    if (h->method == SWH_METHOD_LEAPFROG_NONLINEAR) {
+        A(&h->nlw, P * 9);
+        A((void **)&h->d_corbe, sizeof(double) * (ny + 1));
+    }
     if (h->maxq_mask) A(&h->mx, P * N_MAXQ);
     if (fv) {
         const size_t FPn = (size_t)h->fpitch * (ny + 1);
@@ -649,6 +736,8 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
         hipMemcpy(h->d_dle, h->dle.data(), sizeof(double) * (nx + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_area, h->area.data(), sizeof(double) * ny, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_cor, h->cor.data(), sizeof(double) * ny, hipMemcpyHostToDevice) != hipSuccess ||
+        (h->d_corbe &&
+         hipMemcpy(h->d_corbe, h->corbe.data(), sizeof(double) * (ny + 1), hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(h->d_tanlat, h->tanlat.data(), sizeof(double) * ny, hipMemcpyHostToDevice) != hipSuccess) {
         swh_domain_destroy(h);
         return fail(SWH_EHIP, "metric upload failed");
@@ -680,7 +769,8 @@ int swh_domain_destroy(swh_domain *h)
                     h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
-                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE};
+                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
+                    h->nlw, h->d_corbe};
     for (void *p : ptrs) dfree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -897,7 +987,7 @@ int swh_stationary_timestep_max(swh_domain *h, double *out)
     if (!h || !out) return fail(SWH_EINVAL, "null argument");
     const int c = h->cur;
     double *res = h->d_work + 2 * (size_t)h->ny + 9;
-    const int linear = h->leapfrog() && h->truely_linear;
+    const int linear = h->tl_solver();
     if (h->prec == SWH_PREC_FP64)
         SWH_HIP(launch_timestep_max<double>((const double *)h->S[c], (const double *)h->E, h->nx, h->ny, h->pitch,
                                             linear, h->msl, h->d_dbe, h->d_dle, res, h->stream));

@@ -27,6 +27,9 @@ struct swh_domain {
     void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
     double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
     double *d_bmax = nullptr;   // per-block CFL maxima of the flux pass
+    // 'leapfrog_nonlinear': nine work planes (swh_nonlinear.hip) and coriolis_bottom_edge
+    void *nlw = nullptr;
+    double *d_corbe = nullptr;
     // leapfrog row coefficients for rc_dt
     void *rc = nullptr;
     double rc_dt = -1.0;
@@ -67,7 +70,13 @@ struct swh_domain {
     long long *d_bfl_idx = nullptr;
     int n_bfl = 0;
 
-    bool leapfrog() const { return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF; }
+    // staggered leapfrog family (time kept on the host, fixed timestep)
+    bool leapfrog() const
+    {
+        return method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF || method == SWH_METHOD_LEAPFROG_NONLINEAR;
+    }
+    // depths relative to msl_linear ('linear' / llpnf with linear_solver_is_truely_linear)
+    bool tl_solver() const { return truely_linear && (method == SWH_METHOD_LINEAR || method == SWH_METHOD_LLPNF); }
     size_t plane_elems() const { return (size_t)pitch * ny; }
 };
 

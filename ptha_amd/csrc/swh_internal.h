@@ -151,6 +151,28 @@ template <typename T>
 hipError_t launch_fv_forcing(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0, int i1,
                              int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s);
 // elevation += frac * fE over the patch (the leapfrog step's elevation forcing)
+// 'leapfrog_nonlinear' momentum half-step (swh_nonlinear.hip). Planes of pitch * ny.
+template <typename T>
+struct NLArgs {
+    int nx, ny, pitch;
+    const T *S0;              // stage before the stage half-step (time-lagged depth)
+    const T *S, *E;           // stage and elevation after the boundary and forcing
+    const T *UH, *VH;         // momenta before the update
+    T *UHo, *VHo;             // updated momenta
+    const T *nsq;             // manning_squared
+    T *DL, *UUH, *UVH, *VUH, *VVH, *fEWu, *fEWv, *fNSu, *fNSv;  // work planes
+    const double *dbe, *dle, *area, *cor, *corbe, *tanlat;     // FP64 metrics
+    T dt, lfc, fric_power, beta, froude_limit;
+    int coriolis, spherical;
+};
+// depth before the stage half-step (launch first)
+template <typename T>
+hipError_t launch_nonlinear_depth0(const NLArgs<T> &a, hipStream_t s);
+// time-lagged depth, advection work, fluxes, momentum update (after the stage half-step,
+// boundary and forcing)
+template <typename T>
+hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, hipStream_t s);
+
 // New stage only (the stage half-step of the fused step, same per-cell operations),
 // S/C -> So/Co over the whole grid, border cells passed through
 template <typename T, bool KAHAN>

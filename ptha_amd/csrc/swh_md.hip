@@ -623,6 +623,10 @@ int swh_md_create(int32_t n_domains, swh_domain *const *domains, const double *a
             delete md;
             return md_fail(SWH_EINVAL, "swh_md_create: null domain");
         }
+        if (domains[k]->method == SWH_METHOD_LEAPFROG_NONLINEAR) {
+            delete md;
+            return md_fail(SWH_EINVAL, "swh_md_create: 'leapfrog_nonlinear' domains are single-domain only here");
+        }
         if (timestepping_refinement_factor[k] < 1) {
             delete md;
             return md_fail(SWH_EINVAL, "swh_md_create: timestepping_refinement_factor must be >= 1");

@@ -14,7 +14,7 @@ module swals_hip_mod
     private
 
     integer(c_int32_t), parameter, public :: SWH_METHOD_RK2 = 1, SWH_METHOD_RK2N = 2, SWH_METHOD_MIDPOINT = 3, &
-        SWH_METHOD_EULER = 4, SWH_METHOD_LINEAR = 5, SWH_METHOD_LLPNF = 6
+        SWH_METHOD_EULER = 4, SWH_METHOD_LINEAR = 5, SWH_METHOD_LLPNF = 6, SWH_METHOD_LEAPFROG_NONLINEAR = 8
     integer(c_int32_t), parameter, public :: SWH_PREC_FP64 = 0, SWH_PREC_FP32K = 1
     integer(c_int32_t), parameter, public :: SWH_MAXQ_MAX_STAGE = 1, SWH_MAXQ_MAX_SPEED = 2, &
         SWH_MAXQ_MAX_FLUX = 4, SWH_MAXQ_ARRIVAL_TIME = 8, SWH_MAXQ_TIME_OF_MAX_STAGE = 16, SWH_MAXQ_MIN_STAGE = 32
