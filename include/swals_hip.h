@@ -177,6 +177,16 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 
 int swh_synchronize(swh_domain *h);
 
+/* Output snapshot (the D->H staging behind write_to_output_files / write_grids,
+ * domain_mod.f90:2188-2218, netcdf_util_mod.f90:437-506). swh_snapshot packs the current
+ * state into a device buffer as FP64 U(nx, ny, 4) and its time, in stream order, and
+ * returns at once: the caller keeps stepping. swh_snapshot_download waits for that
+ * snapshot only (not for the steps after it) and copies it to U (4 * nx * ny doubles)
+ * and *time on a separate copy stream; pinned host memory makes the copy a DMA that
+ * overlaps the solver. One buffer per domain: a new snapshot replaces the last. */
+int swh_snapshot(swh_domain *h);
+int swh_snapshot_download(swh_domain *h, double *U, double *time);
+
 /* Forward run with gauges: samples the gauges set by swh_gauges_set now and after
  * every 'every' steps of nsteps (nsteps/every + 1 samples in all). */
 int swh_evolve_record_gauges(swh_domain *h, double timestep, int32_t nsteps, int32_t every);

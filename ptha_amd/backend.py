@@ -70,6 +70,8 @@ SYMBOLS = [
     ("swh_set_time", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("swh_set_stage_boundary_sine", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _PI32]),
     ("swh_set_flather_boundary", ctypes.c_int, [ctypes.c_void_p, _PI32, ctypes.c_int32, _D, _D, _D]),
+    ("swh_snapshot", ctypes.c_int, [ctypes.c_void_p]),
+    ("swh_snapshot_download", ctypes.c_int, [ctypes.c_void_p, _PD, _PD]),
     ("swh_evolve_one_step", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("swh_evolve_steps", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_get_scalars", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Scalars)]),

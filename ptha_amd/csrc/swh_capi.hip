@@ -772,6 +772,9 @@ int swh_domain_destroy(swh_domain *h)
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
                     h->nlw, h->d_corbe};
     for (void *p : ptrs) dfree(p);
+    dfree(h->snap);
+    if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
+    if (h->cstream) (void)hipStreamDestroy(h->cstream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SWH_OK;
@@ -1069,6 +1072,46 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
         SWH_HIP(hipMemcpy(times, h->d_gtime, sizeof(double) * h->gsamples, hipMemcpyDeviceToHost));
     }
     *nsamples = h->gsamples;
+    return SWH_OK;
+}
+
+int swh_snapshot(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    const size_t n = (size_t)h->nx * h->ny;
+    if (!h->snap) {
+        SWH_HIP(hipStreamSynchronize(h->stream));
+        int rc = dmalloc((void **)&h->snap, (4 * n + 1) * sizeof(double));
+        if (rc) return rc;
+        SWH_HIP(hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
+        SWH_HIP(hipStreamCreate(&h->cstream));
+    }
+    const int c = h->cur;
+    const DevScalars *tsrc = h->leapfrog() ? nullptr : h->sc;
+    if (h->prec == SWH_PREC_FP64)
+        SWH_HIP(launch_snapshot<double>((const double *)h->S[c], nullptr, (const double *)h->UH[c],
+                                        (const double *)h->VH[c], (const double *)h->E, h->nx, h->ny, h->pitch,
+                                        h->snap, tsrc, h->time, h->stream));
+    else
+        SWH_HIP(launch_snapshot<float>((const float *)h->S[c], (const float *)h->C[c], (const float *)h->UH[c],
+                                       (const float *)h->VH[c], (const float *)h->E, h->nx, h->ny, h->pitch, h->snap,
+                                       tsrc, h->time, h->stream));
+    SWH_HIP(hipEventRecord(h->snap_ev, h->stream));
+    h->snap_ok = 1;
+    return SWH_OK;
+}
+
+int swh_snapshot_download(swh_domain *h, double *U, double *time)
+{
+    if (!h || !U) return fail(SWH_EINVAL, "null argument");
+    if (!h->snap_ok) return fail(SWH_ESTATE, "no snapshot taken (swh_snapshot)");
+    const size_t n = (size_t)h->nx * h->ny;
+    double t = 0.0;
+    SWH_HIP(hipStreamWaitEvent(h->cstream, h->snap_ev, 0));
+    SWH_HIP(hipMemcpyAsync(U, h->snap, 4 * n * sizeof(double), hipMemcpyDeviceToHost, h->cstream));
+    SWH_HIP(hipMemcpyAsync(&t, h->snap + 4 * n, sizeof(double), hipMemcpyDeviceToHost, h->cstream));
+    SWH_HIP(hipStreamSynchronize(h->cstream));
+    if (time) *time = t;
     return SWH_OK;
 }
 

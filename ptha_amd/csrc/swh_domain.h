@@ -27,6 +27,12 @@ struct swh_domain {
     void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
     double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
     double *d_bmax = nullptr;   // per-block CFL maxima of the flux pass
+    // output snapshot: dense FP64 [4][ny][nx] copy of the state plus its time, packed on
+    // the domain stream; downloaded on a copy stream while the solver runs on
+    double *snap = nullptr;
+    hipEvent_t snap_ev = nullptr;
+    hipStream_t cstream = nullptr;
+    int snap_ok = 0;
     // 'leapfrog_nonlinear': nine work planes (swh_nonlinear.hip) and coriolis_bottom_edge
     void *nlw = nullptr;
     double *d_corbe = nullptr;

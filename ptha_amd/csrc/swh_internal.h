@@ -173,6 +173,13 @@ hipError_t launch_nonlinear_depth0(const NLArgs<T> &a, hipStream_t s);
 template <typename T>
 hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, hipStream_t s);
 
+// Dense FP64 copy of the state [S (- C), UH, VH, E] as out[(var * ny + j) * nx + i]
+// (the bytes of Fortran U(nx, ny, 4)), with out[4 * nx * ny] = the time: sc->time when
+// sc is given, else t_host.
+template <typename T>
+hipError_t launch_snapshot(const T *S, const T *C, const T *UH, const T *VH, const T *E, int nx, int ny, int pitch,
+                           double *out, const DevScalars *sc, double t_host, hipStream_t s);
+
 // New stage only (the stage half-step of the fused step, same per-cell operations),
 // S/C -> So/Co over the whole grid, border cells passed through
 template <typename T, bool KAHAN>

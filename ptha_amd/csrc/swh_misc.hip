@@ -200,6 +200,36 @@ template hipError_t launch_add_scaled_kahan<double>(double *, double *, const do
 template hipError_t launch_add_scaled_kahan<float>(float *, float *, const float *, int, int, int, int, int, float,
                                                    hipStream_t);
 
+// ---- output snapshot ----
+template <typename T>
+__global__ void snapshot_kernel(const T *S, const T *C, const T *UH, const T *VH, const T *E, int nx, int ny,
+                                int pitch, double *out, const DevScalars *sc, double t_host)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const long long n = (long long)nx * ny;
+    if (i == 0 && j == 0) out[4 * n] = sc ? sc->time : t_host;
+    if (i >= nx || j >= ny) return;
+    const long long o = (long long)j * pitch + i, q = (long long)j * nx + i;
+    out[q] = C ? (double)S[o] - (double)C[o] : (double)S[o];
+    out[n + q] = (double)UH[o];
+    out[2 * n + q] = (double)VH[o];
+    out[3 * n + q] = (double)E[o];
+}
+
+template <typename T>
+hipError_t launch_snapshot(const T *S, const T *C, const T *UH, const T *VH, const T *E, int nx, int ny, int pitch,
+                           double *out, const DevScalars *sc, double t_host, hipStream_t s)
+{
+    hipLaunchKernelGGL(snapshot_kernel<T>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, S, C, UH, VH, E, nx, ny, pitch,
+                       out, sc, t_host);
+    return hipGetLastError();
+}
+template hipError_t launch_snapshot<double>(const double *, const double *, const double *, const double *,
+                                            const double *, int, int, int, double *, const DevScalars *, double,
+                                            hipStream_t);
+template hipError_t launch_snapshot<float>(const float *, const float *, const float *, const float *, const float *,
+                                           int, int, int, double *, const DevScalars *, double, hipStream_t);
+
 // ---- Flather boundary (boundary_mod.f90:502-720) ----
 // One edge cell b with the cell n inside it. West and south edges (sgn < 0) take the
 // incoming characteristic from outside as w1 = vel + sqrt(g/h) stage, east and north

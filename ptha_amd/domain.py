@@ -234,6 +234,22 @@ class Domain:
         B.check(self._lib.swh_stationary_timestep_max(self._h, ctypes.byref(v)), "swh_stationary_timestep_max")
         return v.value
 
+    def snapshot(self):
+        """Pack the current state and time on the device for output (swh_snapshot) and
+        return at once; fetch it later with snapshot_download while stepping goes on."""
+        B.check(self._lib.swh_snapshot(self._h), "swh_snapshot")
+
+    def snapshot_download(self, out=None):
+        """(U (4, ny, nx) float64, time) of the last snapshot. out: an optional
+        preallocated (ideally pinned) float64 array of that shape."""
+        if out is None:
+            out = np.empty((4, self.ny, self.nx))
+        if out.shape != (4, self.ny, self.nx) or out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array of shape (4, ny, nx)")
+        t = ctypes.c_double(0.0)
+        B.check(self._lib.swh_snapshot_download(self._h, _p(out), ctypes.byref(t)), "swh_snapshot_download")
+        return out, t.value
+
     def max_U(self, name):
         out = np.empty((self.ny, self.nx))
         B.check(self._lib.swh_download_max_U(self._h, B.MAXQ_BITS[name], _p(out)), "swh_download_max_U")
