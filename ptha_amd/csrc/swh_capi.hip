@@ -303,7 +303,11 @@ int nonlinear_step(swh_domain *h, double dt)
     n.beta = T(1);            // advection_beta (global_mod.f90:38)
     n.froude_limit = T(10);   // leapfrog_froude_limit (domain_mod.f90:210)
     n.coriolis = h->coriolis; n.spherical = h->spherical;
-    SWH_HIP(launch_nonlinear_depth0<T>(n, h->stream));
+    // the fused flux pass forms the lagged depth from the old and new stage, which needs
+    // the elevation unchanged through the step; with elevation forcing, the depth plane
+    const double frac = forcing_fraction(h, t_half, dt);
+    const bool elev_forced = frac != 0.0 && h->fE;
+    if (elev_forced) SWH_HIP(launch_nonlinear_depth0<T>(n, h->stream));
 
     LFArgs<T> a;
     a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
@@ -322,7 +326,6 @@ int nonlinear_step(swh_domain *h, double dt)
         SWH_HIP(launch_flather<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (const T *)h->E, h->nx, h->ny,
                                   h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl,
                                   bc_stage, nullptr, h->stream));
-    const double frac = forcing_fraction(h, t_half, dt);
     if (frac != 0.0 && h->fS)
         SWH_HIP(launch_add_scaled_kahan<T>((T *)h->S[o], Cb, (const T *)h->fS, h->pitch, h->f_i0, h->f_i1, h->f_j0,
                                            h->f_j1, (T)frac, h->stream));
@@ -331,7 +334,7 @@ int nonlinear_step(swh_domain *h, double dt)
                                      (T)frac, h->stream));
     SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
                                      h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
-    SWH_HIP(launch_nonlinear_momentum<T>(n, h->stream));
+    SWH_HIP(launch_nonlinear_momentum<T>(n, !elev_forced, h->stream));
     h->cur = o;
     h->time = t_after;
     h->dt_last_update = dt;

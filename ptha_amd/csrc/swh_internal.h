@@ -169,9 +169,11 @@ struct NLArgs {
 template <typename T>
 hipError_t launch_nonlinear_depth0(const NLArgs<T> &a, hipStream_t s);
 // time-lagged depth, advection work, fluxes, momentum update (after the stage half-step,
-// boundary and forcing)
+// boundary and forcing). fused: fluxes straight from the stage before and after the
+// stage half-step (valid while the elevation is unchanged in the step; then
+// launch_nonlinear_depth0 is not needed)
 template <typename T>
-hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, hipStream_t s);
+hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, bool fused, hipStream_t s);
 
 // Dense FP64 copy of the state [S (- C), UH, VH, E] as out[(var * ny + j) * nx + i]
 // (the bytes of Fortran U(nx, ny, 4)), with out[4 * nx * ny] = the time: sc->time when

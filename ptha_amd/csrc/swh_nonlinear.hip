@@ -57,48 +57,75 @@ __device__ __forceinline__ T nl_inv_depth(T d)
     return d > T(kMinDepth) ? T(1) / d : T(0);
 }
 
-// advection work: index maps of the reference's extrapolations (0-based)
+// Time-lagged depth at cell q of row j: from the plane, or (no elevation forcing in the
+// step, so the elevation is unchanged) formed from the old and new stage exactly as
+// nl_depth0_kernel and nl_depth1_kernel form it.
+template <typename T, bool FLY>
+__device__ __forceinline__ T nl_dl(const NLArgs<T> &a, long long q, int j)
+{
+    if (!FLY) return a.DL[q];
+    const T d0 = nl_max0<T>(a.S0[q] - a.E[q]);
+    return j < a.ny - 1 ? T(0.5) * (d0 + nl_max0<T>(a.S[q] - a.E[q])) : d0;
+}
+
+// The four advection work values of cell (i, j) with the reference's extrapolations
+// to the east columns and north rows as index maps (0-based), :274-328.
+template <typename T, bool FLY>
+__device__ __forceinline__ T nl_uuh(const NLArgs<T> &a, int i, int j)
+{
+    // U*UH at (i+1/2, j): columns < nx-1; column nx-1 copies nx-2; row ny-1 copies ny-2
+    const int jj = j == a.ny - 1 ? a.ny - 2 : j;
+    const int ii = i == a.nx - 1 ? a.nx - 2 : i;
+    const long long q = (long long)jj * a.pitch + ii;
+    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + 1, jj)));
+    return (a.UH[q] * a.UH[q]) * inv;
+}
+
+template <typename T, bool FLY>
+__device__ __forceinline__ T nl_uvh(const NLArgs<T> &a, int i, int j)
+{
+    // U*VH at (i+1/2, j): columns < nx-2; columns nx-2, nx-1 copy nx-3; row ny-1 copies ny-2
+    const int jj = j == a.ny - 1 ? a.ny - 2 : j;
+    const int ii = i >= a.nx - 2 ? a.nx - 3 : i;
+    const int jm = jj > 0 ? jj - 1 : 0;
+    const long long q = (long long)jj * a.pitch + ii, qm = (long long)jm * a.pitch + ii;
+    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + 1, jj)));
+    return a.UH[q] * inv * T(0.25) * (a.VH[qm] + a.VH[qm + 1] + a.VH[q] + a.VH[q + 1]);
+}
+
+template <typename T, bool FLY>
+__device__ __forceinline__ T nl_vvh(const NLArgs<T> &a, int i, int j)
+{
+    // V*VH at (i, j+1/2): rows < ny-1; row ny-1 copies ny-2
+    const int jj = j == a.ny - 1 ? a.ny - 2 : j;
+    const long long q = (long long)jj * a.pitch + i;
+    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + a.pitch, jj + 1)));
+    return a.VH[q] * a.VH[q] * inv;
+}
+
+template <typename T, bool FLY>
+__device__ __forceinline__ T nl_vuh(const NLArgs<T> &a, int i, int j)
+{
+    // V*UH at (i, j+1/2): columns > 0, rows < ny-2; column 0 copies 1; rows ny-2, ny-1 copy ny-3
+    const int jj = j >= a.ny - 2 ? a.ny - 3 : j;
+    const int ii = i == 0 ? 1 : i;
+    const long long q = (long long)jj * a.pitch + ii;
+    const int P = a.pitch;
+    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + P, jj + 1)));
+    return a.VH[q] * inv * T(0.25) * (a.UH[q - 1] + a.UH[q] + a.UH[q - 1 + P] + a.UH[q + P]);
+}
+
+// advection work planes (used when the elevation changes within the step)
 template <typename T>
 __global__ void nl_advection_kernel(NLArgs<T> a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= a.nx || j >= a.ny) return;
-    const int nx = a.nx, ny = a.ny, P = a.pitch;
-    const T *UH = a.UH, *VH = a.VH, *DL = a.DL;
-    const long long o = (long long)j * P + i;
-    const T Q = T(0.25), H = T(0.5);
-    {
-        // U*UH at (i+1/2, j): columns < nx-1; column nx-1 copies nx-2; row ny-1 copies ny-2
-        const int jj = j == ny - 1 ? ny - 2 : j;
-        const int ii = i == nx - 1 ? nx - 2 : i;
-        const long long q = (long long)jj * P + ii;
-        const T inv = nl_inv_depth<T>(H * (DL[q] + DL[q + 1]));
-        a.UUH[o] = (UH[q] * UH[q]) * inv;
-    }
-    {
-        // U*VH at (i+1/2, j): columns < nx-2; columns nx-2, nx-1 copy nx-3
-        const int jj = j == ny - 1 ? ny - 2 : j;
-        const int ii = i >= nx - 2 ? nx - 3 : i;
-        const int jm = jj > 0 ? jj - 1 : 0;
-        const long long q = (long long)jj * P + ii, qm = (long long)jm * P + ii;
-        const T inv = nl_inv_depth<T>(H * (DL[q] + DL[q + 1]));
-        a.UVH[o] = UH[q] * inv * Q * (VH[qm] + VH[qm + 1] + VH[q] + VH[q + 1]);
-    }
-    {
-        // V*VH at (i, j+1/2): rows < ny-1; row ny-1 copies ny-2
-        const int jj = j == ny - 1 ? ny - 2 : j;
-        const long long q = (long long)jj * P + i;
-        const T inv = nl_inv_depth<T>(H * (DL[q] + DL[q + P]));
-        a.VVH[o] = VH[q] * VH[q] * inv;
-    }
-    {
-        // V*UH at (i, j+1/2): columns > 0, rows < ny-2; column 0 copies 1; rows ny-2, ny-1 copy ny-3
-        const int jj = j >= ny - 2 ? ny - 3 : j;
-        const int ii = i == 0 ? 1 : i;
-        const long long q = (long long)jj * P + ii;
-        const T inv = nl_inv_depth<T>(H * (DL[q] + DL[q + P]));
-        a.VUH[o] = VH[q] * inv * Q * (UH[q - 1] + UH[q] + UH[q - 1 + P] + UH[q + P]);
-    }
+    const long long o = (long long)j * a.pitch + i;
+    a.UUH[o] = nl_uuh<T, false>(a, i, j);
+    a.UVH[o] = nl_uvh<T, false>(a, i, j);
+    a.VVH[o] = nl_vvh<T, false>(a, i, j);
+    a.VUH[o] = nl_vuh<T, false>(a, i, j);
 }
 
 template <typename T>
@@ -107,8 +134,10 @@ __device__ __forceinline__ bool nl_wet(const NLArgs<T> &a, long long q)
     return a.S[q] > a.E[q] + T(kMinDepth);
 }
 
-// the four upwind momentum fluxes (fNSu / fNSv row j: the edge at the bottom of row j)
-template <typename T>
+// the four upwind momentum fluxes (fNSu / fNSv row j: the edge at the bottom of row j).
+// FUSED: the advection work values are formed where needed (each from the same
+// operations as nl_advection_kernel) instead of read from its planes.
+template <typename T, bool FUSED>
 __global__ void nl_flux_kernel(NLArgs<T> a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
@@ -117,17 +146,22 @@ __global__ void nl_flux_kernel(NLArgs<T> a)
     const T *UH = a.UH, *VH = a.VH;
     const long long o = (long long)j * P + i;
     const T H = T(0.5), Z = T(0);
+    auto UUH = [&](int ii, int jj) { return FUSED ? nl_uuh<T, true>(a, ii, jj) : a.UUH[(long long)jj * P + ii]; };
+    auto UVH = [&](int ii, int jj) { return FUSED ? nl_uvh<T, true>(a, ii, jj) : a.UVH[(long long)jj * P + ii]; };
+    auto VVH = [&](int ii, int jj) { return FUSED ? nl_vvh<T, true>(a, ii, jj) : a.VVH[(long long)jj * P + ii]; };
+    auto VUH = [&](int ii, int jj) { return FUSED ? nl_vuh<T, true>(a, ii, jj) : a.VUH[(long long)jj * P + ii]; };
     // EW flux of UH (rows < ny-1)
     {
         T f = Z;
         if (j < ny - 1) {
             const T dle = (T)a.dle[i];
             if (i == 0 || i == nx - 1) {
-                f = a.UUH[o] * dle;
+                f = UUH(i, j) * dle;
             } else {
                 const T us = UH[o] + UH[o - 1];
-                f = dle * (us > Z ? a.UUH[o - 1] : a.UUH[o]);
-                if (us == Z) f = dle * H * (a.UUH[o - 1] + a.UUH[o]);
+                const T ul = UUH(i - 1, j), ur = UUH(i, j);
+                f = dle * (us > Z ? ul : ur);
+                if (us == Z) f = dle * H * (ul + ur);
             }
             if (!nl_wet(a, o)) f = Z;
         }
@@ -142,8 +176,9 @@ __global__ void nl_flux_kernel(NLArgs<T> a)
             const long long qa = (long long)ja * P + i, qb = (long long)jb * P + i;
             const T dbe = (T)a.dbe[j];
             const T vs = VH[qa] + VH[qa + 1];
-            f = dbe * (vs > Z ? a.UVH[qa] : a.UVH[qb]);
-            if (vs == Z) f = dbe * H * (a.UVH[qa] + a.UVH[qb]);
+            const T wa = UVH(i, ja), wb = UVH(i, jb);
+            f = dbe * (vs > Z ? wa : wb);
+            if (vs == Z) f = dbe * H * (wa + wb);
             if (!(nl_wet(a, qa) && nl_wet(a, qa + 1) && nl_wet(a, qb) && nl_wet(a, qb + 1))) f = Z;
         }
         a.fNSu[o] = f;
@@ -156,8 +191,9 @@ __global__ void nl_flux_kernel(NLArgs<T> a)
             const long long qm = (long long)jm * P + i;
             const T dbe = (T)(0.5 * (a.dbe[j] + a.dbe[j + 1]));
             const T vs = VH[o] + VH[qm];
-            f = dbe * (vs > Z ? a.VVH[qm] : a.VVH[o]);
-            if (vs == Z) f = dbe * H * (a.VVH[qm] + a.VVH[o]);
+            const T vm = VVH(i, jm), v0 = VVH(i, j);
+            f = dbe * (vs > Z ? vm : v0);
+            if (vs == Z) f = dbe * H * (vm + v0);
             if (!nl_wet(a, o)) f = Z;
         }
         a.fNSv[o] = f;
@@ -171,9 +207,10 @@ __global__ void nl_flux_kernel(NLArgs<T> a)
             const long long q = (long long)j * P + ic;
             const T dle = (T)a.dle[ic - 1];
             const T us = UH[q - 1] + UH[q - 1 + P];
-            f = dle * (us > Z ? a.VUH[q - 1] : a.VUH[q]);
+            const T xl = VUH(ic - 1, j), xr = VUH(ic, j);
+            f = dle * (us > Z ? xl : xr);
             if (i > 0) {
-                if (us == Z) f = dle * H * (a.VUH[q - 1] + a.VUH[q]);
+                if (us == Z) f = dle * H * (xl + xr);
                 if (!(nl_wet(a, q - 1) && nl_wet(a, q) && nl_wet(a, q - 1 + P) && nl_wet(a, q + P))) f = Z;
             }
         }
@@ -299,20 +336,24 @@ hipError_t launch_nonlinear_depth0(const NLArgs<T> &a, hipStream_t s)
 }
 
 template <typename T>
-hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, hipStream_t s)
+hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, bool fused, hipStream_t s)
 {
     if (a.nx < 3 || a.ny < 3) return hipErrorInvalidValue;
     const dim3 grid((a.nx + 255) / 256, a.ny), block(256);
-    hipLaunchKernelGGL(nl_depth1_kernel<T>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(nl_advection_kernel<T>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(nl_flux_kernel<T>, grid, block, 0, s, a);
+    if (fused) {
+        hipLaunchKernelGGL((nl_flux_kernel<T, true>), grid, block, 0, s, a);
+    } else {
+        hipLaunchKernelGGL(nl_depth1_kernel<T>, grid, block, 0, s, a);
+        hipLaunchKernelGGL(nl_advection_kernel<T>, grid, block, 0, s, a);
+        hipLaunchKernelGGL((nl_flux_kernel<T, false>), grid, block, 0, s, a);
+    }
     hipLaunchKernelGGL(nl_momentum_kernel<T>, grid, block, 0, s, a);
     return hipGetLastError();
 }
 
 template hipError_t launch_nonlinear_depth0<double>(const NLArgs<double> &, hipStream_t);
 template hipError_t launch_nonlinear_depth0<float>(const NLArgs<float> &, hipStream_t);
-template hipError_t launch_nonlinear_momentum<double>(const NLArgs<double> &, hipStream_t);
-template hipError_t launch_nonlinear_momentum<float>(const NLArgs<float> &, hipStream_t);
+template hipError_t launch_nonlinear_momentum<double>(const NLArgs<double> &, bool, hipStream_t);
+template hipError_t launch_nonlinear_momentum<float>(const NLArgs<float> &, bool, hipStream_t);
 
 }  // namespace swh
