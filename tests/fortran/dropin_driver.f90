@@ -5,25 +5,36 @@
 ! metric arrays and U) and on the CPU with domain%evolve_one_step. Prints the max
 ! |difference| of each U plane and 'BITWISE_EQUAL' when every plane matches exactly.
 !
-! Usage: dropin_driver <method> <nsteps>  (method: linear | llpnf)
+! Usage: dropin_driver <method> <nsteps>  (method: linear | llpnf | nonlinear)
+!
+! 'nonlinear' runs 'leapfrog_nonlinear' without walls, with the reference's Flather
+! boundary (domain%boundary_subroutine => flather_boundary) and a co-seismic forcing
+! patch (forcing_patch_type%setup + apply_forcing_patch), each mapped to the backend as
+! INTEGRATION.md describes (swh_set_flather_boundary, swh_set_forcing_patch with the
+! patch's own forcing_work).
 program dropin_driver
     use iso_c_binding
     use global_mod, only: dp, ip, charlen
     use domain_mod, only: domain_type, STG, UH, VH, ELV
+    use boundary_mod, only: flather_boundary
+    use forcing_mod, only: forcing_patch_type, apply_forcing_patch
     use swals_hip_mod
     implicit none
 
     type(domain_type), target :: dom
+    type(forcing_patch_type), target :: fp
     type(swh_domain_desc) :: desc
     type(c_ptr) :: h
     real(dp), allocatable, target :: Ugpu(:,:,:)
     real(dp) :: dt, lw(2), ll(2), r2, diff(4)
     integer(ip) :: nx, ny, i, j, k, nsteps
+    integer(c_int32_t) :: sides(4)
     character(len=64) :: arg
-    logical :: llpnf
+    logical :: llpnf, nonlinear
 
     call get_command_argument(1, arg)
     llpnf = (trim(arg) == 'llpnf')
+    nonlinear = (trim(arg) == 'nonlinear')
     call get_command_argument(2, arg)
     read(arg, *) nsteps
 
@@ -32,6 +43,8 @@ program dropin_driver
     lw = [16.0_dp, 13.333333333333334_dp]
     if(llpnf) then
         dom%timestepping_method = 'leapfrog_linear_plus_nonlinear_friction'
+    else if(nonlinear) then
+        dom%timestepping_method = 'leapfrog_nonlinear'
     else
         dom%timestepping_method = 'linear'
     end if
@@ -47,17 +60,37 @@ program dropin_driver
             dom%U(i, j, STG) = merge(exp(-r2), 0.0_dp, dom%U(i, j, ELV) < 0.0_dp)
         end do
     end do
-    dom%U(1, :, ELV) = 1.0e6_dp; dom%U(nx, :, ELV) = 1.0e6_dp
-    dom%U(:, 1, ELV) = 1.0e6_dp; dom%U(:, ny, ELV) = 1.0e6_dp
+    if(.not. nonlinear) then
+        dom%U(1, :, ELV) = 1.0e6_dp; dom%U(nx, :, ELV) = 1.0e6_dp
+        dom%U(:, 1, ELV) = 1.0e6_dp; dom%U(:, ny, ELV) = 1.0e6_dp
+    end if
     dom%U(:, :, UH:VH) = 0.0_dp
     if(llpnf) dom%manning_squared = 0.03_dp**2
     dt = dom%stationary_timestep_max()
+    if(nonlinear) then
+        ! open boundaries and an uplift over 10 steps starting inside the first
+        dom%boundary_subroutine => flather_boundary
+        call fp%setup(0.4_dp * dt, 10.4_dp * dt, 30_ip, 50_ip, 20_ip, 45_ip, STG, ELV)
+        do j = 20, 45
+            do i = 30, 50
+                r2 = ((i - 40.0_dp)**2 + (j - 32.0_dp)**2) / 40.0_dp
+                fp%forcing_work(i, j, STG) = 2.0_dp * exp(-r2)
+                fp%forcing_work(i, j, UH) = 0.0_dp
+                fp%forcing_work(i, j, VH) = 0.0_dp
+                fp%forcing_work(i, j, ELV) = 2.0_dp * exp(-r2)
+            end do
+        end do
+        dom%forcing_context_cptr = c_loc(fp)
+        dom%forcing_subroutine => apply_forcing_patch
+    end if
 
     ! --- GPU: the same domain handed over through the C ABI ---
     desc%nx = nx; desc%ny = ny
     desc%precision = SWH_PREC_FP64
     if(llpnf) then
         desc%timestepping_method = SWH_METHOD_LLPNF
+    else if(nonlinear) then
+        desc%timestepping_method = SWH_METHOD_LEAPFROG_NONLINEAR
     else
         desc%timestepping_method = SWH_METHOD_LINEAR
     end if
@@ -76,8 +109,16 @@ program dropin_driver
     desc%coriolis_bottom_edge = c_loc(dom%coriolis_bottom_edge)
     desc%tanlat_on_radius_earth = c_loc(dom%tanlat_on_radius_earth)
     call swh_check(swh_domain_create(desc, h), 'swh_domain_create')
-    if(llpnf) call swh_check(swh_upload_manning_squared(h, dom%manning_squared), 'swh_upload_manning_squared')
+    if(llpnf .or. nonlinear) &
+        call swh_check(swh_upload_manning_squared(h, dom%manning_squared), 'swh_upload_manning_squared')
     call swh_check(swh_upload_U(h, dom%U), 'swh_upload_U')
+    if(nonlinear) then
+        sides = merge(1_c_int32_t, 0_c_int32_t, dom%boundary_exterior)
+        call swh_check(swh_set_flather_boundary(h, sides, 0_c_int32_t, 0.0_dp, 1.0_dp, 0.0_dp), 'flather')
+        call swh_check(swh_set_forcing_patch(h, fp%start_time, fp%end_time, int(fp%i0, c_int32_t), &
+            int(fp%i1, c_int32_t), int(fp%j0, c_int32_t), int(fp%j1, c_int32_t), int(fp%k0, c_int32_t), &
+            int(fp%k1, c_int32_t), fp%forcing_work), 'forcing')
+    end if
     call swh_check(swh_evolve_steps(h, dt, int(nsteps, c_int32_t)), 'swh_evolve_steps')
     allocate(Ugpu(nx, ny, 4))
     call swh_check(swh_download_U(h, Ugpu), 'swh_download_U')

@@ -1,7 +1,10 @@
 """GPU: the Fortran boundary. A Fortran program sets up the reference's own
 domain_type (compiled from the reference sources), passes c_loc of its metric arrays
 and U to the backend through swals_hip_mod, advances the domain on the GPU, and
-compares with domain%evolve_one_step on the CPU: FP64 must be bit-identical."""
+compares with domain%evolve_one_step on the CPU: FP64 must be bit-identical.
+'nonlinear' is 'leapfrog_nonlinear' (no friction) with the reference's Flather
+boundary and a forcing patch set up through forcing_mod, mapped as INTEGRATION.md
+shows."""
 import os
 import subprocess
 
@@ -13,7 +16,7 @@ EXE = os.path.join(ROOT, "oracle", "_ref", "dropin_driver_sph")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method", ["linear", "llpnf"])
+@pytest.mark.parametrize("method", ["linear", "llpnf", "nonlinear"])
 def test_fortran_dropin_bitwise(method):
     print("CASE fortran dropin", method, flush=True)
     if not os.path.exists(EXE):
