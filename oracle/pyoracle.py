@@ -73,6 +73,10 @@ def _load(kind, spherical):
     f("domain_get_max_U", None, [hd, _I, _PD])
     f("domain_set_stage_bc", None, [hd, _D, _D, _D, ctypes.POINTER(_I)])
     f("domain_set_flather_bc", None, [hd, _I, _D, _D, _D, ctypes.POINTER(_I)])
+    f("domain_dispersion_setup", None, [hd, _D, _D, _I])
+    f("domain_dispersion_store", None, [hd])
+    f("domain_dispersion_append", None, [hd])
+    f("domain_dispersion_solve", None, [hd, _I, _I, _D])
     f("domain_set_record_max_U", None, [hd, _I])
     if kind == "ref":
         f("domain_sample", None, [hd, _I, ctypes.POINTER(_I), _PD])
@@ -191,6 +195,30 @@ class CpuDomain:
         s = (_I * 4)(*sides)
         use_fun = amp is not None
         self.f["domain_set_flather_bc"](self.h, int(use_fun), float(amp or 0.0), float(period), float(msl), s)
+
+    def dispersion_setup(self, td1=0.0, td2=-1.0, niter=2):
+        self.f["domain_dispersion_setup"](self.h, float(td1), float(td2), int(niter))
+
+    def dispersion_store(self):
+        self.f["domain_dispersion_store"](self.h)
+
+    def dispersion_append(self):
+        self.f["domain_dispersion_append"](self.h)
+
+    def dispersion_solve(self, rhs_up, estimate, forward_time):
+        self.f["domain_dispersion_solve"](self.h, int(bool(rhs_up)), int(bool(estimate)), float(forward_time))
+
+    def dispersive_step(self, dt, outer_iterations=2):
+        """One step as multidomain_mod.f90:1245-1410 runs a single dispersive domain:
+        append the extrapolation level, store U, the shallow-water step, then the outer
+        dispersive iterations (the first with a new right-hand side and the solution
+        extrapolated forward in time)."""
+        self.dispersion_append()
+        self.dispersion_store()
+        self.evolve(dt, 1)
+        t = self.scalars()["time"]
+        for it in range(outer_iterations):
+            self.dispersion_solve(it > 0, it == 0, t)
 
     def set_forcing_patch(self, start_time, end_time, i0, i1, j0, j1, k0, k1, work):
         """forcing_patch_type applied by apply_forcing_patch (compiled reference only):

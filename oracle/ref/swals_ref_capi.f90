@@ -206,6 +206,46 @@ contains
         pool(h)%record_max_U = (flag /= 0)
     end subroutine
 
+    ! Dispersion (linear_dispersive_solver_mod.f90) driven as multidomain_mod.f90:1245-1410
+    ! drives it for one domain: the caller appends the extrapolation level, stores U,
+    ! steps, then solves.
+    subroutine ref_domain_dispersion_setup(h, td1, td2, niter) bind(C, name='ref_domain_dispersion_setup')
+        integer(c_int), value :: h, niter
+        real(c_double), value :: td1, td2
+        integer :: nx, ny
+        nx = pool(h)%nx(1); ny = pool(h)%nx(2)
+        pool(h)%use_dispersion = .true.
+        ! the allocations of setup_dispersive_solver (linear_dispersive_solver_mod.f90:144-179)
+        ! made here: its OpenMP first-touch loop takes its bound from a dummy argument that
+        ! DEFAULT(PRIVATE) makes private, which faults when compiled with amdflang
+        pool(h)%ds%is_staggered_grid = pool(h)%is_staggered_grid
+        allocate(pool(h)%ds%RHS(nx, ny, UH:VH), pool(h)%ds%last_U(nx, ny, UH:ELV), pool(h)%ds%Ax(nx, ny, UH:VH))
+        pool(h)%ds%RHS = 0.0_dp
+        pool(h)%ds%last_U = 0.0_dp
+        pool(h)%ds%Ax = 0.0_dp
+        call pool(h)%ds%qet%setup(nx, ny, 2)
+        pool(h)%ds%td1 = td1
+        pool(h)%ds%td2 = td2
+        pool(h)%ds%tridiagonal_inner_iter = niter
+    end subroutine
+
+    subroutine ref_domain_dispersion_store(h) bind(C, name='ref_domain_dispersion_store')
+        integer(c_int), value :: h
+        call pool(h)%copy_U_to_dispersive_last_timestep()
+    end subroutine
+
+    subroutine ref_domain_dispersion_append(h) bind(C, name='ref_domain_dispersion_append')
+        integer(c_int), value :: h
+        call pool(h)%ds%qet%append_value_and_time_for_quadratic_extrapolation(pool(h)%time, pool(h)%U(:,:,UH:VH))
+    end subroutine
+
+    subroutine ref_domain_dispersion_solve(h, rhs_up, estimate, ftime) bind(C, name='ref_domain_dispersion_solve')
+        integer(c_int), value :: h, rhs_up, estimate
+        real(c_double), value :: ftime
+        call pool(h)%solve_dispersive_terms(rhs_is_up_to_date=(rhs_up /= 0), &
+            estimate_solution_forward_in_time=(estimate /= 0), forward_time=ftime)
+    end subroutine
+
     subroutine ref_domain_set_time(h, t) bind(C, name='ref_domain_set_time')
         integer(c_int), value :: h
         real(c_double), value :: t

@@ -53,6 +53,11 @@ struct or_domain {
      * outside state comes from the sine stage function (else msl_linear at rest) */
     int bc_on, bc_fun;
     double bc_amp, bc_period, bc_msl;
+    /* dispersion (staggered grids): RHS (UH, VH), last_U (UH, VH, ELV), Ax (UH, VH),
+     * the three extrapolation levels (yn2, yn1, y0; UH and VH each) and their times */
+    int disp_on, disp_iter;
+    double td1, td2;
+    double *dRHS, *dlast, *dAx, *qy[3], qt[3];
 };
 
 #define IX(d, i, j) ((size_t)((j) - 1) * (size_t)(d)->nx + (size_t)((i) - 1))
@@ -1133,6 +1138,251 @@ static void leapfrog_nonlinear_step(or_domain *d, double dt)
 #undef WET
     d->time = d->time + 0.5 * dt;
     free(wk);
+}
+
+/* ---- Dispersion on the staggered grid: linear_dispersive_solver_mod.f90 (without
+ * DISPERSIVE_PEREGRINE_IN_FLUX_FORM, the default) and quadratic_extrapolation_mod.f90 ---- */
+
+/* quadratic_extrapolation_mod.f90:17 (missing_time) */
+static const double QET_MISSING = DBL_MAX * 0.987654321;
+
+void or_domain_dispersion_setup(or_domain *d, double td1, double td2, int niter)
+{
+    const size_t n = (size_t)d->nx * d->ny;
+    d->disp_on = 1; d->td1 = td1; d->td2 = td2; d->disp_iter = niter;
+    d->dRHS = calloc(2 * n, sizeof(double));
+    d->dlast = calloc(3 * n, sizeof(double));
+    d->dAx = calloc(2 * n, sizeof(double));
+    for (int k = 0; k < 3; k++) {
+        d->qy[k] = calloc(2 * n, sizeof(double));
+        d->qt[k] = QET_MISSING;
+    }
+}
+
+/* store_last_U (linear_dispersive_solver_mod.f90:127-142): UH, VH, ELV */
+void or_domain_dispersion_store(or_domain *d)
+{
+    const size_t n = (size_t)d->nx * d->ny;
+    memcpy(d->dlast, P(d, UH), 3 * n * sizeof(double));
+}
+
+/* append_value_and_time_for_quadratic_extrapolation (quadratic_extrapolation_mod.f90:52-78):
+ * the levels shift down one, the newest is U(UH:VH) at the domain time */
+void or_domain_dispersion_append(or_domain *d)
+{
+    const size_t n = (size_t)d->nx * d->ny;
+    double *old = d->qy[0];
+    d->qy[0] = d->qy[1]; d->qt[0] = d->qt[1];
+    d->qy[1] = d->qy[2]; d->qt[1] = d->qt[2];
+    d->qy[2] = old;
+    memcpy(d->qy[2], P(d, UH), 2 * n * sizeof(double));
+    d->qt[2] = d->time;
+}
+
+/* LAPACK dgtsv for one right-hand side (the reference vendors it:
+ * util/lapack_tridiagonal_solves.f:462-673): Gaussian elimination with partial
+ * pivoting by rows; dl[k], du[k] are the sub/super-diagonal entries of rows k+1 / k. */
+static void or_dgtsv(int n, double *dl, double *dg, double *du, double *b)
+{
+    double fact, temp;
+    for (int i = 0; i < n - 2; i++) {
+        if (fabs(dg[i]) >= fabs(dl[i])) {
+            if (dg[i] != 0.0) {
+                fact = dl[i] / dg[i];
+                dg[i + 1] = dg[i + 1] - fact * du[i];
+                b[i + 1] = b[i + 1] - fact * b[i];
+            } else {
+                return;
+            }
+            dl[i] = 0.0;
+        } else {
+            fact = dg[i] / dl[i];
+            dg[i] = dl[i];
+            temp = dg[i + 1];
+            dg[i + 1] = du[i] - fact * temp;
+            dl[i] = du[i + 1];
+            du[i + 1] = -fact * dl[i];
+            du[i] = temp;
+            temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - fact * b[i + 1];
+        }
+    }
+    if (n > 1) {
+        const int i = n - 2;
+        if (fabs(dg[i]) >= fabs(dl[i])) {
+            if (dg[i] != 0.0) {
+                fact = dl[i] / dg[i];
+                dg[i + 1] = dg[i + 1] - fact * du[i];
+                b[i + 1] = b[i + 1] - fact * b[i];
+            } else {
+                return;
+            }
+        } else {
+            fact = dg[i] / dl[i];
+            dg[i] = dl[i];
+            temp = dg[i + 1];
+            dg[i + 1] = du[i] - fact * temp;
+            du[i] = temp;
+            temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - fact * b[i + 1];
+        }
+    }
+    if (dg[n - 1] == 0.0) return;
+    b[n - 1] = b[n - 1] / dg[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dg[n - 2];
+    for (int i = n - 3; i >= 0; i--) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / dg[i];
+}
+
+/* depth at a UH / VH point for the dispersive terms: msl-relative, zero unless both
+ * cells are below msl_linear (:209-210, :266-267) */
+static double disp_depth(double msl, double e1, double e2)
+{
+    return (e1 < msl && e2 < msl) ? 0.5 * (msl - e1 + msl - e2) : 0.0;
+}
+
+static double disp_premult(const or_domain *d, double dep, double r)
+{
+    return dep * dep / (3.0 * r) * dmin(1.0, dmax(0.0, (dep - d->td2) / (d->td1 - d->td2)));
+}
+
+/* linear_dispersive_staggered_matmult (:182-417) of (elev, uh, vh) into Ax */
+static void disp_matmult(or_domain *d, const double *elev, const double *uh, const double *vh, double *axu,
+                         double *axv)
+{
+    const int nx = d->nx, ny = d->ny;
+    const double msl = d->msl_linear, dlon = d->dx[0], dlat = d->dx[1];
+#define E2(a, i, j) ((a)[IX(d, i, j)])
+    for (int j = 2; j <= ny - 1; j++) {
+        const double Rdlon = 0.5 * (d->dbe[j] + d->dbe[j - 1]);
+        const double Rdlat = Rdlon * dlat / dlon;
+        const double cjph = d->dbe[j] * dlat / (dlon * d->dle[0]);
+        const double cjmh = d->dbe[j - 1] * dlat / (dlon * d->dle[0]);
+        for (int i = 2; i <= nx - 1; i++) {
+            const double dep = disp_depth(msl, E2(elev, i, j), E2(elev, i + 1, j));
+            const double pm = disp_premult(d, dep, Rdlon);
+            E2(axu, i, j) = pm * (1.0 / Rdlon * (E2(uh, i + 1, j) - 2.0 * E2(uh, i, j) + E2(uh, i - 1, j)) +
+                                  1.0 / Rdlat * ((cjph * E2(vh, i + 1, j) - cjmh * E2(vh, i + 1, j - 1)) -
+                                                 (cjph * E2(vh, i, j) - cjmh * E2(vh, i, j - 1))));
+        }
+    }
+    for (int j = 2; j <= ny - 1; j++) {
+        const double r_dlat = d->dle[0];
+        const double Rdlon_jp1 = 0.5 * (d->dbe[j + 1] + d->dbe[j]);
+        const double Rdlon_j = 0.5 * (d->dbe[j] + d->dbe[j - 1]);
+        const double Rdlat_jp1 = Rdlon_jp1 * dlat / dlon;
+        const double Rdlat_j = Rdlon_j * dlat / dlon;
+        const double cjp1h = d->dbe[j + 1] * dlat / (d->dle[0] * dlon);
+        const double cjph = d->dbe[j] * dlat / (d->dle[0] * dlon);
+        const double cjmh = d->dbe[j - 1] * dlat / (d->dle[0] * dlon);
+        for (int i = 2; i <= nx - 1; i++) {
+            const double dep = disp_depth(msl, E2(elev, i, j), E2(elev, i, j + 1));
+            const double pm = disp_premult(d, dep, r_dlat);
+            E2(axv, i, j) = pm * ((1.0 / Rdlon_jp1 * (E2(uh, i, j + 1) - E2(uh, i - 1, j + 1)) -
+                                   1.0 / Rdlon_j * (E2(uh, i, j) - E2(uh, i - 1, j))) +
+                                  (1.0 / Rdlat_jp1 * (cjp1h * E2(vh, i, j + 1) - cjph * E2(vh, i, j)) -
+                                   1.0 / Rdlat_j * (cjph * E2(vh, i, j) - cjmh * E2(vh, i, j - 1))));
+        }
+    }
+}
+
+/* linear_dispersive_sweep_staggered_TRIDIAG (:663-921): every row of UH, then every
+ * column of VH, each a tridiagonal solve */
+static void disp_sweep(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    const double msl = d->msl_linear, dlon = d->dx[0], dlat = d->dx[1];
+    const double *elev = P(d, ELV);
+    double *uh = P(d, UH), *vh = P(d, VH);
+    const double *ru = d->dRHS, *rv = d->dRHS + (size_t)nx * ny;
+    const int nmax = nx > ny ? nx : ny;
+    double *dg = malloc(nmax * sizeof(double)), *lo = malloc(nmax * sizeof(double));
+    double *up = malloc(nmax * sizeof(double)), *b = malloc(nmax * sizeof(double));
+    for (int j = 2; j <= ny - 1; j++) {
+        const double Rdlon = 0.5 * (d->dbe[j] + d->dbe[j - 1]);
+        const double Rdlat = Rdlon * dlat / dlon;
+        const double cjph = d->dbe[j] * dlat / (dlon * d->dle[0]);
+        const double cjmh = d->dbe[j - 1] * dlat / (dlon * d->dle[0]);
+        const int N = nx;
+        dg[0] = 1.0; dg[N - 1] = 1.0;
+        lo[0] = 0.0; lo[N - 1] = 0.0; up[0] = 0.0; up[N - 1] = 0.0;
+        b[0] = E2(ru, 1, j); b[N - 1] = E2(ru, N, j);
+        for (int i = 2; i <= N - 1; i++) {
+            const double dep = disp_depth(msl, E2(elev, i, j), E2(elev, i + 1, j));
+            const double pm = disp_premult(d, dep, Rdlon);
+            dg[i - 1] = 1.0 + 2.0 * pm * 1.0 / Rdlon;
+            lo[i - 1] = -pm * 1.0 / Rdlon;
+            up[i - 1] = -pm * 1.0 / Rdlon;
+            b[i - 1] = E2(ru, i, j) + pm * (1.0 / Rdlat * ((cjph * E2(vh, i + 1, j) - cjmh * E2(vh, i + 1, j - 1)) -
+                                                          (cjph * E2(vh, i, j) - cjmh * E2(vh, i, j - 1))));
+        }
+        or_dgtsv(N, lo + 1, dg, up, b);
+        for (int i = 1; i <= N; i++) E2(uh, i, j) = b[i - 1];
+    }
+    for (int i = 2; i <= nx - 1; i++) {
+        const int N = ny;
+        dg[0] = 1.0; dg[N - 1] = 1.0;
+        lo[0] = 0.0; lo[N - 1] = 0.0; up[0] = 0.0; up[N - 1] = 0.0;
+        b[0] = E2(rv, i, 1); b[N - 1] = E2(rv, i, N);
+        for (int j = 2; j <= N - 1; j++) {
+            const double r_dlat = d->dle[0];
+            const double Rdlon_jp1 = 0.5 * (d->dbe[j + 1] + d->dbe[j]);
+            const double Rdlon_j = 0.5 * (d->dbe[j] + d->dbe[j - 1]);
+            const double Rdlat_jp1 = Rdlon_jp1 * dlat / dlon;
+            const double Rdlat_j = Rdlon_j * dlat / dlon;
+            const double cjp1h = d->dbe[j + 1] * dlat / (d->dle[0] * dlon);
+            const double cjph = d->dbe[j] * dlat / (d->dle[0] * dlon);
+            const double cjmh = d->dbe[j - 1] * dlat / (d->dle[0] * dlon);
+            const double dep = disp_depth(msl, E2(elev, i, j), E2(elev, i, j + 1));
+            const double pm = disp_premult(d, dep, r_dlat);
+            dg[j - 1] = 1.0 + pm * (1.0 / Rdlat_jp1 * cjph + 1.0 / Rdlat_j * cjph);
+            lo[j - 1] = -pm * 1.0 / Rdlat_j * cjmh;
+            up[j - 1] = -pm * 1.0 / Rdlat_jp1 * cjp1h;
+            b[j - 1] = E2(rv, i, j) + pm * ((1.0 / Rdlon_jp1 * (E2(uh, i, j + 1) - E2(uh, i - 1, j + 1)) -
+                                            1.0 / Rdlon_j * (E2(uh, i, j) - E2(uh, i - 1, j))));
+        }
+        or_dgtsv(N, lo + 1, dg, up, b);
+        for (int j = 1; j <= N; j++) E2(vh, i, j) = b[j - 1];
+    }
+    free(dg); free(lo); free(up); free(b);
+}
+
+/* linear_dispersive_solve_TRIDIAG (:1193-1354) as solve_dispersive_terms calls it
+ * (domain_mod.f90:4319-4343) */
+void or_domain_dispersion_solve(or_domain *d, int rhs_up, int estimate, double ftime)
+{
+    const int nx = d->nx, ny = d->ny;
+    const size_t n = (size_t)nx * ny;
+    if (!rhs_up) {
+        disp_matmult(d, d->dlast + 2 * n, d->dlast, d->dlast + n, d->dAx, d->dAx + n);
+        for (int q = 0; q < 2; q++) {
+            const double *u = P(d, UH + q), *ax = d->dAx + q * n;
+            double *r = d->dRHS + q * n;
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    const int edge = (j == 1 || j == ny || i == 1 || i == nx);
+                    E2(r, i, j) = edge ? E2(u, i, j) : E2(u, i, j) - E2(ax, i, j);
+                }
+        }
+    }
+    int niter = d->disp_iter;
+    if (estimate) {
+        if (d->qt[0] == QET_MISSING || d->qt[1] == QET_MISSING || d->qt[2] == QET_MISSING) {
+            niter = 10 * d->disp_iter;
+        } else {
+            /* extrapolate_in_time (quadratic_extrapolation_mod.f90:91-193) */
+            const double tn2 = d->qt[0] - d->qt[2], tn1 = d->qt[1] - d->qt[2], t1 = ftime - d->qt[2];
+            const double premult = 1.0 / (tn1 * (tn2 * tn2) - tn2 * (tn1 * tn1));
+            const double a = (1.0 + t1 * premult * (-(tn2 * tn2) + tn1 * tn1) + (t1 * t1) * premult * (tn2 - tn1));
+            const double bb = (t1 * premult * (tn2 * tn2) + (t1 * t1) * premult * (-tn2));
+            const double c = (t1 * premult * (-(tn1 * tn1)) + (t1 * t1) * premult * tn1);
+            double *u = P(d, UH);
+            for (size_t k = 0; k < 2 * n; k++) u[k] = d->qy[2][k] * a + d->qy[1][k] * bb + d->qy[0][k] * c;
+        }
+    }
+    for (int it = 0; it < niter; it++) disp_sweep(d);
+#undef E2
 }
 
 /* evolve_one_step: domain_mod.f90:1679-1813 (max_U_update_frequency = 1) */

@@ -46,6 +46,14 @@ void or_domain_set_flather_bc(or_domain *d, int use_fun, double amp, double peri
 void or_domain_set_record_max_U(or_domain *d, int on);
 void or_domain_get_gauges(const or_domain *d, int n, const int *ij, double *out);
 
+/* Dispersion on staggered grids (linear_dispersive_solver_mod.f90, default form):
+ * setup (td1, td2, tridiagonal_inner_iter), store_last_U, the extrapolation level
+ * append (U(UH:VH) at the domain time), and solve_dispersive_terms. */
+void or_domain_dispersion_setup(or_domain *d, double td1, double td2, int niter);
+void or_domain_dispersion_store(or_domain *d);
+void or_domain_dispersion_append(or_domain *d);
+void or_domain_dispersion_solve(or_domain *d, int rhs_up, int estimate, double ftime);
+
 /* Individual pieces, exposed for unit tests of the restatement itself. */
 void or_mc_limited_gradient(const double *u_local, const double *u_lower, const double *u_upper,
                             const double *theta, double *grad, int n);

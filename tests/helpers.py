@@ -66,6 +66,31 @@ def nonlinear_leapfrog_cases():
     }
 
 
+def dispersive_cases():
+    """Configurations for the staggered dispersive solve (linear_dispersive_solver_mod.f90):
+    a 20 m deep Cartesian channel at 25 m cells (dispersion strong: d^2 / (3 dx^2) = 0.2)
+    with a 1 m hump and land on its east side, a spherical basin, and
+    'leapfrog_nonlinear' on the channel. Each entry: (config, steps, td1, td2, outer
+    iterations)."""
+    def channel(method="linear"):
+        cfg = C.gaussian_hump_cartesian(nx=72, ny=40, dx=25.0, sigma=150.0)
+        x = np.arange(72)[None, :] * np.ones((40, 1))
+        U = np.zeros((4, 40, 72))
+        U[3] = -20.0 + 0.4 * np.maximum(x - 50.0, 0.0)
+        U[0] = np.maximum(np.exp(-((x - 25.0) ** 2) / 40.0) * 1.0, U[3])
+        cfg = dict(cfg, U=U, method=method)
+        if method != "linear":
+            cfg["manning_squared"] = np.zeros((40, 72))
+        return cfg
+
+    return {
+        "channel_linear": (lambda: channel(), 40, 0.0, -1.0, 1),
+        "channel_linear_taper": (lambda: channel(), 40, 15.0, 5.0, 2),
+        "basin_sph_linear": (lambda: open_basin(), 30, 0.0, -1.0, 1),
+        "channel_nonlinear": (lambda: channel("leapfrog_nonlinear"), 40, 0.0, -1.0, 2),
+    }
+
+
 def with_flather(cfg, sides=(1, 1, 1, 1), amp=None, period=600.0):
     """cfg with a Flather boundary (boundary_mod.f90:502-720) instead of any stage
     boundary: outside sea at rest, or the sine stage of amplitude amp."""
