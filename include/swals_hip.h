@@ -177,6 +177,24 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 
 int swh_synchronize(swh_domain *h);
 
+/* Dispersion on the staggered grid (linear_dispersive_solver_mod.f90, the default form
+ * without DISPERSIVE_PEREGRINE_IN_FLUX_FORM), at the domain procedures that
+ * multidomain_type's dispersive loop calls (multidomain_mod.f90:1245-1410):
+ *   swh_dispersion_enable       domain%use_dispersion = .true. + ds%setup (:144-179), with
+ *                               dlon, dlat = domain%dx(1:2), ds%td1, ds%td2 and
+ *                               ds%tridiagonal_inner_iter;
+ *   swh_dispersion_store_last_U domain%copy_U_to_dispersive_last_timestep (domain_mod.f90:4305);
+ *   swh_dispersion_append_extrapolation
+ *                               ds%qet%append_value_and_time_for_quadratic_extrapolation(
+ *                               domain%time, domain%U(:,:,UH:VH)) (quadratic_extrapolation_mod.f90:52);
+ *   swh_dispersion_solve        domain%solve_dispersive_terms (domain_mod.f90:4319-4343).
+ * Staggered (leapfrog) methods only. */
+int swh_dispersion_enable(swh_domain *h, double dlon, double dlat, double td1, double td2, int32_t niter);
+int swh_dispersion_store_last_U(swh_domain *h);
+int swh_dispersion_append_extrapolation(swh_domain *h);
+int swh_dispersion_solve(swh_domain *h, int32_t rhs_is_up_to_date, int32_t estimate_solution_forward_in_time,
+                         double forward_time);
+
 /* Output snapshot (the D->H staging behind write_to_output_files / write_grids,
  * domain_mod.f90:2188-2218, netcdf_util_mod.f90:437-506). swh_snapshot packs the current
  * state into a device buffer as FP64 U(nx, ny, 4) and its time, in stream order, and

@@ -773,7 +773,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
-                    h->nlw, h->d_corbe};
+                    h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork};
     for (void *p : ptrs) dfree(p);
     dfree(h->snap);
     if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
@@ -1076,6 +1076,105 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
     }
     *nsamples = h->gsamples;
     return SWH_OK;
+}
+
+int swh_dispersion_enable(swh_domain *h, double dlon, double dlat, double td1, double td2, int32_t niter)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (!h->leapfrog()) return fail(SWH_EINVAL, "dispersion is provided for the staggered (leapfrog) methods");
+    if (niter < 1 || !(dlon > 0.0) || !(dlat > 0.0)) return fail(SWH_EINVAL, "bad dispersion parameters");
+    if (!(td1 != td2)) return fail(SWH_EINVAL, "td1 must differ from td2");
+    if (!h->disp_on) {
+        SWH_HIP(hipStreamSynchronize(h->stream));
+        const size_t P = h->plane_elems() * h->esz;
+        int rc = dmalloc(&h->dlast, 3 * P);
+        if (!rc) rc = dmalloc(&h->drhs, 2 * P);
+        for (int k = 0; k < 3 && !rc; k++) rc = dmalloc(&h->dq[k], 2 * P);
+        if (!rc) rc = dmalloc(&h->dwork, 4 * (size_t)h->nx * h->ny * h->esz);
+        if (rc) return rc;
+        h->disp_on = 1;
+    }
+    h->disp_td1 = td1; h->disp_td2 = td2; h->disp_iter = niter;
+    h->disp_dlon = dlon; h->disp_dlat = dlat;
+    h->dq_n = 0;
+    return SWH_OK;
+}
+
+int swh_dispersion_store_last_U(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (!h->disp_on) return fail(SWH_ESTATE, "dispersion not enabled (swh_dispersion_enable)");
+    const size_t P = h->plane_elems() * h->esz;
+    char *d = (char *)h->dlast;
+    const int c = h->cur;
+    SWH_HIP(hipMemcpyAsync(d, h->UH[c], P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync(d + P, h->VH[c], P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync(d + 2 * P, h->E, P, hipMemcpyDeviceToDevice, h->stream));
+    return SWH_OK;
+}
+
+int swh_dispersion_append_extrapolation(swh_domain *h)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (!h->disp_on) return fail(SWH_ESTATE, "dispersion not enabled (swh_dispersion_enable)");
+    void *old = h->dq[0];
+    h->dq[0] = h->dq[1]; h->dqt[0] = h->dqt[1];
+    h->dq[1] = h->dq[2]; h->dqt[1] = h->dqt[2];
+    h->dq[2] = old; h->dqt[2] = h->time;
+    if (h->dq_n < 3) h->dq_n++;
+    const size_t P = h->plane_elems() * h->esz;
+    const int c = h->cur;
+    SWH_HIP(hipMemcpyAsync(h->dq[2], h->UH[c], P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync((char *)h->dq[2] + P, h->VH[c], P, hipMemcpyDeviceToDevice, h->stream));
+    return SWH_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static int dispersion_solve(swh_domain *h, int rhs_up, int estimate, double ftime)
+{
+    const size_t P = h->plane_elems();
+    const int c = h->cur;
+    DispArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch;
+    a.UH = (T *)h->UH[c]; a.VH = (T *)h->VH[c]; a.E = (const T *)h->E;
+    a.LU = (const T *)h->dlast; a.LV = a.LU + P; a.LE = a.LU + 2 * P;
+    a.RU = (T *)h->drhs; a.RV = a.RU + P;
+    a.Y0 = (const T *)h->dq[2]; a.Y1 = (const T *)h->dq[1]; a.Y2 = (const T *)h->dq[0];
+    a.W = (T *)h->dwork;
+    a.dbe = h->d_dbe; a.dle = h->d_dle;
+    a.msl = (T)h->msl; a.dlon = (T)h->disp_dlon; a.dlat = (T)h->disp_dlat;
+    a.td1 = (T)h->disp_td1; a.td2 = (T)h->disp_td2;
+    int niter = h->disp_iter, extrap = 0;
+    double ca = 0.0, cb = 0.0, cc = 0.0;
+    if (estimate) {
+        if (h->dq_n < 3) {
+            niter = 10 * h->disp_iter;
+        } else {
+            // extrapolate_in_time (quadratic_extrapolation_mod.f90:171-181)
+            const double tn2 = h->dqt[0] - h->dqt[2], tn1 = h->dqt[1] - h->dqt[2], t1 = ftime - h->dqt[2];
+            const double pm = 1.0 / (tn1 * (tn2 * tn2) - tn2 * (tn1 * tn1));
+            ca = (1.0 + t1 * pm * (-(tn2 * tn2) + tn1 * tn1) + (t1 * t1) * pm * (tn2 - tn1));
+            cb = (t1 * pm * (tn2 * tn2) + (t1 * t1) * pm * (-tn2));
+            cc = (t1 * pm * (-(tn1 * tn1)) + (t1 * t1) * pm * tn1);
+            extrap = 1;
+        }
+    }
+    SWH_HIP(launch_dispersion<T>(a, rhs_up ? 0 : 1, extrap, ca, cb, cc, niter, h->stream));
+    return SWH_OK;
+}
+
+extern "C" {
+
+int swh_dispersion_solve(swh_domain *h, int32_t rhs_is_up_to_date, int32_t estimate_solution_forward_in_time,
+                         double forward_time)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (!h->disp_on) return fail(SWH_ESTATE, "dispersion not enabled (swh_dispersion_enable)");
+    return h->prec == SWH_PREC_FP64
+               ? dispersion_solve<double>(h, rhs_is_up_to_date, estimate_solution_forward_in_time, forward_time)
+               : dispersion_solve<float>(h, rhs_is_up_to_date, estimate_solution_forward_in_time, forward_time);
 }
 
 int swh_snapshot(swh_domain *h)

@@ -175,6 +175,26 @@ hipError_t launch_nonlinear_depth0(const NLArgs<T> &a, hipStream_t s);
 template <typename T>
 hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, bool fused, hipStream_t s);
 
+// Dispersive solve on the staggered grid (swh_dispersion.hip). Planes of pitch * ny
+// except W (4 * nx * ny work for the tridiagonal systems).
+template <typename T>
+struct DispArgs {
+    int nx, ny, pitch;
+    T *UH, *VH;                  // current momenta (updated)
+    const T *E;                  // current elevation
+    const T *LU, *LV, *LE;       // UH, VH, elevation stored before the shallow-water step
+    T *RU, *RV;                  // right-hand side
+    const T *Y0, *Y1, *Y2;       // extrapolation levels, newest first (UH plane, then VH plane)
+    T *W;
+    const double *dbe, *dle;
+    T msl, dlon, dlat, td1, td2;
+};
+// make_rhs: explicit part and right-hand side; extrapolate: U(UH:VH) = ca Y0 + cb Y1 +
+// cc Y2; then niter tridiagonal iterations (rows of UH, then columns of VH)
+template <typename T>
+hipError_t launch_dispersion(const DispArgs<T> &a, int make_rhs, int extrapolate, double ca, double cb, double cc,
+                             int niter, hipStream_t s);
+
 // Dense FP64 copy of the state [S (- C), UH, VH, E] as out[(var * ny + j) * nx + i]
 // (the bytes of Fortran U(nx, ny, 4)), with out[4 * nx * ny] = the time: sc->time when
 // sc is given, else t_host.

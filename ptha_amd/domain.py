@@ -234,6 +234,34 @@ class Domain:
         B.check(self._lib.swh_stationary_timestep_max(self._h, ctypes.byref(v)), "swh_stationary_timestep_max")
         return v.value
 
+    # -- dispersion (linear_dispersive_solver_mod.f90; staggered methods) --
+    def dispersion_enable(self, td1=0.0, td2=-1.0, niter=2):
+        B.check(self._lib.swh_dispersion_enable(self._h, float(self.dx[0]), float(self.dx[1]), float(td1), float(td2),
+                                                int(niter)), "swh_dispersion_enable")
+
+    def dispersion_store(self):
+        B.check(self._lib.swh_dispersion_store_last_U(self._h), "swh_dispersion_store_last_U")
+
+    def dispersion_append(self):
+        B.check(self._lib.swh_dispersion_append_extrapolation(self._h), "swh_dispersion_append_extrapolation")
+
+    def dispersion_solve(self, rhs_is_up_to_date, estimate_solution_forward_in_time, forward_time):
+        B.check(self._lib.swh_dispersion_solve(self._h, int(bool(rhs_is_up_to_date)),
+                                               int(bool(estimate_solution_forward_in_time)), float(forward_time)),
+                "swh_dispersion_solve")
+
+    def dispersive_step(self, dt, outer_iterations=1):
+        """One step of a dispersive domain as multidomain_mod.f90:1245-1410 runs it: the
+        extrapolation level appended, U stored, the shallow-water step, then the outer
+        dispersive iterations (the first with a new right-hand side and the solution
+        extrapolated forward in time)."""
+        self.dispersion_append()
+        self.dispersion_store()
+        self.evolve_one_step(dt)
+        t = self.time
+        for it in range(outer_iterations):
+            self.dispersion_solve(it > 0, it == 0, t)
+
     def snapshot(self):
         """Pack the current state and time on the device for output (swh_snapshot) and
         return at once; fetch it later with snapshot_download while stepping goes on."""

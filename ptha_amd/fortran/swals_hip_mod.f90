@@ -73,7 +73,8 @@ module swals_hip_mod
 
     public :: swh_last_error, swh_domain_create, swh_domain_destroy, swh_upload_U, swh_download_U, &
         swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_set_flather_boundary, &
-        swh_set_forcing_patch, swh_clear_forcing_patch, swh_evolve_one_step, &
+        swh_set_forcing_patch, swh_clear_forcing_patch, swh_evolve_one_step, swh_dispersion_enable, &
+        swh_dispersion_store_last_U, swh_dispersion_append_extrapolation, swh_dispersion_solve, &
         swh_evolve_steps, swh_get_scalars, swh_volume_interior, swh_stationary_timestep_max, &
         swh_download_max_U, swh_gauges_set, swh_gauges_sample, swh_gauges_download, swh_synchronize, &
         swh_domain_set_launch, swh_check
@@ -142,6 +143,30 @@ module swals_hip_mod
             integer(c_int32_t), intent(in) :: sides(4)
             integer(c_int32_t), value :: use_stage_function
             real(c_double), value :: amp, period, msl
+        end function
+        ! dispersion (linear_dispersive_solver_mod.f90), staggered methods
+        integer(c_int) function swh_dispersion_enable(h, dlon, dlat, td1, td2, niter) &
+                bind(C, name='swh_dispersion_enable')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            real(c_double), value :: dlon, dlat, td1, td2
+            integer(c_int32_t), value :: niter
+        end function
+        integer(c_int) function swh_dispersion_store_last_U(h) bind(C, name='swh_dispersion_store_last_U')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function swh_dispersion_append_extrapolation(h) &
+                bind(C, name='swh_dispersion_append_extrapolation')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function swh_dispersion_solve(h, rhs_is_up_to_date, estimate_forward, forward_time) &
+                bind(C, name='swh_dispersion_solve')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: rhs_is_up_to_date, estimate_forward
+            real(c_double), value :: forward_time
         end function
         integer(c_int) function swh_evolve_one_step(h, timestep) bind(C, name='swh_evolve_one_step')
             import c_int, c_ptr, c_double
