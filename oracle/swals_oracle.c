@@ -753,13 +753,15 @@ static void update_U(or_domain *d, double dt)
     double *u = d->vel, *v = d->vel + n;
     d->dt_last_update = dt;
     double dt_gravity = dt * GRAVITY;
-    for (int j = 1; j <= ny; j++) {
+    /* with dispersion the edge cells are not updated (update_U_DE1_inner_include.f90:19-31) */
+    const int e = d->disp_on ? 1 : 0;
+    for (int j = 1 + e; j <= ny - e; j++) {
         double inv_cell_area_dt = dt / d->area[j - 1];
-        for (int i = 1; i <= nx; i++)
+        for (int i = 1 + e; i <= nx - e; i++)
             for (int kk = 0; kk < 3; kk++)
                 A(d, kk, i, j) = A(d, kk, i, j) - inv_cell_area_dt * ((FNS(i, j + 1, kk) - FNS(i, j, kk)) +
                                                                       (FEW(i + 1, j, kk) - FEW(i, j, kk)));
-        for (int i = 1; i <= nx; i++) {
+        for (int i = 1 + e; i <= nx - e; i++) {
             size_t k = IX(d, i, j);
             double depth = d->depth[k];
             if (A(d, STG, i, j) <= (A(d, ELV, i, j) + MIN_DEPTH)) {
@@ -857,6 +859,8 @@ static void midpoint_step(or_domain *d, double timestep)
     d->max_dt = dt_first_step;
     if (timestep > 0.0) dt_first_step = timestep;
     update_U(d, dt_first_step * 0.5);
+    /* dispersion in the predictor (timestepping_alternatives_include.f90:286-297) */
+    if (d->disp_on) or_domain_dispersion_solve(d, 0, 1, backup_time + dt_first_step * 0.5);
     compute_fluxes(d);
     memcpy(d->U, d->backup_U, n3 * sizeof(double));
     d->time = backup_time;
@@ -1348,14 +1352,126 @@ static void disp_sweep(or_domain *d)
     free(dg); free(lo); free(up); free(b);
 }
 
+/* cell-centred depth for the dispersive terms: msl-relative where the cell and its two
+ * neighbours along the line are all below msl_linear (:451, :493) */
+static double disp_depth_cc(double msl, double em, double e0, double ep)
+{
+    return (em < msl && e0 < msl && ep < msl) ? (msl - e0) : 0.0;
+}
+
+/* linear_dispersive_cellcentred_matmult (:419-660) */
+static void disp_matmult_cc(or_domain *d, const double *elev, const double *uh, const double *vh, double *axu,
+                            double *axv)
+{
+    const int nx = d->nx, ny = d->ny;
+    const double msl = d->msl_linear, dlon = d->dx[0], dlat = d->dx[1];
+    for (int j = 2; j <= ny - 1; j++) {
+        const double Rdlon = 0.5 * (d->dbe[j] + d->dbe[j - 1]);
+        const double Rdlat = Rdlon * dlat / dlon;
+        const double cjp1 = 0.5 * (d->dbe[j + 1] + d->dbe[j]) * dlat / (dlon * d->dle[0]);
+        const double cjm1 = 0.5 * (d->dbe[j - 1] + d->dbe[j - 2]) * dlat / (dlon * d->dle[0]);
+        for (int i = 2; i <= nx - 1; i++) {
+            const double dep = disp_depth_cc(msl, E2(elev, i - 1, j), E2(elev, i, j), E2(elev, i + 1, j));
+            const double pm = disp_premult(d, dep, Rdlon);
+            E2(axu, i, j) = pm * (1.0 / Rdlon * (E2(uh, i + 1, j) - 2.0 * E2(uh, i, j) + E2(uh, i - 1, j)) +
+                                  1.0 / Rdlat * ((cjp1 * E2(vh, i + 1, j + 1) - cjm1 * E2(vh, i + 1, j - 1)) * 0.25 -
+                                                 (cjp1 * E2(vh, i - 1, j + 1) - cjm1 * E2(vh, i - 1, j - 1)) * 0.25));
+        }
+    }
+    for (int j = 2; j <= ny - 1; j++) {
+        const double r_dlat = d->dle[0];
+        const double Rdlon_jph = d->dbe[j], Rdlon_jmh = d->dbe[j - 1];
+        const double Rdlat_jph = Rdlon_jph * dlat / dlon, Rdlat_jmh = Rdlon_jmh * dlat / dlon;
+        const double cjp1 = 0.5 * (d->dbe[j + 1] + d->dbe[j]) * dlat / (d->dle[0] * dlon);
+        const double cj = 0.5 * (d->dbe[j] + d->dbe[j - 1]) * dlat / (d->dle[0] * dlon);
+        const double cjm1 = 0.5 * (d->dbe[j - 1] + d->dbe[j - 2]) * dlat / (d->dle[0] * dlon);
+        for (int i = 2; i <= nx - 1; i++) {
+            const double dep = disp_depth_cc(msl, E2(elev, i, j - 1), E2(elev, i, j), E2(elev, i, j + 1));
+            const double pm = disp_premult(d, dep, r_dlat);
+            E2(axv, i, j) =
+                pm * ((1.0 / Rdlon_jph *
+                           (E2(uh, i + 1, j + 1) - E2(uh, i - 1, j + 1) + E2(uh, i + 1, j) - E2(uh, i - 1, j)) * 0.25 -
+                       1.0 / Rdlon_jmh *
+                           (E2(uh, i + 1, j - 1) - E2(uh, i - 1, j - 1) + E2(uh, i + 1, j) - E2(uh, i - 1, j)) * 0.25) +
+                      (1.0 / Rdlat_jph * (cjp1 * E2(vh, i, j + 1) - cj * E2(vh, i, j)) -
+                       1.0 / Rdlat_jmh * (cj * E2(vh, i, j) - cjm1 * E2(vh, i, j - 1))));
+        }
+    }
+}
+
+/* linear_dispersive_sweep_cellcentred_TRIDIAG (:923-1191) */
+static void disp_sweep_cc(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    const double msl = d->msl_linear, dlon = d->dx[0], dlat = d->dx[1];
+    const double *elev = P(d, ELV);
+    double *uh = P(d, UH), *vh = P(d, VH);
+    const double *ru = d->dRHS, *rv = d->dRHS + (size_t)nx * ny;
+    const int nmax = nx > ny ? nx : ny;
+    double *dg = malloc(nmax * sizeof(double)), *lo = malloc(nmax * sizeof(double));
+    double *up = malloc(nmax * sizeof(double)), *b = malloc(nmax * sizeof(double));
+    for (int j = 2; j <= ny - 1; j++) {
+        const double Rdlon = 0.5 * (d->dbe[j] + d->dbe[j - 1]);
+        const double Rdlat = Rdlon * dlat / dlon;
+        const double cjp1 = 0.5 * (d->dbe[j + 1] + d->dbe[j]) * dlat / (dlon * d->dle[0]);
+        const double cjm1 = 0.5 * (d->dbe[j - 1] + d->dbe[j - 2]) * dlat / (dlon * d->dle[0]);
+        const int N = nx;
+        dg[0] = 1.0; dg[N - 1] = 1.0;
+        lo[0] = 0.0; lo[N - 1] = 0.0; up[0] = 0.0; up[N - 1] = 0.0;
+        b[0] = E2(ru, 1, j); b[N - 1] = E2(ru, N, j);
+        for (int i = 2; i <= N - 1; i++) {
+            const double dep = disp_depth_cc(msl, E2(elev, i - 1, j), E2(elev, i, j), E2(elev, i + 1, j));
+            const double pm = disp_premult(d, dep, Rdlon);
+            dg[i - 1] = 1.0 + 2.0 * pm * 1.0 / Rdlon;
+            lo[i - 1] = -pm * 1.0 / Rdlon;
+            up[i - 1] = -pm * 1.0 / Rdlon;
+            b[i - 1] = E2(ru, i, j) + pm * (1.0 / Rdlat * ((cjp1 * E2(vh, i + 1, j + 1) - cjm1 * E2(vh, i + 1, j - 1)) * 0.25 -
+                                                          (cjp1 * E2(vh, i - 1, j + 1) - cjm1 * E2(vh, i - 1, j - 1)) * 0.25));
+        }
+        or_dgtsv(N, lo + 1, dg, up, b);
+        for (int i = 1; i <= N; i++) E2(uh, i, j) = b[i - 1];
+    }
+    for (int i = 2; i <= nx - 1; i++) {
+        const int N = ny;
+        dg[0] = 1.0; dg[N - 1] = 1.0;
+        lo[0] = 0.0; lo[N - 1] = 0.0; up[0] = 0.0; up[N - 1] = 0.0;
+        b[0] = E2(rv, i, 1); b[N - 1] = E2(rv, i, N);
+        for (int j = 2; j <= N - 1; j++) {
+            const double r_dlat = d->dle[0];
+            const double Rdlon_jph = d->dbe[j], Rdlon_jmh = d->dbe[j - 1];
+            const double Rdlat_jph = Rdlon_jph * dlat / dlon, Rdlat_jmh = Rdlon_jmh * dlat / dlon;
+            const double cjp1 = 0.5 * (d->dbe[j + 1] + d->dbe[j]) * dlat / (d->dle[0] * dlon);
+            const double cj = 0.5 * (d->dbe[j] + d->dbe[j - 1]) * dlat / (d->dle[0] * dlon);
+            const double cjm1 = 0.5 * (d->dbe[j - 1] + d->dbe[j - 2]) * dlat / (d->dle[0] * dlon);
+            const double dep = disp_depth_cc(msl, E2(elev, i, j - 1), E2(elev, i, j), E2(elev, i, j + 1));
+            const double pm = disp_premult(d, dep, r_dlat);
+            dg[j - 1] = 1.0 + pm * (1.0 / Rdlat_jph * cj + 1.0 / Rdlat_jmh * cj);
+            lo[j - 1] = -pm * 1.0 / Rdlat_jmh * cjm1;
+            up[j - 1] = -pm * 1.0 / Rdlat_jph * cjp1;
+            b[j - 1] = E2(rv, i, j) +
+                       pm * ((1.0 / Rdlon_jph *
+                                  (E2(uh, i + 1, j + 1) - E2(uh, i - 1, j + 1) + E2(uh, i + 1, j) - E2(uh, i - 1, j)) * 0.25 -
+                              1.0 / Rdlon_jmh *
+                                  (E2(uh, i + 1, j - 1) - E2(uh, i - 1, j - 1) + E2(uh, i + 1, j) - E2(uh, i - 1, j)) * 0.25));
+        }
+        or_dgtsv(N, lo + 1, dg, up, b);
+        for (int j = 1; j <= N; j++) E2(vh, i, j) = b[j - 1];
+    }
+    free(dg); free(lo); free(up); free(b);
+}
+
 /* linear_dispersive_solve_TRIDIAG (:1193-1354) as solve_dispersive_terms calls it
  * (domain_mod.f90:4319-4343) */
 void or_domain_dispersion_solve(or_domain *d, int rhs_up, int estimate, double ftime)
 {
     const int nx = d->nx, ny = d->ny;
     const size_t n = (size_t)nx * ny;
+    const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF || d->method == OR_LEAPFROG_NONLINEAR);
     if (!rhs_up) {
-        disp_matmult(d, d->dlast + 2 * n, d->dlast, d->dlast + n, d->dAx, d->dAx + n);
+        if (stag)
+            disp_matmult(d, d->dlast + 2 * n, d->dlast, d->dlast + n, d->dAx, d->dAx + n);
+        else
+            disp_matmult_cc(d, d->dlast + 2 * n, d->dlast, d->dlast + n, d->dAx, d->dAx + n);
         for (int q = 0; q < 2; q++) {
             const double *u = P(d, UH + q), *ax = d->dAx + q * n;
             double *r = d->dRHS + q * n;
@@ -1381,7 +1497,10 @@ void or_domain_dispersion_solve(or_domain *d, int rhs_up, int estimate, double f
             for (size_t k = 0; k < 2 * n; k++) u[k] = d->qy[2][k] * a + d->qy[1][k] * bb + d->qy[0][k] * c;
         }
     }
-    for (int it = 0; it < niter; it++) disp_sweep(d);
+    for (int it = 0; it < niter; it++) {
+        if (stag) disp_sweep(d);
+        else disp_sweep_cc(d);
+    }
 #undef E2
 }
 

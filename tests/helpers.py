@@ -69,8 +69,9 @@ def nonlinear_leapfrog_cases():
 def dispersive_cases():
     """Configurations for the staggered dispersive solve (linear_dispersive_solver_mod.f90):
     a 20 m deep Cartesian channel at 25 m cells (dispersion strong: d^2 / (3 dx^2) = 0.2)
-    with a 1 m hump and land on its east side, a spherical basin, and
-    'leapfrog_nonlinear' on the channel. Each entry: (config, steps, td1, td2, outer
+    with a 1 m hump and land on its east side, a spherical basin, 'leapfrog_nonlinear'
+    on the channel, and the cell-centred form with 'midpoint' (dispersion also in its
+    predictor) on the channel. Each entry: (config, steps, td1, td2, outer
     iterations)."""
     def channel(method="linear"):
         cfg = C.gaussian_hump_cartesian(nx=72, ny=40, dx=25.0, sigma=150.0)
@@ -88,6 +89,8 @@ def dispersive_cases():
         "channel_linear_taper": (lambda: channel(), 40, 15.0, 5.0, 2),
         "basin_sph_linear": (lambda: open_basin(), 30, 0.0, -1.0, 1),
         "channel_nonlinear": (lambda: channel("leapfrog_nonlinear"), 40, 0.0, -1.0, 2),
+        "channel_midpoint": (lambda: channel("midpoint"), 40, 0.0, -1.0, 1),
+        "channel_midpoint_taper": (lambda: channel("midpoint"), 30, 15.0, 5.0, 2),
     }
 
 
