@@ -177,8 +177,9 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 
 int swh_synchronize(swh_domain *h);
 
-/* Dispersion on the staggered grid (linear_dispersive_solver_mod.f90, the default form
- * without DISPERSIVE_PEREGRINE_IN_FLUX_FORM), at the domain procedures that
+/* Dispersion (linear_dispersive_solver_mod.f90, the default form without
+ * DISPERSIVE_PEREGRINE_IN_FLUX_FORM; staggered form for the leapfrog methods, cell-centred
+ * for 'midpoint', 'rk2', 'euler'), at the domain procedures that
  * multidomain_type's dispersive loop calls (multidomain_mod.f90:1245-1410):
  *   swh_dispersion_enable       domain%use_dispersion = .true. + ds%setup (:144-179), with
  *                               dlon, dlat = domain%dx(1:2), ds%td1, ds%td2 and
@@ -188,7 +189,8 @@ int swh_synchronize(swh_domain *h);
  *                               ds%qet%append_value_and_time_for_quadratic_extrapolation(
  *                               domain%time, domain%U(:,:,UH:VH)) (quadratic_extrapolation_mod.f90:52);
  *   swh_dispersion_solve        domain%solve_dispersive_terms (domain_mod.f90:4319-4343).
- * Staggered (leapfrog) methods only. */
+ * Once enabled, 'midpoint' also solves in its predictor (include.f90:286-297) and update_U
+ * leaves the edge cells (domain_mod_update_U_DE1_inner_include.f90:19-31). Not 'rk2n'. */
 int swh_dispersion_enable(swh_domain *h, double dlon, double dlat, double td1, double td2, int32_t niter);
 int swh_dispersion_store_last_U(swh_domain *h);
 int swh_dispersion_append_extrapolation(swh_domain *h);

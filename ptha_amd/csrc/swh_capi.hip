@@ -464,6 +464,7 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
     a.fric_power = (h->friction_type == 1) ? -2.0 : -7.0 / 3.0;
     a.spherical = h->spherical; a.coriolis = h->coriolis; a.n_ext = h->n_ext;
     a.sc = h->sc;
+    a.skip_edges = h->disp_on;
     return a;
 }
 
@@ -502,6 +503,34 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
 
 // The finite-volume schemes (domain_mod_timestepping_alternatives_include.f90): the
 // state at the start of the step stays in buffer A (backup_U), the result ends in B.
+template <typename T>
+int dispersion_solve(swh_domain *h, int buf, int rhs_up, int estimate, double ftime, bool ftime_on_device)
+{
+    const size_t P = h->plane_elems();
+    DispArgs<T> a;
+    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch;
+    a.UH = (T *)h->UH[buf]; a.VH = (T *)h->VH[buf]; a.E = (const T *)h->E;
+    a.LU = (const T *)h->dlast; a.LV = a.LU + P; a.LE = a.LU + 2 * P;
+    a.RU = (T *)h->drhs; a.RV = a.RU + P;
+    a.Y0 = (const T *)h->dq[2]; a.Y1 = (const T *)h->dq[1]; a.Y2 = (const T *)h->dq[0];
+    a.qt = h->d_qt;
+    a.W = (T *)h->dwork;
+    a.dbe = h->d_dbe; a.dle = h->d_dle;
+    a.msl = (T)h->msl; a.dlon = (T)h->disp_dlon; a.dlat = (T)h->disp_dlat;
+    a.td1 = (T)h->disp_td1; a.td2 = (T)h->disp_td2;
+    a.cell = h->leapfrog() ? 0 : 1;
+    a.sc = ftime_on_device ? h->sc : nullptr;
+    a.t_fwd = ftime;
+    // no extrapolation before three levels: ten times the iterations instead (:1267-1291)
+    int niter = h->disp_iter, extrap = 0;
+    if (estimate) {
+        if (h->dq_n < 3) niter = 10 * h->disp_iter;
+        else extrap = 1;
+    }
+    SWH_HIP(launch_dispersion<T>(a, rhs_up ? 0 : 1, extrap, niter, h->stream));
+    return SWH_OK;
+}
+
 // A timestep dt <= 0 means "no timestep argument": the CFL step of the first flux
 // computation, decided on the device.
 template <typename T>
@@ -534,8 +563,14 @@ int fv_step(swh_domain *h, double dt)
         break;
     case SWH_METHOD_MIDPOINT:
         // one_midpoint_step (include.f90:257-336): half a step for the fluxes, then the
-        // full step from backup_U with them
+        // full step from backup_U with them; a dispersive domain solves the dispersive
+        // terms on the half-step state first (:286-297), forward time backup_time + dt/2
+        // (the device time after the half step)
         STAGE(B, A, -1, dt, 0.5, first, 0.0);
+        if (h->disp_on) {
+            rc = dispersion_solve<T>(h, B, 0, 1, 0.0, true);
+            if (rc) return rc;
+        }
         STAGE(B, B, A, dt, 1.0, acc | kStageFromTime0 | kStageFinish, dt);
         break;
     case SWH_METHOD_RK2N: {
@@ -773,7 +808,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
-                    h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork};
+                    h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt};
     for (void *p : ptrs) dfree(p);
     dfree(h->snap);
     if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
@@ -1081,7 +1116,8 @@ int swh_gauges_download(swh_domain *h, double *values, double *times, int32_t *n
 int swh_dispersion_enable(swh_domain *h, double dlon, double dlat, double td1, double td2, int32_t niter)
 {
     if (!h) return fail(SWH_EINVAL, "null argument");
-    if (!h->leapfrog()) return fail(SWH_EINVAL, "dispersion is provided for the staggered (leapfrog) methods");
+    // the methods the reference allows (domain_mod.f90:1313-1327); 'cliffs' is not built
+    if (h->method == SWH_METHOD_RK2N) return fail(SWH_EINVAL, "dispersion is not supported for 'rk2n'");
     if (niter < 1 || !(dlon > 0.0) || !(dlat > 0.0)) return fail(SWH_EINVAL, "bad dispersion parameters");
     if (!(td1 != td2)) return fail(SWH_EINVAL, "td1 must differ from td2");
     if (!h->disp_on) {
@@ -1091,6 +1127,7 @@ int swh_dispersion_enable(swh_domain *h, double dlon, double dlat, double td1, d
         if (!rc) rc = dmalloc(&h->drhs, 2 * P);
         for (int k = 0; k < 3 && !rc; k++) rc = dmalloc(&h->dq[k], 2 * P);
         if (!rc) rc = dmalloc(&h->dwork, 4 * (size_t)h->nx * h->ny * h->esz);
+        if (!rc) rc = dmalloc((void **)&h->d_qt, 3 * sizeof(double));
         if (rc) return rc;
         h->disp_on = 1;
     }
@@ -1118,10 +1155,11 @@ int swh_dispersion_append_extrapolation(swh_domain *h)
     if (!h) return fail(SWH_EINVAL, "null argument");
     if (!h->disp_on) return fail(SWH_ESTATE, "dispersion not enabled (swh_dispersion_enable)");
     void *old = h->dq[0];
-    h->dq[0] = h->dq[1]; h->dqt[0] = h->dqt[1];
-    h->dq[1] = h->dq[2]; h->dqt[1] = h->dqt[2];
-    h->dq[2] = old; h->dqt[2] = h->time;
+    h->dq[0] = h->dq[1];
+    h->dq[1] = h->dq[2];
+    h->dq[2] = old;
     if (h->dq_n < 3) h->dq_n++;
+    SWH_HIP(launch_dispersion_qt_append(h->d_qt, h->leapfrog() ? nullptr : h->sc, h->time, h->stream));
     const size_t P = h->plane_elems() * h->esz;
     const int c = h->cur;
     SWH_HIP(hipMemcpyAsync(h->dq[2], h->UH[c], P, hipMemcpyDeviceToDevice, h->stream));
@@ -1129,52 +1167,15 @@ int swh_dispersion_append_extrapolation(swh_domain *h)
     return SWH_OK;
 }
 
-}  // extern "C"
-
-template <typename T>
-static int dispersion_solve(swh_domain *h, int rhs_up, int estimate, double ftime)
-{
-    const size_t P = h->plane_elems();
-    const int c = h->cur;
-    DispArgs<T> a;
-    a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch;
-    a.UH = (T *)h->UH[c]; a.VH = (T *)h->VH[c]; a.E = (const T *)h->E;
-    a.LU = (const T *)h->dlast; a.LV = a.LU + P; a.LE = a.LU + 2 * P;
-    a.RU = (T *)h->drhs; a.RV = a.RU + P;
-    a.Y0 = (const T *)h->dq[2]; a.Y1 = (const T *)h->dq[1]; a.Y2 = (const T *)h->dq[0];
-    a.W = (T *)h->dwork;
-    a.dbe = h->d_dbe; a.dle = h->d_dle;
-    a.msl = (T)h->msl; a.dlon = (T)h->disp_dlon; a.dlat = (T)h->disp_dlat;
-    a.td1 = (T)h->disp_td1; a.td2 = (T)h->disp_td2;
-    int niter = h->disp_iter, extrap = 0;
-    double ca = 0.0, cb = 0.0, cc = 0.0;
-    if (estimate) {
-        if (h->dq_n < 3) {
-            niter = 10 * h->disp_iter;
-        } else {
-            // extrapolate_in_time (quadratic_extrapolation_mod.f90:171-181)
-            const double tn2 = h->dqt[0] - h->dqt[2], tn1 = h->dqt[1] - h->dqt[2], t1 = ftime - h->dqt[2];
-            const double pm = 1.0 / (tn1 * (tn2 * tn2) - tn2 * (tn1 * tn1));
-            ca = (1.0 + t1 * pm * (-(tn2 * tn2) + tn1 * tn1) + (t1 * t1) * pm * (tn2 - tn1));
-            cb = (t1 * pm * (tn2 * tn2) + (t1 * t1) * pm * (-tn2));
-            cc = (t1 * pm * (-(tn1 * tn1)) + (t1 * t1) * pm * tn1);
-            extrap = 1;
-        }
-    }
-    SWH_HIP(launch_dispersion<T>(a, rhs_up ? 0 : 1, extrap, ca, cb, cc, niter, h->stream));
-    return SWH_OK;
-}
-
-extern "C" {
-
 int swh_dispersion_solve(swh_domain *h, int32_t rhs_is_up_to_date, int32_t estimate_solution_forward_in_time,
                          double forward_time)
 {
     if (!h) return fail(SWH_EINVAL, "null argument");
     if (!h->disp_on) return fail(SWH_ESTATE, "dispersion not enabled (swh_dispersion_enable)");
-    return h->prec == SWH_PREC_FP64
-               ? dispersion_solve<double>(h, rhs_is_up_to_date, estimate_solution_forward_in_time, forward_time)
-               : dispersion_solve<float>(h, rhs_is_up_to_date, estimate_solution_forward_in_time, forward_time);
+    return h->prec == SWH_PREC_FP64 ? dispersion_solve<double>(h, h->cur, rhs_is_up_to_date,
+                                                               estimate_solution_forward_in_time, forward_time, false)
+                                    : dispersion_solve<float>(h, h->cur, rhs_is_up_to_date,
+                                                              estimate_solution_forward_in_time, forward_time, false);
 }
 
 int swh_snapshot(swh_domain *h)

@@ -27,12 +27,12 @@ struct swh_domain {
     void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
     double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
     double *d_bmax = nullptr;   // per-block CFL maxima of the flux pass
-    // dispersion (staggered methods): stored UH/VH/elevation, right-hand side, three
+    // dispersion (leapfrog methods, midpoint/rk2/euler): stored UH/VH/elevation, right-hand side, three
     // extrapolation levels (oldest first) with their times, tridiagonal work
     int disp_on = 0, disp_iter = 2;
     double disp_td1 = 0.0, disp_td2 = -1.0, disp_dlon = 1.0, disp_dlat = 1.0;
     void *dlast = nullptr, *drhs = nullptr, *dq[3] = {}, *dwork = nullptr;
-    double dqt[3] = {0.0, 0.0, 0.0};
+    double *d_qt = nullptr;   // level times (device: finite-volume times live there)
     int dq_n = 0;   // extrapolation levels stored (0..3)
     // output snapshot: dense FP64 [4][ny][nx] copy of the state plus its time, packed on
     // the domain stream; downloaded on a copy stream while the solver runs on

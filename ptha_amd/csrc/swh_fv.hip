@@ -418,6 +418,13 @@ __global__ void fv_update_kernel(FVArgs<T> a)
         q[1] = a.UHq[o];
         q[2] = a.VHq[o];
     }
+    if (a.skip_edges && (i == 0 || j == 0 || i == a.nx - 1 || j == a.ny - 1)) {
+        // dispersive domains: the edge cells keep their values (update_U_DE1_inner_include.f90:19-31)
+        a.S[o] = q[0];
+        a.UH[o] = q[1];
+        a.VH[o] = q[2];
+        fixed = C.fixed ? 1ull : 0ull;
+    } else {
     double un[3];
     for (int kk = 0; kk < 3; kk++) {
         const T *fn = a.fNS + kk * FP;
@@ -442,6 +449,7 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     a.UH[o] = (T)un[1];
     a.VH[o] = (T)un[2];
     fixed = C.fixed ? 1ull : 0ull;
+    }
     }
     // negative_depth_fix_counter: wave-level count, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) fixed += __shfl_xor(fixed, off);

@@ -185,15 +185,20 @@ struct DispArgs {
     const T *LU, *LV, *LE;       // UH, VH, elevation stored before the shallow-water step
     T *RU, *RV;                  // right-hand side
     const T *Y0, *Y1, *Y2;       // extrapolation levels, newest first (UH plane, then VH plane)
+    const double *qt;            // their times, oldest first (device)
     T *W;
     const double *dbe, *dle;
     T msl, dlon, dlat, td1, td2;
+    int cell;                    // 1: cell-centred (finite-volume) form, 0: staggered
+    const DevScalars *sc;        // forward time of the extrapolation from sc->time when set
+    double t_fwd;                // else this forward time
 };
-// make_rhs: explicit part and right-hand side; extrapolate: U(UH:VH) = ca Y0 + cb Y1 +
-// cc Y2; then niter tridiagonal iterations (rows of UH, then columns of VH)
+// make_rhs: explicit part and right-hand side; extrapolate: U(UH:VH) extrapolated in time
+// from Y0..Y2; then niter tridiagonal iterations (rows of UH, then columns of VH)
 template <typename T>
-hipError_t launch_dispersion(const DispArgs<T> &a, int make_rhs, int extrapolate, double ca, double cb, double cc,
-                             int niter, hipStream_t s);
+hipError_t launch_dispersion(const DispArgs<T> &a, int make_rhs, int extrapolate, int niter, hipStream_t s);
+// shift the level times down one and record the new level's time (sc->time or t_host)
+hipError_t launch_dispersion_qt_append(double *qt, const DevScalars *sc, double t_host, hipStream_t s);
 
 // Dense FP64 copy of the state [S (- C), UH, VH, E] as out[(var * ny + j) * nx + i]
 // (the bytes of Fortran U(nx, ny, 4)), with out[4 * nx * ny] = the time: sc->time when
@@ -296,6 +301,7 @@ struct FVArgs {
     double fric_power;
     int spherical, coriolis, n_ext;
     DevScalars *sc;
+    int skip_edges = 0;          // dispersive domains: update_U leaves the edge cells
 };
 
 template <typename T>
