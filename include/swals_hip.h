@@ -134,6 +134,24 @@ int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double
 int swh_set_flather_boundary(swh_domain *h, const int32_t sides[4], int32_t use_stage_function, double amp,
                              double period, double msl);
 
+/* Any of boundary_mod.f90's boundary subroutines on the flagged sides (N, E, S, W), at the
+ * reference's update_boundary calls:
+ *   SWH_BC_STAGE_TRANSMISSIVE_NORMAL_MOMENTUM  boundary_stage_transmissive_normal_momentum (:78-164)
+ *   SWH_BC_FLATHER                             flather_boundary (:502-720)
+ *   SWH_BC_TRANSMISSIVE                        transmissive_boundary (:23-75; elevation too)
+ *   SWH_BC_STAGE_RADIATION_MOMENTUM            boundary_stage_radiation_momentum (:166-318)
+ *   SWH_BC_STAGE_TRANSMISSIVE_MOMENTUM         boundary_stage_transmissive_momentum (:321-405)
+ * use_stage_function: the boundary_function is the stage msl + amp sin(2 pi t / period)
+ * with zero momenta and the cell's elevation (required by the first and last kinds);
+ * else there is none (Flather: the sea at rest; radiation: msl_linear). */
+#define SWH_BC_STAGE_TRANSMISSIVE_NORMAL_MOMENTUM 0
+#define SWH_BC_FLATHER 1
+#define SWH_BC_TRANSMISSIVE 2
+#define SWH_BC_STAGE_RADIATION_MOMENTUM 3
+#define SWH_BC_STAGE_TRANSMISSIVE_MOMENTUM 4
+int swh_set_boundary(swh_domain *h, int32_t kind, const int32_t sides[4], int32_t use_stage_function, double amp,
+                     double period, double msl);
+
 /* domain%evolve_one_step(timestep) (domain_mod.f90:1679-1813) for 'linear',
  * 'leapfrog_linear_plus_nonlinear_friction', 'rk2', 'euler', 'midpoint' and 'rk2n'
  * (domain_mod_timestepping_alternatives_include.f90:8-336). timestep <= 0 means

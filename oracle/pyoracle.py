@@ -73,6 +73,7 @@ def _load(kind, spherical):
     f("domain_get_max_U", None, [hd, _I, _PD])
     f("domain_set_stage_bc", None, [hd, _D, _D, _D, ctypes.POINTER(_I)])
     f("domain_set_flather_bc", None, [hd, _I, _D, _D, _D, ctypes.POINTER(_I)])
+    f("domain_set_bc", None, [hd, _I, _I, _D, _D, _D, ctypes.POINTER(_I)])
     f("domain_dispersion_setup", None, [hd, _D, _D, _I])
     f("domain_dispersion_store", None, [hd])
     f("domain_dispersion_append", None, [hd])
@@ -188,6 +189,15 @@ class CpuDomain:
     def set_stage_bc(self, amp, period, msl=0.0, sides=(0, 0, 0, 1)):
         s = (_I * 4)(*sides)
         self.f["domain_set_stage_bc"](self.h, float(amp), float(period), float(msl), s)
+
+    BC_KINDS = {"transmissive": 3, "stage_radiation_momentum": 4, "stage_transmissive_momentum": 5}
+
+    def set_bc(self, kind, sides=(1, 1, 1, 1), amp=None, period=1.0, msl=0.0):
+        """kind: 'transmissive', 'stage_radiation_momentum', 'stage_transmissive_momentum'
+        (boundary_mod.f90); amp None: no boundary_function."""
+        s = (_I * 4)(*sides)
+        self.f["domain_set_bc"](self.h, self.BC_KINDS[kind], int(amp is not None), float(amp or 0.0), float(period),
+                                float(msl), s)
 
     def set_flather_bc(self, sides=(1, 1, 1, 1), amp=None, period=1.0, msl=0.0):
         """Flather boundary on sides [N, E, S, W]; amp None: outside sea at rest at

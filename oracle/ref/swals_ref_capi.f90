@@ -11,7 +11,8 @@ module swals_ref_capi
     use iso_c_binding
     use global_mod, only: dp, ip, charlen, force_double
     use domain_mod, only: domain_type, STG, UH, VH, ELV
-    use boundary_mod, only: boundary_stage_transmissive_normal_momentum, flather_boundary
+    use boundary_mod, only: boundary_stage_transmissive_normal_momentum, flather_boundary, transmissive_boundary, &
+        boundary_stage_radiation_momentum, boundary_stage_transmissive_momentum
     use forcing_mod, only: forcing_patch_type, apply_forcing_patch
     use multidomain_mod, only: multidomain_type
     use timestepping_metadata_mod, only: timestepping_method_index
@@ -318,6 +319,28 @@ contains
             pool(h)%boundary_function => null()
         end if
         pool(h)%boundary_subroutine => flather_boundary
+    end subroutine
+
+    subroutine ref_domain_set_bc(h, kind, use_fun, amp, period, msl, sides) bind(C, name='ref_domain_set_bc')
+        ! kind 3 transmissive_boundary, 4 boundary_stage_radiation_momentum, 5
+        ! boundary_stage_transmissive_momentum (boundary_mod.f90:23-75, 166-405) on the sides
+        ! flagged in sides(1:4) = [N, E, S, W]; use_fun /= 0: the sine stage function
+        integer(c_int), value :: h, kind, use_fun
+        real(c_double), value :: amp, period, msl
+        integer(c_int), intent(in) :: sides(4)
+        bc_amp(h) = amp; bc_period(h) = period; bc_msl(h) = msl
+        pool(h)%local_index = h
+        pool(h)%boundary_exterior = (sides /= 0)
+        if(use_fun /= 0) then
+            pool(h)%boundary_function => ref_stage_fun
+        else
+            pool(h)%boundary_function => null()
+        end if
+        select case(kind)
+        case(3); pool(h)%boundary_subroutine => transmissive_boundary
+        case(4); pool(h)%boundary_subroutine => boundary_stage_radiation_momentum
+        case(5); pool(h)%boundary_subroutine => boundary_stage_transmissive_momentum
+        end select
     end subroutine
 
     ! ---------------------------------------------------------------------

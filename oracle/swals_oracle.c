@@ -273,6 +273,73 @@ static void flather_boundary(or_domain *d)
     }
 }
 
+/* boundary_mod.f90:23-75 (transmissive_boundary): every variable, elevation included,
+ * copied from the cell inside; west/east columns, then south/north rows */
+static void transmissive_boundary(or_domain *d)
+{
+    const int nx = d->nx, ny = d->ny;
+    for (int k = 0; k < 4; k++)
+        for (int j = 1; j <= ny; j++) {
+            if (d->boundary_exterior[3]) A(d, k, 1, j) = A(d, k, 2, j);
+            if (d->boundary_exterior[1]) A(d, k, nx, j) = A(d, k, nx - 1, j);
+        }
+    for (int k = 0; k < 4; k++)
+        for (int i = 1; i <= nx; i++) {
+            if (d->boundary_exterior[2]) A(d, k, i, 1) = A(d, k, i, 2);
+            if (d->boundary_exterior[0]) A(d, k, i, ny) = A(d, k, i, ny - 1);
+        }
+}
+
+/* boundary values [stage, elevation] at (i, j): the sine stage function with the cell's
+ * elevation, or (radiation without a function) msl_linear and the cell's elevation */
+static void bc_values(const or_domain *d, int i, int j, double *stage, double *elev)
+{
+    if (d->bc_fun) {
+        const double two_pi = 2.0 * acos(-1.0);
+        *stage = d->bc_msl + d->bc_amp * sin(two_pi * d->time / d->bc_period);
+    } else {
+        *stage = d->msl_linear;
+    }
+    *elev = A(d, ELV, i, j);
+}
+
+/* boundary_mod.f90:166-318 (boundary_stage_radiation_momentum, sgn = -1 west/south) and
+ * :321-405 (boundary_stage_transmissive_momentum); staggered grids set stage and
+ * elevation only */
+static void stage_momentum_cell(or_domain *d, int ib, int jb, int ii, int ji, double sgn, int normal, int radiation,
+                                int stag)
+{
+    const int tang = normal == UH ? VH : UH;
+    double st, el;
+    bc_values(d, ib, jb, &st, &el);
+    A(d, STG, ib, jb) = st;
+    A(d, ELV, ib, jb) = el;
+    if (stag) return;
+    if (radiation) {
+        const double local_h = dmax(st - el, 0.0);
+        const double q = (sqrt(GRAVITY * dmax(A(d, STG, ii, ji) - el, 0.0)) - sqrt(GRAVITY * local_h)) * local_h;
+        A(d, normal, ib, jb) = sgn < 0 ? -q : q;
+        A(d, tang, ib, jb) = A(d, tang, ii, ji);
+    } else {
+        A(d, normal, ib, jb) = A(d, normal, ii, ji);
+        A(d, tang, ib, jb) = A(d, tang, ii, ji);
+    }
+}
+
+static void stage_momentum_boundary(or_domain *d, int radiation)
+{
+    const int nx = d->nx, ny = d->ny;
+    const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF || d->method == OR_LEAPFROG_NONLINEAR);
+    for (int j = 1; j <= ny; j++) {
+        if (d->boundary_exterior[3]) stage_momentum_cell(d, 1, j, 2, j, -1.0, UH, radiation, stag);
+        if (d->boundary_exterior[1]) stage_momentum_cell(d, nx, j, nx - 1, j, 1.0, UH, radiation, stag);
+    }
+    for (int i = 1; i <= nx; i++) {
+        if (d->boundary_exterior[2]) stage_momentum_cell(d, i, 1, i, 2, -1.0, VH, radiation, stag);
+        if (d->boundary_exterior[0]) stage_momentum_cell(d, i, ny, i, ny - 1, 1.0, VH, radiation, stag);
+    }
+}
+
 static void update_boundary(or_domain *d)
 {
     if (!d->bc_on) return;
@@ -280,6 +347,9 @@ static void update_boundary(or_domain *d)
         flather_boundary(d);
         return;
     }
+    if (d->bc_on == 3) { transmissive_boundary(d); return; }
+    if (d->bc_on == 4) { stage_momentum_boundary(d, 1); return; }
+    if (d->bc_on == 5) { stage_momentum_boundary(d, 0); return; }
     const int nx = d->nx, ny = d->ny;
     const int stag = (d->method == OR_LINEAR || d->method == OR_LLPNF || d->method == OR_LEAPFROG_NONLINEAR);
     const double two_pi = 2.0 * acos(-1.0);
@@ -313,6 +383,12 @@ static void update_boundary(or_domain *d)
 void or_domain_set_stage_bc(or_domain *d, double amp, double period, double msl, const int sides[4])
 {
     d->bc_on = 1; d->bc_fun = 1; d->bc_amp = amp; d->bc_period = period; d->bc_msl = msl;
+    for (int k = 0; k < 4; k++) d->boundary_exterior[k] = sides[k] != 0;
+}
+
+void or_domain_set_bc(or_domain *d, int kind, int use_fun, double amp, double period, double msl, const int sides[4])
+{
+    d->bc_on = kind; d->bc_fun = use_fun != 0; d->bc_amp = amp; d->bc_period = period; d->bc_msl = msl;
     for (int k = 0; k < 4; k++) d->boundary_exterior[k] = sides[k] != 0;
 }
 

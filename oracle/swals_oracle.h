@@ -43,6 +43,11 @@ void or_domain_set_stage_bc(or_domain *d, double amp, double period, double msl,
 /* Flather boundary (boundary_mod.f90:502-720) on sides [N, E, S, W]; use_fun: outside
  * state from the sine stage msl + amp*sin(2*pi*t/period) instead of the sea at rest. */
 void or_domain_set_flather_bc(or_domain *d, int use_fun, double amp, double period, double msl, const int sides[4]);
+/* Other boundary subroutines of boundary_mod.f90 on sides [N, E, S, W]: kind 3
+ * transmissive_boundary, 4 boundary_stage_radiation_momentum, 5
+ * boundary_stage_transmissive_momentum (use_fun: the sine stage function, else
+ * msl_linear; 5 needs it). */
+void or_domain_set_bc(or_domain *d, int kind, int use_fun, double amp, double period, double msl, const int sides[4]);
 void or_domain_set_record_max_U(or_domain *d, int on);
 void or_domain_get_gauges(const or_domain *d, int n, const int *ij, double *out);
 

@@ -70,6 +70,7 @@ SYMBOLS = [
     ("swh_set_time", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("swh_set_stage_boundary_sine", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _PI32]),
     ("swh_set_flather_boundary", ctypes.c_int, [ctypes.c_void_p, _PI32, ctypes.c_int32, _D, _D, _D]),
+    ("swh_set_boundary", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _PI32, ctypes.c_int32, _D, _D, _D]),
     ("swh_dispersion_enable", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, ctypes.c_int32]),
     ("swh_dispersion_store_last_U", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_dispersion_append_extrapolation", ctypes.c_int, [ctypes.c_void_p]),

@@ -194,9 +194,9 @@ int leapfrog_step(swh_domain *h, double dt)
     if (h->n_nest_strips)
         SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->VH[c], (const T *)h->UH[c], 0,
                                      0, h->pitch, 1, dt, h->d_nest_int, h->stream));
-    // Flather boundary: it needs the new stage inside the edges before the momentum
-    // half-step, and changes UH/VH on the edges that the momentum half-step reads
-    const bool flather = h->bc_mask && h->bc_kind == 1;
+    // Flather and transmissive boundaries: they need the new stage inside the edges before
+    // the momentum half-step, and change UH/VH on the edges that the momentum half-step reads
+    const bool flather = h->bc_mask && (h->bc_kind == 1 || h->bc_kind == 2);
     // boundary fluxes read the pre-momentum UH/VH (include.f90:126-165)
     if (!flather)
         SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
@@ -212,7 +212,10 @@ int leapfrog_step(swh_domain *h, double dt)
     a.lin_fric_mult = (T)(1.0 / (1.0 + dt * h->lfc));
     a.fric_power = (T)((h->friction_type == 1) ? -2.0 : -7.0 / 3.0);
     a.bc_mask = h->bc_mask;
-    const double bc_stage = h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period);
+    // the boundary stage: the sine function, or msl_linear (radiation without a function)
+    const double bc_stage = h->bc_use_fun
+                                ? h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period)
+                                : h->msl;
     a.bc_stage = (T)bc_stage;
     a.rows_per_block = h->rows_per_block;
     a.src_fast = 0;
@@ -233,9 +236,14 @@ int leapfrog_step(swh_domain *h, double dt)
         a.bc_mask = 0;
         SWH_HIP(launch_lf_stage<T, KAHAN>(a, h->stream));
         T *Cb = KAHAN ? (T *)h->C[o] : nullptr;
-        SWH_HIP(launch_flather<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (const T *)h->E, h->nx, h->ny,
-                                  h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl,
-                                  bc_stage, nullptr, h->stream));
+        if (h->bc_kind == 1)
+            SWH_HIP(launch_flather<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (const T *)h->E, h->nx, h->ny,
+                                      h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl,
+                                      bc_stage, nullptr, h->stream));
+        else
+            SWH_HIP(launch_edge_bc<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (T *)h->E, h->nx, h->ny,
+                                      h->pitch, h->bc_mask, h->bc_kind, h->bc_use_fun, h->bc_amp, h->bc_period,
+                                      h->bc_msl, h->msl, bc_stage, 1, nullptr, h->stream));
         if (frac != 0.0 && h->fS)
             SWH_HIP(launch_add_scaled_kahan<T>((T *)h->S[o], Cb, (const T *)h->fS, h->pitch, h->f_i0, h->f_i1,
                                                h->f_j0, h->f_j1, (T)frac, h->stream));
@@ -306,7 +314,9 @@ int nonlinear_step(swh_domain *h, double dt)
     // the fused flux pass forms the lagged depth from the old and new stage, which needs
     // the elevation unchanged through the step; with elevation forcing, the depth plane
     const double frac = forcing_fraction(h, t_half, dt);
-    const bool elev_forced = frac != 0.0 && h->fE;
+    // elevation forcing or a transmissive boundary (which copies the elevation) change the
+    // elevation within the step
+    const bool elev_forced = (frac != 0.0 && h->fE) || (h->bc_mask && h->bc_kind == 2);
     if (elev_forced) SWH_HIP(launch_nonlinear_depth0<T>(n, h->stream));
 
     LFArgs<T> a;
@@ -315,17 +325,23 @@ int nonlinear_step(swh_domain *h, double dt)
     a.E = (const T *)h->E;
     a.So = (T *)h->S[o]; a.Co = (T *)h->C[o]; a.UHo = (T *)h->UH[o]; a.VHo = (T *)h->VH[o];
     a.rc = rc_view<T>(h);
-    const bool flather = h->bc_mask && h->bc_kind == 1;
-    const double bc_stage = h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period);
+    const bool flather = h->bc_mask && (h->bc_kind == 1 || h->bc_kind == 2);
+    const double bc_stage = h->bc_use_fun
+                                ? h->bc_msl + h->bc_amp * std::sin(2.0 * std::acos(-1.0) * t_half / h->bc_period)
+                                : h->msl;
     a.bc_mask = flather ? 0 : h->bc_mask;
     a.bc_stage = (T)bc_stage;
     a.maxq_mask = 0;
     SWH_HIP(launch_lf_stage<T, KAHAN>(a, h->stream));
     T *Cb = KAHAN ? (T *)h->C[o] : nullptr;
-    if (flather)
+    if (flather && h->bc_kind == 1)
         SWH_HIP(launch_flather<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (const T *)h->E, h->nx, h->ny,
                                   h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl,
                                   bc_stage, nullptr, h->stream));
+    else if (flather)
+        SWH_HIP(launch_edge_bc<T>((T *)h->S[o], Cb, (T *)h->UH[c], (T *)h->VH[c], (T *)h->E, h->nx, h->ny, h->pitch,
+                                  h->bc_mask, h->bc_kind, h->bc_use_fun, h->bc_amp, h->bc_period, h->bc_msl, h->msl,
+                                  bc_stage, 1, nullptr, h->stream));
     if (frac != 0.0 && h->fS)
         SWH_HIP(launch_add_scaled_kahan<T>((T *)h->S[o], Cb, (const T *)h->fS, h->pitch, h->f_i0, h->f_i1, h->f_j0,
                                            h->f_j1, (T)frac, h->stream));
@@ -490,6 +506,10 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
         SWH_HIP(launch_flather<T>((T *)h->S[dst], nullptr, (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->E, h->nx,
                                   h->ny, h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period,
                                   h->bc_msl, 0.0, h->sc, h->stream));
+    else if (h->bc_mask && h->bc_kind >= 2)
+        SWH_HIP(launch_edge_bc<T>((T *)h->S[dst], nullptr, (T *)h->UH[dst], (T *)h->VH[dst], (T *)h->E, h->nx, h->ny,
+                                  h->pitch, h->bc_mask, h->bc_kind, h->bc_use_fun, h->bc_amp, h->bc_period,
+                                  h->bc_msl, h->msl, 0.0, 0, h->sc, h->stream));
     else if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], h->nx, h->ny, h->pitch,
                                       h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
@@ -885,6 +905,24 @@ int swh_set_stage_boundary_sine(swh_domain *h, double amp, double period, double
     for (int k = 0; k < 4; k++) h->bex[k] = sides[k] != 0;
     h->bc_amp = amp; h->bc_period = period; h->bc_msl = msl;
     h->bc_kind = 0; h->bc_use_fun = 1;
+    return SWH_OK;
+}
+
+int swh_set_boundary(swh_domain *h, int32_t kind, const int32_t sides[4], int32_t use_stage_function, double amp,
+                     double period, double msl)
+{
+    if (!h || !sides) return fail(SWH_EINVAL, "null argument");
+    if (kind < SWH_BC_STAGE_TRANSMISSIVE_NORMAL_MOMENTUM || kind > SWH_BC_STAGE_TRANSMISSIVE_MOMENTUM)
+        return fail(SWH_EINVAL, "unknown boundary kind");
+    if (!use_stage_function &&
+        (kind == SWH_BC_STAGE_TRANSMISSIVE_NORMAL_MOMENTUM || kind == SWH_BC_STAGE_TRANSMISSIVE_MOMENTUM))
+        return fail(SWH_EINVAL, "this boundary needs a boundary_function (the stage function)");
+    if (use_stage_function && !(period != 0.0)) return fail(SWH_EINVAL, "period must be non-zero");
+    if (h->nx < 3 || h->ny < 3) return fail(SWH_EINVAL, "boundaries need at least 3 x 3 cells");
+    h->bc_mask = (sides[0] ? 1 : 0) | (sides[1] ? 2 : 0) | (sides[2] ? 4 : 0) | (sides[3] ? 8 : 0);
+    for (int k = 0; k < 4; k++) h->bex[k] = sides[k] != 0;
+    h->bc_amp = amp; h->bc_period = period; h->bc_msl = msl;
+    h->bc_kind = kind; h->bc_use_fun = use_stage_function != 0;
     return SWH_OK;
 }
 

@@ -221,6 +221,13 @@ template <typename T>
 hipError_t launch_flather(T *S, T *C, T *UH, T *VH, const T *E, int nx, int ny, int pitch, int mask, double msl_linear,
                           int use_fun, double amp, double period, double fun_msl, double fun_stage,
                           const DevScalars *sc, hipStream_t s);
+// transmissive_boundary (kind 2), boundary_stage_radiation_momentum (3),
+// boundary_stage_transmissive_momentum (4) on the edges in mask (1 N, 2 E, 4 S, 8 W);
+// stag: staggered grid (kinds 3, 4 then set stage and elevation only)
+template <typename T>
+hipError_t launch_edge_bc(T *S, T *C, T *UH, T *VH, T *E, int nx, int ny, int pitch, int mask, int kind, int use_fun,
+                          double amp, double period, double fun_msl, double msl_linear, double fun_stage, int stag,
+                          const DevScalars *sc, hipStream_t s);
 // Kahan-compensated stage += frac * fS over the patch (a leapfrog step's stage forcing
 // applied outside the fused step)
 template <typename T>

@@ -200,6 +200,84 @@ template hipError_t launch_add_scaled_kahan<double>(double *, double *, const do
 template hipError_t launch_add_scaled_kahan<float>(float *, float *, const float *, int, int, int, int, int, float,
                                                    hipStream_t);
 
+// ---- other boundary subroutines (boundary_mod.f90:23-75, 166-405) ----
+// kind 2: transmissive_boundary (every variable, elevation included, from the cell inside);
+// kind 3: boundary_stage_radiation_momentum; kind 4: boundary_stage_transmissive_momentum.
+// One block: west/east columns, then south/north rows. The boundary stage is the sine
+// function (use_fun; t from sc->time, else fun_stage from the host) or msl_linear, the
+// boundary elevation the cell's own. C (Kahan compensation, may be null) follows the
+// stage: copied with it (kind 2) or zeroed where the stage is set.
+template <typename T>
+__device__ void edge_bc_cell(T *S, T *C, T *Qn, T *Qt, T *E, long long b, long long n, bool neg, int kind,
+                             double stage_b, int stag)
+{
+    if (kind == 2) {
+        S[b] = S[n];
+        if (C) C[b] = C[n];
+        Qn[b] = Qn[n];
+        Qt[b] = Qt[n];
+        E[b] = E[n];
+        return;
+    }
+    const double el = (double)E[b];
+    S[b] = (T)stage_b;
+    if (C) C[b] = T(0);
+    if (stag) return;
+    if (kind == 3) {
+        const double local_h = fmax(stage_b - el, 0.0);
+        const double sin_ = C ? (double)S[n] - (double)C[n] : (double)S[n];
+        const double q = (sqrt(kGravity * fmax(sin_ - el, 0.0)) - sqrt(kGravity * local_h)) * local_h;
+        Qn[b] = (T)(neg ? -q : q);
+        Qt[b] = Qt[n];
+    } else {
+        Qn[b] = Qn[n];
+        Qt[b] = Qt[n];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void edge_bc_kernel(T *S, T *C, T *UH, T *VH, T *E, int nx, int ny, int pitch,
+                                                       int m, int kind, int use_fun, double amp, double period,
+                                                       double fun_msl, double msl_linear, double fun_stage, int stag,
+                                                       const DevScalars *sc)
+{
+    double stage_b = msl_linear;
+    if (use_fun) {
+        stage_b = fun_stage;
+        if (sc) stage_b = fun_msl + amp * sin(2.0 * acos(-1.0) * sc->time / period);
+    }
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        const long long o = (long long)j * pitch;
+        if (m & 8) edge_bc_cell<T>(S, C, UH, VH, E, o, o + 1, true, kind, stage_b, stag);
+        if (m & 2) edge_bc_cell<T>(S, C, UH, VH, E, o + nx - 1, o + nx - 2, false, kind, stage_b, stag);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        if (m & 4) edge_bc_cell<T>(S, C, VH, UH, E, i, (long long)pitch + i, true, kind, stage_b, stag);
+        if (m & 1) {
+            const long long o = (long long)(ny - 1) * pitch + i;
+            edge_bc_cell<T>(S, C, VH, UH, E, o, o - pitch, false, kind, stage_b, stag);
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_edge_bc(T *S, T *C, T *UH, T *VH, T *E, int nx, int ny, int pitch, int mask, int kind, int use_fun,
+                          double amp, double period, double fun_msl, double msl_linear, double fun_stage, int stag,
+                          const DevScalars *sc, hipStream_t s)
+{
+    if (nx < 3 || ny < 3 || kind < 2 || kind > 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(edge_bc_kernel<T>, dim3(1), dim3(1024), 0, s, S, C, UH, VH, E, nx, ny, pitch, mask, kind,
+                       use_fun, amp, period, fun_msl, msl_linear, fun_stage, stag, sc);
+    return hipGetLastError();
+}
+template hipError_t launch_edge_bc<double>(double *, double *, double *, double *, double *, int, int, int, int, int,
+                                           int, double, double, double, double, double, int, const DevScalars *,
+                                           hipStream_t);
+template hipError_t launch_edge_bc<float>(float *, float *, float *, float *, float *, int, int, int, int, int, int,
+                                          double, double, double, double, double, int, const DevScalars *,
+                                          hipStream_t);
+
 // ---- output snapshot ----
 template <typename T>
 __global__ void snapshot_kernel(const T *S, const T *C, const T *UH, const T *VH, const T *E, int nx, int ny,

@@ -178,6 +178,17 @@ class Domain:
         B.check(self._lib.swh_set_stage_boundary_sine(self._h, float(amp), float(period), float(msl), s),
                 "swh_set_stage_boundary_sine")
 
+    BOUNDARY_KINDS = {"stage_transmissive_normal_momentum": 0, "flather": 1, "transmissive": 2,
+                      "stage_radiation_momentum": 3, "stage_transmissive_momentum": 4}
+
+    def set_boundary(self, kind, sides=(1, 1, 1, 1), amp=None, period=1.0, msl=0.0):
+        """domain%boundary_subroutine => one of boundary_mod.f90's subroutines (kind: a key
+        of BOUNDARY_KINDS) on sides [N, E, S, W]; amp None: no boundary_function, else the
+        stage msl + amp*sin(2*pi*t/period)."""
+        s = (B._I32 * 4)(*[int(bool(v)) for v in sides])
+        B.check(self._lib.swh_set_boundary(self._h, self.BOUNDARY_KINDS[kind], s, int(amp is not None),
+                                           float(amp or 0.0), float(period), float(msl)), "swh_set_boundary")
+
     def set_flather_boundary(self, sides=(1, 1, 1, 1), amp=None, period=1.0, msl=0.0):
         """domain%boundary_subroutine => flather_boundary (boundary_mod.f90:502-720) on
         sides [N, E, S, W]. amp None: no boundary_function (outside sea at rest at

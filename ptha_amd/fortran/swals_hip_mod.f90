@@ -15,6 +15,8 @@ module swals_hip_mod
 
     integer(c_int32_t), parameter, public :: SWH_METHOD_RK2 = 1, SWH_METHOD_RK2N = 2, SWH_METHOD_MIDPOINT = 3, &
         SWH_METHOD_EULER = 4, SWH_METHOD_LINEAR = 5, SWH_METHOD_LLPNF = 6, SWH_METHOD_LEAPFROG_NONLINEAR = 8
+    integer(c_int32_t), parameter, public :: SWH_BC_STAGE_TRANSMISSIVE_NORMAL_MOMENTUM = 0, SWH_BC_FLATHER = 1, &
+        SWH_BC_TRANSMISSIVE = 2, SWH_BC_STAGE_RADIATION_MOMENTUM = 3, SWH_BC_STAGE_TRANSMISSIVE_MOMENTUM = 4
     integer(c_int32_t), parameter, public :: SWH_PREC_FP64 = 0, SWH_PREC_FP32K = 1
     integer(c_int32_t), parameter, public :: SWH_MAXQ_MAX_STAGE = 1, SWH_MAXQ_MAX_SPEED = 2, &
         SWH_MAXQ_MAX_FLUX = 4, SWH_MAXQ_ARRIVAL_TIME = 8, SWH_MAXQ_TIME_OF_MAX_STAGE = 16, SWH_MAXQ_MIN_STAGE = 32
@@ -73,6 +75,7 @@ module swals_hip_mod
 
     public :: swh_last_error, swh_domain_create, swh_domain_destroy, swh_upload_U, swh_download_U, &
         swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_set_flather_boundary, &
+        swh_set_boundary, &
         swh_set_forcing_patch, swh_clear_forcing_patch, swh_evolve_one_step, swh_dispersion_enable, &
         swh_dispersion_store_last_U, swh_dispersion_append_extrapolation, swh_dispersion_solve, &
         swh_evolve_steps, swh_get_scalars, swh_volume_interior, swh_stationary_timestep_max, &
@@ -134,6 +137,16 @@ module swals_hip_mod
         integer(c_int) function swh_clear_forcing_patch(h) bind(C, name='swh_clear_forcing_patch')
             import c_int, c_ptr
             type(c_ptr), value :: h
+        end function
+        ! any boundary subroutine of boundary_mod.f90 (SWH_BC_*) on sides [N, E, S, W]
+        integer(c_int) function swh_set_boundary(h, kind, sides, use_stage_function, amp, period, msl) &
+                bind(C, name='swh_set_boundary')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: kind
+            integer(c_int32_t), intent(in) :: sides(4)
+            integer(c_int32_t), value :: use_stage_function
+            real(c_double), value :: amp, period, msl
         end function
         ! flather_boundary (boundary_mod.f90:502-720) on sides [N, E, S, W]
         integer(c_int) function swh_set_flather_boundary(h, sides, use_stage_function, amp, period, msl) &

@@ -94,6 +94,16 @@ def dispersive_cases():
     }
 
 
+def with_bc(cfg, kind, sides=(1, 1, 1, 1), amp=None, period=600.0):
+    """cfg with one of boundary_mod.f90's other boundary subroutines ('transmissive',
+    'stage_radiation_momentum', 'stage_transmissive_momentum') instead of any stage
+    boundary."""
+    cfg = dict(cfg)
+    cfg.pop("stage_bc", None)
+    cfg["bc"] = dict(kind=kind, sides=sides, amp=amp, period=period)
+    return cfg
+
+
 def with_flather(cfg, sides=(1, 1, 1, 1), amp=None, period=600.0):
     """cfg with a Flather boundary (boundary_mod.f90:502-720) instead of any stage
     boundary: outside sea at rest, or the sine stage of amplitude amp."""
@@ -119,6 +129,8 @@ def cpu_run(kind, cfg, nsteps, dt=None, record=None, **kw):
         d.set_stage_bc(b["amp"], b["period"], 0.0, b["sides"])
     if "flather" in cfg:
         d.set_flather_bc(**cfg["flather"])
+    if "bc" in cfg:
+        d.set_bc(**cfg["bc"])
     if dt is None and cfg["method"] not in FV_METHODS:
         dt = d.stationary_timestep_max()
     hist = []
@@ -142,6 +154,8 @@ def gpu_domain(cfg, precision="fp64", max_U_variables=("max_stage", "max_speed",
         g.set_stage_boundary_sine(b["amp"], b["period"], 0.0, b["sides"])
     if "flather" in cfg:
         g.set_flather_boundary(**cfg["flather"])
+    if "bc" in cfg:
+        g.set_boundary(**cfg["bc"])
     return g
 
 

@@ -1,7 +1,9 @@
 """GPU: Flather boundaries (flather_boundary, boundary_mod.f90:502-720, through
-swh_set_flather_boundary) against the oracle, whose Flather restatement is pinned bit
-for bit to the compiled reference (tests/test_oracle_vs_reference.py, the *_flather
-cases). The open basin has no land and no walls, a hump next to the west edge and
+swh_set_flather_boundary) and the other boundary subroutines (transmissive_boundary,
+boundary_stage_radiation_momentum, boundary_stage_transmissive_momentum, :23-405, through
+swh_set_boundary) against the oracle, whose restatements are pinned bit for bit to the
+compiled reference (tests/test_oracle_vs_reference.py, the *_flather, *_transmissive and
+*_radiation cases). The open basin has no land and no walls, a hump next to the west edge and
 currents across the north and east edges, so every side carries flow.
 
 * leapfrog FP64 ('linear' Cartesian and spherical with Coriolis, truely-linear llpnf;
@@ -26,6 +28,10 @@ pytestmark = pytest.mark.gpu
 MAXQ = ["max_stage", "max_speed", "max_flux", "arrival_time", "time_of_max_stage", "min_stage"]
 
 LF_CASES = {
+    "linear_sph_transmissive": lambda: H.with_bc(H.open_basin(), "transmissive"),
+    "linear_sph_radiation_fun": lambda: H.with_bc(H.open_basin(), "stage_radiation_momentum", amp=0.3),
+    "lfnl_sph_transmissive": lambda: H.with_bc(dict(H.open_basin(method="leapfrog_nonlinear"),
+                                                    manning_squared=np.zeros((80, 90))), "transmissive"),
     "linear_sph": lambda: H.with_flather(H.open_basin()),
     "linear_sph_fun": lambda: H.with_flather(H.open_basin(), sides=(1, 0, 0, 1), amp=0.3),
     "linear_cart": lambda: H.with_flather(dict(H.open_basin(), spherical=False)),
@@ -35,6 +41,13 @@ LF_CASES = {
 }
 
 FV_CASES = {
+    "rk2_sph_transmissive": (lambda: H.with_bc(H.open_basin(method="rk2"), "transmissive"), 30),
+    "rk2_sph_radiation": (lambda: H.with_bc(H.open_basin(method="rk2"), "stage_radiation_momentum"), 30),
+    "rk2_beach_radiation_fun": (lambda: H.with_bc(H.small_beach(), "stage_radiation_momentum", sides=(0, 0, 0, 1),
+                                                  amp=1.0), 60),
+    "midpoint_beach_transmissive_momentum": (lambda: H.with_bc(dict(H.small_beach(), method="midpoint"),
+                                                               "stage_transmissive_momentum", sides=(0, 0, 0, 1),
+                                                               amp=1.0), 40),
     "rk2_sph": (lambda: H.with_flather(H.open_basin(method="rk2")), 30),
     "midpoint_sph_fun": (lambda: H.with_flather(H.open_basin(method="midpoint"), amp=0.5), 30),
     "euler_beach_fun": (lambda: H.with_flather(dict(H.small_beach(), method="euler"), sides=(0, 0, 0, 1), amp=1.0),

@@ -66,6 +66,20 @@ CASES["euler_beach_flather"] = (lambda: H.with_flather(dict(H.small_beach(), met
                                                       amp=1.0), {}, 40)
 
 
+# the other boundary subroutines (boundary_mod.f90:23-75, 166-405)
+CASES["linear_sph_transmissive"] = (lambda: H.with_bc(H.open_basin(), "transmissive"), {}, 60)
+CASES["rk2_sph_transmissive"] = (lambda: H.with_bc(H.open_basin(method="rk2"), "transmissive"), {}, 30)
+CASES["rk2_sph_radiation"] = (lambda: H.with_bc(H.open_basin(method="rk2"), "stage_radiation_momentum"), {}, 30)
+CASES["rk2_beach_radiation_fun"] = (lambda: H.with_bc(H.small_beach(), "stage_radiation_momentum", sides=(0, 0, 0, 1),
+                                                      amp=1.0), {}, 60)
+CASES["midpoint_beach_transmissive_momentum"] = (lambda: H.with_bc(dict(H.small_beach(), method="midpoint"),
+                                                                   "stage_transmissive_momentum", sides=(0, 0, 0, 1),
+                                                                   amp=1.0), {}, 40)
+CASES["linear_sph_radiation_fun"] = (lambda: H.with_bc(H.open_basin(), "stage_radiation_momentum", amp=0.3), {}, 60)
+CASES["lfnl_sph_transmissive"] = (lambda: H.with_bc(dict(H.open_basin(method="leapfrog_nonlinear"),
+                                                         manning_squared=np.full((80, 90), 0.025 ** 2)), "transmissive"),
+                                  {}, 60)
+
 # 'leapfrog_nonlinear' (domain_mod_leapfrog_nonlinear_include.f90)
 for _k, (_make, _n, _kw) in H.nonlinear_leapfrog_cases().items():
     CASES["lfnl_" + _k] = (_make, _kw, _n)
