@@ -192,8 +192,8 @@ int leapfrog_step(swh_domain *h, double dt)
     // nesting flux integrals: dt * (UH, VH) before the update, mass only
     // (domain_mod_linear_solver_include.f90:74-77)
     if (h->n_nest_strips)
-        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->VH[c], (const T *)h->UH[c], 0,
-                                     0, h->pitch, 1, dt, h->d_nest_int, h->stream));
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, h->nest_strip_max, (const T *)h->VH[c],
+                                     (const T *)h->UH[c], 0, 0, h->pitch, 1, dt, h->d_nest_int, h->stream));
     // Flather and transmissive boundaries: they need the new stage inside the edges before
     // the momentum half-step, and change UH/VH on the edges that the momentum half-step reads
     const bool flather = h->bc_mask && (h->bc_kind == 1 || h->bc_kind == 2);
@@ -499,8 +499,8 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
     SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
     if (h->n_nest_strips && nest_dt != 0.0) {
         const long long FPn = (long long)h->fpitch * (h->ny + 1), EPn = (long long)h->fpitch * h->ny;
-        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, (const T *)h->fNS, (const T *)h->fEW, FPn,
-                                     EPn, h->fpitch, 3, nest_dt, h->d_nest_int, h->stream));
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, h->nest_strip_max, (const T *)h->fNS,
+                                     (const T *)h->fEW, FPn, EPn, h->fpitch, 3, nest_dt, h->d_nest_int, h->stream));
     }
     if (h->bc_mask && h->bc_kind == 1)
         SWH_HIP(launch_flather<T>((T *)h->S[dst], nullptr, (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->E, h->nx,

@@ -74,6 +74,7 @@ struct swh_domain {
     // multidomain nesting: box-edge strips whose flux integrals each step accumulates
     const swh::NestStrip *d_nest_strips = nullptr;
     int n_nest_strips = 0;
+    int nest_strip_max = 0;   // longest strip (cells), for the accumulation's grid
     double *d_nest_int = nullptr;
     // launch shape of the step kernel
     int block_x = 256, rows_per_block = 32, variant = 2;

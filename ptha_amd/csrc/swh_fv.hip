@@ -19,8 +19,11 @@ namespace swh {
 
 constexpr double kLimC1 = 0.1, kLimC2 = 0.3, kLimC3 = 100.0;  // domain_mod.f90:111-117
 
-__device__ __forceinline__ double fmax_(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double fmin_(double a, double b) { return a < b ? a : b; }
+// MAX / MIN as single v_max_f64 / v_min_f64 instructions: they differ from
+// 'a > b ? a : b' only for NaN operands and in the sign of a zero result (max(-0, +0)),
+// which no nonzero value of the scheme depends on
+__device__ __forceinline__ double fmax_(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double fmin_(double a, double b) { return __builtin_fmin(a, b); }
 
 // ---- clip + depth/velocity of one cell (domain_mod.f90:1374-1412) ----
 // Evaluated where it is needed (by the flux pass for a cell and its neighbours, by
@@ -326,7 +329,7 @@ int fv_flux_blocks(int nx, int ny) { return ((nx + kFTX - 1) / kFTX) * ((ny + kF
 constexpr int kScalarThreads = 1024;
 
 // the step's closing bookkeeping (defined with the kernels that run it)
-__device__ void fv_finish(DevScalars *sc, int mode);
+__device__ void fv_finish(DevScalars &sc, int mode);
 
 template <typename T>
 __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny,
@@ -335,65 +338,95 @@ __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS,
                                                                    const double *bmax, int nbmax)
 {
     constexpr int Q = kScalarThreads / 4;
+    constexpr int B = 8;   // loads issued together before any is used
     __shared__ double part[4][Q];
-    __shared__ double pm[kScalarThreads];
+    __shared__ double pm[kScalarThreads / 64];
+    // thread 0 reads the step's scalars now, so that their latency overlaps the sums
+    // below (neg_depth_count is not touched)
+    DevScalars L;
+    if (threadIdx.x == 0) {
+        L.time = sc->time; L.dt = sc->dt; L.bf_evolve = sc->bf_evolve; L.bf_time = sc->bf_time;
+        L.first_max_dt = sc->first_max_dt; L.time0 = sc->time0;
+    }
     const int g = threadIdx.x / Q, lane = threadIdx.x % Q;
+    // boundary line g: N and S rows of flux_NS, E and W columns of flux_EW; each thread
+    // adds every Q'th entry in index order, B loads at a time
+    const bool ns = (g == 0 || g == 2);
+    const T *src = ns ? fNS : fEW;
+    const long long base =
+        ns ? (long long)((g == 0) ? ny - n_ext : n_ext) * fpitch : (long long)((g == 1) ? nx - n_ext : n_ext);
+    const long long step = ns ? 1 : fpitch;
+    const int last = ns ? nx - n_ext - 1 : ny - n_ext - 1;
     double acc = 0.0;
-    if (g == 0 || g == 2) {
-        const long long row = (long long)((g == 0) ? ny - n_ext : n_ext) * fpitch;
-#pragma unroll 8
-        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += Q) acc += (double)fNS[row + i];
-    } else {
-        const int ic = (g == 1) ? nx - n_ext : n_ext;
-#pragma unroll 8
-        for (int jj = n_ext + lane; jj <= ny - n_ext - 1; jj += Q) acc += (double)fEW[(long long)jj * fpitch + ic];
+    for (int i0 = n_ext + lane; i0 <= last; i0 += B * Q) {
+        double v[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int i = i0 + u * Q;
+            v[u] = i <= last ? (double)src[base + i * step] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++)
+            if (i0 + u * Q <= last) acc += v[u];
     }
     part[g][lane] = acc;
     // CFL: maximum over the flux pass's blocks, from 1/maximum_timestep
-    // (compute_fluxes_DE1_inner_include.f90:92-95)
+    // (compute_fluxes_DE1_inner_include.f90:92-95); the entries are >= 0
     double m = 1.0 / kMaxTimestep;
-#pragma unroll 8
-    for (int k = threadIdx.x; k < nbmax; k += kScalarThreads) m = fmax_(m, bmax[k]);
-    pm[threadIdx.x] = m;
+    for (int k0 = threadIdx.x; k0 < nbmax; k0 += B * kScalarThreads) {
+        double v[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int k = k0 + u * kScalarThreads;
+            v[u] = k < nbmax ? bmax[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++) m = fmax_(m, v[u]);
+    }
+    // the maximum needs no fixed order: across each wavefront by shuffles, then over
+    // the wavefronts by thread 0
+    for (int off = 32; off > 0; off >>= 1) m = fmax_(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) pm[threadIdx.x >> 6] = m;
     __syncthreads();
     for (int w = Q / 2; w > 0; w >>= 1) {
         if (lane < w) part[g][lane] = part[g][lane] + part[g][lane + w];
         __syncthreads();
     }
-    for (int w = kScalarThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) pm[threadIdx.x] = fmax_(pm[threadIdx.x], pm[threadIdx.x + w]);
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        double st[4];
-        for (int q = 0; q < 4; q++) st[q] = part[q][0];
-        sc->bf_store[0] = st[0];
-        sc->bf_store[1] = st[1];
-        sc->bf_store[2] = -st[2];
-        sc->bf_store[3] = -st[3];
+        double mx = pm[0];
+        for (int w = 1; w < kScalarThreads / 64; w++) mx = fmax_(mx, pm[w]);
+        L.bf_store[0] = part[0][0];
+        L.bf_store[1] = part[1][0];
+        L.bf_store[2] = -part[2][0];
+        L.bf_store[3] = -part[3][0];
         if (mode & kStageBegin) {
             // evolve_one_step resets the step's flux integral (domain_mod.f90:1713-1715);
             // the schemes keep the start time (timestepping_alternatives_include.f90:78, 167, 271)
-            sc->bf_evolve = 0.0;
-            sc->time0 = sc->time;
+            L.bf_evolve = 0.0;
+            L.time0 = L.time;
         }
-        const double max_dt = 1.0 / pm[0];
-        sc->max_dt = max_dt;
+        const double max_dt = 1.0 / mx;
+        L.max_dt = max_dt;
         if (mode & kStageFirst) {
-            sc->dt = fixed_dt > 0.0 ? fixed_dt : max_dt;
-            sc->first_max_dt = max_dt;
+            L.dt = fixed_dt > 0.0 ? fixed_dt : max_dt;
+            L.first_max_dt = max_dt;
         }
         // the stage's step: the base step, or midpoint's half step / rk2n's reduced step
-        const double ts = (dt_mult == 1.0) ? sc->dt : dt_mult * sc->dt;
-        sc->stage_dt = ts;
-        if (mode & kStageScaleBfN)
-            sc->bf_evolve = sc->bf_evolve * (kRk2nN - 1.0) * (1.0 / (1.0 * kRk2nN));
+        const double ts = (dt_mult == 1.0) ? L.dt : dt_mult * L.dt;
+        L.stage_dt = ts;
+        if (mode & kStageScaleBfN) L.bf_evolve = L.bf_evolve * (kRk2nN - 1.0) * (1.0 / (1.0 * kRk2nN));
         if (mode & kStageAccBf)
-            sc->bf_evolve =
-                sc->bf_evolve + ts * (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]);
+            L.bf_evolve = L.bf_evolve + ts * (((L.bf_store[0] + L.bf_store[1]) + L.bf_store[2]) + L.bf_store[3]);
         // update_U advances the time (update_U_DE1_inner_include.f90:98)
-        sc->time = (mode & kStageFromTime0) ? sc->time0 + ts : sc->time + ts;
-        if (mode & kStageFinish) fv_finish(sc, kFinishEuler);
+        L.time = (mode & kStageFromTime0) ? L.time0 + ts : L.time + ts;
+        if (mode & kStageFinish) {
+            fv_finish(L, kFinishEuler);
+            sc->evolve_dt = L.evolve_dt;
+        }
+        sc->time = L.time; sc->dt = L.dt; sc->max_dt = L.max_dt;
+        for (int q = 0; q < 4; q++) sc->bf_store[q] = L.bf_store[q];
+        sc->bf_evolve = L.bf_evolve; sc->bf_time = L.bf_time; sc->first_max_dt = L.first_max_dt;
+        sc->time0 = L.time0; sc->stage_dt = L.stage_dt;
     }
 }
 
@@ -463,7 +496,7 @@ __global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
-    if (i == 0 && j == 0) fv_finish(sc, kFinishRk2);
+    if (i == 0 && j == 0) fv_finish(*sc, kFinishRk2);
     if (i >= nx || j >= ny) return;
     const long long o = (long long)j * pitch + i;
     S[o] = T(0.5) * (S[o] + S0[o]);
@@ -495,22 +528,22 @@ __global__ __launch_bounds__(256) void fv_stage_bc_kernel(T *S, T *UH, T *VH, in
     }
 }
 
-__device__ void fv_finish(DevScalars *sc, int mode)
+__device__ void fv_finish(DevScalars &sc, int mode)
 {
     if (mode == kFinishRk2) {
         // timestepping_alternatives_include.f90:136-142
-        sc->time = sc->time0 + sc->dt;
-        sc->bf_evolve = 0.5 * sc->bf_evolve;
+        sc.time = sc.time0 + sc.dt;
+        sc.bf_evolve = 0.5 * sc.bf_evolve;
     } else if (mode == kFinishRk2n) {
         // timestepping_alternatives_include.f90:244-249
-        sc->time = sc->time0 + sc->dt * (kRk2nN - 1);
+        sc.time = sc.time0 + sc.dt * (kRk2nN - 1);
     }
     // the reported CFL step is the first flux computation's (euler and midpoint
     // have only that one as max_dt; rk2 and rk2n restore it)
-    sc->max_dt = sc->first_max_dt;
+    sc.max_dt = sc.first_max_dt;
     // evolve_one_step (domain_mod.f90:1799-1803)
-    sc->evolve_dt = sc->time - sc->time0;
-    sc->bf_time = sc->bf_time + sc->bf_evolve;
+    sc.evolve_dt = sc.time - sc.time0;
+    sc.bf_time = sc.bf_time + sc.bf_evolve;
 }
 
 // ---- rk2n combination (timestepping_alternatives_include.f90:212-241) ----
@@ -520,7 +553,7 @@ __global__ void fv_rk2n_combine_kernel(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
-    if (part == 1 && i == 0 && j == 0) fv_finish(sc, kFinishRk2n);
+    if (part == 1 && i == 0 && j == 0) fv_finish(*sc, kFinishRk2n);
     if (i >= nx || j >= ny) return;
     const long long o = (long long)j * pitch + i;
     if (part == 0) {

@@ -434,8 +434,9 @@ struct NestReint {
 template <typename T>
 hipError_t launch_nest_zero(double *p, long long n, hipStream_t s);
 template <typename T>
-hipError_t launch_nest_accum(const NestStrip *strips, int n_strips, const T *fNS, const T *fEW, long long plane_ns,
-                             long long plane_ew, int pitch, int nvar, double dt, double *integ, hipStream_t s);
+hipError_t launch_nest_accum(const NestStrip *strips, int n_strips, int max_n, const T *fNS, const T *fEW,
+                             long long plane_ns, long long plane_ew, int pitch, int nvar, double dt, double *integ,
+                             hipStream_t s);
 template <typename T>
 hipError_t launch_nest_pack(const NestDoms<T> &doms, const NestPack *packs, int n_packs, int n_blocks,
                             double *const *integ, double *arena, hipStream_t s);

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "swh_domain.h"
@@ -148,7 +149,8 @@ void offset_inside_or_out(const swh_md *md, int my, int dm_outside, int out_inde
 }
 
 // The flux-correction updates of domain k in the reference's loop order
-// (domain_mod.f90:2803-3226): N, S, E, W passes, each over the receive boxes.
+// (domain_mod.f90:2803-3226): N, S, E, W passes, each over the receive boxes; one
+// entry per corrected cell.
 int build_fixes(swh_md *md, int k, std::vector<NestFix> &fix, std::vector<int> &seg)
 {
     const swh_domain *h = md->dom[k];
@@ -209,13 +211,41 @@ int build_fixes(swh_md *md, int k, std::vector<NestFix> &fix, std::vector<int> &
                     const NestStrip &st = md->h_strips[k][md->recv_strip0[k][c] + dir];
                     f.err = st.out + (q - lo);
                     f.stride = st.n;
-                    fix.push_back(f);
+                    // one entry per cell, so that a segment's cells go to separate threads
+                    for (int p = p0 - 1; p <= p1 - 1; p++) {
+                        f.p0 = p;
+                        f.p1 = p;
+                        fix.push_back(f);
+                    }
                 }
             }
             if ((int)fix.size() > start) seg.push_back(start);
         }
     }
     seg.push_back((int)fix.size());
+    // consecutive segments with no cell in common are merged: a cell's update reads only
+    // that cell (and the read-only error arena), so the merged group gives the same
+    // values as the segments one after the other, with fewer barriers
+    std::vector<int> merged;
+    std::unordered_set<long long> cells;
+    for (size_t g = 0; g + 1 < seg.size(); g++) {
+        bool clash = merged.empty();
+        for (int q = seg[g]; q < seg[g + 1] && !clash; q++) {
+            const NestFix &f = fix[q];
+            const long long key = f.is_ew ? (long long)f.line * nx + f.p0 : (long long)f.p0 * nx + f.line;
+            clash = cells.count(key) != 0;
+        }
+        if (clash) {
+            merged.push_back(seg[g]);
+            cells.clear();
+        }
+        for (int q = seg[g]; q < seg[g + 1]; q++) {
+            const NestFix &f = fix[q];
+            cells.insert(f.is_ew ? (long long)f.line * nx + f.p0 : (long long)f.p0 * nx + f.line);
+        }
+    }
+    merged.push_back((int)fix.size());
+    seg.swap(merged);
     return SWH_OK;
 }
 
@@ -315,6 +345,7 @@ void md_free(swh_md *md)
     for (auto h : md->dom) {
         h->d_nest_strips = nullptr;
         h->n_nest_strips = 0;
+        h->nest_strip_max = 0;
         h->d_nest_int = nullptr;
     }
 }
@@ -598,6 +629,8 @@ int finalize(swh_md *md)
         swh_domain *h = md->dom[k];
         h->d_nest_strips = md->d_strips[k];
         h->n_nest_strips = md->n_strips[k];
+        h->nest_strip_max = 0;
+        for (const auto &st : md->h_strips[k]) h->nest_strip_max = std::max(h->nest_strip_max, st.n);
         h->d_nest_int = md->d_int[k];
     }
     md->finalized = true;

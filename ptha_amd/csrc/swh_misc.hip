@@ -67,17 +67,35 @@ __global__ __launch_bounds__(kBfThreads) void bflux_leapfrog_kernel(const T *__r
                                                                    double time_after, DevScalars *sc)
 {
     constexpr int Q = kBfThreads / 4;
+    constexpr int B = 8;   // loads issued together before any is used
     __shared__ double part_hi[4][Q], part_lo[4][Q];
+    // thread 0 reads the metric terms and the time integral now, so that their latency
+    // overlaps the sums
+    double m[4] = {0.0, 0.0, 0.0, 0.0}, bf_time = 0.0;
+    if (threadIdx.x == 0) {
+        m[0] = dbe[ny - n_ext]; m[1] = dle[nx - n_ext]; m[2] = dbe[n_ext]; m[3] = dle[n_ext];
+        bf_time = sc->bf_time;
+    }
     const int g = threadIdx.x / Q, lane = threadIdx.x % Q;
+    // side g: N and S rows of VH, E and W columns of UH; each thread adds every Q'th
+    // entry in index order, B loads at a time
+    const bool ns = (g == 0 || g == 2);
+    const T *src = ns ? VH : UH;
+    const long long base = ns ? (long long)((g == 0) ? ny - n_ext - 1 : n_ext - 1) * pitch
+                              : (long long)((g == 1) ? nx - n_ext - 1 : n_ext - 1);
+    const long long step = ns ? 1 : pitch;
+    const int last = ns ? nx - n_ext - 1 : ny - n_ext - 1;
     DD acc{0.0, 0.0};
-    if (g == 0 || g == 2) {
-        const long long row = (long long)((g == 0) ? ny - n_ext - 1 : n_ext - 1) * pitch;
-#pragma unroll 8
-        for (int i = n_ext + lane; i <= nx - n_ext - 1; i += Q) acc = dd_add(acc, (double)VH[row + i]);
-    } else {
-        const int i = (g == 1) ? nx - n_ext - 1 : n_ext - 1;
-#pragma unroll 8
-        for (int j = n_ext + lane; j <= ny - n_ext - 1; j += Q) acc = dd_add(acc, (double)UH[(long long)j * pitch + i]);
+    for (int i0 = n_ext + lane; i0 <= last; i0 += B * Q) {
+        double v[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int i = i0 + u * Q;
+            v[u] = i <= last ? (double)src[base + i * step] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++)
+            if (i0 + u * Q <= last) acc = dd_add(acc, v[u]);
     }
     part_hi[g][lane] = acc.hi;
     part_lo[g][lane] = acc.lo;
@@ -93,13 +111,14 @@ __global__ __launch_bounds__(kBfThreads) void bflux_leapfrog_kernel(const T *__r
     if (threadIdx.x == 0) {
         double store[4];
         for (int q = 0; q < 4; q++) store[q] = part_hi[q][0] + part_lo[q][0];
-        sc->bf_store[0] = store[0] * dbe[ny - n_ext];
-        sc->bf_store[1] = store[1] * dle[nx - n_ext];
-        sc->bf_store[2] = -store[2] * dbe[n_ext];
-        sc->bf_store[3] = -store[3] * dle[n_ext];
-        const double ev = (((sc->bf_store[0] + sc->bf_store[1]) + sc->bf_store[2]) + sc->bf_store[3]) * dt;
+        const double b0 = store[0] * m[0], b1 = store[1] * m[1], b2 = -store[2] * m[2], b3 = -store[3] * m[3];
+        sc->bf_store[0] = b0;
+        sc->bf_store[1] = b1;
+        sc->bf_store[2] = b2;
+        sc->bf_store[3] = b3;
+        const double ev = (((b0 + b1) + b2) + b3) * dt;
         sc->bf_evolve = ev;
-        sc->bf_time = sc->bf_time + ev;
+        sc->bf_time = bf_time + ev;
         sc->time = time_after;
     }
 }

@@ -88,31 +88,32 @@ __global__ __launch_bounds__(256) void nest_accum_kernel(const NestStrip *__rest
                                                          const T *fEW, long long plane_ns, long long plane_ew,
                                                          int pitch, int nvar, double dt, double *integ)
 {
+    // one cell of one strip per thread (blockIdx.y picks the strip's chunk), so the
+    // read-modify-writes of a strip are in flight together
     const NestStrip st = strips[blockIdx.x];
-    if (st.line < 0) return;
-    for (int r = threadIdx.x; r < st.n; r += blockDim.x) {
-        const long long o = st.kind == 0 ? (long long)st.line * pitch + st.lo + r
-                                         : (long long)(st.lo + r) * pitch + st.line;
-        for (int k = 0; k < nvar; k++) {
-            const double f = st.kind == 0 ? (double)fNS[k * plane_ns + o] : (double)fEW[k * plane_ew + o];
-            double *x = integ + st.out + (long long)k * st.n + r;
-            *x = *x + dt * f * st.edge;
-        }
+    const int r = blockIdx.y * blockDim.x + threadIdx.x;
+    if (st.line < 0 || r >= st.n) return;
+    const long long o = st.kind == 0 ? (long long)st.line * pitch + st.lo + r : (long long)(st.lo + r) * pitch + st.line;
+    for (int k = 0; k < nvar; k++) {
+        const double f = st.kind == 0 ? (double)fNS[k * plane_ns + o] : (double)fEW[k * plane_ew + o];
+        double *x = integ + st.out + (long long)k * st.n + r;
+        *x = *x + dt * f * st.edge;
     }
 }
 
 template <typename T>
-hipError_t launch_nest_accum(const NestStrip *strips, int n_strips, const T *fNS, const T *fEW, long long plane_ns,
-                             long long plane_ew, int pitch, int nvar, double dt, double *integ, hipStream_t s)
+hipError_t launch_nest_accum(const NestStrip *strips, int n_strips, int max_n, const T *fNS, const T *fEW,
+                             long long plane_ns, long long plane_ew, int pitch, int nvar, double dt, double *integ,
+                             hipStream_t s)
 {
-    if (n_strips <= 0) return hipSuccess;
-    hipLaunchKernelGGL(nest_accum_kernel<T>, dim3(n_strips), dim3(256), 0, s, strips, fNS, fEW, plane_ns, plane_ew,
-                       pitch, nvar, dt, integ);
+    if (n_strips <= 0 || max_n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(nest_accum_kernel<T>, dim3(n_strips, (max_n + 255) / 256), dim3(256), 0, s, strips, fNS, fEW,
+                       plane_ns, plane_ew, pitch, nvar, dt, integ);
     return hipGetLastError();
 }
-template hipError_t launch_nest_accum<double>(const NestStrip *, int, const double *, const double *, long long,
+template hipError_t launch_nest_accum<double>(const NestStrip *, int, int, const double *, const double *, long long,
                                               long long, int, int, double, double *, hipStream_t);
-template hipError_t launch_nest_accum<float>(const NestStrip *, int, const float *, const float *, long long,
+template hipError_t launch_nest_accum<float>(const NestStrip *, int, int, const float *, const float *, long long,
                                              long long, int, int, double, double *, hipStream_t);
 
 // ---- pack (process_data_to_send_2022Feb) ----
@@ -418,7 +419,7 @@ template hipError_t launch_nest_stag_fix<float>(const NestDoms<float> &, const N
 // receive box, in the reference's loop order); the cells of one segment are
 // distinct, and the block finishes a segment before it starts the next.
 template <typename T>
-__global__ __launch_bounds__(256) void nest_fix_kernel(NestDoms<T> doms, const NestFixDom *__restrict__ fds)
+__global__ __launch_bounds__(1024) void nest_fix_kernel(NestDoms<T> doms, const NestFixDom *__restrict__ fds)
 {
     const NestFixDom fd = fds[blockIdx.x];
     const NestDomPtr<T> &d = doms.d[fd.dom];
@@ -432,11 +433,13 @@ __global__ __launch_bounds__(256) void nest_fix_kernel(NestDoms<T> doms, const N
                 if (!go) go = nval(d, 1, i + 1, j + 1) >= nval(d, 4, i + 1, j + 1) + kMinDepth;
                 if (!go) go = fd.pnp[(long long)j * d.nx + i] == 1;
                 if (!go) continue;
+                // all variables read before any is written, so their loads are in flight together
+                double u[3];
                 for (int k = f.var1; k <= f.varN; k++) {
                     const double q_k = fd.err[f.err + (long long)k * f.stride] / f.area;
-                    const double u0 = nval(d, k + 1, i + 1, j + 1);
-                    nset(d, k + 1, i + 1, j + 1, u0 + f.sgn * q_k);
+                    u[k - f.var1] = nval(d, k + 1, i + 1, j + 1) + f.sgn * q_k;
                 }
+                for (int k = f.var1; k <= f.varN; k++) nset(d, k + 1, i + 1, j + 1, u[k - f.var1]);
             }
         }
         __syncthreads();
@@ -447,7 +450,7 @@ template <typename T>
 hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, int n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(nest_fix_kernel<T>, dim3(n), dim3(256), 0, s, doms, fds);
+    hipLaunchKernelGGL(nest_fix_kernel<T>, dim3(n), dim3(1024), 0, s, doms, fds);
     return hipGetLastError();
 }
 template hipError_t launch_nest_fix<double>(const NestDoms<double> &, const NestFixDom *, int, hipStream_t);
