@@ -490,11 +490,18 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
 // in this stage) and the boundary condition (update_U calls update_boundary,
 // domain_mod.f90:1564-1582).
 template <typename T>
-int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double dt_mult, int mode, double nest_dt)
+int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double dt_mult, int mode, double nest_dt,
+             int avg_src = -1)
 {
     FVArgs<T> a = fv_args<T>(h, dst, src);
     if (qsrc >= 0) {
         a.Sq = (const T *)h->S[qsrc]; a.UHq = (const T *)h->UH[qsrc]; a.VHq = (const T *)h->VH[qsrc];
+    }
+    if (avg_src >= 0) {
+        // rk2's average with state avg_src in the update (only where nothing else
+        // follows the second update: no boundary condition, no forcing)
+        if (h->bc_mask || h->f_on) return fail(SWH_ESTATE, "internal: folded rk2 average with a boundary or forcing");
+        a.avgS = (const T *)h->S[avg_src]; a.avgUH = (const T *)h->UH[avg_src]; a.avgVH = (const T *)h->VH[avg_src];
     }
     SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
     if (h->n_nest_strips && nest_dt != 0.0) {
@@ -576,10 +583,15 @@ int fv_step(swh_domain *h, double dt)
         // one_rk2_step (include.f90:64-146): two Euler stages with the same step,
         // nesting fluxes tracked half a step each, then the average with backup_U
         STAGE(B, A, -1, dt, 1.0, first | acc, dt * 0.5);
-        STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5);
-        SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A],
-                                     (const T *)h->UH[A], (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->sc,
-                                     h->stream));
+        if (!h->bc_mask && !h->f_on) {
+            // the average folded into the second update (nothing acts between them)
+            STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5, A);
+        } else {
+            STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5);
+            SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A],
+                                         (const T *)h->UH[A], (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->sc,
+                                         h->stream));
+        }
         break;
     case SWH_METHOD_MIDPOINT:
         // one_midpoint_step (include.f90:257-336): half a step for the fluxes, then the

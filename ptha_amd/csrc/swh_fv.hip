@@ -437,6 +437,9 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     unsigned long long fixed = 0;
+    // with the rk2 average folded in, the step's closing bookkeeping (it does not touch
+    // stage_dt, the one scalar this kernel reads)
+    if (a.avgS && i == 0 && j == 0) fv_finish(*a.sc, kFinishRk2);
     if (i < a.nx && j < a.ny) {
     const double dt = a.sc->stage_dt;
     const long long o = (long long)j * a.pitch + i;
@@ -451,11 +454,12 @@ __global__ void fv_update_kernel(FVArgs<T> a)
         q[1] = a.UHq[o];
         q[2] = a.VHq[o];
     }
+    T out[3];
     if (a.skip_edges && (i == 0 || j == 0 || i == a.nx - 1 || j == a.ny - 1)) {
         // dispersive domains: the edge cells keep their values (update_U_DE1_inner_include.f90:19-31)
-        a.S[o] = q[0];
-        a.UH[o] = q[1];
-        a.VH[o] = q[2];
+        out[0] = q[0];
+        out[1] = q[1];
+        out[2] = q[2];
         fixed = C.fixed ? 1ull : 0ull;
     } else {
     double un[3];
@@ -478,11 +482,21 @@ __global__ void fv_update_kernel(FVArgs<T> a)
         un[2] = un[2] + inv_cell_area_dt * (a.esV[o] + a.esJ[fo + a.fpitch]);
         un[2] = un[2] * implicit_factor;
     }
-    a.S[o] = (T)un[0];
-    a.UH[o] = (T)un[1];
-    a.VH[o] = (T)un[2];
+    out[0] = (T)un[0];
+    out[1] = (T)un[1];
+    out[2] = (T)un[2];
     fixed = C.fixed ? 1ull : 0ull;
     }
+    if (a.avgS) {
+        // one_rk2_step's average with backup_U (timestepping_alternatives_include.f90:123-134),
+        // on the stored (T) values as the separate pass forms it
+        out[0] = T(0.5) * (out[0] + a.avgS[o]);
+        out[1] = T(0.5) * (out[1] + a.avgUH[o]);
+        out[2] = T(0.5) * (out[2] + a.avgVH[o]);
+    }
+    a.S[o] = out[0];
+    a.UH[o] = out[1];
+    a.VH[o] = out[2];
     }
     // negative_depth_fix_counter: wave-level count, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) fixed += __shfl_xor(fixed, off);

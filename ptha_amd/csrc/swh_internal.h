@@ -309,6 +309,9 @@ struct FVArgs {
     int spherical, coriolis, n_ext;
     DevScalars *sc;
     int skip_edges = 0;          // dispersive domains: update_U leaves the edge cells
+    // rk2's second update with the average folded in: the stored state is
+    // 0.5 * (updated + backup) (null: plain update)
+    const T *avgS = nullptr, *avgUH = nullptr, *avgVH = nullptr;
 };
 
 template <typename T>
