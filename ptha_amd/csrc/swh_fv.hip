@@ -220,22 +220,27 @@ __device__ __forceinline__ void cell_sources(const FVArgs<T> &a, int j, const Ce
 }
 
 // ---- pass 2: fluxes + explicit sources + CFL ----
-// A block owns a 64 x 4 tile of cells (each cell owns its south and west edges). It
-// first evaluates the clipped state of the tile and a halo of two cells west/south
-// and one cell east/north into LDS (each cell once, instead of once per edge that
-// needs it), then the limited gradients of every cell that an edge of the tile needs
-// (north-south ones for rows j0-1..j0+3, east-west ones for columns i0-1..i0+63: each
-// once, where neighbouring edges would otherwise both form it), then each thread
-// forms its cell's two edges from LDS.
+// A block owns a 64 x 4 tile of cells (each cell owns its south and west edges; a
+// wavefront is one row of the tile). It first evaluates the clipped state of the tile
+// and a halo of two cells west/south and one cell east/north into LDS (each cell once,
+// instead of once per edge that needs it), then the limited gradients of every cell
+// that an edge of the tile needs (north-south ones for rows j0-1..j0+3, east-west ones
+// for columns i0-1..i0+63: each once, where neighbouring edges would otherwise both
+// form it), then each thread forms its cell's two edges from LDS. The gradients are
+// kept as one LDS array per quantity, so that a wavefront's accesses fall in distinct
+// banks. (A 64 x 8 tile does less halo work but was slower: with half as many blocks
+// per CU, fewer of them overlap one another's load and compute phases.)
 constexpr int kFTX = 64, kFTY = 4, kFTW = kFTX + 3, kFTH = kFTY + 3;
+constexpr int kFThreads = kFTX * kFTY;
 constexpr int kGNS = (kFTY + 1) * kFTX;   // north-south gradients per tile
 constexpr int kGEW = kFTY * (kFTX + 1);   // east-west gradients per tile
 
 template <typename T>
-__global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
+__global__ __launch_bounds__(kFThreads) void fv_flux_kernel(FVArgs<T> a)
 {
     __shared__ T ls_s[kFTH][kFTW], ls_d[kFTH][kFTW], ls_u[kFTH][kFTW], ls_v[kFTH][kFTW], ls_e[kFTH][kFTW];
-    __shared__ Grad4 g_ns[kFTY + 1][kFTX], g_ew[kFTY][kFTX + 1];
+    __shared__ double gns_st[kFTY + 1][kFTX], gns_dp[kFTY + 1][kFTX], gns_u[kFTY + 1][kFTX], gns_v[kFTY + 1][kFTX];
+    __shared__ double gew_st[kFTY][kFTX + 1], gew_dp[kFTY][kFTX + 1], gew_u[kFTY][kFTX + 1], gew_v[kFTY][kFTX + 1];
     const int i0 = blockIdx.x * kFTX, j0 = blockIdx.y * kFTY;
     const int gx0 = i0 - 2, gy0 = j0 - 2;
     for (int t = threadIdx.x; t < kFTW * kFTH; t += blockDim.x) {
@@ -264,13 +269,13 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
             const int ii = i0 + c, jj = j0 - 1 + r;
             Grad4 g = zero;
             if (ii < a.nx && jj > 0 && jj < a.ny - 1) g = grad3(G(ii, jj - 1), G(ii, jj), G(ii, jj + 1), a.theta);
-            g_ns[r][c] = g;
+            gns_st[r][c] = g.st; gns_dp[r][c] = g.dp; gns_u[r][c] = g.u; gns_v[r][c] = g.v;
         } else {
             const int q = t - kGNS, r = q / (kFTX + 1), c = q % (kFTX + 1);
             const int ii = i0 - 1 + c, jj = j0 + r;
             Grad4 g = zero;
             if (jj < a.ny && ii > 0 && ii < a.nx - 1) g = grad3(G(ii - 1, jj), G(ii, jj), G(ii + 1, jj), a.theta);
-            g_ew[r][c] = g;
+            gew_st[r][c] = g.st; gew_dp[r][c] = g.dp; gew_u[r][c] = g.u; gew_v[r][c] = g.v;
         }
     }
     __syncthreads();
@@ -278,6 +283,8 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
     const int i = i0 + ci;
     const int j = j0 + rj;
     auto L = [&](int di, int dj) { return G(i + di, j + dj); };
+    auto GNS = [&](int r, int c) { return Grad4{gns_st[r][c], gns_dp[r][c], gns_u[r][c], gns_v[r][c]}; };
+    auto GEW = [&](int r, int c) { return Grad4{gew_st[r][c], gew_dp[r][c], gew_u[r][c], gew_v[r][c]}; };
     auto LE = [&](int di, int dj) { return ls_e[j + dj - gy0][i + di - gx0]; };
     double mx = 0.0;
     if (i >= 1 && i < a.nx && j >= 1 && j < a.ny) {
@@ -287,13 +294,13 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
         // the cell (with its momenta, for the spherical terms) and its south and west
         // neighbours; gradients are zero on the border rows / columns
         const Cell<T> C = cell_at(a, o);
-        const EdgeOut r = ns_edge(a, j, L(0, -1), C, g_ns[rj][ci], g_ns[rj + 1][ci], (double)LE(0, -1),
+        const EdgeOut r = ns_edge(a, j, L(0, -1), C, GNS(rj, ci), GNS(rj + 1, ci), (double)LE(0, -1),
                                   (double)LE(0, 0));
         a.fNS[fo] = (T)r.f0;
         a.fNS[FP + fo] = (T)r.f1;
         a.fNS[2 * FP + fo] = (T)r.f2;
         a.esJ[fo] = r.es_neg;
-        const EdgeOut q = ew_edge(a, i, j, L(-1, 0), C, g_ew[rj][ci], g_ew[rj][ci + 1], (double)LE(-1, 0),
+        const EdgeOut q = ew_edge(a, i, j, L(-1, 0), C, GEW(rj, ci), GEW(rj, ci + 1), (double)LE(-1, 0),
                                   (double)LE(0, 0));
         a.fEW[fo] = (T)q.f0;
         a.fEW[EPn + fo] = (T)q.f1;
@@ -308,7 +315,7 @@ __global__ __launch_bounds__(256) void fv_flux_kernel(FVArgs<T> a)
     // block maximum (max is exact, so the order does not matter); the scalar pass
     // takes the maximum over blocks, instead of every wavefront contending for one
     // atomic
-    __shared__ double wmax[4];
+    __shared__ double wmax[kFThreads / 64];
     for (int off = 32; off > 0; off >>= 1) mx = fmax_(mx, __shfl_xor(mx, off));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
     __syncthreads();
@@ -586,7 +593,8 @@ template <typename T>
 hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, int mode, hipStream_t s)
 {
     dim3 block(256), grid((a.nx + 255) / 256, a.ny);
-    hipLaunchKernelGGL(fv_flux_kernel<T>, dim3((a.nx + kFTX - 1) / kFTX, (a.ny + kFTY - 1) / kFTY), block, 0, s, a);
+    hipLaunchKernelGGL(fv_flux_kernel<T>, dim3((a.nx + kFTX - 1) / kFTX, (a.ny + kFTY - 1) / kFTY), dim3(kFThreads), 0,
+                       s, a);
     hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(kScalarThreads), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
                        fixed_dt, dt_mult, mode, a.sc, (const double *)a.bmax, fv_flux_blocks(a.nx, a.ny));
     hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
