@@ -410,12 +410,11 @@ struct NestFix {
     int stride;
 };
 
-// The flux-correction work of one domain: 'fix' grouped in n_seg segments
-// (seg[s] .. seg[s+1]-1), applied in order.
+// The flux-correction work of one domain: its updates 'fix', one per cell (the
+// launches take them group by group).
 struct NestFixDom {
     const NestFix *fix;
-    const int *seg;
-    int n_seg, dom;
+    int dom;
     const double *err;      // the domain's flux-error arena
     const int *pnp;         // is_priority_domain_not_periodic, nx*ny, 0-based row-major
     int dry_all;            // flux_correct_dry_cells_outside_priority_domain
@@ -450,7 +449,7 @@ template <typename T>
 hipError_t launch_nest_stag_fix(const NestDoms<T> &doms, const NestUnpack *ups, int n_ups, int n_blocks,
                                 hipStream_t s);
 template <typename T>
-hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, int n, hipStream_t s);
+hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, const int2 *cells, int n, hipStream_t s);
 template <typename T>
 hipError_t launch_nest_reint(const NestDoms<T> &doms, const NestReint *rs, int n_rs, int n_blocks, double *copy,
                              hipStream_t s);

@@ -69,7 +69,9 @@ struct swh_md {
     int n_ups = 0, up_blocks = 0, n_fixups = 0, fixup_blocks = 0;
     NestFixDom *d_fixdoms = nullptr;
     int n_fixdoms = 0;
-    std::vector<void *> owned;                // fix / seg arrays
+    std::vector<int2 *> fix_level;            // per group level: (fixdom, update) of every domain's group
+    std::vector<int> fix_level_n;
+    std::vector<void *> owned;                // fix and group-level arrays
     std::vector<int *> d_pnp;
     NestReint *d_reint = nullptr;
     int n_reint = 0, reint_blocks = 0;
@@ -259,7 +261,8 @@ int exchange(swh_md *md, bool flux_correct)
                                   s));
     SWH_HIP(launch_nest_stag_fix<T>(doms, md->d_fixups, md->n_fixups, md->fixup_blocks, s));
     if (flux_correct) {
-        SWH_HIP(launch_nest_fix<T>(doms, md->d_fixdoms, md->n_fixdoms, s));
+        for (size_t L = 0; L < md->fix_level.size(); L++)
+            SWH_HIP(launch_nest_fix<T>(doms, md->d_fixdoms, md->fix_level[L], md->fix_level_n[L], s));
         SWH_HIP(launch_nest_reint<T>(doms, md->d_reint, md->n_reint, md->reint_blocks, md->d_copy, s));
     }
     return SWH_OK;
@@ -553,6 +556,7 @@ int finalize(swh_md *md)
     // ---- priority masks, flux correction ----
     md->d_pnp.assign(nd, nullptr);
     std::vector<NestFixDom> fds;
+    std::vector<std::vector<int>> fd_seg;
     for (int k = 0; k < nd; k++) {
         swh_domain *h = md->dom[k];
         if (md->pdi[k].empty()) return md_fail(SWH_ESTATE, "swh_md_set_priority was not called for every domain");
@@ -564,17 +568,12 @@ int finalize(swh_md *md)
         if (rc) return rc;
         if (fix.empty()) continue;
         NestFix *dfix = nullptr;
-        int *dseg = nullptr;
         rc = upload_vec(&dfix, fix);
         if (rc) return rc;
         md->owned.push_back(dfix);
-        rc = upload_vec(&dseg, seg);
-        if (rc) return rc;
-        md->owned.push_back(dseg);
+        fd_seg.push_back(seg);
         NestFixDom fd{};
         fd.fix = dfix;
-        fd.seg = dseg;
-        fd.n_seg = (int)seg.size() - 1;
         fd.dom = k;
         fd.err = md->d_err[k];
         fd.pnp = md->d_pnp[k];
@@ -585,6 +584,26 @@ int finalize(swh_md *md)
     md->n_fixdoms = (int)fds.size();
     rc = upload_vec(&md->d_fixdoms, fds);
     if (rc) return rc;
+    // group level L: every domain's L-th group
+    md->fix_level.clear();
+    md->fix_level_n.clear();
+    for (size_t L = 0;; L++) {
+        std::vector<int2> cells;
+        bool any = false;
+        for (size_t f = 0; f < fd_seg.size(); f++) {
+            const std::vector<int> &sg = fd_seg[f];
+            if (L + 1 >= sg.size()) continue;
+            any = true;
+            for (int q = sg[L]; q < sg[L + 1]; q++) cells.push_back(make_int2((int)f, q));
+        }
+        if (!any) break;
+        int2 *dc = nullptr;
+        rc = upload_vec(&dc, cells);
+        if (rc) return rc;
+        md->owned.push_back(dc);
+        md->fix_level.push_back(dc);
+        md->fix_level_n.push_back((int)cells.size());
+    }
 
     // ---- re-interpolation of fine receive boxes ----
     std::vector<NestReint> rs;

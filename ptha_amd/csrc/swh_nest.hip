@@ -415,46 +415,47 @@ template hipError_t launch_nest_stag_fix<double>(const NestDoms<double> &, const
 template hipError_t launch_nest_stag_fix<float>(const NestDoms<float> &, const NestUnpack *, int, int, hipStream_t);
 
 // ---- flux correction (domain_mod.f90:2803-3226) ----
-// One block per domain. The updates are grouped in segments (direction, then
-// receive box, in the reference's loop order); the cells of one segment are
-// distinct, and the block finishes a segment before it starts the next.
+// One launch per group level: the updates of each domain are grouped (direction,
+// then receive box, in the reference's loop order; consecutive passes with no cell in
+// common share a group), the cells of one group are distinct, and level L holds every
+// domain's L-th group, one cell per thread. Launches on one stream run in order, so
+// each domain's groups apply one after the other.
 template <typename T>
-__global__ __launch_bounds__(1024) void nest_fix_kernel(NestDoms<T> doms, const NestFixDom *__restrict__ fds)
+__global__ __launch_bounds__(256) void nest_fix_kernel(NestDoms<T> doms, const NestFixDom *__restrict__ fds,
+                                                       const int2 *__restrict__ cells, int n)
 {
-    const NestFixDom fd = fds[blockIdx.x];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int2 c = cells[t];
+    const NestFixDom fd = fds[c.x];
     const NestDomPtr<T> &d = doms.d[fd.dom];
-    for (int s = 0; s < fd.n_seg; s++) {
-        const int a = fd.seg[s], b = fd.seg[s + 1];
-        for (int q = a + threadIdx.x; q < b; q += blockDim.x) {
-            const NestFix f = fd.fix[q];
-            for (int p = f.p0; p <= f.p1; p++) {
-                const int i = f.is_ew ? p : f.line, j = f.is_ew ? f.line : p;   // 0-based
-                bool go = fd.dry_all != 0;
-                if (!go) go = nval(d, 1, i + 1, j + 1) >= nval(d, 4, i + 1, j + 1) + kMinDepth;
-                if (!go) go = fd.pnp[(long long)j * d.nx + i] == 1;
-                if (!go) continue;
-                // all variables read before any is written, so their loads are in flight together
-                double u[3];
-                for (int k = f.var1; k <= f.varN; k++) {
-                    const double q_k = fd.err[f.err + (long long)k * f.stride] / f.area;
-                    u[k - f.var1] = nval(d, k + 1, i + 1, j + 1) + f.sgn * q_k;
-                }
-                for (int k = f.var1; k <= f.varN; k++) nset(d, k + 1, i + 1, j + 1, u[k - f.var1]);
-            }
+    const NestFix f = fd.fix[c.y];
+    for (int p = f.p0; p <= f.p1; p++) {
+        const int i = f.is_ew ? p : f.line, j = f.is_ew ? f.line : p;   // 0-based
+        bool go = fd.dry_all != 0;
+        if (!go) go = nval(d, 1, i + 1, j + 1) >= nval(d, 4, i + 1, j + 1) + kMinDepth;
+        if (!go) go = fd.pnp[(long long)j * d.nx + i] == 1;
+        if (!go) continue;
+        // all variables read before any is written, so their loads are in flight together
+        double u[3];
+        for (int k = f.var1; k <= f.varN; k++) {
+            const double q_k = fd.err[f.err + (long long)k * f.stride] / f.area;
+            u[k - f.var1] = nval(d, k + 1, i + 1, j + 1) + f.sgn * q_k;
         }
-        __syncthreads();
+        for (int k = f.var1; k <= f.varN; k++) nset(d, k + 1, i + 1, j + 1, u[k - f.var1]);
     }
 }
 
 template <typename T>
-hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, int n, hipStream_t s)
+hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, const int2 *cells, int n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(nest_fix_kernel<T>, dim3(n), dim3(1024), 0, s, doms, fds);
+    hipLaunchKernelGGL(nest_fix_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, doms, fds, cells, n);
     return hipGetLastError();
 }
-template hipError_t launch_nest_fix<double>(const NestDoms<double> &, const NestFixDom *, int, hipStream_t);
-template hipError_t launch_nest_fix<float>(const NestDoms<float> &, const NestFixDom *, int, hipStream_t);
+template hipError_t launch_nest_fix<double>(const NestDoms<double> &, const NestFixDom *, const int2 *, int,
+                                            hipStream_t);
+template hipError_t launch_nest_fix<float>(const NestDoms<float> &, const NestFixDom *, const int2 *, int, hipStream_t);
 
 // ---- re-interpolation of fine receive boxes (domain_mod.f90:3228-3468) ----
 template <typename T>
