@@ -451,8 +451,8 @@ hipError_t launch_nest_stag_fix(const NestDoms<T> &doms, const NestUnpack *ups, 
 template <typename T>
 hipError_t launch_nest_fix(const NestDoms<T> &doms, const NestFixDom *fds, const int2 *cells, int n, hipStream_t s);
 template <typename T>
-hipError_t launch_nest_reint(const NestDoms<T> &doms, const NestReint *rs, int n_rs, int n_blocks, double *copy,
-                             hipStream_t s);
+hipError_t launch_nest_reint(const NestDoms<T> &doms, const NestReint *rs, int n_rs, int max_len, int n_blocks,
+                             double *copy, hipStream_t s);
 
 
 }  // namespace swh

@@ -74,7 +74,7 @@ struct swh_md {
     std::vector<void *> owned;                // fix and group-level arrays
     std::vector<int *> d_pnp;
     NestReint *d_reint = nullptr;
-    int n_reint = 0, reint_blocks = 0;
+    int n_reint = 0, reint_blocks = 0, reint_max_len = 0;
     double *d_copy = nullptr;
     double *d_vol = nullptr;
 };
@@ -263,7 +263,8 @@ int exchange(swh_md *md, bool flux_correct)
     if (flux_correct) {
         for (size_t L = 0; L < md->fix_level.size(); L++)
             SWH_HIP(launch_nest_fix<T>(doms, md->d_fixdoms, md->fix_level[L], md->fix_level_n[L], s));
-        SWH_HIP(launch_nest_reint<T>(doms, md->d_reint, md->n_reint, md->reint_blocks, md->d_copy, s));
+        SWH_HIP(launch_nest_reint<T>(doms, md->d_reint, md->n_reint, md->reint_max_len, md->reint_blocks, md->d_copy,
+                                     s));
     }
     return SWH_OK;
 }
@@ -638,6 +639,8 @@ int finalize(swh_md *md)
     }
     md->n_reint = (int)rs.size();
     md->reint_blocks = blocks;
+    md->reint_max_len = 0;
+    for (const auto &r : rs) md->reint_max_len = std::max(md->reint_max_len, std::max(r.len_ns, r.len_ew));
     rc = upload_vec(&md->d_reint, rs);
     if (rc) return rc;
     SWH_HIP(hipMalloc((void **)&md->d_copy, std::max<long long>(copy_n, 1) * sizeof(double)));

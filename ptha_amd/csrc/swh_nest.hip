@@ -458,6 +458,8 @@ template hipError_t launch_nest_fix<double>(const NestDoms<double> &, const Nest
 template hipError_t launch_nest_fix<float>(const NestDoms<float> &, const NestFixDom *, const int2 *, int, hipStream_t);
 
 // ---- re-interpolation of fine receive boxes (domain_mod.f90:3228-3468) ----
+// blockIdx.x: the box; blockIdx.y: a chunk of its 4 sides x 4 variables x L copies,
+// one per thread
 template <typename T>
 __global__ __launch_bounds__(256) void nest_reint_copy_kernel(NestDoms<T> doms, const NestReint *__restrict__ rs,
                                                               double *copy)
@@ -465,15 +467,14 @@ __global__ __launch_bounds__(256) void nest_reint_copy_kernel(NestDoms<T> doms, 
     const NestReint r = rs[blockIdx.x];
     const NestDomPtr<T> &d = doms.d[r.dom];
     const int L = r.len_ns > r.len_ew ? r.len_ns : r.len_ew;
-    for (int side = 0; side < 4; side++) {
-        const int len = side < 2 ? r.len_ns : r.len_ew;
-        for (int t = threadIdx.x; t < 4 * len; t += blockDim.x) {
-            const int k = t / len, pos = t - k * len;
-            const int i = side < 2 ? r.n0 + pos : r.ind[side];
-            const int j = side < 2 ? r.ind[side] : r.m0 + pos;
-            copy[r.copy + ((long long)side * 4 + k) * L + pos] = nval(d, k + 1, i, j);
-        }
-    }
+    const int t = blockIdx.y * blockDim.x + threadIdx.x;
+    if (t >= 16 * L) return;
+    const int side = t / (4 * L), rem = t - side * 4 * L, k = rem / L, pos = rem - k * L;
+    const int len = side < 2 ? r.len_ns : r.len_ew;
+    if (pos >= len) return;
+    const int i = side < 2 ? r.n0 + pos : r.ind[side];
+    const int j = side < 2 ? r.ind[side] : r.m0 + pos;
+    copy[r.copy + ((long long)side * 4 + k) * L + pos] = nval(d, k + 1, i, j);
 }
 
 template <typename T>
@@ -543,18 +544,19 @@ __global__ __launch_bounds__(256) void nest_reint_kernel(NestDoms<T> doms, const
 }
 
 template <typename T>
-hipError_t launch_nest_reint(const NestDoms<T> &doms, const NestReint *rs, int n_rs, int n_blocks, double *copy,
-                             hipStream_t s)
+hipError_t launch_nest_reint(const NestDoms<T> &doms, const NestReint *rs, int n_rs, int max_len, int n_blocks,
+                             double *copy, hipStream_t s)
 {
-    if (n_rs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(nest_reint_copy_kernel<T>, dim3(n_rs), dim3(256), 0, s, doms, rs, copy);
+    if (n_rs <= 0 || max_len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(nest_reint_copy_kernel<T>, dim3(n_rs, (16 * max_len + 255) / 256), dim3(256), 0, s, doms, rs,
+                       copy);
     if (n_blocks > 0)
         hipLaunchKernelGGL(nest_reint_kernel<T>, dim3(n_blocks), dim3(256), 0, s, doms, rs, n_rs, (const double *)copy);
     return hipGetLastError();
 }
-template hipError_t launch_nest_reint<double>(const NestDoms<double> &, const NestReint *, int, int, double *,
+template hipError_t launch_nest_reint<double>(const NestDoms<double> &, const NestReint *, int, int, int, double *,
                                               hipStream_t);
-template hipError_t launch_nest_reint<float>(const NestDoms<float> &, const NestReint *, int, int, double *,
+template hipError_t launch_nest_reint<float>(const NestDoms<float> &, const NestReint *, int, int, int, double *,
                                              hipStream_t);
 
 }  // namespace swh
