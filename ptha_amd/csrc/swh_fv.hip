@@ -526,11 +526,11 @@ __global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
 }
 
 // ---- stage boundary on a co-located grid (boundary_mod.f90:78-164) ----
-// One block: west/east columns first, then south/north rows, as in the reference
-// (so corner cells end with the south/north treatment).
+// One block of 1024 threads: west/east columns first, then south/north rows, as in
+// the reference (so corner cells end with the south/north treatment).
 template <typename T>
-__global__ __launch_bounds__(256) void fv_stage_bc_kernel(T *S, T *UH, T *VH, int nx, int ny, int pitch, int m,
-                                                          double amp, double period, double msl, const DevScalars *sc)
+__global__ __launch_bounds__(1024) void fv_stage_bc_kernel(T *S, T *UH, T *VH, int nx, int ny, int pitch, int m,
+                                                           double amp, double period, double msl, const DevScalars *sc)
 {
     const double two_pi = 2.0 * acos(-1.0);
     const T stage = (T)(msl + amp * sin(two_pi * sc->time / period));
@@ -614,7 +614,7 @@ template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
                               double msl, const DevScalars *sc, hipStream_t s)
 {
-    hipLaunchKernelGGL(fv_stage_bc_kernel<T>, dim3(1), dim3(256), 0, s, S, UH, VH, nx, ny, pitch, mask, amp, period,
+    hipLaunchKernelGGL(fv_stage_bc_kernel<T>, dim3(1), dim3(1024), 0, s, S, UH, VH, nx, ny, pitch, mask, amp, period,
                        msl, sc);
     return hipGetLastError();
 }
