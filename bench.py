@@ -52,17 +52,8 @@ def build_batch(cfg, precision, nsources, nsteps_per_source, every, max_samples,
     b = UnitSourceBatch(cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], nsources, precision=precision)
     b.set_elevation(cfg["elev"])
     b.set_nsteps(nsteps_per_source[:nsources])
-    x, y = cfg["x"], cfg["y"]
-    land = cfg["elev"] >= 0.0
-    sig = cfg["sigma_km"] * 1e3
     for k in range(nsources):
-        cx, cy = cfg["centres"][k]
-        # separable Gaussian (local equirectangular distance, as configs.gaussian_on_sphere)
-        gx = np.exp(-(np.radians(x - cx) * C.RADIUS_EARTH * math.cos(math.radians(cy))) ** 2 / (2 * sig ** 2))
-        gy = np.exp(-(np.radians(y - cy) * C.RADIUS_EARTH) ** 2 / (2 * sig ** 2))
-        stage = np.outer(gy, gx)
-        stage[land] = 0.0
-        b.set_source(k, stage)
+        b.set_source(k, C.unit_source_stage(cfg, k))
     b.setup_point_gauges(cfg["gauges_xy"], every=every, max_samples=max_samples)
     if block:
         b.set_launch(*block)

@@ -271,6 +271,16 @@ int swh_batch_set_nsteps(swh_batch *b, const int32_t *nsteps_per_source);
 /* Gauges sampled for every active source at the steps where step % every == 0
  * (step 0 = initial state), up to max_samples per source. */
 int swh_batch_gauges_set(swh_batch *b, int32_t n, const int32_t *ij, int32_t every, int32_t max_samples);
+/* Gauges sampled for every active source at the listed steps (strictly increasing,
+ * step 0 = initial state); a source records the listed steps up to its own step count.
+ * This carries the reference driver's write-time rule: generic_model.f90:380-397
+ * (examples/generic_example, the unit-source production driver) writes gauges when
+ * domain%time - last_write_time >= approximate_writeout_frequency, then advances
+ * last_write_time by that frequency, so the samples fall on the first step on or after
+ * each write time; the caller forms the steps from the FP64 time sequence of the
+ * leapfrog (two half-steps per step). Must be called before the first step. */
+int swh_batch_gauges_set_schedule(swh_batch *b, int32_t n, const int32_t *ij, int32_t nsamples,
+                                  const int64_t *sample_steps);
 /* Advance the batch nsteps leapfrog steps of size dt (the shared fixed step). */
 int swh_batch_evolve(swh_batch *b, double dt, int32_t nsteps);
 /* Gauge values of source k: values[(sample * 3 + var) * n + gauge]; nsamples out. */

@@ -19,6 +19,7 @@ _D = ctypes.c_double
 _I32 = ctypes.c_int32
 _PD = ctypes.POINTER(_D)
 _PI32 = ctypes.POINTER(_I32)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
 
 
 class DomainDesc(ctypes.Structure):
@@ -95,6 +96,7 @@ SYMBOLS = [
     ("swh_batch_download_source", ctypes.c_int, [ctypes.c_void_p, _I32, _PD]),
     ("swh_batch_set_nsteps", ctypes.c_int, [ctypes.c_void_p, _PI32]),
     ("swh_batch_gauges_set", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _I32, _I32]),
+    ("swh_batch_gauges_set_schedule", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _I32, _PI64]),
     ("swh_batch_evolve", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_batch_gauges_download", ctypes.c_int, [ctypes.c_void_p, _I32, _PD, _PI32]),
     ("swh_batch_status", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _PI32]),

@@ -14,6 +14,46 @@ from . import backend as B
 from .domain import build_metrics, _p
 
 
+def leapfrog_times(dt, nsteps):
+    """domain%time after 0..nsteps leapfrog steps of size dt, accumulated as the
+    reference does: two half-steps per step (domain_mod_linear_solver_include.f90:119,
+    571), in FP64."""
+    t = np.empty(int(nsteps) + 1)
+    x = 0.0
+    t[0] = x
+    h = float(dt) * 0.5
+    for n in range(1, int(nsteps) + 1):
+        x = x + h
+        x = x + h
+        t[n] = x
+    return t
+
+
+def reference_write_schedule(dt, final_time, writeout_frequency):
+    """Steps taken and gauge write steps of the reference's unit-source driver
+    (examples/generic_example/generic_model.f90:380-405): last_write_time starts at
+    -frequency; before each step, if time - last_write_time >= frequency the gauges
+    are written and last_write_time advances by the frequency; the run stops before the
+    first step taken with time > final_time. Returns (nsteps, sample_steps,
+    sample_times)."""
+    dt = float(dt)
+    freq = float(writeout_frequency)
+    last = -freq
+    steps, times = [], []
+    t, n, h = 0.0, 0, dt * 0.5
+    while True:
+        if t - last >= freq:
+            last = last + freq
+            steps.append(n)
+            times.append(t)
+        if t > final_time:
+            break
+        t = t + h
+        t = t + h
+        n += 1
+    return n, np.array(steps, dtype=np.int64), np.array(times)
+
+
 class UnitSourceBatch:
     def __init__(self, nx, ny, lw, lower_left, nsources, timestepping_method="linear", precision="fp32k",
                  spherical=True, coriolis=None, msl_linear=0.0, linear_friction_coeff=0.0, friction_type="manning"):
@@ -102,16 +142,26 @@ class UnitSourceBatch:
         B.check(self._lib.swh_batch_download_source(self._h, int(k), _p(U3)), "swh_batch_download_source")
         return U3
 
-    def setup_point_gauges(self, xy, every, max_samples):
+    def setup_point_gauges(self, xy, every=None, max_samples=None, sample_steps=None):
+        """Gauges at the cells nearest xy, sampled at steps 0, every, 2 every, ...
+        (max_samples of them), or at the given sample_steps (for example the write
+        steps of ``reference_write_schedule``)."""
         xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2)
         i = 1 + np.minimum(np.floor((xy[:, 0] - self.lower_left[0]) / self.dx[0]), self.nx - 1)
         j = 1 + np.minimum(np.floor((xy[:, 1] - self.lower_left[1]) / self.dx[1]), self.ny - 1)
         ij = np.ascontiguousarray(np.stack([i, j], axis=1).astype(np.int32))
         self.gauge_ij = ij
         self._ng = len(ij)
-        self._gmax = int(max_samples)
-        B.check(self._lib.swh_batch_gauges_set(self._h, len(ij), ij.ctypes.data_as(B._PI32), int(every),
-                                               int(max_samples)), "swh_batch_gauges_set")
+        if sample_steps is not None:
+            st = np.ascontiguousarray(sample_steps, dtype=np.int64)
+            self._gmax = len(st)
+            B.check(self._lib.swh_batch_gauges_set_schedule(self._h, len(ij), ij.ctypes.data_as(B._PI32), len(st),
+                                                            st.ctypes.data_as(B._PI64)),
+                    "swh_batch_gauges_set_schedule")
+        else:
+            self._gmax = int(max_samples)
+            B.check(self._lib.swh_batch_gauges_set(self._h, len(ij), ij.ctypes.data_as(B._PI32), int(every),
+                                                   int(max_samples)), "swh_batch_gauges_set")
         return ij
 
     def gauge_series(self, k):

@@ -151,6 +151,20 @@ def unit_sources(nx=1200, ny=1200, n_src_x=32, n_src_y=16, spacing_km=50.0, sigm
                 gauge_interval=60.0)
 
 
+def unit_source_stage(cfg, k):
+    """Initial stage of unit source k of ``unit_sources``: a 1 m Gaussian of width
+    sigma_km at the source centre (local equirectangular distance, evaluated as the
+    product of its longitude and latitude factors), zero on land."""
+    x, y = cfg["x"], cfg["y"]
+    cx, cy = cfg["centres"][k]
+    sig = cfg["sigma_km"] * 1e3
+    gx = np.exp(-(np.radians(x - cx) * RADIUS_EARTH * math.cos(math.radians(cy))) ** 2 / (2 * sig ** 2))
+    gy = np.exp(-(np.radians(y - cy) * RADIUS_EARTH) ** 2 / (2 * sig ** 2))
+    stage = np.outer(gy, gx)
+    stage[cfg["elev"] >= 0.0] = 0.0
+    return stage
+
+
 def gaussian_on_sphere(x, y, cx, cy, sigma_m, amp=1.0):
     """Gaussian hump of width sigma_m (metres) centred at (cx, cy) degrees, using the
     local equirectangular distance."""

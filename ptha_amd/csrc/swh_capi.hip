@@ -1451,7 +1451,8 @@ struct swh_batch {
     int cur = 0;                   // buffer holding the running sources' state
     std::vector<int> slot_buf;     // buffer holding each finished source's final state
     // gauges
-    int ng = 0, every = 1, gmax = 0;
+    int ng = 0, gmax = 0;
+    std::vector<long long> sched;  // steps (ascending) at which the gauges are sampled
     long long *d_gidx = nullptr;
     double *d_gval = nullptr;      // [slot][sample][3][ng]
     int block_x = 256, rows_per_block = 32, variant = 2, src_fast = 0;
@@ -1475,6 +1476,18 @@ __global__ void batch_gauges_kernel(const T *__restrict__ S, const T *__restrict
     v[2 * n + g] = (double)VH[o];
 }
 
+// Number of scheduled samples at steps <= s (so the index of the sample due at step s,
+// if there is one, is this count - 1).
+int samples_upto(const swh_batch *b, long long s)
+{
+    return (int)(std::upper_bound(b->sched.begin(), b->sched.end(), s) - b->sched.begin());
+}
+
+bool sample_due(const swh_batch *b, long long s)
+{
+    return b->ng && std::binary_search(b->sched.begin(), b->sched.end(), s);
+}
+
 int batch_active(const swh_batch *b, long long step)
 {
     // number of slots whose step count exceeds 'step' (slots sorted descending)
@@ -1490,9 +1503,8 @@ int batch_active(const swh_batch *b, long long step)
 template <typename T, bool KAHAN>
 int batch_sample(swh_batch *b, int n_active, int buf)
 {
-    if (!b->ng || n_active <= 0) return SWH_OK;
-    const long long sample = b->step / b->every;
-    if (sample >= b->gmax) return SWH_OK;
+    if (!sample_due(b, b->step) || n_active <= 0) return SWH_OK;
+    const long long sample = samples_upto(b, b->step) - 1;
     swh_domain *g = b->g;
     dim3 grid((b->ng + 63) / 64, n_active);
     hipLaunchKernelGGL(batch_gauges_kernel<T>, grid, dim3(64), 0, g->stream, (const T *)b->S[buf],
@@ -1581,19 +1593,19 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         const int n_active = batch_active(b, b->step);
         if (n_active == 0) break;
         const int c = b->cur, o = 1 - c;
-        const bool sample_next = b->ng && ((b->step + 1) % b->every) == 0;
+        const bool sample_next = sample_due(b, b->step + 1);
         int n_last;   // sources that took the last step of this round
         const int n2 = can2 ? batch_active(b, b->step + 1) : 0;   // sources with two or more steps left
         if (n2 > 0 && k + 1 < nsteps) {
             // those take both steps in one launch; the ones with one step left take
             // it in a one-step launch. A gauge sample due after the first step is
             // taken one step ahead from the current state (same operations).
-            if (sample_next && (b->step + 1) / b->every < b->gmax) {
+            if (sample_next) {
                 a.src_base = 0;
                 SWH_HIP(launch_gauge_step1<T, KAHAN>(batch_cur_args(b, a, c), b->g->coriolis, fric_mode,
                                                      b->g->truely_linear, b->g->lfc != 0.0, n_active, b->ng,
                                                      b->d_gidx, b->d_gval, b->gmax,
-                                                     (int)((b->step + 1) / b->every), b->g->stream));
+                                                     samples_upto(b, b->step + 1) - 1, b->g->stream));
             }
             rc = batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c);
             if (rc) return rc;
@@ -1615,7 +1627,7 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         }
         b->cur = o;
         for (int sl = batch_active(b, b->step); sl < n_active; sl++) b->slot_buf[sl] = o;
-        if (b->ng && (b->step % b->every) == 0 && n_last > 0) {
+        if (sample_due(b, b->step) && n_last > 0) {
             rc = batch_sample<T, KAHAN>(b, n_last, o);
             if (rc) return rc;
         }
@@ -1824,7 +1836,21 @@ int swh_batch_download_source(swh_batch *b, int32_t k, double *U3)
 
 int swh_batch_gauges_set(swh_batch *b, int32_t n, const int32_t *ij, int32_t every, int32_t max_samples)
 {
-    if (!b || n < 0 || (n > 0 && !ij) || every < 1 || max_samples < 1) return fail(SWH_EINVAL, "bad gauge arguments");
+    if (!b || every < 1 || max_samples < 1) return fail(SWH_EINVAL, "bad gauge arguments");
+    std::vector<int64_t> steps(max_samples);
+    for (int q = 0; q < max_samples; q++) steps[q] = (int64_t)q * every;
+    return swh_batch_gauges_set_schedule(b, n, ij, max_samples, steps.data());
+}
+
+int swh_batch_gauges_set_schedule(swh_batch *b, int32_t n, const int32_t *ij, int32_t nsamples,
+                                  const int64_t *sample_steps)
+{
+    if (!b || n < 0 || (n > 0 && !ij) || nsamples < 1 || !sample_steps) return fail(SWH_EINVAL, "bad gauge arguments");
+    for (int q = 0; q < nsamples; q++)
+        if (sample_steps[q] < 0 || (q > 0 && sample_steps[q] <= sample_steps[q - 1]))
+            return fail(SWH_EINVAL, "sample steps must be >= 0 and strictly increasing");
+    if (b->step > 0) return fail(SWH_ESTATE, "set the gauges before the first step");
+    const int max_samples = nsamples;
     swh_domain *g = b->g;
     std::vector<long long> idx(n);
     for (int q = 0; q < n; q++) {
@@ -1839,7 +1865,8 @@ int swh_batch_gauges_set(swh_batch *b, int32_t n, const int32_t *ij, int32_t eve
     if (!rc) rc = dmalloc((void **)&b->d_gval, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1) * max_samples * b->nsrc);
     if (rc) return rc;
     if (n > 0) SWH_HIP(hipMemcpy(b->d_gidx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
-    b->ng = n; b->every = every; b->gmax = max_samples;
+    b->ng = n; b->gmax = max_samples;
+    b->sched.assign(sample_steps, sample_steps + nsamples);
     return SWH_OK;
 }
 
@@ -1858,7 +1885,7 @@ int swh_batch_gauges_download(swh_batch *b, int32_t k, double *values, int32_t *
     SWH_HIP(hipStreamSynchronize(b->g->stream));
     const int slot = b->slot_of[k];
     const long long taken = std::min<long long>(b->step, b->nsteps_slot[slot]);
-    const long long ns = std::min<long long>(taken / b->every + 1, b->gmax);
+    const long long ns = samples_upto(b, taken);
     if (b->ng > 0 && ns > 0)
         SWH_HIP(hipMemcpy(values, b->d_gval + (size_t)slot * b->gmax * 3 * b->ng, sizeof(double) * 3 * b->ng * ns,
                           hipMemcpyDeviceToHost));
@@ -1973,7 +2000,7 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
     int ns_all = 0;
     for (int k = 0; k < b->nsrc; k++) {
         const long long taken = std::min<long long>(b->step, b->nsteps_slot[k]);
-        ns_slot[k] = (int)std::min<long long>(taken / b->every + 1, b->gmax);
+        ns_slot[k] = samples_upto(b, taken);
         ns_all = std::max(ns_all, ns_slot[k]);
     }
     const int TE = superpose_event_tile();
