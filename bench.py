@@ -46,7 +46,7 @@ def linear_timestep(elev, metrics, cfl=0.7):
     return float(ts * cfl)
 
 
-def build_batch(cfg, precision, nsources, nsteps_per_source, every, max_samples, block=None):
+def build_batch(cfg, precision, nsources, nsteps_per_source, sample_steps, block=None):
     from ptha_amd import configs as C
     from ptha_amd.batch import UnitSourceBatch
     b = UnitSourceBatch(cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], nsources, precision=precision)
@@ -54,7 +54,7 @@ def build_batch(cfg, precision, nsources, nsteps_per_source, every, max_samples,
     b.set_nsteps(nsteps_per_source[:nsources])
     for k in range(nsources):
         b.set_source(k, C.unit_source_stage(cfg, k))
-    b.setup_point_gauges(cfg["gauges_xy"], every=every, max_samples=max_samples)
+    b.setup_point_gauges(cfg["gauges_xy"], sample_steps=sample_steps)
     if block:
         b.set_launch(*block)
     return b
@@ -391,16 +391,21 @@ def main():
 
     from ptha_amd import configs as C
     from ptha_amd.domain import build_metrics
+    from ptha_amd.batch import reference_write_schedule
     block = tuple(int(v) for v in args.block.split(",")) if args.block else None
     log("building workload")
     cfg = C.unit_sources()
     nx, ny = cfg["nx"], cfg["ny"]
     m = build_metrics(nx, ny, cfg["lw"], cfg["ll"], True)
     dt = linear_timestep(cfg["elev"], m)
-    nsteps_src = np.ceil(cfg["durations"] / dt).astype(np.int32)
-    every = max(1, int(round(cfg["gauge_interval"] / dt)))
+    # steps and gauge write steps of the reference's unit-source driver
+    # (generic_model.f90:380-405): every source stops before the first step past its
+    # duration; gauges are written at the first step on or after each 60 s
+    plans = [reference_write_schedule(dt, float(d), cfg["gauge_interval"]) for d in cfg["durations"]]
+    nsteps_src = np.array([p[0] for p in plans], dtype=np.int32)
+    sched = plans[int(np.argmax(nsteps_src))][1]
     nsrc = args.sources
-    max_samples = int(nsteps_src.max()) // every + 2
+    max_samples = len(sched)
     total_steps = max(args.warmup + args.steps, 1)
     if nsteps_src[:nsrc].min() < total_steps:
         raise SystemExit("timed steps exceed the shortest source")
@@ -410,7 +415,7 @@ def main():
     tuned = {}
     for prec in (["fp32k"] if args.no_fp64 else ["fp32k", "fp64"]):
         log("timing", prec)
-        b = build_batch(cfg, prec, nsrc, nsteps_src, every, max_samples, block)
+        b = build_batch(cfg, prec, nsrc, nsteps_src, sched, block)
         shape = block
         if not block:
             shape, tune_ms = b.autotune(dt, reps=3)
@@ -431,7 +436,7 @@ def main():
     full = None
     if not args.no_full:
         log("whole configuration (every source to its duration)")
-        b = build_batch(cfg, "fp32k", nsrc, nsteps_src, every, max_samples, tuned.get("fp32k"))
+        b = build_batch(cfg, "fp32k", nsrc, nsteps_src, sched, tuned.get("fp32k"))
         b.synchronize()
         t0 = time.perf_counter()
         chunk = 2000
@@ -489,7 +494,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched unit sources: %d Gaussian sources on one 1200x1200 spherical 1-arcmin "
                                "'linear' leapfrog grid (Coriolis), FP32 state + Kahan-compensated stage" % nsrc,
-                   "nx": nx, "ny": ny, "sources": nsrc, "dt_s": dt, "gauges": 64, "gauge_every_steps": every,
+                   "nx": nx, "ny": ny, "sources": nsrc, "dt_s": dt, "gauges": 64, "gauge_write_s": cfg["gauge_interval"],
                    "launch_fp32k": list(tuned["fp32k"])},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),

@@ -178,6 +178,21 @@ int swh_get_scalars(swh_domain *h, swh_scalars *out);
 /* volume_interior (domain_mod.f90:1815-1849), FP64 compensated, deterministic. */
 int swh_volume_interior(swh_domain *h, double *out);
 
+/* compute_domain_statistics_new (domain_mod.f90:683-849) on the device: over the cells
+ * more than exterior_cells_width from the edge, the maximum and minimum stage and speed
+ * of wet cells and the potential and kinetic energy / rho (FP64, each row summed in
+ * column order and the rows in row order, as the reference's work arrays are). */
+typedef struct swh_domain_stats {
+    double max_stage, max_speed, min_stage, min_speed;
+    double energy_potential_on_rho, energy_kinetic_on_rho, energy_total_on_rho;
+} swh_domain_stats;
+int swh_domain_statistics(swh_domain *h, swh_domain_stats *out);
+
+/* The scan of check_multidomain_stability (multidomain_mod.f90:348-455) for one domain:
+ * *n_bad = number of interior values of stage, uh, vh, elevation that are NaN or beyond
+ * +-HUGE(1.0)/2. The reference stops the run when it is non-zero; the caller decides. */
+int swh_domain_check_stability(swh_domain *h, int64_t *n_bad);
+
 /* stationary_timestep_max (domain_mod.f90:1964-2022) on the current state. */
 int swh_stationary_timestep_max(swh_domain *h, double *out);
 
@@ -334,6 +349,14 @@ int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elap
  * ceiling quoted next to the roofline. */
 int swh_probe_copy(int64_t bytes, int32_t reps, double *ms_per_copy);
 
+/* Check utility (no counterpart in the reference): evaluates the FP64 friction pow
+ * pow(x[k], y[k]) and the boundary sine sin(t[k]) that the kernels use in place of
+ * libm's (where the reference calls glibc: Manning depth^(-7/3),
+ * domain_mod_update_U_DE1_inner_include.f90:74-75; the stage-function sine of the
+ * boundary conditions) on the device and on the host, for n arguments. */
+int swh_probe_math(int32_t n, const double *x, const double *y, const double *t, double *pow_dev, double *sin_dev,
+                   double *pow_host, double *sin_host);
+
 /* ---------------------------------------------------------------------------
  * Multidomain nesting on one GPU (multidomain_mod.f90, nested_grid_comms_mod.f90).
  *
@@ -395,6 +418,16 @@ int swh_md_evolve_one_step(swh_md *md, double dt);
 int swh_md_evolve_steps(swh_md *md, double dt, int32_t nsteps);
 /* md%get_flow_volume per domain: volume_in_priority_domain (domain_mod.f90:1851-1888). */
 int swh_md_volumes(swh_md *md, double *out);
+/* print_multidomain's statistics (multidomain_mod.f90:2301-2447): per_domain[k] (n entries,
+ * may be NULL) = domain k's compute_domain_statistics over its priority cells; global =
+ * their extrema and the energy sums in domain order; *energy_is_finite = 0 when the
+ * global total energy is NaN or infinite (the check the reference uses to stop unstable
+ * runs). Any output pointer may be NULL. */
+int swh_md_statistics(swh_md *md, swh_domain_stats *per_domain, swh_domain_stats *global,
+                      int32_t *energy_is_finite);
+/* check_multidomain_stability (multidomain_mod.f90:348-455) over every domain: *n_bad = the
+ * number of interior values that are NaN or beyond +-HUGE(1.0)/2. */
+int swh_md_check_stability(swh_md *md, int64_t *n_bad);
 /* The receive buffer of domain k's receive box c (both 1-based), as filled by the
  * last exchange (diagnostics). */
 int swh_md_download_recv_buffer(swh_md *md, int32_t k, int32_t c, double *out);

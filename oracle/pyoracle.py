@@ -80,6 +80,7 @@ def _load(kind, spherical):
     f("domain_dispersion_solve", None, [hd, _I, _I, _D])
     f("domain_set_record_max_U", None, [hd, _I])
     if kind == "ref":
+        f("domain_statistics", None, [hd, _PD])
         f("domain_sample", None, [hd, _I, ctypes.POINTER(_I), _PD])
         f("domain_set_forcing", None, [hd, _D, _D, _I, _I, _I, _I, _I, _I, _PD])
     _libs[key] = fns
@@ -237,6 +238,15 @@ class CpuDomain:
         assert w.shape == (k1 - k0 + 1, j1 - j0 + 1, i1 - i0 + 1)
         self.f["domain_set_forcing"](self.h, float(start_time), float(end_time), i0, i1, j0, j1, k0, k1, _p(w))
 
+    STATS_KEYS = ("max_stage", "max_speed", "min_stage", "min_speed", "energy_potential_on_rho",
+                  "energy_kinetic_on_rho", "energy_total_on_rho")
+
+    def statistics(self):
+        """compute_domain_statistics (compiled reference only)."""
+        out = np.zeros(7)
+        self.f["domain_statistics"](self.h, _p(out))
+        return dict(zip(self.STATS_KEYS, out.tolist()))
+
     def sample(self, ij):
         """[stage, uh, vh] (3, n) at 1-based (i, j) cells (compiled reference only)."""
         ij = np.ascontiguousarray(np.asarray(ij, dtype=np.int32).reshape(-1, 2))
@@ -284,6 +294,7 @@ def _load_md(spherical):
     f("comm", None, [_I, _I, _I, _PI, _PD])
     f("comm_buffer", None, [_I, _I, _I, _PD])
     f("volume", None, [_PD])
+    f("statistics", None, [_I, _PD])
     _libs[key] = fns
     return fns
 
@@ -403,6 +414,13 @@ class RefMultidomain:
         pnp = np.zeros((d["ny"], d["nx"]), dtype=np.int32)
         self.f["priority"](k + 1, _pi(pdi), _pi(pnp))
         return pdi, pnp
+
+    def statistics(self, k):
+        """Domain k's compute_domain_statistics (priority cells): max/min stage, max/min
+        speed, potential, kinetic, total energy / rho."""
+        out = np.zeros(7)
+        self.f["statistics"](k + 1, _p(out))
+        return out
 
     def all_dx_and_methods(self):
         dx = np.zeros((self.nd, 2))

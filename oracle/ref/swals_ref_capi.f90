@@ -259,6 +259,16 @@ contains
         v = pool(h)%volume_interior()
     end function
 
+    subroutine ref_domain_statistics(h, out) bind(C, name='ref_domain_statistics')
+        ! compute_domain_statistics (domain_mod.f90:683-849): max/min stage, max/min speed,
+        ! potential, kinetic and total energy / rho
+        integer(c_int), value :: h
+        real(c_double), intent(out) :: out(7)
+        real(dp) :: v(7)
+        call pool(h)%compute_domain_statistics(v(1), v(2), v(3), v(4), v(5), v(6), v(7))
+        out = real(v, c_double)
+    end subroutine
+
     function ref_domain_stationary_timestep_max(h) result(v) bind(C, name='ref_domain_stationary_timestep_max')
         integer(c_int), value :: h
         real(c_double) :: v
@@ -497,6 +507,15 @@ contains
         do n = 1, nsteps
             call md%evolve_one_step(real(dt, dp))
         end do
+    end subroutine
+
+    subroutine ref_md_statistics(k, out) bind(C, name='ref_md_statistics')
+        ! domain k's compute_domain_statistics inside the multidomain (priority cells only)
+        integer(c_int), value :: k
+        real(c_double), intent(out) :: out(7)
+        real(dp) :: v(7)
+        call md%domains(k)%compute_domain_statistics(v(1), v(2), v(3), v(4), v(5), v(6), v(7))
+        out = real(v, c_double)
     end subroutine
 
     subroutine ref_md_time(k, t) bind(C, name='ref_md_time')

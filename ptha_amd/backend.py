@@ -43,6 +43,12 @@ class Scalars(ctypes.Structure):
     ]
 
 
+class DomainStats(ctypes.Structure):
+    """swh_domain_stats (compute_domain_statistics_new's outputs)."""
+    _fields_ = [("max_stage", _D), ("max_speed", _D), ("min_stage", _D), ("min_speed", _D),
+                ("energy_potential_on_rho", _D), ("energy_kinetic_on_rho", _D), ("energy_total_on_rho", _D)]
+
+
 class NestBox(ctypes.Structure):
     """swh_nest_box: one send or receive box of a multidomain (1-based indices)."""
     _fields_ = [
@@ -83,6 +89,8 @@ SYMBOLS = [
     ("swh_get_scalars", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Scalars)]),
     ("swh_volume_interior", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_stationary_timestep_max", ctypes.c_int, [ctypes.c_void_p, _PD]),
+    ("swh_domain_statistics", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DomainStats)]),
+    ("swh_domain_check_stability", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     ("swh_download_max_U", ctypes.c_int, [ctypes.c_void_p, _I32, _PD]),
     ("swh_gauges_set", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _I32]),
     ("swh_gauges_sample", ctypes.c_int, [ctypes.c_void_p]),
@@ -117,6 +125,7 @@ SYMBOLS = [
     ("swh_adjoint_inject", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_adjoint_run", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _I32, _PD]),
     ("swh_probe_copy", ctypes.c_int, [ctypes.c_int64, _I32, _PD]),
+    ("swh_probe_math", ctypes.c_int, [_I32, _PD, _PD, _PD, _PD, _PD, _PD, _PD]),
     ("swh_md_create", ctypes.c_int, [_I32, ctypes.POINTER(ctypes.c_void_p), _PD, _PI32, _PI32,
                                      ctypes.POINTER(ctypes.c_void_p)]),
     ("swh_md_destroy", ctypes.c_int, [ctypes.c_void_p]),
@@ -129,6 +138,9 @@ SYMBOLS = [
     ("swh_md_evolve_steps", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_md_volumes", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_md_download_recv_buffer", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _PD]),
+    ("swh_md_statistics", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DomainStats), ctypes.POINTER(DomainStats),
+                                         _PI32]),
+    ("swh_md_check_stability", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     ("swh_md_synchronize", ctypes.c_int, [ctypes.c_void_p]),
 ]
 

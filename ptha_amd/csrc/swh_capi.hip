@@ -669,9 +669,26 @@ int domain_refresh_elevation(swh_domain *h)
     return SWH_OK;
 }
 
+int domain_stats_buffer(swh_domain *h)
+{
+    if (h->d_stats) return SWH_OK;
+    return dmalloc((void **)&h->d_stats, sizeof(double) * (6 * (size_t)h->ny + 8));
+}
+
 }  // namespace swh
 
 namespace {
+
+void stats_from(const double *v, swh_domain_stats *o)
+{
+    o->max_stage = v[0];
+    o->max_speed = v[1];
+    o->min_stage = v[2];
+    o->min_speed = v[3];
+    o->energy_potential_on_rho = v[4];
+    o->energy_kinetic_on_rho = v[5];
+    o->energy_total_on_rho = v[6];
+}
 
 int sync_scalars(swh_domain *h, DevScalars *out)
 {
@@ -840,7 +857,8 @@ int swh_domain_destroy(swh_domain *h)
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
-                    h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt};
+                    h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt,
+                    h->d_stats};
     for (void *p : ptrs) dfree(p);
     dfree(h->snap);
     if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
@@ -1070,6 +1088,34 @@ int swh_volume_interior(swh_domain *h, double *out)
                                      h->ny, h->pitch, h->n_ext, h->d_area, h->d_work, res, h->stream));
     SWH_HIP(hipStreamSynchronize(h->stream));
     SWH_HIP(hipMemcpy(out, res, sizeof(double), hipMemcpyDeviceToHost));
+    return SWH_OK;
+}
+
+int swh_domain_statistics(swh_domain *h, swh_domain_stats *out)
+{
+    if (!h || !out) return fail(SWH_EINVAL, "null argument");
+    int rc = domain_stats_buffer(h);
+    if (rc) return rc;
+    double *res = h->d_stats + 6 * (size_t)h->ny;
+    if ((rc = domain_statistics_launch(h, nullptr, h->d_stats, res, h->stream))) return rc;
+    double v[7];
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(v, res, sizeof v, hipMemcpyDeviceToHost));
+    stats_from(v, out);
+    return SWH_OK;
+}
+
+int swh_domain_check_stability(swh_domain *h, int64_t *n_bad)
+{
+    if (!h || !n_bad) return fail(SWH_EINVAL, "null argument");
+    int rc = domain_stats_buffer(h);
+    if (rc) return rc;
+    unsigned long long *cnt = (unsigned long long *)(h->d_stats + 6 * (size_t)h->ny + 7);
+    if ((rc = domain_stability_launch(h, cnt, h->stream))) return rc;
+    unsigned long long v = 0;
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(&v, cnt, sizeof v, hipMemcpyDeviceToHost));
+    *n_bad = (int64_t)v;
     return SWH_OK;
 }
 
@@ -1931,6 +1977,30 @@ int swh_probe_copy(int64_t bytes, int32_t reps, double *ms_per_copy)
     dfree(a);
     dfree(b);
     return rc;
+}
+
+int swh_probe_math(int32_t n, const double *x, const double *y, const double *t, double *pow_dev, double *sin_dev,
+                   double *pow_host, double *sin_host)
+{
+    if (n < 1 || !x || !y || !t || !pow_dev || !sin_dev || !pow_host || !sin_host)
+        return fail(SWH_EINVAL, "swh_probe_math: bad argument");
+    for (int k = 0; k < n; k++) {
+        pow_host[k] = swhm::pow_acc(x[k], y[k]);
+        sin_host[k] = swhm::sin_acc(t[k]);
+    }
+    double *d = nullptr;
+    const size_t B = sizeof(double) * (size_t)n;
+    int rc = dmalloc((void **)&d, 5 * B);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(d, x, B, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, y, B, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 2 * (size_t)n, t, B, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_math(d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, d + 4 * (size_t)n, n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(pow_dev, d + 3 * (size_t)n, B, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(sin_dev, d + 4 * (size_t)n, B, hipMemcpyDeviceToHost);
+    dfree(d);
+    if (e != hipSuccess) return hip_fail(e, "swh_probe_math", __FILE__, __LINE__);
+    return SWH_OK;
 }
 
 int swh_batch_autotune(swh_batch *b, double dt, int32_t reps, int32_t *best, double *best_ms)

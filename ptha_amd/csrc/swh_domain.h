@@ -48,6 +48,7 @@ struct swh_domain {
     double rc_dt = -1.0;
     DevScalars *sc = nullptr;
     double *d_work = nullptr;  // reductions scratch (2*ny + 8 doubles)
+    double *d_stats = nullptr; // statistics scratch: 6 row arrays, 7 results, a counter (6*ny + 8 doubles)
     // host-side scalars (leapfrog keeps time on the host; rk2 on the device)
     double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;
     long long nsteps = 0;
@@ -101,4 +102,12 @@ int domain_evolve(swh_domain *h, double dt);
 // from it (the reference builds friction_work on the first step, after any halo
 // exchange has set the elevation in the halos).
 int domain_refresh_elevation(swh_domain *h);
+// compute_domain_statistics_new on the device (swh_stats.hip): rows = 6*ny doubles of
+// scratch, out = 7 doubles (max stage, max speed, min stage, min speed, potential,
+// kinetic, total energy / rho); pmask = priority mask (nx, ny) or null.
+int domain_statistics_launch(swh_domain *h, const int *pmask, double *rows, double *out, hipStream_t stream);
+// Stability scan: count (cleared here) of interior values that are NaN or beyond +-HUGE(1.0)/2.
+int domain_stability_launch(swh_domain *h, unsigned long long *count, hipStream_t stream);
+// Allocates h->d_stats on first use.
+int domain_stats_buffer(swh_domain *h);
 }  // namespace swh

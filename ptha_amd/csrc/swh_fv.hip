@@ -482,7 +482,7 @@ __global__ void fv_update_kernel(FVArgs<T> a)
         un[2] = 0.0;
     } else {
         const double uu = (double)C.u, vv = (double)C.v;
-        const double fs = (double)a.nsq[o] * sqrt(uu * uu + vv * vv) * pow(fmax_(depth, kMinDepth), a.fric_power);
+        const double fs = (double)a.nsq[o] * sqrt(uu * uu + vv * vv) * pow_friction(fmax_(depth, kMinDepth), a.fric_power);
         const double implicit_factor = 1.0 / (1.0 + (dt * kGravity) * fmax_(depth, 0.0) * fs + dt * a.lfc);
         un[1] = un[1] + inv_cell_area_dt * (a.esU[o] + a.esE[fo + 1]);
         un[1] = un[1] * implicit_factor;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(1024) void fv_stage_bc_kernel(T *S, T *UH, T *VH, i
                                                            double amp, double period, double msl, const DevScalars *sc)
 {
     const double two_pi = 2.0 * acos(-1.0);
-    const T stage = (T)(msl + amp * sin(two_pi * sc->time / period));
+    const T stage = (T)(msl + amp * swhm::sin_acc(two_pi * sc->time / period));
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         const long long o = (long long)j * pitch;
         if (m & 8) { S[o] = stage; UH[o] = UH[o + 1]; VH[o] = T(0); }

@@ -8,6 +8,13 @@
 #include <vector>
 
 #include "swals_hip.h"
+#include "swh_mathx.h"
+
+
+// Friction depth power (Manning depth^(-7/3), Chezy depth^(-2)): FP64 through the
+// accurate pow of swh_mathx.h, FP32 through the library's powf.
+__device__ inline double pow_friction(double x, double p) { return swhm::pow_acc(x, p); }
+__device__ inline float pow_friction(float x, float p) { return powf(x, p); }
 
 namespace swh {
 
@@ -127,6 +134,8 @@ hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode
 
 // Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
 hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
+hipError_t launch_probe_math(const double *x, const double *y, const double *t, double *pw, double *sn, int n,
+                             hipStream_t s);
 
 // Events from unit-source gauge series (swh_superpose.hip): out = W * (X - msl on the
 // stage) scaled and shifted per event.

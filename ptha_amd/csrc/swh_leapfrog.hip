@@ -295,8 +295,8 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
                             const T dep_j = dmaxT(HALF * (s0 - e0 + sup - eu), MIN);
                             const T nsq_i = HALF * (n0 + ns[o + 1]);
                             const T nsq_j = HALF * (n0 + ns[o + a.pitch]);
-                            fwu = T(kGravity) * nsq_i * pow(dep_i, a.fric_power);
-                            fwv = T(kGravity) * nsq_j * pow(dep_j, a.fric_power);
+                            fwu = T(kGravity) * nsq_i * pow_friction(dep_i, a.fric_power);
+                            fwv = T(kGravity) * nsq_j * pow_friction(dep_j, a.fric_power);
                         }
                         const T vbar = HALF * (vh_jmh + vh_jph);
                         const T ubar = HALF * (uh_ij + uh_ijp1);

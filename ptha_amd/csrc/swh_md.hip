@@ -6,6 +6,7 @@
 // exchange on the multidomain stream (pack -> unpack -> staggered fix-ups -> flux
 // correction -> re-interpolation of fine receive boxes).
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -828,6 +829,69 @@ int swh_md_volumes(swh_md *md, double *out)
     }
     SWH_HIP(hipStreamSynchronize(md->stream));
     SWH_HIP(hipMemcpy(out, md->d_vol, md->nd * sizeof(double), hipMemcpyDeviceToHost));
+    return fork_domains(md);
+}
+
+int swh_md_statistics(swh_md *md, swh_domain_stats *per_domain, swh_domain_stats *global, int32_t *energy_is_finite)
+{
+    if (!md || !md->finalized) return md_fail(SWH_EINVAL, "swh_md_statistics: bad argument");
+    int rc = join_domains(md);
+    if (rc) return rc;
+    for (int k = 0; k < md->nd; k++) {
+        swh_domain *h = md->dom[k];
+        if ((rc = domain_stats_buffer(h))) return rc;
+        if ((rc = domain_statistics_launch(h, md->d_pnp[k], h->d_stats, h->d_stats + 6 * (size_t)h->ny, md->stream)))
+            return rc;
+    }
+    SWH_HIP(hipStreamSynchronize(md->stream));
+    // print_multidomain's accumulation over domains, in domain order
+    double gmaxs = -DBL_MAX, gmins = DBL_MAX, gmaxv = 0.0, gminv = DBL_MAX, gpot = 0.0, gkin = 0.0, gtot = 0.0;
+    for (int k = 0; k < md->nd; k++) {
+        swh_domain *h = md->dom[k];
+        double v[7];
+        SWH_HIP(hipMemcpy(v, h->d_stats + 6 * (size_t)h->ny, sizeof v, hipMemcpyDeviceToHost));
+        if (per_domain) {
+            swh_domain_stats &o = per_domain[k];
+            o.max_stage = v[0]; o.max_speed = v[1]; o.min_stage = v[2]; o.min_speed = v[3];
+            o.energy_potential_on_rho = v[4]; o.energy_kinetic_on_rho = v[5]; o.energy_total_on_rho = v[6];
+        }
+        gmaxs = std::max(gmaxs, v[0]);
+        gmins = std::min(gmins, v[2]);
+        gmaxv = std::max(gmaxv, v[1]);
+        gminv = std::min(gminv, v[3]);
+        gpot = gpot + v[4];
+        gkin = gkin + v[5];
+        gtot = gtot + v[6];
+    }
+    if (global) {
+        global->max_stage = gmaxs; global->max_speed = gmaxv; global->min_stage = gmins; global->min_speed = gminv;
+        global->energy_potential_on_rho = gpot; global->energy_kinetic_on_rho = gkin;
+        global->energy_total_on_rho = gtot;
+    }
+    if (energy_is_finite) *energy_is_finite = (gtot > DBL_MAX || gtot != gtot) ? 0 : 1;
+    return fork_domains(md);
+}
+
+int swh_md_check_stability(swh_md *md, int64_t *n_bad)
+{
+    if (!md || !n_bad || !md->finalized) return md_fail(SWH_EINVAL, "swh_md_check_stability: bad argument");
+    int rc = join_domains(md);
+    if (rc) return rc;
+    for (int k = 0; k < md->nd; k++) {
+        swh_domain *h = md->dom[k];
+        if ((rc = domain_stats_buffer(h))) return rc;
+        if ((rc = domain_stability_launch(h, (unsigned long long *)(h->d_stats + 6 * (size_t)h->ny + 7), md->stream)))
+            return rc;
+    }
+    SWH_HIP(hipStreamSynchronize(md->stream));
+    int64_t total = 0;
+    for (int k = 0; k < md->nd; k++) {
+        swh_domain *h = md->dom[k];
+        unsigned long long v = 0;
+        SWH_HIP(hipMemcpy(&v, h->d_stats + 6 * (size_t)h->ny + 7, sizeof v, hipMemcpyDeviceToHost));
+        total += (int64_t)v;
+    }
+    *n_bad = total;
     return fork_domains(md);
 }
 

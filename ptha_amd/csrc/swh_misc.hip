@@ -263,7 +263,7 @@ __global__ __launch_bounds__(1024) void edge_bc_kernel(T *S, T *C, T *UH, T *VH,
     double stage_b = msl_linear;
     if (use_fun) {
         stage_b = fun_stage;
-        if (sc) stage_b = fun_msl + amp * sin(2.0 * acos(-1.0) * sc->time / period);
+        if (sc) stage_b = fun_msl + amp * swhm::sin_acc(2.0 * acos(-1.0) * sc->time / period);
     }
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         const long long o = (long long)j * pitch;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(1024) void flather_kernel(T *S, T *C, T *UH, T *VH,
     double stage_fun = fun_stage;
     if (use_fun && sc) {
         const double two_pi = 2.0 * acos(-1.0);
-        stage_fun = fun_msl + amp * sin(two_pi * sc->time / period);
+        stage_fun = fun_msl + amp * swhm::sin_acc(two_pi * sc->time / period);
     }
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         const long long o = (long long)j * pitch;
@@ -655,6 +655,25 @@ hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStr
     const long long blocks = std::min<long long>((n + 255) / 256, 256LL * 64);
     hipLaunchKernelGGL(probe_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)src,
                        (float4 *)dst, n);
+    return hipGetLastError();
+}
+
+// ---- the FP64 friction pow and boundary sine (swh_mathx.h) on the device ----
+// One argument per thread; the same functions compiled for the host give the same bits.
+__global__ __launch_bounds__(256) void probe_math_kernel(const double *__restrict__ x, const double *__restrict__ y,
+                                                         const double *__restrict__ t, double *__restrict__ pw,
+                                                         double *__restrict__ sn, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    pw[k] = swhm::pow_acc(x[k], y[k]);
+    sn[k] = swhm::sin_acc(t[k]);
+}
+
+hipError_t launch_probe_math(const double *x, const double *y, const double *t, double *pw, double *sn, int n,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(probe_math_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, t, pw, sn, n);
     return hipGetLastError();
 }
 

@@ -277,8 +277,8 @@ __global__ void nl_momentum_kernel(NLArgs<T> a)
         const T nsq0 = a.nsq[o];
         const T dep_i = dmaxT_nl(H * (s0 - e0 + se - ee), MIN);
         const T dep_j = dmaxT_nl(H * (s0 - e0 + sn - en), MIN);
-        const T fwu = G * (H * (nsq0 + a.nsq[oe])) * pow(dep_i, a.fric_power);
-        const T fwv = G * (H * (nsq0 + a.nsq[on])) * pow(dep_j, a.fric_power);
+        const T fwu = G * (H * (nsq0 + a.nsq[oe])) * pow_friction(dep_i, a.fric_power);
+        const T fwv = G * (H * (nsq0 + a.nsq[on])) * pow_friction(dep_j, a.fric_power);
         const T vbar = H * (vh_jmh + vh_jph);
         const T ubar = H * (uh_j + uh_jp1);
         const T fm_uh = ONE / (ONE + dt * a.lfc + dt * fwu * sqrt(uh * uh + vbar * vbar));

@@ -240,6 +240,22 @@ class Domain:
         """volume_interior + boundary_flux_time_integral (domain_mod.f90:1891-1904)."""
         return self.volume_interior() + self.scalars()["boundary_flux_time_integral"]
 
+    STATS_KEYS = ("max_stage", "max_speed", "min_stage", "min_speed", "energy_potential_on_rho",
+                  "energy_kinetic_on_rho", "energy_total_on_rho")
+
+    def statistics(self):
+        """compute_domain_statistics (domain_mod.f90:683-849) on the device, as a dict."""
+        st = B.DomainStats()
+        B.check(self._lib.swh_domain_statistics(self._h, ctypes.byref(st)), "swh_domain_statistics")
+        return {k: getattr(st, k) for k in self.STATS_KEYS}
+
+    def check_stability(self):
+        """Number of interior values that are NaN or beyond +-HUGE(1.0)/2
+        (check_multidomain_stability's scan, multidomain_mod.f90:348-455)."""
+        n = ctypes.c_int64()
+        B.check(self._lib.swh_domain_check_stability(self._h, ctypes.byref(n)), "swh_domain_check_stability")
+        return n.value
+
     def stationary_timestep_max(self):
         v = ctypes.c_double()
         B.check(self._lib.swh_stationary_timestep_max(self._h, ctypes.byref(v)), "swh_stationary_timestep_max")

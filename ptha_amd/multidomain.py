@@ -131,6 +131,25 @@ class MultiDomain:
         B.check(self._lib.swh_md_volumes(self._h, _p(v)), "swh_md_volumes")
         return v
 
+    def statistics(self):
+        """print_multidomain's statistics (multidomain_mod.f90:2301-2447): (per-domain list
+        of dicts over each domain's priority cells, global dict, energy_is_finite)."""
+        nd = len(self.domains)
+        per = (B.DomainStats * nd)()
+        glob = B.DomainStats()
+        fin = ctypes.c_int32()
+        B.check(self._lib.swh_md_statistics(self._h, per, ctypes.byref(glob), ctypes.byref(fin)), "swh_md_statistics")
+        keys = [f[0] for f in B.DomainStats._fields_]
+        return ([{k: getattr(per[i], k) for k in keys} for i in range(nd)], {k: getattr(glob, k) for k in keys},
+                bool(fin.value))
+
+    def check_stability(self):
+        """Number of interior values over all domains that are NaN or beyond +-HUGE(1.0)/2
+        (check_multidomain_stability, multidomain_mod.f90:348-455)."""
+        n = ctypes.c_int64()
+        B.check(self._lib.swh_md_check_stability(self._h, ctypes.byref(n)), "swh_md_check_stability")
+        return n.value
+
     def recv_buffer(self, k, c):
         """Receive buffer of domain k's receive box c (0-based), from the last exchange."""
         b = np.zeros(self._recv_sizes[k][c])

@@ -68,6 +68,10 @@ def test_config2_fullsize(precision):
     wet = (z["max_stage"] - z["elev"]) > 1e-2
     res = dict(steps=int(z["nsteps"]), time=float(d.time), ref_time=float(z["final_time"]),
                gauge_stage=np.abs(vals[:, 0] - ref_series[:, 0]).max(),
+               # per gauge: the error over the gauge's own range of stage in the run
+               gauge_stage_rel=(np.abs(vals[:, 0] - ref_series[:, 0]).max(axis=0) /
+                                np.ptp(ref_series[:, 0], axis=0)).max(),
+               gauge_stage_rel_to_max_amp=np.abs(vals[:, 0] - ref_series[:, 0]).max() / np.abs(ref_series[:, 0]).max(),
                gauge_uh=np.abs(vals[:, 1] - ref_series[:, 1]).max(),
                max_stage=np.abs(d.max_U("max_stage")[::sub, ::sub] - z["max_stage"])[wet].max(),
                final_stage=np.abs(U[0][::sub, ::sub] - z["stage"])[wet].max(),

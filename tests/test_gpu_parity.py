@@ -173,7 +173,7 @@ def test_fv_fp64(oracle_built, case, method):
     wet = cfg["U"][3] < 1e5
     for k in range(3):
         d = np.abs(Ug[k] - Uo[k])[wet].max()
-        print(method, "plane", k, "max diff", d)
+        print(method, case, "plane", k, "max diff", d, "cells differing", int((Ug[k] != Uo[k]).sum()), flush=True)
         assert d <= 1e-9, k
     sg, so = g.scalars(), o.scalars()
     assert sg["negative_depth_fix_counter"] == so["negative_depth_fix_counter"]
