@@ -188,6 +188,16 @@ typedef struct swh_domain_stats {
 } swh_domain_stats;
 int swh_domain_statistics(swh_domain *h, swh_domain_stats *out);
 
+/* swh_domain_statistics without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same inputs,
+ * outputs and kernels; the row sums live in a static device array of the library
+ * (786 KB per GPU, held until the process ends) instead of a buffer the handle
+ * allocates on first use. Domains of more than 16384 rows are rejected (SWH_EINVAL).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams or host threads) race and must be serialised by the caller. */
+int swh_domain_statistics_noalloc(swh_domain *h, swh_domain_stats *out);
+
 /* The scan of check_multidomain_stability (multidomain_mod.f90:348-455) for one domain:
  * *n_bad = number of interior values of stage, uh, vh, elevation that are NaN or beyond
  * +-HUGE(1.0)/2. The reference stops the run when it is non-zero; the caller decides. */

@@ -90,6 +90,7 @@ SYMBOLS = [
     ("swh_volume_interior", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_stationary_timestep_max", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_domain_statistics", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DomainStats)]),
+    ("swh_domain_statistics_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DomainStats)]),
     ("swh_domain_check_stability", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     ("swh_download_max_U", ctypes.c_int, [ctypes.c_void_p, _I32, _PD]),
     ("swh_gauges_set", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _I32]),

@@ -13,6 +13,7 @@
 // of stage, uh, vh and elevation that are NaN or beyond +-HUGE(1.0)/2 (the reference's
 // single-precision bound, used to stop a run that has blown up).
 #include <cfloat>
+#include <mutex>
 
 #include "swh_domain.h"
 
@@ -184,6 +185,17 @@ __global__ __launch_bounds__(256) void stability_kernel(const T *__restrict__ S,
 
 __global__ void clear_u64_kernel(unsigned long long *p) { *p = 0ULL; }
 
+__global__ void stats_empty_kernel(double *out)
+{
+    out[0] = -DBL_MAX;
+    out[1] = 0.0;
+    out[2] = DBL_MAX;
+    out[3] = sqrt(DBL_MAX);
+    out[4] = 0.0;
+    out[5] = 0.0;
+    out[6] = 0.0;
+}
+
 }  // namespace
 
 int domain_statistics_launch(swh_domain *h, const int *pmask, double *rows, double *out, hipStream_t stream)
@@ -192,8 +204,8 @@ int domain_statistics_launch(swh_domain *h, const int *pmask, double *rows, doub
     const int ecw = h->n_ext;
     if (h->ny - 2 * ecw <= 0 || h->nx - 2 * ecw <= 0) {
         // no interior cell: the reference's initial values
-        const double init[7] = {-DBL_MAX, 0.0, DBL_MAX, sqrt(DBL_MAX), 0.0, 0.0, 0.0};
-        SWH_HIP(hipMemcpyAsync(out, init, sizeof init, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(stats_empty_kernel, dim3(1), dim3(1), 0, stream, out);
+        SWH_HIP(hipGetLastError());
         return SWH_OK;
     }
     const int blocks = (h->ny - 2 * ecw + 3) / 4;
@@ -236,3 +248,79 @@ int domain_stability_launch(swh_domain *h, unsigned long long *count, hipStream_
 }
 
 }  // namespace swh
+
+// ---------------------------------------------------------------------------
+// The statistics entry point that allocates nothing (swh_domain_statistics_noalloc):
+// its row sums and results live in this file-scope device array instead of the
+// handle's lazily allocated buffer. Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (the array's name is a host-side stand-in).
+// ---------------------------------------------------------------------------
+// 6 row arrays of up to 16384 rows, 7 results, a spare slot: 786 KB per GPU
+constexpr int kScratchMaxRows = 16384;
+constexpr size_t kScratchDoubles = 6 * (size_t)kScratchMaxRows + 8;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace swh {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch array, looked up once per device with
+// that device current.
+int scratch_on_current_device(double **out)
+{
+    int dev = 0;
+    SWH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("swh_domain_statistics_noalloc: device index out of range");
+        return SWH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        SWH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SWH_OK;
+}
+
+}  // namespace
+
+}  // namespace swh
+
+extern "C" int swh_domain_statistics_noalloc(swh_domain *h, swh_domain_stats *out)
+{
+    using namespace swh;
+    if (!h || !out) {
+        set_error("swh_domain_statistics_noalloc: null argument");
+        return SWH_EINVAL;
+    }
+    if (h->ny > kScratchMaxRows) {
+        set_error("swh_domain_statistics_noalloc: the domain has " + std::to_string(h->ny) +
+                  " rows, more than the scratch array's maximum of " + std::to_string(kScratchMaxRows) +
+                  " (use swh_domain_statistics)");
+        return SWH_EINVAL;
+    }
+    double *buf = nullptr;
+    int rc = scratch_on_current_device(&buf);
+    if (rc) return rc;
+    // every value the final pass reads is written earlier in this call (the interior
+    // rows, then the results), so nothing needs clearing
+    double *res = buf + 6 * (size_t)h->ny;
+    if ((rc = domain_statistics_launch(h, nullptr, buf, res, h->stream))) return rc;
+    double v[7];
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(v, res, sizeof v, hipMemcpyDeviceToHost));
+    out->max_stage = v[0];
+    out->max_speed = v[1];
+    out->min_stage = v[2];
+    out->min_speed = v[3];
+    out->energy_potential_on_rho = v[4];
+    out->energy_kinetic_on_rho = v[5];
+    out->energy_total_on_rho = v[6];
+    return SWH_OK;
+}
