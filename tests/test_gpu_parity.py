@@ -156,25 +156,34 @@ def _fv_case(case, method):
 def test_fv_fp64(oracle_built, case, method):
     """The finite-volume schemes (DE1 low-Froude HLL, wet/dry, Manning): rk2, euler,
     midpoint and rk2n, with adaptive CFL steps decided on the device, against the
-    oracle. Device pow/sin vs glibc differ by ulps, so the bar is round-off: states
-    within 1e-9 (absolute, metres and m^2/s), max_dt and time within 1e-12 relative."""
+    oracle: bit for bit (states, every step's max_dt and time). The device's friction
+    pow and boundary sine are correctly rounded; glibc's (which the oracle and the
+    reference call) are not in about 1 call in 1200, which these cases never meet
+    except rk2n on the beach: at its default CFL (0.99) rk2n is unstable there (a 1 m
+    wave grows to 68 m in 200 steps in the reference itself), so such one-ulp
+    differences grow, and its bar is round-off: states within 1e-9, scalars 1e-12."""
     _print_case(method + "_" + case)
     cfg, nsteps = _fv_case(case, method)
     o, _, hist = H.cpu_run("oracle", cfg, nsteps, record=lambda d: d.scalars())
     g = H.gpu_domain(cfg)
+    exact = not (method == "rk2n" and case == "beach")
     for k in range(nsteps):
         g.evolve_one_step()
         sg = g.scalars()
-        assert sg["max_dt"] == pytest.approx(hist[k]["max_dt"], rel=1e-12)
-        assert sg["time"] == pytest.approx(hist[k]["time"], rel=1e-12)
-        assert sg["evolve_step_dt"] == pytest.approx(hist[k]["evolve_step_dt"], rel=1e-12)
-        assert sg["dt_last_update"] == pytest.approx(hist[k]["dt_last_update"], rel=1e-12)
+        for q in ["max_dt", "time", "evolve_step_dt", "dt_last_update"]:
+            if exact:
+                assert sg[q] == hist[k][q], (k, q)
+            else:
+                assert sg[q] == pytest.approx(hist[k][q], rel=1e-12), (k, q)
     Ug, Uo = g.U, o.get_U()
     wet = cfg["U"][3] < 1e5
     for k in range(3):
         d = np.abs(Ug[k] - Uo[k])[wet].max()
         print(method, case, "plane", k, "max diff", d, "cells differing", int((Ug[k] != Uo[k]).sum()), flush=True)
-        assert d <= 1e-9, k
+        if exact:
+            assert np.array_equal(Ug[k][wet], Uo[k][wet]), k
+        else:
+            assert d <= 1e-9, k
     sg, so = g.scalars(), o.scalars()
     assert sg["negative_depth_fix_counter"] == so["negative_depth_fix_counter"]
     assert sg["boundary_flux_time_integral"] == pytest.approx(so["boundary_flux_time_integral"], rel=1e-9, abs=1e-9)
@@ -200,6 +209,8 @@ def test_fv_fixed_timestep_fp64(oracle_built, method):
     assert sg["boundary_flux_time_integral"] == pytest.approx(so["boundary_flux_time_integral"], rel=1e-9, abs=1e-9)
     Ug, Uo = g.U, o.get_U()
     for k in range(3):
+        print(method, "fixed dt plane", k, "max diff", np.abs(Ug[k] - Uo[k]).max(), "cells differing",
+              int((Ug[k] != Uo[k]).sum()), flush=True)
         assert np.abs(Ug[k] - Uo[k]).max() <= 1e-9, k
     g.close()
 
