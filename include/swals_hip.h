@@ -319,13 +319,23 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active);
  * 16 = (batches) two fused steps per launch wherever no gauge sample falls between
  * them (friction modes without per-step friction work only), 32 = (single grids)
  * one step per launch in swh_evolve_steps (by default pairs of steps run as one
- * two-step launch where no stage boundary, nesting or max_U update falls between).
+ * two-step launch where no stage boundary, nesting or max_U update falls between),
+ * 64 = (batches, two-step launches) XCD-aware tile order: each of the 8 XCDs walks a
+ * contiguous range of tiles, columns fastest, then the sources of a group of
+ * 2^((variant >> 7) & 7) sources, then strips, so neighbouring tiles, the sources of a
+ * group (same elevation) and consecutive strips share its L2 (bit 2 also keeps the
+ * next row's loads in flight in the two-step kernel), bits 1024..4096 = (batches, with
+ * 16) up to 2 + ((variant >> 10) & 7) fused steps per launch (at most 6) wherever no
+ * gauge sample falls inside them.
  * Results do not depend on the launch shape. */
 int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, int32_t variant);
-/* Launch-shape autotuner (no counterpart in the reference): times every shape
- * (variant 0..15 x block_x {64,128,256} x rows {16,32,64}) over reps repeats of the
- * next step on this batch's own grid and running sources, keeps the fastest, and
- * returns it in best[3] = {block_x, rows_per_block, variant} with its time per step.
+/* Launch-shape autotuner (no counterpart in the reference): times every one-step shape
+ * (variant 0..15 x block_x {64,128,256} x rows {16,32,64}) and every fused shape (2..5
+ * steps per launch x block_x {64,128,256} x rows {32,64,128,256} x tile order:
+ * hardware, or XCD order with sources interleaved in groups of 16) over reps repeats
+ * of the next launch on this batch's own grid and running sources, keeps the fastest
+ * per cell-update, and returns it in best[3] = {block_x, rows_per_block, variant} with
+ * its time per step.
  * The state and step count are left as they were. Needs the step counts and the
  * elevation set. */
 int swh_batch_autotune(swh_batch *b, double dt, int32_t reps, int32_t *best, double *best_ms);

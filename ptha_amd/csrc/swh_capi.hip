@@ -1503,6 +1503,9 @@ struct swh_batch {
     double *d_gval = nullptr;      // [slot][sample][3][ng]
     int block_x = 256, rows_per_block = 32, variant = 2, src_fast = 0;
     int two_step = 0;              // two fused steps per launch where possible
+    int fuse = 2;                  // most fused steps per launch (2..4) where two_step is on
+    int xcd_order = 0;             // two-step launches: XCD-aware tile order
+    int src_group = 1;             // ... interleaving the tiles of this many sources
 };
 
 namespace {
@@ -1617,9 +1620,32 @@ int batch_launch2(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     a.src_base = first;
     a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
     a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
+    a.lf2_order = b->xcd_order;
+    a.lf2_group = b->src_group;
     SWH_HIP(launch_leapfrog2<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n, b->block_x,
-                                       g->stream));
+                                       g->stream, (b->variant & 2) ? 1 : 0));
     a.src_base = 0;
+    a.lf2_order = 0;
+    return SWH_OK;
+}
+
+// One launch advancing slots [first, first + n) by ns (3 or 4) steps (reads buffer c).
+template <typename T, bool KAHAN>
+int batch_launchn(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, int c, int ns)
+{
+    swh_domain *g = b->g;
+    const int o = 1 - c;
+    a.rows_per_block = b->rows_per_block;
+    a.src_fast = 0;
+    a.src_base = first;
+    a.S = (const T *)b->S[c]; a.C = (const T *)b->C[c]; a.UH = (const T *)b->UH[c]; a.VH = (const T *)b->VH[c];
+    a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
+    a.lf2_order = b->xcd_order;
+    a.lf2_group = b->src_group;
+    SWH_HIP(launch_leapfrogn<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n, b->block_x,
+                                       g->stream, (b->variant & 2) ? 1 : 0, ns));
+    a.src_base = 0;
+    a.lf2_order = 0;
     return SWH_OK;
 }
 
@@ -1635,6 +1661,7 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         if (rc) return rc;
     }
     const bool can2 = b->two_step && fric_mode <= 1 && (long long)b->g->plane_elems() < (1LL << 31);
+    const int depth = can2 ? std::max(2, std::min(6, b->fuse)) : 1;   // most steps per launch
     for (int k = 0; k < nsteps;) {
         const int n_active = batch_active(b, b->step);
         if (n_active == 0) break;
@@ -1642,7 +1669,38 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         const bool sample_next = sample_due(b, b->step + 1);
         int n_last;   // sources that took the last step of this round
         const int n2 = can2 ? batch_active(b, b->step + 1) : 0;   // sources with two or more steps left
-        if (n2 > 0 && k + 1 < nsteps) {
+        // the deepest round (3 or 4 steps) that no gauge sample falls inside
+        int D = 0;
+        if (depth >= 3 && k + 2 < nsteps && !sample_next && !sample_due(b, b->step + 2) &&
+            batch_active(b, b->step + 2) > 0) {
+            D = 3;
+            while (D < depth && k + D < nsteps && !sample_due(b, b->step + D) && batch_active(b, b->step + D) > 0) D++;
+        }
+        if (D >= 3) {
+            // D steps in one launch for the sources with D or more steps left; those with
+            // fewer take them in shallower launches, all reading this round's buffer
+            int hi = n_active;   // sources [0, hi) have at least d steps left in the loop below
+            for (int d = 1; d <= D; d++) {
+                const int nd = d < D ? batch_active(b, b->step + d) : 0;   // sources with more than d steps left
+                const int lo = nd;
+                if (hi > lo) {
+                    if (d == 1) {
+                        a.src_base = lo;
+                        rc = batch_launch<T, KAHAN>(b, a, fric_mode, hi - lo, c);
+                        a.src_base = 0;
+                    } else if (d == 2) {
+                        rc = batch_launch2<T, KAHAN>(b, a, fric_mode, lo, hi - lo, c);
+                    } else {
+                        rc = batch_launchn<T, KAHAN>(b, a, fric_mode, lo, hi - lo, c, d);
+                    }
+                    if (rc) return rc;
+                }
+                hi = lo;
+            }
+            n_last = batch_active(b, b->step + D - 1);
+            b->step += D;
+            k += D;
+        } else if (n2 > 0 && k + 1 < nsteps) {
             // those take both steps in one launch; the ones with one step left take
             // it in a one-step launch. A gauge sample due after the first step is
             // taken one step ahead from the current state (same operations).
@@ -1682,11 +1740,12 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
 }
 
 // Launch-shape autotuner: times every candidate shape on the batch's own grid and
-// sources by repeating the next step (reads the current buffer, writes the other,
-// which the next real step overwrites), so the state and step count are unchanged.
-// One-step shapes: variant 0..15 x block_x {64,128,256} x rows {16,32,64}; then the
-// two-steps-per-launch kernel (variant bit 16) over block_x x rows {16,...,128},
-// compared per cell-update.
+// sources by repeating the next step(s) (each launch reads the current buffer and
+// writes the other, which the next real step overwrites), so the state and step count
+// are unchanged. One-step shapes: variant 0..15 x block_x {64,128,256} x rows
+// {16,32,64}; then the fused kernels: 2..5 steps per launch x block_x x rows
+// {32,...,256} x tile order (hardware, or XCD order with sources interleaved in groups
+// of 16), compared per cell-update of the running sources.
 template <typename T, bool KAHAN>
 int batch_autotune(swh_batch *b, double dt, int reps, int32_t *best, double *best_ms)
 {
@@ -1697,55 +1756,79 @@ int batch_autotune(swh_batch *b, double dt, int reps, int32_t *best, double *bes
     if (rc) return rc;
     const int n_active = batch_active(b, b->step);
     if (n_active == 0) return fail(SWH_ESTATE, "autotune: no source is still running");
-    const int n2 = batch_active(b, b->step + 1);
     const int c = b->cur;
     hipEvent_t e0, e1;
     SWH_HIP(hipEventCreate(&e0));
     SWH_HIP(hipEventCreate(&e1));
-    const int bxs[3] = {64, 128, 256}, rws[4] = {16, 32, 64, 128};
+    const int bxs[3] = {64, 128, 256};
     double bms = 1e300;   // ms per step of n_active sources
     int bb = b->block_x, br = b->rows_per_block, bv = b->variant | (b->src_fast << 3) | (b->two_step << 4);
-    auto timed = [&](bool two, double &ms_out) -> int {
-        int r = two ? batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c)
-                    : batch_launch<T, KAHAN>(b, a, fric_mode, n_active, c);   // warm-up
+    int bx_order = b->xcd_order, b_group = b->src_group, b_fuse = b->fuse;
+    // one launch of depth D (1..5) over the sources with D or more steps left
+    auto launch = [&](int D, int nD) -> int {
+        if (D == 1) return batch_launch<T, KAHAN>(b, a, fric_mode, nD, c);
+        if (D == 2) return batch_launch2<T, KAHAN>(b, a, fric_mode, 0, nD, c);
+        return batch_launchn<T, KAHAN>(b, a, fric_mode, 0, nD, c, D);
+    };
+    auto timed = [&](int D, double &ms_out) -> int {
+        const int nD = batch_active(b, b->step + D - 1);
+        if (nD <= 0) { ms_out = 1e300; return SWH_OK; }
+        int r = launch(D, nD);   // warm-up
         if (r) return r;
         if (hipEventRecord(e0, g->stream) != hipSuccess) return fail(SWH_EHIP, "event record");
-        for (int k = 0; k < reps; k++) {
-            r = two ? batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c)
-                    : batch_launch<T, KAHAN>(b, a, fric_mode, n_active, c);
-            if (r) return r;
-        }
+        for (int k = 0; k < reps; k++)
+            if ((r = launch(D, nD))) return r;
         float ms = 0.f;
         if (hipEventRecord(e1, g->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
             return fail(SWH_EHIP, "event timing");
         // per step of n_active sources
-        ms_out = two ? (double)ms / reps / 2.0 * ((double)n_active / n2) : (double)ms / reps;
+        ms_out = (double)ms / reps / D * ((double)n_active / nD);
         return SWH_OK;
     };
+    const int rws1[3] = {16, 32, 64};
     for (int v = 0; v < 16 && rc == SWH_OK; v++)
         for (int ib = 0; ib < 3 && rc == SWH_OK; ib++)
             for (int ir = 0; ir < 3 && rc == SWH_OK; ir++) {
-                b->block_x = bxs[ib]; b->rows_per_block = rws[ir]; b->variant = v & 7; b->src_fast = (v >> 3) & 1;
+                b->block_x = bxs[ib]; b->rows_per_block = rws1[ir]; b->variant = v & 7; b->src_fast = (v >> 3) & 1;
                 double ms;
-                rc = timed(false, ms);
-                if (!rc && ms < bms) { bms = ms; bb = bxs[ib]; br = rws[ir]; bv = v; }
+                rc = timed(1, ms);
+                if (!rc && ms < bms) { bms = ms; bb = bxs[ib]; br = rws1[ir]; bv = v; }
             }
-    if (!rc && fric_mode <= 1 && n2 > 0 && (long long)g->plane_elems() < (1LL << 31)) {
-        const int v1 = bv;
+    if (!rc && fric_mode <= 1 && (long long)g->plane_elems() < (1LL << 31)) {
+        // fused kernels with the next row's loads in flight
+        const int v1 = bv | 2;
         b->variant = v1 & 7; b->src_fast = (v1 >> 3) & 1;
-        for (int ib = 0; ib < 3 && rc == SWH_OK; ib++)
-            for (int ir = 0; ir < 4 && rc == SWH_OK; ir++) {
-                b->block_x = bxs[ib]; b->rows_per_block = rws[ir];
-                double ms;
-                rc = timed(true, ms);
-                if (!rc && ms < bms) { bms = ms; bb = bxs[ib]; br = rws[ir]; bv = v1 | 16; }
-            }
+        const int orders[2][2] = {{0, 1}, {1, 16}};
+        const int rws[4] = {32, 64, 128, 256};
+        for (int D = 2; D <= 5 && rc == SWH_OK; D++)
+            for (int io = 0; io < 2 && rc == SWH_OK; io++)
+                for (int ib = 0; ib < 3 && rc == SWH_OK; ib++)
+                    for (int ir = 0; ir < 4 && rc == SWH_OK; ir++) {
+                        b->block_x = bxs[ib]; b->rows_per_block = rws[ir];
+                        b->xcd_order = orders[io][0]; b->src_group = orders[io][1];
+                        double ms;
+                        rc = timed(D, ms);
+                        if (!rc && ms < bms) {
+                            bms = ms; bb = bxs[ib]; br = rws[ir]; bv = v1 | 16;
+                            bx_order = orders[io][0]; b_group = orders[io][1]; b_fuse = D;
+                        }
+                    }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->block_x = bb; b->rows_per_block = br; b->variant = bv & 7; b->src_fast = (bv >> 3) & 1;
     b->two_step = (bv >> 4) & 1;
+    b->xcd_order = bx_order; b->src_group = b_group;
+    b->fuse = b->two_step ? b_fuse : 2;
+    if (b->two_step) {
+        if (bx_order) {
+            int lg = 0;
+            while ((1 << lg) < b_group) lg++;
+            bv |= 64 | (lg << 7);
+        }
+        bv |= (b->fuse - 2) << 10;
+    }
     if (rc) return rc;
     if (best) { best[0] = bb; best[1] = br; best[2] = bv; }
     if (best_ms) *best_ms = bms;
@@ -1771,11 +1854,13 @@ int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **
         return rc;
     }
     b->nsrc = nsources;
-    // default launch: two steps per launch, 4-wave blocks (the autotuner's choice on
-    // the 512-source 1200^2 workload; swh_batch_autotune refines it per batch)
+    // default launch: five (FP32) / three (FP64) fused steps per launch, 2-wave blocks,
+    // XCD tile order with sources in groups of 16 (the fastest shapes measured on the
+    // 512-source 1200^2 workload; swh_batch_autotune refines it per batch)
     b->block_x = 256; b->two_step = 1; b->src_fast = 0;
-    if (b->g->prec == SWH_PREC_FP64) { b->rows_per_block = 64; b->variant = 1; }
-    else { b->rows_per_block = 16; b->variant = 3; }
+    b->xcd_order = 1; b->src_group = 16;
+    if (b->g->prec == SWH_PREC_FP64) { b->block_x = 128; b->rows_per_block = 128; b->variant = 3; b->fuse = 3; }
+    else { b->block_x = 128; b->rows_per_block = 256; b->variant = 3; b->fuse = 5; }
     const size_t bytes = b->g->plane_elems() * b->g->esz * (size_t)nsources;
     const bool kahan = b->g->prec == SWH_PREC_FP32K;
     for (int k = 0; k < 2 && rc == SWH_OK; k++) {
@@ -1949,8 +2034,8 @@ int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active)
 
 static int check_launch(int32_t block_x, int32_t rows_per_block, int32_t variant)
 {
-    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 63)
-        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..63");
+    if (block_x < 64 || block_x > 256 || (block_x % 64) || rows_per_block < 1 || variant < 0 || variant > 8191)
+        return fail(SWH_EINVAL, "block_x must be 64..256 (multiple of 64), rows_per_block >= 1, variant 0..8191");
     return SWH_OK;
 }
 
@@ -2018,6 +2103,9 @@ int swh_batch_set_launch(swh_batch *b, int32_t block_x, int32_t rows_per_block, 
     if (rc) return rc;
     b->block_x = block_x; b->rows_per_block = rows_per_block;
     b->variant = variant & 7; b->src_fast = (variant >> 3) & 1; b->two_step = (variant >> 4) & 1;
+    b->xcd_order = (variant >> 6) & 1;
+    b->src_group = 1 << ((variant >> 7) & 7);
+    b->fuse = 2 + ((variant >> 10) & 7);
     return SWH_OK;
 }
 

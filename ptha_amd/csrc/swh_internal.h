@@ -113,6 +113,11 @@ struct LFArgs {
     // 1: the inputs S/C already hold the new stage (formed by lf_stage_kernel and the
     // boundary); the step passes it through and writes only UH/VH
     int stage_given = 0;
+    // two-step kernel: 1 = XCD-aware block order (each XCD walks a contiguous range of
+    // tiles, columns fastest, then the sources of a group of lf2_group, then strips,
+    // then groups), 0 = the hardware's order
+    int lf2_order = 0;
+    int lf2_group = 1;
 };
 
 // Kernel launchers (defined in the .hip files)
@@ -124,7 +129,12 @@ hipError_t launch_leapfrog(const LFArgs<T> &a, int coriolis, int friction_mode, 
 // boundary stage, max_U or flux tracking; friction modes 0 and 1 only.
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                            int linear_friction, int nsrc, int block_x, hipStream_t s);
+                            int linear_friction, int nsrc, int block_x, hipStream_t s, int prefetch = 0);
+// ns (3 or 4) fused steps per launch (same operations; the state is read and written
+// once per ns steps).
+template <typename T, bool KAHAN>
+hipError_t launch_leapfrogn(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                            int linear_friction, int nsrc, int block_x, hipStream_t s, int prefetch, int ns);
 
 // UH/VH one step ahead at cells idx of a single grid (src_stride 0), written into
 // a.UHo / a.VHo there; friction modes 0 and 1, no boundary stage.

@@ -148,19 +148,49 @@ __device__ __forceinline__ T mom_v(const LFArgs<T> &a, bool valid, T cv, T fv, T
 
 }  // namespace
 
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+// The tile (column block, strip, source) of this workgroup. Workgroups are dispatched
+// to the 8 XCDs (each with its own L2) round-robin by linear index; in XCD order the
+// i-th workgroup on an XCD takes the i-th tile of that XCD's contiguous range of tiles,
+// ordered columns fastest, then the sources of a group of 'group' sources, then
+// strips, then groups. Neighbouring tiles of a strip, the sources of a group (which
+// read the same elevation and row coefficients) and the next strip (which reads the
+// halo rows of this one) thus run on the same L2 shortly after one another.
+__device__ __forceinline__ void lf2_tile(int order, int group, int &bx, int &by, int &bz)
+{
+    bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z;
+    if (!order) return;
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned B = gx * gy * gz;
+    const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned xcd = b % 8u, slot = b / 8u, n8 = B / 8u, rem = B % 8u;
+    const unsigned L = xcd < rem ? xcd * (n8 + 1u) + slot : rem * (n8 + 1u) + (xcd - rem) * n8 + slot;
+    const unsigned G = (unsigned)group;
+    const unsigned gi = L / (gx * gy * G);                 // source group
+    const unsigned within = L - gi * (gx * gy * G);
+    const unsigned Gi = min(G, gz - gi * G);               // sources in this group (the last may be short)
+    const unsigned t = within / gx;
+    bx = (int)(within - t * gx);
+    bz = (int)(gi * G + t % Gi);
+    by = (int)(t / Gi);
+}
+
+// PF: the loads of the next row are issued before the current row is computed, so
+// their latency overlaps the arithmetic (one row of loads in flight per lane).
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool PF>
 __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
 {
+    int bx, by, bz;
+    lf2_tile(a.lf2_order, a.lf2_group, bx, by, bz);
     const int lane = threadIdx.x & 63;
-    const int wg = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wg = bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wg * kOwn >= a.nx) return;                       // whole wavefront past the east edge
     const int col = wg * kOwn - 1 + lane;                // column of this lane (may be -1 or >= nx)
     const int ci = min(max(col, 0), a.nx - 1);           // column it loads (clamped into the row)
     const bool own = lane >= 1 && lane <= kOwn && col < a.nx;
-    const int j0 = blockIdx.y * a.rows_per_block;
+    const int j0 = by * a.rows_per_block;
     if (j0 >= a.ny) return;
     const int j1 = min(j0 + a.rows_per_block, a.ny);
-    const long long so = (long long)(blockIdx.z + a.src_base) * a.src_stride;
+    const long long so = (long long)(bz + a.src_base) * a.src_stride;
     const T *__restrict__ S = a.S + so;
     const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
     const T *__restrict__ UH = a.UH + so;
@@ -200,18 +230,32 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
         e_p_r = shl1(e_p);
     }
 
+    // the loads of one row (lane 0 also reads its own west neighbour's UH)
+    auto load_row = [&](int m, T &s0, T &c0, T &uh0, T &vh0, T &e, T &uhw) {
+        const int mr = min(max(m, 0), a.ny - 1);
+        const int o = mr * a.pitch + ci;
+        s0 = S[o];
+        c0 = KAHAN ? C[o] : T(0);
+        uh0 = UH[o];
+        vh0 = VH[o];
+        e = a.E[o];
+        uhw = (lane == 0 && col > 0) ? UH[o - 1] : uh0;
+    };
+    T n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw;
+    if (PF) load_row(j0 - 1, n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw);
+
     // offsets within one source's plane fit in 32 bits (uniform base + per-lane offset)
     for (int m = j0 - 1; m <= j1 + 1; m++) {
         const int mr = min(max(m, 0), a.ny - 1);
-        const int o = mr * a.pitch + ci;
-        const T s0 = S[o];
-        const T c0 = KAHAN ? C[o] : T(0);
-        const T uh0 = UH[o];
-        const T vh0 = VH[o];
-        const T e = a.E[o];
+        T s0, c0, uh0, vh0, e, uh0_mem;
+        if (PF) {
+            s0 = n_s0; c0 = n_c0; uh0 = n_uh0; vh0 = n_vh0; e = n_e; uh0_mem = n_uhw;
+            if (m + 1 <= j1 + 1) load_row(m + 1, n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw);
+        } else {
+            load_row(m, s0, c0, uh0, vh0, e, uh0_mem);
+        }
         const RowCo<T> q = row_co(a, m);
         // UH(i-1): from the lane to the west; lane 0 has none and reads it itself
-        const T uh0_mem = (lane == 0 && col > 0) ? UH[o - 1] : uh0;
         const T uh0_sh = shr1(uh0);
         const T uh0_l = west ? uh0 : (lane == 0 ? uh0_mem : uh0_sh);
         const T vh0_r = shl1(vh0), e_r = shl1(e);
@@ -271,6 +315,176 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
         e_pp = e_p; e_p = e; e_p_r = e_r;
         q_pp = q_p; q_p = q;
         fw_u_p = fw_u1; fw_v_p = fw_v1;
+    }
+}
+
+// NS fused steps per launch (NS = 3: the state is read and written once per three
+// steps). The same per-cell operations as the one-step kernel, scheduled as in
+// leapfrog2_kernel but for any number of levels: at row m, level n (the state after n
+// steps) forms its stage and UH at row m-n+1 and its VH at row m-n from level n-1,
+// whose stage and UH of row m-n+1 were formed in the previous iteration (level 0: the
+// row loaded now) and whose VH of rows m-n+1 (this iteration) and m-n (the previous
+// one) are at hand; row m-NS of level NS is written. Each level consumes one lane
+// of the wavefront to the west (UH(i-1)) and one to the east (stage(i+1), VH(i+1)), so a
+// wavefront owns 65 - 2 NS of its 64 columns; each strip recomputes NS rows below and
+// NS - 1 above it.
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool PF, int NS>
+__global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
+{
+    constexpr int OWN = 65 - 2 * NS;
+    int bx, by, bz;
+    lf2_tile(a.lf2_order, a.lf2_group, bx, by, bz);
+    const int lane = threadIdx.x & 63;
+    const int wg = bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wg * OWN >= a.nx) return;                        // whole wavefront past the east edge
+    const int col = wg * OWN - (NS - 1) + lane;          // column of this lane (may be < 0 or >= nx)
+    const int ci = min(max(col, 0), a.nx - 1);           // column it loads (clamped into the row)
+    const bool own = lane >= NS - 1 && lane < NS - 1 + OWN && col < a.nx;
+    const int j0 = by * a.rows_per_block;
+    if (j0 >= a.ny) return;
+    const int j1 = min(j0 + a.rows_per_block, a.ny);
+    const long long so = (long long)(bz + a.src_base) * a.src_stride;
+    const T *__restrict__ S = a.S + so;
+    const T *__restrict__ C = KAHAN ? a.C + so : nullptr;
+    const T *__restrict__ UH = a.UH + so;
+    const T *__restrict__ VH = a.VH + so;
+    T *__restrict__ So = a.So + so;
+    T *__restrict__ Co = KAHAN ? a.Co + so : nullptr;
+    T *__restrict__ UHo = a.UHo + so;
+    T *__restrict__ VHo = a.VHo + so;
+    const bool west = col == 0;                          // UH(i-1) clamps to UH(0) (include.f90:319-322)
+    const T dle_i = a.rc.dle[min(max(col, 0), a.nx)];
+    const T dle_ip1 = a.rc.dle[min(max(col + 1, 0), a.nx)];
+    const bool col_int = col > 0 && col < a.nx - 1;      // stage: interior column
+    const bool col_mom = col >= 0 && col < a.nx - 1;     // momentum: UH/VH of this column update
+    const int cf = min(max(col, 0), a.nx - 1);           // friction-work column
+
+    // carried between rows, per level n (0 = the loaded state)
+    T s_p[NS + 1], c_p[NS + 1];          // stage of level n at its stage row of the previous iteration
+    T uh_p[NS + 1], uh_p_l[NS + 1];      // UH of level n, previous iteration (level 0: row m-1), and its west value
+    T uh_pp[NS + 1], uh_pp_l[NS + 1];    // the iteration before that (levels >= 1)
+    T vh_p[NS + 1], vh_p_r[NS + 1];      // VH of level n, previous iteration, and its east value
+    T e_r[NS + 1], e_rr[NS + 1];         // elevation of row m-k and its east value
+    T fwu_r[NS], fwv_r[NS];              // friction work: UH at row m-k, VH at row m-1-k
+    RowCo<T> q_r[NS + 1];                // row coefficients of row m-k
+#pragma unroll
+    for (int n = 0; n <= NS; n++) {
+        s_p[n] = T(0); c_p[n] = T(0); uh_p[n] = T(0); uh_p_l[n] = T(0); uh_pp[n] = T(0); uh_pp_l[n] = T(0);
+        vh_p[n] = T(0); vh_p_r[n] = T(0); e_r[n] = T(0); e_rr[n] = T(0);
+    }
+#pragma unroll
+    for (int n = 0; n < NS; n++) { fwu_r[n] = T(0); fwv_r[n] = T(0); }
+    const int m0 = j0 - NS + 1;
+    // the rings as they stand after row m0-1 (each iteration first slides them by one)
+#pragma unroll
+    for (int k = 0; k <= NS; k++) q_r[k] = row_co(a, m0 - 1 - k);
+    {
+        // the row below the first one: pre-step VH/UH/E of row m0-1
+        const int r = min(max(m0 - 1, 0), a.ny - 1);
+        const int o = r * a.pitch + ci;
+        vh_p[0] = VH[o];
+        uh_p[0] = UH[o];
+        e_r[0] = a.E[o];
+        vh_p_r[0] = shl1(vh_p[0]);
+        const T uh_sh = shr1(uh_p[0]);
+        uh_p_l[0] = west ? uh_p[0] : uh_sh;
+        e_rr[0] = shl1(e_r[0]);
+    }
+    auto load_row = [&](int m, T &s0, T &c0, T &uh0, T &vh0, T &e, T &uhw) {
+        const int mr = min(max(m, 0), a.ny - 1);
+        const int o = mr * a.pitch + ci;
+        s0 = S[o];
+        c0 = KAHAN ? C[o] : T(0);
+        uh0 = UH[o];
+        vh0 = VH[o];
+        e = a.E[o];
+        uhw = (lane == 0 && col > 0) ? UH[o - 1] : uh0;
+    };
+    T n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw;
+    if (PF) load_row(m0, n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw);
+
+    for (int m = m0; m <= j1 + NS - 1; m++) {
+        T s0, c0, uh0, vh0, e0, uh0_mem;
+        if (PF) {
+            s0 = n_s0; c0 = n_c0; uh0 = n_uh0; vh0 = n_vh0; e0 = n_e; uh0_mem = n_uhw;
+            if (m + 1 <= j1 + NS - 1) load_row(m + 1, n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw);
+        } else {
+            load_row(m, s0, c0, uh0, vh0, e0, uh0_mem);
+        }
+        // slide the rings of rows: index k now holds row m-k
+#pragma unroll
+        for (int k = NS; k >= 1; k--) { e_r[k] = e_r[k - 1]; e_rr[k] = e_rr[k - 1]; q_r[k] = q_r[k - 1]; }
+        e_r[0] = e0;
+        e_rr[0] = shl1(e0);
+        q_r[0] = row_co(a, m);
+        if (FRIC) {
+#pragma unroll
+            for (int k = NS - 1; k >= 1; k--) { fwu_r[k] = fwu_r[k - 1]; fwv_r[k] = fwv_r[k - 1]; }
+            const int mr = min(max(m, 0), a.ny - 1);
+            fwu_r[0] = a.fwU[mr * a.pitch + cf];
+            fwv_r[0] = a.fwV[min(max(m - 1, 0), a.ny - 1) * a.pitch + cf];
+        }
+        // level 0 at row m (the loaded row); UH(i-1) from the west lane, lane 0 from memory
+        const T uh0_sh = shr1(uh0);
+        const T uh0_l = west ? uh0 : (lane == 0 ? uh0_mem : uh0_sh);
+        // inputs of the level being formed: stage/UH of the level below at this level's
+        // stage row, its VH there (cur) and one row below (below)
+        T in_s = s0, in_c = c0, in_uh = uh0, in_uh_l = uh0_l;
+        T in_uhb = uh_p[0], in_uhb_l = uh_p_l[0];     // UH of the level below one row lower
+        T vh_cur = vh0, vh_cur_r = shl1(vh0);
+        T out_s = T(0), out_c = T(0), out_uh = T(0), out_vh = T(0);
+#pragma unroll
+        for (int n = 1; n <= NS; n++) {
+            const int r = m - n + 1;                  // this level's stage/UH row
+            const RowCo<T> &q = q_r[n - 1];           // row r
+            const RowCo<T> &qb = q_r[n];              // row r-1
+            // VH of the level below at row r-1, clamped to row r at the southern edge
+            const T vhb = r > 0 ? vh_p[n - 1] : vh_cur, vhb_r = r > 0 ? vh_p_r[n - 1] : vh_cur_r;
+            T sn, cn;
+            stage_cell<T, KAHAN>(col_int && q.interior, q.sa, dle_i, q.dbe_lo, q.dbe_hi, in_s, in_c, in_uh, in_uh_l,
+                                 vh_cur, vhb, sn, cn);
+            const T sn_r = shl1(sn);
+            const T cn_r = KAHAN ? shl1(cn) : T(0);
+            const T uhn = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q.valid, q.cu, q.fu, dle_ip1,
+                                                                FRIC ? fwu_r[n - 1] : T(0), in_uh, sn, cn, sn_r, cn_r,
+                                                                e_r[n - 1], e_rr[n - 1], vhb, vhb_r, vh_cur, vh_cur_r);
+            // VH at row r-1: stage of this level at rows r-1 (previous iteration) and r,
+            // UH of the level below at rows r-1 and r
+            const T vhn = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && qb.valid, qb.cv, qb.fv, qb.dbe_hi,
+                                                                FRIC ? fwv_r[n - 1] : T(0), vh_p[n - 1], s_p[n],
+                                                                c_p[n], sn, cn, e_r[n], e_r[n - 1], in_uhb, in_uhb_l,
+                                                                in_uh, in_uh_l);
+            const T uhn_sh = shr1(uhn);
+            const T uhn_l = west ? uhn : uhn_sh;
+            const T vhn_r = shl1(vhn);
+            // the level below's VH of this iteration becomes its 'previous' one
+            if (n == 1) {
+                // level 0: the loaded row's UH/VH become row m-1's
+                uh_p[0] = uh0; uh_p_l[0] = uh0_l;
+            }
+            vh_p[n - 1] = vh_cur; vh_p_r[n - 1] = vh_cur_r;
+            if (n == NS) {
+                out_s = s_p[n]; out_c = c_p[n]; out_uh = uh_p[n]; out_vh = vhn;
+            }
+            // inputs of level n+1: this level's stage/UH of the previous iteration (row
+            // m-n), this level's VH now (row m-n)
+            in_s = s_p[n]; in_c = c_p[n]; in_uh = uh_p[n]; in_uh_l = uh_p_l[n];
+            in_uhb = uh_pp[n]; in_uhb_l = uh_pp_l[n];
+            vh_cur = vhn; vh_cur_r = vhn_r;
+            // slide this level
+            s_p[n] = sn; c_p[n] = cn;
+            uh_pp[n] = uh_p[n]; uh_pp_l[n] = uh_p_l[n];
+            uh_p[n] = uhn; uh_p_l[n] = uhn_l;
+        }
+        vh_p[NS] = vh_cur; vh_p_r[NS] = vh_cur_r;
+        const int k = m - NS;
+        if (own && k >= j0 && k < j1) {
+            const int ow = k * a.pitch + col;
+            So[ow] = out_s;
+            if (KAHAN) Co[ow] = out_c;
+            UHo[ow] = out_uh;
+            VHo[ow] = out_vh;
+        }
     }
 }
 
@@ -430,40 +644,99 @@ template hipError_t launch_gauge_step1<float, true>(const LFArgs<float> &, int, 
                                                     const long long *, double *, int, int, hipStream_t);
 
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-static hipError_t launch_lf2(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)
+static hipError_t launch_lf2(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s, int pf)
 {
     const int waves = (a.nx + kOwn - 1) / kOwn, wpb = block_x / 64;
     dim3 grid((waves + wpb - 1) / wpb, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
-    hipLaunchKernelGGL((leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF>), grid, dim3(block_x), 0, s, a);
+    if (pf)
+        hipLaunchKernelGGL((leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF, true>), grid, dim3(block_x), 0, s, a);
+    else
+        hipLaunchKernelGGL((leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF, false>), grid, dim3(block_x), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename T, bool KAHAN, bool COR, int FRIC>
-static hipError_t lf2_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int bx, hipStream_t s)
+static hipError_t lf2_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int bx, hipStream_t s, int pf)
 {
-    if (tl) return linf ? launch_lf2<T, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s)
-                        : launch_lf2<T, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s);
-    return linf ? launch_lf2<T, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s)
-                : launch_lf2<T, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s);
+    if (tl) return linf ? launch_lf2<T, KAHAN, COR, FRIC, true, true>(a, nsrc, bx, s, pf)
+                        : launch_lf2<T, KAHAN, COR, FRIC, true, false>(a, nsrc, bx, s, pf);
+    return linf ? launch_lf2<T, KAHAN, COR, FRIC, false, true>(a, nsrc, bx, s, pf)
+                : launch_lf2<T, KAHAN, COR, FRIC, false, false>(a, nsrc, bx, s, pf);
 }
 
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                            int linear_friction, int nsrc, int block_x, hipStream_t s)
+                            int linear_friction, int nsrc, int block_x, hipStream_t s, int pf)
 {
     if (friction_mode > 1 || a.bc_mask || a.maxq_mask || (block_x % 64) || block_x < 64 || block_x > 256 ||
         (long long)a.pitch * a.ny >= (1LL << 31))
         return hipErrorInvalidValue;
     const int linf = (friction_mode == 0) && linear_friction;
     if (coriolis) {
-        return friction_mode ? lf2_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s)
-                             : lf2_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s);
+        return friction_mode ? lf2_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, block_x, s, pf)
+                             : lf2_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, block_x, s, pf);
     }
-    return friction_mode ? lf2_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s)
-                         : lf2_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s);
+    return friction_mode ? lf2_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, block_x, s, pf)
+                         : lf2_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, block_x, s, pf);
 }
 
-template hipError_t launch_leapfrog2<double, false>(const LFArgs<double> &, int, int, int, int, int, int, hipStream_t);
-template hipError_t launch_leapfrog2<float, true>(const LFArgs<float> &, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_leapfrog2<double, false>(const LFArgs<double> &, int, int, int, int, int, int, hipStream_t,
+                                                    int);
+
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, int NS>
+static hipError_t launch_lfn(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s, int pf)
+{
+    constexpr int OWN = 65 - 2 * NS;
+    const int waves = (a.nx + OWN - 1) / OWN, wpb = block_x / 64;
+    dim3 grid((waves + wpb - 1) / wpb, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
+    if (pf)
+        hipLaunchKernelGGL((leapfrogn_kernel<T, KAHAN, COR, FRIC, TL, LINF, true, NS>), grid, dim3(block_x), 0, s, a);
+    else
+        hipLaunchKernelGGL((leapfrogn_kernel<T, KAHAN, COR, FRIC, TL, LINF, false, NS>), grid, dim3(block_x), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, bool KAHAN, bool COR, int FRIC, int NS>
+static hipError_t lfn_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int bx, hipStream_t s, int pf)
+{
+    if (tl) return linf ? launch_lfn<T, KAHAN, COR, FRIC, true, true, NS>(a, nsrc, bx, s, pf)
+                        : launch_lfn<T, KAHAN, COR, FRIC, true, false, NS>(a, nsrc, bx, s, pf);
+    return linf ? launch_lfn<T, KAHAN, COR, FRIC, false, true, NS>(a, nsrc, bx, s, pf)
+                : launch_lfn<T, KAHAN, COR, FRIC, false, false, NS>(a, nsrc, bx, s, pf);
+}
+
+template <typename T, bool KAHAN, int NS>
+static hipError_t lfn_cf(const LFArgs<T> &a, int coriolis, int friction_mode, int tl, int linf, int nsrc, int bx,
+                         hipStream_t s, int pf)
+{
+    if (coriolis)
+        return friction_mode ? lfn_tl<T, KAHAN, true, 1, NS>(a, tl, linf, nsrc, bx, s, pf)
+                             : lfn_tl<T, KAHAN, true, 0, NS>(a, tl, linf, nsrc, bx, s, pf);
+    return friction_mode ? lfn_tl<T, KAHAN, false, 1, NS>(a, tl, linf, nsrc, bx, s, pf)
+                         : lfn_tl<T, KAHAN, false, 0, NS>(a, tl, linf, nsrc, bx, s, pf);
+}
+
+template <typename T, bool KAHAN>
+hipError_t launch_leapfrogn(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
+                            int linear_friction, int nsrc, int block_x, hipStream_t s, int pf, int ns)
+{
+    if (friction_mode > 1 || a.bc_mask || a.maxq_mask || (block_x % 64) || block_x < 64 || block_x > 256 ||
+        (long long)a.pitch * a.ny >= (1LL << 31) || ns < 3 || ns > 6)
+        return hipErrorInvalidValue;
+    const int linf = (friction_mode == 0) && linear_friction;
+    switch (ns) {
+    case 3: return lfn_cf<T, KAHAN, 3>(a, coriolis, friction_mode, truely_linear, linf, nsrc, block_x, s, pf);
+    case 4: return lfn_cf<T, KAHAN, 4>(a, coriolis, friction_mode, truely_linear, linf, nsrc, block_x, s, pf);
+    case 5: return lfn_cf<T, KAHAN, 5>(a, coriolis, friction_mode, truely_linear, linf, nsrc, block_x, s, pf);
+    default: return lfn_cf<T, KAHAN, 6>(a, coriolis, friction_mode, truely_linear, linf, nsrc, block_x, s, pf);
+    }
+}
+
+template hipError_t launch_leapfrogn<double, false>(const LFArgs<double> &, int, int, int, int, int, int, hipStream_t,
+                                                    int, int);
+template hipError_t launch_leapfrog2<float, true>(const LFArgs<float> &, int, int, int, int, int, int, hipStream_t,
+                                                  int);
+template hipError_t launch_leapfrogn<float, true>(const LFArgs<float> &, int, int, int, int, int, int, hipStream_t,
+                                                  int, int);
 
 }  // namespace swh

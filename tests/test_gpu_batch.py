@@ -22,9 +22,17 @@ def _cfg():
 
 
 @pytest.mark.parametrize("precision,two_step", [("fp64", False), ("fp32k", False), ("fp64", True),
-                                                ("fp32k", True)])
+                                                ("fp32k", True), ("fp64", "xcd"), ("fp32k", "xcd"),
+                                                ("fp64", "three"), ("fp32k", "three"), ("fp64", "four"),
+                                                ("fp32k", "four"), ("fp64", "five"), ("fp32k", "five")])
 def test_batch_matches_single_source_runs(oracle_built, precision, two_step):
-    print("CASE batch", precision, "two-step" if two_step else "one-step", flush=True)
+    """two_step: False = one step per launch; True = two fused steps per launch (next-row
+    loads in flight); 'xcd' = the same with the XCD-aware tile order; 'three' = three
+    fused steps per launch where no gauge sample falls inside them (else two or one);
+    'four', 'five' = up to four or five."""
+    print("CASE batch", precision, {False: "one-step", True: "two-step", "xcd": "two-step xcd",
+                                     "three": "three-step", "four": "four-step",
+                                     "five": "five-step"}[two_step], flush=True)
     from ptha_amd.batch import UnitSourceBatch
     u = _cfg()
     nx, ny = u["nx"], u["ny"]
@@ -32,7 +40,16 @@ def test_batch_matches_single_source_runs(oracle_built, precision, two_step):
     nsteps = np.array([51, 80, 33], dtype=np.int32) if two_step else np.array([50, 80, 30], dtype=np.int32)
     every = 10
     b = UnitSourceBatch(nx, ny, u["lw"], u["ll"], 3, precision=precision)
-    if two_step:
+    if two_step == "three":
+        b.set_launch(128, 8, 16 | 64 | 2 | (1 << 7) | 1024)
+    elif two_step == "four":
+        b.set_launch(64, 16, 16 | 64 | 2 | (2 << 7) | 2048)
+    elif two_step == "five":
+        b.set_launch(128, 32, 16 | 64 | 2 | (4 << 7) | (3 << 10))
+    elif two_step == "xcd":
+        # XCD order with sources interleaved in groups of 2 (3 sources: a short last group)
+        b.set_launch(128, 8, 16 | 64 | 2 | (1 << 7))
+    elif two_step:
         b.set_launch(64, 16, 16 | (3 if precision == "fp32k" else 1))
     else:
         b.set_launch(128, 16, 3 if precision == "fp32k" else 1)
@@ -79,8 +96,8 @@ def test_batch_matches_single_source_runs(oracle_built, precision, two_step):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32k"])
 def test_two_step_friction_and_autotune_bitwise(precision):
-    """'leapfrog_linear_plus_nonlinear_friction' batch (Manning n = 0.03): two steps
-    per launch equal one step per launch bit for bit; running the autotuner first
+    """'leapfrog_linear_plus_nonlinear_friction' batch (Manning n = 0.03): two to five
+    steps per launch equal one step per launch bit for bit; running the autotuner first
     changes nothing in the results."""
     print("CASE batch llpnf two-step/autotune", precision, flush=True)
     from ptha_amd.batch import UnitSourceBatch
@@ -88,7 +105,7 @@ def test_two_step_friction_and_autotune_bitwise(precision):
     nx, ny = u["nx"], u["ny"]
     nsteps = np.array([41, 64, 27], dtype=np.int32)
     runs = []
-    for mode in ["one", "two", "tuned"]:
+    for mode in ["one", "two", "three", "four", "five", "tuned"]:
         b = UnitSourceBatch(nx, ny, u["lw"], u["ll"], 3, precision=precision,
                             timestepping_method="leapfrog_linear_plus_nonlinear_friction")
         b.set_manning_squared(np.full((ny, nx), 0.03 ** 2))
@@ -103,7 +120,13 @@ def test_two_step_friction_and_autotune_bitwise(precision):
         if mode == "one":
             b.set_launch(128, 16, 1)
         elif mode == "two":
-            b.set_launch(128, 32, 17)
+            b.set_launch(128, 32, 16 | 64 | 2)
+        elif mode == "three":
+            b.set_launch(64, 16, 16 | 64 | 2 | 1024)
+        elif mode == "four":
+            b.set_launch(128, 8, 16 | 64 | 2 | 2048)
+        elif mode == "five":
+            b.set_launch(64, 7, 16 | 64 | 2 | (3 << 10))
         else:
             shape, ms = b.autotune(dt, reps=1)
             assert ms > 0 and len(shape) == 3
