@@ -254,6 +254,18 @@ int swh_snapshot_download(swh_domain *h, double *U, double *time);
  * every 'every' steps of nsteps (nsteps/every + 1 samples in all). */
 int swh_evolve_record_gauges(swh_domain *h, double timestep, int32_t nsteps, int32_t every);
 
+/* Forward run with gauges on a schedule: samples the gauges at the listed steps
+ * (strictly increasing; 0 = now), stepping between them with swh_evolve_steps, all in
+ * stream order with no host synchronisation (the write-time rule of the reference's
+ * driver gives the steps, see swh_batch_gauges_set_schedule). */
+int swh_evolve_record_gauges_schedule(swh_domain *h, double timestep, int32_t nsamples, const int64_t *sample_steps);
+
+/* Device checkpoints of the leapfrog state (no counterpart in the reference): a copy of
+ * the current stage, UH, VH (and the FP32 Kahan term) and the time in device memory,
+ * by slot (allocated on first use, freed with the domain); restore puts it back. */
+int swh_checkpoint_save(swh_domain *h, int32_t slot);
+int swh_checkpoint_restore(swh_domain *h, int32_t slot);
+
 /* ---- Adjoint of the 'linear' (truely-linear) leapfrog step ----
  * No counterpart in the reference (which has no adjoint); it serves source inversion
  * over unit sources. The adjoint state (lS, lU, lV) lives on the device; a step
@@ -268,6 +280,11 @@ int swh_adjoint_download(swh_domain *h, double *lambda3);
 int swh_adjoint_step(swh_domain *h, double dt, int32_t nsteps);
 int swh_adjoint_inject(swh_domain *h, const double *values);   /* lS[gauge g] += values[g] */
 int swh_adjoint_run(swh_domain *h, double dt, int32_t nsteps, int32_t every, const double *grad);
+/* The backward sweep of swh_evolve_record_gauges_schedule(h, dt, nsamples, sample_steps)
+ * run to nsteps: grad[(q * ngauges) + g] = d(loss)/d(stage at gauge g in sample q). The
+ * gradients are uploaded once and injected on the device. */
+int swh_adjoint_run_schedule(swh_domain *h, double dt, int32_t nsteps, int32_t nsamples, const int64_t *sample_steps,
+                             const double *grad);
 
 /* Launch shape of the leapfrog step kernel (see swh_batch_set_launch). */
 int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t variant);

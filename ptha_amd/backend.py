@@ -119,6 +119,10 @@ SYMBOLS = [
     ("swh_batch_superpose", ctypes.c_int, [ctypes.c_void_p, _I32, _PI32, _PI32, _PD, _PD, _D, _I32, _PD, _PI32,
                                             _PD]),
     ("swh_evolve_record_gauges", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _I32]),
+    ("swh_evolve_record_gauges_schedule", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PI64]),
+    ("swh_checkpoint_save", ctypes.c_int, [ctypes.c_void_p, _I32]),
+    ("swh_checkpoint_restore", ctypes.c_int, [ctypes.c_void_p, _I32]),
+    ("swh_adjoint_run_schedule", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _I32, _PI64, _PD]),
     ("swh_adjoint_reset", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_adjoint_upload", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_adjoint_download", ctypes.c_int, [ctypes.c_void_p, _PD]),

@@ -858,8 +858,9 @@ int swh_domain_destroy(swh_domain *h)
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
                     h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt,
-                    h->d_stats};
+                    h->d_stats, h->d_ggrad};
     for (void *p : ptrs) dfree(p);
+    for (void *p : h->ckpt) dfree(p);
     dfree(h->snap);
     if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
     if (h->cstream) (void)hipStreamDestroy(h->cstream);
@@ -1455,6 +1456,112 @@ int swh_adjoint_run(swh_domain *h, double dt, int32_t nsteps, int32_t every, con
     for (int n = nsteps; n >= 0; n--) {
         if (n % every == 0) {
             if ((rc = adj_inject_host(h, grad + (size_t)(n / every) * ng))) return rc;
+        }
+        if (n > 0) {
+            rc = h->prec == SWH_PREC_FP64 ? adj_steps<double>(h, dt, 1) : adj_steps<float>(h, dt, 1);
+            if (rc) return rc;
+        }
+    }
+    return SWH_OK;
+}
+
+int swh_evolve_record_gauges_schedule(swh_domain *h, double timestep, int32_t nsamples, const int64_t *sample_steps)
+{
+    if (!h || nsamples < 1 || !sample_steps || !(timestep > 0.0)) return fail(SWH_EINVAL, "bad arguments");
+    if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    for (int q = 0; q < nsamples; q++)
+        if (sample_steps[q] < 0 || (q > 0 && sample_steps[q] <= sample_steps[q - 1]) ||
+            sample_steps[q] > (int64_t)INT32_MAX)
+            return fail(SWH_EINVAL, "sample steps must be >= 0 and strictly increasing");
+    if (h->gsamples + nsamples > h->gmax) return fail(SWH_ESTATE, "gauge sample buffer too small");
+    long long n = 0;
+    int rc = SWH_OK;
+    for (int q = 0; q < nsamples && !rc; q++) {
+        const long long k = sample_steps[q] - n;
+        if (k > 0) rc = swh_evolve_steps(h, timestep, (int32_t)k);
+        n = sample_steps[q];
+        if (!rc) rc = swh_gauges_sample(h);
+    }
+    return rc;
+}
+
+int swh_checkpoint_save(swh_domain *h, int32_t slot)
+{
+    if (!h || slot < 0 || slot > 100000) return fail(SWH_EINVAL, "bad checkpoint slot");
+    if (!h->leapfrog()) return fail(SWH_ESTATE, "checkpoints are provided for the leapfrog methods");
+    const size_t P = h->plane_elems() * h->esz;
+    if ((int)h->ckpt.size() <= slot) {
+        h->ckpt.resize(slot + 1, nullptr);
+        h->ckpt_time.resize(slot + 1, 0.0);
+    }
+    if (!h->ckpt[slot]) {
+        int rc = dmalloc(&h->ckpt[slot], 4 * P);
+        if (rc) return rc;
+    }
+    const int c = h->cur;
+    char *dst = (char *)h->ckpt[slot];
+    SWH_HIP(hipMemcpyAsync(dst, h->S[c], P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync(dst + P, h->UH[c], P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync(dst + 2 * P, h->VH[c], P, hipMemcpyDeviceToDevice, h->stream));
+    if (h->C[c]) SWH_HIP(hipMemcpyAsync(dst + 3 * P, h->C[c], P, hipMemcpyDeviceToDevice, h->stream));
+    h->ckpt_time[slot] = h->time;
+    return SWH_OK;
+}
+
+int swh_checkpoint_restore(swh_domain *h, int32_t slot)
+{
+    if (!h || slot < 0 || slot >= (int)h->ckpt.size() || !h->ckpt[slot])
+        return fail(SWH_EINVAL, "no checkpoint in that slot");
+    const size_t P = h->plane_elems() * h->esz;
+    const int c = h->cur;
+    const char *src = (const char *)h->ckpt[slot];
+    SWH_HIP(hipMemcpyAsync(h->S[c], src, P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync(h->UH[c], src + P, P, hipMemcpyDeviceToDevice, h->stream));
+    SWH_HIP(hipMemcpyAsync(h->VH[c], src + 2 * P, P, hipMemcpyDeviceToDevice, h->stream));
+    if (h->C[c]) SWH_HIP(hipMemcpyAsync(h->C[c], src + 3 * P, P, hipMemcpyDeviceToDevice, h->stream));
+    h->time = h->ckpt_time[slot];
+    return SWH_OK;
+}
+
+int swh_adjoint_run_schedule(swh_domain *h, double dt, int32_t nsteps, int32_t nsamples, const int64_t *sample_steps,
+                             const double *grad)
+{
+    if (!h || !grad || !sample_steps || !(dt > 0.0) || nsteps < 0 || nsamples < 0)
+        return fail(SWH_EINVAL, "bad adjoint run arguments");
+    if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    for (int q = 0; q < nsamples; q++)
+        if (sample_steps[q] < 0 || sample_steps[q] > nsteps || (q > 0 && sample_steps[q] <= sample_steps[q - 1]))
+            return fail(SWH_EINVAL, "sample steps must be in 0..nsteps and strictly increasing");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    const int ng = h->ng;
+    // every sample's gradient on the device at once (no host round trip in the sweep)
+    const size_t need = (size_t)(nsamples > 0 ? nsamples : 1) * (ng > 0 ? ng : 1);
+    if (h->n_ggrad < need) {
+        SWH_HIP(hipStreamSynchronize(h->stream));
+        dfree(h->d_ggrad);
+        h->d_ggrad = nullptr;
+        h->n_ggrad = 0;
+        if ((rc = dmalloc((void **)&h->d_ggrad, sizeof(double) * need))) return rc;
+        h->n_ggrad = need;
+    }
+    if (nsamples > 0 && ng > 0) {
+        SWH_HIP(hipMemcpyAsync(h->d_ggrad, grad, sizeof(double) * nsamples * ng, hipMemcpyHostToDevice, h->stream));
+        // grad must stay valid until the copy has been consumed
+        SWH_HIP(hipStreamSynchronize(h->stream));
+    }
+    // backward sweep: the sample taken after forward step n is injected before the
+    // transposed step n is applied
+    int q = nsamples - 1;
+    for (int n = nsteps; n >= 0; n--) {
+        if (q >= 0 && sample_steps[q] == n) {
+            void *lS = h->adj[h->adj_cur][0];
+            const double *v = h->d_ggrad + (size_t)q * ng;
+            if (h->prec == SWH_PREC_FP64)
+                SWH_HIP(launch_adjoint_inject<double>((double *)lS, ng, h->d_gidx, v, h->stream));
+            else
+                SWH_HIP(launch_adjoint_inject<float>((float *)lS, ng, h->d_gidx, v, h->stream));
+            q--;
         }
         if (n > 0) {
             rc = h->prec == SWH_PREC_FP64 ? adj_steps<double>(h, dt, 1) : adj_steps<float>(h, dt, 1);

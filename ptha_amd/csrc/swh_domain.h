@@ -72,6 +72,11 @@ struct swh_domain {
     void *adj[2][3] = {};
     int adj_cur = 0;
     double *d_gvec = nullptr;  // one sample of gauge values for injection
+    double *d_ggrad = nullptr;     // a whole backward sweep's gauge gradients (swh_adjoint_run_schedule)
+    size_t n_ggrad = 0;
+    // device checkpoints of the state (S, C, UH, VH planes of the current buffer), by slot
+    std::vector<void *> ckpt;
+    std::vector<double> ckpt_time;
     // multidomain nesting: box-edge strips whose flux integrals each step accumulates
     const swh::NestStrip *d_nest_strips = nullptr;
     int n_nest_strips = 0;
