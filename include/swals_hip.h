@@ -51,7 +51,8 @@ extern "C" {
 
 /* Arithmetic of the bulk update */
 #define SWH_PREC_FP64 0    /* FP64 throughout (the reference's default dp = C_DOUBLE) */
-#define SWH_PREC_FP32K 1   /* FP32 state, Kahan-compensated stage, FP64 metrics/reductions */
+#define SWH_PREC_FP32K 1   /* FP32 state, Kahan-compensated stage (every method), FP64 metrics/reductions;
+                              finite-volume kernels form the clipped cells and fluxes in FP64 */
 
 /* max_U tracking mask bits; names and order as in the reference's
  * gridded_output_variables_max_U (util/netcdf_util_mod.f90:39-41) */

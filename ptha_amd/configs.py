@@ -243,10 +243,12 @@ def nested_c3(scale=1, seed=20261202):
     100-200E, lat 60S-0 at 2 arcmin (3000 x 1800), a
     'leapfrog_linear_plus_nonlinear_friction' child at 1/3 of its cell size (600 x 600,
     Manning n = 0.03) and an 'rk2' grandchild at 1/5 of the child's (1000 x 1000, n =
-    0.03), both on the eastern shelf; power-law bathymetry (slope -2, mean -4000 m,
-    std 1000 m) with a shelf ramp over lon 191-197 to +50 m land (the shore lies
-    inside the grandchild); an elliptical Gaussian uplift of 3 m peak, sigma 200 km
-    north-south by 50 km east-west, at lon 190, lat 26.5S; 6 h simulated.
+    0.03), on the eastern shelf; power-law bathymetry (slope -2, mean -4000 m, std
+    1000 m, no shallower than 200 m) with a shelf ramp over lon 191-197 to +50 m land
+    (the shore lies inside the child); the grandchild (lon 193.0-195.2) lies wholly in
+    water, at least 24 m deep, so its nesting boundary never crosses the coast; an
+    elliptical Gaussian uplift of 3 m peak, sigma 200 km north-south by 50 km
+    east-west, at lon 190, lat 26.5S; 6 h simulated.
 
     ``scale`` > 1 coarsens every grid by that factor (same extents) for quick runs.
     Returns the domain list for a multidomain setup and ``state(x, y)``, which gives
@@ -284,7 +286,7 @@ def nested_c3(scale=1, seed=20261202):
         dict(lw=lwp, ll=llp, nx=nxp, ny=nyp, method="linear"),
         dict(parent=0, ll=(192.0, -30.0), ur=(192.0 + 600 * a2 / 3, -30.0 + 600 * a2 / 3), dx_refinement_factor=3,
              method="leapfrog_linear_plus_nonlinear_friction"),
-        dict(parent=1, ll=(195.5, -27.5), ur=(195.5 + 1000 * a2 / 15, -27.5 + 1000 * a2 / 15),
+        dict(parent=1, ll=(193.0, -27.5), ur=(193.0 + 1000 * a2 / 15, -27.5 + 1000 * a2 / 15),
              dx_refinement_factor=5, method="rk2"),
     ]
     return dict(domains=domains, state=state, elevation=elevation, manning_n=0.03, duration=6 * 3600.0,

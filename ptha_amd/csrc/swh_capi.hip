@@ -469,6 +469,7 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
     a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.fpitch = h->fpitch;
     a.S = (T *)h->S[dst]; a.UH = (T *)h->UH[dst]; a.VH = (T *)h->VH[dst];
     a.S0 = (const T *)h->S[src]; a.UH0 = (const T *)h->UH[src]; a.VH0 = (const T *)h->VH[src];
+    a.C0 = (const T *)h->C[src]; a.Cw = (T *)h->C[dst];   // FP32: compensated stage (null in FP64)
     a.E = (const T *)h->E;
     a.Sq = nullptr; a.UHq = nullptr; a.VHq = nullptr;
     a.bmax = h->d_bmax;
@@ -496,12 +497,14 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
     FVArgs<T> a = fv_args<T>(h, dst, src);
     if (qsrc >= 0) {
         a.Sq = (const T *)h->S[qsrc]; a.UHq = (const T *)h->UH[qsrc]; a.VHq = (const T *)h->VH[qsrc];
+        a.Cq = (const T *)h->C[qsrc];
     }
     if (avg_src >= 0) {
         // rk2's average with state avg_src in the update (only where nothing else
         // follows the second update: no boundary condition, no forcing)
         if (h->bc_mask || h->f_on) return fail(SWH_ESTATE, "internal: folded rk2 average with a boundary or forcing");
         a.avgS = (const T *)h->S[avg_src]; a.avgUH = (const T *)h->UH[avg_src]; a.avgVH = (const T *)h->VH[avg_src];
+        a.avgC = (const T *)h->C[avg_src];
     }
     SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
     if (h->n_nest_strips && nest_dt != 0.0) {
@@ -509,22 +512,23 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
         SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, h->nest_strip_max, (const T *)h->fNS,
                                      (const T *)h->fEW, FPn, EPn, h->fpitch, 3, nest_dt, h->d_nest_int, h->stream));
     }
+    T *Cd = (T *)h->C[dst];
     if (h->bc_mask && h->bc_kind == 1)
-        SWH_HIP(launch_flather<T>((T *)h->S[dst], nullptr, (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->E, h->nx,
+        SWH_HIP(launch_flather<T>((T *)h->S[dst], Cd, (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->E, h->nx,
                                   h->ny, h->pitch, h->bc_mask, h->msl, h->bc_use_fun, h->bc_amp, h->bc_period,
                                   h->bc_msl, 0.0, h->sc, h->stream));
     else if (h->bc_mask && h->bc_kind >= 2)
-        SWH_HIP(launch_edge_bc<T>((T *)h->S[dst], nullptr, (T *)h->UH[dst], (T *)h->VH[dst], (T *)h->E, h->nx, h->ny,
+        SWH_HIP(launch_edge_bc<T>((T *)h->S[dst], Cd, (T *)h->UH[dst], (T *)h->VH[dst], (T *)h->E, h->nx, h->ny,
                                   h->pitch, h->bc_mask, h->bc_kind, h->bc_use_fun, h->bc_amp, h->bc_period,
                                   h->bc_msl, h->msl, 0.0, 0, h->sc, h->stream));
     else if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], h->nx, h->ny, h->pitch,
-                                      h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream));
+                                      h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream, Cd));
     if (h->f_on)
         // update_U then apply_forcing (domain_mod.f90:1579-1580), with the stage's step
         SWH_HIP(launch_fv_forcing<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->fS,
                                      (const T *)h->fUH, (const T *)h->fVH, h->pitch, h->f_i0, h->f_i1, h->f_j0,
-                                     h->f_j1, h->f_start, h->f_end, h->sc, h->stream));
+                                     h->f_j1, h->f_start, h->f_end, h->sc, h->stream, Cd));
     return SWH_OK;
 }
 
@@ -590,7 +594,7 @@ int fv_step(swh_domain *h, double dt)
             STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5);
             SWH_HIP(launch_fv_average<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (const T *)h->S[A],
                                          (const T *)h->UH[A], (const T *)h->VH[A], h->nx, h->ny, h->pitch, h->sc,
-                                         h->stream));
+                                         h->stream, (T *)h->C[B], (const T *)h->C[A]));
         }
         break;
     case SWH_METHOD_MIDPOINT:
@@ -614,10 +618,12 @@ int fv_step(swh_domain *h, double dt)
         STAGE(B, A, -1, dt1, 1.0, first | acc, nest1);
         for (int j = 2; j <= kRk2nN - 1; j++) STAGE(B, B, -1, dt1, 1.0, acc, nest1);
         SWH_HIP(launch_fv_rk2n_combine<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (T *)h->S[A], (T *)h->UH[A],
-                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 0, h->sc, h->stream));
+                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 0, h->sc, h->stream, (T *)h->C[B],
+                                          (T *)h->C[A]));
         STAGE(B, B, -1, dt1, reduce, acc | kStageScaleBfN, dt > 0.0 ? reduce * dt1 : 0.0);
         SWH_HIP(launch_fv_rk2n_combine<T>((T *)h->S[B], (T *)h->UH[B], (T *)h->VH[B], (T *)h->S[A], (T *)h->UH[A],
-                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 1, h->sc, h->stream));
+                                          (T *)h->VH[A], h->nx, h->ny, h->pitch, 1, h->sc, h->stream, (T *)h->C[B],
+                                          (T *)h->C[A]));
         break;
     }
     default:
@@ -792,7 +798,7 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     A((void **)&h->d_tanlat, sizeof(double) * ny);
     for (int b = 0; b < 2; b++) {
         A(&h->S[b], P); A(&h->UH[b], P); A(&h->VH[b], P);
-        if (h->prec == SWH_PREC_FP32K && !fv) A(&h->C[b], P);
+        if (h->prec == SWH_PREC_FP32K) A(&h->C[b], P);
     }
     A(&h->E, P);
     A(&h->nsq, P);

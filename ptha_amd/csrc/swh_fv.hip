@@ -29,32 +29,35 @@ __device__ __forceinline__ double fmin_(double a, double b) { return __builtin_f
 // Evaluated where it is needed (by the flux pass for a cell and its neighbours, by
 // the update for its own cell) instead of being stored: the values are the same
 // bits as a stored pass would give, and the state is read once instead of being
-// copied into three more planes.
-template <typename T>
+// copied into three more planes. Always in FP64: with FP32 state the stored values
+// (and the stage's Kahan term) are widened first, so depth = stage - elevation and the
+// velocities carry no FP32 rounding (in deep water FP32 depth alone would be off by
+// its ulp, 2.4e-4 m at 3000 m).
 struct Cell {
-    T s, uh, vh, d, u, v;
+    double s, uh, vh, d, u, v;
     bool fixed;   // negative depth clipped (negative_depth_fix_counter)
 };
 
 template <typename T>
-__device__ __forceinline__ Cell<T> cell_at(const FVArgs<T> &a, long long o)
+__device__ __forceinline__ Cell cell_at(const FVArgs<T> &a, long long o)
 {
-    Cell<T> c;
-    T s = a.S0[o], uh = a.UH0[o], vh = a.VH0[o];
-    const T e = a.E[o];
-    T d = s - e, u, v;
+    Cell c;
+    double s = a.C0 ? (double)a.S0[o] - (double)a.C0[o] : (double)a.S0[o];
+    double uh = a.UH0[o], vh = a.VH0[o];
+    const double e = a.E[o];
+    double d = s - e, u, v;
     c.fixed = false;
-    if (d > T(kMinDepth)) {
-        const T inv = T(1) / d;
+    if (d > kMinDepth) {
+        const double inv = 1.0 / d;
         u = uh * inv;
         v = vh * inv;
     } else {
-        if (d < T(0)) {
-            d = T(0);
+        if (d < 0.0) {
+            d = 0.0;
             s = e;
             c.fixed = true;
         }
-        u = T(0); v = T(0); uh = T(0); vh = T(0);
+        u = 0.0; v = 0.0; uh = 0.0; vh = 0.0;
     }
     c.s = s; c.uh = uh; c.vh = vh; c.d = d; c.u = u; c.v = v;
     return c;
@@ -87,8 +90,7 @@ struct Grad4 {
 
 // Gradients of (stage, depth, u, v) at a cell from its two neighbours along one
 // axis (the caller passes zero gradients for cells on that axis' border).
-template <typename T>
-__device__ __forceinline__ Grad4 grad3(const Cell<T> &lo, const Cell<T> &mid, const Cell<T> &hi, double theta)
+__device__ __forceinline__ Grad4 grad3(const Cell &lo, const Cell &mid, const Cell &hi, double theta)
 {
     Grad4 g;
     const double dl = lo.d, dc = mid.d, dh = hi.d;
@@ -159,7 +161,7 @@ struct EdgeOut {
 };
 
 template <typename T>
-__device__ __forceinline__ EdgeOut ns_edge(const FVArgs<T> &a, int j, const Cell<T> &Sc, const Cell<T> &C,
+__device__ __forceinline__ EdgeOut ns_edge(const FVArgs<T> &a, int j, const Cell &Sc, const Cell &C,
                                            const Grad4 &gl, const Grad4 &gc, double e_s, double e_c)
 {
     const double stage_neg = (double)Sc.s + 0.5 * gl.st;
@@ -179,7 +181,7 @@ __device__ __forceinline__ EdgeOut ns_edge(const FVArgs<T> &a, int j, const Cell
 }
 
 template <typename T>
-__device__ __forceinline__ EdgeOut ew_edge(const FVArgs<T> &a, int i, int j, const Cell<T> &Wc, const Cell<T> &C,
+__device__ __forceinline__ EdgeOut ew_edge(const FVArgs<T> &a, int i, int j, const Cell &Wc, const Cell &C,
                                            const Grad4 &el, const Grad4 &ec, double e_w, double e_c)
 {
     const double st_neg = (double)Wc.s + 0.5 * el.st;
@@ -201,7 +203,7 @@ __device__ __forceinline__ EdgeOut ew_edge(const FVArgs<T> &a, int i, int j, con
 // The owner cell's spherical terms: the 1/2 g h^2 metric source, Coriolis and the
 // Williamson tan(lat)/R term (include.f90:674-710), added to its explicit sources.
 template <typename T>
-__device__ __forceinline__ void cell_sources(const FVArgs<T> &a, int j, const Cell<T> &C, double &esu, double &esv)
+__device__ __forceinline__ void cell_sources(const FVArgs<T> &a, int j, const Cell &C, double &esu, double &esv)
 {
     if (!a.spherical) return;
     const double half_gravity = 0.5 * kGravity;
@@ -238,7 +240,8 @@ constexpr int kGEW = kFTY * (kFTX + 1);   // east-west gradients per tile
 template <typename T>
 __global__ __launch_bounds__(kFThreads) void fv_flux_kernel(FVArgs<T> a)
 {
-    __shared__ T ls_s[kFTH][kFTW], ls_d[kFTH][kFTW], ls_u[kFTH][kFTW], ls_v[kFTH][kFTW], ls_e[kFTH][kFTW];
+    __shared__ double ls_s[kFTH][kFTW], ls_d[kFTH][kFTW], ls_u[kFTH][kFTW], ls_v[kFTH][kFTW];
+    __shared__ T ls_e[kFTH][kFTW];
     __shared__ double gns_st[kFTY + 1][kFTX], gns_dp[kFTY + 1][kFTX], gns_u[kFTY + 1][kFTX], gns_v[kFTY + 1][kFTX];
     __shared__ double gew_st[kFTY][kFTX + 1], gew_dp[kFTY][kFTX + 1], gew_u[kFTY][kFTX + 1], gew_v[kFTY][kFTX + 1];
     const int i0 = blockIdx.x * kFTX, j0 = blockIdx.y * kFTY;
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(kFThreads) void fv_flux_kernel(FVArgs<T> a)
         const int lx = t % kFTW, ly = t / kFTW;
         const int gi = min(max(gx0 + lx, 0), a.nx - 1), gj = min(max(gy0 + ly, 0), a.ny - 1);
         const long long go = (long long)gj * a.pitch + gi;
-        const Cell<T> c = cell_at(a, go);
+        const Cell c = cell_at(a, go);
         ls_s[ly][lx] = c.s; ls_d[ly][lx] = c.d; ls_u[ly][lx] = c.u; ls_v[ly][lx] = c.v;
         ls_e[ly][lx] = a.E[go];
     }
@@ -255,9 +258,9 @@ __global__ __launch_bounds__(kFThreads) void fv_flux_kernel(FVArgs<T> a)
     // cell (ii, jj) of the grid from the tile
     auto G = [&](int ii, int jj) {
         const int lx = ii - gx0, ly = jj - gy0;
-        Cell<T> c;
+        Cell c;
         c.s = ls_s[ly][lx]; c.d = ls_d[ly][lx]; c.u = ls_u[ly][lx]; c.v = ls_v[ly][lx];
-        c.uh = T(0); c.vh = T(0); c.fixed = false;
+        c.uh = 0.0; c.vh = 0.0; c.fixed = false;
         return c;
     };
     const Grad4 zero{0.0, 0.0, 0.0, 0.0};
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kFThreads) void fv_flux_kernel(FVArgs<T> a)
         const long long fo = (long long)j * a.fpitch + i;
         // the cell (with its momenta, for the spherical terms) and its south and west
         // neighbours; gradients are zero on the border rows / columns
-        const Cell<T> C = cell_at(a, o);
+        const Cell C = cell_at(a, o);
         const EdgeOut r = ns_edge(a, j, L(0, -1), C, GNS(rj, ci), GNS(rj + 1, ci), (double)LE(0, -1),
                                   (double)LE(0, 0));
         a.fNS[fo] = (T)r.f0;
@@ -454,14 +457,14 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     const long long FP = (long long)a.fpitch * (a.ny + 1);
     const long long EPn = (long long)a.fpitch * a.ny;
     const double inv_cell_area_dt = dt / a.area[j];
-    const Cell<T> C = cell_at(a, o);
-    T q[3] = {C.s, C.uh, C.vh};
+    const Cell C = cell_at(a, o);
+    double q[3] = {C.s, C.uh, C.vh};
     if (a.Sq) {
-        q[0] = a.Sq[o];
+        q[0] = a.Cq ? (double)a.Sq[o] - (double)a.Cq[o] : (double)a.Sq[o];
         q[1] = a.UHq[o];
         q[2] = a.VHq[o];
     }
-    T out[3];
+    double out[3];
     if (a.skip_edges && (i == 0 || j == 0 || i == a.nx - 1 || j == a.ny - 1)) {
         // dispersive domains: the edge cells keep their values (update_U_DE1_inner_include.f90:19-31)
         out[0] = q[0];
@@ -473,8 +476,8 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     for (int kk = 0; kk < 3; kk++) {
         const T *fn = a.fNS + kk * FP;
         const T *fe = a.fEW + kk * EPn;
-        un[kk] = (double)q[kk] - inv_cell_area_dt * (((double)fn[fo + a.fpitch] - (double)fn[fo]) +
-                                                     ((double)fe[fo + 1] - (double)fe[fo]));
+        un[kk] = q[kk] - inv_cell_area_dt * (((double)fn[fo + a.fpitch] - (double)fn[fo]) +
+                                             ((double)fe[fo + 1] - (double)fe[fo]));
     }
     const double depth = (double)C.d;
     if (un[0] <= ((double)a.E[o] + kMinDepth)) {
@@ -489,21 +492,31 @@ __global__ void fv_update_kernel(FVArgs<T> a)
         un[2] = un[2] + inv_cell_area_dt * (a.esV[o] + a.esJ[fo + a.fpitch]);
         un[2] = un[2] * implicit_factor;
     }
-    out[0] = (T)un[0];
-    out[1] = (T)un[1];
-    out[2] = (T)un[2];
+    out[0] = un[0];
+    out[1] = un[1];
+    out[2] = un[2];
     fixed = C.fixed ? 1ull : 0ull;
     }
     if (a.avgS) {
         // one_rk2_step's average with backup_U (timestepping_alternatives_include.f90:123-134),
-        // on the stored (T) values as the separate pass forms it
-        out[0] = T(0.5) * (out[0] + a.avgS[o]);
-        out[1] = T(0.5) * (out[1] + a.avgUH[o]);
-        out[2] = T(0.5) * (out[2] + a.avgVH[o]);
+        // on the stored values as the separate pass forms it (FP32: UH/VH as stored, the
+        // stage with its compensation)
+        if (a.Cw) {
+            const double avs = a.avgC ? (double)a.avgS[o] - (double)a.avgC[o] : (double)a.avgS[o];
+            out[0] = 0.5 * (out[0] + avs);
+            out[1] = (double)(T(0.5) * ((T)out[1] + a.avgUH[o]));
+            out[2] = (double)(T(0.5) * ((T)out[2] + a.avgVH[o]));
+        } else {
+            out[0] = 0.5 * (out[0] + (double)a.avgS[o]);
+            out[1] = 0.5 * (out[1] + (double)a.avgUH[o]);
+            out[2] = 0.5 * (out[2] + (double)a.avgVH[o]);
+        }
     }
-    a.S[o] = out[0];
-    a.UH[o] = out[1];
-    a.VH[o] = out[2];
+    const T s_new = (T)out[0];
+    a.S[o] = s_new;
+    if (a.Cw) a.Cw[o] = (T)((double)s_new - out[0]);   // represented stage S - C = out[0]
+    a.UH[o] = (T)out[1];
+    a.VH[o] = (T)out[2];
     }
     // negative_depth_fix_counter: wave-level count, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) fixed += __shfl_xor(fixed, off);
@@ -513,14 +526,22 @@ __global__ void fv_update_kernel(FVArgs<T> a)
 // ---- rk2 average (timestepping_alternatives_include.f90:123-134) ----
 template <typename T>
 __global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny,
-                                  int pitch, DevScalars *sc)
+                                  int pitch, DevScalars *sc, T *C, const T *C0)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i == 0 && j == 0) fv_finish(*sc, kFinishRk2);
     if (i >= nx || j >= ny) return;
     const long long o = (long long)j * pitch + i;
-    S[o] = T(0.5) * (S[o] + S0[o]);
+    if (C) {
+        // FP32 state: the represented stages S - C, kept compensated
+        const double v = 0.5 * (((double)S[o] - (double)C[o]) + ((double)S0[o] - (double)C0[o]));
+        const T sn = (T)v;
+        S[o] = sn;
+        C[o] = (T)((double)sn - v);
+    } else {
+        S[o] = T(0.5) * (S[o] + S0[o]);
+    }
     UH[o] = T(0.5) * (UH[o] + UH0[o]);
     VH[o] = T(0.5) * (VH[o] + VH0[o]);
 }
@@ -530,21 +551,28 @@ __global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
 // the reference (so corner cells end with the south/north treatment).
 template <typename T>
 __global__ __launch_bounds__(1024) void fv_stage_bc_kernel(T *S, T *UH, T *VH, int nx, int ny, int pitch, int m,
-                                                           double amp, double period, double msl, const DevScalars *sc)
+                                                           double amp, double period, double msl, const DevScalars *sc,
+                                                           T *C)
 {
     const double two_pi = 2.0 * acos(-1.0);
-    const T stage = (T)(msl + amp * swhm::sin_acc(two_pi * sc->time / period));
+    const double stage_d = msl + amp * swhm::sin_acc(two_pi * sc->time / period);
+    const T stage = (T)stage_d;
+    // FP32 state: the compensation that makes S - C the FP64 boundary stage
+    const T cstage = (T)((double)stage - stage_d);
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         const long long o = (long long)j * pitch;
-        if (m & 8) { S[o] = stage; UH[o] = UH[o + 1]; VH[o] = T(0); }
-        if (m & 2) { const long long e = o + nx - 1; S[e] = stage; UH[e] = UH[e - 1]; VH[e] = T(0); }
+        if (m & 8) { S[o] = stage; if (C) C[o] = cstage; UH[o] = UH[o + 1]; VH[o] = T(0); }
+        if (m & 2) {
+            const long long e = o + nx - 1;
+            S[e] = stage; if (C) C[e] = cstage; UH[e] = UH[e - 1]; VH[e] = T(0);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-        if (m & 4) { S[i] = stage; VH[i] = VH[pitch + i]; UH[i] = T(0); }
+        if (m & 4) { S[i] = stage; if (C) C[i] = cstage; VH[i] = VH[pitch + i]; UH[i] = T(0); }
         if (m & 1) {
             const long long o = (long long)(ny - 1) * pitch + i;
-            S[o] = stage; VH[o] = VH[o - pitch]; UH[o] = T(0);
+            S[o] = stage; if (C) C[o] = cstage; VH[o] = VH[o - pitch]; UH[o] = T(0);
         }
     }
 }
@@ -570,7 +598,7 @@ __device__ void fv_finish(DevScalars &sc, int mode)
 // ---- rk2n combination (timestepping_alternatives_include.f90:212-241) ----
 template <typename T>
 __global__ void fv_rk2n_combine_kernel(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch,
-                                       int part, DevScalars *sc)
+                                       int part, DevScalars *sc, T *C, T *C0)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -579,11 +607,25 @@ __global__ void fv_rk2n_combine_kernel(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0
     const long long o = (long long)j * pitch + i;
     if (part == 0) {
         const T n_inverse = T(1.0 / (1.0 * kRk2nN));
-        S0[o] = (S0[o] - S[o]) * n_inverse;
+        if (C) {
+            const double v = (((double)S0[o] - (double)C0[o]) - ((double)S[o] - (double)C[o])) * (1.0 / (1.0 * kRk2nN));
+            const T sn = (T)v;
+            S0[o] = sn;
+            C0[o] = (T)((double)sn - v);
+        } else {
+            S0[o] = (S0[o] - S[o]) * n_inverse;
+        }
         UH0[o] = (UH0[o] - UH[o]) * n_inverse;
         VH0[o] = (VH0[o] - VH[o]) * n_inverse;
     } else {
-        S[o] = S[o] + S0[o];
+        if (C) {
+            const double v = ((double)S[o] - (double)C[o]) + ((double)S0[o] - (double)C0[o]);
+            const T sn = (T)v;
+            S[o] = sn;
+            C[o] = (T)((double)sn - v);
+        } else {
+            S[o] = S[o] + S0[o];
+        }
         UH[o] = UH[o] + UH0[o];
         VH[o] = VH[o] + VH0[o];
     }
@@ -603,45 +645,46 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
 
 template <typename T>
 hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny, int pitch,
-                             DevScalars *sc, hipStream_t s)
+                             DevScalars *sc, hipStream_t s, T *C, const T *C0)
 {
     dim3 block(256), grid((nx + 255) / 256, ny);
-    hipLaunchKernelGGL(fv_average_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, sc);
+    hipLaunchKernelGGL(fv_average_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, sc, C, C0);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
-                              double msl, const DevScalars *sc, hipStream_t s)
+                              double msl, const DevScalars *sc, hipStream_t s, T *C)
 {
     hipLaunchKernelGGL(fv_stage_bc_kernel<T>, dim3(1), dim3(1024), 0, s, S, UH, VH, nx, ny, pitch, mask, amp, period,
-                       msl, sc);
+                       msl, sc, C);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
-                                  DevScalars *sc, hipStream_t s)
+                                  DevScalars *sc, hipStream_t s, T *C, T *C0)
 {
     dim3 block(256), grid((nx + 255) / 256, ny);
     hipLaunchKernelGGL(fv_rk2n_combine_kernel<T>, grid, block, 0, s, S, UH, VH, S0, UH0, VH0, nx, ny, pitch, part,
-                       sc);
+                       sc, C, C0);
     return hipGetLastError();
 }
 
 template hipError_t launch_fv_euler<double>(const FVArgs<double> &, double, double, int, hipStream_t);
 template hipError_t launch_fv_euler<float>(const FVArgs<float> &, double, double, int, hipStream_t);
 template hipError_t launch_fv_rk2n_combine<double>(double *, double *, double *, double *, double *, double *, int,
-                                                   int, int, int, DevScalars *, hipStream_t);
+                                                   int, int, int, DevScalars *, hipStream_t, double *, double *);
 template hipError_t launch_fv_rk2n_combine<float>(float *, float *, float *, float *, float *, float *, int, int, int,
-                                                  int, DevScalars *, hipStream_t);
+                                                  int, DevScalars *, hipStream_t, float *, float *);
 template hipError_t launch_fv_average<double>(double *, double *, double *, const double *, const double *,
-                                              const double *, int, int, int, DevScalars *, hipStream_t);
+                                              const double *, int, int, int, DevScalars *, hipStream_t, double *,
+                                              const double *);
 template hipError_t launch_fv_average<float>(float *, float *, float *, const float *, const float *, const float *,
-                                             int, int, int, DevScalars *, hipStream_t);
+                                             int, int, int, DevScalars *, hipStream_t, float *, const float *);
 template hipError_t launch_fv_stage_bc<double>(double *, double *, double *, int, int, int, int, double, double,
-                                               double, const DevScalars *, hipStream_t);
+                                               double, const DevScalars *, hipStream_t, double *);
 template hipError_t launch_fv_stage_bc<float>(float *, float *, float *, int, int, int, int, double, double, double,
-                                              const DevScalars *, hipStream_t);
+                                              const DevScalars *, hipStream_t, float *);
 
 }  // namespace swh

@@ -168,7 +168,8 @@ struct SupArgs {
 // the state over the patch cells (0-based, inclusive).
 template <typename T>
 hipError_t launch_fv_forcing(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0, int i1,
-                             int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s);
+                             int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s,
+                             T *C = nullptr);
 // elevation += frac * fE over the patch (the leapfrog step's elevation forcing)
 // 'leapfrog_nonlinear' momentum half-step (swh_nonlinear.hip). Planes of pitch * ny.
 template <typename T>
@@ -331,6 +332,11 @@ struct FVArgs {
     // rk2's second update with the average folded in: the stored state is
     // 0.5 * (updated + backup) (null: plain update)
     const T *avgS = nullptr, *avgUH = nullptr, *avgVH = nullptr;
+    // FP32 state: Kahan compensation planes of S0, S, Sq and avgS (the represented stage
+    // is S - C); null in FP64
+    const T *C0 = nullptr;
+    T *Cw = nullptr;
+    const T *Cq = nullptr, *avgC = nullptr;
 };
 
 template <typename T>
@@ -339,14 +345,14 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
 // (kFinishRk2n)
 template <typename T>
 hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
-                                  DevScalars *sc, hipStream_t s);
+                                  DevScalars *sc, hipStream_t s, T *C = nullptr, T *C0 = nullptr);
 // rk2 average with backup_U; the step's bookkeeping (kFinishRk2) rides along
 template <typename T>
 hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, int nx, int ny,
-                             int pitch, DevScalars *sc, hipStream_t s);
+                             int pitch, DevScalars *sc, hipStream_t s, T *C = nullptr, const T *C0 = nullptr);
 template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
-                              double msl, const DevScalars *sc, hipStream_t s);
+                              double msl, const DevScalars *sc, hipStream_t s, T *C = nullptr);
 // number of blocks of the flux pass (size of FVArgs::bmax)
 int fv_flux_blocks(int nx, int ny);
 

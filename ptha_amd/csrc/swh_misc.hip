@@ -140,7 +140,7 @@ template hipError_t launch_bflux_leapfrog<float>(const float *, const float *, i
 // ---- forcing patches (forcing_mod.f90:80-156) ----
 template <typename T>
 __global__ void fv_forcing_kernel(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0,
-                                  int i1, int j0, int j1, double start, double end, const DevScalars *sc)
+                                  int i1, int j0, int j1, double start, double end, const DevScalars *sc, T *C)
 {
     // update_U advanced the time by the stage's step: the step covers (time - dt, time]
     const double time = sc->time, dt = sc->stage_dt;
@@ -152,7 +152,17 @@ __global__ void fv_forcing_kernel(T *S, T *UH, T *VH, const T *fS, const T *fUH,
     const int j = j0 + blockIdx.y;
     if (i > i1 || j > j1) return;
     const long long o = (long long)j * pitch + i;
-    if (fS) S[o] = (T)((double)S[o] + frac * (double)fS[o]);
+    if (fS) {
+        if (C) {
+            // FP32 state: the represented stage S - C, kept compensated
+            const double v = ((double)S[o] - (double)C[o]) + frac * (double)fS[o];
+            const T sn = (T)v;
+            S[o] = sn;
+            C[o] = (T)((double)sn - v);
+        } else {
+            S[o] = (T)((double)S[o] + frac * (double)fS[o]);
+        }
+    }
     if (fUH) UH[o] = (T)((double)UH[o] + frac * (double)fUH[o]);
     if (fVH) VH[o] = (T)((double)VH[o] + frac * (double)fVH[o]);
 }
@@ -169,11 +179,11 @@ __global__ void add_scaled_kernel(T *E, const T *fE, int pitch, int i0, int i1, 
 
 template <typename T>
 hipError_t launch_fv_forcing(T *S, T *UH, T *VH, const T *fS, const T *fUH, const T *fVH, int pitch, int i0, int i1,
-                             int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s)
+                             int j0, int j1, double start, double end, const DevScalars *sc, hipStream_t s, T *C)
 {
     if (i1 < i0 || j1 < j0) return hipSuccess;
     hipLaunchKernelGGL(fv_forcing_kernel<T>, dim3((i1 - i0 + 256) / 256, j1 - j0 + 1), dim3(256), 0, s, S, UH, VH, fS,
-                       fUH, fVH, pitch, i0, i1, j0, j1, start, end, sc);
+                       fUH, fVH, pitch, i0, i1, j0, j1, start, end, sc, C);
     return hipGetLastError();
 }
 
@@ -426,9 +436,9 @@ template hipError_t launch_flather<float>(float *, float *, float *, float *, co
 
 template hipError_t launch_fv_forcing<double>(double *, double *, double *, const double *, const double *,
                                               const double *, int, int, int, int, int, double, double,
-                                              const DevScalars *, hipStream_t);
+                                              const DevScalars *, hipStream_t, double *);
 template hipError_t launch_fv_forcing<float>(float *, float *, float *, const float *, const float *, const float *,
-                                             int, int, int, int, int, double, double, const DevScalars *, hipStream_t);
+                                             int, int, int, int, int, double, double, const DevScalars *, hipStream_t, float *);
 template hipError_t launch_add_scaled<double>(double *, const double *, int, int, int, int, int, double, hipStream_t);
 template hipError_t launch_add_scaled<float>(float *, const float *, int, int, int, int, int, float, hipStream_t);
 

@@ -7,12 +7,8 @@ device (the drop-in runs md%setup in Fortran); bench.py's nested workload reads
 them, so the GPU run does not need the reference.
 
 The time-stepping refinement factors are chosen here from the bathymetry: the
-parent takes the 'linear' CFL step (0.7), the leapfrog child the number of sub-steps
-that keeps its step below 0.8 of its stationary CFL step, and the rk2 grandchild
-below 0.4 of its own: its flow speeds add to the wave speed and its fronts run up the
-shore. (At 0.8, i.e. 6 sub-steps, the nest becomes non-finite after about 2.5 h of
-the 6 h run where the grandchild's southern boundary crosses the coast, in the
-reference's own multidomain as in this backend.)
+parent takes the 'linear' CFL step (0.7), and each nested domain the number of
+sub-steps that keeps its step below 0.8 of its stationary CFL step.
 
     python tests/golden/make_nested_tables.py [scale]
 writes tests/golden/nested_c3_tables[_sN].npz
@@ -60,7 +56,7 @@ def main(scale=1):
         e = cfg["elevation"](x, y)
         steps.append(cfl_step(e, m, 0.7) if doms[k]["method"] != "rk2" else cfl_step(e, m, 0.99) * 0.5)
     dt = steps[0]
-    safety = [0.4 if d["method"] == "rk2" else 0.8 for d in doms]
+    safety = [0.8 for d in doms]
     trf = [1] + [int(math.ceil(dt / (f * s))) for f, s in zip(safety[1:], steps[1:])]
     for d, t in zip(doms, trf):
         d["timestepping_refinement_factor"] = t
