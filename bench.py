@@ -120,31 +120,45 @@ def cpu_model():
 
 
 def read_traffic(name):
-    """(HBM bytes per launch, steps per launch) of the dominant kernel from the committed
-    rocprofv3 --pmc summary, if present."""
+    """The dominant kernel's committed rocprofv3 counter summary (profiles/), if present:
+    measured HBM bytes per launch, the compulsory bytes of that launch and its VALU issue
+    figures."""
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
     if os.path.exists(p):
         with open(p) as f:
-            t = json.load(f)
-        return t.get("bytes_per_launch"), t.get("steps_per_launch")
-    return None, None
+            return json.load(f)
+    return {}
 
 
 def roofline(r):
-    """The HBM roofline of the batched step. 'achieved' counts the algorithmic bytes of
-    the workload's unit (32 B per cell-update: one read and one write of stage, Kahan
-    term, UH, VH per step, elevation shared) over the timed region's device time; the
-    fused kernel takes several steps per pass over the state, so this can exceed the
-    peak. 'traffic' is the measured HBM bytes of one launch of the fused kernel
-    (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a plain copy kernel), and
-    'achieved_measured' / 'frac_measured' are those bytes per step over the same time."""
-    traffic, spl = read_traffic("leapfrog_batch_fp32k")
-    out = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-           "algorithmic_bytes_per_cell_update": BYTES_PER_CELL["fp32k"]}
-    if traffic and spl:
-        gbs = traffic / spl / (r["launch_ms"] * 1e-3) / 1e9
-        out.update(steps_per_launch=spl, achieved_measured=round(gbs, 1), frac_measured=round(gbs / HBM_PEAK_GBS, 4))
+    """The HBM roofline of the batched step kernel. A fused launch advances each of its
+    sources by several steps while reading and writing their state (stage, Kahan term,
+    UH, VH) once, so its compulsory traffic is 32 B per cell per launch and source (the
+    shared elevation, 4 B per cell per launch, is left out). 'achieved' is that traffic,
+    summed over the timed region's launches, over the region's device time (HIP events);
+    'per_step_equivalent_gbs' counts 32 B per cell-update instead (what a kernel taking
+    one step per pass would have to move), which exceeds the peak when launches fuse
+    steps. 'traffic' is the measured HBM bytes of one full launch (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy kernel) and 'achieved_measured' scales 'achieved' by
+    its ratio to that launch's compulsory bytes. 'valu_issue_frac' is the same launch's
+    VALU wave-instructions over 256 CUs x its busy cycles (one wave-instruction per CU
+    per cycle)."""
+    t = read_traffic("leapfrog_batch_fp32k")
+    bpc = BYTES_PER_CELL["fp32k"]
+    out = {"bound": "hbm", "achieved": round(r["comp_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(r["comp_gbs"] / HBM_PEAK_GBS, 4), "traffic": t.get("bytes_per_launch"),
+           "bytes_basis": "one read and one write of S, C, UH, VH per source and fused launch (32 B per cell)",
+           "steps_per_launch": round(r["steps_per_launch"], 3), "launches": r["launches"],
+           "per_step_equivalent_gbs": round(r["achieved_gbs"], 1),
+           "per_step_equivalent_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes_per_cell_update": bpc}
+    if t.get("bytes_per_launch") and t.get("compulsory_bytes_per_launch"):
+        ratio = t["bytes_per_launch"] / t["compulsory_bytes_per_launch"]
+        gbs = r["comp_gbs"] * ratio
+        out.update(traffic_over_compulsory=round(ratio, 4), achieved_measured=round(gbs, 1),
+                   frac_measured=round(gbs / HBM_PEAK_GBS, 4))
+    if t.get("valu_issue_frac"):
+        out["valu_issue_frac"] = t["valu_issue_frac"]
     return out
 
 
@@ -161,6 +175,23 @@ def copy_probe(mib):
     log("copy probe: %d MiB read + %d MiB written per copy, %.3f ms, %.0f GB/s" % (mib, mib, ms.value,
                                                                                 2 * n / ms.value / 1e6))
     return 2 * n / ms.value / 1e6
+
+
+def valu_probe():
+    """VALU issue reference (swh_probe_valu): FMA wave-instructions per second for FP32
+    (unpacked) and FP64 chains, 32 wavefronts per CU; the rate that the step kernels'
+    SQ_INSTS_VALU counts are compared with."""
+    import ctypes
+    from ptha_amd import backend as B
+    lib = B.load()
+    out = {}
+    for name, fp64 in (("fp32", 0), ("fp64", 1)):
+        ms, wi = ctypes.c_double(), ctypes.c_double()
+        B.check(lib.swh_probe_valu(fp64, 2048, 20000, 5, ctypes.byref(ms), ctypes.byref(wi)), "swh_probe_valu")
+        out[name] = dict(ms=round(ms.value, 4), wave_insts=wi.value, g_wave_insts_per_s=round(wi.value / ms.value / 1e6, 1))
+        log("VALU probe %s: %.0f FMA wave-instructions in %.3f ms = %.1f G/s" % (name, wi.value, ms.value,
+                                                                               wi.value / ms.value / 1e6))
+    return out
 
 
 def secondary_legs(steps):
@@ -393,10 +424,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=str, default="",
                     help="block_x,rows_per_block,variant of the step kernel (default: the batch's autotuned shape)")
-    ap.add_argument("--workload", choices=["unit_sources", "nested"], default="unit_sources",
-                    help="unit_sources (the default measurement) or the three-level nested multidomain")
+    ap.add_argument("--workload", choices=["unit_sources", "nested", "paths"], default="unit_sources",
+                    help="unit_sources (the default measurement), the three-level nested multidomain, or "
+                         "paths: only the single-grid legs (rk2 beach, 2048^2 linear, nonlinear, adjoint)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the rk2 beach and single 2048^2 grid throughput legs")
+    ap.add_argument("--valu-probe", action="store_true", help="after the timed legs, measure the VALU FMA issue rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed legs, run a plain device copy of MIB MiB (for calibrating "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE) and log its bandwidth")
@@ -408,6 +441,9 @@ def main():
 
     if args.workload == "nested":
         return nested_main(args)
+    if args.workload == "paths":
+        print(json.dumps(secondary_legs(args.steps)))
+        return
 
     from ptha_amd import configs as C
     from ptha_amd.domain import build_metrics
@@ -441,14 +477,21 @@ def main():
             shape, tune_ms = b.autotune(dt, reps=3)
             log(prec, "autotuned launch shape (block_x, rows, variant) =", shape, "%.3f ms/step" % tune_ms)
         tuned[prec] = shape
-        wall, ms = time_steps(b, dt, args.steps, args.warmup)
+        if args.warmup:
+            b.evolve(dt, args.warmup)
+        c0 = b.launch_counts()
+        wall, ms = time_steps(b, dt, args.steps, 0)
+        c1 = b.launch_counts()
         step, nact = b.status()
         assert nact == nsrc
         gcell = nsrc * cells * args.steps / wall / 1e9
         launch_ms = ms / args.steps
         alg = BYTES_PER_CELL[prec] * nsrc * cells
+        nl, sl, ss = (c1[i] - c0[i] for i in range(3))
+        assert ss == nsrc * args.steps
         res[prec] = dict(wall=wall, event_ms=ms, gcell=gcell, launch_ms=launch_ms,
-                         achieved_gbs=alg / (launch_ms * 1e-3) / 1e9)
+                         achieved_gbs=alg / (launch_ms * 1e-3) / 1e9, launches=nl, steps_per_launch=ss / sl,
+                         comp_gbs=BYTES_PER_CELL[prec] * cells * sl / (ms * 1e-3) / 1e9)
         log(prec, "Gcell/s %.1f  ms/step %.3f  HBM %.0f GB/s" % (gcell, 1e3 * wall / args.steps,
                                                                  res[prec]["achieved_gbs"]))
         b.close()
@@ -484,6 +527,8 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+    if args.valu_probe:
+        valu_probe()
 
     secondary = None
     if not args.no_secondary:
@@ -522,8 +567,10 @@ def main():
     if "fp64" in res:
         r64 = res["fp64"]
         out["fp64"] = {"value": round(r64["gcell"], 3), "ms_per_step": round(1e3 * r64["wall"] / args.steps, 4),
-                       "achieved_gbs": round(r64["achieved_gbs"], 1),
-                       "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                       "achieved_gbs": round(r64["comp_gbs"], 1),
+                       "frac": round(r64["comp_gbs"] / HBM_PEAK_GBS, 4),
+                       "steps_per_launch": round(r64["steps_per_launch"], 3),
+                       "per_step_equivalent_gbs": round(r64["achieved_gbs"], 1),
                        "fp32k_over_fp64": round(r32["gcell"] / r64["gcell"], 3)}
     if secondary:
         out["secondary"] = secondary

@@ -297,8 +297,10 @@ int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block
  * own duration, writing gauge series (point_gauge_mod.f90:247-294). A batch advances all
  * of them together: one launch per step covers every active source; a source stops
  * (its state and gauges freeze) once it has taken its own number of steps. The desc
- * describes the shared grid; method must be SWH_METHOD_LINEAR or SWH_METHOD_LLPNF and
- * max_U tracking is not available in a batch. */
+ * describes the shared grid; method must be SWH_METHOD_LINEAR or SWH_METHOD_LLPNF (the
+ * latter with linear_solver_is_truely_linear set: the batch precomputes the friction work
+ * from the elevation, so SWH_EINVAL otherwise) and max_U tracking is not available in a
+ * batch. */
 typedef struct swh_batch swh_batch;
 
 int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **out);
@@ -330,6 +332,12 @@ int swh_batch_evolve(swh_batch *b, double dt, int32_t nsteps);
 int swh_batch_gauges_download(swh_batch *b, int32_t k, double *values, int32_t *nsamples);
 /* Steps taken so far by the batch, and number of sources still active. */
 int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active);
+/* Step-kernel launches so far (the autotuner's included): their number, the sources
+ * they advanced summed over launches, and sources x fused steps summed over launches.
+ * Each launch reads and writes every advanced source's state once, so differences of
+ * these counters give a region's compulsory state traffic and its mean fused depth.
+ * No counterpart in the reference (measurement). */
+int swh_batch_launch_counts(swh_batch *b, int64_t *launches, int64_t *source_launches, int64_t *source_steps);
 /* Launch shape of the leapfrog step kernel: threads per block (64..256, multiple of
  * 64), rows per thread strip, and variant bits: 1 = 16-byte vector lanes (4 floats /
  * 2 doubles per lane), 2 = keep the next row's loads in flight, 4 = non-temporal
@@ -386,6 +394,12 @@ int swh_batch_evolve_timed(swh_batch *b, double dt, int32_t nsteps, double *elap
  * reference for rocprofv3 FETCH_SIZE / WRITE_SIZE and the streaming-bandwidth
  * ceiling quoted next to the roofline. */
 int swh_probe_copy(int64_t bytes, int32_t reps, double *ms_per_copy);
+/* Measurement reference for VALU issue (no counterpart in the reference): blocks x 256
+ * threads each run eight independent FMA chains for iters rounds in FP32 (unpacked) or
+ * FP64; returns the mean device time of one launch over reps launches and the FMA
+ * wave-instructions per launch. */
+int swh_probe_valu(int32_t fp64, int32_t blocks, int32_t iters, int32_t reps, double *ms_per_launch,
+                   double *wave_insts);
 
 /* Check utility (no counterpart in the reference): evaluates the FP64 friction pow
  * pow(x[k], y[k]) and the boundary sine sin(t[k]) that the kernels use in place of

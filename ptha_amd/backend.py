@@ -109,6 +109,8 @@ SYMBOLS = [
     ("swh_batch_evolve", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
     ("swh_batch_gauges_download", ctypes.c_int, [ctypes.c_void_p, _I32, _PD, _PI32]),
     ("swh_batch_status", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _PI32]),
+    ("swh_batch_launch_counts", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     ("swh_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
     ("swh_batch_autotune", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PI32, _PD]),
     ("swh_domain_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
@@ -130,6 +132,7 @@ SYMBOLS = [
     ("swh_adjoint_inject", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_adjoint_run", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _I32, _PD]),
     ("swh_probe_copy", ctypes.c_int, [ctypes.c_int64, _I32, _PD]),
+    ("swh_probe_valu", ctypes.c_int, [_I32, _I32, _I32, _I32, _PD, _PD]),
     ("swh_probe_math", ctypes.c_int, [_I32, _PD, _PD, _PD, _PD, _PD, _PD, _PD]),
     ("swh_md_create", ctypes.c_int, [_I32, ctypes.POINTER(ctypes.c_void_p), _PD, _PI32, _PI32,
                                      ctypes.POINTER(ctypes.c_void_p)]),

@@ -240,3 +240,10 @@ class UnitSourceBatch:
         na = B._I32()
         B.check(self._lib.swh_batch_status(self._h, ctypes.byref(step), ctypes.byref(na)), "swh_batch_status")
         return step.value, na.value
+
+    def launch_counts(self):
+        """(launches, sources summed over launches, source steps summed over launches) of
+        the step kernels so far, the autotuner's included."""
+        v = [ctypes.c_int64() for _ in range(3)]
+        B.check(self._lib.swh_batch_launch_counts(self._h, *[ctypes.byref(x) for x in v]), "swh_batch_launch_counts")
+        return tuple(x.value for x in v)

@@ -1619,6 +1619,9 @@ struct swh_batch {
     int fuse = 2;                  // most fused steps per launch (2..4) where two_step is on
     int xcd_order = 0;             // two-step launches: XCD-aware tile order
     int src_group = 1;             // ... interleaving the tiles of this many sources
+    // step-kernel launches so far: count, sources summed over launches, and source
+    // steps (sources x fused steps) summed over launches
+    long long n_launch = 0, src_launch = 0, src_steps = 0;
 };
 
 namespace {
@@ -1711,6 +1714,7 @@ int batch_launch(swh_batch *b, LFArgs<T> &a, int fric_mode, int n_active, int c)
     a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
     SWH_HIP(launch_leapfrog<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n_active,
                                       b->block_x, b->variant, g->stream));
+    b->n_launch++; b->src_launch += n_active; b->src_steps += n_active;
     return SWH_OK;
 }
 
@@ -1737,6 +1741,7 @@ int batch_launch2(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     a.lf2_group = b->src_group;
     SWH_HIP(launch_leapfrog2<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n, b->block_x,
                                        g->stream, (b->variant & 2) ? 1 : 0));
+    b->n_launch++; b->src_launch += n; b->src_steps += 2LL * n;
     a.src_base = 0;
     a.lf2_order = 0;
     return SWH_OK;
@@ -1757,6 +1762,7 @@ int batch_launchn(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     a.lf2_group = b->src_group;
     SWH_HIP(launch_leapfrogn<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n, b->block_x,
                                        g->stream, (b->variant & 2) ? 1 : 0, ns));
+    b->n_launch++; b->src_launch += n; b->src_steps += (long long)ns * n;
     a.src_base = 0;
     a.lf2_order = 0;
     return SWH_OK;
@@ -1958,6 +1964,9 @@ int swh_batch_create(const swh_domain_desc *desc, int32_t nsources, swh_batch **
     *out = nullptr;
     if (desc->timestepping_method != SWH_METHOD_LINEAR && desc->timestepping_method != SWH_METHOD_LLPNF)
         return fail(SWH_EINVAL, "batches support the leapfrog methods only");
+    if (desc->timestepping_method == SWH_METHOD_LLPNF && !desc->linear_solver_is_truely_linear)
+        return fail(SWH_EINVAL, "batches of 'leapfrog_linear_plus_nonlinear_friction' need "
+                                "linear_solver_is_truely_linear (friction work precomputed from the elevation)");
     swh_domain_desc d = *desc;
     d.max_U_mask = 0;
     swh_batch *b = new swh_batch();
@@ -2137,6 +2146,15 @@ int swh_batch_gauges_download(swh_batch *b, int32_t k, double *values, int32_t *
     return SWH_OK;
 }
 
+int swh_batch_launch_counts(swh_batch *b, int64_t *launches, int64_t *source_launches, int64_t *source_steps)
+{
+    if (!b || !launches || !source_launches || !source_steps) return fail(SWH_EINVAL, "null argument");
+    *launches = b->n_launch;
+    *source_launches = b->src_launch;
+    *source_steps = b->src_steps;
+    return SWH_OK;
+}
+
 int swh_batch_status(swh_batch *b, int64_t *step, int32_t *n_active)
 {
     if (!b || !step || !n_active) return fail(SWH_EINVAL, "null argument");
@@ -2174,6 +2192,34 @@ int swh_probe_copy(int64_t bytes, int32_t reps, double *ms_per_copy)
     if (e1) (void)hipEventDestroy(e1);
     dfree(a);
     dfree(b);
+    return rc;
+}
+
+int swh_probe_valu(int32_t fp64, int32_t blocks, int32_t iters, int32_t reps, double *ms_per_launch,
+                   double *wave_insts)
+{
+    if (blocks < 1 || blocks > (1 << 20) || iters < 1 || reps < 1 || !ms_per_launch || !wave_insts)
+        return fail(SWH_EINVAL, "swh_probe_valu: bad argument");
+    void *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = dmalloc(&out, (size_t)blocks * 256 * (fp64 ? 8 : 4));
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(SWH_EHIP, "swh_probe_valu: event creation");
+    float ms = 0.f;
+    if (!rc && launch_probe_valu(fp64, blocks, iters, out, nullptr) != hipSuccess) rc = fail(SWH_EHIP, "probe launch");
+    if (!rc && hipEventRecord(e0, nullptr) != hipSuccess) rc = fail(SWH_EHIP, "event record");
+    for (int k = 0; k < reps && !rc; k++)
+        if (launch_probe_valu(fp64, blocks, iters, out, nullptr) != hipSuccess) rc = fail(SWH_EHIP, "probe launch");
+    if (!rc && (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(SWH_EHIP, "event timing");
+    if (!rc) {
+        *ms_per_launch = (double)ms / reps;
+        *wave_insts = (double)blocks * 4 * 8 * iters;   // 4 wavefronts per block, 8 FMAs per round
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(out);
     return rc;
 }
 
@@ -2280,9 +2326,24 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
     // events per launch: bounded device output (<= 2^28 doubles) and weight matrix
     long long chunk = std::max<long long>(TE, ((1LL << 28) / std::max<long long>(ncol, Kpad)) / TE * TE);
     chunk = std::min<long long>(chunk, (n_events + TE - 1) / TE * TE);
-    double *d_w = nullptr, *d_scale = nullptr, *d_out = nullptr;
-    int *d_nv = nullptr;
-    int rc = dmalloc((void **)&d_w, sizeof(double) * chunk * Kpad);
+    double *d_w = nullptr, *d_scale = nullptr, *d_out = nullptr, *d_m = nullptr;
+    int *d_nv = nullptr, *d_flags = nullptr;
+    // slots whose series hold a non-finite value (a source that blew up): only the
+    // events that contain them may see it
+    std::vector<int> flagged(b->nsrc, 0);
+    bool any_flagged = false;
+    int rc = dmalloc((void **)&d_flags, sizeof(int) * b->nsrc);
+    if (!rc) {
+        hipError_t he = hipMemsetAsync(d_flags, 0, sizeof(int) * b->nsrc, g->stream);
+        if (he == hipSuccess)
+            he = launch_nonfinite_slots(b->d_gval, b->nsrc, (long long)b->gmax * 3 * b->ng, d_flags, g->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(g->stream);
+        if (he == hipSuccess) he = hipMemcpy(flagged.data(), d_flags, sizeof(int) * b->nsrc, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) rc = hip_fail(he, "superposition non-finite scan", __FILE__, __LINE__);
+        for (int k = 0; k < b->nsrc; k++) any_flagged |= flagged[k] != 0;
+    }
+    if (!rc) rc = dmalloc((void **)&d_w, sizeof(double) * chunk * Kpad);
+    if (!rc && any_flagged) rc = dmalloc((void **)&d_m, sizeof(double) * chunk * Kpad);
     if (!rc) rc = dmalloc((void **)&d_scale, sizeof(double) * chunk);
     if (!rc) rc = dmalloc((void **)&d_nv, sizeof(int) * chunk);
     if (!rc) rc = dmalloc((void **)&d_out, sizeof(double) * chunk * ncol);
@@ -2290,11 +2351,12 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
     double total_ms = 0.0;
     if (!rc && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess))
         rc = fail(SWH_EHIP, "hipEventCreate failed");
-    std::vector<double> W, scale;
+    std::vector<double> W, M, scale;
     std::vector<int> nv;
     for (long long e0 = 0; !rc && e0 < n_events; e0 += chunk) {
         const int E = (int)std::min<long long>(chunk, n_events - e0);
         W.assign((size_t)chunk * Kpad, 0.0);
+        if (any_flagged) M.assign((size_t)chunk * Kpad, 0.0);
         scale.assign(chunk, 0.0);
         nv.assign(chunk, 0);
         int smax = 1;
@@ -2306,6 +2368,7 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
                 // uniform slip: unit weights, the slip scales the sum
                 // (sum_tsunami_unit_sources.R:658-669); stochastic: per-source slips (:672-687)
                 W[(size_t)e * Kpad + slot] += event_w ? event_w[q] : 1.0;
+                if (any_flagged && flagged[slot]) M[(size_t)e * Kpad + slot] = 1.0;
                 n_e = std::min(n_e, ns_slot[slot]);
             }
             scale[e] = event_slip ? event_slip[ev] : 1.0;
@@ -2315,7 +2378,9 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
         }
         if (hipMemcpy(d_w, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(d_scale, scale.data(), sizeof(double) * chunk, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(d_nv, nv.data(), sizeof(int) * chunk, hipMemcpyHostToDevice) != hipSuccess) {
+            hipMemcpy(d_nv, nv.data(), sizeof(int) * chunk, hipMemcpyHostToDevice) != hipSuccess ||
+            (any_flagged &&
+             hipMemcpy(d_m, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice) != hipSuccess)) {
             rc = fail(SWH_EHIP, "superposition upload failed");
             break;
         }
@@ -2325,6 +2390,7 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
         a.W = d_w; a.Kpad = Kpad; a.E = E; a.scale = d_scale; a.nvalid = d_nv; a.msl = msl;
         a.s_chunk = 16; a.s_max = std::min(smax, b->gmax);
         a.out = d_out;
+        a.M = any_flagged ? d_m : nullptr;
         hipError_t he = hipEventRecord(t0, g->stream);
         if (he == hipSuccess) he = launch_superpose(a, summary == SWH_SUPERPOSE_SERIES ? 0 : 1, g->stream);
         if (he == hipSuccess) he = hipEventRecord(t1, g->stream);
@@ -2341,7 +2407,7 @@ int swh_batch_superpose(swh_batch *b, int32_t n_events, const int32_t *event_ptr
     }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
-    dfree(d_w); dfree(d_scale); dfree(d_nv); dfree(d_out);
+    dfree(d_w); dfree(d_scale); dfree(d_nv); dfree(d_out); dfree(d_m); dfree(d_flags);
     if (!rc && kernel_ms) *kernel_ms = total_ms;
     return rc;
 }

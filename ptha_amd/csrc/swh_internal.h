@@ -144,6 +144,7 @@ hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode
 
 // Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
 hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
+hipError_t launch_probe_valu(int fp64, int blocks, int iters, void *out, hipStream_t s);
 hipError_t launch_probe_math(const double *x, const double *y, const double *t, double *pw, double *sn, int n,
                              hipStream_t s);
 
@@ -161,6 +162,10 @@ struct SupArgs {
     double msl;
     int s_chunk, s_max;      // maximum-stage summary: samples per block, samples to scan
     double *out;             // mode 0: [E][gmax * 3 * ng]; mode 1: [E][ng]
+    // set when some slot's series holds a non-finite value: [Epad][Kpad] 1 where the
+    // event has such a slot (non-finite values then enter the product as 0, and the
+    // outputs that an event's non-finite members reach are NaN)
+    const double *M = nullptr;
 };
 // Forcing patch on a finite-volume state after update_U (forcing_mod.f90:80-156):
 // the fraction of [start, end] that the stage (time - stage_dt, time] covers, read
@@ -259,6 +264,9 @@ hipError_t launch_add_scaled(T *E, const T *fE, int pitch, int i0, int i1, int j
 // rows of the event tile (W, scale and nvalid must hold a multiple of it)
 int superpose_event_tile();
 hipError_t launch_superpose(const SupArgs &a, int mode, hipStream_t s);
+// flags[slot] = 1 where the slot's series (slot_stride values each) holds a non-finite
+// value; flags must start at 0.
+hipError_t launch_nonfinite_slots(const double *X, int nsrc, long long slot_stride, int *flags, hipStream_t s);
 
 // Gauge values after one step, from the current state (batched runs).
 template <typename T, bool KAHAN>

@@ -668,6 +668,37 @@ hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStr
     return hipGetLastError();
 }
 
+// ---- VALU throughput (measurement reference) ----
+// Eight independent fused multiply-add chains per lane, iters rounds: 8 * iters VALU
+// FMA wave-instructions per wavefront (FP32 ones unpacked: this code is built without
+// packed FP32), the issue rate that the stencil kernels' VALU counts are compared with.
+template <typename T>
+__global__ __launch_bounds__(256) void probe_valu_kernel(T x, T y, int iters, T *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    T a[8];
+    for (int k = 0; k < 8; k++) a[k] = (T)(t + k);
+    for (int it = 0; it < iters; it++)
+        for (int k = 0; k < 8; k++) {
+            if constexpr (sizeof(T) == 8) a[k] = __builtin_fma(a[k], x, y);
+            else a[k] = __builtin_fmaf(a[k], x, y);
+        }
+    T r = a[0];
+    for (int k = 1; k < 8; k++) r += a[k];
+    out[t] = r;
+}
+
+hipError_t launch_probe_valu(int fp64, int blocks, int iters, void *out, hipStream_t s)
+{
+    if (fp64)
+        hipLaunchKernelGGL(probe_valu_kernel<double>, dim3(blocks), dim3(256), 0, s, 0.999999, 1e-7, iters,
+                           (double *)out);
+    else
+        hipLaunchKernelGGL(probe_valu_kernel<float>, dim3(blocks), dim3(256), 0, s, 0.999999f, 1e-7f, iters,
+                           (float *)out);
+    return hipGetLastError();
+}
+
 // ---- the FP64 friction pow and boundary sine (swh_mathx.h) on the device ----
 // One argument per thread; the same functions compiled for the host give the same bits.
 __global__ __launch_bounds__(256) void probe_math_kernel(const double *__restrict__ x, const double *__restrict__ y,

@@ -59,3 +59,23 @@ def test_argument_checks_of_md_batch_and_probe_without_device():
     assert lib.swh_md_evolve_one_step(None, 1.0) != 0
     assert lib.swh_batch_autotune(None, 1.0, 1, None, None) == -1
     assert lib.swh_batch_superpose(None, 1, None, None, None, None, 0.0, 0, None, None, None) == -1
+
+
+def test_batch_rejects_llpnf_without_truely_linear_solver():
+    """A 'leapfrog_linear_plus_nonlinear_friction' batch needs the truely-linear solver
+    (its friction work is precomputed from the elevation): rejected on the host with a
+    message naming the flag, before any device call."""
+    from ptha_amd import backend as B
+    lib = B.load()
+    d = B.DomainDesc()
+    d.nx, d.ny = 64, 64
+    d.timestepping_method = 6   # SWH_METHOD_LLPNF
+    d.linear_solver_is_truely_linear = 0
+    h = ctypes.c_void_p()
+    assert lib.swh_batch_create(ctypes.byref(d), 4, ctypes.byref(h)) == -1
+    buf = ctypes.create_string_buffer(256)
+    lib.swh_last_error(buf, 256)
+    assert b"linear_solver_is_truely_linear" in buf.value
+    assert not h.value
+    n = ctypes.c_int64()
+    assert lib.swh_batch_launch_counts(None, ctypes.byref(n), ctypes.byref(n), ctypes.byref(n)) == -1

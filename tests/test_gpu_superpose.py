@@ -94,3 +94,40 @@ def test_superposed_event_equals_run_of_event():
     assert np.abs(got[0] - direct).max() <= 1e-11 * np.abs(direct).max()
     one.close()
     b.close()
+
+
+def test_non_finite_source_reaches_only_its_events():
+    """A unit source whose run goes non-finite (a NaN planted in its initial stage at the
+    centre gauge's cell) makes NaN exactly the outputs of the events that contain it
+    where its series is non-finite, zero weight included, as the reference's sum over
+    each event's own sources does; every other event equals the oracle's finite result."""
+    print("CASE superpose with a non-finite unit source", flush=True)
+    nsteps = [60, 60, 40, 60, 60]
+    u, b, bumps = _batch(nsteps)
+    ij = C.gauge_site_index(u["gauges_xy"], u["ll"], (u["lw"][0] / u["nx"], u["lw"][1] / u["ny"]), u["nx"],
+                            u["ny"])
+    i, j = ij[len(ij) // 2]
+    s = bumps[2].copy()
+    s[j - 1, i - 1] = np.nan
+    b.set_source(2, s)
+    b.evolve(20.0, 70)
+    series = [b.gauge_series(k) for k in range(NSRC)]
+    ns = [len(q) for q in series]
+    assert not np.isfinite(series[2]).all() and np.isfinite(series[2]).any()
+    assert all(np.isfinite(series[k]).all() for k in (0, 1, 3, 4))
+    scale = max(np.abs(q[np.isfinite(q)]).max() for q in series)
+    ev = [[0, 1], [2], [1, 2, 3], [3, 4], [0, 2, 4], [4]]
+    w = [[1.0, 2.0], [0.5], [1.0, 0.0, 1.0], [2.0, 1.0], [0.0, 0.0, 1.0], [1.5]]
+    slip = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]
+    for msl in (0.0, 0.4):
+        got, nv = b.superpose(ev, slip=slip, msl=msl, summary="series")
+        for e in range(len(ev)):
+            ref = SP.event_series(series, ns, ev[e], slip=slip[e], msl=msl)
+            assert nv[e] == len(ref), e
+            np.testing.assert_allclose(got[e, :nv[e]], ref, rtol=0, atol=1e-12 * scale * max(slip), equal_nan=True)
+            assert np.isfinite(got[e, :nv[e]]).all() == (2 not in ev[e]), e
+        got, nv = b.superpose(ev, weights=w, msl=msl, summary="max_stage")
+        for e in range(len(ev)):
+            ref = SP.event_max_stage(series, ns, ev[e], weights=w[e], msl=msl)
+            np.testing.assert_allclose(got[e], ref, rtol=0, atol=1e-12 * scale * NSRC * 3, equal_nan=True)
+    b.close()

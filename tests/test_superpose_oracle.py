@@ -82,3 +82,27 @@ def test_uniform_and_stochastic_forms_agree(oracle_built):
     b = SP.event_series(runs, ns, [0, 1, 3], weights=[1.7, 1.7, 1.7])
     assert a.shape == (7, 3, 5)
     np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-14)
+
+
+def test_non_finite_source_reaches_only_its_events():
+    """The reference sums each event's own unit sources (sum_tsunami_unit_sources.R:
+    658-690, R arithmetic: NaN + x and 0 * NaN are NaN, and max over a vector holding
+    NaN is NaN): a unit source whose series went non-finite makes NaN exactly the
+    entries of the events that contain it where its series is non-finite (whatever its
+    weight, zero included), and every other event stays finite."""
+    rng = np.random.default_rng(5)
+    ns, ng = [12, 12, 9], 4
+    series = [rng.normal(size=(n, 3, ng)) for n in ns]
+    series[1][7:, 0, 2] = np.nan      # stage at gauge 2 from sample 7
+    series[1][9, 1, 0] = np.inf       # one uh value
+    bad = ~np.isfinite(series[1])
+    for src, w in (([0, 2], None), ([0, 1], None), ([1, 2], [0.0, 2.0]), ([1], [3.0])):
+        ev = SP.event_series(series, ns, src, weights=w, slip=2.0 if w is None else None, msl=0.3)
+        n = min(ns[k] for k in src)
+        if 1 in src:
+            assert np.array_equal(~np.isfinite(ev), bad[:n]), src
+        else:
+            assert np.isfinite(ev).all(), src
+        mx = SP.event_max_stage(series, ns, src, weights=w, slip=2.0 if w is None else None, msl=0.3)
+        want_nan = bad[:n, 0, :].any(axis=0) if 1 in src else np.zeros(ng, bool)
+        assert np.array_equal(np.isnan(mx), want_nan), src
