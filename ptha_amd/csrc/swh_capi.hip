@@ -500,11 +500,14 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
         a.Cq = (const T *)h->C[qsrc];
     }
     if (avg_src >= 0) {
-        // rk2's average with state avg_src in the update (only where nothing else
-        // follows the second update: no boundary condition, no forcing)
-        if (h->bc_mask || h->f_on) return fail(SWH_ESTATE, "internal: folded rk2 average with a boundary or forcing");
+        // rk2's average with state avg_src in the update (only where nothing but the
+        // stage boundary follows the second update: no forcing, no Flather or edge
+        // boundary); the stage boundary then averages its own cells from the ring
+        if (h->f_on || (h->bc_mask && h->bc_kind != 0))
+            return fail(SWH_ESTATE, "internal: folded rk2 average with a forcing or a Flather/edge boundary");
         a.avgS = (const T *)h->S[avg_src]; a.avgUH = (const T *)h->UH[avg_src]; a.avgVH = (const T *)h->VH[avg_src];
         a.avgC = (const T *)h->C[avg_src];
+        if (h->bc_mask) a.ring = (T *)h->ring;
     }
     SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
     if (h->n_nest_strips && nest_dt != 0.0) {
@@ -521,6 +524,11 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
         SWH_HIP(launch_edge_bc<T>((T *)h->S[dst], Cd, (T *)h->UH[dst], (T *)h->VH[dst], (T *)h->E, h->nx, h->ny,
                                   h->pitch, h->bc_mask, h->bc_kind, h->bc_use_fun, h->bc_amp, h->bc_period,
                                   h->bc_msl, h->msl, 0.0, 0, h->sc, h->stream));
+    else if (h->bc_mask && avg_src >= 0)
+        SWH_HIP(launch_fv_stage_bc_avg<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->S[avg_src],
+                                          (const T *)h->UH[avg_src], (const T *)h->VH[avg_src], (const T *)h->ring,
+                                          h->nx, h->ny, h->pitch, h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl,
+                                          h->sc, h->stream, Cd, (const T *)h->C[avg_src]));
     else if (h->bc_mask)
         SWH_HIP(launch_fv_stage_bc<T>((T *)h->S[dst], (T *)h->UH[dst], (T *)h->VH[dst], h->nx, h->ny, h->pitch,
                                       h->bc_mask, h->bc_amp, h->bc_period, h->bc_msl, h->sc, h->stream, Cd));
@@ -587,8 +595,9 @@ int fv_step(swh_domain *h, double dt)
         // one_rk2_step (include.f90:64-146): two Euler stages with the same step,
         // nesting fluxes tracked half a step each, then the average with backup_U
         STAGE(B, A, -1, dt, 1.0, first | acc, dt * 0.5);
-        if (!h->bc_mask && !h->f_on) {
-            // the average folded into the second update (nothing acts between them)
+        if (!h->f_on && (!h->bc_mask || h->bc_kind == 0)) {
+            // the average folded into the second update (nothing but a stage boundary
+            // acts between them, and that averages its own cells)
             STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5, A);
         } else {
             STAGE(B, B, -1, dt, 1.0, acc, dt * 0.5);
@@ -817,6 +826,7 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
         A((void **)&h->esV, h->plane_elems() * sizeof(double));
         A((void **)&h->esJ, FPn * sizeof(double));
         A((void **)&h->esE, FPn * sizeof(double));
+        A(&h->ring, (size_t)2 * (nx + ny) * h->esz);
     }
     A(&h->rc, h->esz * ((size_t)5 * ny + (nx + 1) + (ny + 1)));
     A((void **)&h->sc, sizeof(DevScalars));
@@ -864,7 +874,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
                     h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt,
-                    h->d_stats, h->d_ggrad};
+                    h->d_stats, h->d_ggrad, h->ring};
     for (void *p : ptrs) dfree(p);
     for (void *p : h->ckpt) dfree(p);
     dfree(h->snap);

@@ -27,6 +27,9 @@ struct swh_domain {
     void *depth = nullptr, *u = nullptr, *v = nullptr, *fNS = nullptr, *fEW = nullptr;
     double *esU = nullptr, *esV = nullptr, *esJ = nullptr, *esE = nullptr;
     double *d_bmax = nullptr;   // per-block CFL maxima of the flux pass
+    // rk2 with a stage boundary and the average folded into the second update: the
+    // pre-average UH of columns 1 and nx-2 and VH of rows 1 and ny-2 (element type T)
+    void *ring = nullptr;
     // dispersion (leapfrog methods, midpoint/rk2/euler): stored UH/VH/elevation, right-hand side, three
     // extrapolation levels (oldest first) with their times, tridiagonal work
     int disp_on = 0, disp_iter = 2;

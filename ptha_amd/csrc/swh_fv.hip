@@ -448,8 +448,9 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     const int j = blockIdx.y;
     unsigned long long fixed = 0;
     // with the rk2 average folded in, the step's closing bookkeeping (it does not touch
-    // stage_dt, the one scalar this kernel reads)
-    if (a.avgS && i == 0 && j == 0) fv_finish(*a.sc, kFinishRk2);
+    // stage_dt, the one scalar this kernel reads); with a stage boundary still to come,
+    // which reads the stage's time, the boundary kernel does it
+    if (a.avgS && !a.ring && i == 0 && j == 0) fv_finish(*a.sc, kFinishRk2);
     if (i < a.nx && j < a.ny) {
     const double dt = a.sc->stage_dt;
     const long long o = (long long)j * a.pitch + i;
@@ -496,6 +497,13 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     out[1] = un[1];
     out[2] = un[2];
     fixed = C.fixed ? 1ull : 0ull;
+    }
+    if (a.ring) {
+        // pre-average momenta next to the boundary lines, for the stage boundary
+        if (i == 1) a.ring[j] = (T)out[1];
+        if (i == a.nx - 2) a.ring[a.ny + j] = (T)out[1];
+        if (j == 1) a.ring[2 * a.ny + i] = (T)out[2];
+        if (j == a.ny - 2) a.ring[2 * a.ny + a.nx + i] = (T)out[2];
     }
     if (a.avgS) {
         // one_rk2_step's average with backup_U (timestepping_alternatives_include.f90:123-134),
@@ -595,6 +603,58 @@ __device__ void fv_finish(DevScalars &sc, int mode)
     sc.bf_time = sc.bf_time + sc.bf_evolve;
 }
 
+// Stage boundary with rk2's average folded into the preceding update (the separate
+// order: update, boundary, average). The boundary cells' pre-average values follow the
+// rules of fv_stage_bc_kernel -- west/east columns first, then south/north rows,
+// which overwrite the corners -- with the updated interior neighbours taken from the
+// ring (the cells (i, 1) of an active column line hold that column's value, VH = 0);
+// each is then averaged with backup_U exactly as fv_average_kernel does.
+template <typename T>
+__device__ __forceinline__ void bc_avg_store(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, T *C,
+                                             const T *C0, long long o, T stage, T cstage, T uh, T vh)
+{
+    if (C) {
+        const double v = 0.5 * (((double)stage - (double)cstage) + ((double)S0[o] - (double)C0[o]));
+        const T sn = (T)v;
+        S[o] = sn;
+        C[o] = (T)((double)sn - v);
+    } else {
+        S[o] = T(0.5) * (stage + S0[o]);
+    }
+    UH[o] = T(0.5) * (uh + UH0[o]);
+    VH[o] = T(0.5) * (vh + VH0[o]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void fv_stage_bc_avg_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
+                                                               const T *VH0, const T *ring, int nx, int ny, int pitch,
+                                                               int m, double amp, double period, double msl,
+                                                               DevScalars *sc, T *C, const T *C0)
+{
+    const double two_pi = 2.0 * acos(-1.0);
+    const double stage_d = msl + amp * swhm::sin_acc(two_pi * sc->time / period);
+    const T stage = (T)stage_d;
+    const T cstage = C ? (T)((double)stage - stage_d) : T(0);
+    // every thread has read the stage's time; then the step's closing bookkeeping
+    __syncthreads();
+    if (threadIdx.x == 0) fv_finish(*sc, kFinishRk2);
+    const T *rW = ring, *rE = ring + ny, *rS = ring + 2 * ny, *rN = ring + 2 * ny + nx;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        const long long o = (long long)j * pitch;
+        if (m & 8) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, o, stage, cstage, rW[j], T(0));
+        if (m & 2) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, o + nx - 1, stage, cstage, rE[j], T(0));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        // VH of the row next to the line: 0 where an active column line holds that cell
+        const bool col = ((m & 8) && i == 0) || ((m & 2) && i == nx - 1);
+        if (m & 4) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, i, stage, cstage, T(0), col ? T(0) : rS[i]);
+        if (m & 1)
+            bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, (long long)(ny - 1) * pitch + i, stage, cstage, T(0),
+                            col ? T(0) : rN[i]);
+    }
+}
+
 // ---- rk2n combination (timestepping_alternatives_include.f90:212-241) ----
 template <typename T>
 __global__ void fv_rk2n_combine_kernel(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch,
@@ -662,6 +722,16 @@ hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int
 }
 
 template <typename T>
+hipError_t launch_fv_stage_bc_avg(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, const T *ring,
+                                  int nx, int ny, int pitch, int mask, double amp, double period, double msl,
+                                  DevScalars *sc, hipStream_t s, T *C, const T *C0)
+{
+    hipLaunchKernelGGL(fv_stage_bc_avg_kernel<T>, dim3(1), dim3(1024), 0, s, S, UH, VH, S0, UH0, VH0, ring, nx, ny,
+                       pitch, mask, amp, period, msl, sc, C, C0);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_fv_rk2n_combine(T *S, T *UH, T *VH, T *S0, T *UH0, T *VH0, int nx, int ny, int pitch, int part,
                                   DevScalars *sc, hipStream_t s, T *C, T *C0)
 {
@@ -686,5 +756,11 @@ template hipError_t launch_fv_stage_bc<double>(double *, double *, double *, int
                                                double, const DevScalars *, hipStream_t, double *);
 template hipError_t launch_fv_stage_bc<float>(float *, float *, float *, int, int, int, int, double, double, double,
                                               const DevScalars *, hipStream_t, float *);
+template hipError_t launch_fv_stage_bc_avg<double>(double *, double *, double *, const double *, const double *,
+                                                   const double *, const double *, int, int, int, int, double, double,
+                                                   double, DevScalars *, hipStream_t, double *, const double *);
+template hipError_t launch_fv_stage_bc_avg<float>(float *, float *, float *, const float *, const float *,
+                                                  const float *, const float *, int, int, int, int, double, double,
+                                                  double, DevScalars *, hipStream_t, float *, const float *);
 
 }  // namespace swh

@@ -345,6 +345,9 @@ struct FVArgs {
     const T *C0 = nullptr;
     T *Cw = nullptr;
     const T *Cq = nullptr, *avgC = nullptr;
+    // with avgS and a stage boundary after the update: the pre-average UH of columns 1
+    // and nx-2 ([2][ny]) then VH of rows 1 and ny-2 ([2][nx]), which the boundary reads
+    T *ring = nullptr;
 };
 
 template <typename T>
@@ -361,6 +364,15 @@ hipError_t launch_fv_average(T *S, T *UH, T *VH, const T *S0, const T *UH0, cons
 template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
                               double msl, const DevScalars *sc, hipStream_t s, T *C = nullptr);
+// The stage boundary of rk2's second update when the average with backup_U (S0, UH0,
+// VH0, C0) is folded into that update: applies the boundary rules in the order of
+// launch_fv_stage_bc to the pre-average values (interior neighbours from the ring the
+// update kept) and stores the boundary cells' averages, as the separate average pass
+// would leave them.
+template <typename T>
+hipError_t launch_fv_stage_bc_avg(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, const T *ring,
+                                  int nx, int ny, int pitch, int mask, double amp, double period, double msl,
+                                  DevScalars *sc, hipStream_t s, T *C, const T *C0);
 // number of blocks of the flux pass (size of FVArgs::bmax)
 int fv_flux_blocks(int nx, int ny);
 
