@@ -364,7 +364,7 @@ def nested_leg(prec, steps, warmup):
                      [td["send"] for td in tables["domains"]], [td["recv"] for td in tables["domains"]], prio)
     md.make_initial_conditions_consistent()
     dt = tables["dt"]
-    md.evolve_steps(dt, warmup)
+    md.evolve_steps(dt, max(warmup, 1))
     md.synchronize()
     v0 = md.volumes()
     t0 = time.perf_counter()
@@ -372,9 +372,19 @@ def nested_leg(prec, steps, warmup):
     md.synchronize()
     wall = time.perf_counter() - t0
     v1 = md.volumes()
+    # the same global steps replayed from the captured graphs (swh_md_graph_launch)
+    md.graph_capture(dt)
+    md.graph_launch(2)
+    md.synchronize()
+    t0 = time.perf_counter()
+    md.graph_launch(steps)
+    md.synchronize()
+    gwall = time.perf_counter() - t0
     out = dict(gcell_per_s=round(cells * steps / wall / 1e9, 3), ms_per_global_step=round(1e3 * wall / steps, 3),
                simulated_s_per_wall_s=round(dt * steps / wall, 1), cell_updates_per_global_step=cells,
-               relative_volume_change=float(abs(v1.sum() - v0.sum()) / v0.sum()))
+               relative_volume_change=float(abs(v1.sum() - v0.sum()) / v0.sum()),
+               graph_gcell_per_s=round(cells * steps / gwall / 1e9, 3),
+               graph_ms_per_global_step=round(1e3 * gwall / steps, 3))
     md.close()
     for d in doms:
         d.close()
@@ -404,7 +414,7 @@ def nested_main(args):
         "data": "synthetic",
         "config": {"workload": "three-level nested multidomain: 'linear' parent 3000x1800 at 2 arcmin, "
                                "'leapfrog_linear_plus_nonlinear_friction' child at 1/3, 'rk2' grandchild at 1/15 "
-                               "(shore inside), FP32 + Kahan-compensated stage",
+                               "(in water on the shelf), FP32 + Kahan-compensated stage",
                    "dt_s": tables["dt"], "timestepping_refinement_factor": tables["trf"],
                    "grids": [[td["dims"]["nx"], td["dims"]["ny"]] for td in tables["domains"]]},
         "nested": res,
@@ -429,6 +439,8 @@ def main():
                          "paths: only the single-grid legs (rk2 beach, 2048^2 linear, nonlinear, adjoint)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the rk2 beach and single 2048^2 grid throughput legs")
+    ap.add_argument("--no-nested", action="store_true",
+                    help="skip the three-level nested multidomain leg of the default run")
     ap.add_argument("--valu-probe", action="store_true", help="after the timed legs, measure the VALU FMA issue rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed legs, run a plain device copy of MIB MiB (for calibrating "
@@ -442,7 +454,12 @@ def main():
     if args.workload == "nested":
         return nested_main(args)
     if args.workload == "paths":
-        print(json.dumps(secondary_legs(args.steps)))
+        out = secondary_legs(args.steps)
+        if args.copy_probe:
+            copy_probe(args.copy_probe)
+        if args.valu_probe:
+            valu_probe()
+        print(json.dumps(out))
         return
 
     from ptha_amd import configs as C
@@ -534,6 +551,11 @@ def main():
     if not args.no_secondary:
         log("secondary legs: rk2 beach, single 2048^2 grid")
         secondary = secondary_legs(50)
+    nested = None
+    if not args.no_nested:
+        log("nested multidomain leg (eager steps and graph replays)")
+        nested = {prec: nested_leg(prec, 20, 2) for prec in ("fp32k", "fp64")}
+        log("nested", nested)
 
     cpu = None
     if not args.no_cpu:
@@ -574,6 +596,8 @@ def main():
                        "fp32k_over_fp64": round(r32["gcell"] / r64["gcell"], 3)}
     if secondary:
         out["secondary"] = secondary
+    if nested:
+        out["nested"] = nested
     if full:
         out["superpose"] = superpose
         out["unit_sources_per_hr"] = round(full["unit_sources_per_hr"], 1)

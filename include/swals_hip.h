@@ -468,6 +468,21 @@ int swh_md_make_initial_conditions_consistent(swh_md *md);
 /* md%evolve_one_step(dt) (multidomain_mod.f90:1093-1212). */
 int swh_md_evolve_one_step(swh_md *md, double dt);
 int swh_md_evolve_steps(swh_md *md, double dt, int32_t nsteps);
+/* Graph capture of the global step (no counterpart in the reference; the device form of
+ * repeating md%evolve_one_step(dt), multidomain_mod.f90:1093-1212): records one global
+ * step -- every domain's sub-steps and the exchange, kernels only -- as a HIP graph (two
+ * when a domain takes an odd number of sub-steps, so that its buffers alternate), without
+ * taking it. swh_md_graph_launch then replays nsteps global steps, each equal bit for bit
+ * to swh_md_evolve_one_step(dt), and keeps the host-side times and step counts as eager
+ * steps would; eager steps may be mixed in between. The kernels' time arguments come
+ * from the device. SWH_ESTATE where a domain's step depends on host-side time or step
+ * counts: 'leapfrog_nonlinear', dispersion, max_U every n > 1 steps or before the first
+ * step (take one eager step first), and on leapfrog domains arrival-time / time-of-max
+ * grids, forcing patches and boundaries with a stage function. Recapture after changing
+ * any domain setting or dt. */
+int swh_md_graph_capture(swh_md *md, double dt);
+int swh_md_graph_launch(swh_md *md, int32_t nsteps);
+int swh_md_graph_destroy(swh_md *md);
 /* md%get_flow_volume per domain: volume_in_priority_domain (domain_mod.f90:1851-1888). */
 int swh_md_volumes(swh_md *md, double *out);
 /* print_multidomain's statistics (multidomain_mod.f90:2301-2447): per_domain[k] (n entries,

@@ -144,6 +144,9 @@ SYMBOLS = [
     ("swh_md_make_initial_conditions_consistent", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_md_evolve_one_step", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("swh_md_evolve_steps", ctypes.c_int, [ctypes.c_void_p, _D, _I32]),
+    ("swh_md_graph_capture", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("swh_md_graph_launch", ctypes.c_int, [ctypes.c_void_p, _I32]),
+    ("swh_md_graph_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_md_volumes", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_md_download_recv_buffer", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _PD]),
     ("swh_md_statistics", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DomainStats), ctypes.POINTER(DomainStats),

@@ -198,9 +198,11 @@ int leapfrog_step(swh_domain *h, double dt)
     // the momentum half-step, and change UH/VH on the edges that the momentum half-step reads
     const bool flather = h->bc_mask && (h->bc_kind == 1 || h->bc_kind == 2);
     // boundary fluxes read the pre-momentum UH/VH (include.f90:126-165)
+    // (under graph capture the kernel forms the end time from the device time: NaN)
+    const double t_after_arg = h->graph_time ? __builtin_nan("") : t_after;
     if (!flather)
         SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                         h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+                                         h->d_dle, h->d_dbe, dt, t_after_arg, h->sc, h->stream));
     LFArgs<T> a;
     a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
     a.S = (const T *)h->S[c]; a.C = (const T *)h->C[c]; a.UH = (const T *)h->UH[c]; a.VH = (const T *)h->VH[c];
@@ -251,7 +253,7 @@ int leapfrog_step(swh_domain *h, double dt)
             SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
                                          (T)frac, h->stream));
         SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                         h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+                                         h->d_dle, h->d_dbe, dt, t_after_arg, h->sc, h->stream));
         a.S = (const T *)h->S[o];
         a.C = (const T *)h->C[o];
         a.stage_given = 1;
@@ -667,6 +669,42 @@ int evolve(swh_domain *h, double dt)
 namespace swh {
 
 int domain_evolve(swh_domain *h, double dt) { return evolve(h, dt); }
+
+std::string domain_graph_unsupported(const swh_domain *h)
+{
+    if (h->method == SWH_METHOD_LEAPFROG_NONLINEAR) return "'leapfrog_nonlinear'";
+    if (h->disp_on) return "dispersion (host-side extrapolation bookkeeping)";
+    if (h->maxq_mask && h->maxq_freq != 1) return "max_U updated every n > 1 steps";
+    if (h->maxq_mask && h->nsteps == 0) return "max_U before the first step (take one eager step first)";
+    if (h->leapfrog()) {
+        if (h->maxq_mask & (SWH_MAXQ_ARRIVAL_TIME | SWH_MAXQ_TIME_OF_MAX_STAGE))
+            return "leapfrog arrival time / time of max stage (host time)";
+        if (h->f_on) return "leapfrog forcing patch (host time)";
+        if (h->bc_mask && h->bc_use_fun) return "leapfrog boundary with a stage function (host time)";
+    }
+    return "";
+}
+
+int domain_prepare_dt(swh_domain *h, double dt)
+{
+    if (!h->leapfrog() || h->rc_dt == dt) return SWH_OK;
+    return h->prec == SWH_PREC_FP64 ? build_rc<double>(h, dt) : build_rc<float>(h, dt);
+}
+
+// The host-side effects of evolve(h, dt) (leapfrog_step / fv_step), without launches.
+void domain_host_advance(swh_domain *h, double dt)
+{
+    if (h->leapfrog()) {
+        const double time0 = h->time;
+        const double t_half = h->time + dt * 0.5;
+        const double t_after = t_half + 0.5 * dt;
+        h->time = t_after;
+        h->dt_last_update = dt;
+        h->evolve_step_dt = h->time - time0;
+    }
+    h->cur = 1 - h->cur;
+    h->nsteps++;
+}
 
 int domain_refresh_elevation(swh_domain *h)
 {

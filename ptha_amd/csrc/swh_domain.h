@@ -55,6 +55,9 @@ struct swh_domain {
     // host-side scalars (leapfrog keeps time on the host; rk2 on the device)
     double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;
     long long nsteps = 0;
+    // set while a multidomain step is captured into a graph: the leapfrog boundary-flux
+    // kernel forms the step's end time from the device time instead of the host's
+    int graph_time = 0;
     int fw_ready = 0;
     // forcing patch (forcing_patch_type, forcing_mod.f90:24-56): work planes over the
     // whole grid, zero outside the patch; bounds 0-based inclusive
@@ -118,4 +121,11 @@ int domain_statistics_launch(swh_domain *h, const int *pmask, double *rows, doub
 int domain_stability_launch(swh_domain *h, unsigned long long *count, hipStream_t stream);
 // Allocates h->d_stats on first use.
 int domain_stats_buffer(swh_domain *h);
+// Graph capture of multidomain steps: whether a step of h can be captured (its kernels'
+// arguments do not depend on the host time or step count; empty string: yes), the
+// per-dt set-up a capture must not contain (leapfrog row coefficients), and the host
+// bookkeeping of one domain_evolve without its launches (for graph replays).
+std::string domain_graph_unsupported(const swh_domain *h);
+int domain_prepare_dt(swh_domain *h, double dt);
+void domain_host_advance(swh_domain *h, double dt);
 }  // namespace swh

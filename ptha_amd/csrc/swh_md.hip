@@ -78,6 +78,15 @@ struct swh_md {
     int n_reint = 0, reint_blocks = 0, reint_max_len = 0;
     double *d_copy = nullptr;
     double *d_vol = nullptr;
+    // captured global steps (swh_md_graph_capture): one graph, or two when the domains'
+    // ping-pong buffers alternate between global steps; the device copy of the
+    // multidomain time the graphs read and advance; graph_dt the captured step
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int n_graphs = 0;
+    std::vector<int> cap_cur;                 // each domain's current buffer when captured
+    double graph_dt = 0.0;
+    bool capturing = false;
+    double *d_time = nullptr;
 };
 
 namespace {
@@ -294,31 +303,149 @@ int fork_domains(swh_md *md)
 }
 
 __global__ void md_set_time_kernel(DevScalars *sc, double t) { sc->time = t; }
+// under graph replay: the end time from the device's multidomain time (same operation
+// as the host's md->time + dt)
+__global__ void md_set_time_dev_kernel(DevScalars *sc, const double *md_time, double dt) { sc->time = *md_time + dt; }
+__global__ void md_advance_time_kernel(double *md_time, double dt) { *md_time = *md_time + dt; }
 
+// One global step (multidomain_mod.f90:1093-1212: each domain's sub-steps, then the
+// halo exchange with flux correction). While capturing, the domain streams are forked
+// from the multidomain stream at the start (so that their work joins the capture) and
+// the times come from the device.
 int md_step(swh_md *md, double dt)
 {
     const double tend = md->time + dt;
+    int rc;
+    if (md->capturing && (rc = fork_domains(md))) return rc;
     for (int k = 0; k < md->nd; k++) {
         swh_domain *h = md->dom[k];
         SWH_HIP(launch_nest_zero<double>(md->d_int[k], md->n_int[k], h->stream));
         const int nt = md->trf[k];
         const double dt_local = dt / (1.0 * nt);
         for (int i = 0; i < nt; i++) {
-            const int rc = swh::domain_evolve(h, dt_local);
+            rc = swh::domain_evolve(h, dt_local);
             if (rc) return rc;
         }
         // numerically identical time for all domains (multidomain_mod.f90:1174-1175)
         h->time = tend;
-        hipLaunchKernelGGL(md_set_time_kernel, dim3(1), dim3(1), 0, h->stream, h->sc, tend);
+        if (md->capturing)
+            hipLaunchKernelGGL(md_set_time_dev_kernel, dim3(1), dim3(1), 0, h->stream, h->sc, md->d_time, dt);
+        else
+            hipLaunchKernelGGL(md_set_time_kernel, dim3(1), dim3(1), 0, h->stream, h->sc, tend);
         SWH_HIP(hipGetLastError());
     }
-    int rc = join_domains(md);
+    rc = join_domains(md);
     if (rc) return rc;
     rc = exchange_any(md, true);
     if (rc) return rc;
-    rc = fork_domains(md);
-    if (rc) return rc;
+    if (md->capturing) {
+        hipLaunchKernelGGL(md_advance_time_kernel, dim3(1), dim3(1), 0, md->stream, md->d_time, dt);
+        SWH_HIP(hipGetLastError());
+    } else if ((rc = fork_domains(md))) {
+        return rc;
+    }
     md->time = tend;
+    return SWH_OK;
+}
+
+void graph_release(swh_md *md)
+{
+    for (auto &g : md->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    md->n_graphs = 0;
+    md->cap_cur.clear();
+}
+
+// The captured graph that continues from the domains' current buffers: 0 (as at the
+// capture), 1 (after an odd number of global steps, when some domain takes an odd
+// number of sub-steps), -1 (neither: the buffers were changed otherwise).
+int graph_for_state(const swh_md *md)
+{
+    for (int g = 0; g < md->n_graphs; g++) {
+        bool ok = true;
+        for (int k = 0; k < md->nd && ok; k++) {
+            const int want = (g == 1 && md->trf[k] % 2 == 1) ? 1 - md->cap_cur[k] : md->cap_cur[k];
+            ok = md->dom[k]->cur == want;
+        }
+        if (ok) return g;
+    }
+    return -1;
+}
+
+// Host bookkeeping of one replayed global step (md_step's, without launches).
+void md_host_advance(swh_md *md, double dt)
+{
+    const double tend = md->time + dt;
+    for (int k = 0; k < md->nd; k++) {
+        swh_domain *h = md->dom[k];
+        const double dt_local = dt / (1.0 * md->trf[k]);
+        for (int i = 0; i < md->trf[k]; i++) swh::domain_host_advance(h, dt_local);
+        h->time = tend;
+    }
+    md->time = tend;
+}
+
+int graph_capture(swh_md *md, double dt)
+{
+    graph_release(md);
+    for (int k = 0; k < md->nd; k++) {
+        const std::string why = swh::domain_graph_unsupported(md->dom[k]);
+        if (!why.empty())
+            return md_fail(SWH_ESTATE, "swh_md_graph_capture: domain " + std::to_string(k + 1) + ": " + why);
+    }
+    // per-dt set-up outside the capture
+    for (int k = 0; k < md->nd; k++) {
+        const int rc = swh::domain_prepare_dt(md->dom[k], dt / (1.0 * md->trf[k]));
+        if (rc) return rc;
+    }
+    for (auto h : md->dom) SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipStreamSynchronize(md->stream));
+    if (!md->d_time) SWH_HIP(hipMalloc((void **)&md->d_time, sizeof(double)));
+    SWH_HIP(hipMemcpy(md->d_time, &md->time, sizeof(double), hipMemcpyHostToDevice));
+    // a second graph when the buffers alternate (an odd number of sub-steps somewhere)
+    bool odd = false;
+    for (int k = 0; k < md->nd; k++) odd |= (md->trf[k] % 2) == 1;
+    const int ng = odd ? 2 : 1;
+    // host state before the capture: the captured steps are not taken
+    struct Saved { double time, esd, dtl; long long nsteps; int cur; };
+    std::vector<Saved> saved;
+    for (auto h : md->dom) saved.push_back({h->time, h->evolve_step_dt, h->dt_last_update, h->nsteps, h->cur});
+    const double md_time = md->time;
+    int rc = SWH_OK;
+    for (int g = 0; g < ng && rc == SWH_OK; g++) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = md_fail(SWH_EHIP, "swh_md_graph_capture: hipStreamBeginCapture failed");
+            break;
+        }
+        md->capturing = true;
+        for (auto h : md->dom) h->graph_time = 1;
+        rc = md_step(md, dt);
+        for (auto h : md->dom) h->graph_time = 0;
+        md->capturing = false;
+        const hipError_t e = hipStreamEndCapture(md->stream, &graph);
+        if (rc == SWH_OK && e != hipSuccess) rc = md_fail(SWH_EHIP, "swh_md_graph_capture: hipStreamEndCapture failed");
+        if (rc == SWH_OK && hipGraphInstantiate(&md->gexec[g], graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = md_fail(SWH_EHIP, "swh_md_graph_capture: hipGraphInstantiate failed");
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    for (int k = 0; k < md->nd; k++) {
+        swh_domain *h = md->dom[k];
+        h->time = saved[k].time; h->evolve_step_dt = saved[k].esd; h->dt_last_update = saved[k].dtl;
+        h->nsteps = saved[k].nsteps; h->cur = saved[k].cur;
+    }
+    md->time = md_time;
+    if (rc) {
+        graph_release(md);
+        return rc;
+    }
+    md->n_graphs = ng;
+    md->cap_cur.clear();
+    for (auto h : md->dom) md->cap_cur.push_back(h->cur);
+    md->graph_dt = dt;
     return SWH_OK;
 }
 
@@ -343,6 +470,8 @@ void md_free(swh_md *md)
     F(md->d_reint);
     F(md->d_copy);
     F(md->d_vol);
+    F(md->d_time);
+    graph_release(md);
     for (auto e : md->ev_dom)
         if (e) (void)hipEventDestroy(e);
     if (md->ev_x) (void)hipEventDestroy(md->ev_x);
@@ -799,6 +928,43 @@ int swh_md_evolve_one_step(swh_md *md, double dt)
     if (!md || !md->finalized) return md_fail(SWH_ESTATE, "swh_md_evolve_one_step: not finalized");
     if (!(dt > 0.0)) return md_fail(SWH_EINVAL, "swh_md_evolve_one_step: dt must be positive");
     return md_step(md, dt);
+}
+
+int swh_md_graph_capture(swh_md *md, double dt)
+{
+    if (!md || !md->finalized) return md_fail(SWH_ESTATE, "swh_md_graph_capture: not finalized");
+    if (!(dt > 0.0)) return md_fail(SWH_EINVAL, "swh_md_graph_capture: dt must be positive");
+    return graph_capture(md, dt);
+}
+
+int swh_md_graph_launch(swh_md *md, int32_t nsteps)
+{
+    if (!md || !md->finalized) return md_fail(SWH_ESTATE, "swh_md_graph_launch: not finalized");
+    if (md->n_graphs == 0) return md_fail(SWH_ESTATE, "swh_md_graph_launch: no captured step (swh_md_graph_capture)");
+    if (nsteps < 0) return md_fail(SWH_EINVAL, "swh_md_graph_launch: nsteps must be >= 0");
+    if (nsteps == 0) return SWH_OK;
+    // the graphs read the device time: the host's, as eager steps left it
+    int rc = join_domains(md);
+    if (rc) return rc;
+    SWH_HIP(hipMemcpyAsync(md->d_time, &md->time, sizeof(double), hipMemcpyHostToDevice, md->stream));
+    SWH_HIP(hipStreamSynchronize(md->stream));
+    for (int32_t n = 0; n < nsteps; n++) {
+        const int g = graph_for_state(md);
+        if (g < 0) {
+            (void)fork_domains(md);
+            return md_fail(SWH_ESTATE, "swh_md_graph_launch: the domains' buffers no longer match the capture");
+        }
+        SWH_HIP(hipGraphLaunch(md->gexec[g], md->stream));
+        md_host_advance(md, md->graph_dt);
+    }
+    return fork_domains(md);
+}
+
+int swh_md_graph_destroy(swh_md *md)
+{
+    if (!md) return md_fail(SWH_EINVAL, "swh_md_graph_destroy: null argument");
+    graph_release(md);
+    return SWH_OK;
 }
 
 int swh_md_evolve_steps(swh_md *md, double dt, int32_t nsteps)

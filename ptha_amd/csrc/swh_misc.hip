@@ -75,6 +75,9 @@ __global__ __launch_bounds__(kBfThreads) void bflux_leapfrog_kernel(const T *__r
     if (threadIdx.x == 0) {
         m[0] = dbe[ny - n_ext]; m[1] = dle[nx - n_ext]; m[2] = dbe[n_ext]; m[3] = dle[n_ext];
         bf_time = sc->bf_time;
+        // NaN: the step's end time from the device time (graph replays), with the
+        // host's operations: (time + dt * 0.5) + 0.5 * dt
+        if (time_after != time_after) time_after = (sc->time + dt * 0.5) + 0.5 * dt;
     }
     const int g = threadIdx.x / Q, lane = threadIdx.x % Q;
     // side g: N and S rows of VH, E and W columns of UH; each thread adds every Q'th

@@ -122,6 +122,17 @@ class MultiDomain:
     def evolve_steps(self, dt, nsteps):
         B.check(self._lib.swh_md_evolve_steps(self._h, float(dt), int(nsteps)), "swh_md_evolve_steps")
 
+    def graph_capture(self, dt):
+        """Record one global step of dt as a HIP graph (not taken); see graph_launch."""
+        B.check(self._lib.swh_md_graph_capture(self._h, float(dt)), "swh_md_graph_capture")
+
+    def graph_launch(self, nsteps):
+        """Replay the captured global step nsteps times (each equal to evolve_one_step)."""
+        B.check(self._lib.swh_md_graph_launch(self._h, int(nsteps)), "swh_md_graph_launch")
+
+    def graph_destroy(self):
+        B.check(self._lib.swh_md_graph_destroy(self._h), "swh_md_graph_destroy")
+
     def synchronize(self):
         B.check(self._lib.swh_md_synchronize(self._h), "swh_md_synchronize")
 

@@ -79,3 +79,11 @@ def test_batch_rejects_llpnf_without_truely_linear_solver():
     assert not h.value
     n = ctypes.c_int64()
     assert lib.swh_batch_launch_counts(None, ctypes.byref(n), ctypes.byref(n), ctypes.byref(n)) == -1
+
+
+def test_md_graph_entry_points_check_arguments_without_device():
+    from ptha_amd import backend as B
+    lib = B.load()
+    assert lib.swh_md_graph_capture(None, 1.0) != 0
+    assert lib.swh_md_graph_launch(None, 1) != 0
+    assert lib.swh_md_graph_destroy(None) == -1

@@ -140,7 +140,9 @@ C3_BARS = {"gauge_stage_l2_rel": (1e-6, None), "volume_rel": (1e-12, 1e-9)}
 C3_LEAPFROG_BAR = 1e-5
 
 
-def _run_config3(precision, z, g, tables, nudge=0):
+def build_config3(precision, g, tables, nudge=0):
+    """The three-level nest of the committed tables (nested_c3_tables.npz) on the device:
+    (domains, multidomain) with the initial conditions made consistent."""
     from ptha_amd.domain import Domain
     from ptha_amd.multidomain import MultiDomain
     cfg = C.nested_c3(scale=tables["scale"])
@@ -163,6 +165,11 @@ def _run_config3(precision, z, g, tables, nudge=0):
     md = MultiDomain(doms, g["all_dx"], g["all_methods"], tables["trf"], [td["send"] for td in tables["domains"]],
                      [td["recv"] for td in tables["domains"]], prio)
     md.make_initial_conditions_consistent()
+    return doms, md
+
+
+def _run_config3(precision, z, g, tables, nudge=0):
+    doms, md = build_config3(precision, g, tables, nudge)
     v0 = md.volumes()
     every, nglobal, dt = int(z["every"]), int(z["nglobal"]), float(z["dt"])
     nsamp = len(z["series_0"])
