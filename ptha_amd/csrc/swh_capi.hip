@@ -484,6 +484,7 @@ FVArgs<T> fv_args(swh_domain *h, int dst, int src)
     a.spherical = h->spherical; a.coriolis = h->coriolis; a.n_ext = h->n_ext;
     a.sc = h->sc;
     a.skip_edges = h->disp_on;
+    a.flux_rows = h->rows_per_block;
     return a;
 }
 
@@ -802,6 +803,7 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     // (16-byte vector lanes; FP32 also keeps the next row's loads in flight)
     if (h->prec == SWH_PREC_FP64) { h->block_x = 128; h->rows_per_block = 16; h->variant = 1; }
     else { h->block_x = 64; h->rows_per_block = 32; h->variant = 3; }
+    if (fv) h->rows_per_block = 8;   // finite volume: rows per wavefront of the flux pass
     {
         // a single grid must still fill the GPU: shorten the row strips until the
         // launch has at least 4096 wavefronts (16 per CU)

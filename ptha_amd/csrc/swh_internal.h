@@ -348,6 +348,8 @@ struct FVArgs {
     // with avgS and a stage boundary after the update: the pre-average UH of columns 1
     // and nx-2 ([2][ny]) then VH of rows 1 and ny-2 ([2][nx]), which the boundary reads
     T *ring = nullptr;
+    // flux pass: rows per wavefront of the column-marching kernel (4..64)
+    int flux_rows = 8;
 };
 
 template <typename T>
