@@ -586,32 +586,40 @@ __global__ void fv_average_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
 }
 
 // ---- stage boundary on a co-located grid (boundary_mod.f90:78-164) ----
-// One block of 1024 threads: west/east columns first, then south/north rows, as in
-// the reference (so corner cells end with the south/north treatment).
+// The reference applies the west/east columns first, then the south/north rows, which
+// overwrite the corners. Every boundary cell at once here: a thread takes row j's west
+// then east cell, or column i's south then north cell (the same order within a line as
+// the reference, for grids of width 2), the column rule leaves the corners on an active
+// row line to the row rule, and the row rule takes VH = 0 for the cell next to it on an
+// active column line (what the column rule left there); no other cell that one rule
+// reads is written by the other.
 template <typename T>
-__global__ __launch_bounds__(1024) void fv_stage_bc_kernel(T *S, T *UH, T *VH, int nx, int ny, int pitch, int m,
-                                                           double amp, double period, double msl, const DevScalars *sc,
-                                                           T *C)
+__global__ __launch_bounds__(256) void fv_stage_bc_kernel(T *S, T *UH, T *VH, int nx, int ny, int pitch, int m,
+                                                          double amp, double period, double msl, const DevScalars *sc,
+                                                          T *C)
 {
     const double two_pi = 2.0 * acos(-1.0);
     const double stage_d = msl + amp * swhm::sin_acc(two_pi * sc->time / period);
     const T stage = (T)stage_d;
     // FP32 state: the compensation that makes S - C the FP64 boundary stage
     const T cstage = (T)((double)stage - stage_d);
-    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ny) {
+        const int j = t;
+        if ((j == 0 && (m & 4)) || (j == ny - 1 && (m & 1))) return;
         const long long o = (long long)j * pitch;
         if (m & 8) { S[o] = stage; if (C) C[o] = cstage; UH[o] = UH[o + 1]; VH[o] = T(0); }
         if (m & 2) {
             const long long e = o + nx - 1;
             S[e] = stage; if (C) C[e] = cstage; UH[e] = UH[e - 1]; VH[e] = T(0);
         }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-        if (m & 4) { S[i] = stage; if (C) C[i] = cstage; VH[i] = VH[pitch + i]; UH[i] = T(0); }
+    } else if (t < ny + nx) {
+        const int i = t - ny;
+        const bool col = ((m & 8) && i == 0) || ((m & 2) && i == nx - 1);
+        if (m & 4) { S[i] = stage; if (C) C[i] = cstage; VH[i] = col ? T(0) : VH[pitch + i]; UH[i] = T(0); }
         if (m & 1) {
             const long long o = (long long)(ny - 1) * pitch + i;
-            S[o] = stage; if (C) C[o] = cstage; VH[o] = VH[o - pitch]; UH[o] = T(0);
+            S[o] = stage; if (C) C[o] = cstage; VH[o] = col ? T(0) : VH[o - pitch]; UH[o] = T(0);
         }
     }
 }
@@ -636,10 +644,10 @@ __device__ void fv_finish(DevScalars &sc, int mode)
 
 // Stage boundary with rk2's average folded into the preceding update (the separate
 // order: update, boundary, average). The boundary cells' pre-average values follow the
-// rules of fv_stage_bc_kernel -- west/east columns first, then south/north rows,
-// which overwrite the corners -- with the updated interior neighbours taken from the
-// ring (the cells (i, 1) of an active column line hold that column's value, VH = 0);
-// each is then averaged with backup_U exactly as fv_average_kernel does.
+// rules of fv_stage_bc_kernel, with the updated interior neighbours taken from the ring
+// (the cells (i, 1) of an active column line hold that column's value, VH = 0); each is
+// then averaged with backup_U exactly as fv_average_kernel does. The block that finishes
+// last, when every block has read the stage's time, runs the step's closing bookkeeping.
 template <typename T>
 __device__ __forceinline__ void bc_avg_store(T *S, T *UH, T *VH, const T *S0, const T *UH0, const T *VH0, T *C,
                                              const T *C0, long long o, T stage, T cstage, T uh, T vh)
@@ -657,32 +665,44 @@ __device__ __forceinline__ void bc_avg_store(T *S, T *UH, T *VH, const T *S0, co
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void fv_stage_bc_avg_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
-                                                               const T *VH0, const T *ring, int nx, int ny, int pitch,
-                                                               int m, double amp, double period, double msl,
-                                                               DevScalars *sc, T *C, const T *C0)
+__global__ __launch_bounds__(256) void fv_stage_bc_avg_kernel(T *S, T *UH, T *VH, const T *S0, const T *UH0,
+                                                              const T *VH0, const T *ring, int nx, int ny, int pitch,
+                                                              int m, double amp, double period, double msl,
+                                                              DevScalars *sc, T *C, const T *C0)
 {
     const double two_pi = 2.0 * acos(-1.0);
     const double stage_d = msl + amp * swhm::sin_acc(two_pi * sc->time / period);
     const T stage = (T)stage_d;
     const T cstage = C ? (T)((double)stage - stage_d) : T(0);
-    // every thread has read the stage's time; then the step's closing bookkeeping
-    __syncthreads();
-    if (threadIdx.x == 0) fv_finish(*sc, kFinishRk2);
     const T *rW = ring, *rE = ring + ny, *rS = ring + 2 * ny, *rN = ring + 2 * ny + nx;
-    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
-        const long long o = (long long)j * pitch;
-        if (m & 8) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, o, stage, cstage, rW[j], T(0));
-        if (m & 2) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, o + nx - 1, stage, cstage, rE[j], T(0));
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ny) {
+        // row j's west and east cells; corners on an active south/north line are the
+        // row rule's
+        const int j = t;
+        if (!((j == 0 && (m & 4)) || (j == ny - 1 && (m & 1)))) {
+            const long long o = (long long)j * pitch;
+            if (m & 8) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, o, stage, cstage, rW[j], T(0));
+            if (m & 2) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, o + nx - 1, stage, cstage, rE[j], T(0));
+        }
+    } else if (t < ny + nx) {
+        const int i = t - ny;
         // VH of the row next to the line: 0 where an active column line holds that cell
         const bool col = ((m & 8) && i == 0) || ((m & 2) && i == nx - 1);
         if (m & 4) bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, i, stage, cstage, T(0), col ? T(0) : rS[i]);
         if (m & 1)
             bc_avg_store<T>(S, UH, VH, S0, UH0, VH0, C, C0, (long long)(ny - 1) * pitch + i, stage, cstage, T(0),
                             col ? T(0) : rN[i]);
+    }
+    __shared__ bool last;
+    __syncthreads();   // this block's threads have read the time
+    if (threadIdx.x == 0) {
+        const unsigned int done = atomicAdd(&sc->bc_done, 1u);
+        last = done == gridDim.x - 1;
+        if (last) {
+            sc->bc_done = 0;
+            fv_finish(*sc, kFinishRk2);
+        }
     }
 }
 
@@ -749,8 +769,8 @@ template <typename T>
 hipError_t launch_fv_stage_bc(T *S, T *UH, T *VH, int nx, int ny, int pitch, int mask, double amp, double period,
                               double msl, const DevScalars *sc, hipStream_t s, T *C)
 {
-    hipLaunchKernelGGL(fv_stage_bc_kernel<T>, dim3(1), dim3(1024), 0, s, S, UH, VH, nx, ny, pitch, mask, amp, period,
-                       msl, sc, C);
+    hipLaunchKernelGGL(fv_stage_bc_kernel<T>, dim3((nx + ny + 255) / 256), dim3(256), 0, s, S, UH, VH, nx, ny, pitch,
+                       mask, amp, period, msl, sc, C);
     return hipGetLastError();
 }
 
@@ -759,8 +779,8 @@ hipError_t launch_fv_stage_bc_avg(T *S, T *UH, T *VH, const T *S0, const T *UH0,
                                   int nx, int ny, int pitch, int mask, double amp, double period, double msl,
                                   DevScalars *sc, hipStream_t s, T *C, const T *C0)
 {
-    hipLaunchKernelGGL(fv_stage_bc_avg_kernel<T>, dim3(1), dim3(1024), 0, s, S, UH, VH, S0, UH0, VH0, ring, nx, ny,
-                       pitch, mask, amp, period, msl, sc, C, C0);
+    hipLaunchKernelGGL(fv_stage_bc_avg_kernel<T>, dim3((nx + ny + 255) / 256), dim3(256), 0, s, S, UH, VH, S0, UH0,
+                       VH0, ring, nx, ny, pitch, mask, amp, period, msl, sc, C, C0);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ struct DevScalars {
     double time0;              // time at the start of the step
     double stage_dt;           // step applied by the current update_U (dt_last_update)
     double evolve_dt;          // evolve_step_dt = time - time0
+    unsigned int bc_done;      // blocks of the averaging stage boundary done (0 between launches)
 };
 
 // Bookkeeping of one finite-volume Euler stage (fv_scalar_kernel), after
