@@ -1663,6 +1663,7 @@ struct swh_batch {
     int ng = 0, gmax = 0;
     std::vector<long long> sched;  // steps (ascending) at which the gauges are sampled
     long long *d_gidx = nullptr;
+    int *d_gij = nullptr;          // the gauges' (i, j), 0-based, for the fused kernel's sampling
     double *d_gval = nullptr;      // [slot][sample][3][ng]
     int block_x = 256, rows_per_block = 32, variant = 2, src_fast = 0;
     int two_step = 0;              // two fused steps per launch where possible
@@ -1797,9 +1798,12 @@ int batch_launch2(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     return SWH_OK;
 }
 
-// One launch advancing slots [first, first + n) by ns (3 or 4) steps (reads buffer c).
+// One launch advancing slots [first, first + n) by ns (3 to 6) steps (reads buffer c);
+// with gauges_inside, the gauge samples due after any of those steps are taken by the
+// launch itself.
 template <typename T, bool KAHAN>
-int batch_launchn(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, int c, int ns)
+int batch_launchn(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, int c, int ns,
+                  bool gauges_inside = false)
 {
     swh_domain *g = b->g;
     const int o = 1 - c;
@@ -1810,8 +1814,16 @@ int batch_launchn(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     a.So = (T *)b->S[o]; a.Co = (T *)b->C[o]; a.UHo = (T *)b->UH[o]; a.VHo = (T *)b->VH[o];
     a.lf2_order = b->xcd_order;
     a.lf2_group = b->src_group;
-    SWH_HIP(launch_leapfrogn<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n, b->block_x,
-                                       g->stream, (b->variant & 2) ? 1 : 0, ns));
+    if (gauges_inside) {
+        a.g_n = b->ng; a.g_max = b->gmax; a.g_ij = b->d_gij; a.g_val = b->d_gval;
+        for (int q = 0; q < 6; q++)
+            a.g_slot[q] = (q < ns && sample_due(b, b->step + q + 1)) ? samples_upto(b, b->step + q + 1) - 1 : -1;
+    }
+    const hipError_t e = launch_leapfrogn<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n,
+                                                    b->block_x, g->stream, (b->variant & 2) ? 1 : 0, ns);
+    a.g_n = 0;
+    for (int q = 0; q < 6; q++) a.g_slot[q] = -1;
+    SWH_HIP(e);
     b->n_launch++; b->src_launch += n; b->src_steps += (long long)ns * n;
     a.src_base = 0;
     a.lf2_order = 0;
@@ -1838,7 +1850,23 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         const bool sample_next = sample_due(b, b->step + 1);
         int n_last;   // sources that took the last step of this round
         const int n2 = can2 ? batch_active(b, b->step + 1) : 0;   // sources with two or more steps left
-        // the deepest round (3 or 4 steps) that no gauge sample falls inside
+        // with 64 gauges or fewer, a round of the full depth that every running source
+        // takes to its end, the gauge samples inside it written by the launch itself
+        int Dg = 0;
+        if (depth >= 3 && b->ng <= 64) {
+            Dg = std::min<long long>(depth, nsteps - k);
+            while (Dg >= 3 && batch_active(b, b->step + Dg - 1) < n_active) Dg--;
+        }
+        if (Dg >= 3) {
+            rc = batch_launchn<T, KAHAN>(b, a, fric_mode, 0, n_active, c, Dg, true);
+            if (rc) return rc;
+            b->step += Dg;
+            k += Dg;
+            b->cur = o;
+            for (int sl = batch_active(b, b->step); sl < n_active; sl++) b->slot_buf[sl] = o;
+            continue;
+        }
+        // otherwise the deepest round (3 to 6 steps) that no gauge sample falls inside
         int D = 0;
         if (depth >= 3 && k + 2 < nsteps && !sample_next && !sample_due(b, b->step + 2) &&
             batch_active(b, b->step + 2) > 0) {
@@ -2058,7 +2086,7 @@ int swh_batch_destroy(swh_batch *b)
     if (!b) return SWH_OK;
     if (b->g && b->g->stream) (void)hipStreamSynchronize(b->g->stream);
     for (int k = 0; k < 2; k++) { dfree(b->S[k]); dfree(b->UH[k]); dfree(b->VH[k]); dfree(b->C[k]); }
-    dfree(b->d_gidx); dfree(b->d_gval);
+    dfree(b->d_gidx); dfree(b->d_gval); dfree(b->d_gij);
     swh_domain_destroy(b->g);
     delete b;
     return SWH_OK;
@@ -2161,13 +2189,17 @@ int swh_batch_gauges_set_schedule(swh_batch *b, int32_t n, const int32_t *ij, in
         if (i < 0 || i >= g->nx || j < 0 || j >= g->ny) return fail(SWH_EINVAL, "gauge cell outside the domain");
         idx[q] = (long long)j * g->pitch + i;
     }
+    std::vector<int> gij(2 * (size_t)n);
+    for (int q = 0; q < n; q++) { gij[2 * q] = ij[2 * q] - 1; gij[2 * q + 1] = ij[2 * q + 1] - 1; }
     SWH_HIP(hipStreamSynchronize(g->stream));
-    dfree(b->d_gidx); dfree(b->d_gval);
-    b->d_gidx = nullptr; b->d_gval = nullptr;
+    dfree(b->d_gidx); dfree(b->d_gval); dfree(b->d_gij);
+    b->d_gidx = nullptr; b->d_gval = nullptr; b->d_gij = nullptr;
     int rc = dmalloc((void **)&b->d_gidx, sizeof(long long) * (n > 0 ? n : 1));
+    if (!rc) rc = dmalloc((void **)&b->d_gij, sizeof(int) * 2 * (n > 0 ? n : 1));
     if (!rc) rc = dmalloc((void **)&b->d_gval, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1) * max_samples * b->nsrc);
     if (rc) return rc;
     if (n > 0) SWH_HIP(hipMemcpy(b->d_gidx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    if (n > 0) SWH_HIP(hipMemcpy(b->d_gij, gij.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice));
     b->ng = n; b->gmax = max_samples;
     b->sched.assign(sample_steps, sample_steps + nsamples);
     return SWH_OK;

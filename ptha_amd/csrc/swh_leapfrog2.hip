@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
 // of the wavefront to the west (UH(i-1)) and one to the east (stage(i+1), VH(i+1)), so a
 // wavefront owns 65 - 2 NS of its 64 columns; each strip recomputes NS rows below and
 // NS - 1 above it.
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool PF, int NS>
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool PF, int NS, bool GS = false>
 __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
 {
     constexpr int OWN = 65 - 2 * NS;
@@ -358,6 +358,16 @@ __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
     const bool col_int = col > 0 && col < a.nx - 1;      // stage: interior column
     const bool col_mom = col >= 0 && col < a.nx - 1;     // momentum: UH/VH of this column update
     const int cf = min(max(col, 0), a.nx - 1);           // friction-work column
+    // gauges whose cell this wavefront owns (its columns and rows j0..j1-1): bit g
+    unsigned long long gmask = 0;
+    if (GS && a.g_n) {
+        bool in = false;
+        if (lane < a.g_n) {
+            const int gi = a.g_ij[2 * lane], gj = a.g_ij[2 * lane + 1];
+            in = gi >= wg * OWN && gi < min(wg * OWN + OWN, a.nx) && gj >= j0 && gj < j1;
+        }
+        gmask = __ballot(in);
+    }
 
     // carried between rows, per level n (0 = the loaded state)
     T s_p[NS + 1], c_p[NS + 1];          // stage of level n at its stage row of the previous iteration
@@ -454,6 +464,20 @@ __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
                                                                 FRIC ? fwv_r[n - 1] : T(0), vh_p[n - 1], s_p[n],
                                                                 c_p[n], sn, cn, e_r[n], e_r[n - 1], in_uhb, in_uhb_l,
                                                                 in_uh, in_uh_l);
+            if (GS && gmask && a.g_slot[n - 1] >= 0) {
+                // a gauge sample falls after this level's step: its stage and UH at row r,
+                // its VH at row r-1 (each formed once per level by the lane owning it)
+                double *v = a.g_val + ((long long)(bz + a.src_base) * a.g_max + a.g_slot[n - 1]) * 3 * a.g_n;
+                for (unsigned long long mm = gmask; mm; mm &= mm - 1) {
+                    const int g = __builtin_ctzll(mm);
+                    const int gi = a.g_ij[2 * g], gj = a.g_ij[2 * g + 1];
+                    if (col == gi && r == gj) {
+                        v[g] = KAHAN ? (double)sn - (double)cn : (double)sn;
+                        v[a.g_n + g] = (double)uhn;
+                    }
+                    if (col == gi && r - 1 == gj) v[2 * a.g_n + g] = (double)vhn;
+                }
+            }
             const T uhn_sh = shr1(uhn);
             const T uhn_l = west ? uhn : uhn_sh;
             const T vhn_r = shl1(vhn);
@@ -689,7 +713,14 @@ static hipError_t launch_lfn(const LFArgs<T> &a, int nsrc, int block_x, hipStrea
     constexpr int OWN = 65 - 2 * NS;
     const int waves = (a.nx + OWN - 1) / OWN, wpb = block_x / 64;
     dim3 grid((waves + wpb - 1) / wpb, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
-    if (pf)
+    bool samples = false;
+    for (int q = 0; q < NS; q++) samples |= a.g_slot[q] >= 0;
+    if (a.g_n && samples)
+        // the variant that writes the gauge samples falling inside the launch (with the
+        // next row's loads in flight): used only by launches that hold a sample
+        hipLaunchKernelGGL((leapfrogn_kernel<T, KAHAN, COR, FRIC, TL, LINF, true, NS, true>), grid, dim3(block_x), 0, s,
+                           a);
+    else if (pf)
         hipLaunchKernelGGL((leapfrogn_kernel<T, KAHAN, COR, FRIC, TL, LINF, true, NS>), grid, dim3(block_x), 0, s, a);
     else
         hipLaunchKernelGGL((leapfrogn_kernel<T, KAHAN, COR, FRIC, TL, LINF, false, NS>), grid, dim3(block_x), 0, s, a);

@@ -57,6 +57,14 @@ module swals_hip_mod
         integer(c_int64_t) :: nsteps_advanced
     end type
 
+    ! Mirrors struct swh_domain_stats (compute_domain_statistics_new's results)
+    type, bind(C), public :: swh_domain_stats
+        real(c_double) :: max_stage, max_speed, min_stage, min_speed
+        real(c_double) :: energy_potential_on_rho, energy_kinetic_on_rho, energy_total_on_rho
+    end type
+
+    integer(c_int32_t), parameter, public :: SWH_SUPERPOSE_SERIES = 0, SWH_SUPERPOSE_MAX_STAGE = 1
+
     ! Mirrors struct swh_nest_box: one two_way_nesting_comms_type box (1-based indices)
     type, bind(C), public :: swh_nest_box
         integer(c_int32_t) :: inds(6) = 0
@@ -71,7 +79,16 @@ module swals_hip_mod
 
     public :: swh_md_create, swh_md_destroy, swh_md_set_priority, swh_md_set_flux_correct_dry_cells, &
         swh_md_add_box, swh_md_finalize, swh_md_make_initial_conditions_consistent, swh_md_evolve_one_step, &
-        swh_md_evolve_steps, swh_md_volumes, swh_md_synchronize
+        swh_md_evolve_steps, swh_md_volumes, swh_md_synchronize, swh_md_statistics, swh_md_check_stability, &
+        swh_md_graph_capture, swh_md_graph_launch, swh_md_graph_destroy
+
+    public :: swh_domain_statistics, swh_domain_statistics_noalloc, swh_domain_check_stability, swh_snapshot, &
+        swh_snapshot_download, swh_checkpoint_save, swh_checkpoint_restore, swh_evolve_record_gauges_schedule
+
+    public :: swh_batch_create, swh_batch_destroy, swh_batch_upload_elevation, swh_batch_upload_manning_squared, &
+        swh_batch_upload_source, swh_batch_download_source, swh_batch_set_nsteps, swh_batch_gauges_set, &
+        swh_batch_gauges_set_schedule, swh_batch_evolve, swh_batch_gauges_download, swh_batch_status, &
+        swh_batch_autotune, swh_batch_synchronize, swh_batch_superpose
 
     public :: swh_last_error, swh_domain_create, swh_domain_destroy, swh_upload_U, swh_download_U, &
         swh_upload_manning_squared, swh_set_time, swh_set_stage_boundary_sine, swh_set_flather_boundary, &
@@ -298,6 +315,177 @@ module swals_hip_mod
         integer(c_int) function swh_md_synchronize(md) bind(C, name='swh_md_synchronize')
             import c_int, c_ptr
             type(c_ptr), value :: md
+        end function
+        ! print_multidomain's statistics (multidomain_mod.f90:2301-2447) and the stability scan
+        integer(c_int) function swh_md_statistics(md, per_domain, global, energy_is_finite) &
+                bind(C, name='swh_md_statistics')
+            import c_int, c_ptr, c_int32_t, swh_domain_stats
+            type(c_ptr), value :: md
+            type(swh_domain_stats), intent(out) :: per_domain(*)
+            type(swh_domain_stats), intent(out) :: global
+            integer(c_int32_t), intent(out) :: energy_is_finite
+        end function
+        integer(c_int) function swh_md_check_stability(md, n_bad) bind(C, name='swh_md_check_stability')
+            import c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: md
+            integer(c_int64_t), intent(out) :: n_bad
+        end function
+        ! The global step as HIP graphs: capture once, replay nsteps global steps
+        integer(c_int) function swh_md_graph_capture(md, dt) bind(C, name='swh_md_graph_capture')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: md
+            real(c_double), value :: dt
+        end function
+        integer(c_int) function swh_md_graph_launch(md, nsteps) bind(C, name='swh_md_graph_launch')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: md
+            integer(c_int32_t), value :: nsteps
+        end function
+        integer(c_int) function swh_md_graph_destroy(md) bind(C, name='swh_md_graph_destroy')
+            import c_int, c_ptr
+            type(c_ptr), value :: md
+        end function
+        ! compute_domain_statistics_new (domain_mod.f90:683-849) and the stability scan
+        integer(c_int) function swh_domain_statistics(h, out) bind(C, name='swh_domain_statistics')
+            import c_int, c_ptr, swh_domain_stats
+            type(c_ptr), value :: h
+            type(swh_domain_stats), intent(out) :: out
+        end function
+        integer(c_int) function swh_domain_statistics_noalloc(h, out) bind(C, name='swh_domain_statistics_noalloc')
+            import c_int, c_ptr, swh_domain_stats
+            type(c_ptr), value :: h
+            type(swh_domain_stats), intent(out) :: out
+        end function
+        integer(c_int) function swh_domain_check_stability(h, n_bad) bind(C, name='swh_domain_check_stability')
+            import c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(out) :: n_bad
+        end function
+        ! Output staging: pack U(nx, ny, 4) now, download it later on a copy stream
+        integer(c_int) function swh_snapshot(h) bind(C, name='swh_snapshot')
+            import c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function swh_snapshot_download(h, U, time) bind(C, name='swh_snapshot_download')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: U(*)
+            real(c_double), intent(out) :: time
+        end function
+        integer(c_int) function swh_checkpoint_save(h, slot) bind(C, name='swh_checkpoint_save')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: slot
+        end function
+        integer(c_int) function swh_checkpoint_restore(h, slot) bind(C, name='swh_checkpoint_restore')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: slot
+        end function
+        ! gauges written at the listed steps (the driver's write-time rule, generic_model.f90:380-405)
+        integer(c_int) function swh_evolve_record_gauges_schedule(h, timestep, nsamples, sample_steps) &
+                bind(C, name='swh_evolve_record_gauges_schedule')
+            import c_int, c_ptr, c_double, c_int32_t, c_int64_t
+            type(c_ptr), value :: h
+            real(c_double), value :: timestep
+            integer(c_int32_t), value :: nsamples
+            integer(c_int64_t), intent(in) :: sample_steps(*)
+        end function
+        ! Batched unit sources on one shared grid (one run_16.sh model run per source)
+        integer(c_int) function swh_batch_create(desc, nsources, b) bind(C, name='swh_batch_create')
+            import c_int, c_ptr, c_int32_t, swh_domain_desc
+            type(swh_domain_desc), intent(in) :: desc
+            integer(c_int32_t), value :: nsources
+            type(c_ptr), intent(out) :: b
+        end function
+        integer(c_int) function swh_batch_destroy(b) bind(C, name='swh_batch_destroy')
+            import c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+        integer(c_int) function swh_batch_upload_elevation(b, elev) bind(C, name='swh_batch_upload_elevation')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: b
+            real(c_double), intent(in) :: elev(*)
+        end function
+        integer(c_int) function swh_batch_upload_manning_squared(b, m) bind(C, name='swh_batch_upload_manning_squared')
+            import c_int, c_ptr, c_double
+            type(c_ptr), value :: b
+            real(c_double), intent(in) :: m(*)
+        end function
+        integer(c_int) function swh_batch_upload_source(b, k, U3) bind(C, name='swh_batch_upload_source')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: k
+            real(c_double), intent(in) :: U3(*)
+        end function
+        integer(c_int) function swh_batch_download_source(b, k, U3) bind(C, name='swh_batch_download_source')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: k
+            real(c_double), intent(out) :: U3(*)
+        end function
+        integer(c_int) function swh_batch_set_nsteps(b, nsteps_per_source) bind(C, name='swh_batch_set_nsteps')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), intent(in) :: nsteps_per_source(*)
+        end function
+        integer(c_int) function swh_batch_gauges_set(b, n, ij, every, max_samples) bind(C, name='swh_batch_gauges_set')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: n, every, max_samples
+            integer(c_int32_t), intent(in) :: ij(*)
+        end function
+        integer(c_int) function swh_batch_gauges_set_schedule(b, n, ij, nsamples, sample_steps) &
+                bind(C, name='swh_batch_gauges_set_schedule')
+            import c_int, c_ptr, c_int32_t, c_int64_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: n, nsamples
+            integer(c_int32_t), intent(in) :: ij(*)
+            integer(c_int64_t), intent(in) :: sample_steps(*)
+        end function
+        integer(c_int) function swh_batch_evolve(b, dt, nsteps) bind(C, name='swh_batch_evolve')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: b
+            real(c_double), value :: dt
+            integer(c_int32_t), value :: nsteps
+        end function
+        integer(c_int) function swh_batch_gauges_download(b, k, values, nsamples) bind(C, name='swh_batch_gauges_download')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: k
+            real(c_double), intent(out) :: values(*)
+            integer(c_int32_t), intent(out) :: nsamples
+        end function
+        integer(c_int) function swh_batch_status(b, step, n_active) bind(C, name='swh_batch_status')
+            import c_int, c_ptr, c_int32_t, c_int64_t
+            type(c_ptr), value :: b
+            integer(c_int64_t), intent(out) :: step
+            integer(c_int32_t), intent(out) :: n_active
+        end function
+        integer(c_int) function swh_batch_autotune(b, dt, reps, best, best_ms) bind(C, name='swh_batch_autotune')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: b
+            real(c_double), value :: dt
+            integer(c_int32_t), value :: reps
+            integer(c_int32_t), intent(out) :: best(3)
+            real(c_double), intent(out) :: best_ms
+        end function
+        integer(c_int) function swh_batch_synchronize(b) bind(C, name='swh_batch_synchronize')
+            import c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+        ! make_tsunami_event_from_unit_sources (sum_tsunami_unit_sources.R:555-711) on the device
+        integer(c_int) function swh_batch_superpose(b, n_events, event_ptr, event_src, event_w, event_slip, msl, &
+                summary, out, nvalid, kernel_ms) bind(C, name='swh_batch_superpose')
+            import c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: n_events, summary
+            integer(c_int32_t), intent(in) :: event_ptr(*), event_src(*)
+            type(c_ptr), value :: event_w, event_slip
+            real(c_double), value :: msl
+            real(c_double), intent(out) :: out(*)
+            integer(c_int32_t), intent(out) :: nvalid(*)
+            type(c_ptr), value :: kernel_ms
         end function
     end interface
 

@@ -4,7 +4,8 @@
 ! conditions); then every domain and every nesting box is handed to the backend
 ! through swals_hip_mod (md_to_gpu below is the glue a SWALS maintainer would add to
 ! multidomain_mod), and the same multidomain is advanced on the GPU with
-! swh_md_evolve_one_step and on the CPU with md%evolve_one_step.
+! swh_md_evolve_one_step (then from captured HIP graphs, swh_md_graph_launch) and on
+! the CPU with md%evolve_one_step.
 !
 ! The nest is the three-level test nest of ptha_amd.configs.nested_small: a 'linear'
 ! parent, a 'leapfrog_linear_plus_nonlinear_friction' child at 1/3 and an 'rk2'
@@ -28,6 +29,9 @@ program dropin_md_driver
     real(dp), allocatable, target :: Ugpu(:,:,:)
     real(dp) :: dt, rel(3), v0(3), v1(3)
     integer(ip) :: nsteps, k, n, q
+    type(swh_domain_stats) :: stats(3), gstats
+    integer(c_int32_t) :: finite
+    integer(c_int64_t) :: n_bad
     character(len=64) :: arg
 
     call get_command_argument(1, arg)
@@ -60,8 +64,18 @@ program dropin_md_driver
     call md_to_gpu(md, SWH_PREC_FP64, hdom, hmd)
     call swh_check(swh_md_make_initial_conditions_consistent(hmd), 'swh_md_make_initial_conditions_consistent')
     call swh_check(swh_md_volumes(hmd, v0), 'swh_md_volumes')
-    call swh_check(swh_md_evolve_steps(hmd, dt, int(nsteps, c_int32_t)), 'swh_md_evolve_steps')
+    ! the first global step eagerly, the rest replayed from the captured graphs
+    call swh_check(swh_md_evolve_steps(hmd, dt, 1_c_int32_t), 'swh_md_evolve_steps')
+    if(nsteps > 1) then
+        call swh_check(swh_md_graph_capture(hmd, dt), 'swh_md_graph_capture')
+        call swh_check(swh_md_graph_launch(hmd, int(nsteps - 1, c_int32_t)), 'swh_md_graph_launch')
+    end if
     call swh_check(swh_md_volumes(hmd, v1), 'swh_md_volumes')
+    ! print_multidomain's statistics and the stability scan, on the device
+    call swh_check(swh_md_statistics(hmd, stats, gstats, finite), 'swh_md_statistics')
+    call swh_check(swh_md_check_stability(hmd, n_bad), 'swh_md_check_stability')
+    write(*, '(A, 2ES14.6, A, I2, A, I6)') 'gpu global max stage, energy: ', gstats%max_stage, &
+        gstats%energy_total_on_rho, ' finite ', finite, ' unstable cells ', n_bad
 
     ! --- CPU: the reference's own multidomain ---
     call md%make_initial_conditions_consistent()

@@ -30,9 +30,11 @@ def test_fortran_dropin_bitwise(method):
 def test_fortran_dropin_multidomain(tmp_path):
     """The reference's multidomain_type set up in Fortran and handed to the backend
     through swals_hip_mod (tests/fortran/dropin_md_driver.f90): after 20 global steps
-    every domain matches md%evolve_one_step to 1e-9 relative (the rk2 grandchild
-    differs from the reference at round-off, which nesting passes on), and the GPU
-    conserves the total priority-domain volume to 1e-12."""
+    (one eager, then 19 replayed from captured HIP graphs) every domain matches
+    md%evolve_one_step to 1e-9 relative (the rk2 grandchild differs from the reference
+    at round-off, which nesting passes on), the GPU conserves the total priority-domain
+    volume to 1e-12, and the device statistics and stability scan called from Fortran
+    report a finite energy and no unstable cell."""
     exe = os.path.join(ROOT, "oracle", "_ref", "dropin_md_driver_sph")
     print("CASE fortran dropin multidomain", flush=True)
     if not os.path.exists(exe):
@@ -51,3 +53,8 @@ def test_fortran_dropin_multidomain(tmp_path):
         assert max(rel) < 1e-9, (k, rel)
     vol = [float(line.split(":")[1]) for line in r.stdout.splitlines() if line.startswith("gpu relative volume")]
     assert vol and vol[0] < 1e-12
+    st = [line for line in r.stdout.splitlines() if line.startswith("gpu global max stage")]
+    assert st, r.stdout
+    f = st[0].split(":")[1].split()
+    assert float(f[0]) > 0 and float(f[1]) > 0
+    assert f[2] == "finite" and int(f[3]) == 1 and f[4] == "unstable" and f[5] == "cells" and int(f[6]) == 0
