@@ -1664,6 +1664,7 @@ struct swh_batch {
     std::vector<long long> sched;  // steps (ascending) at which the gauges are sampled
     long long *d_gidx = nullptr;
     int *d_gij = nullptr;          // the gauges' (i, j), 0-based, for the fused kernel's sampling
+    std::vector<int> h_gij;
     double *d_gval = nullptr;      // [slot][sample][3][ng]
     int block_x = 256, rows_per_block = 32, variant = 2, src_fast = 0;
     int two_step = 0;              // two fused steps per launch where possible
@@ -1798,6 +1799,21 @@ int batch_launch2(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     return SWH_OK;
 }
 
+// The fused kernel's lanes hold two gauges each (8-bit indices, 16-bit rows): at most
+// two gauges may share a column within one row strip of the launch shape.
+bool gauges_fit_lanes(const swh_batch *b)
+{
+    if (b->ng > 255 || b->g->ny > 65535) return false;
+    const int rows = std::max(1, b->rows_per_block);
+    for (int p = 0; p < b->ng; p++) {
+        int same = 0;
+        for (int q = 0; q < b->ng; q++)
+            same += b->h_gij[2 * q] == b->h_gij[2 * p] && b->h_gij[2 * q + 1] / rows == b->h_gij[2 * p + 1] / rows;
+        if (same > 2) return false;
+    }
+    return true;
+}
+
 // One launch advancing slots [first, first + n) by ns (3 to 6) steps (reads buffer c);
 // with gauges_inside, the gauge samples due after any of those steps are taken by the
 // launch itself.
@@ -1815,14 +1831,18 @@ int batch_launchn(swh_batch *b, LFArgs<T> &a, int fric_mode, int first, int n, i
     a.lf2_order = b->xcd_order;
     a.lf2_group = b->src_group;
     if (gauges_inside) {
+        // the one sample that falls after one of the launch's steps (the caller checks)
         a.g_n = b->ng; a.g_max = b->gmax; a.g_ij = b->d_gij; a.g_val = b->d_gval;
-        for (int q = 0; q < 6; q++)
-            a.g_slot[q] = (q < ns && sample_due(b, b->step + q + 1)) ? samples_upto(b, b->step + q + 1) - 1 : -1;
+        a.g_level = 0;
+        for (int q = 1; q <= ns && !a.g_level; q++)
+            if (sample_due(b, b->step + q)) {
+                a.g_level = q;
+                a.g_sample = samples_upto(b, b->step + q) - 1;
+            }
     }
     const hipError_t e = launch_leapfrogn<T, KAHAN>(a, g->coriolis, fric_mode, g->truely_linear, g->lfc != 0.0, n,
                                                     b->block_x, g->stream, (b->variant & 2) ? 1 : 0, ns);
-    a.g_n = 0;
-    for (int q = 0; q < 6; q++) a.g_slot[q] = -1;
+    a.g_n = 0; a.g_level = 0;
     SWH_HIP(e);
     b->n_launch++; b->src_launch += n; b->src_steps += (long long)ns * n;
     a.src_base = 0;
@@ -1850,12 +1870,16 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
         const bool sample_next = sample_due(b, b->step + 1);
         int n_last;   // sources that took the last step of this round
         const int n2 = can2 ? batch_active(b, b->step + 1) : 0;   // sources with two or more steps left
-        // with 64 gauges or fewer, a round of the full depth that every running source
-        // takes to its end, the gauge samples inside it written by the launch itself
+        // a round of the full depth that every running source takes to its end, the
+        // gauge samples inside it written by the launch itself
         int Dg = 0;
-        if (depth >= 3 && b->ng <= 64) {
+        if (depth >= 3 && gauges_fit_lanes(b)) {
             Dg = std::min<long long>(depth, nsteps - k);
             while (Dg >= 3 && batch_active(b, b->step + Dg - 1) < n_active) Dg--;
+            // at most one sample inside the round (the launch writes one)
+            int due = 0;
+            for (int q = 1; q <= Dg; q++) due += sample_due(b, b->step + q) ? 1 : 0;
+            if (due > 1) Dg = 0;
         }
         if (Dg >= 3) {
             rc = batch_launchn<T, KAHAN>(b, a, fric_mode, 0, n_active, c, Dg, true);
@@ -2200,6 +2224,7 @@ int swh_batch_gauges_set_schedule(swh_batch *b, int32_t n, const int32_t *ij, in
     if (rc) return rc;
     if (n > 0) SWH_HIP(hipMemcpy(b->d_gidx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
     if (n > 0) SWH_HIP(hipMemcpy(b->d_gij, gij.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice));
+    b->h_gij = gij;
     b->ng = n; b->gmax = max_samples;
     b->sched.assign(sample_steps, sample_steps + nsamples);
     return SWH_OK;

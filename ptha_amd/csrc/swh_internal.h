@@ -119,14 +119,13 @@ struct LFArgs {
     // then groups), 0 = the hardware's order
     int lf2_order = 0;
     int lf2_group = 1;
-    // fused kernel: gauge samples taken at the level where they fall (batched runs, at
-    // most 64 gauges): g_slot[n-1] = the sample index written after the launch's n-th
-    // step, or -1; g_ij[2g], g_ij[2g+1] = the gauge's cell (i, j), 0-based;
-    // g_val[slot][sample][3][g_n]
+    // fused kernel: a gauge sample taken at the level where it falls (batched runs, at
+    // most 64 gauges): sample g_sample is written after the launch's g_level-th step;
+    // g_ij[2g], g_ij[2g+1] = the gauge's cell (i, j), 0-based; g_val[slot][sample][3][g_n]
     int g_n = 0, g_max = 0;
+    int g_level = 0, g_sample = 0;
     const int *g_ij = nullptr;
     double *g_val = nullptr;
-    int g_slot[6] = {-1, -1, -1, -1, -1, -1};
 };
 
 // Kernel launchers (defined in the .hip files)

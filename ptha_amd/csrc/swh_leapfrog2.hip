@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
 // wavefront owns 65 - 2 NS of its 64 columns; each strip recomputes NS rows below and
 // NS - 1 above it.
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool PF, int NS, bool GS = false>
-__global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
+__global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn_kernel(LFArgs<T> a)
 {
     constexpr int OWN = 65 - 2 * NS;
     int bx, by, bz;
@@ -358,15 +358,17 @@ __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
     const bool col_int = col > 0 && col < a.nx - 1;      // stage: interior column
     const bool col_mom = col >= 0 && col < a.nx - 1;     // momentum: UH/VH of this column update
     const int cf = min(max(col, 0), a.nx - 1);           // friction-work column
-    // gauges whose cell this wavefront owns (its columns and rows j0..j1-1): bit g
-    unsigned long long gmask = 0;
-    if (GS && a.g_n) {
-        bool in = false;
-        if (lane < a.g_n) {
-            const int gi = a.g_ij[2 * lane], gj = a.g_ij[2 * lane + 1];
-            in = gi >= wg * OWN && gi < min(wg * OWN + OWN, a.nx) && gj >= j0 && gj < j1;
+    // the gauges in this lane's own column and rows j0..j1-1 (the host sees that there are
+    // at most two): indices + 1 in bytes 0 and 1 of gk (0: none), rows in the halves of gr
+    int gk = 0, gr = 0;
+    if (GS && own) {
+        for (int g = 0; g < a.g_n; g++) {
+            const int gi = a.g_ij[2 * g], gj = a.g_ij[2 * g + 1];
+            if (gi == col && gj >= j0 && gj < j1) {
+                if (gk == 0) { gk = g + 1; gr = gj; }
+                else { gk |= (g + 1) << 8; gr |= gj << 16; }
+            }
         }
-        gmask = __ballot(in);
     }
 
     // carried between rows, per level n (0 = the loaded state)
@@ -412,6 +414,7 @@ __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
     };
     T n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw;
     if (PF) load_row(m0, n_s0, n_c0, n_uh0, n_vh0, n_e, n_uhw);
+    T cap_s = T(0), cap_c = T(0), cap_uh = T(0), cap_vh = T(0);   // the sampled level (gauges)
 
     for (int m = m0; m <= j1 + NS - 1; m++) {
         T s0, c0, uh0, vh0, e0, uh0_mem;
@@ -464,19 +467,9 @@ __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
                                                                 FRIC ? fwv_r[n - 1] : T(0), vh_p[n - 1], s_p[n],
                                                                 c_p[n], sn, cn, e_r[n], e_r[n - 1], in_uhb, in_uhb_l,
                                                                 in_uh, in_uh_l);
-            if (GS && gmask && a.g_slot[n - 1] >= 0) {
-                // a gauge sample falls after this level's step: its stage and UH at row r,
-                // its VH at row r-1 (each formed once per level by the lane owning it)
-                double *v = a.g_val + ((long long)(bz + a.src_base) * a.g_max + a.g_slot[n - 1]) * 3 * a.g_n;
-                for (unsigned long long mm = gmask; mm; mm &= mm - 1) {
-                    const int g = __builtin_ctzll(mm);
-                    const int gi = a.g_ij[2 * g], gj = a.g_ij[2 * g + 1];
-                    if (col == gi && r == gj) {
-                        v[g] = KAHAN ? (double)sn - (double)cn : (double)sn;
-                        v[a.g_n + g] = (double)uhn;
-                    }
-                    if (col == gi && r - 1 == gj) v[2 * a.g_n + g] = (double)vhn;
-                }
+            if (GS && n == a.g_level) {
+                // the sampled level's values at this iteration's rows
+                cap_s = sn; cap_c = cn; cap_uh = uhn; cap_vh = vhn;
             }
             const T uhn_sh = shr1(uhn);
             const T uhn_l = west ? uhn : uhn_sh;
@@ -501,6 +494,19 @@ __global__ __launch_bounds__(256) void leapfrogn_kernel(LFArgs<T> a)
             uh_p[n] = uhn; uh_p_l[n] = uhn_l;
         }
         vh_p[NS] = vh_cur; vh_p_r[NS] = vh_cur_r;
+        if (GS && gk) {
+            // the gauge samples after the launch's g_level-th step: stage and UH of row r,
+            // VH of row r-1
+            const int r = m - a.g_level + 1;
+            double *v = a.g_val + ((long long)(bz + a.src_base) * a.g_max + a.g_sample) * 3 * a.g_n;
+            const double st = KAHAN ? (double)cap_s - (double)cap_c : (double)cap_s;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int g = ((gk >> (8 * q)) & 255) - 1, row = (gr >> (16 * q)) & 0xffff;
+                if (g >= 0 && r == row) { v[g] = st; v[a.g_n + g] = (double)cap_uh; }
+                if (g >= 0 && r - 1 == row) v[2 * a.g_n + g] = (double)cap_vh;
+            }
+        }
         const int k = m - NS;
         if (own && k >= j0 && k < j1) {
             const int ow = k * a.pitch + col;
@@ -713,9 +719,7 @@ static hipError_t launch_lfn(const LFArgs<T> &a, int nsrc, int block_x, hipStrea
     constexpr int OWN = 65 - 2 * NS;
     const int waves = (a.nx + OWN - 1) / OWN, wpb = block_x / 64;
     dim3 grid((waves + wpb - 1) / wpb, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
-    bool samples = false;
-    for (int q = 0; q < NS; q++) samples |= a.g_slot[q] >= 0;
-    if (a.g_n && samples)
+    if (a.g_n && a.g_level >= 1 && a.g_level <= NS)
         // the variant that writes the gauge samples falling inside the launch (with the
         // next row's loads in flight): used only by launches that hold a sample
         hipLaunchKernelGGL((leapfrogn_kernel<T, KAHAN, COR, FRIC, TL, LINF, true, NS, true>), grid, dim3(block_x), 0, s,
