@@ -512,12 +512,15 @@ int fv_stage(swh_domain *h, int dst, int src, int qsrc, double fixed_dt, double 
         a.avgC = (const T *)h->C[avg_src];
         if (h->bc_mask) a.ring = (T *)h->ring;
     }
-    SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
     if (h->n_nest_strips && nest_dt != 0.0) {
-        const long long FPn = (long long)h->fpitch * (h->ny + 1), EPn = (long long)h->fpitch * h->ny;
-        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, h->nest_strip_max, (const T *)h->fNS,
-                                     (const T *)h->fEW, FPn, EPn, h->fpitch, 3, nest_dt, h->d_nest_int, h->stream));
+        // the nesting flux integrals ride on the update launch (after the fluxes)
+        a.nest_strips = h->d_nest_strips;
+        a.n_nest_strips = h->n_nest_strips;
+        a.nest_chunks = (h->nest_strip_max + 255) / 256;
+        a.nest_dt = nest_dt;
+        a.nest_int = h->d_nest_int;
     }
+    SWH_HIP(launch_fv_euler<T>(a, fixed_dt, dt_mult, mode, h->stream));
     T *Cd = (T *)h->C[dst];
     if (h->bc_mask && h->bc_kind == 1)
         SWH_HIP(launch_flather<T>((T *)h->S[dst], Cd, (T *)h->UH[dst], (T *)h->VH[dst], (const T *)h->E, h->nx,

@@ -475,6 +475,26 @@ __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS,
 template <typename T>
 __global__ void fv_update_kernel(FVArgs<T> a)
 {
+    if ((int)blockIdx.y >= a.ny) {
+        // the blocks past the grid: the stage's nesting flux integrals
+        // (nested_grid_comms_mod.f90:2308-2461), x += nest_dt * flux on each strip cell
+        // (the fluxes are pre-multiplied by the edge length); the update does not touch
+        // the flux planes or the integrals, so they run beside it
+        const int e = (blockIdx.y - a.ny) * gridDim.x + blockIdx.x;
+        if (e >= a.n_nest_strips * a.nest_chunks) return;
+        const NestStrip st = a.nest_strips[e / a.nest_chunks];
+        const int r = (e % a.nest_chunks) * blockDim.x + threadIdx.x;
+        if (st.line < 0 || r >= st.n) return;
+        const long long FP = (long long)a.fpitch * (a.ny + 1), EPn = (long long)a.fpitch * a.ny;
+        const long long o = st.kind == 0 ? (long long)st.line * a.fpitch + st.lo + r
+                                         : (long long)(st.lo + r) * a.fpitch + st.line;
+        for (int k = 0; k < 3; k++) {
+            const double f = st.kind == 0 ? (double)a.fNS[k * FP + o] : (double)a.fEW[k * EPn + o];
+            double *x = a.nest_int + st.out + (long long)k * st.n + r;
+            *x = *x + a.nest_dt * f * st.edge;
+        }
+        return;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     unsigned long long fixed = 0;
@@ -752,6 +772,8 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
     hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(kScalarThreads), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
                        fixed_dt, dt_mult, mode, a.sc, (const double *)a.bmax,
                        fv_flux_grid_blocks(a.nx, a.ny, a.flux_rows));
+    if (a.nest_strips && a.n_nest_strips > 0 && a.nest_chunks > 0 && a.nest_dt != 0.0)
+        grid.y += (a.n_nest_strips * a.nest_chunks + grid.x - 1) / grid.x;   // the nesting blocks
     hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
     return hipGetLastError();
 }

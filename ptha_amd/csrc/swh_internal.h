@@ -324,6 +324,8 @@ template <typename T>
 hipError_t launch_adjoint_inject(T *lS, int n, const long long *idx, const double *values, hipStream_t s);
 
 // ---- finite-volume (rk2) ----
+struct NestStrip;
+
 template <typename T>
 struct FVArgs {
     int nx, ny, pitch;           // state planes: pitch x ny
@@ -358,6 +360,12 @@ struct FVArgs {
     T *ring = nullptr;
     // flux pass: rows per wavefront of the column-marching kernel (4..64)
     int flux_rows = 8;
+    // nesting flux integrals of this stage (x += nest_dt * flux on each strip cell),
+    // done by extra blocks of the update launch: null or nest_dt == 0: none
+    const NestStrip *nest_strips = nullptr;
+    int n_nest_strips = 0, nest_chunks = 0;
+    double nest_dt = 0.0;
+    double *nest_int = nullptr;
 };
 
 template <typename T>

@@ -114,6 +114,35 @@ def test_config1_full_size_fp64(oracle_built):
     g.close()
 
 
+def test_config1_full_size_fp32k(oracle_built):
+    """Config 1 at its real size in FP32 with the Kahan-compensated stage: the 16 gauges'
+    stage (every 50 steps), the max-stage grid and the final stage within 1e-5 m of the
+    FP64 oracle (the errors are printed)."""
+    from ptha_amd import configs as C
+    _print_case("config1_full_fp32k")
+    cfg = C.gaussian_hump_cartesian()
+    o = pyoracle.CpuDomain("oracle", cfg["nx"], cfg["ny"], cfg["lw"], cfg["ll"], method="linear")
+    o.set_U(cfg["U"])
+    dt = o.stationary_timestep_max()
+    g = H.gpu_domain(cfg, precision="fp32k", max_U_variables=("max_stage",))
+    ij = g.setup_point_gauges(cfg["gauges_xy"], max_samples=64)
+    ref_g = []
+    for block in range(20):
+        o.evolve(dt, 50)
+        g.evolve_steps(dt, 50)
+        g.sample_gauges()
+        U = o.get_U()
+        ref_g.append(np.array([U[0][j - 1, i - 1] for (i, j) in ij]))
+    _, series = g.gauge_series()
+    err = dict(gauge_stage=float(np.abs(series[:, 0, :] - np.array(ref_g)).max()),
+               max_stage=float(np.abs(g.max_U("max_stage") - o.max_U("max_stage")).max()),
+               final_stage=float(np.abs(g.U[0] - o.get_U()[0]).max()))
+    print("config1 fp32k", err, flush=True)
+    for q, e in err.items():
+        assert e <= 1e-5, q
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["linear_sph", "linear_cart"])
 def test_leapfrog_fp32k_vs_fp64_oracle(oracle_built, case):
     """FP32 state with Kahan-compensated stage: stage within 1e-5 m of the FP64
