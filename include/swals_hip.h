@@ -116,6 +116,11 @@ int swh_domain_destroy(swh_domain *h);
 
 /* Host <-> device copies of domain%U (domain_mod.f90:332) and fields. */
 int swh_upload_U(swh_domain *h, const double *U);
+/* The state without the elevation: stage, UH, VH as (nx, ny) FP64 planes (any may be
+ * NULL: zeros); the elevation, friction work and metrics stay as they are (a new
+ * initial condition on the same grid, as the reference's set-up of U(:,:,STG:VH) for a
+ * new run). In FP32K the stage is stored with its Kahan term, as swh_upload_U does. */
+int swh_upload_state(swh_domain *h, const double *stage, const double *uh, const double *vh);
 int swh_download_U(swh_domain *h, double *U);
 int swh_upload_manning_squared(swh_domain *h, const double *manning_squared); /* domain_mod.f90:347 */
 int swh_set_time(swh_domain *h, double time);

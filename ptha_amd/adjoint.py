@@ -53,10 +53,16 @@ class GaugeMap:
         self.ngauges = len(self.gauge_ij)
         self.checkpoint_every = int(checkpoint_every)
         self.checkpoint_steps = []
+        self._elev_set = False
 
-    def _reset(self, U):
-        # a fresh forward run: new state, time zero, empty gauge buffer
-        self.dom.U = U
+    def _reset(self, eta0, uh0=None, vh0=None):
+        # a fresh forward run: new state (the elevation stays), time zero, empty gauge buffer
+        if not self._elev_set:
+            U = np.zeros((4, self.ny, self.nx))
+            U[3] = self.elev
+            self.dom.U = U
+            self._elev_set = True
+        self.dom.set_state(eta0, uh0, vh0)
         self.dom.set_time(0.0)
         self.gauge_ij = self.dom.setup_point_gauges(self.gauges_xy, max_samples=self.nsamples)
 
@@ -78,14 +84,7 @@ class GaugeMap:
 
     def forward(self, eta0, uh0=None, vh0=None):
         """Gauge stage series (nsamples, ngauges) from the initial state."""
-        U = np.zeros((4, self.ny, self.nx))
-        U[0] = eta0
-        if uh0 is not None:
-            U[1] = uh0
-        if vh0 is not None:
-            U[2] = vh0
-        U[3] = self.elev
-        self._reset(U)
+        self._reset(eta0, uh0, vh0)
         lib = self.dom._lib
         B.check(lib.swh_checkpoint_save(self.dom._h, 0), "swh_checkpoint_save")
         self.checkpoint_steps = [0]

@@ -72,6 +72,7 @@ SYMBOLS = [
     ("swh_domain_create", ctypes.c_int, [ctypes.POINTER(DomainDesc), ctypes.POINTER(ctypes.c_void_p)]),
     ("swh_domain_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_upload_U", ctypes.c_int, [ctypes.c_void_p, _PD]),
+    ("swh_upload_state", ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD]),
     ("swh_download_U", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_upload_manning_squared", ctypes.c_int, [ctypes.c_void_p, _PD]),
     ("swh_set_time", ctypes.c_int, [ctypes.c_void_p, _D]),

@@ -49,36 +49,43 @@ void dfree(void *p)
     if (p) (void)hipFree(p);
 }
 
-template <typename T>
-int upload_plane(swh_domain *h, void *dst, const double *src)
+// Host transfers go through one dense FP64 staging plane on the device; the conversion
+// to the storage type and the pitch are applied there.
+int stage_buffer(swh_domain *h)
 {
-    std::vector<T> tmp(h->plane_elems(), T(0));
-    for (int j = 0; j < h->ny; j++)
-        for (int i = 0; i < h->nx; i++) tmp[(size_t)j * h->pitch + i] = (T)src[(size_t)j * h->nx + i];
-    SWH_HIP(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (h->d_stage) return SWH_OK;
+    return dmalloc((void **)&h->d_stage, sizeof(double) * (size_t)h->nx * h->ny);
+}
+
+// A dense FP64 plane (src; null: zeros) into the pitched plane dst; with comp (FP32
+// state), the Kahan term that makes dst - comp the FP64 values. Returns when done.
+template <typename T>
+int upload_plane(swh_domain *h, void *dst, const double *src, void *comp = nullptr)
+{
+    int rc = stage_buffer(h);
+    if (rc) return rc;
+    if (src) SWH_HIP(hipMemcpy(h->d_stage, src, sizeof(double) * (size_t)h->nx * h->ny, hipMemcpyHostToDevice));
+    SWH_HIP(launch_plane_from_dense<T>(src ? h->d_stage : nullptr, (T *)dst, (T *)comp, h->nx, h->ny, h->pitch,
+                                       h->stream));
+    SWH_HIP(hipStreamSynchronize(h->stream));
     return SWH_OK;
 }
 
 template <typename T>
 int download_plane(swh_domain *h, const void *src, const void *comp, double *dst)
 {
-    std::vector<T> tmp(h->plane_elems()), tc;
-    SWH_HIP(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
-    if (comp) {
-        tc.resize(h->plane_elems());
-        SWH_HIP(hipMemcpy(tc.data(), comp, tc.size() * sizeof(T), hipMemcpyDeviceToHost));
-    }
-    for (int j = 0; j < h->ny; j++)
-        for (int i = 0; i < h->nx; i++) {
-            size_t o = (size_t)j * h->pitch + i;
-            dst[(size_t)j * h->nx + i] = comp ? (double)tmp[o] - (double)tc[o] : (double)tmp[o];
-        }
+    int rc = stage_buffer(h);
+    if (rc) return rc;
+    SWH_HIP(launch_plane_to_dense<T>((const T *)src, (const T *)comp, h->d_stage, h->nx, h->ny, h->pitch, h->stream));
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    SWH_HIP(hipMemcpy(dst, h->d_stage, sizeof(double) * (size_t)h->nx * h->ny, hipMemcpyDeviceToHost));
     return SWH_OK;
 }
 
-int upload_plane_any(swh_domain *h, void *dst, const double *src)
+int upload_plane_any(swh_domain *h, void *dst, const double *src, void *comp = nullptr)
 {
-    return h->prec == SWH_PREC_FP64 ? upload_plane<double>(h, dst, src) : upload_plane<float>(h, dst, src);
+    return h->prec == SWH_PREC_FP64 ? upload_plane<double>(h, dst, src)
+                                    : upload_plane<float>(h, dst, src, comp);
 }
 
 // friction_work for the truely-linear leapfrog + friction solver, built once on the
@@ -917,7 +924,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
                     h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
                     h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt,
-                    h->d_stats, h->d_ggrad, h->ring};
+                    h->d_stats, h->d_ggrad, h->ring, h->d_stage};
     for (void *p : ptrs) dfree(p);
     for (void *p : h->ckpt) dfree(p);
     dfree(h->snap);
@@ -935,13 +942,24 @@ int swh_upload_U(swh_domain *h, const double *U)
     const size_t n = (size_t)h->nx * h->ny;
     const int c = h->cur;
     int rc;
-    if ((rc = upload_plane_any(h, h->S[c], U))) return rc;
+    // FP32 state: the stage with its Kahan term, so that S - C holds the FP64 stage
+    if ((rc = upload_plane_any(h, h->S[c], U, h->C[c]))) return rc;
     if ((rc = upload_plane_any(h, h->UH[c], U + n))) return rc;
     if ((rc = upload_plane_any(h, h->VH[c], U + 2 * n))) return rc;
     if ((rc = upload_plane_any(h, h->E, U + 3 * n))) return rc;
-    if (h->C[c]) SWH_HIP(hipMemset(h->C[c], 0, h->plane_elems() * h->esz));
     h->h_elev.assign(U + 3 * n, U + 4 * n);
     return build_friction_work(h);
+}
+
+int swh_upload_state(swh_domain *h, const double *stage, const double *uh, const double *vh)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    SWH_HIP(hipStreamSynchronize(h->stream));
+    const int c = h->cur;
+    int rc;
+    if ((rc = upload_plane_any(h, h->S[c], stage, h->C[c]))) return rc;
+    if ((rc = upload_plane_any(h, h->UH[c], uh))) return rc;
+    return upload_plane_any(h, h->VH[c], vh);
 }
 
 int swh_download_U(swh_domain *h, double *U)
@@ -2164,10 +2182,9 @@ int swh_batch_upload_source(swh_batch *b, int32_t k, const double *U3)
     const size_t n = (size_t)g->nx * g->ny, off = (size_t)b->slot_of[k] * g->plane_elems() * g->esz;
     const int c = b->cur;
     int rc;
-    if ((rc = upload_plane_any(g, (char *)b->S[c] + off, U3))) return rc;
+    if ((rc = upload_plane_any(g, (char *)b->S[c] + off, U3, b->C[c] ? (char *)b->C[c] + off : nullptr))) return rc;
     if ((rc = upload_plane_any(g, (char *)b->UH[c] + off, U3 + n))) return rc;
     if ((rc = upload_plane_any(g, (char *)b->VH[c] + off, U3 + 2 * n))) return rc;
-    if (b->C[c]) SWH_HIP(hipMemset((char *)b->C[c] + off, 0, g->plane_elems() * g->esz));
     return SWH_OK;
 }
 

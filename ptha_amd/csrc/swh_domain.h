@@ -51,6 +51,7 @@ struct swh_domain {
     double rc_dt = -1.0;
     DevScalars *sc = nullptr;
     double *d_work = nullptr;  // reductions scratch (2*ny + 8 doubles)
+    double *d_stage = nullptr; // host-transfer staging: one dense FP64 plane (nx*ny), lazily
     double *d_stats = nullptr; // statistics scratch: 6 row arrays, 7 results, a counter (6*ny + 8 doubles)
     // host-side scalars (leapfrog keeps time on the host; rk2 on the device)
     double time = 0.0, evolve_step_dt = 0.0, dt_last_update = 0.0;

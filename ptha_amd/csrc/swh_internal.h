@@ -153,6 +153,13 @@ hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode
 // Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
 hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
 hipError_t launch_probe_valu(int fp64, int blocks, int iters, void *out, hipStream_t s);
+// Host transfers: a dense FP64 plane (nx*ny, i fastest) into a pitched plane of T
+// (padding columns zeroed; with comp, the FP32 Kahan term that makes dst - comp the FP64
+// value; src null: zeros), and back (dst = src - comp).
+template <typename T>
+hipError_t launch_plane_from_dense(const double *src, T *dst, T *comp, int nx, int ny, int pitch, hipStream_t s);
+template <typename T>
+hipError_t launch_plane_to_dense(const T *src, const T *comp, double *dst, int nx, int ny, int pitch, hipStream_t s);
 hipError_t launch_probe_math(const double *x, const double *y, const double *t, double *pw, double *sn, int n,
                              hipStream_t s);
 

@@ -671,6 +671,52 @@ hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStr
     return hipGetLastError();
 }
 
+// ---- host transfers: dense FP64 planes <-> pitched planes of the storage type ----
+template <typename T>
+__global__ void plane_from_dense_kernel(const double *__restrict__ src, T *__restrict__ dst, T *__restrict__ comp,
+                                        int nx, int ny, int pitch)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= pitch || j >= ny) return;
+    const long long o = (long long)j * pitch + i;
+    const double v = (i < nx && src) ? src[(long long)j * nx + i] : 0.0;
+    const T t = (T)v;
+    dst[o] = t;
+    if (comp) comp[o] = (T)((double)t - v);   // dst - comp represents v
+}
+
+template <typename T>
+__global__ void plane_to_dense_kernel(const T *__restrict__ src, const T *__restrict__ comp, double *__restrict__ dst,
+                                      int nx, int ny, int pitch)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long long o = (long long)j * pitch + i;
+    dst[(long long)j * nx + i] = comp ? (double)src[o] - (double)comp[o] : (double)src[o];
+}
+
+template <typename T>
+hipError_t launch_plane_from_dense(const double *src, T *dst, T *comp, int nx, int ny, int pitch, hipStream_t s)
+{
+    hipLaunchKernelGGL(plane_from_dense_kernel<T>, dim3((pitch + 255) / 256, ny), dim3(256), 0, s, src, dst, comp, nx,
+                       ny, pitch);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_plane_to_dense(const T *src, const T *comp, double *dst, int nx, int ny, int pitch, hipStream_t s)
+{
+    hipLaunchKernelGGL(plane_to_dense_kernel<T>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, src, comp, dst, nx, ny,
+                       pitch);
+    return hipGetLastError();
+}
+
+template hipError_t launch_plane_from_dense<double>(const double *, double *, double *, int, int, int, hipStream_t);
+template hipError_t launch_plane_from_dense<float>(const double *, float *, float *, int, int, int, hipStream_t);
+template hipError_t launch_plane_to_dense<double>(const double *, const double *, double *, int, int, int,
+                                                  hipStream_t);
+template hipError_t launch_plane_to_dense<float>(const float *, const float *, double *, int, int, int, hipStream_t);
+
 // ---- VALU throughput (measurement reference) ----
 // Eight independent fused multiply-add chains per lane, iters rounds: 8 * iters VALU
 // FMA wave-instructions per wavefront (FP32 ones unpacked: this code is built without

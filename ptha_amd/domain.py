@@ -153,6 +153,20 @@ class Domain:
             raise ValueError("U must have shape (4, ny, nx) = %s" % ((4, self.ny, self.nx),))
         B.check(self._lib.swh_upload_U(self._h, _p(a)), "swh_upload_U")
 
+    def set_state(self, stage, uh=None, vh=None):
+        """A new initial state on the same grid: stage, UH, VH as (ny, nx) arrays (None:
+        zeros); the elevation stays (swh_upload_state)."""
+        def plane(v):
+            if v is None:
+                return None
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            if a.shape != (self.ny, self.nx):
+                raise ValueError("planes must have shape (ny, nx) = %s" % ((self.ny, self.nx),))
+            return a
+        planes = [plane(stage), plane(uh), plane(vh)]
+        ptrs = [_p(a) if a is not None else ctypes.POINTER(ctypes.c_double)() for a in planes]
+        B.check(self._lib.swh_upload_state(self._h, *ptrs), "swh_upload_state")
+
     def set_manning_squared(self, m):
         a = np.ascontiguousarray(np.broadcast_to(m, (self.ny, self.nx)), dtype=np.float64)
         B.check(self._lib.swh_upload_manning_squared(self._h, _p(a)), "swh_upload_manning_squared")
