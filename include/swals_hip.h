@@ -294,6 +294,11 @@ int swh_adjoint_run_schedule(swh_domain *h, double dt, int32_t nsteps, int32_t n
 
 /* Launch shape of the leapfrog step kernel (see swh_batch_set_launch). */
 int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block, int32_t variant);
+/* Launch shape of the two-step leapfrog kernel of swh_evolve_steps: threads per
+ * workgroup (64..256, a multiple of 64) and rows per strip (4..4096, or 0: chosen from
+ * the kernel's occupancy, the default). Results do not depend on it. No counterpart in
+ * the reference (a tuning knob of this backend). */
+int swh_domain_set_launch2(swh_domain *h, int32_t block_x, int32_t rows_per_block);
 
 /* ---- Batched unit sources on one shared grid ----
  * PTHA unit-source production runs one serial 'linear' model per source over a shared

@@ -115,6 +115,7 @@ SYMBOLS = [
     ("swh_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
     ("swh_batch_autotune", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PI32, _PD]),
     ("swh_domain_set_launch", ctypes.c_int, [ctypes.c_void_p, _I32, _I32, _I32]),
+    ("swh_domain_set_launch2", ctypes.c_int, [ctypes.c_void_p, _I32, _I32]),
     ("swh_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("swh_batch_evolve_timed", ctypes.c_int, [ctypes.c_void_p, _D, _I32, _PD]),
     ("swh_set_forcing_patch", ctypes.c_int, [ctypes.c_void_p, _D, _D, _I32, _I32, _I32, _I32, _I32, _I32, _PD]),

@@ -181,6 +181,26 @@ double forcing_fraction(const swh_domain *h, double time, double dt)
     return (std::min(time, h->f_end) - std::max(time - dt, h->f_start)) / (h->f_end - h->f_start);
 }
 
+// The step's boundary-flux sums in line workgroups of the step launch (of block_x
+// threads, a workgroup per block_x cells of each side: one cell per thread);
+// t_after1/2: the end times of the launch's steps (NaN: from the device time).
+BfLines bf_args(const swh_domain *h, int block_x, double dt, double t_after1, double t_after2)
+{
+    BfLines b;
+    const int lmax = std::max(std::max(h->nx, h->ny) - 2 * h->n_ext, 1);
+    b.want = 4 * ((lmax + block_x - 1) / block_x);
+    b.cap = h->bf_cap;
+    b.n_ext = h->n_ext;
+    b.dle = h->d_dle;
+    b.dbe = h->d_dbe;
+    b.sc = h->sc;
+    b.part = h->d_bfpart;
+    b.dt = dt;
+    b.t_after[0] = t_after1;
+    b.t_after[1] = t_after2;
+    return b;
+}
+
 template <typename T, bool KAHAN>
 int leapfrog_step(swh_domain *h, double dt)
 {
@@ -207,9 +227,6 @@ int leapfrog_step(swh_domain *h, double dt)
     // boundary fluxes read the pre-momentum UH/VH (include.f90:126-165)
     // (under graph capture the kernel forms the end time from the device time: NaN)
     const double t_after_arg = h->graph_time ? __builtin_nan("") : t_after;
-    if (!flather)
-        SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                         h->d_dle, h->d_dbe, dt, t_after_arg, h->sc, h->stream));
     LFArgs<T> a;
     a.nx = h->nx; a.ny = h->ny; a.pitch = h->pitch; a.src_stride = 0;
     a.S = (const T *)h->S[c]; a.C = (const T *)h->C[c]; a.UH = (const T *)h->UH[c]; a.VH = (const T *)h->VH[c];
@@ -259,17 +276,17 @@ int leapfrog_step(swh_domain *h, double dt)
         if (frac != 0.0 && h->fE)
             SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
                                          (T)frac, h->stream));
-        SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                         h->d_dle, h->d_dbe, dt, t_after_arg, h->sc, h->stream));
         a.S = (const T *)h->S[o];
         a.C = (const T *)h->C[o];
         a.stage_given = 1;
+        a.bf = bf_args(h, h->block_x, dt, t_after_arg, 0.0);
         SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
                                           h->variant, h->stream));
     } else {
         if (frac != 0.0) {
             a.fS = (const T *)h->fS; a.fE = (const T *)h->fE; a.f_frac = (T)frac;
         }
+        a.bf = bf_args(h, h->block_x, dt, t_after_arg, 0.0);
         SWH_HIP(launch_leapfrog<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, h->lfc != 0.0, 1, h->block_x,
                                           h->variant, h->stream));
         if (frac != 0.0 && h->fE)
@@ -394,36 +411,16 @@ bool two_step_ok(const swh_domain *h, double dt)
     return true;
 }
 
-int bfl_cells(swh_domain *h)
-{
-    if (h->d_bfl_idx) return SWH_OK;
-    // the rows and columns bflux_leapfrog_kernel reads (include.f90:126-165)
-    std::vector<long long> idx;
-    const int ne = h->n_ext;
-    for (int i = ne; i <= h->nx - ne - 1; i++) {
-        idx.push_back((long long)(h->ny - ne - 1) * h->pitch + i);
-        idx.push_back((long long)(ne - 1) * h->pitch + i);
-    }
-    for (int j = ne; j <= h->ny - ne - 1; j++) {
-        idx.push_back((long long)j * h->pitch + h->nx - ne - 1);
-        idx.push_back((long long)j * h->pitch + ne - 1);
-    }
-    int rc = dmalloc((void **)&h->d_bfl_idx, sizeof(long long) * idx.size());
-    if (rc) return rc;
-    SWH_HIP(hipMemcpy(h->d_bfl_idx, idx.data(), sizeof(long long) * idx.size(), hipMemcpyHostToDevice));
-    h->n_bfl = (int)idx.size();
-    return SWH_OK;
-}
-
+// Two leapfrog steps in one launch. mid_sample: also record a gauge sample of the state
+// after the first step (taken one step ahead from the current state, with the same
+// operations, before the launch).
 template <typename T, bool KAHAN>
-int leapfrog_step2(swh_domain *h, double dt)
+int leapfrog_step2(swh_domain *h, double dt, bool mid_sample = false)
 {
     if (h->rc_dt != dt) {
         int rc = build_rc<T>(h, dt);
         if (rc) return rc;
     }
-    int rc = bfl_cells(h);
-    if (rc) return rc;
     int fric_mode = 0;
     if (h->method == SWH_METHOD_LLPNF) fric_mode = 1;
     if (fric_mode == 1 && !h->fw_ready) return fail(SWH_ESTATE, "friction_work not built: upload U first");
@@ -449,15 +446,17 @@ int leapfrog_step2(swh_domain *h, double dt)
     a.t_after = (T)t_after2;
     a.arrival_stage = (T)(h->msl + h->arrival_thresh);
     const int lin = h->lfc != 0.0;
-    // first step's boundary fluxes from the current UH/VH, the second's from UH/VH one
-    // step ahead on the lines (in the output planes, overwritten by the launch after)
-    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                     h->d_dle, h->d_dbe, dt, t_after1, h->sc, h->stream));
-    SWH_HIP(launch_line_step1<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, lin, h->n_bfl, h->d_bfl_idx,
-                                        h->stream));
-    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[o], (const T *)h->VH[o], h->nx, h->ny, h->pitch, h->n_ext,
-                                     h->d_dle, h->d_dbe, dt, t_after2, h->sc, h->stream));
-    SWH_HIP(launch_leapfrog2<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, lin, 1, 256, h->stream));
+    // both steps' boundary fluxes in line workgroups of the launch: the first step's from
+    // the current UH/VH, the second's from UH/VH one step ahead on the lines
+    a.bf = bf_args(h, h->block2, dt, t_after1, t_after2);
+    if (mid_sample) {
+        if (!h->d_gidx || h->ng < 1) return fail(SWH_ESTATE, "gauges not set");
+        if (h->gsamples >= h->gmax) return fail(SWH_ESTATE, "gauge sample buffer full");
+        SWH_HIP(launch_gauge_step1<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, lin, 1, h->ng, h->d_gidx,
+                                             h->d_gval, h->gmax, h->gsamples, h->d_gtime, t_after1, h->stream));
+        h->gsamples++;
+    }
+    SWH_HIP(launch_leapfrog2<T, KAHAN>(a, h->coriolis, fric_mode, h->truely_linear, lin, 1, h->block2, h->stream));
     h->cur = o;
     h->time = t_after2;
     h->dt_last_update = dt;
@@ -465,7 +464,7 @@ int leapfrog_step2(swh_domain *h, double dt)
     h->nsteps += 2;
     // max_U due after the second step (not after the first: two_step_ok)
     if (h->maxq_mask && ((h->nsteps - 1) % h->maxq_freq) == 0) {
-        rc = maxq_now<T>(h, nullptr, h->time);
+        const int rc = maxq_now<T>(h, nullptr, h->time);
         if (rc) return rc;
     }
     return SWH_OK;
@@ -822,9 +821,10 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
         const long long waves_x = (lanes + 63) / 64;
         while (h->rows_per_block > 4 && waves_x * ((d->ny + h->rows_per_block - 1) / h->rows_per_block) < 4096)
             h->rows_per_block /= 2;
-        // the two-step kernel: a wavefront owns 61 columns of a strip (3 more rows each)
-        const long long waves2_x = (d->nx + 60) / 61;
-        while (h->rows2 > 8 && waves2_x * ((d->ny + h->rows2 - 1) / h->rows2) < 2048) h->rows2 /= 2;
+        // the two-step kernel: rows per strip chosen by its launcher from the kernel's
+        // occupancy (rows2 = 0); workgroups of 128 threads (FP32) or 256 (FP64)
+        h->rows2 = 0;
+        h->block2 = h->prec == SWH_PREC_FP64 ? 256 : 128;
     }
     const int nx = h->nx, ny = h->ny;
     h->dbe.assign(d->distance_bottom_edge, d->distance_bottom_edge + ny + 1);
@@ -880,6 +880,10 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     }
     A(&h->rc, h->esz * ((size_t)5 * ny + (nx + 1) + (ny + 1)));
     A((void **)&h->sc, sizeof(DevScalars));
+    // line workgroups: at most 4 ceil(max(nx, ny) / 64) wanted, in rows at most
+    // ceil(nx / 64) workgroups wide
+    h->bf_cap = 4 * ((std::max(nx, ny) + 63) / 64) + (nx + 63) / 64;
+    A((void **)&h->d_bfpart, sizeof(double) * 4 * (size_t)h->bf_cap);
     A((void **)&h->d_work, sizeof(double) * (2 * (size_t)ny + 16));
     if (rc) {
         swh_domain_destroy(h);
@@ -922,7 +926,7 @@ int swh_domain_destroy(swh_domain *h)
                     h->VH[0], h->VH[1], h->C[0], h->C[1], h->E, h->fwU, h->fwV, h->nsq, h->mx, h->depth, h->u,
                     h->v, h->fNS, h->fEW, h->esU, h->esV, h->esJ, h->esE, h->rc, h->sc, h->d_work, h->d_gidx,
                     h->d_gval, h->d_gtime, h->adj[0][0], h->adj[0][1], h->adj[0][2], h->adj[1][0],
-                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfl_idx, h->fS, h->fUH, h->fVH, h->fE,
+                    h->adj[1][1], h->adj[1][2], h->d_gvec, h->d_bmax, h->d_bfpart, h->fS, h->fUH, h->fVH, h->fE,
                     h->nlw, h->d_corbe, h->dlast, h->drhs, h->dq[0], h->dq[1], h->dq[2], h->dwork, h->d_qt,
                     h->d_stats, h->d_ggrad, h->ring, h->d_stage};
     for (void *p : ptrs) dfree(p);
@@ -1551,13 +1555,30 @@ int swh_evolve_record_gauges_schedule(swh_domain *h, double timestep, int32_t ns
             sample_steps[q] > (int64_t)INT32_MAX)
             return fail(SWH_EINVAL, "sample steps must be >= 0 and strictly increasing");
     if (h->gsamples + nsamples > h->gmax) return fail(SWH_ESTATE, "gauge sample buffer too small");
+    // steps in pairs wherever two_step_ok allows, whatever the sample spacing: a sample
+    // due between the two steps of a pair is taken one step ahead by the pair itself
+    const long long last = sample_steps[nsamples - 1];
     long long n = 0;
-    int rc = SWH_OK;
-    for (int q = 0; q < nsamples && !rc; q++) {
-        const long long k = sample_steps[q] - n;
-        if (k > 0) rc = swh_evolve_steps(h, timestep, (int32_t)k);
-        n = sample_steps[q];
-        if (!rc) rc = swh_gauges_sample(h);
+    int q = 0, rc = SWH_OK;
+    if (sample_steps[0] == 0) {
+        rc = swh_gauges_sample(h);
+        q = 1;
+    }
+    while (!rc && n < last) {
+        const bool mid = q < nsamples && sample_steps[q] == n + 1;
+        if (n + 2 <= last && two_step_ok(h, timestep) && (!mid || h->ng > 0)) {
+            rc = h->prec == SWH_PREC_FP64 ? leapfrog_step2<double, false>(h, timestep, mid)
+                                          : leapfrog_step2<float, true>(h, timestep, mid);
+            if (mid) q++;
+            n += 2;
+        } else {
+            rc = evolve(h, timestep);
+            n += 1;
+        }
+        if (!rc && q < nsamples && sample_steps[q] == n) {
+            rc = swh_gauges_sample(h);
+            q++;
+        }
     }
     return rc;
 }
@@ -1652,6 +1673,14 @@ int swh_evolve_record_gauges(swh_domain *h, double timestep, int32_t nsteps, int
 {
     if (!h || nsteps < 0 || every < 1) return fail(SWH_EINVAL, "bad arguments");
     if (!h->d_gidx) return fail(SWH_ESTATE, "gauges not set");
+    if (h->leapfrog() && timestep > 0.0) {
+        // the schedule 0, every, 2 every, ... (steps in pairs), then the steps after it
+        std::vector<int64_t> steps;
+        for (long long n = 0; n <= nsteps; n += every) steps.push_back(n);
+        int rc = swh_evolve_record_gauges_schedule(h, timestep, (int32_t)steps.size(), steps.data());
+        if (!rc && nsteps > steps.back()) rc = swh_evolve_steps(h, timestep, (int32_t)(nsteps - steps.back()));
+        return rc;
+    }
     int rc = swh_gauges_sample(h);
     if (rc) return rc;
     for (int n = 1; n <= nsteps; n++) {
@@ -1951,7 +1980,8 @@ int batch_evolve(swh_batch *b, double dt, int nsteps)
                 SWH_HIP(launch_gauge_step1<T, KAHAN>(batch_cur_args(b, a, c), b->g->coriolis, fric_mode,
                                                      b->g->truely_linear, b->g->lfc != 0.0, n_active, b->ng,
                                                      b->d_gidx, b->d_gval, b->gmax,
-                                                     samples_upto(b, b->step + 1) - 1, b->g->stream));
+                                                     samples_upto(b, b->step + 1) - 1, nullptr, 0.0,
+                                                     b->g->stream));
             }
             rc = batch_launch2<T, KAHAN>(b, a, fric_mode, 0, n2, c);
             if (rc) return rc;
@@ -2402,6 +2432,19 @@ int swh_domain_set_launch(swh_domain *h, int32_t block_x, int32_t rows_per_block
     if (rc) return rc;
     h->block_x = block_x; h->rows_per_block = rows_per_block; h->variant = variant & 7;
     h->two_step = (variant & 32) ? 0 : 1;
+    return SWH_OK;
+}
+
+int swh_domain_set_launch2(swh_domain *h, int32_t block_x, int32_t rows_per_block)
+{
+    if (!h) return fail(SWH_EINVAL, "null argument");
+    if (block_x < 64 || block_x > 256 || (block_x % 64) || (rows_per_block != 0 && rows_per_block < 4) ||
+        rows_per_block > 4096)
+        return fail(SWH_EINVAL,
+                    "two-step launch: block_x in 64..256 (a multiple of 64), rows_per_block 0 (chosen from the "
+                    "kernel's occupancy) or in 4..4096");
+    h->block2 = block_x;
+    h->rows2 = rows_per_block;
     return SWH_OK;
 }
 

@@ -92,10 +92,13 @@ struct swh_domain {
     // launch shape of the step kernel
     int block_x = 256, rows_per_block = 32, variant = 2;
     // two leapfrog steps per launch in swh_evolve_steps (where nothing is needed between
-    // them): rows per strip of that kernel, and the cells of the boundary-flux lines
-    int two_step = 1, rows2 = 32;
-    long long *d_bfl_idx = nullptr;
-    int n_bfl = 0;
+    // them): rows per strip of that kernel (0: chosen from its occupancy) and threads
+    // per workgroup
+    int two_step = 1, rows2 = 0, block2 = 256;
+    // partial sums of the boundary-flux line workgroups of the step launches (16
+    // doubles per workgroup, room for bf_cap of them)
+    double *d_bfpart = nullptr;
+    int bf_cap = 0;
 
     // staggered leapfrog family (time kept on the host, fixed timestep)
     bool leapfrog() const

@@ -51,7 +51,31 @@ struct DevScalars {
     double stage_dt;           // step applied by the current update_U (dt_last_update)
     double evolve_dt;          // evolve_step_dt = time - time0
     unsigned int bc_done;      // blocks of the averaging stage boundary done (0 between launches)
+    unsigned int bf_done;      // boundary-flux line workgroups done (0 between launches)
 };
+
+// Boundary-flux sums of a leapfrog step taken by extra workgroups of the step launch
+// (swh_bflux.h): the caller asks for 'want' of them (0: none), the launcher places
+// ey = bf_rows() rows of its grid's width ahead of the tiles. part: the workgroups'
+// partial sums, room for 'cap' workgroups of 4 doubles; t_after: the end time of
+// each step of the launch (NaN: from the device time).
+struct BfLines {
+    int want = 0, cap = 0, ey = 0, n_ext = 2;
+    const double *dle = nullptr, *dbe = nullptr;
+    DevScalars *sc = nullptr;
+    double *part = nullptr;
+    double dt = 0.0;
+    double t_after[2] = {0.0, 0.0};
+};
+// Rows of line workgroups for a launch gx workgroups wide: enough for 'want' of them
+// (at least 4: one per side), within 'cap'; -1 if they do not fit.
+inline int bf_rows(const BfLines &b, int gx)
+{
+    if (!b.want) return 0;
+    if (gx < 1 || b.want < 4) return -1;
+    const int ey = (b.want + gx - 1) / gx;
+    return ey * gx <= b.cap ? ey : -1;
+}
 
 // Bookkeeping of one finite-volume Euler stage (fv_scalar_kernel), after
 // one_euler_step / one_midpoint_step / one_rk2n_step
@@ -126,6 +150,8 @@ struct LFArgs {
     int g_level = 0, g_sample = 0;
     const int *g_ij = nullptr;
     double *g_val = nullptr;
+    // single grid: the step's boundary-flux sums in extra workgroups of this launch
+    BfLines bf;
 };
 
 // Kernel launchers (defined in the .hip files)
@@ -143,12 +169,6 @@ hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode,
 template <typename T, bool KAHAN>
 hipError_t launch_leapfrogn(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                             int linear_friction, int nsrc, int block_x, hipStream_t s, int prefetch, int ns);
-
-// UH/VH one step ahead at cells idx of a single grid (src_stride 0), written into
-// a.UHo / a.VHo there; friction modes 0 and 1, no boundary stage.
-template <typename T, bool KAHAN>
-hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                             int linear_friction, int n, const long long *idx, hipStream_t s);
 
 // Plain streaming copy of 'bytes' (a multiple of 16) for measurement.
 hipError_t launch_probe_copy(const void *src, void *dst, long long bytes, hipStream_t s);
@@ -283,11 +303,13 @@ hipError_t launch_superpose(const SupArgs &a, int mode, hipStream_t s);
 // value; flags must start at 0.
 hipError_t launch_nonfinite_slots(const double *X, int nsrc, long long slot_stride, int *flags, hipStream_t s);
 
-// Gauge values after one step, from the current state (batched runs).
+// Gauge values after one step, from the current state (batched runs, or a single grid
+// whose sample falls between the two steps of a two-step launch); times: null, or
+// times[sample] = t.
 template <typename T, bool KAHAN>
 hipError_t launch_gauge_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                               int linear_friction, int nsrc, int n, const long long *idx, double *values, int gmax,
-                              int sample, hipStream_t s);
+                              int sample, double *times, double t, hipStream_t s);
 
 template <typename T>
 hipError_t launch_bflux_leapfrog(const T *UH, const T *VH, int nx, int ny, int pitch, int n_ext,

@@ -17,7 +17,7 @@
 // than exchanged, so a launch touches each field once from HBM (neighbour columns hit
 // in L1/L2) and needs no barriers or LDS. The stencil has no dense-matrix structure,
 // so there is no MFMA here: the kernel is HBM-bound.
-#include "swh_internal.h"
+#include "swh_bflux.h"
 
 namespace swh {
 
@@ -189,13 +189,19 @@ __device__ __forceinline__ T dminT(T x, T y) { return x < y ? x : y; }
 template <typename T, int VEC, bool PF, bool NT, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
 {
+    // single grid: the first a.bf.ey rows of workgroups take the step's boundary-flux
+    // sums (of the input UH/VH) and write nothing else
+    if (a.bf.ey && (int)blockIdx.y < a.bf.ey) {
+        bf_lines<T, 1>(a.bf, a.nx, a.ny, a.pitch, a.UH, a.VH, [](int, int, T &, T &) {});
+        return;
+    }
     // Block order: with src_fast, consecutive workgroups are the same tile of
     // consecutive sources (they share the elevation tile in L2).
     const int bsrc = a.src_fast ? blockIdx.x : blockIdx.z;
     const int bxt = a.src_fast ? blockIdx.z : blockIdx.x;
     const int i = (bxt * blockDim.x + threadIdx.x) * VEC;
     if (i >= a.nx) return;
-    const int j0 = blockIdx.y * a.rows_per_block;
+    const int j0 = ((int)blockIdx.y - a.bf.ey) * a.rows_per_block;
     if (j0 >= a.ny) return;
     const int j1 = min(j0 + a.rows_per_block, a.ny);
     const long long so = (long long)(bsrc + a.src_base) * a.src_stride;
@@ -414,11 +420,16 @@ template hipError_t launch_lf_stage<double, false>(const LFArgs<double> &, hipSt
 template hipError_t launch_lf_stage<float, true>(const LFArgs<float> &, hipStream_t);
 
 template <typename T, int VEC, bool PF, bool NT, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-static hipError_t launch_lf(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s)
+static hipError_t launch_lf(const LFArgs<T> &a0, int nsrc, int block_x, hipStream_t s)
 {
     dim3 block(block_x);
-    const int lanes = (a.nx + VEC - 1) / VEC;
-    const int gx = (lanes + block_x - 1) / block_x, gy = (a.ny + a.rows_per_block - 1) / a.rows_per_block;
+    const int lanes = (a0.nx + VEC - 1) / VEC;
+    const int gx = (lanes + block_x - 1) / block_x;
+    LFArgs<T> a = a0;
+    a.bf.ey = bf_rows(a.bf, gx);
+    if (a.bf.ey < 0 || (a.bf.ey && (nsrc != 1 || a.src_fast || a.src_stride != 0 || !a.bf.part || !a.bf.sc)))
+        return hipErrorInvalidValue;
+    const int gy = (a.ny + a.rows_per_block - 1) / a.rows_per_block + a.bf.ey;
     dim3 grid = a.src_fast ? dim3(nsrc, gy, gx) : dim3(gx, gy, nsrc);
     hipLaunchKernelGGL((leapfrog_kernel<T, VEC, PF, NT, KAHAN, COR, FRIC, TL, LINF>), grid, block, 0, s, a);
     return hipGetLastError();

@@ -15,7 +15,11 @@
 // from the adjacent lanes (cross-lane permutes, no LDS tile, no barriers); the
 // stencil reaches one column each way per step, which the 3 halo lanes cover.
 // Rows below/above the strip are recomputed by the strip itself (3 extra rows).
-#include "swh_internal.h"
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+
+#include "swh_bflux.h"
 
 namespace swh {
 
@@ -155,13 +159,14 @@ __device__ __forceinline__ T mom_v(const LFArgs<T> &a, bool valid, T cv, T fv, T
 // strips, then groups. Neighbouring tiles of a strip, the sources of a group (which
 // read the same elevation and row coefficients) and the next strip (which reads the
 // halo rows of this one) thus run on the same L2 shortly after one another.
-__device__ __forceinline__ void lf2_tile(int order, int group, int &bx, int &by, int &bz)
+// ey: rows of workgroups ahead of the tiles that do other work (single grid only).
+__device__ __forceinline__ void lf2_tile(int order, int group, int ey, int &bx, int &by, int &bz)
 {
-    bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z;
+    bx = blockIdx.x; by = (int)blockIdx.y - ey; bz = blockIdx.z;
     if (!order) return;
-    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned gx = gridDim.x, gy = gridDim.y - ey, gz = gridDim.z;
     const unsigned B = gx * gy * gz;
-    const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned b = blockIdx.x + gx * ((blockIdx.y - ey) + gy * blockIdx.z);
     const unsigned xcd = b % 8u, slot = b / 8u, n8 = B / 8u, rem = B % 8u;
     const unsigned L = xcd < rem ? xcd * (n8 + 1u) + slot : rem * (n8 + 1u) + (xcd - rem) * n8 + slot;
     const unsigned G = (unsigned)group;
@@ -174,13 +179,28 @@ __device__ __forceinline__ void lf2_tile(int order, int group, int &bx, int &by,
     by = (int)(t / Gi);
 }
 
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+__device__ __forceinline__ void step1_cell(const LFArgs<T> &a, const T *__restrict__ S, const T *__restrict__ C,
+                                           const T *__restrict__ UH, const T *__restrict__ VH, int i, int j, T &s_c,
+                                           T &c_c, T &uh1, T &vh1);
+
 // PF: the loads of the next row are issued before the current row is computed, so
 // their latency overlaps the arithmetic (one row of loads in flight per lane).
+// a.bf.ey > 0 (single grid): the first a.bf.ey rows of workgroups take the boundary-flux
+// sums of both steps (the second from UH/VH one step ahead on the lines) and write
+// nothing else.
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool PF>
 __global__ __launch_bounds__(256) void leapfrog2_kernel(LFArgs<T> a)
 {
+    if (a.bf.ey && (int)blockIdx.y < a.bf.ey) {
+        bf_lines<T, 2>(a.bf, a.nx, a.ny, a.pitch, a.UH, a.VH, [&](int i, int j, T &uh1, T &vh1) {
+            T s_c, c_c;
+            step1_cell<T, KAHAN, COR, FRIC, TL, LINF>(a, a.S, a.C, a.UH, a.VH, i, j, s_c, c_c, uh1, vh1);
+        });
+        return;
+    }
     int bx, by, bz;
-    lf2_tile(a.lf2_order, a.lf2_group, bx, by, bz);
+    lf2_tile(a.lf2_order, a.lf2_group, a.bf.ey, bx, by, bz);
     const int lane = threadIdx.x & 63;
     const int wg = bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wg * kOwn >= a.nx) return;                       // whole wavefront past the east edge
@@ -333,7 +353,7 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
 {
     constexpr int OWN = 65 - 2 * NS;
     int bx, by, bz;
-    lf2_tile(a.lf2_order, a.lf2_group, bx, by, bz);
+    lf2_tile(a.lf2_order, a.lf2_group, 0, bx, by, bz);
     const int lane = threadIdx.x & 63;
     const int wg = bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wg * OWN >= a.nx) return;                        // whole wavefront past the east edge
@@ -564,12 +584,15 @@ __device__ __forceinline__ void step1_cell(const LFArgs<T> &a, const T *__restri
 }
 
 // Gauge values one step ahead (so a two-step launch can pass over a sampling step).
-// One lane per (gauge, source); values[((slot * gmax + sample) * 3 + var) * n + g].
+// One lane per (gauge, source); values[((slot * gmax + sample) * 3 + var) * n + g];
+// times (single grid, or null): times[sample] = t.
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 __global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, const long long *__restrict__ idx,
-                                                         double *values, int gmax, int sample)
+                                                         double *values, int gmax, int sample, double *times,
+                                                         double t)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (times && g == 0 && blockIdx.y == 0) times[sample] = t;
     if (g >= n) return;
     const int slot = blockIdx.y + a.src_base;
     const long long so = (long long)slot * a.src_stride;
@@ -583,101 +606,105 @@ __global__ __launch_bounds__(64) void gauge_step1_kernel(LFArgs<T> a, int n, con
     v[2 * n + g] = (double)vh1;
 }
 
-// UH and VH one step ahead at the given cells of a single grid, written into the
-// output planes there (the next step's boundary-flux lines, read before a two-step
-// launch overwrites the planes).
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-__global__ __launch_bounds__(64) void line_step1_kernel(LFArgs<T> a, int n, const long long *__restrict__ idx)
-{
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const long long o = idx[g];
-    const int i = (int)(o % a.pitch), j = (int)(o / a.pitch);
-    T s_c, c_c, uh1, vh1;
-    step1_cell<T, KAHAN, COR, FRIC, TL, LINF>(a, a.S, a.C, a.UH, a.VH, i, j, s_c, c_c, uh1, vh1);
-    a.UHo[o] = uh1;
-    a.VHo[o] = vh1;
-}
-
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-static hipError_t launch_l1(const LFArgs<T> &a, int n, const long long *idx, hipStream_t s)
-{
-    hipLaunchKernelGGL((line_step1_kernel<T, KAHAN, COR, FRIC, TL, LINF>), dim3((n + 63) / 64), dim3(64), 0, s, a, n,
-                       idx);
-    return hipGetLastError();
-}
-
-template <typename T, bool KAHAN, bool COR, int FRIC>
-static hipError_t l1_tl(const LFArgs<T> &a, int tl, int linf, int n, const long long *idx, hipStream_t s)
-{
-    if (tl) return linf ? launch_l1<T, KAHAN, COR, FRIC, true, true>(a, n, idx, s)
-                        : launch_l1<T, KAHAN, COR, FRIC, true, false>(a, n, idx, s);
-    return linf ? launch_l1<T, KAHAN, COR, FRIC, false, true>(a, n, idx, s)
-                : launch_l1<T, KAHAN, COR, FRIC, false, false>(a, n, idx, s);
-}
-
-template <typename T, bool KAHAN>
-hipError_t launch_line_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
-                             int linear_friction, int n, const long long *idx, hipStream_t s)
-{
-    if (friction_mode > 1 || a.bc_mask || n <= 0 || a.src_stride != 0) return hipErrorInvalidValue;
-    const int linf = (friction_mode == 0) && linear_friction;
-    if (coriolis)
-        return friction_mode ? l1_tl<T, KAHAN, true, 1>(a, truely_linear, linf, n, idx, s)
-                             : l1_tl<T, KAHAN, true, 0>(a, truely_linear, linf, n, idx, s);
-    return friction_mode ? l1_tl<T, KAHAN, false, 1>(a, truely_linear, linf, n, idx, s)
-                         : l1_tl<T, KAHAN, false, 0>(a, truely_linear, linf, n, idx, s);
-}
-
-template hipError_t launch_line_step1<double, false>(const LFArgs<double> &, int, int, int, int, int,
-                                                     const long long *, hipStream_t);
-template hipError_t launch_line_step1<float, true>(const LFArgs<float> &, int, int, int, int, int, const long long *,
-                                                   hipStream_t);
-
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
 static hipError_t launch_g1(const LFArgs<T> &a, int nsrc, int n, const long long *idx, double *values, int gmax,
-                            int sample, hipStream_t s)
+                            int sample, double *times, double t, hipStream_t s)
 {
     hipLaunchKernelGGL((gauge_step1_kernel<T, KAHAN, COR, FRIC, TL, LINF>), dim3((n + 63) / 64, nsrc), dim3(64), 0, s,
-                       a, n, idx, values, gmax, sample);
+                       a, n, idx, values, gmax, sample, times, t);
     return hipGetLastError();
 }
 
 template <typename T, bool KAHAN, bool COR, int FRIC>
 static hipError_t g1_tl(const LFArgs<T> &a, int tl, int linf, int nsrc, int n, const long long *idx, double *values,
-                        int gmax, int sample, hipStream_t s)
+                        int gmax, int sample, double *times, double t, hipStream_t s)
 {
-    if (tl) return linf ? launch_g1<T, KAHAN, COR, FRIC, true, true>(a, nsrc, n, idx, values, gmax, sample, s)
-                        : launch_g1<T, KAHAN, COR, FRIC, true, false>(a, nsrc, n, idx, values, gmax, sample, s);
-    return linf ? launch_g1<T, KAHAN, COR, FRIC, false, true>(a, nsrc, n, idx, values, gmax, sample, s)
-                : launch_g1<T, KAHAN, COR, FRIC, false, false>(a, nsrc, n, idx, values, gmax, sample, s);
+    if (tl)
+        return linf ? launch_g1<T, KAHAN, COR, FRIC, true, true>(a, nsrc, n, idx, values, gmax, sample, times, t, s)
+                    : launch_g1<T, KAHAN, COR, FRIC, true, false>(a, nsrc, n, idx, values, gmax, sample, times, t, s);
+    return linf ? launch_g1<T, KAHAN, COR, FRIC, false, true>(a, nsrc, n, idx, values, gmax, sample, times, t, s)
+                : launch_g1<T, KAHAN, COR, FRIC, false, false>(a, nsrc, n, idx, values, gmax, sample, times, t, s);
 }
 
 template <typename T, bool KAHAN>
 hipError_t launch_gauge_step1(const LFArgs<T> &a, int coriolis, int friction_mode, int truely_linear,
                               int linear_friction, int nsrc, int n, const long long *idx, double *values, int gmax,
-                              int sample, hipStream_t s)
+                              int sample, double *times, double t, hipStream_t s)
 {
     if (friction_mode > 1 || a.bc_mask || n <= 0 || nsrc <= 0) return hipErrorInvalidValue;
     const int linf = (friction_mode == 0) && linear_friction;
     if (coriolis) {
-        return friction_mode ? g1_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s)
-                             : g1_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s);
+        return friction_mode
+                   ? g1_tl<T, KAHAN, true, 1>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, times, t, s)
+                   : g1_tl<T, KAHAN, true, 0>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, times, t, s);
     }
-    return friction_mode ? g1_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s)
-                         : g1_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, s);
+    return friction_mode
+               ? g1_tl<T, KAHAN, false, 1>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, times, t, s)
+               : g1_tl<T, KAHAN, false, 0>(a, truely_linear, linf, nsrc, n, idx, values, gmax, sample, times, t, s);
 }
 
 template hipError_t launch_gauge_step1<double, false>(const LFArgs<double> &, int, int, int, int, int, int,
-                                                      const long long *, double *, int, int, hipStream_t);
+                                                      const long long *, double *, int, int, double *, double,
+                                                      hipStream_t);
 template hipError_t launch_gauge_step1<float, true>(const LFArgs<float> &, int, int, int, int, int, int,
-                                                    const long long *, double *, int, int, hipStream_t);
+                                                    const long long *, double *, int, int, double *, double,
+                                                    hipStream_t);
+
+// Compute units of the current device (cached per device).
+static int device_cus()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// Rows per strip of a single grid when the caller leaves them open (rows_per_block <= 0).
+// The kernel is latency-bound (each wavefront marches its strip row by row, one row of
+// loads in flight), so it wants as many resident wavefronts as fit in one round of
+// workgroups, while each strip recomputes 3 rows of halo: the fewest rows whose launch
+// fills about 70 % of the device's resident-wave capacity for this kernel (workgroups
+// per CU from its register use). Measured on a 2048 x 2048 grid: 12-16 rows for the
+// FP32 kernel, 24 for the FP64 one, against 32 for a fixed 2048-wavefront target.
+template <typename K>
+static int lf2_auto_rows(K kernel, int nx, int ny, int block_x)
+{
+    static std::atomic<int> occ[5];   // workgroups per CU, by block_x / 64
+    const int wpb = block_x / 64;
+    int nb = occ[wpb].load(std::memory_order_relaxed);
+    if (nb <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)kernel, block_x, 0) != hipSuccess ||
+            nb <= 0)
+            nb = 1;
+        occ[wpb].store(nb, std::memory_order_relaxed);
+    }
+    const double target = 0.7 * nb * wpb * device_cus();
+    const double waves_x = (double)((nx + kOwn - 1) / kOwn);
+    const int rows = (int)std::ceil(ny * waves_x / target);
+    return std::min(std::max(rows, 4), 1024);
+}
 
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
-static hipError_t launch_lf2(const LFArgs<T> &a, int nsrc, int block_x, hipStream_t s, int pf)
+static hipError_t launch_lf2(const LFArgs<T> &a0, int nsrc, int block_x, hipStream_t s, int pf)
 {
-    const int waves = (a.nx + kOwn - 1) / kOwn, wpb = block_x / 64;
-    dim3 grid((waves + wpb - 1) / wpb, (a.ny + a.rows_per_block - 1) / a.rows_per_block, nsrc);
+    const int waves = (a0.nx + kOwn - 1) / kOwn, wpb = block_x / 64;
+    const int gx = (waves + wpb - 1) / wpb;
+    LFArgs<T> a = a0;
+    a.bf.ey = bf_rows(a.bf, gx);
+    if (a.bf.ey < 0) return hipErrorInvalidValue;
+    if (a.rows_per_block <= 0) {
+        if (nsrc != 1) return hipErrorInvalidValue;
+        a.rows_per_block = pf ? lf2_auto_rows(leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF, true>, a.nx, a.ny,
+                                              block_x)
+                              : lf2_auto_rows(leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF, false>, a.nx, a.ny,
+                                              block_x);
+    }
+    dim3 grid(gx, (a.ny + a.rows_per_block - 1) / a.rows_per_block + a.bf.ey, nsrc);
     if (pf)
         hipLaunchKernelGGL((leapfrog2_kernel<T, KAHAN, COR, FRIC, TL, LINF, true>), grid, dim3(block_x), 0, s, a);
     else
@@ -699,7 +726,8 @@ hipError_t launch_leapfrog2(const LFArgs<T> &a, int coriolis, int friction_mode,
                             int linear_friction, int nsrc, int block_x, hipStream_t s, int pf)
 {
     if (friction_mode > 1 || a.bc_mask || a.maxq_mask || (block_x % 64) || block_x < 64 || block_x > 256 ||
-        (long long)a.pitch * a.ny >= (1LL << 31))
+        (long long)a.pitch * a.ny >= (1LL << 31) ||
+        (a.bf.want && (nsrc != 1 || a.src_stride != 0 || !a.bf.part || !a.bf.sc)))
         return hipErrorInvalidValue;
     const int linf = (friction_mode == 0) && linear_friction;
     if (coriolis) {

@@ -4,29 +4,9 @@
 // two-sum compensation term, so results do not depend on scheduling.
 #include <algorithm>
 
-#include "swh_internal.h"
+#include "swh_bflux.h"
 
 namespace swh {
-
-// ---- compensated (double-double) accumulation ----
-struct DD {
-    double hi, lo;
-};
-
-__device__ __forceinline__ DD dd_add(DD a, double b)
-{
-    const double s = a.hi + b;
-    const double bp = s - a.hi;
-    const double err = (a.hi - (s - bp)) + (b - bp);
-    return DD{s, a.lo + err};
-}
-
-__device__ __forceinline__ DD dd_add(DD a, DD b)
-{
-    DD r = dd_add(a, b.hi);
-    r.lo += b.lo;
-    return r;
-}
 
 // Reduce one DD per thread over a block of blockDim.x threads (power of two <= 256).
 __device__ DD block_reduce_dd(DD v)

@@ -231,6 +231,12 @@ class Domain:
         B.check(self._lib.swh_domain_set_launch(self._h, int(block_x), int(rows_per_block), int(variant)),
                 "swh_domain_set_launch")
 
+    def set_launch2(self, block_x=256, rows_per_block=0):
+        """Launch shape of the two-step leapfrog kernel of evolve_steps (results do not
+        depend on it); rows_per_block 0: chosen from the kernel's occupancy."""
+        B.check(self._lib.swh_domain_set_launch2(self._h, int(block_x), int(rows_per_block)),
+                "swh_domain_set_launch2")
+
     # -- diagnostics --
     def scalars(self):
         s = B.Scalars()

@@ -346,6 +346,74 @@ def test_single_grid_two_step_bitwise(oracle_built, case, precision):
         assert np.array_equal(u0, o.get_U())
 
 
+@pytest.mark.parametrize("shape", [(64, 4), (128, 7), (192, 13), (256, 64), (64, 100), (128, 0), (256, 0)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_two_step_launch_shapes_do_not_change_results(oracle_built, shape, precision):
+    """Every launch shape of the two-step kernel (threads per workgroup, rows per strip;
+    with them the number of boundary-flux line workgroups and the rows they take ahead
+    of the tiles) gives the one-step-per-launch run's state and scalars bit for bit."""
+    _print_case("two-step shape %s %s" % (shape, precision))
+    cfg = H.small_spherical(nx=91, ny=77)
+    U = cfg["U"].copy()
+    U[:3] = np.random.default_rng(11).standard_normal((3, 77, 91))
+    cfg = dict(cfg, U=U)
+    o, dt, _ = H.cpu_run("oracle", cfg, 0)
+    runs = []
+    for variant, sh in ((2 | 32, None), (2, shape)):
+        g = H.gpu_domain(cfg, precision=precision)
+        g.set_launch(variant=variant)
+        if sh:
+            g.set_launch2(*sh)
+        g.evolve_steps(dt, 30)
+        runs.append((g.U, g.scalars()))
+        g.close()
+    (u0, s0), (u1, s1) = runs
+    assert np.array_equal(u0, u1)
+    for k in s0:
+        assert s0[k] == s1[k], k
+
+
+@pytest.mark.parametrize("case", ["linear_sph", "llpnf_sph"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_scheduled_gauges_in_step_pairs_bitwise(oracle_built, case, precision):
+    """swh_evolve_record_gauges_schedule and swh_evolve_record_gauges keep stepping in
+    pairs whatever the sample spacing: a sample due between the two steps of a pair is
+    taken one step ahead from the pair's input state. Gauge values and times, the state
+    and every scalar (boundary-flux integrals taken inside the step launches) equal the
+    one-step-per-launch run bit for bit; FP64 also equals the oracle's state."""
+    _print_case("gauge schedule pairs %s %s" % (case, precision))
+    from ptha_amd import backend as B
+    cfg = H.small_spherical(nx=91, ny=77)
+    if case.startswith("llpnf"):
+        cfg = dict(cfg, method="leapfrog_linear_plus_nonlinear_friction",
+                   manning_squared=np.full((cfg["ny"], cfg["nx"]), 0.03 ** 2))
+    o, dt, _ = H.cpu_run("oracle", cfg, 0)
+    ij = np.array([[1, 1], [3, 3], [45, 40], [90, 2], [88, 76], [91, 77], [2, 60]])
+    sched = np.array([0, 1, 4, 5, 6, 9, 12, 13, 17, 18, 22, 25], dtype=np.int64)
+    runs = []
+    for variant in (2, 2 | 32):
+        g = H.gpu_domain(cfg, precision=precision)
+        g.set_launch(variant=variant)
+        g.setup_point_gauges(ij=ij, max_samples=64)
+        B.check(g._lib.swh_evolve_record_gauges_schedule(g._h, dt, len(sched), sched.ctypes.data_as(B._PI64)),
+                "swh_evolve_record_gauges_schedule")
+        # then 14 steps sampled every 3 (samples at 0, 3, ..., 12 of them, then 2 steps)
+        B.check(g._lib.swh_evolve_record_gauges(g._h, dt, 14, 3), "swh_evolve_record_gauges")
+        t, v = g.gauge_series()
+        runs.append((t, v, g.U, g.scalars()))
+        g.close()
+    (t0, v0, u0, s0), (t1, v1, u1, s1) = runs
+    assert len(t0) == len(sched) + 5
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(v0, v1)
+    assert np.array_equal(u0, u1)
+    for k in s0:
+        assert s0[k] == s1[k], k
+    if precision == "fp64":
+        o, _, _ = H.cpu_run("oracle", cfg, int(sched[-1]) + 14, dt=dt)
+        assert np.array_equal(u0, o.get_U())
+
+
 @pytest.mark.parametrize("shape", [(256, 32, 0), (64, 7, 3), (128, 64, 1), (192, 5, 2), (64, 9, 7), (256, 3, 6),
                                    (128, 11, 5), (64, 16, 4)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32k"])
