@@ -345,6 +345,7 @@ struct AdjArgs {
     LFRowCoef<T> rc;
     T msl, lin_fric_mult;
     int coriolis;
+    int rows = 0;            // rows per strip of the fused kernel (0: from its occupancy)
 };
 
 template <typename T>

@@ -15,11 +15,10 @@
 // from the adjacent lanes (cross-lane permutes, no LDS tile, no barriers); the
 // stencil reaches one column each way per step, which the 3 halo lanes cover.
 // Rows below/above the strip are recomputed by the strip itself (3 extra rows).
-#include <algorithm>
 #include <atomic>
-#include <cmath>
 
 #include "swh_bflux.h"
+#include "swh_launch.h"
 
 namespace swh {
 
@@ -650,43 +649,14 @@ template hipError_t launch_gauge_step1<float, true>(const LFArgs<float> &, int, 
                                                     const long long *, double *, int, int, double *, double,
                                                     hipStream_t);
 
-// Compute units of the current device (cached per device).
-static int device_cus()
-{
-    static std::atomic<int> cache[64];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    int n = cache[dev].load(std::memory_order_relaxed);
-    if (n <= 0) {
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cache[dev].store(n, std::memory_order_relaxed);
-    }
-    return n;
-}
-
-// Rows per strip of a single grid when the caller leaves them open (rows_per_block <= 0).
-// The kernel is latency-bound (each wavefront marches its strip row by row, one row of
-// loads in flight), so it wants as many resident wavefronts as fit in one round of
-// workgroups, while each strip recomputes 3 rows of halo: the fewest rows whose launch
-// fills about 70 % of the device's resident-wave capacity for this kernel (workgroups
-// per CU from its register use). Measured on a 2048 x 2048 grid: 12-16 rows for the
-// FP32 kernel, 24 for the FP64 one, against 32 for a fixed 2048-wavefront target.
+// Rows per strip of a single grid when the caller leaves them open (rows_per_block <= 0):
+// see occupancy_rows. Measured on a 2048 x 2048 grid: 12-16 rows for the FP32 kernel, 24
+// for the FP64 one, against 32 for a fixed 2048-wavefront target.
 template <typename K>
 static int lf2_auto_rows(K kernel, int nx, int ny, int block_x)
 {
     static std::atomic<int> occ[5];   // workgroups per CU, by block_x / 64
-    const int wpb = block_x / 64;
-    int nb = occ[wpb].load(std::memory_order_relaxed);
-    if (nb <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)kernel, block_x, 0) != hipSuccess ||
-            nb <= 0)
-            nb = 1;
-        occ[wpb].store(nb, std::memory_order_relaxed);
-    }
-    const double target = 0.7 * nb * wpb * device_cus();
-    const double waves_x = (double)((nx + kOwn - 1) / kOwn);
-    const int rows = (int)std::ceil(ny * waves_x / target);
-    return std::min(std::max(rows, 4), 1024);
+    return occupancy_rows((const void *)kernel, occ, nx, ny, block_x, kOwn, 3);
 }
 
 template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
