@@ -71,9 +71,37 @@ def time_steps(b, dt, steps, warmup):
     return wall, ms
 
 
+REF_BUILDS = {
+    "ref_fast": "amdflang -O3 -ffast-math -fopenmp (the reference's own default, -Ofast)",
+    "ref": "amdflang -O2 -ffp-contract=off -fopenmp (the FP64 parity build)",
+}
+
+
 def cpu_baseline_leg(cfg, dt, budget_s):
-    """Times the compiled reference solver (oracle/_ref/libswals_ref_sph.so, OpenMP) on
-    one unit source of the same grid, in a child process so OMP_NUM_THREADS applies."""
+    """Times the compiled reference solver on one unit source of the same grid, built two
+    ways (oracle/_ref/libswals_ref_sph_fast.so with the reference's own default
+    optimisation, and the parity build libswals_ref_sph.so), each for half the budget in
+    a child process so OMP_NUM_THREADS applies; the faster build is the baseline and
+    both figures are in the sample text."""
+    runs = {}
+    for kind in REF_BUILDS:
+        try:
+            runs[kind] = _cpu_ref_run(kind, dt, budget_s / len(REF_BUILDS))
+        except Exception as e:  # a build may be absent from a tree
+            log("cpu baseline (%s) unavailable: %s" % (kind, e))
+    if not runs:
+        raise RuntimeError("no reference build could be timed")
+    best = max(runs, key=lambda k: runs[k]["gcell"])
+    o = runs[best]
+    others = "; ".join("%s: %.4f Gcell/s over %d steps (%.1f s)" % (REF_BUILDS[k], r["gcell"], r["steps"], r["seconds"])
+                       for k, r in runs.items())
+    return dict(value=o["gcell"], unit="Gcell-updates/s", cores=o["cores"], kind="reference",
+                sample="compiled reference SWALS 'linear' spherical+Coriolis leapfrog (OpenMP), one unit source "
+                       "of the 1200x1200 1-arcmin grid, no max_U/gauges; value from the faster build. "
+                       "Builds timed: %s. CPU model: %s" % (others, cpu_model()))
+
+
+def _cpu_ref_run(kind, dt, budget_s):
     code = r"""
 import json, os, sys, time, math
 import numpy as np
@@ -82,7 +110,7 @@ import pyoracle
 from ptha_amd import configs as C
 cfg = C.unit_sources()
 nx, ny = cfg['nx'], cfg['ny']
-d = pyoracle.CpuDomain('ref', nx, ny, cfg['lw'], cfg['ll'], method='linear', spherical=True)
+d = pyoracle.CpuDomain(%(kind)r, nx, ny, cfg['lw'], cfg['ll'], method='linear', spherical=True)
 d.set_record_max_U(False)
 U = np.zeros((4, ny, nx)); U[3] = cfg['elev']
 U[0] = C.gaussian_on_sphere(cfg['x'], cfg['y'], *cfg['centres'][0], cfg['sigma_km']*1e3)
@@ -93,7 +121,7 @@ t0 = time.perf_counter(); d.evolve(dt, 20); t1 = time.perf_counter()
 n = max(20, int(%(budget)r / max(t1 - t0, 1e-9) * 20))
 t0 = time.perf_counter(); d.evolve(dt, n); el = time.perf_counter() - t0
 print(json.dumps(dict(steps=n, seconds=el, cells=nx*ny)))
-""" % dict(root=ROOT, dt=dt, budget=budget_s)
+""" % dict(root=ROOT, dt=dt, budget=budget_s, kind=kind)
     env = dict(os.environ)
     ncores = int(env.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     env["OMP_NUM_THREADS"] = str(ncores)
@@ -102,11 +130,8 @@ print(json.dumps(dict(steps=n, seconds=el, cells=nx*ny)))
     if r.returncode != 0:
         raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
     o = json.loads(r.stdout.strip().splitlines()[-1])
-    gcell = o["cells"] * o["steps"] / o["seconds"] / 1e9
-    return dict(value=gcell, unit="Gcell-updates/s", cores=ncores, kind="reference",
-                sample="compiled reference SWALS (amdflang -O2, OpenMP) 'linear' spherical+Coriolis leapfrog, "
-                       "one unit source of the 1200x1200 1-arcmin grid, %d steps (%.1f s), no max_U/gauges; "
-                       "CPU model: %s" % (o["steps"], o["seconds"], cpu_model()))
+    return dict(gcell=o["cells"] * o["steps"] / o["seconds"] / 1e9, steps=o["steps"], seconds=o["seconds"],
+                cores=ncores)
 
 
 def cpu_model():
@@ -223,7 +248,9 @@ def secondary_legs(steps):
     from ptha_amd.domain import build_metrics
     m = build_metrics(ca["nx"], ca["ny"], ca["lw"], ca["ll"], True)
     dt = linear_timestep(ca["elev"], m)
-    for prec, bpc in (("fp32k", 36.0), ("fp64", 56.0)):
+    # compulsory bytes per cell and launch: read S, (C,) UH, VH, E, write S, (C,) UH, VH;
+    # swh_evolve_steps takes two steps per launch, so per cell-update half of that
+    for prec, bpc in (("fp32k", 36.0 / 2), ("fp64", 56.0 / 2)):
         d = Domain(ca["nx"], ca["ny"], ca["lw"], ca["ll"], "linear", precision=prec, spherical=True,
                    max_U_variables=())
         U = np.zeros((4, ca["ny"], ca["nx"]))
@@ -237,7 +264,8 @@ def secondary_legs(steps):
         wall = time.perf_counter() - t0
         g = ca["nx"] * ca["ny"] * steps / wall / 1e9
         out["linear_2048sq_" + prec] = dict(gcell_per_s=round(g, 3), ms_per_step=round(1e3 * wall / steps, 4),
-                                            hbm_frac=round(g * bpc / HBM_PEAK_GBS, 4))
+                                            hbm_frac=round(g * bpc / HBM_PEAK_GBS, 4), steps_per_launch=2,
+                                            bytes_per_cell_update=bpc)
         d.close()
     # the nonlinear leapfrog on the same grid (Manning 0.025, a 1 m hump), stationary CFL step
     X, Y = np.meshgrid(ca["x"], ca["y"])
@@ -259,10 +287,13 @@ def secondary_legs(steps):
         out["nonlinear_2048sq_" + prec] = dict(gcell_per_s=round(ca["nx"] * ca["ny"] * steps / wall / 1e9, 3),
                                                ms_per_step=round(1e3 * wall / steps, 4))
         d.close()
-    # adjoint source inversion grid: forward (gauges every 17 steps, checkpoints every
-    # 256) and the adjoint sweep, FP32 + Kahan
+    # adjoint source inversion grid: forward (gauges every 17 steps, device checkpoints
+    # every 256) and the adjoint sweep, FP32 + Kahan, 2000 steps. An inversion calls both
+    # once per iteration on the same map: the first calls (which also allocate the
+    # checkpoints and the adjoint state and upload the elevation) are reported apart from
+    # repeated ones.
     from ptha_amd.adjoint import GaugeMap
-    nsteps = 4 * steps
+    nsteps = 2000
     gm = GaugeMap(ca["nx"], ca["ny"], ca["lw"], ca["ll"], ca["elev"], dt, nsteps, 17, ca["gauges_xy"],
                   precision="fp32k", spherical=True)
     X, Y = np.meshgrid(ca["x"], ca["y"])
@@ -272,9 +303,16 @@ def secondary_legs(steps):
     t1 = time.perf_counter()
     lam = gm.adjoint(np.ones_like(g))
     t2 = time.perf_counter()
+    g = gm.forward(eta0)
+    t3 = time.perf_counter()
+    lam = gm.adjoint(np.ones_like(g))
+    t4 = time.perf_counter()
     cells = ca["nx"] * ca["ny"] * nsteps
-    out["adjoint_2048sq_fp32k"] = dict(forward_gcell_per_s=round(cells / (t1 - t0) / 1e9, 3),
-                                        adjoint_gcell_per_s=round(cells / (t2 - t1) / 1e9, 3), steps=nsteps)
+    out["adjoint_2048sq_fp32k"] = dict(forward_gcell_per_s=round(cells / (t3 - t2) / 1e9, 3),
+                                        adjoint_gcell_per_s=round(cells / (t4 - t3) / 1e9, 3),
+                                        first_forward_gcell_per_s=round(cells / (t1 - t0) / 1e9, 3),
+                                        first_adjoint_gcell_per_s=round(cells / (t2 - t1) / 1e9, 3), steps=nsteps,
+                                        gauge_every=17)
     assert np.isfinite(lam).all()
     gm.close()
     return out

@@ -31,6 +31,10 @@ _libs = {}
 def lib_path(kind, spherical=False):
     if kind == "oracle":
         return os.path.join(REF_DIR, "liboracle.so")
+    if kind == "ref_fast":
+        # the spherical reference built with its own default optimisation (-O3 and
+        # fast-math, as -Ofast), for CPU timing only: not a parity checker
+        return os.path.join(REF_DIR, "libswals_ref_sph_fast.so")
     return os.path.join(REF_DIR, "libswals_ref_%s.so" % ("sph" if spherical else "cart"))
 
 
@@ -39,7 +43,7 @@ def available(kind, spherical=False):
 
 
 def _load(kind, spherical):
-    key = (kind, spherical if kind == "ref" else False)
+    key = (kind, spherical if kind != "oracle" else False)
     if key in _libs:
         return _libs[key]
     path = lib_path(kind, spherical)
@@ -79,7 +83,7 @@ def _load(kind, spherical):
     f("domain_dispersion_append", None, [hd])
     f("domain_dispersion_solve", None, [hd, _I, _I, _D])
     f("domain_set_record_max_U", None, [hd, _I])
-    if kind == "ref":
+    if kind != "oracle":
         f("domain_statistics", None, [hd, _PD])
         f("domain_sample", None, [hd, _I, ctypes.POINTER(_I), _PD])
         f("domain_set_forcing", None, [hd, _D, _D, _I, _I, _I, _I, _I, _I, _PD])
