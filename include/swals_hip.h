@@ -114,12 +114,17 @@ int swh_last_error(char *buf, int len);
 int swh_domain_create(const swh_domain_desc *desc, swh_domain **out);
 int swh_domain_destroy(swh_domain *h);
 
-/* Host <-> device copies of domain%U (domain_mod.f90:332) and fields. */
+/* Host <-> device copies of domain%U (domain_mod.f90:332) and fields. In FP32K the
+ * stage is stored with its Kahan term so that it represents the FP64 stage; where depth
+ * is stage - elevation (every method but the truely-linear leapfrog ones) it is taken on
+ * the FP32 elevation instead: a cell dry in FP64 gets the FP32 elevation as its stage
+ * (depth 0), a wet one its FP64 stage but no less than the FP32 elevation. */
 int swh_upload_U(swh_domain *h, const double *U);
 /* The state without the elevation: stage, UH, VH as (nx, ny) FP64 planes (any may be
  * NULL: zeros); the elevation, friction work and metrics stay as they are (a new
  * initial condition on the same grid, as the reference's set-up of U(:,:,STG:VH) for a
- * new run). In FP32K the stage is stored with its Kahan term, as swh_upload_U does. */
+ * new run). In FP32K the stage is stored as swh_upload_U stores it (on the FP32
+ * elevation uploaded with U where depth is stage - elevation). */
 int swh_upload_state(swh_domain *h, const double *stage, const double *uh, const double *vh);
 int swh_download_U(swh_domain *h, double *U);
 int swh_upload_manning_squared(swh_domain *h, const double *manning_squared); /* domain_mod.f90:347 */

@@ -88,6 +88,24 @@ int upload_plane_any(swh_domain *h, void *dst, const double *src, void *comp = n
                                     : upload_plane<float>(h, dst, src, comp);
 }
 
+// The FP32K stage to upload where depth is stage - elevation (every method but the
+// truely-linear leapfrog ones): the elevation is stored rounded to FP32, so the FP64
+// stage itself would leave a dry cell (stage = elevation) a depth of plus or minus that
+// rounding. A cell dry in FP64 (stage <= elevation) takes the FP32 elevation as its stage
+// (depth exactly 0, as the reference's); a wet one keeps its FP64 stage, at least the
+// FP32 elevation (rounding is monotonic, so no depth turns negative). elev: FP64
+// elevation (nx * ny); stage null: zeros.
+std::vector<double> stage_on_fp32_elevation(const swh_domain *h, const double *stage, const double *elev)
+{
+    const size_t n = (size_t)h->nx * h->ny;
+    std::vector<double> t(n);
+    for (size_t k = 0; k < n; k++) {
+        const double s = stage ? stage[k] : 0.0, e = elev[k], e32 = (double)(float)e;
+        t[k] = s <= e ? e32 : std::max(s, e32);
+    }
+    return t;
+}
+
 // friction_work for the truely-linear leapfrog + friction solver, built once on the
 // host in FP64 (domain_mod_leapfrog_solver_friction_include.f90:21-56).
 int build_friction_work(swh_domain *h)
@@ -946,8 +964,15 @@ int swh_upload_U(swh_domain *h, const double *U)
     const size_t n = (size_t)h->nx * h->ny;
     const int c = h->cur;
     int rc;
-    // FP32 state: the stage with its Kahan term, so that S - C holds the FP64 stage
-    if ((rc = upload_plane_any(h, h->S[c], U, h->C[c]))) return rc;
+    // FP32 state: the stage with its Kahan term, so that S - C represents the FP64 stage (on
+    // the FP32 elevation where depth is stage - elevation)
+    if (h->prec != SWH_PREC_FP64 && !h->tl_solver()) {
+        const std::vector<double> t = stage_on_fp32_elevation(h, U, U + 3 * n);
+        rc = upload_plane_any(h, h->S[c], t.data(), h->C[c]);
+    } else {
+        rc = upload_plane_any(h, h->S[c], U, h->C[c]);
+    }
+    if (rc) return rc;
     if ((rc = upload_plane_any(h, h->UH[c], U + n))) return rc;
     if ((rc = upload_plane_any(h, h->VH[c], U + 2 * n))) return rc;
     if ((rc = upload_plane_any(h, h->E, U + 3 * n))) return rc;
@@ -961,7 +986,14 @@ int swh_upload_state(swh_domain *h, const double *stage, const double *uh, const
     SWH_HIP(hipStreamSynchronize(h->stream));
     const int c = h->cur;
     int rc;
-    if ((rc = upload_plane_any(h, h->S[c], stage, h->C[c]))) return rc;
+    if (h->prec != SWH_PREC_FP64 && !h->tl_solver()) {
+        if (h->h_elev.size() != (size_t)h->nx * h->ny) return fail(SWH_ESTATE, "upload U (with the elevation) first");
+        const std::vector<double> t = stage_on_fp32_elevation(h, stage, h->h_elev.data());
+        rc = upload_plane_any(h, h->S[c], t.data(), h->C[c]);
+    } else {
+        rc = upload_plane_any(h, h->S[c], stage, h->C[c]);
+    }
+    if (rc) return rc;
     if ((rc = upload_plane_any(h, h->UH[c], uh))) return rc;
     return upload_plane_any(h, h->VH[c], vh);
 }

@@ -79,16 +79,27 @@ __device__ __forceinline__ void stage_cell(bool interior, T sa, T dle_i, T dbe_l
     cn = interior ? cc : c;
 }
 
+// Depth at a face of the truely-linear scheme (include.f90:265-280) from the elevations
+// of its two cells (e_far: the east or north one); it depends on the elevation alone.
+template <typename T>
+__device__ __forceinline__ T tl_face_depth(T e_near, T e_far, T msl)
+{
+    const T MIN = T(kMinDepth), HALF = T(0.5);
+    return (e_near < -MIN + msl && e_far < -MIN + msl) ? msl - HALF * (e_far + e_near) : T(0);
+}
+
 // UH at (ii + 1/2, j) from the new stage of cells ii, ii+1 and the pre-step VH of
-// rows j-1 (vhm0, vhm1 at ii, ii+1) and j (vh0, vh1).
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+// rows j-1 (vhm0, vhm1 at ii, ii+1) and j (vh0, vh1). HPRE (truely linear only): e0
+// carries the face depth h_iph itself (tl_face_depth of the two elevations), e1 unused.
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool HPRE = false>
 __device__ __forceinline__ T mom_u(const LFArgs<T> &a, bool valid, T cu, T fu, T dle_ip1, T fwu, T uh_old, T s0,
                                    T c0, T s1, T c1, T e0, T e1, T vhm0, T vhm1, T vh0, T vh1)
 {
+    static_assert(TL || !HPRE, "precomputed face depths are for the truely-linear scheme");
     const T MIN = T(kMinDepth), HALF = T(0.5), msl = a.msl;
     T h_iph, wet_iph;
     if (TL) {
-        h_iph = (e0 < -MIN + msl && e1 < -MIN + msl) ? msl - HALF * (e1 + e0) : T(0);
+        h_iph = HPRE ? e0 : tl_face_depth(e0, e1, msl);
         wet_iph = T(1);
     } else {
         h_iph = (s0 - e0 > MIN && s1 - e1 > MIN) ? HALF * ((s1 + s0) - (e1 + e0)) : T(0);
@@ -115,15 +126,17 @@ __device__ __forceinline__ T mom_u(const LFArgs<T> &a, bool valid, T cu, T fu, T
 }
 
 // VH at (ii, j + 1/2) from the new stage of rows j, j+1 and the pre-step UH of
-// row j (uh_i at ii, uh_im1 at ii-1) and row j+1 (uhu_i, uhu_im1).
-template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF>
+// row j (uh_i at ii, uh_im1 at ii-1) and row j+1 (uhu_i, uhu_im1). HPRE: e0 carries
+// h_jph itself, eu unused.
+template <typename T, bool KAHAN, bool COR, int FRIC, bool TL, bool LINF, bool HPRE = false>
 __device__ __forceinline__ T mom_v(const LFArgs<T> &a, bool valid, T cv, T fv, T dbe_hi, T fwv, T vh_old, T s0, T c0,
                                    T sup, T cup, T e0, T eu, T uh_i, T uh_im1, T uhu_i, T uhu_im1)
 {
+    static_assert(TL || !HPRE, "precomputed face depths are for the truely-linear scheme");
     const T MIN = T(kMinDepth), HALF = T(0.5), msl = a.msl;
     T h_jph, wet_jph;
     if (TL) {
-        h_jph = (eu < -MIN + msl && e0 < -MIN + msl) ? msl - HALF * (eu + e0) : T(0);
+        h_jph = HPRE ? e0 : tl_face_depth(e0, eu, msl);
         wet_jph = T(1);
     } else {
         h_jph = (sup - eu > MIN && s0 - e0 > MIN) ? HALF * ((sup + s0) - (eu + e0)) : T(0);
@@ -396,6 +409,10 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
     T uh_pp[NS + 1], uh_pp_l[NS + 1];    // the iteration before that (levels >= 1)
     T vh_p[NS + 1], vh_p_r[NS + 1];      // VH of level n, previous iteration, and its east value
     T e_r[NS + 1], e_rr[NS + 1];         // elevation of row m-k and its east value
+    // truely linear: the face depths instead (they depend on the elevation alone, so each
+    // is formed once, when its row is loaded, not once per level): h_iph of row m-k,
+    // h_jph of the face above row m-1-k, and the elevation of row m-1
+    T hi_r[NS], hj_r[NS], e_prev = T(0);
     T fwu_r[NS], fwv_r[NS];              // friction work: UH at row m-k, VH at row m-1-k
     RowCo<T> q_r[NS + 1];                // row coefficients of row m-k
 #pragma unroll
@@ -403,6 +420,8 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
         s_p[n] = T(0); c_p[n] = T(0); uh_p[n] = T(0); uh_p_l[n] = T(0); uh_pp[n] = T(0); uh_pp_l[n] = T(0);
         vh_p[n] = T(0); vh_p_r[n] = T(0); e_r[n] = T(0); e_rr[n] = T(0);
     }
+#pragma unroll
+    for (int n = 0; n < NS; n++) { hi_r[n] = T(0); hj_r[n] = T(0); }
 #pragma unroll
     for (int n = 0; n < NS; n++) { fwu_r[n] = T(0); fwv_r[n] = T(0); }
     const int m0 = j0 - NS + 1;
@@ -420,6 +439,7 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
         const T uh_sh = shr1(uh_p[0]);
         uh_p_l[0] = west ? uh_p[0] : uh_sh;
         e_rr[0] = shl1(e_r[0]);
+        e_prev = e_r[0];
     }
     auto load_row = [&](int m, T &s0, T &c0, T &uh0, T &vh0, T &e, T &uhw) {
         const int mr = min(max(m, 0), a.ny - 1);
@@ -445,10 +465,20 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
         }
         // slide the rings of rows: index k now holds row m-k
 #pragma unroll
-        for (int k = NS; k >= 1; k--) { e_r[k] = e_r[k - 1]; e_rr[k] = e_rr[k - 1]; q_r[k] = q_r[k - 1]; }
-        e_r[0] = e0;
-        e_rr[0] = shl1(e0);
+        for (int k = NS; k >= 1; k--) q_r[k] = q_r[k - 1];
         q_r[0] = row_co(a, m);
+        if constexpr (TL) {
+#pragma unroll
+            for (int k = NS - 1; k >= 1; k--) { hi_r[k] = hi_r[k - 1]; hj_r[k] = hj_r[k - 1]; }
+            hi_r[0] = tl_face_depth(e0, shl1(e0), a.msl);
+            hj_r[0] = tl_face_depth(e_prev, e0, a.msl);
+            e_prev = e0;
+        } else {
+#pragma unroll
+            for (int k = NS; k >= 1; k--) { e_r[k] = e_r[k - 1]; e_rr[k] = e_rr[k - 1]; }
+            e_r[0] = e0;
+            e_rr[0] = shl1(e0);
+        }
         if (FRIC) {
 #pragma unroll
             for (int k = NS - 1; k >= 1; k--) { fwu_r[k] = fwu_r[k - 1]; fwv_r[k] = fwv_r[k - 1]; }
@@ -477,15 +507,19 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
                                  vh_cur, vhb, sn, cn);
             const T sn_r = shl1(sn);
             const T cn_r = KAHAN ? shl1(cn) : T(0);
-            const T uhn = mom_u<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && q.valid, q.cu, q.fu, dle_ip1,
-                                                                FRIC ? fwu_r[n - 1] : T(0), in_uh, sn, cn, sn_r, cn_r,
-                                                                e_r[n - 1], e_rr[n - 1], vhb, vhb_r, vh_cur, vh_cur_r);
+            // the face depths of this level's rows (truely linear: precomputed), or the
+            // elevations they are formed from
+            const T fu_a = TL ? hi_r[n - 1] : e_r[n - 1], fu_b = TL ? T(0) : e_rr[n - 1];
+            const T fv_a = TL ? hj_r[n - 1] : e_r[n], fv_b = TL ? T(0) : e_r[n - 1];
+            const T uhn = mom_u<T, KAHAN, COR, FRIC, TL, LINF, TL>(a, col_mom && q.valid, q.cu, q.fu, dle_ip1,
+                                                                    FRIC ? fwu_r[n - 1] : T(0), in_uh, sn, cn, sn_r,
+                                                                    cn_r, fu_a, fu_b, vhb, vhb_r, vh_cur, vh_cur_r);
             // VH at row r-1: stage of this level at rows r-1 (previous iteration) and r,
             // UH of the level below at rows r-1 and r
-            const T vhn = mom_v<T, KAHAN, COR, FRIC, TL, LINF>(a, col_mom && qb.valid, qb.cv, qb.fv, qb.dbe_hi,
-                                                                FRIC ? fwv_r[n - 1] : T(0), vh_p[n - 1], s_p[n],
-                                                                c_p[n], sn, cn, e_r[n], e_r[n - 1], in_uhb, in_uhb_l,
-                                                                in_uh, in_uh_l);
+            const T vhn = mom_v<T, KAHAN, COR, FRIC, TL, LINF, TL>(a, col_mom && qb.valid, qb.cv, qb.fv, qb.dbe_hi,
+                                                                    FRIC ? fwv_r[n - 1] : T(0), vh_p[n - 1], s_p[n],
+                                                                    c_p[n], sn, cn, fv_a, fv_b, in_uhb, in_uhb_l,
+                                                                    in_uh, in_uh_l);
             if (GS && n == a.g_level) {
                 // the sampled level's values at this iteration's rows
                 cap_s = sn; cap_c = cn; cap_uh = uhn; cap_vh = vhn;

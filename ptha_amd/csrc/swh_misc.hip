@@ -662,7 +662,7 @@ __global__ void plane_from_dense_kernel(const double *__restrict__ src, T *__res
     const double v = (i < nx && src) ? src[(long long)j * nx + i] : 0.0;
     const T t = (T)v;
     dst[o] = t;
-    if (comp) comp[o] = (T)((double)t - v);   // dst - comp represents v
+    if (comp) comp[o] = (T)((double)t - v);   // dst - comp represents v (to about 48 bits in FP32)
 }
 
 template <typename T>

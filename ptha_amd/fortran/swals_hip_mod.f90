@@ -97,7 +97,7 @@ module swals_hip_mod
         swh_dispersion_store_last_U, swh_dispersion_append_extrapolation, swh_dispersion_solve, &
         swh_evolve_steps, swh_get_scalars, swh_volume_interior, swh_stationary_timestep_max, &
         swh_download_max_U, swh_gauges_set, swh_gauges_sample, swh_gauges_download, swh_synchronize, &
-        swh_domain_set_launch, swh_check
+        swh_domain_set_launch, swh_domain_set_launch2, swh_upload_state, swh_check
 
     interface
         integer(c_int) function swh_last_error(buf, len) bind(C, name='swh_last_error')
@@ -118,6 +118,12 @@ module swals_hip_mod
             import c_int, c_ptr, c_double
             type(c_ptr), value :: h
             real(c_double), intent(in) :: U(*)
+        end function
+        ! A new initial condition on the same grid (elevation kept): c_loc of stage, UH,
+        ! VH arrays of (nx, ny), or c_null_ptr for zeros.
+        integer(c_int) function swh_upload_state(h, stage, uh, vh) bind(C, name='swh_upload_state')
+            import c_int, c_ptr
+            type(c_ptr), value :: h, stage, uh, vh
         end function
         integer(c_int) function swh_download_U(h, U) bind(C, name='swh_download_U')
             import c_int, c_ptr, c_double
@@ -255,6 +261,12 @@ module swals_hip_mod
             import c_int, c_ptr, c_int32_t
             type(c_ptr), value :: h
             integer(c_int32_t), value :: block_x, rows_per_block, variant
+        end function
+        integer(c_int) function swh_domain_set_launch2(h, block_x, rows_per_block) &
+                bind(C, name='swh_domain_set_launch2')
+            import c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: block_x, rows_per_block
         end function
         ! ---- multidomain (one GPU) ----
         integer(c_int) function swh_md_create(n, domains, all_dx, all_methods, trf, md) bind(C, name='swh_md_create')

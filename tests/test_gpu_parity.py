@@ -346,6 +346,42 @@ def test_single_grid_two_step_bitwise(oracle_built, case, precision):
         assert np.array_equal(u0, o.get_U())
 
 
+def test_fp32k_upload_dry_cells_and_exact_stage(oracle_built):
+    """FP32K uploads: where depth is stage - elevation (finite volume), a cell dry in FP64
+    comes back with exactly the FP32 elevation as its stage and no depth is negative (an
+    FP64 stage over the FP32-rounded elevation would leave dry cells a depth of plus or
+    minus that rounding, and the negative-depth fix would run on them every step); a wet
+    one keeps its FP64 stage (at least the FP32 elevation) to the 48 bits that S - C of two
+    floats carries. The truely-linear leapfrog keeps the FP64 stage itself, land included.
+    swh_upload_state follows the same rule."""
+    _print_case("fp32k upload")
+    cfg = H.small_beach()
+    U0 = cfg["U"].copy()
+    U0[3] = U0[3] + np.random.default_rng(4).uniform(-1e-3, 1e-3, U0[3].shape)   # elevations off the FP32 grid
+    U0[0] = np.maximum(U0[0], U0[3])
+    cfg = dict(cfg, U=U0)
+    e32 = U0[3].astype(np.float32).astype(np.float64)
+    dry = U0[0] <= U0[3]
+    assert dry.any() and (~dry).any()
+    g = H.gpu_domain(cfg, precision="fp32k")
+    for U in (g.U, None):
+        if U is None:
+            g.set_state(U0[0], U0[1], U0[2])
+            U = g.U
+        assert np.array_equal(U[3], e32)
+        assert np.array_equal(U[0][dry], e32[dry])
+        assert (U[0] - U[3] >= 0).all()
+        want = np.maximum(U0[0], e32)[~dry]
+        assert (np.abs(U[0][~dry] - want) <= 2.0 ** -46 * np.abs(want)).all()
+    g.close()
+    cfg = H.small_spherical()
+    U0 = cfg["U"].copy()
+    U0[0] = np.random.default_rng(5).standard_normal(U0[0].shape) * 0.3
+    g = H.gpu_domain(dict(cfg, U=U0), precision="fp32k")
+    assert (np.abs(g.U[0] - U0[0]) <= 2.0 ** -46 * np.abs(U0[0])).all()
+    g.close()
+
+
 @pytest.mark.parametrize("shape", [(64, 4), (128, 7), (192, 13), (256, 64), (64, 100), (128, 0), (256, 0)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32k"])
 def test_two_step_launch_shapes_do_not_change_results(oracle_built, shape, precision):
