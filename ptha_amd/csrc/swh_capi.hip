@@ -826,19 +826,29 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
     h->max_timestep = d->maximum_timestep > 0 ? d->maximum_timestep : kMaxTimestep;
     h->arrival_thresh = d->arrival_stage_threshold_above_msl_linear;
     h->esz = h->prec == SWH_PREC_FP64 ? 8 : 4;
-    // default launch shape of the step kernel, from the measured sweep over shapes
-    // (16-byte vector lanes; FP32 also keeps the next row's loads in flight)
-    if (h->prec == SWH_PREC_FP64) { h->block_x = 128; h->rows_per_block = 16; h->variant = 1; }
-    else { h->block_x = 64; h->rows_per_block = 32; h->variant = 3; }
-    if (fv) h->rows_per_block = 8;   // finite volume: rows per wavefront of the flux pass
-    {
-        // a single grid must still fill the GPU: shorten the row strips until the
-        // launch has at least 4096 wavefronts (16 per CU)
-        const int vec = (h->variant & 1) ? (h->prec == SWH_PREC_FP64 ? 2 : 4) : 1;
-        const long long lanes = (d->nx + vec - 1) / vec;
-        const long long waves_x = (lanes + 63) / 64;
+    // default launch shape of the one-step kernel, from measured sweeps over shapes on
+    // 400^2 to 3000x1800 grids: 128-thread workgroups; FP64 with 16-byte (two-column)
+    // lanes, FP32 with one column per lane and the next row's loads in flight; row strips
+    // shortened from 32 rows until the launch has at least 8192 wavefronts (32 per CU),
+    // down to one row (the kernel is latency-bound on small grids: 612^2 llpnf FP32K
+    // 11 us per step at one row against 50 us at four rows with 16-byte lanes)
+    if (h->prec == SWH_PREC_FP64) { h->block_x = 128; h->rows_per_block = 32; h->variant = 1; }
+    else { h->block_x = 128; h->rows_per_block = 32; h->variant = 2; }
+    if (fv) {
+        // finite volume: rows per wavefront of the flux pass, 8, or 4 where a grid would
+        // have fewer than 4096 wavefronts of 16-byte lanes at 8 (measured on the beach)
+        h->rows_per_block = 8;
+        const int vec = h->prec == SWH_PREC_FP64 ? 2 : 4;
+        const long long waves_x = ((d->nx + vec - 1) / vec + 63) / 64;
         while (h->rows_per_block > 4 && waves_x * ((d->ny + h->rows_per_block - 1) / h->rows_per_block) < 4096)
             h->rows_per_block /= 2;
+    } else {
+        const int vec = (h->variant & 1) ? (h->prec == SWH_PREC_FP64 ? 2 : 4) : 1;
+        const long long waves_x = ((d->nx + vec - 1) / vec + 63) / 64;
+        while (h->rows_per_block > 1 && waves_x * ((d->ny + h->rows_per_block - 1) / h->rows_per_block) < 8192)
+            h->rows_per_block /= 2;
+    }
+    {
         // the two-step kernel: rows per strip chosen by its launcher from the kernel's
         // occupancy (rows2 = 0); workgroups of 128 threads (FP32) or 256 (FP64)
         h->rows2 = 0;

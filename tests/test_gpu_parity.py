@@ -451,7 +451,7 @@ def test_scheduled_gauges_in_step_pairs_bitwise(oracle_built, case, precision):
 
 
 @pytest.mark.parametrize("shape", [(256, 32, 0), (64, 7, 3), (128, 64, 1), (192, 5, 2), (64, 9, 7), (256, 3, 6),
-                                   (128, 11, 5), (64, 16, 4)])
+                                   (128, 11, 5), (64, 16, 4), (128, 1, 2), (128, 2, 1)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32k"])
 def test_launch_shapes_do_not_change_results(oracle_built, shape, precision):
     """Every launch shape of the step kernel (lanes per block, rows per strip, vector
