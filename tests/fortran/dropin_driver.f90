@@ -9,9 +9,8 @@
 !
 ! 'nonlinear' runs 'leapfrog_nonlinear' without walls, with the reference's Flather
 ! boundary (domain%boundary_subroutine => flather_boundary) and a co-seismic forcing
-! patch (forcing_patch_type%setup + apply_forcing_patch), each mapped to the backend as
-! INTEGRATION.md describes (swh_set_flather_boundary, swh_set_forcing_patch with the
-! patch's own forcing_work).
+! patch (forcing_patch_type%setup + apply_forcing_patch), each mapped to the backend
+! (swh_set_flather_boundary; swh_set_forcing_patch with the patch's own forcing_work).
 program dropin_driver
     use iso_c_binding
     use global_mod, only: dp, ip, charlen

@@ -3,8 +3,8 @@ domain_type (compiled from the reference sources), passes c_loc of its metric ar
 and U to the backend through swals_hip_mod, advances the domain on the GPU, and
 compares with domain%evolve_one_step on the CPU: FP64 must be bit-identical.
 'nonlinear' is 'leapfrog_nonlinear' (no friction) with the reference's Flather
-boundary and a forcing patch set up through forcing_mod, mapped as INTEGRATION.md
-shows."""
+boundary and a forcing patch set up through forcing_mod, mapped to
+swh_set_flather_boundary and swh_set_forcing_patch."""
 import os
 import subprocess
 

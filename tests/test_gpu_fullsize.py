@@ -35,8 +35,8 @@ def _report(tag, d):
           flush=True)
 
 
-# FP64 against the reference: the north star asks for gauge time series within 1e-6
-# relative; here that is read per gauge as ||g - g_ref||_2 / ||g_ref||_2 over the whole
+# FP64 against the reference: the target is gauge time series within 1e-6 relative;
+# here that is read per gauge as ||g - g_ref||_2 / ||g_ref||_2 over the whole
 # series (gauges whose stage never moves by 1 mm are left out). The max-stage grid,
 # volume and the negative-depth counter are barred too. The run is chaotic at the
 # shore (32 600 adaptive steps with wet/dry switching): glibc's pow, which the
@@ -44,8 +44,8 @@ def _report(tag, d):
 # 1200, while the device rounds correctly, and such one-ulp differences grow. The
 # problem's own sensitivity to them is measured in the same test: two more device runs
 # whose Manning n^2 is moved by one ulp in every cell (random sign; seeds 1 and 2)
-# give the floor, and the FP64 bar on the gauges is the north star's 1e-6 or twice
-# that floor, whichever is larger.
+# give the floor, and the FP64 bar on the gauges is the target's 1e-6 or twice that
+# floor, whichever is larger.
 # FP32 state with the stage Kahan-compensated and the clipped cell (depth, velocities)
 # formed in FP64: bars per quantity.
 C2_BARS = {"gauge_stage_l2_rel": (1e-6, 1e-5), "gauge_stage": (None, 1e-4), "max_stage": (1e-4, 1e-4),
@@ -124,7 +124,7 @@ def test_config2_fullsize(precision):
     assert res["neg_depth"] == res["ref_neg_depth"]
 
 
-# FP64 against the reference's own multidomain: the north star's 1e-6 relative on the
+# FP64 against the reference's own multidomain: the target's 1e-6 relative on the
 # gauge series (per gauge, L2 over the series, gauges whose stage moves by 1 mm or
 # more), or twice the nest's own one-ulp sensitivity measured in the same test (two more
 # device runs with the Manning n^2 of the friction domains moved by one ulp per cell)
