@@ -63,17 +63,31 @@ struct AdjRow {
     bool mom, in;
 };
 
+// A value that is the same on every lane (a row coefficient), kept in scalar registers
+// so that the three rows carried across iterations take no vector registers.
+__device__ __forceinline__ float uniform(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <typename T>
 __device__ __forceinline__ AdjRow<T> adj_row(const AdjArgs<T> &a, int r)
 {
     const int rc = min(max(r, 0), a.ny - 1);
     AdjRow<T> q;
-    q.sa = a.rc.sa[rc];
-    q.cu = a.rc.cu[rc];
-    q.cv = a.rc.cv[rc];
-    q.fu = a.rc.fu[rc];
-    q.fv = a.rc.fv[rc];
-    q.dbe1 = a.rc.dbe[rc + 1];
+    q.sa = uniform(a.rc.sa[rc]);
+    q.cu = uniform(a.rc.cu[rc]);
+    q.cv = uniform(a.rc.cv[rc]);
+    q.fu = uniform(a.rc.fu[rc]);
+    q.fv = uniform(a.rc.fv[rc]);
+    q.dbe1 = uniform(a.rc.dbe[rc + 1]);
     q.mom = r >= 0 && r < a.ny - 1;
     q.in = r > 0 && r < a.ny - 1;
     return q;
