@@ -42,6 +42,19 @@ struct RowCo {
     bool interior, valid;                   // stage row (1..ny-2), momentum row (0..ny-2)
 };
 
+// A value that is the same on every lane (a row coefficient), kept in scalar registers.
+__device__ __forceinline__ float uniform(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <typename T>
 __device__ __forceinline__ RowCo<T> row_co(const LFArgs<T> &a, int r)
 {
@@ -56,6 +69,27 @@ __device__ __forceinline__ RowCo<T> row_co(const LFArgs<T> &a, int r)
     q.dbe_hi = a.rc.dbe[rc + 1];
     q.interior = r > 0 && r < a.ny - 1;
     q.valid = r >= 0 && r < a.ny - 1;
+    return q;
+}
+
+// row_co for a row that is the same on every lane of the wavefront (the fused n-step
+// kernel's rows), FP64: the coefficients in scalar registers, so that the rows carried
+// across iterations take no vector registers (150 -> 105 VGPRs at three steps, a fourth
+// wavefront per SIMD, FP64 batch +3 %). FP32 keeps them as the compiler places them: its
+// loop is issue-bound and the scalar-register spills this adds cost more than the
+// wavefront it gains (-7 %). Not for per-lane rows (step1_cell).
+template <typename T>
+__device__ __forceinline__ RowCo<T> row_co_u(const LFArgs<T> &a, int r)
+{
+    RowCo<T> q = row_co(a, r);
+    if (sizeof(T) == 4) return q;
+    q.sa = uniform(q.sa);
+    q.cu = uniform(q.cu);
+    q.cv = uniform(q.cv);
+    q.fu = uniform(q.fu);
+    q.fv = uniform(q.fv);
+    q.dbe_lo = uniform(q.dbe_lo);
+    q.dbe_hi = uniform(q.dbe_hi);
     return q;
 }
 
@@ -427,7 +461,7 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
     const int m0 = j0 - NS + 1;
     // the rings as they stand after row m0-1 (each iteration first slides them by one)
 #pragma unroll
-    for (int k = 0; k <= NS; k++) q_r[k] = row_co(a, m0 - 1 - k);
+    for (int k = 0; k <= NS; k++) q_r[k] = row_co_u(a, m0 - 1 - k);
     {
         // the row below the first one: pre-step VH/UH/E of row m0-1
         const int r = min(max(m0 - 1, 0), a.ny - 1);
@@ -466,7 +500,7 @@ __global__ __launch_bounds__(256, (GS && sizeof(T) == 4) ? 4 : 1) void leapfrogn
         // slide the rings of rows: index k now holds row m-k
 #pragma unroll
         for (int k = NS; k >= 1; k--) q_r[k] = q_r[k - 1];
-        q_r[0] = row_co(a, m);
+        q_r[0] = row_co_u(a, m);
         if constexpr (TL) {
 #pragma unroll
             for (int k = NS - 1; k >= 1; k--) { hi_r[k] = hi_r[k - 1]; hj_r[k] = hj_r[k - 1]; }
