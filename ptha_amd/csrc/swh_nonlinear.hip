@@ -22,6 +22,7 @@
 // The old UH/VH stay in the current planes and the update writes the spare ones, so
 // the row-carried Coriolis averages of the reference are read directly.
 #include "swh_internal.h"
+#include "swh_launch.h"
 
 namespace swh {
 
@@ -57,31 +58,21 @@ __device__ __forceinline__ T nl_inv_depth(T d)
     return d > T(kMinDepth) ? T(1) / d : T(0);
 }
 
-// Time-lagged depth at cell q of row j: from the plane, or (no elevation forcing in the
-// step, so the elevation is unchanged) formed from the old and new stage exactly as
-// nl_depth0_kernel and nl_depth1_kernel form it.
-template <typename T, bool FLY>
-__device__ __forceinline__ T nl_dl(const NLArgs<T> &a, long long q, int j)
-{
-    if (!FLY) return a.DL[q];
-    const T d0 = nl_max0<T>(a.S0[q] - a.E[q]);
-    return j < a.ny - 1 ? T(0.5) * (d0 + nl_max0<T>(a.S[q] - a.E[q])) : d0;
-}
 
 // The four advection work values of cell (i, j) with the reference's extrapolations
 // to the east columns and north rows as index maps (0-based), :274-328.
-template <typename T, bool FLY>
+template <typename T>
 __device__ __forceinline__ T nl_uuh(const NLArgs<T> &a, int i, int j)
 {
     // U*UH at (i+1/2, j): columns < nx-1; column nx-1 copies nx-2; row ny-1 copies ny-2
     const int jj = j == a.ny - 1 ? a.ny - 2 : j;
     const int ii = i == a.nx - 1 ? a.nx - 2 : i;
     const long long q = (long long)jj * a.pitch + ii;
-    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + 1, jj)));
+    const T inv = nl_inv_depth<T>(T(0.5) * (a.DL[q] + a.DL[q + 1]));
     return (a.UH[q] * a.UH[q]) * inv;
 }
 
-template <typename T, bool FLY>
+template <typename T>
 __device__ __forceinline__ T nl_uvh(const NLArgs<T> &a, int i, int j)
 {
     // U*VH at (i+1/2, j): columns < nx-2; columns nx-2, nx-1 copy nx-3; row ny-1 copies ny-2
@@ -89,21 +80,21 @@ __device__ __forceinline__ T nl_uvh(const NLArgs<T> &a, int i, int j)
     const int ii = i >= a.nx - 2 ? a.nx - 3 : i;
     const int jm = jj > 0 ? jj - 1 : 0;
     const long long q = (long long)jj * a.pitch + ii, qm = (long long)jm * a.pitch + ii;
-    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + 1, jj)));
+    const T inv = nl_inv_depth<T>(T(0.5) * (a.DL[q] + a.DL[q + 1]));
     return a.UH[q] * inv * T(0.25) * (a.VH[qm] + a.VH[qm + 1] + a.VH[q] + a.VH[q + 1]);
 }
 
-template <typename T, bool FLY>
+template <typename T>
 __device__ __forceinline__ T nl_vvh(const NLArgs<T> &a, int i, int j)
 {
     // V*VH at (i, j+1/2): rows < ny-1; row ny-1 copies ny-2
     const int jj = j == a.ny - 1 ? a.ny - 2 : j;
     const long long q = (long long)jj * a.pitch + i;
-    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + a.pitch, jj + 1)));
+    const T inv = nl_inv_depth<T>(T(0.5) * (a.DL[q] + a.DL[q + a.pitch]));
     return a.VH[q] * a.VH[q] * inv;
 }
 
-template <typename T, bool FLY>
+template <typename T>
 __device__ __forceinline__ T nl_vuh(const NLArgs<T> &a, int i, int j)
 {
     // V*UH at (i, j+1/2): columns > 0, rows < ny-2; column 0 copies 1; rows ny-2, ny-1 copy ny-3
@@ -111,7 +102,7 @@ __device__ __forceinline__ T nl_vuh(const NLArgs<T> &a, int i, int j)
     const int ii = i == 0 ? 1 : i;
     const long long q = (long long)jj * a.pitch + ii;
     const int P = a.pitch;
-    const T inv = nl_inv_depth<T>(T(0.5) * (nl_dl<T, FLY>(a, q, jj) + nl_dl<T, FLY>(a, q + P, jj + 1)));
+    const T inv = nl_inv_depth<T>(T(0.5) * (a.DL[q] + a.DL[q + P]));
     return a.VH[q] * inv * T(0.25) * (a.UH[q - 1] + a.UH[q] + a.UH[q - 1 + P] + a.UH[q + P]);
 }
 
@@ -122,10 +113,10 @@ __global__ void nl_advection_kernel(NLArgs<T> a)
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= a.nx || j >= a.ny) return;
     const long long o = (long long)j * a.pitch + i;
-    a.UUH[o] = nl_uuh<T, false>(a, i, j);
-    a.UVH[o] = nl_uvh<T, false>(a, i, j);
-    a.VVH[o] = nl_vvh<T, false>(a, i, j);
-    a.VUH[o] = nl_vuh<T, false>(a, i, j);
+    a.UUH[o] = nl_uuh<T>(a, i, j);
+    a.UVH[o] = nl_uvh<T>(a, i, j);
+    a.VVH[o] = nl_vvh<T>(a, i, j);
+    a.VUH[o] = nl_vuh<T>(a, i, j);
 }
 
 template <typename T>
@@ -134,10 +125,9 @@ __device__ __forceinline__ bool nl_wet(const NLArgs<T> &a, long long q)
     return a.S[q] > a.E[q] + T(kMinDepth);
 }
 
-// the four upwind momentum fluxes (fNSu / fNSv row j: the edge at the bottom of row j).
-// FUSED: the advection work values are formed where needed (each from the same
-// operations as nl_advection_kernel) instead of read from its planes.
-template <typename T, bool FUSED>
+// the four upwind momentum fluxes (fNSu / fNSv row j: the edge at the bottom of row j)
+// from the advection work planes
+template <typename T>
 __global__ void nl_flux_kernel(NLArgs<T> a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
@@ -146,10 +136,10 @@ __global__ void nl_flux_kernel(NLArgs<T> a)
     const T *UH = a.UH, *VH = a.VH;
     const long long o = (long long)j * P + i;
     const T H = T(0.5), Z = T(0);
-    auto UUH = [&](int ii, int jj) { return FUSED ? nl_uuh<T, true>(a, ii, jj) : a.UUH[(long long)jj * P + ii]; };
-    auto UVH = [&](int ii, int jj) { return FUSED ? nl_uvh<T, true>(a, ii, jj) : a.UVH[(long long)jj * P + ii]; };
-    auto VVH = [&](int ii, int jj) { return FUSED ? nl_vvh<T, true>(a, ii, jj) : a.VVH[(long long)jj * P + ii]; };
-    auto VUH = [&](int ii, int jj) { return FUSED ? nl_vuh<T, true>(a, ii, jj) : a.VUH[(long long)jj * P + ii]; };
+    auto UUH = [&](int ii, int jj) { return a.UUH[(long long)jj * P + ii]; };
+    auto UVH = [&](int ii, int jj) { return a.UVH[(long long)jj * P + ii]; };
+    auto VVH = [&](int ii, int jj) { return a.VVH[(long long)jj * P + ii]; };
+    auto VUH = [&](int ii, int jj) { return a.VUH[(long long)jj * P + ii]; };
     // EW flux of UH (rows < ny-1)
     {
         T f = Z;
@@ -215,6 +205,182 @@ __global__ void nl_flux_kernel(NLArgs<T> a)
             }
         }
         a.fEWv[o] = f;
+    }
+}
+
+// The flux pass when the elevation is unchanged in the step (no forcing of it): the
+// time-lagged depth, the advection work and the four fluxes in one column march.
+// A wavefront owns kNLOwn columns of a strip of rows: lane l holds column
+// strip * kNLOwn - 2 + l (lanes 0, 1 hold the two columns west of the strip, lane 63
+// the column east of it: the west neighbour's V*UH needs the UH west of it) and the
+// wavefront walks the strip's rows upward. Each lane forms the time-lagged
+// depth and the four advection work values of its own cell once per row (the
+// operations of nl_depth0/1_kernel and nl_uuh .. nl_vuh), takes its west and east
+// neighbours' values from the adjacent lanes and the row below's from what it carried.
+// The reference's extrapolations at the east columns (U*UH, U*VH), the west column
+// (V*UH) and the north rows (U*VH, V*VH, V*UH) pick the adjacent lane's or the carried
+// value, which is the value the index map of nl_uuh .. nl_vuh points to, so every flux
+// is the one nl_flux_kernel forms from the planes. The rows below the strip that the
+// carried values need (one, two where the strip begins at the last row) are walked
+// first without storing.
+constexpr int kNLOwn = 61;
+
+template <typename T>
+__device__ __forceinline__ T nl_shl1(T v) { return __shfl_down(v, 1); }   // value of lane + 1
+
+template <typename T>
+__device__ __forceinline__ T nl_shr1(T v) { return __shfl_up(v, 1); }     // value of lane - 1
+
+template <typename T>
+struct NLRaw {
+    T s0, s, e, uh, vh;
+};
+
+template <typename T>
+__device__ __forceinline__ NLRaw<T> nl_raw(const NLArgs<T> &a, int ci, int r)
+{
+    const long long q = (long long)min(r, a.ny - 1) * a.pitch + ci;
+    return NLRaw<T>{a.S0[q], a.S[q], a.E[q], a.UH[q], a.VH[q]};
+}
+
+template <typename T>
+struct NLCell {
+    T uh, vh, dl;
+    int wet;
+};
+
+// the cell of row r: momenta, time-lagged depth (:265-268, as nl_depth0/1_kernel), wet
+template <typename T>
+__device__ __forceinline__ NLCell<T> nl_cell(const NLRaw<T> &w, int r, int ny)
+{
+    const T d0 = nl_max0<T>(w.s0 - w.e);
+    NLCell<T> c;
+    c.uh = w.uh;
+    c.vh = w.vh;
+    c.dl = r < ny - 1 ? T(0.5) * (d0 + nl_max0<T>(w.s - w.e)) : d0;
+    c.wet = w.s > w.e + T(kMinDepth) ? 1 : 0;
+    return c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void nl_flux_march_kernel(NLArgs<T> a, int rows)
+{
+    const int nx = a.nx, ny = a.ny, P = a.pitch;
+    const int lane = threadIdx.x;
+    const int i = (int)blockIdx.x * kNLOwn - 2 + lane;
+    const int ci = min(max(i, 0), nx - 1);
+    const bool own = lane >= 2 && lane <= kNLOwn + 1 && i < nx;
+    const int j0 = (int)blockIdx.y * rows;
+    if (j0 >= ny) return;
+    const int j1 = min(j0 + rows, ny);
+    const int js = j0 == 0 ? 0 : (j0 >= ny - 1 ? j0 - 2 : j0 - 1);
+    const T H = T(0.5), Z = T(0), Q = T(0.25);
+    const T dle_u = (T)a.dle[ci];                 // EW flux of UH: dle(i)
+    const T dle_v = (T)a.dle[max(ci, 1) - 1];     // EW flux of VH: dle(max(i, 1) - 1)
+    NLCell<T> cur = nl_cell(nl_raw(a, ci, js), js, ny);
+    NLCell<T> nxt = nl_cell(nl_raw(a, ci, js + 1), js + 1, ny);
+    NLRaw<T> p1 = nl_raw(a, ci, js + 2), p2 = p1;
+    // carried: VH of the row below (row 0 on row 0) and its east neighbour, UH west of
+    // this row, the row below's wet flag and selected U*VH, V*VH, V*UH
+    T vh_m = a.VH[(long long)max(js - 1, 0) * P + ci];
+    T vh_me = nl_shl1(vh_m);
+    T uh_w = nl_shr1(cur.uh);
+    T uvh_m = Z, vvh_m = Z, vuh_m = Z;
+    int wet_m = 0;
+    for (int j = js; j < j1; j++) {
+        if (j + 3 <= j1) p2 = nl_raw(a, ci, j + 3);
+        const bool first = j == 0;
+        const T dl_e = nl_shl1(cur.dl);
+        const T vh_e = nl_shl1(cur.vh);
+        const T uh_nw = nl_shr1(nxt.uh);
+        // the four advection work values of this lane's cell (nl_uuh .. nl_vuh without
+        // their index maps)
+        const T inv_u = nl_inv_depth<T>(H * (cur.dl + dl_e));
+        const T inv_v = nl_inv_depth<T>(H * (cur.dl + nxt.dl));
+        const T uuh_r = (cur.uh * cur.uh) * inv_u;
+        const T uvh_r = cur.uh * inv_u * Q * (vh_m + vh_me + cur.vh + vh_e);
+        const T vvh_r = cur.vh * cur.vh * inv_v;
+        const T vuh_r = cur.vh * inv_v * Q * (uh_w + cur.uh + uh_nw + nxt.uh);
+        const T uuh_w = nl_shr1(uuh_r);
+        const T uvh_w = nl_shr1(uvh_r);
+        const T vuh_e = nl_shl1(vuh_r);
+        // the index maps: U*UH column nx-1 from nx-2, U*VH column nx-2 from nx-3 (nx-1's
+        // is not used), V*UH column 0 from 1; U*VH, V*VH row ny-1 from ny-2, V*UH rows
+        // ny-2, ny-1 from ny-3 (U*UH of row ny-1 is not used)
+        const T uuh = i == nx - 1 ? uuh_w : uuh_r;
+        T uvh = i == nx - 2 ? uvh_w : uvh_r;
+        T vvh = vvh_r;
+        T vuh = i == 0 ? vuh_e : vuh_r;
+        if (j == ny - 1) {
+            uvh = uvh_m;
+            vvh = vvh_m;
+        }
+        if (j >= ny - 2) vuh = vuh_m;
+        const T vuh_w = nl_shr1(vuh);
+        const int wm = first ? cur.wet : wet_m;
+        const int wbits = wm | (cur.wet << 1) | (nxt.wet << 2);
+        const int we = nl_shl1(wbits), ww = nl_shr1(wbits);
+        if (j >= j0 && own) {
+            const long long o = (long long)j * P + i;
+            // EW flux of UH (rows < ny-1)
+            T f = Z;
+            if (j < ny - 1) {
+                if (i == 0 || i == nx - 1) {
+                    f = uuh * dle_u;
+                } else {
+                    const T us = cur.uh + uh_w;
+                    const T ul = uuh_w, ur = uuh_r;
+                    f = dle_u * (us > Z ? ul : ur);
+                    if (us == Z) f = dle_u * H * (ul + ur);
+                }
+                if (!cur.wet) f = Z;
+            }
+            a.fEWu[o] = f;
+            // NS flux of UH on the bottom edge of row j
+            f = Z;
+            if (i < nx - 1) {
+                const T dbe = (T)a.dbe[j];
+                const T vs = vh_m + vh_me;
+                const T wa = first ? uvh : uvh_m, wb = uvh;
+                f = dbe * (vs > Z ? wa : wb);
+                if (vs == Z) f = dbe * H * (wa + wb);
+                if (!(wm && (we & 1) && cur.wet && (we & 2))) f = Z;
+            }
+            a.fNSu[o] = f;
+            // NS flux of VH on the bottom edge of row j
+            f = Z;
+            if (i < nx - 1) {
+                const T dbe = (T)(0.5 * (a.dbe[j] + a.dbe[j + 1]));
+                const T vs = cur.vh + vh_m;
+                const T vm = first ? vvh : vvh_m, v0 = vvh;
+                f = dbe * (vs > Z ? vm : v0);
+                if (vs == Z) f = dbe * H * (vm + v0);
+                if (!cur.wet) f = Z;
+            }
+            a.fNSv[o] = f;
+            // EW flux of VH (rows < ny-1)
+            f = Z;
+            if (j < ny - 1) {
+                const T us = i == 0 ? cur.uh + nxt.uh : uh_w + uh_nw;
+                const T xl = i == 0 ? vuh : vuh_w, xr = vuh;
+                f = dle_v * (us > Z ? xl : xr);
+                if (i > 0) {
+                    if (us == Z) f = dle_v * H * (xl + xr);
+                    if (!((ww & 2) && cur.wet && (ww & 4) && nxt.wet)) f = Z;
+                }
+            }
+            a.fEWv[o] = f;
+        }
+        vh_m = cur.vh;
+        vh_me = vh_e;
+        uh_w = uh_nw;
+        wet_m = cur.wet;
+        uvh_m = uvh;
+        vvh_m = vvh;
+        vuh_m = vuh;
+        cur = nxt;
+        nxt = nl_cell(p1, j + 2, ny);
+        p1 = p2;
     }
 }
 
@@ -341,11 +507,14 @@ hipError_t launch_nonlinear_momentum(const NLArgs<T> &a, bool fused, hipStream_t
     if (a.nx < 3 || a.ny < 3) return hipErrorInvalidValue;
     const dim3 grid((a.nx + 255) / 256, a.ny), block(256);
     if (fused) {
-        hipLaunchKernelGGL((nl_flux_kernel<T, true>), grid, block, 0, s, a);
+        static std::atomic<int> occ[2];
+        const int rows = occupancy_rows((const void *)nl_flux_march_kernel<T>, occ, a.nx, a.ny, 64, kNLOwn, 1);
+        hipLaunchKernelGGL(nl_flux_march_kernel<T>, dim3((a.nx + kNLOwn - 1) / kNLOwn, (a.ny + rows - 1) / rows),
+                           dim3(64), 0, s, a, rows);
     } else {
         hipLaunchKernelGGL(nl_depth1_kernel<T>, grid, block, 0, s, a);
         hipLaunchKernelGGL(nl_advection_kernel<T>, grid, block, 0, s, a);
-        hipLaunchKernelGGL((nl_flux_kernel<T, false>), grid, block, 0, s, a);
+        hipLaunchKernelGGL(nl_flux_kernel<T>, grid, block, 0, s, a);
     }
     hipLaunchKernelGGL(nl_momentum_kernel<T>, grid, block, 0, s, a);
     return hipGetLastError();

@@ -55,6 +55,21 @@ def test_nonlinear_no_friction_fp64_bitexact(oracle_built, case):
     g.close()
 
 
+@pytest.mark.parametrize("nx,ny", [(62, 81), (63, 82), (123, 83)])
+def test_nonlinear_strip_edges_fp64_bitexact(oracle_built, nx, ny):
+    """The flux pass walks strips of 61 columns and 4 rows on grids this small: column
+    nx-1 (62, 123) or nx-2 (63) is the first column of a strip, and the last strip
+    begins at row ny-1 (81), ny-2 (82) or ny-3 (83), so the index maps at the east
+    columns and north rows are taken across strip edges."""
+    print("CASE leapfrog_nonlinear strip edges fp64", nx, ny, flush=True)
+    cfg = _no_friction(H.open_basin(nx=nx, ny=ny, method="leapfrog_nonlinear"))
+    o, g = _run_both(cfg, 40, {})
+    Ug, Uo = g.U, o.get_U()
+    for k in range(4):
+        assert np.array_equal(Ug[k], Uo[k]), (k, np.abs(Ug[k] - Uo[k]).max())
+    g.close()
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_nonlinear_fp64(oracle_built, case):
     print("CASE leapfrog_nonlinear fp64", case, flush=True)
