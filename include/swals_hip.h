@@ -47,7 +47,7 @@ extern "C" {
 #define SWH_METHOD_EULER 4
 #define SWH_METHOD_LINEAR 5
 #define SWH_METHOD_LLPNF 6 /* 'leapfrog_linear_plus_nonlinear_friction' */
-#define SWH_METHOD_LEAPFROG_NONLINEAR 8 /* 'leapfrog_nonlinear' (single domains) */
+#define SWH_METHOD_LEAPFROG_NONLINEAR 8 /* 'leapfrog_nonlinear' */
 
 /* Arithmetic of the bulk update */
 #define SWH_PREC_FP64 0    /* FP64 throughout (the reference's default dp = C_DOUBLE) */

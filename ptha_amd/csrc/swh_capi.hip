@@ -326,7 +326,6 @@ int leapfrog_step(swh_domain *h, double dt)
 template <typename T, bool KAHAN>
 int nonlinear_step(swh_domain *h, double dt)
 {
-    if (h->n_nest_strips) return fail(SWH_ESTATE, "'leapfrog_nonlinear' is not provided inside a multidomain");
     if (h->rc_dt != dt) {
         int rc = build_rc<T>(h, dt);
         if (rc) return rc;
@@ -339,6 +338,11 @@ int nonlinear_step(swh_domain *h, double dt)
     const int c = h->cur, o = 1 - c;
     const double t_half = h->time + dt * 0.5;
     const double t_after = t_half + 0.5 * dt;
+    // nesting flux integrals: dt * (UH, VH) before the update, mass only
+    // (domain_mod_leapfrog_nonlinear_include.f90:93-102)
+    if (h->n_nest_strips)
+        SWH_HIP(launch_nest_accum<T>(h->d_nest_strips, h->n_nest_strips, h->nest_strip_max, (const T *)h->VH[c],
+                                     (const T *)h->UH[c], 0, 0, h->pitch, 1, dt, h->d_nest_int, h->stream));
     const size_t P = h->plane_elems();
     T *w = (T *)h->nlw;
     NLArgs<T> n;

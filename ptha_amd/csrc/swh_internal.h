@@ -487,7 +487,7 @@ struct NestUnpack {
     int err_out;             // offset of the flux error in the error arena (same layout as the strips)
     long long buf;
     int block0;
-    int stag_fix;            // 0: none, 1: elevation rule (truely linear), 2: depth rule
+    int stag_fix;            // 0: none, 1: elevation rule (truely linear), 2: depth rule, 3: nonlinear rule
     double elev_lim;         // msl_linear - minimum_allowed_depth
 };
 

@@ -661,7 +661,9 @@ int finalize(swh_md *md)
             u.block0 = blocks;
             blocks += (nI * nJ * nK + nflux + 255) / 256;
             // domain_mod.f90:3741-3832
-            if (h->leapfrog()) {
+            if (h->method == SWH_METHOD_LEAPFROG_NONLINEAR) {
+                u.stag_fix = 3;
+            } else if (h->leapfrog()) {
                 u.stag_fix = h->truely_linear ? 1 : 2;
                 u.elev_lim = h->msl - kMinDepth;
             }
@@ -807,10 +809,6 @@ int swh_md_create(int32_t n_domains, swh_domain *const *domains, const double *a
         if (!domains[k]) {
             delete md;
             return md_fail(SWH_EINVAL, "swh_md_create: null domain");
-        }
-        if (domains[k]->method == SWH_METHOD_LEAPFROG_NONLINEAR) {
-            delete md;
-            return md_fail(SWH_EINVAL, "swh_md_create: 'leapfrog_nonlinear' domains are single-domain only here");
         }
         if (timestepping_refinement_factor[k] < 1) {
             delete md;

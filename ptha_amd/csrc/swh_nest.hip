@@ -388,6 +388,24 @@ __global__ __launch_bounds__(256) void nest_stag_fix_kernel(NestDoms<T> doms, co
     if (t >= nI * nJ) return;
     const int ii = u.iL + t % nI, jj = u.jL + t / nI;
     const int ip1 = imin(ii + 1, d.nx), jp1 = imin(jj + 1, d.ny);
+    if (u.stag_fix == 3) {
+        // 'leapfrog_nonlinear' (domain_mod.f90:3834-3882): no outflow from dry cells, and
+        // outflow from a wet cell into a dry neighbour only above the neighbour's elevation
+        const double s0 = nval(d, 1, ii, jj);
+        const bool dry0 = s0 - nval(d, 4, ii, jj) <= kMinDepth;
+        const bool has_e = ip1 == ii + 1, has_n = jp1 == jj + 1;
+        const double ee = nval(d, 4, ip1, jj), en = nval(d, 4, ii, jp1);
+        const bool dry_e = has_e && nval(d, 1, ip1, jj) - ee <= kMinDepth;
+        const bool dry_n = has_n && nval(d, 1, ii, jp1) - en <= kMinDepth;
+        const double uh = nval(d, 2, ii, jj), vh = nval(d, 3, ii, jj);
+        // each rule zeroes a positive or a negative value, so after one has fired no
+        // later rule changes it
+        const bool zu = (dry0 && uh > 0.0) || (dry_e && uh < 0.0) || (uh > 0.0 && dry_e && s0 <= ee + kMinDepth);
+        const bool zv = (dry0 && vh > 0.0) || (dry_n && vh < 0.0) || (vh > 0.0 && dry_n && s0 <= en + kMinDepth);
+        if (zu) nset(d, 2, ii, jj, 0.0);
+        if (zv) nset(d, 3, ii, jj, 0.0);
+        return;
+    }
     bool zu, zv;
     if (u.stag_fix == 1) {
         const double lim = u.elev_lim;

@@ -7,7 +7,9 @@ usage: python tests/md_ref_run.py OUT_PREFIX NSTEPS [planar | flow:METHOD]
 writes OUT_PREFIX.json (setup tables) and OUT_PREFIX.npz (arrays).
 
 With 'flow:METHOD' the grandchild uses that finite-volume method ('euler',
-'midpoint', 'rk2n') instead of 'rk2'.
+'midpoint', 'rk2n') instead of 'rk2'. With 'child:METHOD' the child uses that
+leapfrog method (e.g. 'leapfrog_nonlinear') instead of
+'leapfrog_linear_plus_nonlinear_friction'.
 
 With 'planar', every domain is 'rk2' and holds the planar field of the
 reference's own nesting test (multidomain_mod.f90:4088-4101): U(:, :, k) =
@@ -44,6 +46,8 @@ def main(prefix, nsteps, mode="flow"):
     cfg = configs.nested_small()
     if mode.startswith("flow:"):
         cfg["domains"][2]["method"] = mode.split(":", 1)[1]
+    if mode.startswith("child:"):
+        cfg["domains"][1]["method"] = mode.split(":", 1)[1]
     if mode == "planar":
         for d in cfg["domains"]:
             d["method"] = "rk2"

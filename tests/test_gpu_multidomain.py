@@ -192,6 +192,81 @@ def test_steps_other_fv_grandchild_fp64(ref_fv):
 
 
 @pytest.fixture(scope="module")
+def ref_nl(tmp_path_factory):
+    """The same nest with a 'leapfrog_nonlinear' child (Manning n = 0.03)."""
+    if not os.path.exists(REF_LIB):
+        pytest.skip("compiled reference not built (make -C oracle ref)")
+    prefix = str(tmp_path_factory.mktemp("mdref_nl") / "run")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    print("reference multidomain run, leapfrog_nonlinear child", prefix, flush=True)
+    subprocess.run([sys.executable, os.path.join(ROOT, "tests", "md_ref_run.py"), prefix, str(NSTEPS),
+                    "child:leapfrog_nonlinear"], check=True, env=env, timeout=300)
+    with open(prefix + ".json") as f:
+        tables = json.load(f)
+    return tables, dict(np.load(prefix + ".npz"))
+
+
+def test_steps_nonlinear_child_fp64(ref_nl):
+    """20 global steps with a 'leapfrog_nonlinear' child: its nesting flux integrals
+    (mass only, domain_mod_leapfrog_nonlinear_include.f90:93-102), its receive-side
+    wet/dry rules (no outflow from dry cells, outflow into a dry neighbour only above
+    its elevation, domain_mod.f90:3834-3882) and the nesting layer the reference's
+    setup chose for it, against the reference's multidomain at the bar of the rk2 nest
+    (measured: bit-identical on every domain); the initial exchange bit for bit."""
+    tables, arrays = ref_nl
+    assert tables["methods"][1] == "leapfrog_nonlinear"
+    dt = tables["dt"]
+    print("multidomain fp64, leapfrog_nonlinear child", flush=True)
+    doms, md = build_gpu(tables, arrays, "fp64")
+    try:
+        md.make_initial_conditions_consistent()
+        for k, d in enumerate(doms):
+            np.testing.assert_array_equal(d.U, arrays["Ucons_%d" % k], err_msg="U of domain %d" % k)
+        v0 = md.volumes()
+        md.evolve_one_step(dt)
+        for k, d in enumerate(doms):
+            U = d.U
+            for v in range(3):
+                err = _rel(U[v], arrays["U1_%d" % k][v])
+                assert err < 1e-9, (k, v, err)
+        md.evolve_steps(dt, NSTEPS - 1)
+        for k, d in enumerate(doms):
+            U = d.U
+            for v in range(3):
+                err = _rel(U[v], arrays["UN_%d" % k][v])
+                print("domain", k, "variable", v, "relative error", err, flush=True)
+                assert err < 1e-9, (k, v, err)
+        v = md.volumes()
+        np.testing.assert_allclose(v, arrays["volN"], rtol=1e-9)
+        assert abs(v.sum() - v0.sum()) <= 1e-12 * v0.sum()
+    finally:
+        _close(doms, md)
+
+
+def test_steps_nonlinear_child_fp32k(ref_nl):
+    """The nest with a 'leapfrog_nonlinear' child in FP32 with compensated stage
+    against the FP64 reference: the bars of the FP32K nest (stage within 1e-3 m of a
+    1 m wave after 20 steps, mass change below 1e-6 relative; measured 1.3e-7 m)."""
+    tables, arrays = ref_nl
+    dt = tables["dt"]
+    print("multidomain fp32k, leapfrog_nonlinear child", flush=True)
+    doms, md = build_gpu(tables, arrays, "fp32k")
+    try:
+        md.make_initial_conditions_consistent()
+        v0 = md.volumes()
+        md.evolve_steps(dt, NSTEPS)
+        for k, d in enumerate(doms):
+            wet = arrays["UN_%d" % k][3] < 0
+            err = np.abs(d.U[0] - arrays["UN_%d" % k][0])[wet].max()
+            print("domain", k, "fp32k stage error (m)", err, flush=True)
+            assert err < 1e-3, (k, err)
+        v = md.volumes()
+        assert abs(v.sum() - v0.sum()) <= 1e-6 * v0.sum()
+    finally:
+        _close(doms, md)
+
+
+@pytest.fixture(scope="module")
 def ref_planar(tmp_path_factory):
     if not os.path.exists(REF_LIB):
         pytest.skip("compiled reference not built (make -C oracle ref)")
