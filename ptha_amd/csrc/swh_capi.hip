@@ -396,8 +396,10 @@ int nonlinear_step(swh_domain *h, double dt)
     if (frac != 0.0 && h->fE)
         SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
                                      (T)frac, h->stream));
+    // (under graph capture the kernel forms the end time from the device time: NaN)
     SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                     h->d_dle, h->d_dbe, dt, t_after, h->sc, h->stream));
+                                     h->d_dle, h->d_dbe, dt, h->graph_time ? __builtin_nan("") : t_after, h->sc,
+                                     h->stream));
     SWH_HIP(launch_nonlinear_momentum<T>(n, !elev_forced, h->stream));
     h->cur = o;
     h->time = t_after;
@@ -704,7 +706,6 @@ int domain_evolve(swh_domain *h, double dt) { return evolve(h, dt); }
 
 std::string domain_graph_unsupported(const swh_domain *h)
 {
-    if (h->method == SWH_METHOD_LEAPFROG_NONLINEAR) return "'leapfrog_nonlinear'";
     if (h->disp_on) return "dispersion (host-side extrapolation bookkeeping)";
     if (h->maxq_mask && h->maxq_freq != 1) return "max_U updated every n > 1 steps";
     if (h->maxq_mask && h->nsteps == 0) return "max_U before the first step (take one eager step first)";

@@ -266,6 +266,39 @@ def test_steps_nonlinear_child_fp32k(ref_nl):
         _close(doms, md)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+def test_nonlinear_child_graph_replay_equals_eager(ref_nl, precision):
+    """The nest with a 'leapfrog_nonlinear' child replayed from HIP graphs (captured
+    after one eager step) equals eager global steps bit for bit: state and time of
+    every domain."""
+    tables, arrays = ref_nl
+    dt = tables["dt"]
+    print("multidomain graph replay, leapfrog_nonlinear child", precision, flush=True)
+    doms, md = build_gpu(tables, arrays, precision)
+    try:
+        md.make_initial_conditions_consistent()
+        md.evolve_steps(dt, 6)
+        md.synchronize()
+        want = [(d.U.copy(), d.time) for d in doms]
+    finally:
+        _close(doms, md)
+    doms, md = build_gpu(tables, arrays, precision)
+    try:
+        md.make_initial_conditions_consistent()
+        md.evolve_steps(dt, 1)
+        md.graph_capture(dt)
+        md.graph_launch(2)
+        md.evolve_steps(dt, 1)
+        md.graph_launch(2)
+        md.synchronize()
+        for k, d in enumerate(doms):
+            assert np.array_equal(d.U, want[k][0]), ("state", k)
+            assert d.time == want[k][1], ("time", k)
+        md.graph_destroy()
+    finally:
+        _close(doms, md)
+
+
 @pytest.fixture(scope="module")
 def ref_planar(tmp_path_factory):
     if not os.path.exists(REF_LIB):
