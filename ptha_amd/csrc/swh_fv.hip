@@ -372,16 +372,20 @@ constexpr int kScalarThreads = 1024;
 // the step's closing bookkeeping (defined with the kernels that run it)
 __device__ void fv_finish(DevScalars &sc, int mode);
 
-template <typename T>
-__global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny,
-                                                                   int fpitch, int n_ext, double fixed_dt,
-                                                                   double dt_mult, int mode, DevScalars *sc,
-                                                                   const double *bmax, int nbmax)
+// The pass's body for a workgroup of NT threads (kScalarThreads, or 256 inside the
+// update launch): the boundary lines are summed over Q = kScalarThreads / 4 lanes each
+// whatever NT is (a thread takes every (NT / Q)'th line), so the sums keep one order.
+// finish: rk2's closing bookkeeping (kFinishRk2) after the stage's.
+template <typename T, int NT>
+__device__ __forceinline__ void fv_scalar_body(const T *fNS, const T *fEW, int nx, int ny, int fpitch, int n_ext,
+                                               double fixed_dt, double dt_mult, int mode, int finish,
+                                               DevScalars *sc, const double *bmax, int nbmax)
 {
     constexpr int Q = kScalarThreads / 4;
-    constexpr int B = 8;   // loads issued together before any is used
+    constexpr int G = NT / Q;   // lines taken at once
+    constexpr int B = 8;        // loads issued together before any is used
     __shared__ double part[4][Q];
-    __shared__ double pm[kScalarThreads / 64];
+    __shared__ double pm[NT / 64];
     // thread 0 reads the step's scalars now, so that their latency overlaps the sums
     // below (neg_depth_count is not touched)
     DevScalars L;
@@ -389,36 +393,38 @@ __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS,
         L.time = sc->time; L.dt = sc->dt; L.bf_evolve = sc->bf_evolve; L.bf_time = sc->bf_time;
         L.first_max_dt = sc->first_max_dt; L.time0 = sc->time0;
     }
-    const int g = threadIdx.x / Q, lane = threadIdx.x % Q;
-    // boundary line g: N and S rows of flux_NS, E and W columns of flux_EW; each thread
-    // adds every Q'th entry in index order, B loads at a time
-    const bool ns = (g == 0 || g == 2);
-    const T *src = ns ? fNS : fEW;
-    const long long base =
-        ns ? (long long)((g == 0) ? ny - n_ext : n_ext) * fpitch : (long long)((g == 1) ? nx - n_ext : n_ext);
-    const long long step = ns ? 1 : fpitch;
-    const int last = ns ? nx - n_ext - 1 : ny - n_ext - 1;
-    double acc = 0.0;
-    for (int i0 = n_ext + lane; i0 <= last; i0 += B * Q) {
-        double v[B];
+    const int lane = threadIdx.x % Q;
+    for (int g = threadIdx.x / Q; g < 4; g += G) {
+        // boundary line g: N and S rows of flux_NS, E and W columns of flux_EW; each
+        // lane adds every Q'th entry in index order, B loads at a time
+        const bool ns = (g == 0 || g == 2);
+        const T *src = ns ? fNS : fEW;
+        const long long base =
+            ns ? (long long)((g == 0) ? ny - n_ext : n_ext) * fpitch : (long long)((g == 1) ? nx - n_ext : n_ext);
+        const long long step = ns ? 1 : fpitch;
+        const int last = ns ? nx - n_ext - 1 : ny - n_ext - 1;
+        double acc = 0.0;
+        for (int i0 = n_ext + lane; i0 <= last; i0 += B * Q) {
+            double v[B];
 #pragma unroll
-        for (int u = 0; u < B; u++) {
-            const int i = i0 + u * Q;
-            v[u] = i <= last ? (double)src[base + i * step] : 0.0;
+            for (int u = 0; u < B; u++) {
+                const int i = i0 + u * Q;
+                v[u] = i <= last ? (double)src[base + i * step] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++)
+                if (i0 + u * Q <= last) acc += v[u];
         }
-#pragma unroll
-        for (int u = 0; u < B; u++)
-            if (i0 + u * Q <= last) acc += v[u];
+        part[g][lane] = acc;
     }
-    part[g][lane] = acc;
     // CFL: maximum over the flux pass's blocks, from 1/maximum_timestep
     // (compute_fluxes_DE1_inner_include.f90:92-95); the entries are >= 0
     double m = 1.0 / kMaxTimestep;
-    for (int k0 = threadIdx.x; k0 < nbmax; k0 += B * kScalarThreads) {
+    for (int k0 = threadIdx.x; k0 < nbmax; k0 += B * NT) {
         double v[B];
 #pragma unroll
         for (int u = 0; u < B; u++) {
-            const int k = k0 + u * kScalarThreads;
+            const int k = k0 + u * NT;
             v[u] = k < nbmax ? bmax[k] : 0.0;
         }
 #pragma unroll
@@ -430,12 +436,13 @@ __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS,
     if ((threadIdx.x & 63) == 0) pm[threadIdx.x >> 6] = m;
     __syncthreads();
     for (int w = Q / 2; w > 0; w >>= 1) {
-        if (lane < w) part[g][lane] = part[g][lane] + part[g][lane + w];
+        if (lane < w)
+            for (int g = threadIdx.x / Q; g < 4; g += G) part[g][lane] = part[g][lane] + part[g][lane + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         double mx = pm[0];
-        for (int w = 1; w < kScalarThreads / 64; w++) mx = fmax_(mx, pm[w]);
+        for (int w = 1; w < NT / 64; w++) mx = fmax_(mx, pm[w]);
         L.bf_store[0] = part[0][0];
         L.bf_store[1] = part[1][0];
         L.bf_store[2] = -part[2][0];
@@ -463,6 +470,9 @@ __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS,
         if (mode & kStageFinish) {
             fv_finish(L, kFinishEuler);
             sc->evolve_dt = L.evolve_dt;
+        } else if (finish) {
+            fv_finish(L, kFinishRk2);
+            sc->evolve_dt = L.evolve_dt;
         }
         sc->time = L.time; sc->dt = L.dt; sc->max_dt = L.max_dt;
         for (int q = 0; q < 4; q++) sc->bf_store[q] = L.bf_store[q];
@@ -471,16 +481,34 @@ __global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS,
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kScalarThreads) void fv_scalar_kernel(const T *fNS, const T *fEW, int nx, int ny,
+                                                                   int fpitch, int n_ext, double fixed_dt,
+                                                                   double dt_mult, int mode, DevScalars *sc,
+                                                                   const double *bmax, int nbmax)
+{
+    fv_scalar_body<T, kScalarThreads>(fNS, fEW, nx, ny, fpitch, n_ext, fixed_dt, dt_mult, mode, 0, sc, bmax, nbmax);
+}
+
 // ---- pass 4: update_U (update_U_DE1_inner_include.f90:38-97) ----
 template <typename T>
 __global__ void fv_update_kernel(FVArgs<T> a)
 {
-    if ((int)blockIdx.y >= a.ny) {
+    if (a.scalar_row && blockIdx.y == 0) {
+        // the stage's scalar pass beside the update (the step is fixed: the update takes
+        // it from the arguments, and neither touches what the other reads or writes)
+        if (blockIdx.x == 0)
+            fv_scalar_body<T, 256>(a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext, a.s_fixed_dt, a.s_dt_mult, a.s_mode,
+                                   a.s_finish, a.sc, a.bmax, a.s_nbmax);
+        return;
+    }
+    const int yb = (int)blockIdx.y - a.scalar_row;   // grid row, then the nesting blocks
+    if (yb >= a.ny) {
         // the blocks past the grid: the stage's nesting flux integrals
         // (nested_grid_comms_mod.f90:2308-2461), x += nest_dt * flux on each strip cell
         // (the fluxes are pre-multiplied by the edge length); the update does not touch
         // the flux planes or the integrals, so they run beside it
-        const int e = (blockIdx.y - a.ny) * gridDim.x + blockIdx.x;
+        const int e = (yb - a.ny) * gridDim.x + blockIdx.x;
         if (e >= a.n_nest_strips * a.nest_chunks) return;
         const NestStrip st = a.nest_strips[e / a.nest_chunks];
         const int r = (e % a.nest_chunks) * blockDim.x + threadIdx.x;
@@ -496,14 +524,15 @@ __global__ void fv_update_kernel(FVArgs<T> a)
         return;
     }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int j = yb;
     unsigned long long fixed = 0;
     // with the rk2 average folded in, the step's closing bookkeeping (it does not touch
-    // stage_dt, the one scalar this kernel reads); with a stage boundary still to come,
-    // which reads the stage's time, the boundary kernel does it
-    if (a.avgS && !a.ring && i == 0 && j == 0) fv_finish(*a.sc, kFinishRk2);
+    // stage_dt, the one scalar this kernel reads; with the scalar pass in this launch,
+    // that pass does it); with a stage boundary still to come, which reads the stage's
+    // time, the boundary kernel does it
+    if (a.avgS && !a.ring && !a.scalar_row && i == 0 && j == 0) fv_finish(*a.sc, kFinishRk2);
     if (i < a.nx && j < a.ny) {
-    const double dt = a.sc->stage_dt;
+    const double dt = a.scalar_row ? a.stage_dt : a.sc->stage_dt;
     const long long o = (long long)j * a.pitch + i;
     const long long fo = (long long)j * a.fpitch + i;
     const long long FP = (long long)a.fpitch * (a.ny + 1);
@@ -769,12 +798,23 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
     const int rows = fv_flux_march_rows(a.flux_rows);
     hipLaunchKernelGGL(fv_flux_march_kernel<T>, dim3((a.nx + kMOwn - 1) / kMOwn, (a.ny + rows - 1) / rows), dim3(64),
                        0, s, a, rows);
-    hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(kScalarThreads), 0, s, a.fNS, a.fEW, a.nx, a.ny, a.fpitch, a.n_ext,
-                       fixed_dt, dt_mult, mode, a.sc, (const double *)a.bmax,
-                       fv_flux_grid_blocks(a.nx, a.ny, a.flux_rows));
+    const int nbmax = fv_flux_grid_blocks(a.nx, a.ny, a.flux_rows);
+    FVArgs<T> u = a;
+    if (fixed_dt > 0.0) {
+        // the stage's step as the scalar pass forms it (every stage of a fixed-step
+        // step is given the same fixed_dt): that pass then runs beside the update
+        u.scalar_row = 1;
+        u.stage_dt = (dt_mult == 1.0) ? fixed_dt : dt_mult * fixed_dt;
+        u.s_fixed_dt = fixed_dt; u.s_dt_mult = dt_mult; u.s_mode = mode; u.s_nbmax = nbmax;
+        u.s_finish = (a.avgS && !a.ring) ? 1 : 0;
+        grid.y += 1;
+    } else {
+        hipLaunchKernelGGL(fv_scalar_kernel<T>, dim3(1), dim3(kScalarThreads), 0, s, a.fNS, a.fEW, a.nx, a.ny,
+                           a.fpitch, a.n_ext, fixed_dt, dt_mult, mode, a.sc, (const double *)a.bmax, nbmax);
+    }
     if (a.nest_strips && a.n_nest_strips > 0 && a.nest_chunks > 0 && a.nest_dt != 0.0)
         grid.y += (a.n_nest_strips * a.nest_chunks + grid.x - 1) / grid.x;   // the nesting blocks
-    hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(fv_update_kernel<T>, grid, block, 0, s, u);
     return hipGetLastError();
 }
 
