@@ -474,7 +474,10 @@ __device__ __forceinline__ void fv_scalar_body(const T *fNS, const T *fEW, int n
             fv_finish(L, kFinishRk2);
             sc->evolve_dt = L.evolve_dt;
         }
-        sc->time = L.time; sc->dt = L.dt; sc->max_dt = L.max_dt;
+        // (dt changes only in a step's first stage: later stages leave it, which the
+        // update beside this pass may be reading)
+        if (mode & kStageFirst) sc->dt = L.dt;
+        sc->time = L.time; sc->max_dt = L.max_dt;
         for (int q = 0; q < 4; q++) sc->bf_store[q] = L.bf_store[q];
         sc->bf_evolve = L.bf_evolve; sc->bf_time = L.bf_time; sc->first_max_dt = L.first_max_dt;
         sc->time0 = L.time0; sc->stage_dt = L.stage_dt;
@@ -532,7 +535,12 @@ __global__ void fv_update_kernel(FVArgs<T> a)
     // time, the boundary kernel does it
     if (a.avgS && !a.ring && !a.scalar_row && i == 0 && j == 0) fv_finish(*a.sc, kFinishRk2);
     if (i < a.nx && j < a.ny) {
-    const double dt = a.scalar_row ? a.stage_dt : a.sc->stage_dt;
+    double dt = a.sc->stage_dt;
+    if (a.scalar_row) {
+        // the scalar pass's stage step, (dt_mult == 1) ? dt : dt_mult * dt
+        const double base = a.stage_dt_dev ? a.sc->dt : a.stage_dt;
+        dt = a.stage_dt_dev ? ((a.s_dt_mult == 1.0) ? base : a.s_dt_mult * base) : a.stage_dt;
+    }
     const long long o = (long long)j * a.pitch + i;
     const long long fo = (long long)j * a.fpitch + i;
     const long long FP = (long long)a.fpitch * (a.ny + 1);
@@ -800,10 +808,13 @@ hipError_t launch_fv_euler(const FVArgs<T> &a, double fixed_dt, double dt_mult, 
                        0, s, a, rows);
     const int nbmax = fv_flux_grid_blocks(a.nx, a.ny, a.flux_rows);
     FVArgs<T> u = a;
-    if (fixed_dt > 0.0) {
-        // the stage's step as the scalar pass forms it (every stage of a fixed-step
-        // step is given the same fixed_dt): that pass then runs beside the update
+    if (fixed_dt > 0.0 || !(mode & kStageFirst)) {
+        // the stage's step as the scalar pass forms it, known before this stage's
+        // fluxes: the fixed step (every stage of a fixed-step step is given the same
+        // fixed_dt), or the step a CFL-step's first stage stored; that pass then runs
+        // beside the update
         u.scalar_row = 1;
+        u.stage_dt_dev = fixed_dt > 0.0 ? 0 : 1;
         u.stage_dt = (dt_mult == 1.0) ? fixed_dt : dt_mult * fixed_dt;
         u.s_fixed_dt = fixed_dt; u.s_dt_mult = dt_mult; u.s_mode = mode; u.s_nbmax = nbmax;
         u.s_finish = (a.avgS && !a.ring) ? 1 : 0;

@@ -396,10 +396,12 @@ struct FVArgs {
     int n_nest_strips = 0, nest_chunks = 0;
     double nest_dt = 0.0;
     double *nest_int = nullptr;
-    // set by launch_fv_euler when the stage's step is known on the host (a fixed dt):
-    // the scalar pass runs as workgroup row 0 of the update launch (beside the update,
-    // which then takes stage_dt from here), with its arguments below
-    int scalar_row = 0;
+    // set by launch_fv_euler when the stage's step is known before the stage's flux pass
+    // (a fixed dt, or any stage after a step's first): the scalar pass runs as workgroup
+    // row 0 of the update launch, beside the update, which takes the stage's step from
+    // stage_dt (stage_dt_dev: from the step the first stage stored, DevScalars::dt,
+    // times s_dt_mult), with the pass's arguments below
+    int scalar_row = 0, stage_dt_dev = 0;
     double stage_dt = 0.0, s_fixed_dt = 0.0, s_dt_mult = 1.0;
     int s_mode = 0, s_finish = 0, s_nbmax = 0;
 };
