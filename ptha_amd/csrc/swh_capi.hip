@@ -380,6 +380,12 @@ int nonlinear_step(swh_domain *h, double dt)
     a.bc_mask = flather ? 0 : h->bc_mask;
     a.bc_stage = (T)bc_stage;
     a.maxq_mask = 0;
+    // the boundary fluxes read the pre-momentum UH/VH (include.f90:149-165): without a
+    // Flather or edge boundary (which change them on the edges first) nothing changes
+    // them before the momentum passes, so line workgroups of the stage launch take them
+    // (under graph capture the end time from the device time: NaN)
+    const double t_after_arg = h->graph_time ? __builtin_nan("") : t_after;
+    if (!flather) a.bf = bf_args(h, 256, dt, t_after_arg, 0.0);
     SWH_HIP(launch_lf_stage<T, KAHAN>(a, h->stream));
     T *Cb = KAHAN ? (T *)h->C[o] : nullptr;
     if (flather && h->bc_kind == 1)
@@ -396,10 +402,9 @@ int nonlinear_step(swh_domain *h, double dt)
     if (frac != 0.0 && h->fE)
         SWH_HIP(launch_add_scaled<T>((T *)h->E, (const T *)h->fE, h->pitch, h->f_i0, h->f_i1, h->f_j0, h->f_j1,
                                      (T)frac, h->stream));
-    // (under graph capture the kernel forms the end time from the device time: NaN)
-    SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch, h->n_ext,
-                                     h->d_dle, h->d_dbe, dt, h->graph_time ? __builtin_nan("") : t_after, h->sc,
-                                     h->stream));
+    if (flather)
+        SWH_HIP(launch_bflux_leapfrog<T>((const T *)h->UH[c], (const T *)h->VH[c], h->nx, h->ny, h->pitch,
+                                         h->n_ext, h->d_dle, h->d_dbe, dt, t_after_arg, h->sc, h->stream));
     SWH_HIP(launch_nonlinear_momentum<T>(n, !elev_forced, h->stream));
     h->cur = o;
     h->time = t_after;

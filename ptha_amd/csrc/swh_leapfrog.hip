@@ -392,12 +392,19 @@ __global__ __launch_bounds__(256) void leapfrog_kernel(LFArgs<T> a)
 }
 
 // The stage half-step alone (for steps whose boundary must see the new stage before
-// the momentum half-step, e.g. the Flather boundary): stage_new on every cell.
+// the momentum half-step, e.g. the Flather boundary): stage_new on every cell. With
+// a.bf.ey, the first ey rows of workgroups take the step's boundary-flux sums from the
+// UH/VH this launch reads (for steps in which nothing changes them before the momentum
+// half-step).
 template <typename T, bool KAHAN>
 __global__ __launch_bounds__(256) void lf_stage_kernel(LFArgs<T> a)
 {
+    if (a.bf.ey && (int)blockIdx.y < a.bf.ey) {
+        bf_lines<T, 1>(a.bf, a.nx, a.ny, a.pitch, a.UH, a.VH, [](int, int, T &, T &) {});
+        return;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int j = (int)blockIdx.y - a.bf.ey;
     if (i >= a.nx || j >= a.ny) return;
     const long long o = (long long)j * a.pitch + i;
     const T s = a.S[o], c = KAHAN ? a.C[o] : T(0);
@@ -413,7 +420,11 @@ template <typename T, bool KAHAN>
 hipError_t launch_lf_stage(const LFArgs<T> &a, hipStream_t s)
 {
     if (a.stage_given || a.src_stride) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((lf_stage_kernel<T, KAHAN>), dim3((a.nx + 255) / 256, a.ny), dim3(256), 0, s, a);
+    LFArgs<T> b = a;
+    const int gx = (a.nx + 255) / 256;
+    b.bf.ey = bf_rows(a.bf, gx);
+    if (b.bf.ey < 0 || (b.bf.ey && (!b.bf.part || !b.bf.sc))) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((lf_stage_kernel<T, KAHAN>), dim3(gx, a.ny + b.bf.ey), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 template hipError_t launch_lf_stage<double, false>(const LFArgs<double> &, hipStream_t);
