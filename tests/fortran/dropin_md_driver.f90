@@ -13,7 +13,8 @@
 ! |GPU - reference| of stage/UH/VH relative to each field's largest magnitude, and
 ! the relative change of the total priority-domain volume on the GPU.
 !
-! Usage: dropin_md_driver <nsteps>   (run in a scratch directory: md%setup writes there)
+! Usage: dropin_md_driver <nsteps> [child method]   (run in a scratch directory:
+! md%setup writes there); the child method may be e.g. 'leapfrog_nonlinear'.
 program dropin_md_driver
     use iso_c_binding
     use global_mod, only: dp, ip, charlen, force_double
@@ -33,9 +34,14 @@ program dropin_md_driver
     integer(c_int32_t) :: finite
     integer(c_int64_t) :: n_bad
     character(len=64) :: arg
+    character(len=charlen) :: child_method
 
     call get_command_argument(1, arg)
     read(arg, *) nsteps
+    ! optional: the child's timestepping method (default the linear leapfrog with
+    ! nonlinear friction)
+    child_method = 'leapfrog_linear_plus_nonlinear_friction'
+    if(command_argument_count() > 1) call get_command_argument(2, child_method)
 
     allocate(md%domains(3))
     md%domains(1)%lower_left = [100.0_dp, -60.0_dp]
@@ -45,7 +51,7 @@ program dropin_md_driver
     md%domains(1)%timestepping_method = 'linear'
     call md%domains(2)%match_geometry_to_parent(parent_domain=md%domains(1), lower_left=[105.0_dp, -57.0_dp], &
         upper_right=[113.0_dp, -50.0_dp], dx_refinement_factor=3_ip, timestepping_refinement_factor=1_ip)
-    md%domains(2)%timestepping_method = 'leapfrog_linear_plus_nonlinear_friction'
+    md%domains(2)%timestepping_method = trim(child_method)
     call md%domains(3)%match_geometry_to_parent(parent_domain=md%domains(2), lower_left=[108.0_dp, -55.0_dp], &
         upper_right=[110.0_dp, -53.0_dp], dx_refinement_factor=3_ip, timestepping_refinement_factor=3_ip)
     md%domains(3)%timestepping_method = 'rk2'

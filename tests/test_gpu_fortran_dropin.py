@@ -27,23 +27,26 @@ def test_fortran_dropin_bitwise(method):
     assert "BITWISE_EQUAL" in r.stdout
 
 
-def test_fortran_dropin_multidomain(tmp_path):
+@pytest.mark.parametrize("child", ["leapfrog_linear_plus_nonlinear_friction", "leapfrog_nonlinear"])
+def test_fortran_dropin_multidomain(tmp_path, child):
     """The reference's multidomain_type set up in Fortran and handed to the backend
     through swals_hip_mod (tests/fortran/dropin_md_driver.f90): after 20 global steps
     (one eager, then 19 replayed from captured HIP graphs) every domain matches
     md%evolve_one_step to 1e-9 relative (the rk2 grandchild differs from the reference
     at round-off, which nesting passes on), the GPU conserves the total priority-domain
     volume to 1e-12, and the device statistics and stability scan called from Fortran
-    report a finite energy and no unstable cell."""
+    report a finite energy and no unstable cell. The child runs the linear leapfrog with
+    nonlinear friction, or 'leapfrog_nonlinear' (with the nesting layer md%setup
+    chooses for that method)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "dropin_md_driver_sph")
-    print("CASE fortran dropin multidomain", flush=True)
+    print("CASE fortran dropin multidomain, child", child, flush=True)
     if not os.path.exists(exe):
         pytest.skip("Fortran drop-in driver not built (needs the reference sources at build time)")
     # md%setup copies '*.f* make*' from the working directory and needs a match
     for name in ("make_placeholder", "placeholder.f90"):
         (tmp_path / name).write_text("")
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    r = subprocess.run([exe, "20"], capture_output=True, text=True, timeout=240, cwd=str(tmp_path), env=env)
+    r = subprocess.run([exe, "20", child], capture_output=True, text=True, timeout=240, cwd=str(tmp_path), env=env)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     rels = [list(map(float, line.split(":")[1].split())) for line in r.stdout.splitlines()
