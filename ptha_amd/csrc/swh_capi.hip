@@ -143,11 +143,13 @@ template <typename T>
 int build_rc(swh_domain *h, double dt)
 {
     const int nx = h->nx, ny = h->ny;
-    std::vector<T> buf((size_t)5 * ny + (nx + 1) + (ny + 1));
+    std::vector<T> buf((size_t)6 * ny + (nx + 1) + (ny + 1));
     T *sa = buf.data(), *cu = sa + ny, *cv = cu + ny, *fu = cv + ny, *fv = fu + ny, *dle = fv + ny,
-      *dbe = dle + nx + 1;
+      *dbe = dle + nx + 1, *sv = dbe + ny + 1;
     for (int j = 0; j < ny; j++) {
         sa[j] = (T)(dt / h->area[j]);
+        // dt over the VH-point area (the nonlinear momentum update's, ':575-783')
+        sv[j] = (j + 1 < ny) ? (T)(dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
         cu[j] = (T)(kGravity * dt / h->area[j]);
         cv[j] = (j + 1 < ny) ? (T)(kGravity * dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
         fu[j] = h->coriolis ? (T)(dt * h->cor[j] * 0.5) : T(0);
@@ -354,6 +356,11 @@ int nonlinear_step(swh_domain *h, double dt)
     n.fEWu = w + 5 * P; n.fEWv = w + 6 * P; n.fNSu = w + 7 * P; n.fNSv = w + 8 * P;
     n.dbe = h->d_dbe; n.dle = h->d_dle; n.area = h->d_area; n.cor = h->d_cor; n.corbe = h->d_corbe;
     n.tanlat = h->d_tanlat;
+    {
+        const LFRowCoef<T> q = rc_view<T>(h);
+        n.ica = q.sa;                       // (T)(dt / area(j))
+        n.ica_vh = q.dbe + h->ny + 1;       // (T)(dt / (0.5 (area(j) + area(j+1))))
+    }
     n.dt = (T)dt; n.lfc = (T)h->lfc;
     n.fric_power = (T)((h->friction_type == 1) ? -2.0 : -7.0 / 3.0);
     n.beta = T(1);            // advection_beta (global_mod.f90:38)
@@ -916,7 +923,7 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
         A((void **)&h->esE, FPn * sizeof(double));
         A(&h->ring, (size_t)2 * (nx + ny) * h->esz);
     }
-    A(&h->rc, h->esz * ((size_t)5 * ny + (nx + 1) + (ny + 1)));
+    A(&h->rc, h->esz * ((size_t)6 * ny + (nx + 1) + (ny + 1)));
     A((void **)&h->sc, sizeof(DevScalars));
     // line workgroups: at most 4 ceil(max(nx, ny) / 64) wanted, in rows at most
     // ceil(nx / 64) workgroups wide

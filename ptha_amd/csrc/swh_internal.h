@@ -222,6 +222,7 @@ struct NLArgs {
     const T *nsq;             // manning_squared
     T *DL, *UUH, *UVH, *VUH, *VVH, *fEWu, *fEWv, *fNSu, *fNSv;  // work planes
     const double *dbe, *dle, *area, *cor, *corbe, *tanlat;     // FP64 metrics
+    const T *ica, *ica_vh;    // per row: (T)(dt / area(j)), (T)(dt / (0.5 (area(j) + area(j+1))))
     T dt, lfc, fric_power, beta, froude_limit;
     int coriolis, spherical;
 };

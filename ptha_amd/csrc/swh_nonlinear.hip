@@ -402,9 +402,9 @@ __global__ void nl_momentum_kernel(NLArgs<T> a)
     T uh = UH[o], vh = VH[o];
     if (i < nx - 1 && j < ny - 1) {
         const T dt = a.dt;
-        const T ica_dt = (T)(a.dt / a.area[j]);
+        const T ica_dt = a.ica[j];          // (T)(dt / area(j)), formed on the host
         const T ica_dt_g = G * ica_dt;
-        const T ica_dt_vh = (T)(a.dt / (0.5 * (a.area[j] + a.area[j + 1])));
+        const T ica_dt_vh = a.ica_vh[j];    // (T)(dt / (0.5 (area(j) + area(j+1))))
         const T ica_dt_vh_g = G * ica_dt_vh;
         const long long oe = o + 1, on = o + P;
         const T s0 = S[o], e0 = E[o], se = S[oe], ee = E[oe], sn = S[on], en = E[on];

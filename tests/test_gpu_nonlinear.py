@@ -11,7 +11,7 @@ bit-identical to the compiled reference on the same cases
   boundaries, a stage boundary);
 * a forcing patch on a frictionless basin: bit-identical to the compiled reference;
 * FP32 + Kahan stage: within 1e-5 m of the FP64 oracle on the all-wet basin (measured
-  8.4e-7 m), momenta within 2e-3 m^2/s (measured 2.1e-4; they reach 50 m^2/s).
+  7.1e-7 m), momenta within 2e-3 m^2/s (measured 1.6e-4; they reach 50 m^2/s).
 """
 import numpy as np
 import pytest
