@@ -148,7 +148,8 @@ int build_rc(swh_domain *h, double dt)
       *dbe = dle + nx + 1, *sv = dbe + ny + 1;
     for (int j = 0; j < ny; j++) {
         sa[j] = (T)(dt / h->area[j]);
-        // dt over the VH-point area (the nonlinear momentum update's, ':575-783')
+        // dt over the VH-point area (the nonlinear momentum update,
+        // domain_mod_leapfrog_nonlinear_include.f90:575-783)
         sv[j] = (j + 1 < ny) ? (T)(dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
         cu[j] = (T)(kGravity * dt / h->area[j]);
         cv[j] = (j + 1 < ny) ? (T)(kGravity * dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
