@@ -143,14 +143,15 @@ template <typename T>
 int build_rc(swh_domain *h, double dt)
 {
     const int nx = h->nx, ny = h->ny;
-    std::vector<T> buf((size_t)6 * ny + (nx + 1) + (ny + 1));
+    const bool nl = h->method == SWH_METHOD_LEAPFROG_NONLINEAR;
+    std::vector<T> buf((size_t)(nl ? 6 : 5) * ny + (nx + 1) + (ny + 1));
     T *sa = buf.data(), *cu = sa + ny, *cv = cu + ny, *fu = cv + ny, *fv = fu + ny, *dle = fv + ny,
       *dbe = dle + nx + 1, *sv = dbe + ny + 1;
     for (int j = 0; j < ny; j++) {
         sa[j] = (T)(dt / h->area[j]);
         // dt over the VH-point area (the nonlinear momentum update,
         // domain_mod_leapfrog_nonlinear_include.f90:575-783)
-        sv[j] = (j + 1 < ny) ? (T)(dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
+        if (nl) sv[j] = (j + 1 < ny) ? (T)(dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
         cu[j] = (T)(kGravity * dt / h->area[j]);
         cv[j] = (j + 1 < ny) ? (T)(kGravity * dt / (0.5 * (h->area[j] + h->area[j + 1]))) : T(0);
         fu[j] = h->coriolis ? (T)(dt * h->cor[j] * 0.5) : T(0);
@@ -924,7 +925,9 @@ int swh_domain_create(const swh_domain_desc *d, swh_domain **out)
         A((void **)&h->esE, FPn * sizeof(double));
         A(&h->ring, (size_t)2 * (nx + ny) * h->esz);
     }
-    A(&h->rc, h->esz * ((size_t)6 * ny + (nx + 1) + (ny + 1)));
+    // (the nonlinear leapfrog's update adds one row array; the other methods' buffer
+    // keeps its size, and with it the placement of what is allocated after it)
+    A(&h->rc, h->esz * ((size_t)(h->method == SWH_METHOD_LEAPFROG_NONLINEAR ? 6 : 5) * ny + (nx + 1) + (ny + 1)));
     A((void **)&h->sc, sizeof(DevScalars));
     // line workgroups: at most 4 ceil(max(nx, ny) / 64) wanted, in rows at most
     // ceil(nx / 64) workgroups wide
